@@ -1,0 +1,119 @@
+/*
+ * otmap_capi.h -- C ABI of libotmap_amd, the MI355X (gfx950) implementation of
+ * otmap's grid transport solver.
+ *
+ * The reference (ggael/otmap) has no FFI; its boundary is the C++ API in
+ * otlib/otsolver_2dgrid.h, otlib/transport_map.h and otlib/utils/rasterizer.h.
+ * Every entry point below replaces one reference call (cited per function).
+ * The C++ drop-in headers under include/otmap/ and the Python package
+ * otmap_amd are thin layers over these functions.
+ *
+ * Conventions (identical to the reference):
+ *   - n x n grid; face (cell) fields hold n*n values in face order j + i*n
+ *     (i = x index, j = y index); an image img(row, col) vectorised
+ *     column-major gives face (i = col, j = row).
+ *   - vertex fields hold (n+1)^2 values in vertex order j + i*(n+1); vertex
+ *     pairs (positions, gradients) are SoA: all x, then all y.
+ *   - point lists are interleaved (x0, y0, x1, y1, ...).
+ *   - all arithmetic is IEEE double unless OTM_PRECISION_MIXED is requested.
+ * Host pointers are caller-owned; device buffers belong to the handles.
+ * Every function returns OTM_OK (0) or a negative error code and, on error,
+ * sets a per-thread message readable with otm_last_error().
+ * Handles are not thread-safe; use one handle per host thread.
+ */
+#ifndef OTMAP_CAPI_H
+#define OTMAP_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OTM_OK 0
+#define OTM_ERR_ARG (-1)
+#define OTM_ERR_HIP (-2)
+#define OTM_ERR_STATE (-3)
+#define OTM_ERR_ALLOC (-4)
+
+/* BetaOpt (otlib/otsolver_2dgrid.h:33-35) */
+enum { OTM_BETA_ZERO = 0, OTM_BETA_CONJUGATE_JACOBIAN = 1 };
+/* arithmetic of the Laplacian solve and stencils */
+enum { OTM_PRECISION_FP64 = 0, OTM_PRECISION_MIXED = 1 };
+/* why a solve stopped */
+enum { OTM_EXIT_NONE = 0, OTM_EXIT_CONVERGED_OR_MAX_ITER = 1, OTM_EXIT_ZERO_DIRECTION = 2 };
+
+/* SolverOptions (otlib/otsolver_2dgrid.h:37-43) plus the arithmetic mode. */
+typedef struct otm_options {
+  int beta;          /* OTM_BETA_CONJUGATE_JACOBIAN (default) or OTM_BETA_ZERO */
+  int max_iter;      /* default 1000 */
+  double threshold;  /* default 1e-7, compared with |r|^2 / A */
+  double max_ratio;  /* default DBL_MAX: cap of max/min of the target density */
+  int precision;     /* OTM_PRECISION_FP64 (default, the drop-in) or OTM_PRECISION_MIXED */
+  int reserved;
+} otm_options;
+
+typedef struct otm_stats {
+  int iterations;           /* Newton iterations performed (reference "it") */
+  int exit_reason;          /* OTM_EXIT_* */
+  double initial_residual;  /* |r_0|^2 / A */
+  double final_residual;    /* residual returned by the last line search */
+  double ms_solve;          /* device time of the solve, density in -> positions out */
+} otm_stats;
+
+typedef struct otm_solver otm_solver;
+typedef struct otm_map otm_map;
+
+/* Fills *opt with the reference defaults (otlib/otsolver_2dgrid.h:37-43). */
+void otm_options_init(otm_options* opt);
+
+/* Text of the last error on this thread ("" if none). */
+const char* otm_last_error(void);
+
+/* GridBasedTransportSolver() (otlib/otsolver_2dgrid.cpp:30-40); binds the
+ * handle to HIP device `device` and creates its stream. */
+int otm_solver_create(otm_solver** out, int device);
+/* ~GridBasedTransportSolver() (otlib/otsolver_2dgrid.cpp:42-45) */
+int otm_solver_destroy(otm_solver* s);
+/* set_verbose_level (otlib/otsolver_2dgrid.h:52) */
+int otm_solver_set_verbose_level(otm_solver* s, int level);
+/* init(n) (otlib/otsolver_2dgrid.cpp:75-104): grid mesh + Laplacian operator.
+ * A repeated init with the same n returns immediately, as in the reference. */
+int otm_solver_init(otm_solver* s, int n);
+/* current grid size (0 before init) */
+int otm_solver_grid_size(const otm_solver* s);
+
+/* solve(density, opt) (otlib/otsolver_2dgrid.cpp:106-245) on host buffers.
+ *   density          n*n input values (face order), not modified
+ *   fwd_xy           out, 2*(n+1)^2: forward mesh vertex positions (SoA)
+ *   adjusted_density out, n*n (nullable): TransportMap::density()
+ *   stats            out (nullable) */
+int otm_solve(otm_solver* s, const double* density, const otm_options* opt, double* fwd_xy,
+              double* adjusted_density, otm_stats* stats);
+/* Same on device buffers (HIP pointers on the handle's device); returns when
+ * the results are complete. */
+int otm_solve_device(otm_solver* s, const double* d_density, const otm_options* opt, double* d_fwd_xy,
+                     double* d_adjusted_density, otm_stats* stats);
+/* Per-iteration residual / alpha / beta of the last solve (the values the
+ * reference prints at verbose level 5, otlib/otsolver_2dgrid.cpp:653-658).
+ * Returns the number of iterations written (<= cap). */
+int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, double* beta, int cap);
+
+/* Component entry points: the reference's protected stencils, on host
+ * buffers, for parity testing and for callers that drive the iteration.
+ *   compute_vertex_gradients (otlib/otsolver_2dgrid.cpp:362-423): grads 2*(n+1)^2
+ *   compute_residual         (otlib/otsolver_2dgrid.cpp:449-465): r n*n, *e = |r|^2/A
+ *   Laplacian solve + mean removal (otlib/otsolver_2dgrid.cpp:174-179): d_hat n*n
+ *   compute_1D_problem_parameters (otlib/otsolver_2dgrid.cpp:467-539): a, b n*n
+ * These use the handle's buffers and clobber the state of a previous solve's
+ * intermediates (not its results already copied out). */
+int otm_vertex_gradients(otm_solver* s, const double* psi, double* grads);
+int otm_residual(otm_solver* s, const double* adjusted_density, const double* psi, double* r, double* e);
+int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat);
+int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* dir, double* a, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OTMAP_CAPI_H */

@@ -854,13 +854,20 @@ int orc_locate_all(int n, const double* xy, double qx, double qy, int* faces, in
 }
 
 // rasterize_image, per-face density mode  (otlib/utils/rasterizer.cpp:138-237)
-// img row-major rows x cols; density may be null (all ones).  Where the
-// reference reads uninitialised weights (bilinear solve fails, D < 0) the
-// pixel is treated as not covered.
+// img row-major rows x cols; density may be null (all ones).
+// Where the bilinear solve fails (D < 0) the reference ignores the return
+// value and tests an uninitialised Eigen::Vector4d (rasterizer.cpp:169-171).
+// In the reference's Release build that vector occupies the same stack slot
+// on every pixel and every face, so it holds the weights of the last pixel
+// whose solve succeeded, in scan order (faces in index order, rows y, then
+// columns x).  That observed behaviour is reproduced here (it matches the
+// reference's images bit for bit); before the first successful solve the
+// weights count as "not covered".
 void orc_rasterize(int n, const double* xy, const double* density, int rows, int cols, double* img) {
   const long V = long(n + 1) * (n + 1), F = long(n) * n;
   std::fill(img, img + long(rows) * cols, 0.);
   const float scale = float(1. / double(F));
+  double w[4] = {-1., -1., -1., -1.};  // carried across pixels and faces (see above)
   for (long f = 0; f < F; ++f) {
     int v[4];
     quad_vertices(n, int(f), v);
@@ -878,8 +885,7 @@ void orc_rasterize(int n, const double* xy, const double* density, int rows, int
     for (int y = y0; y < y1; ++y)
       for (int x = x0; x < x1; ++x) {
         const P2 c = {(double(x) + 0.5) / double(cols), (double(y) + 0.5) / double(rows)};
-        double w[4];
-        if (!bilinear_weights(c, p, w)) continue;
+        bilinear_weights(c, p, w);  // leaves w unchanged when D < 0
         bool ok = true;
         for (int t = 0; t < 4; ++t) ok = ok && w[t] <= 1.0f && w[t] >= 0.0f;
         if (ok) img[long(y) * cols + x] = mass / area;

@@ -1,0 +1,13 @@
+"""MI355X (gfx950) implementation of otmap's grid transport solver.
+
+Drop-in mirror of the reference's API (ggael/otmap):
+  GridBasedTransportSolver, SolverOptions, BetaOpt      otlib/otsolver_2dgrid.h
+  TransportMap, apply_inverse_map, transport_cost        otlib/transport_map.h
+  rasterize_image                                        otlib/utils/rasterizer.h
+backed by hand-written HIP kernels in libotmap_amd.so (C ABI: include/otmap_capi.h).
+"""
+from .solver import BetaOpt, GridBasedTransportSolver, Precision, SolverOptions, SolveStats
+from .transport_map import GridMesh, TransportMap
+
+__all__ = ["BetaOpt", "GridBasedTransportSolver", "Precision", "SolverOptions", "SolveStats", "GridMesh",
+           "TransportMap"]

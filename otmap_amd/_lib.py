@@ -1,0 +1,103 @@
+"""ctypes binding of libotmap_amd.so (the C ABI declared in include/otmap_capi.h).
+
+The library is loaded from this package directory only (built in-tree by
+otmap_amd/build.py).  There is no fallback: a missing or unloadable library
+raises immediately, so nothing silently runs on another code path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libotmap_amd.so")
+HEADERS = [os.path.join(ROOT, "include", "otmap_capi.h")]
+
+OTM_OK = 0
+OTM_BETA_ZERO, OTM_BETA_CONJUGATE_JACOBIAN = 0, 1
+OTM_PRECISION_FP64, OTM_PRECISION_MIXED = 0, 1
+
+
+class OtmOptions(C.Structure):
+    _fields_ = [("beta", C.c_int), ("max_iter", C.c_int), ("threshold", C.c_double),
+                ("max_ratio", C.c_double), ("precision", C.c_int), ("reserved", C.c_int)]
+
+
+class OtmStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("exit_reason", C.c_int), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double), ("ms_solve", C.c_double)]
+
+
+class OtmError(RuntimeError):
+    pass
+
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_SIGNATURES = {
+    "otm_options_init": (None, [C.POINTER(OtmOptions)]),
+    "otm_last_error": (C.c_char_p, []),
+    "otm_solver_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "otm_solver_destroy": (C.c_int, [_vp]),
+    "otm_solver_set_verbose_level": (C.c_int, [_vp, C.c_int]),
+    "otm_solver_init": (C.c_int, [_vp, C.c_int]),
+    "otm_solver_grid_size": (C.c_int, [_vp]),
+    "otm_solve": (C.c_int, [_vp, _dp, C.POINTER(OtmOptions), _dp, _dp, C.POINTER(OtmStats)]),
+    "otm_solve_device": (C.c_int, [_vp, _vp, C.POINTER(OtmOptions), _vp, _vp, C.POINTER(OtmStats)]),
+    "otm_solver_trace": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int]),
+    "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
+    "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),
+    "otm_line_search_coefficients": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+}
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared by the public C headers."""
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(otm_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def lib():
+    """The loaded library (raises OtmError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OtmError(f"{LIB_PATH} is missing: build it with `python -m otmap_amd.build` "
+                       "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != OTM_OK:
+        msg = lib().otm_last_error().decode(errors="replace")
+        raise OtmError(f"{what} failed (code {rc}): {msg}")
+
+
+def ptr(a):
+    """double* of a C-contiguous float64 numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_ip)

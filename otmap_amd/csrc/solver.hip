@@ -1,0 +1,372 @@
+// Solver handle: device buffers for one grid size, the DCT plan, and the
+// host-side orchestration of GridBasedTransportSolver::init()/solve()
+// (otlib/otsolver_2dgrid.cpp:75-245).
+//
+// One Newton iteration is seven kernels on the handle's stream:
+//   4 DCT GEMMs (d_hat = M^+ (r - S e0))            otsolver_2dgrid.cpp:174-183
+//   k_beta      (conjugate-Jacobian beta)            :191-200, :248-261
+//   k_dir_coef  (d, quartic coefficients, alpha)      :467-598
+//   k_update    (psi, G0, r)                          :599-605
+// All scalar decisions are taken on the device; the host only checks the
+// stop flag once per chunk of iterations, and kernels enqueued past the stop
+// return immediately.
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/otmap_capi.h"
+#include "otm_internal.h"
+
+namespace otm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace otm
+
+using namespace otm;
+
+struct otm_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int verbose = 1;
+  int n = 0;
+  Fields f;
+  DctPlan plan;
+  double* d_in = nullptr;      // staging of host-API inputs (n*n)
+  double* d_xy = nullptr;      // staging of host-API vertex outputs (2V)
+  double* h_sum = nullptr;     // pinned: host copy of partials
+  SolveState* h_state = nullptr;  // pinned
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int last_iters = 0;
+};
+
+namespace {
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <class T>
+void dalloc(T*& p, size_t count) {
+  OTM_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+}
+
+void release_fields(otm_solver* s) {
+  Fields& f = s->f;
+  dfree(f.rho); dfree(f.psi); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
+  dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
+  dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.counters); dfree(f.trace);
+  dfree(s->d_in); dfree(s->d_xy);
+  dct_plan_destroy(s->plan);
+  f = Fields();
+  s->n = 0;
+}
+
+long partial_slots(int n) {
+  const long gemm_tiles = long((n + 63) / 64) * ((n + 63) / 64);
+  return std::max<long>(std::max<long>(num_tiles(n), gemm_tiles), 3 * 1024);
+}
+
+void ensure_trace(otm_solver* s, int cap) {
+  Fields& f = s->f;
+  if (cap <= f.trace_cap) return;
+  dfree(f.trace);
+  dalloc(f.trace, 3 * size_t(cap));
+  f.trace_cap = cap;
+}
+
+void allocate(otm_solver* s, int n) {
+  Fields& f = s->f;
+  f.n = n;
+  f.F = long(n) * n;
+  f.V = long(n + 1) * (n + 1);
+  const size_t F = f.F, V = f.V;
+  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 2 * V); dalloc(f.r, F); dalloc(f.jd, F);
+  dalloc(f.dhat, F); dalloc(f.d[0], F); dalloc(f.d[1], F); dalloc(f.dpair, 2);
+  dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, 8 * size_t(partial_slots(n)));
+  dalloc(f.tmp0, F); dalloc(f.tmp1, F); dalloc(f.state, 1); dalloc(f.counters, CNT_COUNT);
+  dalloc(s->d_in, F); dalloc(s->d_xy, 2 * V);
+  OTM_HIP(hipMemcpy(f.dpair, f.d, 2 * sizeof(double*), hipMemcpyHostToDevice));
+  OTM_HIP(hipMemset(f.counters, 0, CNT_COUNT * sizeof(unsigned)));
+  OTM_HIP(hipMemset(f.state, 0, sizeof(SolveState)));
+  ensure_trace(s, 1024);
+  dct_plan_create(s->plan, n);
+  s->n = n;
+}
+
+int fail_hip(const HipError& e) {
+  set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+  return OTM_ERR_HIP;
+}
+
+#define OTM_GUARD_BEGIN try {
+#define OTM_GUARD_END                         \
+  }                                           \
+  catch (const HipError& e) {                 \
+    return fail_hip(e);                       \
+  }                                           \
+  catch (const std::bad_alloc&) {             \
+    set_error("host allocation failed");      \
+    return OTM_ERR_ALLOC;                     \
+  }
+
+int check_ready(otm_solver* s) {
+  if (!s) { set_error("null solver handle"); return OTM_ERR_ARG; }
+  if (s->n <= 0) { set_error("solver not initialised: call otm_solver_init(n) first"); return OTM_ERR_STATE; }
+  return OTM_OK;
+}
+
+void enqueue_iteration(otm_solver* s) {
+  launch_dct_solve(s->plan, s->f, s->f.r, s->f.dhat, s->stream);
+  launch_beta(s->f, s->stream);
+  launch_dir_coef(s->f, s->stream);
+  launch_update(s->f, s->stream);
+}
+
+int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_in, double* d_fwd, double* d_adj,
+                 otm_stats* stats) {
+  otm_options opt;
+  otm_options_init(&opt);
+  if (opt_in) opt = *opt_in;
+  if (opt.max_iter < 0) opt.max_iter = 0;
+  if (opt.precision != OTM_PRECISION_FP64 && opt.precision != OTM_PRECISION_MIXED) {
+    set_error("unknown precision mode");
+    return OTM_ERR_ARG;
+  }
+  Fields& f = s->f;
+  hipStream_t st = s->stream;
+  ensure_trace(s, std::max(opt.max_iter, 1));
+  OTM_HIP(hipEventRecord(s->ev0, st));
+  launch_adjust_density(d_density, f, opt.max_ratio, st);
+  launch_fill(f.psi, f.F, 0., st);
+  launch_fill(f.g0, 2 * f.V, 0., st);
+  launch_fill(f.d[0], f.F, 0., st);
+  launch_fill(f.d[1], f.F, 0., st);
+  launch_initial_residual(f, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, st);
+  // Iterations are enqueued in chunks; the stop flag is read back once per
+  // chunk.  Iterations enqueued after the stop are no-ops on the device.
+  long enqueued = 0;
+  int chunk = 4;
+  for (;;) {
+    for (int c = 0; c < chunk && enqueued <= long(opt.max_iter); ++c, ++enqueued) enqueue_iteration(s);
+    OTM_HIP(hipMemcpyAsync(s->h_state, f.state, sizeof(SolveState), hipMemcpyDeviceToHost, st));
+    OTM_HIP(hipStreamSynchronize(st));
+    if (s->h_state->done) break;
+    if (enqueued > long(opt.max_iter)) {
+      set_error("internal error: solve did not reach its stop condition");
+      return OTM_ERR_STATE;
+    }
+    chunk = std::min(chunk * 2, 16);
+  }
+  launch_forward_positions(f.psi, d_fwd, s->n, st);
+  if (d_adj) OTM_HIP(hipMemcpyAsync(d_adj, f.rho, f.F * sizeof(double), hipMemcpyDeviceToDevice, st));
+  OTM_HIP(hipEventRecord(s->ev1, st));
+  OTM_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  OTM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  const SolveState& hs = *s->h_state;
+  s->last_iters = hs.it;
+  if (stats) {
+    stats->iterations = hs.it;
+    stats->exit_reason = hs.exit_reason;
+    stats->initial_residual = hs.e0;
+    stats->final_residual = hs.e;
+    stats->ms_solve = ms;
+  }
+  return OTM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void otm_options_init(otm_options* opt) {
+  if (!opt) return;
+  opt->beta = OTM_BETA_CONJUGATE_JACOBIAN;
+  opt->max_iter = 1000;
+  opt->threshold = 1e-7;
+  opt->max_ratio = DBL_MAX;
+  opt->precision = OTM_PRECISION_FP64;
+  opt->reserved = 0;
+}
+
+const char* otm_last_error(void) { return g_last_error.c_str(); }
+
+int otm_solver_create(otm_solver** out, int device) {
+  if (!out) { set_error("null output pointer"); return OTM_ERR_ARG; }
+  *out = nullptr;
+  OTM_GUARD_BEGIN
+  int count = 0;
+  OTM_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) { set_error("invalid HIP device index"); return OTM_ERR_ARG; }
+  OTM_HIP(hipSetDevice(device));
+  auto* s = new otm_solver;
+  s->device = device;
+  OTM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_state), sizeof(SolveState), hipHostMallocDefault));
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_sum), 8 * 4096 * sizeof(double), hipHostMallocDefault));
+  OTM_HIP(hipEventCreate(&s->ev0));
+  OTM_HIP(hipEventCreate(&s->ev1));
+  *out = s;
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_solver_destroy(otm_solver* s) {
+  if (!s) return OTM_OK;
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  if (s->stream) OTM_HIP(hipStreamSynchronize(s->stream));
+  release_fields(s);
+  if (s->h_state) (void)hipHostFree(s->h_state);
+  if (s->h_sum) (void)hipHostFree(s->h_sum);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_solver_set_verbose_level(otm_solver* s, int level) {
+  if (!s) { set_error("null solver handle"); return OTM_ERR_ARG; }
+  s->verbose = level;
+  return OTM_OK;
+}
+
+int otm_solver_init(otm_solver* s, int n) {
+  if (!s) { set_error("null solver handle"); return OTM_ERR_ARG; }
+  if (n < 1 || n > 16384) { set_error("grid size must be in [1, 16384]"); return OTM_ERR_ARG; }
+  if (s->n == n) return OTM_OK;  // already set up (otsolver_2dgrid.cpp:79-83)
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  release_fields(s);
+  allocate(s, n);
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_solver_grid_size(const otm_solver* s) { return s ? s->n : 0; }
+
+int otm_solve_device(otm_solver* s, const double* d_density, const otm_options* opt, double* d_fwd_xy,
+                     double* d_adjusted_density, otm_stats* stats) {
+  if (int rc = check_ready(s)) return rc;
+  if (!d_density || !d_fwd_xy) { set_error("null density or output pointer"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  return solve_device(s, d_density, opt, d_fwd_xy, d_adjusted_density, stats);
+  OTM_GUARD_END
+}
+
+int otm_solve(otm_solver* s, const double* density, const otm_options* opt, double* fwd_xy, double* adjusted_density,
+              otm_stats* stats) {
+  if (int rc = check_ready(s)) return rc;
+  if (!density || !fwd_xy) { set_error("null density or output pointer"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  const Fields& f = s->f;
+  OTM_HIP(hipMemcpyAsync(s->d_in, density, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  int rc = solve_device(s, s->d_in, opt, s->d_xy, nullptr, stats);
+  if (rc) return rc;
+  OTM_HIP(hipMemcpyAsync(fwd_xy, s->d_xy, 2 * f.V * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  if (adjusted_density)
+    OTM_HIP(hipMemcpyAsync(adjusted_density, f.rho, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, double* beta, int cap) {
+  if (!s || s->n <= 0) { set_error("solver not initialised"); return OTM_ERR_STATE; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  const int k = std::min(std::min(cap, s->last_iters), s->f.trace_cap);
+  if (k <= 0) return 0;
+  const int tc = s->f.trace_cap;
+  if (residual) OTM_HIP(hipMemcpy(residual, s->f.trace, k * sizeof(double), hipMemcpyDeviceToHost));
+  if (alpha) OTM_HIP(hipMemcpy(alpha, s->f.trace + tc, k * sizeof(double), hipMemcpyDeviceToHost));
+  if (beta) OTM_HIP(hipMemcpy(beta, s->f.trace + 2 * tc, k * sizeof(double), hipMemcpyDeviceToHost));
+  return k;
+  OTM_GUARD_END
+}
+
+int otm_vertex_gradients(otm_solver* s, const double* psi, double* grads) {
+  if (int rc = check_ready(s)) return rc;
+  if (!psi || !grads) { set_error("null pointer"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  const Fields& f = s->f;
+  OTM_HIP(hipMemcpyAsync(s->d_in, psi, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  launch_vertex_gradients(s->d_in, s->d_xy, s->n, s->stream);
+  OTM_HIP(hipMemcpyAsync(grads, s->d_xy, 2 * f.V * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_residual(otm_solver* s, const double* adjusted_density, const double* psi, double* r, double* e) {
+  if (int rc = check_ready(s)) return rc;
+  if (!adjusted_density || !psi || !r) { set_error("null pointer"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  const Fields& f = s->f;
+  const int nb = num_tiles(s->n);
+  OTM_HIP(hipMemcpyAsync(f.rho, adjusted_density, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  OTM_HIP(hipMemcpyAsync(f.psi, psi, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  launch_residual_only(f.psi, f.rho, f.r, f.partials, s->n, s->stream);
+  OTM_HIP(hipMemcpyAsync(r, f.r, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  std::vector<double> part(nb);
+  OTM_HIP(hipMemcpyAsync(part.data(), f.partials, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  double acc = 0.;
+  for (double p : part) acc += p;
+  if (e) *e = acc / (1.0 / (double(s->n) * double(s->n)));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat) {
+  if (int rc = check_ready(s)) return rc;
+  if (!r || !d_hat) { set_error("null pointer"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  const Fields& f = s->f;
+  double sum = 0.;
+  for (long k = 0; k < f.F; ++k) sum += r[k];
+  SolveState st;
+  std::memset(&st, 0, sizeof(st));
+  st.sum_r = sum;
+  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->stream));
+  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  launch_dct_solve(s->plan, f, f.r, f.dhat, s->stream);
+  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* dir, double* a, double* b) {
+  if (int rc = check_ready(s)) return rc;
+  if (!g0 || !dir || !a || !b) { set_error("null pointer"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  const Fields& f = s->f;
+  OTM_HIP(hipMemcpyAsync(f.g0, g0, 2 * f.V * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  OTM_HIP(hipMemcpyAsync(f.dhat, dir, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  launch_params_1d(f.g0, f.dhat, f.a, f.b, s->n, s->stream);
+  OTM_HIP(hipMemcpyAsync(a, f.a, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipMemcpyAsync(b, f.b, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// Stencil, reduction and device-side scalar kernels of the grid transport solver.
+//
+// Every stencil kernel works on a TI x TJ tile of faces (TJ along the
+// contiguous index j) with a one-face halo staged in LDS, derives the
+// (TI+1) x (TJ+1) vertex gradients of the tile in LDS, and then evaluates its
+// per-face quantity.  Reductions write one partial per workgroup; the last
+// workgroup to finish (atomic ticket) folds the partials in a fixed order, so
+// results are deterministic and the scalar logic of an iteration (beta, the
+// quartic line search, convergence) runs on the device without host syncs.
+//
+// Arithmetic follows the reference's operation order (the library is built
+// with -ffp-contract=off):
+//   vertex gradients       otlib/otsolver_2dgrid.cpp:362-423
+//   quad area              otlib/otsolver_2dgrid.cpp:425-447
+//   residual               otlib/otsolver_2dgrid.cpp:449-465
+//   conjugate-Jacobian     otlib/otsolver_2dgrid.cpp:248-261
+//   1-D problem parameters otlib/otsolver_2dgrid.cpp:467-539
+//   quartic line search    otlib/otsolver_2dgrid.cpp:541-606
+//   density adjustment     otlib/otsolver_2dgrid.cpp:47-73
+
+#include <cfloat>
+
+#include "otm_internal.h"
+#include "reduce.h"
+
+namespace otm {
+namespace {
+
+constexpr int LDF = TJ + 2;         // face-halo row length
+constexpr int NF = (TI + 2) * LDF;  // face-halo values per tile
+constexpr int LDV = TJ + 1;         // tile vertex row length
+constexpr int NV = (TI + 1) * LDV;  // tile vertices
+
+// Loads face values fn(face) on [i0-1, i0+TI] x [j0-1, j0+TJ] (0 outside the grid).
+template <class Fn>
+__device__ inline void load_halo(double* s, int n, int i0, int j0, Fn fn) {
+  for (int t = threadIdx.x; t < NF; t += NT) {
+    const int a = t / LDF, b = t - a * LDF;
+    const int fi = i0 - 1 + a, fj = j0 - 1 + b;
+    s[t] = (fi >= 0 && fi < n && fj >= 0 && fj < n) ? fn(long(fi) * n + fj) : 0.0;
+  }
+}
+
+// Gradient at vertex (vi, vj) = tile-local (a, b) from the face halo s.
+// Interior vertices: 2-wide SSE packet order for vj < s2, scalar-tail order
+// for vj >= s2, s2 = 2*floor((n-1)/2) (otlib/otsolver_2dgrid.cpp:374-402).
+__device__ inline void vertex_grad(const double* s, int n, int a, int b, int vi, int vj, double& gx, double& gy) {
+  const double w = double(n);
+  const double hw = 0.5 * w;
+  const double p00 = s[a * LDF + b];            // face (vi-1, vj-1)
+  const double p01 = s[a * LDF + b + 1];        // face (vi-1, vj)
+  const double p10 = s[(a + 1) * LDF + b];      // face (vi,   vj-1)
+  const double p11 = s[(a + 1) * LDF + b + 1];  // face (vi,   vj)
+  const bool ei = (vi == 0) || (vi == n);
+  const bool ej = (vj == 0) || (vj == n);
+  if (ei && ej) {
+    gx = 0.; gy = 0.;
+  } else if (vj == 0) {
+    gx = w * (p11 - p01); gy = 0.;
+  } else if (vj == n) {
+    gx = w * (p10 - p00); gy = 0.;
+  } else if (vi == 0) {
+    gx = 0.; gy = w * (p11 - p10);
+  } else if (vi == n) {
+    gx = 0.; gy = w * (p01 - p00);
+  } else if (vj < ((n - 1) / 2) * 2) {
+    gx = hw * ((p10 + p11) - (p00 + p01));
+    gy = hw * ((p01 + p11) - (p00 + p10));
+  } else {
+    gx = hw * (((p10 + p11) - p00) - p01);
+    gy = hw * (((p01 + p11) - p00) - p10);
+  }
+}
+
+__device__ inline void tile_grads(const double* s, double* gx, double* gy, int n, int i0, int j0) {
+  for (int t = threadIdx.x; t < NV; t += NT) {
+    const int a = t / LDV, b = t - a * LDV;
+    const int vi = i0 + a, vj = j0 + b;
+    double x = 0., y = 0.;
+    if (vi <= n && vj <= n) vertex_grad(s, n, a, b, vi, vj, x, y);
+    gx[t] = x;
+    gy[t] = y;
+  }
+}
+
+// half the cross product of the quad diagonals, tile-local face (a, b)
+__device__ inline double quad_area(const double* gx, const double* gy, int a, int b, double e) {
+  const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
+  return 0.5 * ((gx[v11] - gx[v00] + e) * (gy[v01] - gy[v10] + e) -
+                (gy[v11] - gy[v00] + e) * (gx[v01] - gx[v10] - e));
+}
+
+// ---------------------------------------------------------------------------
+// Exact quartic line search on the device.
+// e(t) = z0 + z1 t + z2 t^2 + z3 t^3 + z4 t^4; the stationary points are the
+// real roots of e'(t)/w3 = t^3 + c2 t^2 + c1 t + c0 (w = (z1, 2z2, 3z3, 4z4),
+// c = w/w3 as the companion matrix of the reference).  Roots come from the
+// closed form of the depressed cubic (Cardano for one real root, Viete's
+// trigonometric form for three) and are polished by Newton steps.  Among the
+// real roots in [-10, 10] the one of least e(t) wins (first on ties); none
+// found -> alpha = 0.5.
+// ---------------------------------------------------------------------------
+__device__ inline double cubic_eval(double t, double c2, double c1, double c0) { return ((t + c2) * t + c1) * t + c0; }
+
+__device__ inline double newton_polish(double t, double c2, double c1, double c0) {
+  for (int k = 0; k < 4; ++k) {
+    const double f = cubic_eval(t, c2, c1, c0);
+    const double df = (3. * t + 2. * c2) * t + c1;
+    if (f == 0. || df == 0.) break;
+    const double tn = t - f / df;
+    if (!(fabs(cubic_eval(tn, c2, c1, c0)) < fabs(f))) break;
+    t = tn;
+  }
+  return t;
+}
+
+__device__ inline int cubic_roots_dev(double c2, double c1, double c0, double* roots) {
+  const double sh = -c2 / 3.;
+  const double p = c1 - c2 * c2 / 3.;
+  const double q = (2. * c2 * c2 * c2) / 27. - (c2 * c1) / 3. + c0;
+  const double hq = 0.5 * q, tp = p / 3.;
+  const double disc = hq * hq + tp * tp * tp;
+  int k = 0;
+  if (disc > 0.) {
+    const double sq = sqrt(disc);
+    const double A = -copysign(cbrt(fabs(hq) + sq), hq);
+    const double B = (A != 0.) ? -tp / A : 0.;
+    roots[k++] = A + B + sh;
+  } else if (p == 0.) {
+    roots[k++] = sh;  // triple root
+  } else {
+    const double m = 2. * sqrt(-tp);
+    double arg = (3. * q) / (p * m);
+    arg = fmin(1., fmax(-1., arg));
+    const double phi = acos(arg) / 3.;
+    const double two_pi_3 = 2.0943951023931954923;  // 2*pi/3
+    roots[k++] = m * cos(phi) + sh;
+    roots[k++] = m * cos(phi - two_pi_3) + sh;
+    roots[k++] = m * cos(phi - 2. * two_pi_3) + sh;
+  }
+  for (int i = 0; i < k; ++i) roots[i] = newton_polish(roots[i], c2, c1, c0);
+  return k;
+}
+
+__device__ inline double pick_step(const double* z, double& emin) {
+  const double w0 = z[1], w1 = 2. * z[2], w2 = 3. * z[3], w3 = 4. * z[4];
+  const double c0 = w0 / w3, c1 = w1 / w3, c2 = w2 / w3;
+  double roots[3];
+  int nr = 0;
+  if (isfinite(c0) && isfinite(c1) && isfinite(c2)) nr = cubic_roots_dev(c2, c1, c0, roots);
+  bool found = false;
+  double alpha = 0., rmin = 0.;
+  for (int k = 0; k < nr; ++k) {
+    const double t = roots[k];
+    if (t >= -10. && t <= 10.) {
+      const double r = z[0] + t * (z[1] + t * (z[2] + t * (z[3] + t * z[4])));
+      if (!found || r < rmin) { found = true; alpha = t; rmin = r; }
+    }
+  }
+  if (!found) {
+    alpha = 0.5;
+    rmin = z[0] + alpha * (z[1] + alpha * (z[2] + alpha * (z[3] + alpha * z[4])));
+  }
+  emin = rmin;
+  return alpha;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_fill(double* p, long count, double v) {
+  for (long i = long(blockIdx.x) * NT + threadIdx.x; i < count; i += long(gridDim.x) * NT) p[i] = v;
+}
+
+// density adjustment: sum / min / max, then scale + ratio cap (otsolver_2dgrid.cpp:47-73)
+__global__ __launch_bounds__(NT) void k_adjust_reduce(const double* in, long F, double A, double* partials,
+                                                      unsigned* counter, SolveState* st) {
+  double s = 0., mn = DBL_MAX, mx = -DBL_MAX;
+  for (long i = long(blockIdx.x) * NT + threadIdx.x; i < F; i += long(gridDim.x) * NT) {
+    const double v = in[i];
+    s += v;
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  // min/max through the sum machinery on negated/duplicated slots is not
+  // order-free; do them with explicit shuffles instead.
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fmin(mn, __shfl_down(mn, off));
+    mx = fmax(mx, __shfl_down(mx, off));
+  }
+  __shared__ double mm[2][NT / 64];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { mm[0][w] = mn; mm[1][w] = mx; }
+  double v1[1] = {s};
+  block_sum<1>(v1);  // contains a __syncthreads after mm writes
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+    double bmn = mm[0][0], bmx = mm[1][0];
+    for (int k = 1; k < NT / 64; ++k) { bmn = fmin(bmn, mm[0][k]); bmx = fmax(bmx, mm[1][k]); }
+    store_agent(&partials[blockIdx.x], v1[0]);
+    store_agent(&partials[nb + blockIdx.x], bmn);
+    store_agent(&partials[2 * nb + blockIdx.x], bmx);
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    is_last = (prev == unsigned(nb - 1)) ? 1 : 0;
+    if (is_last) __threadfence();
+  }
+  __syncthreads();
+  if (!is_last) return;
+  if (threadIdx.x == 0) {
+    double tot = 0., gmn = DBL_MAX, gmx = -DBL_MAX;
+    for (int j = 0; j < nb; ++j) {
+      tot += load_agent(&partials[j]);
+      gmn = fmin(gmn, load_agent(&partials[nb + j]));
+      gmx = fmax(gmx, load_agent(&partials[2 * nb + j]));
+    }
+    const double I = tot * A;
+    const double nmn = gmn / I, nmx = gmx / I;  // division by I > 0 is monotone
+    const double ratio = fmin(st->max_ratio, nmx / nmn);
+    const double cand = (ratio * nmn - nmx) / (1. - ratio);
+    st->dens_scale = I;
+    st->dens_shift = (0. < cand) ? cand : 0.;  // std::max(0., cand), NaN -> 0
+    *counter = 0u;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_adjust_apply(const double* in, double* rho, long F, const SolveState* st) {
+  const double I = st->dens_scale, b = st->dens_shift;
+  for (long i = long(blockIdx.x) * NT + threadIdx.x; i < F; i += long(gridDim.x) * NT)
+    rho[i] = (in[i] / I + b) / (1. + b);
+}
+
+// initial residual at psi = 0 and solver state initialisation
+__global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, double* r, int n, double* partials,
+                                                        unsigned* counter, SolveState* st, int max_iter,
+                                                        double threshold, int cj) {
+  __shared__ double gx[NV], gy[NV];
+  __shared__ double s[NF];
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  load_halo(s, n, i0, j0, [](long) { return 0.0; });
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const double e = 1. / double(n), A = 1.0 / (double(n) * double(n));
+  double acc[2] = {0., 0.};
+  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+    const int a = t / TJ, b = t % TJ;
+    const int fi = i0 + a, fj = j0 + b;
+    if (fi < n && fj < n) {
+      const long f = long(fi) * n + fj;
+      const double rv = quad_area(gx, gy, a, b, e) - A * rho[f];
+      r[f] = rv;
+      acc[0] += rv * rv;
+      acc[1] += rv;
+    }
+  }
+  if (!publish_partials<2>(acc, partials, counter)) return;
+  fold_partials<2>(acc, partials, counter, block_count());
+  if (threadIdx.x == 0) {
+    const double e0 = acc[0] / A;
+    st->e = e0;
+    st->e0 = e0;
+    st->sum_r = acc[1];
+    st->alpha = 0.;
+    st->beta = 0.;
+    st->dnorm = 0.;
+    st->it = 0;
+    st->max_iter = max_iter;
+    st->threshold = threshold;
+    st->cj = cj;
+    st->update = 0;
+    st->use_prev = 0;
+    const bool run = (0 < max_iter) && (e0 > threshold);
+    st->done = run ? 0 : 1;
+    st->exit_reason = run ? 0 : 1;
+  }
+}
+
+// conjugate-Jacobian beta: residual at psi - eps*d_hat (eps = alpha/2),
+// beta = max(-1, Jd.(r_eps - r_k) / |Jd|^2 / eps * alpha)   (otsolver_2dgrid.cpp:248-261)
+__global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dhat, const double* rho, const double* r,
+                                            const double* jd, int n, double* partials, unsigned* counter,
+                                            SolveState* st) {
+  if (st->done) return;
+  if (st->it < 1 || !st->cj) {
+    if (block_id() == 0 && threadIdx.x == 0) st->use_prev = 0;  // d = d_hat, beta unchanged
+    return;
+  }
+  __shared__ double gx[NV], gy[NV];
+  __shared__ double s[NF];
+  const double eps = st->alpha / 2.;
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  load_halo(s, n, i0, j0, [&](long f) { return psi[f] - eps * dhat[f]; });
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const double e = 1. / double(n), A = 1.0 / (double(n) * double(n));
+  double acc[2] = {0., 0.};  // Jd.(r_eps - r), Jd.Jd
+  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+    const int a = t / TJ, b = t % TJ;
+    const int fi = i0 + a, fj = j0 + b;
+    if (fi < n && fj < n) {
+      const long f = long(fi) * n + fj;
+      const double reps = quad_area(gx, gy, a, b, e) - A * rho[f];
+      const double j = jd[f];
+      acc[0] += j * (reps - r[f]);
+      acc[1] += j * j;
+    }
+  }
+  if (!publish_partials<2>(acc, partials, counter)) return;
+  fold_partials<2>(acc, partials, counter, block_count());
+  if (threadIdx.x == 0) {
+    const double cand = acc[0] / acc[1] / eps * st->alpha;
+    st->beta = (-1. < cand) ? cand : -1.;  // std::max(-1., cand)
+    st->use_prev = 1;
+  }
+}
+
+// direction d = d_hat (+ beta d_prev), vertex gradients of d, quartic
+// coefficients a, b per quad, and the four dot products of the line search;
+// the last workgroup picks alpha and tests convergence.
+__global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* const* dbuf, const double* g0,
+                                                const double* r, double* aout, double* bout, int n, double* partials,
+                                                unsigned* counter, SolveState* st, double* trace, int trace_cap) {
+  if (st->done) return;
+  __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
+  __shared__ double s[NF];
+  const int k = st->it;
+  const double* dprev = dbuf[k & 1];
+  double* dnew = dbuf[(k + 1) & 1];
+  const bool use_prev = st->use_prev != 0;
+  const double beta = st->beta;
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  if (use_prev)
+    load_halo(s, n, i0, j0, [&](long f) { return dhat[f] + beta * dprev[f]; });
+  else
+    load_halo(s, n, i0, j0, [&](long f) { return dhat[f]; });
+  const long V1 = n + 1;
+  const long V = V1 * V1;
+  for (int t = threadIdx.x; t < NV; t += NT) {
+    const int a = t / LDV, b = t - a * LDV;
+    const int vi = i0 + a, vj = j0 + b;
+    const bool in = vi <= n && vj <= n;
+    hx[t] = in ? g0[vi * V1 + vj] : 0.;
+    hy[t] = in ? g0[V + vi * V1 + vj] : 0.;
+  }
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const double e = 1. / double(n);
+  double acc[5] = {0., 0., 0., 0., 0.};  // b.r, a.r, b.b, a.b, a.a
+  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+    const int a = t / TJ, b = t % TJ;
+    const int fi = i0 + a, fj = j0 + b;
+    if (fi < n && fj < n) {
+      const long f = long(fi) * n + fj;
+      dnew[f] = s[(a + 1) * LDF + b + 1];
+      const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
+      const double d0ax = hx[v11] - hx[v00] + e;
+      const double d0ay = hy[v11] - hy[v00] + e;
+      const double d0bx = hx[v01] - hx[v10] - e;
+      const double d0by = hy[v01] - hy[v10] + e;
+      const double ddax = gx[v11] - gx[v00];
+      const double dday = gy[v11] - gy[v00];
+      const double ddbx = gx[v01] - gx[v10];
+      const double ddby = gy[v01] - gy[v10];
+      const double av = 0.5 * (ddax * ddby - dday * ddbx);
+      const double bv = 0.5 * (ddax * d0by - dday * d0bx + d0ax * ddby - d0ay * ddbx);
+      aout[f] = av;
+      bout[f] = bv;
+      const double rv = r[f];
+      acc[0] += bv * rv;
+      acc[1] += av * rv;
+      acc[2] += bv * bv;
+      acc[3] += av * bv;
+      acc[4] += av * av;
+    }
+  }
+  if (!publish_partials<5>(acc, partials, counter)) return;
+  fold_partials<5>(acc, partials, counter, block_count());
+  if (threadIdx.x == 0) {
+    const double A = 1.0 / (double(n) * double(n));
+    double z[5];
+    z[0] = st->e * A;
+    z[1] = 2. * acc[0];
+    z[2] = acc[2] + 2 * acc[1];
+    z[3] = 2. * acc[3];
+    z[4] = acc[4];
+    double emin;
+    const double alpha = pick_step(z, emin);
+    for (int q = 0; q < 5; ++q) st->z[q] = z[q];
+    st->alpha = alpha;
+    st->e = emin / A;
+    if (k < trace_cap) {
+      trace[k] = st->e;
+      trace[trace_cap + k] = alpha;
+      trace[2 * trace_cap + k] = st->beta;
+    }
+    const int it = k + 1;
+    st->it = it;
+    st->update = 1;
+    if (!(it < st->max_iter && st->e > st->threshold)) {
+      st->done = 1;
+      st->exit_reason = 1;
+    }
+  }
+}
+
+// line-search update: psi += alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
+// (otsolver_2dgrid.cpp:599-604); Jd = r_{k+1} - r_k; sum of r_{k+1} for the next solve.
+__global__ __launch_bounds__(NT) void k_update(double* psi, double* const* dbuf, double* g0, const double* a,
+                                              const double* b, double* r, double* jd, int n, double* partials,
+                                              unsigned* counter, SolveState* st) {
+  if (!st->update) return;
+  __shared__ double gx[NV], gy[NV];
+  __shared__ double s[NF];
+  const int k = st->it - 1;
+  const double* d = dbuf[(k + 1) & 1];
+  const double alpha = st->alpha;
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  load_halo(s, n, i0, j0, [&](long f) { return d[f]; });
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const long V1 = n + 1;
+  const long V = V1 * V1;
+  // vertices owned by this tile: [i0, i0+TI) x [j0, j0+TJ), plus the last row/column of the grid
+  for (int t = threadIdx.x; t < NV; t += NT) {
+    const int va = t / LDV, vb = t - va * LDV;
+    const int vi = i0 + va, vj = j0 + vb;
+    const bool own_i = (va < TI && vi <= n) || (vi == n);
+    const bool own_j = (vb < TJ && vj <= n) || (vj == n);
+    if (own_i && own_j && vi <= n && vj <= n) {
+      const long v = vi * V1 + vj;
+      g0[v] = g0[v] + alpha * gx[t];
+      g0[V + v] = g0[V + v] + alpha * gy[t];
+    }
+  }
+  double acc[1] = {0.};
+  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+    const int fa = t / TJ, fb = t % TJ;
+    const int fi = i0 + fa, fj = j0 + fb;
+    if (fi < n && fj < n) {
+      const long f = long(fi) * n + fj;
+      psi[f] = psi[f] + alpha * s[(fa + 1) * LDF + fb + 1];
+      const double rk = r[f];
+      const double rn = a[f] * (alpha * alpha) + b[f] * alpha + rk;
+      r[f] = rn;
+      jd[f] = rn - rk;
+      acc[0] += rn;
+    }
+  }
+  if (!publish_partials<1>(acc, partials, counter)) return;
+  fold_partials<1>(acc, partials, counter, block_count());
+  if (threadIdx.x == 0) {
+    st->sum_r = acc[0];
+    st->update = 0;
+  }
+}
+
+// vertex gradients of a face field into a (2V SoA) array
+__global__ __launch_bounds__(NT) void k_vertex_gradients(const double* psi, double* G, int n, int add_origin) {
+  __shared__ double gx[NV], gy[NV];
+  __shared__ double s[NF];
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  load_halo(s, n, i0, j0, [&](long f) { return psi[f]; });
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const long V1 = n + 1;
+  const long V = V1 * V1;
+  const double dx = 1. / double(n);
+  for (int t = threadIdx.x; t < NV; t += NT) {
+    const int va = t / LDV, vb = t - va * LDV;
+    const int vi = i0 + va, vj = j0 + vb;
+    const bool own_i = (va < TI) || (vi == n);
+    const bool own_j = (vb < TJ) || (vj == n);
+    if (own_i && own_j && vi <= n && vj <= n) {
+      const long v = vi * V1 + vj;
+      if (add_origin) {  // forward positions: origin point double(i)*(1/n) + gradient
+        G[v] = double(vi) * dx + gx[t];
+        G[V + v] = double(vj) * dx + gy[t];
+      } else {
+        G[v] = gx[t];
+        G[V + v] = gy[t];
+      }
+    }
+  }
+}
+
+// residual only (component entry point): r = area(psi) - A rho, partial |r|^2
+__global__ __launch_bounds__(NT) void k_residual_only(const double* psi, const double* rho, double* r, int n,
+                                                     double* partials) {
+  __shared__ double gx[NV], gy[NV];
+  __shared__ double s[NF];
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  load_halo(s, n, i0, j0, [&](long f) { return psi[f]; });
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const double e = 1. / double(n), A = 1.0 / (double(n) * double(n));
+  double acc[1] = {0.};
+  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+    const int a = t / TJ, b = t % TJ;
+    const int fi = i0 + a, fj = j0 + b;
+    if (fi < n && fj < n) {
+      const long f = long(fi) * n + fj;
+      const double rv = quad_area(gx, gy, a, b, e) - A * rho[f];
+      r[f] = rv;
+      acc[0] += rv * rv;
+    }
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) partials[block_id()] = acc[0];
+}
+
+// quartic coefficients only (component entry point)
+__global__ __launch_bounds__(NT) void k_params_1d(const double* g0, const double* dir, double* aout, double* bout,
+                                                 int n) {
+  __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
+  __shared__ double s[NF];
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  load_halo(s, n, i0, j0, [&](long f) { return dir[f]; });
+  const long V1 = n + 1;
+  const long V = V1 * V1;
+  for (int t = threadIdx.x; t < NV; t += NT) {
+    const int a = t / LDV, b = t - a * LDV;
+    const int vi = i0 + a, vj = j0 + b;
+    const bool in = vi <= n && vj <= n;
+    hx[t] = in ? g0[vi * V1 + vj] : 0.;
+    hy[t] = in ? g0[V + vi * V1 + vj] : 0.;
+  }
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const double e = 1. / double(n);
+  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+    const int a = t / TJ, b = t % TJ;
+    const int fi = i0 + a, fj = j0 + b;
+    if (fi < n && fj < n) {
+      const long f = long(fi) * n + fj;
+      const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
+      const double d0ax = hx[v11] - hx[v00] + e;
+      const double d0ay = hy[v11] - hy[v00] + e;
+      const double d0bx = hx[v01] - hx[v10] - e;
+      const double d0by = hy[v01] - hy[v10] + e;
+      const double ddax = gx[v11] - gx[v00];
+      const double dday = gy[v11] - gy[v00];
+      const double ddbx = gx[v01] - gx[v10];
+      const double ddby = gy[v01] - gy[v10];
+      aout[f] = 0.5 * (ddax * ddby - dday * ddbx);
+      bout[f] = 0.5 * (ddax * d0by - dday * d0bx + d0ax * ddby - d0ay * ddbx);
+    }
+  }
+}
+
+inline dim3 tile_grid(int n) { return dim3(tiles_j(n), tiles_i(n)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_fill(double* p, long count, double v, hipStream_t s) {
+  if (count <= 0) return;
+  const long blocks = std::min<long>((count + NT - 1) / NT, 4096);
+  hipLaunchKernelGGL(k_fill, dim3(unsigned(blocks)), dim3(NT), 0, s, p, count, v);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_adjust_density(const double* in, const Fields& f, double max_ratio, hipStream_t s) {
+  const long blocks = std::min<long>((f.F + NT * 4 - 1) / (NT * 4), 1024);
+  // the ratio cap is read by the reduction's last workgroup
+  OTM_HIP(hipMemcpyAsync(&f.state->max_ratio, &max_ratio, sizeof(double), hipMemcpyHostToDevice, s));
+  const double A = 1.0 / (double(f.n) * double(f.n));
+  hipLaunchKernelGGL(k_adjust_reduce, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.F, A, f.partials,
+                     f.counters + CNT_ADJUST, f.state);
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_adjust_apply, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.rho, f.F,
+                     (const SolveState*)f.state);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_initial_residual(const Fields& f, int max_iter, double threshold, int cj, hipStream_t s) {
+  hipLaunchKernelGGL(k_initial_residual, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.rho, f.r, f.n, f.partials,
+                     f.counters + CNT_RESID, f.state, max_iter, threshold, cj);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_beta(const Fields& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_beta, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.psi, (const double*)f.dhat,
+                     (const double*)f.rho, (const double*)f.r, (const double*)f.jd, f.n, f.partials,
+                     f.counters + CNT_BETA, f.state);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_forward_positions(const double* psi, double* xy, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_vertex_gradients, tile_grid(n), dim3(NT), 0, s, psi, xy, n, 1);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_vertex_gradients, tile_grid(n), dim3(NT), 0, s, psi, G, n, 0);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_residual_only(const double* psi, const double* rho, double* r, double* partials, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual_only, tile_grid(n), dim3(NT), 0, s, psi, rho, r, n, partials);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_params_1d(const double* g0, const double* dir, double* a, double* b, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_params_1d, tile_grid(n), dim3(NT), 0, s, g0, dir, a, b, n);
+  OTM_HIP(hipGetLastError());
+}
+
+// The direction / update kernels take the d ping-pong pair through a small
+// device array so that one captured iteration serves every iteration.
+void launch_dir_coef(const Fields& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_dir_coef, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.dhat,
+                     (double* const*)f.dpair, (const double*)f.g0,
+                     (const double*)f.r, f.a, f.b, f.n, f.partials, f.counters + CNT_COEF, f.state, f.trace,
+                     f.trace_cap);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_update(const Fields& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_update, tile_grid(f.n), dim3(NT), 0, s, f.psi,
+                     (double* const*)f.dpair, f.g0, (const double*)f.a,
+                     (const double*)f.b, f.r, f.jd, f.n, f.partials, f.counters + CNT_UPDATE, f.state);
+  OTM_HIP(hipGetLastError());
+}
+}  // namespace otm

@@ -1,0 +1,42 @@
+"""CPU checks of the C ABI: the library loads and exports every declared symbol.
+
+No compute entry point is called (no GPU here); parity runs in the gpu tests.
+"""
+import ctypes
+import os
+
+import pytest
+
+from otmap_amd import _lib
+
+
+def test_library_built():
+    assert os.path.exists(_lib.LIB_PATH), "run __graft_entry__.build() first"
+
+
+def test_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 10
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_declared_symbols():
+    assert set(_lib.declared_symbols()) <= set(_lib._SIGNATURES)
+
+
+def test_options_defaults_match_reference():
+    # SolverOptions defaults, otlib/otsolver_2dgrid.h:37-43
+    o = _lib.OtmOptions()
+    _lib.lib().otm_options_init(o)
+    assert o.beta == _lib.OTM_BETA_CONJUGATE_JACOBIAN
+    assert o.max_iter == 1000 and o.threshold == 1e-7
+    assert o.max_ratio == pytest.approx(1.7976931348623157e308)
+    assert o.precision == _lib.OTM_PRECISION_FP64
+
+
+def test_errors_without_device_are_reported():
+    # null handles are rejected before any HIP call
+    assert _lib.lib().otm_solver_init(None, 16) != 0
+    assert b"null" in _lib.lib().otm_last_error()

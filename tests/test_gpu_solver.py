@@ -1,0 +1,132 @@
+"""GPU parity of the solver path against the FP64 oracle (HIP through the C ABI).
+
+Bars:
+  * stencils (vertex gradients, residual r, quartic coefficients a, b):
+    bit-exact -- the kernels keep the reference's operation order and the
+    library is built without FMA contraction;
+  * scalar reductions (|r|^2/A): 1e-13 relative (summation order differs);
+  * Laplacian solve: DCT-on-MFMA vs the oracle's sparse LDL^T, 1e-11 relative
+    to max|d_hat| (both are O(1e-15) accurate; the bound leaves room for the
+    conditioning ~4n^2/pi^2 at n = 256);
+  * full solves (FP64 path): same iteration count (+-1 allowed, 0 observed),
+    forward vertex positions within 1e-9 in unit-square coordinates (the
+    drop-in bar is 1e-6), traces of alpha within 1e-7.
+"""
+import numpy as np
+import pytest
+
+from otmap_amd import densities as D
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 5, 16, 33, 64, 100, 256]
+
+
+def _rng(n):
+    return np.random.default_rng(12345 + n)
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_vertex_gradients_bitexact(n, oracle, solver_factory):
+    s = solver_factory(n)
+    psi = _rng(n).standard_normal(n * n) * 1e-2
+    G = s.vertex_gradients(psi)
+    Go = oracle.vertex_gradients(n, psi)
+    assert np.array_equal(G, Go), np.abs(G - Go).max()
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_residual_bitexact(n, oracle, solver_factory):
+    s = solver_factory(n)
+    rng = _rng(n)
+    rho = oracle.adjust_density(n, rng.uniform(0.2, 3.0, n * n))
+    psi = rng.standard_normal(n * n) * 1e-3
+    e, r = s.residual(rho, psi)
+    eo, ro = oracle.residual(n, rho, psi)
+    assert np.array_equal(r, ro), np.abs(r - ro).max()
+    assert abs(e - eo) <= 1e-13 * abs(eo) + 1e-300
+
+
+@pytest.mark.parametrize("n", [3, 5, 16, 33, 64, 100, 128, 256])
+def test_laplacian_solve_matches_ldlt(n, oracle, solver_factory):
+    s = solver_factory(n)
+    r = _rng(n).standard_normal(n * n)
+    for rhs in (r, r - r.mean()):
+        d = s.laplacian_solve(rhs)
+        do = oracle.laplacian_solve(n, rhs)
+        d = d - d.mean()
+        err = np.abs(d - do).max() / np.abs(do).max()
+        assert err < 1e-11, err
+
+
+@pytest.mark.parametrize("n", [2, 3, 16, 33, 64, 256])
+def test_line_search_coefficients_bitexact(n, oracle, solver_factory):
+    s = solver_factory(n)
+    rng = _rng(n)
+    g0 = oracle.vertex_gradients(n, rng.standard_normal(n * n) * 1e-3)
+    d = rng.standard_normal(n * n) * 1e-3
+    a, b = s.line_search_coefficients(g0, d)
+    ao, bo = oracle.params_1d(n, g0, d)
+    assert np.array_equal(a, ao) and np.array_equal(b, bo)
+
+
+@pytest.mark.parametrize("n,seed", [(16, 7), (32, 8), (64, 9), (128, 10), (256, 1000)])
+def test_full_solve_fp64_matches_oracle(n, seed, oracle, solver_factory):
+    rho = D.gaussian_mixture(n, seed)
+    s = solver_factory(n)
+    tm = s.solve(rho)
+    o = oracle.solve(n, rho)
+    st = s.last_stats
+    assert abs(st.iterations - o["iters"]) <= 1, (st.iterations, o["iters"])
+    assert st.final_residual <= 1e-7
+    err = np.abs(tm.fwd_mesh().xy - o["fwd"]).max()
+    assert err < 1e-9, err
+    assert np.allclose(tm.density(), o["density"], rtol=1e-14, atol=0)
+    res, alpha, beta = s.trace()
+    k = min(len(alpha), len(o["alpha"]))
+    assert np.abs(alpha[:k] - o["alpha"][:k]).max() < 1e-7
+    assert np.abs(beta[:k] - o["beta"][:k]).max() < 1e-7
+
+
+def test_constant_density_zero_iterations(solver_factory):
+    n = 64
+    s = solver_factory(n)
+    tm = s.solve(np.full(n * n, 3.0))
+    assert s.last_stats.iterations == 0
+    ref = np.concatenate([np.repeat(np.arange(n + 1.0), n + 1), np.tile(np.arange(n + 1.0), n + 1)]) * (1.0 / n)
+    assert np.abs(tm.fwd_mesh().xy - ref).max() <= 1e-15
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 3])
+def test_max_iter_cap(max_iter, oracle, solver_factory):
+    from otmap_amd import SolverOptions
+    n = 32
+    rho = D.gaussian_mixture(n, 3)
+    s = solver_factory(n)
+    tm = s.solve(rho, SolverOptions(max_iter=max_iter))
+    o = oracle.solve(n, rho, max_iter=max_iter)
+    assert s.last_stats.iterations == o["iters"] == max_iter
+    assert np.abs(tm.fwd_mesh().xy - o["fwd"]).max() < 1e-12
+
+
+def test_beta_zero_mode(oracle, solver_factory):
+    from otmap_amd import BetaOpt, SolverOptions
+    n = 32
+    rho = D.gaussian_mixture(n, 4)
+    s = solver_factory(n)
+    tm = s.solve(rho, SolverOptions(beta=BetaOpt.Zero))
+    o = oracle.solve(n, rho, beta_cj=False)
+    assert abs(s.last_stats.iterations - o["iters"]) <= 1
+    assert np.abs(tm.fwd_mesh().xy - o["fwd"]).max() < 1e-9
+
+
+def test_max_ratio_adjustment(oracle, solver_factory):
+    from otmap_amd import SolverOptions
+    n = 64
+    rho = D.gaussian_mixture(n, 5, floor_rel=1e-6)
+    s = solver_factory(n)
+    tm = s.solve(rho, SolverOptions(max_ratio=50.0))
+    o = oracle.solve(n, rho, max_ratio=50.0)
+    assert np.allclose(tm.density(), o["density"], rtol=1e-14, atol=0)
+    assert abs(tm.density().max() / tm.density().min() - 50.0) < 1e-9
+    assert np.abs(tm.fwd_mesh().xy - o["fwd"]).max() < 1e-9
