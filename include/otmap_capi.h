@@ -99,6 +99,19 @@ int otm_solve_device(otm_solver* s, const double* d_density, const otm_options* 
  * Returns the number of iterations written (<= cap). */
 int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, double* beta, int cap);
 
+/* Kernel timing (no reference counterpart; used by bench.py for the roofline
+ * figures): runs `reps` launches of one kernel of the iteration on the
+ * handle's stream, each between two HIP events, on the buffers of the last
+ * solve (which it overwrites), and returns the mean ms per launch. */
+enum {
+  OTM_KERNEL_DCT_GEMM = 0,  /* one n x n x n FP64-MFMA GEMM stage of the DCT solve */
+  OTM_KERNEL_BETA = 1,      /* conjugate-Jacobian residual + reductions */
+  OTM_KERNEL_DIR_COEF = 2,  /* direction, quartic coefficients + line search */
+  OTM_KERNEL_UPDATE = 3,    /* psi / G0 / r update */
+  OTM_KERNEL_DCT_SOLVE = 4  /* the whole Laplacian solve (4 GEMM stages) */
+};
+int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
+
 /* Component entry points: the reference's protected stencils, on host
  * buffers, for parity testing and for callers that drive the iteration.
  *   compute_vertex_gradients (otlib/otsolver_2dgrid.cpp:362-423): grads 2*(n+1)^2
@@ -111,6 +124,45 @@ int otm_vertex_gradients(otm_solver* s, const double* psi, double* grads);
 int otm_residual(otm_solver* s, const double* adjusted_density, const double* psi, double* r, double* e);
 int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat);
 int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* dir, double* a, double* b);
+
+/* ---- TransportMap (otlib/transport_map.h:29-63) -------------------------
+ * A map holds the forward mesh (2*(n+1)^2 SoA positions) and the adjusted
+ * density (n*n) of one solve.  The BVH of the forward quads is built on the
+ * first inverse query (or by otm_map_init_inverse), as the reference's lazy
+ * init_inverse() (otlib/transport_map.cpp:36-43). */
+/* TransportMap(origin, fwd, density) (otlib/transport_map.cpp:30-34) on host buffers */
+int otm_map_create(otm_map** out, int device, int n, const double* fwd_xy, const double* density);
+/* same from device buffers (e.g. the outputs of otm_solve_device) */
+int otm_map_create_device(otm_map** out, int device, int n, const double* d_fwd_xy, const double* d_density);
+/* ~TransportMap() (otlib/transport_map.cpp:45-48) */
+int otm_map_destroy(otm_map* m);
+/* init_inverse() (otlib/transport_map.cpp:36-43): BVH2D::build(fwd, 4, 24) */
+int otm_map_init_inverse(otm_map* m);
+/* number of BVH nodes (0 before the build) */
+int otm_map_bvh_size(const otm_map* m);
+/* inv / inv_fast / apply_inverse_map (otlib/transport_map.cpp:50-128) for
+ * `count` interleaved points.  mode 0 = inv (all overlapping faces, the
+ * reference's selection), 1 = inv_fast (first hit).  out: interleaved
+ * positions (NaN where no face holds the point -- the reference prints an
+ * error there); face (nullable): selected face index, -1 if none; nhits
+ * (nullable): number of overlapping faces found. */
+int otm_map_inverse(otm_map* m, const double* pts, int64_t count, int mode, double* out, int32_t* face,
+                    int32_t* nhits);
+/* same on device buffers */
+int otm_map_inverse_device(otm_map* m, const double* d_pts, int64_t count, int mode, double* d_out,
+                           int32_t* d_face, int32_t* d_nhits);
+
+/* ---- rasterize_image (otlib/utils/rasterizer.h:21-22, rasterizer.cpp:188-308)
+ * Per-face density mode on the (n+1)^2-vertex quad mesh `mesh_xy` (SoA):
+ * every pixel covered by a face gets float(1/n^2) * density(f) / |area(f)|,
+ * the largest covering face index wins, uncovered pixels are 0.  density may
+ * be NULL (all ones: the reference's 2-argument overload).  img is row-major
+ * rows x cols: img[y*cols + x] is the reference's img(y, x). */
+int otm_rasterize(int device, int n, const double* mesh_xy, const double* density, int rows, int cols, double* img);
+/* same on device buffers; img_f32 != 0 writes float, else double; d_winner:
+ * rows*cols int32 scratch (receives the winning face index, -1 if none). */
+int otm_rasterize_device(int device, int n, const double* d_mesh_xy, const double* d_density, int rows, int cols,
+                         int img_f32, void* d_img, int32_t* d_winner);
 
 #ifdef __cplusplus
 }

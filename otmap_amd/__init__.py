@@ -7,7 +7,7 @@ Drop-in mirror of the reference's API (ggael/otmap):
 backed by hand-written HIP kernels in libotmap_amd.so (C ABI: include/otmap_capi.h).
 """
 from .solver import BetaOpt, GridBasedTransportSolver, Precision, SolverOptions, SolveStats
-from .transport_map import GridMesh, TransportMap
+from .transport_map import GridMesh, TransportMap, apply_inverse_map, rasterize_image
 
 __all__ = ["BetaOpt", "GridBasedTransportSolver", "Precision", "SolverOptions", "SolveStats", "GridMesh",
-           "TransportMap"]
+           "TransportMap", "apply_inverse_map", "rasterize_image"]

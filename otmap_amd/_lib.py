@@ -18,6 +18,7 @@ HEADERS = [os.path.join(ROOT, "include", "otmap_capi.h")]
 OTM_OK = 0
 OTM_BETA_ZERO, OTM_BETA_CONJUGATE_JACOBIAN = 0, 1
 OTM_PRECISION_FP64, OTM_PRECISION_MIXED = 0, 1
+OTM_KERNEL_DCT_GEMM, OTM_KERNEL_BETA, OTM_KERNEL_DIR_COEF, OTM_KERNEL_UPDATE, OTM_KERNEL_DCT_SOLVE = range(5)
 
 
 class OtmOptions(C.Structure):
@@ -48,10 +49,20 @@ _SIGNATURES = {
     "otm_solve": (C.c_int, [_vp, _dp, C.POINTER(OtmOptions), _dp, _dp, C.POINTER(OtmStats)]),
     "otm_solve_device": (C.c_int, [_vp, _vp, C.POINTER(OtmOptions), _vp, _vp, C.POINTER(OtmStats)]),
     "otm_solver_trace": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int]),
+    "otm_solver_profile_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
     "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),
     "otm_line_search_coefficients": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "otm_map_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _dp, _dp]),
+    "otm_map_create_device": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp]),
+    "otm_map_destroy": (C.c_int, [_vp]),
+    "otm_map_init_inverse": (C.c_int, [_vp]),
+    "otm_map_bvh_size": (C.c_int, [_vp]),
+    "otm_map_inverse": (C.c_int, [_vp, _dp, C.c_int64, C.c_int, _dp, _ip, _ip]),
+    "otm_map_inverse_device": (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _vp, _vp, _vp]),
+    "otm_rasterize": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]),
+    "otm_rasterize_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
 }
 
 _lib = None

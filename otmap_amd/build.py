@@ -25,6 +25,8 @@ SOURCES = [
     "stencil.hip",
     "dct.hip",
     "solver.hip",
+    "inverse.hip",
+    "raster.hip",
 ]
 
 FLAGS = [

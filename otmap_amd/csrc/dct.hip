@@ -114,7 +114,7 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, c
     buf ^= 1;
   }
 
-  // epilogue: lane holds D[4*lk + v][lr] of each 16x16 tile
+  // epilogue: for v_mfma_f64_16x16x4f64 lane l holds D[l/16 + 4v][l%16] of each 16x16 tile
   double nrm[1] = {0.};
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, c
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm + i * 16 + lk * 4 + v;
+        const int row = m0 + wm + i * 16 + lk + 4 * v;
         const int col = n0 + wn + j * 16 + lr;
         if (EDGE && (row >= n || col >= n)) continue;
         double val = acc[i][j][v];
@@ -193,6 +193,11 @@ void dct_plan_destroy(DctPlan& p) {
   if (p.Ct) (void)hipFree(p.Ct);
   if (p.sq) (void)hipFree(p.sq);
   p = DctPlan();
+}
+
+// one plain stage (T3 = C^T Y) without gating, for kernel timing
+void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s) {
+  gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, p.n, nullptr, p.sq, f.partials, f.counters + CNT_AUX, s);
 }
 
 // d_hat = C^T [(C R' C^T) / lambda] C.  Every GEMM is skipped once state->done is set.

@@ -1,0 +1,414 @@
+// TransportMap inverse: BVH over the forward quads + one GPU query per lane.
+//
+// Reference: TransportMap::init_inverse / inv / inv_fast / apply_inverse_map
+// (otlib/transport_map.cpp:36-128) over BVH2D (otlib/utils/bvh2d.cpp:29-243).
+//
+// Build (host, the reference's algorithm): face centroids; per node the AABB of
+// its faces' vertices inflated by 2*eps*diag + eps; leaf when <= 4 faces or
+// depth >= 24; otherwise split the longest axis at the float-rounded box
+// midpoint with the reference's in-place two-pointer partition.  The same
+// partition keeps the same face order inside leaves and the same child
+// order, so hits come out in the reference's traversal order.
+//
+// Device layout: nodes flattened to 48-byte records {box, first, count,
+// escape}, where `escape` is the next node in depth-first order once a
+// subtree is finished or skipped (child1 -> child2 -> parent's escape).  A
+// query then walks the tree without a stack in exactly the reference's
+// recursive order (child 1 before child 2).
+//
+// Selection in inv() with several hits keeps the reference's behaviour
+// (otlib/transport_map.cpp:70-85): the face may switch to a later hit of
+// higher density, the weights stay those of the first hit, and after the
+// first switch the comparison value becomes the bool true (1.0).
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+#include "../../include/otmap_capi.h"
+#include "geometry.h"
+#include "otm_internal.h"
+
+using namespace otm;
+using namespace otm::geo;
+
+namespace otm {
+
+struct alignas(16) BNode {
+  double lox, loy, hix, hiy;
+  int first;   // inner: first child; leaf: first face slot
+  int count;   // leaf: number of faces; inner: -1
+  int escape;  // next node after this subtree (-1: end)
+  int pad;
+};
+
+namespace {
+
+struct HostBVH {
+  int n = 0;
+  std::vector<P2> pts;
+  std::vector<int> faces;
+  std::vector<P2> cent;
+  std::vector<BNode> nodes;
+};
+
+__host__ __device__ inline void quad_vertices(int n, int f, int* v) {
+  const int i = f / n, j = f - (f / n) * n;
+  v[0] = j + i * (n + 1);
+  v[1] = j + (i + 1) * (n + 1);
+  v[2] = (j + 1) + (i + 1) * (n + 1);
+  v[3] = (j + 1) + i * (n + 1);
+}
+
+int split(HostBVH& B, int start, int end, int dim, float sv) {
+  auto c = [&](int k) { return dim == 0 ? B.cent[k].x : B.cent[k].y; };
+  int l = start, r = end - 1;
+  while (l < r) {
+    while (l < end && c(l) < sv) ++l;
+    while (r >= start && c(r) >= sv) --r;
+    if (l >= r) break;
+    std::swap(B.cent[l], B.cent[r]);
+    std::swap(B.faces[l], B.faces[r]);
+    ++l;
+    --r;
+  }
+  if (l >= end) return end;
+  return c(l) <= sv ? std::min(end, l + 1) : l;
+}
+
+void build_node(HostBVH& B, int id, int start, int end, int level, int leaf_size, int max_depth) {
+  double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
+  for (int k = start; k < end; ++k) {
+    int v[4];
+    quad_vertices(B.n, B.faces[k], v);
+    for (int q = 0; q < 4; ++q) {
+      const P2 p = B.pts[v[q]];
+      lo[0] = std::min(lo[0], p.x); lo[1] = std::min(lo[1], p.y);
+      hi[0] = std::max(hi[0], p.x); hi[1] = std::max(hi[1], p.y);
+    }
+  }
+  const double eps = DBL_EPSILON;
+  const double diag[2] = {hi[0] - lo[0], hi[1] - lo[1]};
+  double blo[2], bhi[2];
+  for (int d = 0; d < 2; ++d) {
+    const double pad = diag[d] * (2. * eps) + eps;
+    blo[d] = lo[d] - pad;
+    bhi[d] = hi[d] + pad;
+  }
+  BNode nd;
+  nd.lox = blo[0]; nd.loy = blo[1]; nd.hix = bhi[0]; nd.hiy = bhi[1];
+  nd.first = start;
+  nd.count = std::max(0, end - start);
+  nd.escape = -1;
+  nd.pad = 0;
+  B.nodes[id] = nd;
+  if (end - start <= leaf_size || level >= max_depth) return;
+  const int dim = diag[1] > diag[0] ? 1 : 0;
+  const float sv = float(0.5 * (bhi[dim] + blo[dim]));
+  const int mid = split(B, start, end, dim, sv);
+  if (mid == start || mid == end) return;
+  const int child = int(B.nodes.size());
+  B.nodes[id].first = child;
+  B.nodes[id].count = -1;
+  B.nodes.resize(B.nodes.size() + 2);
+  build_node(B, child, start, mid, level + 1, leaf_size, max_depth);
+  build_node(B, child + 1, mid, end, level + 1, leaf_size, max_depth);
+}
+
+void build_bvh(HostBVH& B, int n, const double* xy, int leaf_size, int max_depth) {
+  B.n = n;
+  const long V = long(n + 1) * (n + 1), F = long(n) * n;
+  B.pts.resize(V);
+  for (long v = 0; v < V; ++v) B.pts[v] = {xy[v], xy[V + v]};
+  B.faces.resize(F);
+  B.cent.resize(F);
+  for (long f = 0; f < F; ++f) {
+    int v[4];
+    quad_vertices(n, int(f), v);
+    P2 c = {0., 0.};
+    for (int q = 0; q < 4; ++q) { c.x += B.pts[v[q]].x; c.y += B.pts[v[q]].y; }
+    c.x /= 4; c.y /= 4;
+    B.cent[f] = c;
+    B.faces[f] = int(f);
+  }
+  B.nodes.assign(1, BNode{});
+  B.nodes.reserve(size_t(F / 2 + 16));
+  build_node(B, 0, 0, int(F), 0, leaf_size, max_depth);
+  // escape links in depth-first order
+  std::vector<int> stack = {0};
+  B.nodes[0].escape = -1;
+  while (!stack.empty()) {
+    const int p = stack.back();
+    stack.pop_back();
+    if (B.nodes[p].count >= 0) continue;
+    const int c1 = B.nodes[p].first, c2 = c1 + 1;
+    B.nodes[c1].escape = c2;
+    B.nodes[c2].escape = B.nodes[p].escape;
+    stack.push_back(c2);
+    stack.push_back(c1);
+  }
+}
+
+__device__ inline bool box_contains(const BNode& b, P2 p) {
+  return b.lox <= p.x && b.loy <= p.y && p.x <= b.hix && p.y <= b.hiy;
+}
+
+// One query per lane.  mode 0: inv (all hits, reference selection), 1: inv_fast (first hit).
+__global__ __launch_bounds__(256) void k_inverse(const BNode* __restrict__ nodes, const int* __restrict__ faces,
+                                                const double* __restrict__ xy, const double* __restrict__ density,
+                                                int n, const double* __restrict__ pts, long m, int fast,
+                                                double* __restrict__ out, int* __restrict__ face_out,
+                                                int* __restrict__ nhits_out) {
+  const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const long V = long(n + 1) * (n + 1);
+  P2 q = {pts[2 * k], pts[2 * k + 1]};
+  q.x = fmin(fmax(q.x, 0.), 1.);  // snap to [0,1]^2 (transport_map.cpp:53)
+  q.y = fmin(fmax(q.y, 0.), 1.);
+  int hits = 0;
+  int first_face = -1, chosen = -1;
+  double w0[4] = {0., 0., 0., 0.};
+  double best = 0.;
+  // invariant: `node` holds q (or is -1 = traversal finished)
+  int node = box_contains(nodes[0], q) ? 0 : -1;
+  while (node >= 0) {
+    const BNode nd = nodes[node];
+    int next;
+    if (nd.count < 0) {
+      next = nd.first;  // inner node: child 1 first
+    } else {
+      bool stop = false;
+      for (int s = nd.first; s < nd.first + nd.count; ++s) {
+        const int f = faces[s];
+        int v[4];
+        quad_vertices(n, f, v);
+        P2 p[4];
+  #pragma unroll
+        for (int t = 0; t < 4; ++t) p[t] = {xy[v[t]], xy[V + v[t]]};
+        if (!inside_quad(q, p)) continue;
+        double w[4];
+        if (!bilinear_weights(q, p, w)) continue;
+        if (!weights_in_unit(w)) continue;
+        if (hits == 0) {
+          first_face = f;
+          chosen = f;
+          for (int t = 0; t < 4; ++t) w0[t] = w[t];
+          best = density[f];
+        } else if (density[f] > best) {
+          chosen = f;
+          best = (density[f] > best) ? 1.0 : 0.0;
+        }
+        ++hits;
+        if (fast) { stop = true; break; }
+      }
+      if (stop) break;
+      next = nd.escape;  // leaf done: next subtree in depth-first order
+    }
+    // skip subtrees whose box does not hold q (child1 -> child2 -> parent's escape ...)
+    while (next >= 0 && !box_contains(nodes[next], q)) next = nodes[next].escape;
+    node = next;
+  }
+  if (nhits_out) nhits_out[k] = hits;
+  if (face_out) face_out[k] = chosen;
+  if (hits == 0) {
+    out[2 * k] = __builtin_nan("");
+    out[2 * k + 1] = __builtin_nan("");
+    return;
+  }
+  (void)first_face;
+  int v[4];
+  quad_vertices(n, chosen, v);
+  const double dx = 1. / double(n);
+  double rx = 0., ry = 0.;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int vi = v[t] / (n + 1), vj = v[t] - vi * (n + 1);
+    const double ox = double(vi) * dx, oy = double(vj) * dx;
+    if (t == 0) { rx = w0[0] * ox; ry = w0[0] * oy; }
+    else { rx += w0[t] * ox; ry += w0[t] * oy; }
+  }
+  out[2 * k] = rx;
+  out[2 * k + 1] = ry;
+}
+
+}  // namespace
+}  // namespace otm
+
+struct otm_map {
+  int device = 0;
+  int n = 0;
+  hipStream_t stream = nullptr;
+  double* xy = nullptr;       // forward positions (device, 2V)
+  double* density = nullptr;  // adjusted density (device, F)
+  std::vector<double> h_xy;   // host copy for the build
+  BNode* nodes = nullptr;
+  int* faces = nullptr;
+  int num_nodes = 0;
+  double* d_pts = nullptr;    // staging for host queries
+  double* d_out = nullptr;
+  int* d_face = nullptr;
+  int* d_nhits = nullptr;
+  long staged = 0;
+  double ms_build = 0.;
+};
+
+namespace {
+
+int map_fail(const HipError& e) {
+  set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+  return OTM_ERR_HIP;
+}
+
+void map_release_staging(otm_map* m) {
+  if (m->d_pts) (void)hipFree(m->d_pts);
+  if (m->d_out) (void)hipFree(m->d_out);
+  if (m->d_face) (void)hipFree(m->d_face);
+  if (m->d_nhits) (void)hipFree(m->d_nhits);
+  m->d_pts = m->d_out = nullptr;
+  m->d_face = m->d_nhits = nullptr;
+  m->staged = 0;
+}
+
+void map_build(otm_map* m) {
+  if (m->nodes) return;
+  HostBVH B;
+  build_bvh(B, m->n, m->h_xy.data(), 4, 24);
+  m->num_nodes = int(B.nodes.size());
+  OTM_HIP(hipMalloc(&m->nodes, B.nodes.size() * sizeof(BNode)));
+  OTM_HIP(hipMalloc(&m->faces, B.faces.size() * sizeof(int)));
+  OTM_HIP(hipMemcpy(m->nodes, B.nodes.data(), B.nodes.size() * sizeof(BNode), hipMemcpyHostToDevice));
+  OTM_HIP(hipMemcpy(m->faces, B.faces.data(), B.faces.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void launch_inverse(otm_map* m, const double* d_pts, long count, int mode, double* d_out, int* d_face, int* d_nh) {
+  if (count <= 0) return;
+  const long blocks = (count + 255) / 256;
+  hipLaunchKernelGGL(k_inverse, dim3(unsigned(blocks)), dim3(256), 0, m->stream, (const BNode*)m->nodes,
+                     (const int*)m->faces, (const double*)m->xy, (const double*)m->density, m->n, d_pts, count,
+                     mode, d_out, d_face, d_nh);
+  OTM_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int otm_map_create(otm_map** out, int device, int n, const double* fwd_xy, const double* density) {
+  if (!out || !fwd_xy || !density || n < 1) { set_error("invalid map arguments"); return OTM_ERR_ARG; }
+  *out = nullptr;
+  try {
+    OTM_HIP(hipSetDevice(device));
+    auto* m = new otm_map;
+    m->device = device;
+    m->n = n;
+    const size_t V = size_t(n + 1) * (n + 1), F = size_t(n) * n;
+    m->h_xy.assign(fwd_xy, fwd_xy + 2 * V);
+    OTM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    OTM_HIP(hipMalloc(&m->xy, 2 * V * sizeof(double)));
+    OTM_HIP(hipMalloc(&m->density, F * sizeof(double)));
+    OTM_HIP(hipMemcpy(m->xy, fwd_xy, 2 * V * sizeof(double), hipMemcpyHostToDevice));
+    OTM_HIP(hipMemcpy(m->density, density, F * sizeof(double), hipMemcpyHostToDevice));
+    *out = m;
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+int otm_map_create_device(otm_map** out, int device, int n, const double* d_fwd_xy, const double* d_density) {
+  if (!out || !d_fwd_xy || !d_density || n < 1) { set_error("invalid map arguments"); return OTM_ERR_ARG; }
+  *out = nullptr;
+  try {
+    OTM_HIP(hipSetDevice(device));
+    const size_t V = size_t(n + 1) * (n + 1), F = size_t(n) * n;
+    std::vector<double> hxy(2 * V), hd(F);
+    OTM_HIP(hipMemcpy(hxy.data(), d_fwd_xy, 2 * V * sizeof(double), hipMemcpyDeviceToHost));
+    OTM_HIP(hipMemcpy(hd.data(), d_density, F * sizeof(double), hipMemcpyDeviceToHost));
+    return otm_map_create(out, device, n, hxy.data(), hd.data());
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+int otm_map_destroy(otm_map* m) {
+  if (!m) return OTM_OK;
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    OTM_HIP(hipStreamSynchronize(m->stream));
+    map_release_staging(m);
+    if (m->nodes) (void)hipFree(m->nodes);
+    if (m->faces) (void)hipFree(m->faces);
+    if (m->xy) (void)hipFree(m->xy);
+    if (m->density) (void)hipFree(m->density);
+    (void)hipStreamDestroy(m->stream);
+    delete m;
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+int otm_map_init_inverse(otm_map* m) {
+  if (!m) { set_error("null map"); return OTM_ERR_ARG; }
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    map_build(m);
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+int otm_map_bvh_size(const otm_map* m) { return m ? m->num_nodes : 0; }
+
+int otm_map_inverse(otm_map* m, const double* pts, int64_t count, int mode, double* out, int32_t* face,
+                    int32_t* nhits) {
+  if (!m || (count > 0 && (!pts || !out)) || count < 0 || (mode != 0 && mode != 1)) {
+    set_error("invalid inverse arguments");
+    return OTM_ERR_ARG;
+  }
+  if (count == 0) return OTM_OK;
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    map_build(m);
+    if (count > m->staged) {
+      map_release_staging(m);
+      OTM_HIP(hipMalloc(&m->d_pts, 2 * size_t(count) * sizeof(double)));
+      OTM_HIP(hipMalloc(&m->d_out, 2 * size_t(count) * sizeof(double)));
+      OTM_HIP(hipMalloc(&m->d_face, size_t(count) * sizeof(int)));
+      OTM_HIP(hipMalloc(&m->d_nhits, size_t(count) * sizeof(int)));
+      m->staged = count;
+    }
+    OTM_HIP(hipMemcpyAsync(m->d_pts, pts, 2 * size_t(count) * sizeof(double), hipMemcpyHostToDevice, m->stream));
+    launch_inverse(m, m->d_pts, count, mode, m->d_out, m->d_face, m->d_nhits);
+    OTM_HIP(hipMemcpyAsync(out, m->d_out, 2 * size_t(count) * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    if (face) OTM_HIP(hipMemcpyAsync(face, m->d_face, size_t(count) * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    if (nhits)
+      OTM_HIP(hipMemcpyAsync(nhits, m->d_nhits, size_t(count) * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    OTM_HIP(hipStreamSynchronize(m->stream));
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+int otm_map_inverse_device(otm_map* m, const double* d_pts, int64_t count, int mode, double* d_out, int32_t* d_face,
+                           int32_t* d_nhits) {
+  if (!m || (count > 0 && (!d_pts || !d_out)) || count < 0 || (mode != 0 && mode != 1)) {
+    set_error("invalid inverse arguments");
+    return OTM_ERR_ARG;
+  }
+  if (count == 0) return OTM_OK;
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    map_build(m);
+    launch_inverse(m, d_pts, count, mode, d_out, d_face, d_nhits);
+    OTM_HIP(hipStreamSynchronize(m->stream));
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+}  // extern "C"

@@ -112,5 +112,6 @@ struct DctPlan {
 void dct_plan_create(DctPlan& p, int n);
 void dct_plan_destroy(DctPlan& p);
 void launch_dct_solve(const DctPlan& p, const Fields& f, const double* r, double* out, hipStream_t s);
+void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s);
 
 }  // namespace otm

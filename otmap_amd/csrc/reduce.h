@@ -60,11 +60,25 @@ __device__ static inline bool publish_partials(double (&v)[K], double* partials,
 
 // In the last workgroup: fold the K partial columns in a fixed order, reset
 // the ticket.  Result valid in thread 0.
+// Each thread issues FOLD_BATCH independent loads before adding them, so the
+// latency of the agent-scope loads overlaps (a serial chain of such loads
+// costs ~1 us each); the summation order stays fixed.
+constexpr int FOLD_BATCH = 8;
+
 template <int K>
 __device__ static inline void fold_partials(double (&v)[K], const double* partials, unsigned* counter, int nb) {
   for (int k = 0; k < K; ++k) {
     double s = 0.;
-    for (int j = threadIdx.x; j < nb; j += NT) s += load_agent(&partials[long(k) * nb + j]);
+    for (int base = 0; base < nb; base += NT * FOLD_BATCH) {
+      double tmp[FOLD_BATCH];
+#pragma unroll
+      for (int u = 0; u < FOLD_BATCH; ++u) {
+        const int j = base + u * NT + threadIdx.x;
+        tmp[u] = (j < nb) ? load_agent(&partials[long(k) * nb + j]) : 0.;
+      }
+#pragma unroll
+      for (int u = 0; u < FOLD_BATCH; ++u) s += tmp[u];
+    }
     v[k] = s;
   }
   block_sum<K>(v);

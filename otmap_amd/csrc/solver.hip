@@ -298,6 +298,53 @@ int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, doubl
   OTM_GUARD_END
 }
 
+int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch) {
+  if (int rc = check_ready(s)) return rc;
+  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_DCT_SOLVE) {
+    set_error("invalid profile arguments");
+    return OTM_ERR_ARG;
+  }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  // A scratch state keeps every gate open: iteration 1, conjugate-Jacobian
+  // beta, update pending.  Intermediate buffers of the last solve are reused
+  // as inputs and overwritten.
+  Fields f = s->f;
+  SolveState* scratch = nullptr;
+  OTM_HIP(hipMalloc(&scratch, sizeof(SolveState)));
+  f.state = scratch;
+  SolveState tpl;
+  std::memset(&tpl, 0, sizeof(tpl));
+  tpl.it = 1; tpl.cj = 1; tpl.max_iter = 1 << 30; tpl.threshold = 0.; tpl.alpha = 0.5; tpl.beta = 0.1;
+  tpl.e = 1.; tpl.update = 1; tpl.use_prev = 1;
+  hipEvent_t a, b;
+  OTM_HIP(hipEventCreate(&a));
+  OTM_HIP(hipEventCreate(&b));
+  double total = 0.;
+  for (int k = 0; k < reps + 1; ++k) {  // first launch is a warm-up
+    OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->stream));
+    OTM_HIP(hipEventRecord(a, s->stream));
+    switch (kernel) {
+      case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(s->plan, f, s->stream); break;
+      case OTM_KERNEL_BETA: launch_beta(f, s->stream); break;
+      case OTM_KERNEL_DIR_COEF: launch_dir_coef(f, s->stream); break;
+      case OTM_KERNEL_UPDATE: launch_update(f, s->stream); break;
+      case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(s->plan, f, f.r, f.dhat, s->stream); break;
+    }
+    OTM_HIP(hipEventRecord(b, s->stream));
+    OTM_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    OTM_HIP(hipEventElapsedTime(&ms, a, b));
+    if (k > 0) total += ms;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(scratch);
+  *ms_per_launch = total / reps;
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
 int otm_vertex_gradients(otm_solver* s, const double* psi, double* grads) {
   if (int rc = check_ready(s)) return rc;
   if (!psi || !grads) { set_error("null pointer"); return OTM_ERR_ARG; }

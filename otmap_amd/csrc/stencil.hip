@@ -208,14 +208,38 @@ __global__ __launch_bounds__(NT) void k_adjust_reduce(const double* in, long F, 
   }
   __syncthreads();
   if (!is_last) return;
-  if (threadIdx.x == 0) {
-    double tot = 0., gmn = DBL_MAX, gmx = -DBL_MAX;
-    for (int j = 0; j < nb; ++j) {
-      tot += load_agent(&partials[j]);
-      gmn = fmin(gmn, load_agent(&partials[nb + j]));
-      gmx = fmax(gmx, load_agent(&partials[2 * nb + j]));
+  // fold: every thread takes a fixed strided subset (independent loads in flight)
+  double tot[1] = {0.};
+  double gmn = DBL_MAX, gmx = -DBL_MAX;
+  for (int base = 0; base < nb; base += NT * FOLD_BATCH) {
+    double ts[FOLD_BATCH], tn[FOLD_BATCH], tx[FOLD_BATCH];
+#pragma unroll
+    for (int u = 0; u < FOLD_BATCH; ++u) {
+      const int j = base + u * NT + threadIdx.x;
+      const bool in = j < nb;
+      ts[u] = in ? load_agent(&partials[j]) : 0.;
+      tn[u] = in ? load_agent(&partials[nb + j]) : DBL_MAX;
+      tx[u] = in ? load_agent(&partials[2 * nb + j]) : -DBL_MAX;
     }
-    const double I = tot * A;
+#pragma unroll
+    for (int u = 0; u < FOLD_BATCH; ++u) {
+      tot[0] += ts[u];
+      gmn = fmin(gmn, tn[u]);
+      gmx = fmax(gmx, tx[u]);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    gmn = fmin(gmn, __shfl_down(gmn, off));
+    gmx = fmax(gmx, __shfl_down(gmx, off));
+  }
+  __syncthreads();
+  if (lane == 0) { mm[0][w] = gmn; mm[1][w] = gmx; }
+  block_sum<1>(tot);  // synchronises after the mm writes
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < NT / 64; ++k) { mm[0][0] = fmin(mm[0][0], mm[0][k]); mm[1][0] = fmax(mm[1][0], mm[1][k]); }
+    gmn = mm[0][0];
+    gmx = mm[1][0];
+    const double I = tot[0] * A;
     const double nmn = gmn / I, nmx = gmx / I;  // division by I > 0 is monotone
     const double ratio = fmin(st->max_ratio, nmx / nmn);
     const double cand = (ratio * nmn - nmx) / (1. - ratio);

@@ -10,6 +10,8 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import _lib
+
 
 class GridMesh:
     def __init__(self, n: int, xy):
@@ -77,12 +79,22 @@ class GridMesh:
 
 
 class TransportMap:
-    """origin mesh, forward mesh and adjusted density of one solve."""
+    """origin mesh, forward mesh and adjusted density of one solve
+    (otlib/transport_map.h:29-63).  The inverse runs on the GPU through a BVH
+    built lazily by init_inverse(), as in the reference."""
 
-    def __init__(self, origin: GridMesh, fwd: GridMesh, density):
+    def __init__(self, origin: GridMesh, fwd: GridMesh, density, device: int = 0):
         self._origin = origin
         self._fwd = fwd
         self._density = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+        self._device = device
+        self._h = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().otm_map_destroy(h)
+            self._h = None
 
     def origin_mesh(self) -> GridMesh:
         return self._origin
@@ -95,3 +107,53 @@ class TransportMap:
 
     fwd_mesh_ptr = fwd_mesh
     density_ptr = density
+
+    def _handle(self):
+        if self._h is None:
+            h = _lib.C.c_void_p()
+            _lib.check(_lib.lib().otm_map_create(_lib.C.byref(h), self._device, self._fwd.n, _lib.ptr(self._fwd.xy),
+                                                 _lib.ptr(self._density)), "otm_map_create")
+            self._h = h
+        return self._h
+
+    def init_inverse(self) -> None:
+        """otlib/transport_map.cpp:36-43"""
+        _lib.check(_lib.lib().otm_map_init_inverse(self._handle()), "init_inverse")
+
+    def inverse(self, points, fast: bool = False):
+        """Batched inv / inv_fast of (m, 2) points -> (positions (m, 2), face (m,), nhits (m,))."""
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+        m = p.size // 2
+        out = np.empty(2 * m)
+        face = np.empty(m, dtype=np.int32)
+        nh = np.empty(m, dtype=np.int32)
+        _lib.check(_lib.lib().otm_map_inverse(self._handle(), _lib.ptr(p), m, 1 if fast else 0, _lib.ptr(out),
+                                              _lib.iptr(face), _lib.iptr(nh)), "inverse")
+        return out.reshape(m, 2), face, nh
+
+    def inv(self, p):
+        """otlib/transport_map.h:44"""
+        return self.inverse(np.asarray(p, dtype=np.float64).reshape(1, 2))[0][0]
+
+    def inv_fast(self, p):
+        """otlib/transport_map.h:45"""
+        return self.inverse(np.asarray(p, dtype=np.float64).reshape(1, 2), fast=True)[0][0]
+
+
+def apply_inverse_map(tmap: TransportMap, points, verbose_level: int = 2):
+    """otlib/transport_map.cpp:105-128: returns the inverse-mapped (m, 2) points."""
+    tmap.init_inverse()
+    return tmap.inverse(points)[0]
+
+
+def rasterize_image(mesh: GridMesh, density=None, shape=None, device: int = 0):
+    """otlib/utils/rasterizer.cpp:188-308 (per-face density).  Returns a
+    (rows, cols) float64 image, img[y, x]; shape defaults to (n, n)."""
+    rows, cols = shape if shape is not None else (mesh.n, mesh.n)
+    img = np.empty(rows * cols)
+    dens = None if density is None else np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+    if dens is not None and dens.size != mesh.n * mesh.n:
+        raise ValueError("density must hold one value per face")
+    _lib.check(_lib.lib().otm_rasterize(device, mesh.n, _lib.ptr(mesh.xy), _lib.ptr(dens), int(rows), int(cols),
+                                        _lib.ptr(img)), "rasterize_image")
+    return img.reshape(rows, cols)

@@ -1,0 +1,105 @@
+"""GPU parity of TransportMap inverse (BVH) and rasterize_image.
+
+Bars: bit-exact against the reference's golden vectors and the oracle --
+positions, selected faces and hit counts of inv()/inv_fast(), every pixel of
+the rasterised images.  Predicates run in FP64 in the reference's operation
+order without FMA contraction, so equality is exact, including the
+reference's multi-hit selection and its carried-over weights in the raster.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from otmap_amd import GridMesh, TransportMap, rasterize_image
+from otmap_amd import densities as D
+
+pytestmark = pytest.mark.gpu
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_inverse_vs_reference_golden(n):
+    g = load("inverse_small.npz")
+    t = f"n{n}"
+    tm = TransportMap(GridMesh.regular(n), GridMesh(n, g[t + "_fwd"]), g[t + "_density"])
+    inv, face, nh = tm.inverse(g[t + "_pts"])
+    ok = g[t + "_nhits"] > 0
+    assert np.array_equal(nh, g[t + "_nhits"])
+    assert np.array_equal(inv[ok], g[t + "_inv"][ok])
+    assert np.all(np.isnan(inv[~ok]))
+    invf, ff, _ = tm.inverse(g[t + "_pts"], fast=True)
+    assert np.array_equal(invf[ok], g[t + "_invfast"][ok])
+    assert np.array_equal(ff[ok], g[t + "_first"][ok])
+
+
+def test_inverse_identity():
+    g = load("inverse_small.npz")
+    n = 16
+    tm = TransportMap(GridMesh.regular(n), GridMesh.regular(n), np.ones(n * n))
+    inv, face, _ = tm.inverse(g["ident_pts"])
+    assert np.array_equal(inv, g["ident_inv"])
+
+
+@pytest.mark.parametrize("n,seed", [(64, 9), (256, 1000)])
+def test_inverse_vs_oracle(n, seed, oracle):
+    o = oracle.solve(n, D.gaussian_mixture(n, seed))
+    pts = np.concatenate([np.random.default_rng(n).uniform(0, 1, (20000, 2)),
+                          np.stack([o["fwd"][: (n + 1) ** 2], o["fwd"][(n + 1) ** 2:]], 1)[::7]])
+    tm = TransportMap(GridMesh.regular(n), GridMesh(n, o["fwd"]), o["density"])
+    for fast in (False, True):
+        inv, face, nh = tm.inverse(pts, fast=fast)
+        oinv, oface, onh = oracle.inverse(n, o["fwd"], o["density"], pts, fast=fast)
+        assert np.array_equal(face, oface)
+        assert np.array_equal(nh, onh)
+        assert np.array_equal(inv, oinv, equal_nan=True)
+
+
+def test_inverse_cfg0_vs_reference_golden():
+    g = load("solve_gaussian256.npz")
+    # the map itself comes from the GPU solve (FP64 path); cell ids must match
+    # the reference's away from cell edges, positions to the map tolerance
+    from otmap_amd import GridBasedTransportSolver
+    n = 256
+    s = GridBasedTransportSolver(0)
+    s.init(n)
+    tm = s.solve(D.gaussian_mixture(n))
+    pts = D.uniform_points(1 << 20, seed=2000 + n)[g["qidx"]]
+    inv, face, nh = tm.inverse(pts)
+    ref_cell = np.floor(g["inv"] * n).astype(np.int64)
+    cell = np.floor(inv * n).astype(np.int64)
+    frac = g["inv"] * n - np.floor(g["inv"] * n)
+    away = np.all((frac > 1e-6) & (frac < 1 - 1e-6), axis=1)
+    assert away.mean() > 0.99
+    assert np.array_equal(cell[away], ref_cell[away])
+    assert np.abs(inv - g["inv"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_raster_vs_reference_golden(n):
+    g = load("raster.npz")
+    mesh = GridMesh(n, g[f"n{n}_fwd"])
+    for rows, cols in ((n, n), (2 * n, 2 * n), (24, 40)):
+        img = rasterize_image(mesh, g[f"n{n}_density"], shape=(rows, cols))
+        assert np.array_equal(img, g[f"n{n}_img_{rows}x{cols}"])
+        img1 = rasterize_image(mesh, None, shape=(rows, cols))
+        assert np.array_equal(img1, g[f"n{n}_ones_{rows}x{cols}"])
+    assert np.array_equal(rasterize_image(GridMesh(n, g[f"n{n}_invmesh"])), g[f"n{n}_invimg"])
+
+
+@pytest.mark.parametrize("n,R", [(64, 64), (64, 200), (256, 256), (256, 97)])
+def test_raster_vs_oracle(n, R, oracle):
+    o = oracle.solve(n, D.gaussian_mixture(n, n + 1))
+    img = rasterize_image(GridMesh(n, o["fwd"]), o["density"], shape=(R, R + 3))
+    assert np.array_equal(img, oracle.rasterize(n, o["fwd"], R, R + 3, o["density"]))
+
+
+def test_raster_identity_closed_form():
+    n = 64
+    img = rasterize_image(GridMesh.regular(n))
+    expect = float(np.float32(1.0 / (n * n))) / (1.0 / (n * n))
+    assert np.allclose(img, expect, rtol=1e-12)

@@ -47,16 +47,36 @@ def test_residual_bitexact(n, oracle, solver_factory):
     assert abs(e - eo) <= 1e-13 * abs(eo) + 1e-300
 
 
-@pytest.mark.parametrize("n", [3, 5, 16, 33, 64, 100, 128, 256])
+def _fft_dct_solve(n, r):
+    """Closed form: pseudo-inverse of the pinned operator applied to r - (sum r) e0
+    through scipy's FFT-based orthonormal DCT-II (accurate to ~n*eps)."""
+    import scipy.fft
+    k = np.arange(n)
+    lam = 2 * (1 - np.outer(np.cos(np.pi * k / n), np.cos(np.pi * k / n)))
+    lam[0, 0] = np.inf
+    rp = r.copy()
+    rp[0] -= r.sum()
+    return scipy.fft.idctn(scipy.fft.dctn(rp.reshape(n, n), type=2, norm="ortho") / lam, type=2,
+                           norm="ortho").reshape(-1)
+
+
+@pytest.mark.parametrize("n", [3, 5, 16, 33, 64, 100, 128, 256, 512])
 def test_laplacian_solve_matches_ldlt(n, oracle, solver_factory):
+    """vs the closed form (FFT DCT): 5e-12 relative (2.1e-13 measured at n=256,
+    a smooth low-mode difference); vs the oracle's sparse LDL^T:
+    within the LDL^T's own error, which grows like the condition number ~ n^2
+    (measured 2.8e-12 at n=128, 2.8e-11 at 256, 1.3e-10 at 512)."""
     s = solver_factory(n)
     r = _rng(n).standard_normal(n * n)
     for rhs in (r, r - r.mean()):
         d = s.laplacian_solve(rhs)
-        do = oracle.laplacian_solve(n, rhs)
         d = d - d.mean()
-        err = np.abs(d - do).max() / np.abs(do).max()
-        assert err < 1e-11, err
+        x = _fft_dct_solve(n, rhs)
+        assert np.abs(d - x).max() / np.abs(x).max() < 5e-12
+        if n <= 256:
+            do = oracle.laplacian_solve(n, rhs)
+            err = np.abs(d - do).max() / np.abs(do).max()
+            assert err < 1e-15 * n * n + 1e-13, err
 
 
 @pytest.mark.parametrize("n", [2, 3, 16, 33, 64, 256])
