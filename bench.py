@@ -90,9 +90,11 @@ def main():
     t_init = time.perf_counter() - t0
 
     results = {}
-    modes = [("fp64", Precision.FP64)]
-    for name, prec in modes:
-        opt = SolverOptions(precision=prec)
+    modes = [("fp64", Precision.FP64, 1)]
+    if n % 64 == 0:
+        modes.append(("mixed", Precision.MIXED, 1))
+    for name, prec, refine in modes:
+        opt = SolverOptions(precision=prec, refine_steps=refine)
         for _ in range(args.warmup):
             s.solve_device(d_rho.data_ptr(), d_fwd.data_ptr(), d_adj.data_ptr(), opt)
         torch.cuda.synchronize()
@@ -126,12 +128,15 @@ def main():
     flops = 2.0 * n ** 3
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     kernel_ms = {}
-    for kname, kid in (("beta", _lib.OTM_KERNEL_BETA), ("dir_coef", _lib.OTM_KERNEL_DIR_COEF),
-                       ("update", _lib.OTM_KERNEL_UPDATE)):
+    kernels = [("beta", _lib.OTM_KERNEL_BETA), ("dir_coef", _lib.OTM_KERNEL_DIR_COEF),
+               ("update", _lib.OTM_KERNEL_UPDATE), ("dct_solve_f64", _lib.OTM_KERNEL_DCT_SOLVE)]
+    if "mixed" in results:
+        kernels += [("mixed_gemm_bf16x3", _lib.OTM_KERNEL_MIXED_GEMM), ("mixed_solve", _lib.OTM_KERNEL_MIXED_SOLVE)]
+    for kname, kid in kernels:
         _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, kid, args.profile_reps, _lib.C.byref(ms)), "profile")
         kernel_ms[kname] = round(ms.value, 4)
 
-    head = "fp64"
+    head = "mixed" if "mixed" in results else "fp64"
     line = {
         "metric": METRIC,
         "value": round(results[head]["ms"], 3),
@@ -143,12 +148,15 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3-split GEMM (f32 accumulate) + f64 refinement/stencils" if head == "mixed" else "f64",
         "data": f"synthetic: seeded smoothed shapes {n}x{n} (3 discs/rings, blur 5 px, floor 1e-3, mean 1)",
         "config": {"workload": f"single {n}x{n} grid transport solve to |r|^2/A <= 1e-7 (conjugate-Jacobian)",
                    "n": n, "threshold": 1e-7, "precision": head},
         "fp64_ms_per_solve": round(results["fp64"]["ms"], 3),
         "fp64_device_ms_per_solve": round(results["fp64"]["dev_ms"], 3),
+        "mixed_ms_per_solve": round(results["mixed"]["ms"], 3) if "mixed" in results else None,
+        "speedup_mixed_vs_fp64": (round(results["fp64"]["ms"] / results["mixed"]["ms"], 3)
+                                  if "mixed" in results else None),
         "iterations": {k: v["iters"] for k, v in results.items()},
         "final_residual": {k: v["res"] for k, v in results.items()},
         "init_s": round(t_init, 4),

@@ -50,7 +50,7 @@ typedef struct otm_options {
   double threshold;  /* default 1e-7, compared with |r|^2 / A */
   double max_ratio;  /* default DBL_MAX: cap of max/min of the target density */
   int precision;     /* OTM_PRECISION_FP64 (default, the drop-in) or OTM_PRECISION_MIXED */
-  int reserved;
+  int refine_steps;  /* mixed only: FP64 refinement steps per Laplacian solve (default 1, 0..8) */
 } otm_options;
 
 typedef struct otm_stats {
@@ -108,7 +108,9 @@ enum {
   OTM_KERNEL_BETA = 1,      /* conjugate-Jacobian residual + reductions */
   OTM_KERNEL_DIR_COEF = 2,  /* direction, quartic coefficients + line search */
   OTM_KERNEL_UPDATE = 3,    /* psi / G0 / r update */
-  OTM_KERNEL_DCT_SOLVE = 4  /* the whole Laplacian solve (4 GEMM stages) */
+  OTM_KERNEL_DCT_SOLVE = 4, /* the whole FP64 Laplacian solve (4 GEMM stages) */
+  OTM_KERNEL_MIXED_GEMM = 5, /* one 3-term BF16-split GEMM stage (6 MFMA products) */
+  OTM_KERNEL_MIXED_SOLVE = 6 /* the whole mixed Laplacian solve (split, 8 stages, refinement) */
 };
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
 
@@ -123,6 +125,9 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
 int otm_vertex_gradients(otm_solver* s, const double* psi, double* grads);
 int otm_residual(otm_solver* s, const double* adjusted_density, const double* psi, double* r, double* e);
 int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat);
+/* same solve in the mixed-precision form (3-term BF16 split GEMMs + `refine_steps`
+ * FP64 refinement steps); n must be a multiple of 64 */
+int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int refine_steps);
 int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* dir, double* a, double* b);
 
 /* ---- TransportMap (otlib/transport_map.h:29-63) -------------------------

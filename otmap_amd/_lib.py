@@ -18,12 +18,13 @@ HEADERS = [os.path.join(ROOT, "include", "otmap_capi.h")]
 OTM_OK = 0
 OTM_BETA_ZERO, OTM_BETA_CONJUGATE_JACOBIAN = 0, 1
 OTM_PRECISION_FP64, OTM_PRECISION_MIXED = 0, 1
-OTM_KERNEL_DCT_GEMM, OTM_KERNEL_BETA, OTM_KERNEL_DIR_COEF, OTM_KERNEL_UPDATE, OTM_KERNEL_DCT_SOLVE = range(5)
+(OTM_KERNEL_DCT_GEMM, OTM_KERNEL_BETA, OTM_KERNEL_DIR_COEF, OTM_KERNEL_UPDATE, OTM_KERNEL_DCT_SOLVE,
+ OTM_KERNEL_MIXED_GEMM, OTM_KERNEL_MIXED_SOLVE) = range(7)
 
 
 class OtmOptions(C.Structure):
     _fields_ = [("beta", C.c_int), ("max_iter", C.c_int), ("threshold", C.c_double),
-                ("max_ratio", C.c_double), ("precision", C.c_int), ("reserved", C.c_int)]
+                ("max_ratio", C.c_double), ("precision", C.c_int), ("refine_steps", C.c_int)]
 
 
 class OtmStats(C.Structure):
@@ -53,6 +54,7 @@ _SIGNATURES = {
     "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
     "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),
+    "otm_laplacian_solve_mixed": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "otm_line_search_coefficients": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_map_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _dp, _dp]),
     "otm_map_create_device": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp]),

@@ -27,6 +27,7 @@ SOURCES = [
     "solver.hip",
     "inverse.hip",
     "raster.hip",
+    "dct_mixed.hip",
 ]
 
 FLAGS = [

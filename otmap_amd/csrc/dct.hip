@@ -38,12 +38,12 @@ enum GemmMode { G_PLAIN = 0, G_RHS = 1, G_SPECTRAL = 2, G_FINAL = 3 };
 // steps of 16 with register prefetch and double-buffered LDS.
 //   G_RHS:      B(0,0) -= state->sum_r while loading (R' = R - S e0 e0^T)
 //   G_SPECTRAL: D(k,l) /= lambda_kl, D(0,0) = 0
-//   G_FINAL:    |D|^2 partials; the last workgroup stores |d_hat| and stops
-//               the solve when |d_hat| <= 2*DBL_MIN (otsolver_2dgrid.cpp:182)
+//   G_FINAL:    |D|^2 workgroup partials (the scalar kernel that follows stops
+//               the solve when |d_hat| <= 2*DBL_MIN, otsolver_2dgrid.cpp:182)
 template <int MODE, bool EDGE>
 __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
                                                 double* __restrict__ D, int n, SolveState* st,
-                                                const double* __restrict__ sq, double* partials, unsigned* counter) {
+                                                const double* __restrict__ sq, double* partials) {
   if (st && st->done) return;
   __shared__ double As[2][BM][BK + 1];
   __shared__ double Bs[2][BK][BN + 2];
@@ -134,28 +134,17 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, c
         if (MODE == G_FINAL) nrm[0] += val * val;
         D[long(row) * n + col] = val;
       }
-  if (MODE == G_FINAL && st) {
-    if (!publish_partials<1>(nrm, partials, counter)) return;
-    fold_partials<1>(nrm, partials, counter, block_count());
-    if (threadIdx.x == 0) {
-      const double dn = sqrt(nrm[0]);
-      st->dnorm = dn;
-      if (dn <= 2 * DBL_MIN) {
-        st->done = 1;
-        st->exit_reason = 2;
-      }
-    }
-  }
+  if (MODE == G_FINAL && partials) write_partials<1>(nrm, partials);
 }
 
 template <int MODE>
 void gemm(const double* A, const double* B, double* D, int n, SolveState* st, const double* sq, double* partials,
-          unsigned* counter, hipStream_t s) {
+          hipStream_t s) {
   const dim3 grid((n + BN - 1) / BN, (n + BM - 1) / BM);
   if (n % 64 == 0)
-    hipLaunchKernelGGL((k_gemm_f64<MODE, false>), grid, dim3(NT), 0, s, A, B, D, n, st, sq, partials, counter);
+    hipLaunchKernelGGL((k_gemm_f64<MODE, false>), grid, dim3(NT), 0, s, A, B, D, n, st, sq, partials);
   else
-    hipLaunchKernelGGL((k_gemm_f64<MODE, true>), grid, dim3(NT), 0, s, A, B, D, n, st, sq, partials, counter);
+    hipLaunchKernelGGL((k_gemm_f64<MODE, true>), grid, dim3(NT), 0, s, A, B, D, n, st, sq, partials);
   OTM_HIP(hipGetLastError());
 }
 
@@ -197,17 +186,18 @@ void dct_plan_destroy(DctPlan& p) {
 
 // one plain stage (T3 = C^T Y) without gating, for kernel timing
 void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s) {
-  gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, p.n, nullptr, p.sq, f.partials, f.counters + CNT_AUX, s);
+  gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, p.n, nullptr, p.sq, nullptr, s);
 }
 
-// d_hat = C^T [(C R' C^T) / lambda] C.  Every GEMM is skipped once state->done is set.
+// d_hat = C^T [(C R' C^T) / lambda] C.  Every GEMM is skipped once state->done
+// is set; the last stage leaves the |d_hat|^2 partials in SITE_DNORM.
 void launch_dct_solve(const DctPlan& p, const Fields& f, const double* r, double* out, hipStream_t s) {
   const int n = p.n;
   SolveState* st = f.state;
-  gemm<G_RHS>(p.C, r, f.tmp0, n, st, p.sq, f.partials, f.counters + CNT_DNORM, s);
-  gemm<G_SPECTRAL>(f.tmp0, p.Ct, f.tmp1, n, st, p.sq, f.partials, f.counters + CNT_DNORM, s);
-  gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, n, st, p.sq, f.partials, f.counters + CNT_DNORM, s);
-  gemm<G_FINAL>(f.tmp0, p.C, out, n, st, p.sq, f.partials, f.counters + CNT_DNORM, s);
+  gemm<G_RHS>(p.C, r, f.tmp0, n, st, p.sq, nullptr, s);
+  gemm<G_SPECTRAL>(f.tmp0, p.Ct, f.tmp1, n, st, p.sq, nullptr, s);
+  gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, n, st, p.sq, nullptr, s);
+  gemm<G_FINAL>(f.tmp0, p.C, out, n, st, p.sq, f.site(SITE_DNORM), s);
 }
 
 }  // namespace otm

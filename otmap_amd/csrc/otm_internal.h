@@ -37,12 +37,14 @@ inline int tiles_j(int n) { return (n + TJ - 1) / TJ; }
 inline int tiles_i(int n) { return (n + TI - 1) / TI; }
 inline int num_tiles(int n) { return tiles_j(n) * tiles_i(n); }
 
-// Last-workgroup reduction counters (one per reduction site, zero between uses).
-enum Counter { CNT_ADJUST = 0, CNT_RESID, CNT_DNORM, CNT_BETA, CNT_COEF, CNT_UPDATE, CNT_AUX, CNT_COUNT };
+// Reduction sites: each owns a region of the partials buffer so that a scalar
+// kernel can fold partials written by two different producers.
+enum Site { SITE_ADJUST = 0, SITE_RESID, SITE_DNORM, SITE_BETA, SITE_COEF, SITE_AUX, SITE_COUNT };
+constexpr int SITE_VALUES = 8;  // values per workgroup a site can hold
 
 // ---------------------------------------------------------------------------
-// Device-resident solver state.  Written only by the last workgroup of a
-// reduction, read (gates, scalars) by the kernels that follow in stream order.
+// Device-resident solver state.  Written only by single-workgroup scalar
+// kernels, read (gates, scalars) by the kernels that follow in stream order.
 // ---------------------------------------------------------------------------
 struct SolveState {
   double e;          // current residual |r|^2 / A (reference "residual")
@@ -80,22 +82,25 @@ struct Fields {
   double** dpair = nullptr;  // device copy of {d[0], d[1]}
   double* a = nullptr;       // quartic coefficient a (F)
   double* b = nullptr;       // quartic coefficient b (F)
-  double* partials = nullptr;  // 8 * max(num_tiles(n), gemm tiles) block partial sums
+  double* partials = nullptr;  // SITE_COUNT regions of SITE_VALUES * pslots workgroup partials
+  long pslots = 0;           // workgroups a site can hold
   double* tmp0 = nullptr;    // DCT intermediates (F each)
   double* tmp1 = nullptr;
   SolveState* state = nullptr;
-  unsigned* counters = nullptr;  // CNT_COUNT
   double* trace = nullptr;   // 3 * trace_cap: residual, alpha, beta per iteration
   int trace_cap = 0;
+  double* site(int s) const { return partials + long(s) * SITE_VALUES * pslots; }
 };
 
 // Stencil kernel launchers (stencil.hip)
 void launch_fill(double* p, long count, double v, hipStream_t s);
 void launch_adjust_density(const double* in, const Fields& f, double max_ratio, hipStream_t s);
 void launch_initial_residual(const Fields& f, int max_iter, double threshold, int cj, hipStream_t s);
-void launch_beta(const Fields& f, hipStream_t s);
-void launch_dir_coef(const Fields& f, hipStream_t s);
+void launch_beta(const Fields& f, hipStream_t s);       // k_beta + scalar kernel
+void launch_dir_coef(const Fields& f, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
 void launch_update(const Fields& f, hipStream_t s);
+// number of workgroups of the final DCT stage (its |d_hat|^2 partials go to SITE_DNORM)
+int dnorm_blocks(int n, bool mixed);
 void launch_forward_positions(const double* psi, double* xy, int n, hipStream_t s);
 // component kernels (single evaluations)
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s);
@@ -113,5 +118,22 @@ void dct_plan_create(DctPlan& p, int n);
 void dct_plan_destroy(DctPlan& p);
 void launch_dct_solve(const DctPlan& p, const Fields& f, const double* r, double* out, hipStream_t s);
 void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s);
+
+// Mixed-precision Laplacian solve (dct_mixed.hip): 3-term BF16 split GEMMs on
+// MFMA + FP64 refinement.  Requires n % 64 == 0.
+struct MixedPlan {
+  int n = 0;
+  float* C32 = nullptr;   // C   [k][i] in FP32 (split into BF16 terms on the fly)
+  float* Ct32 = nullptr;  // C^T [i][k]
+  float* work = nullptr;  // 2 intermediates (FP32)
+  float* rhs = nullptr;   // right-hand side of the current transform (FP32)
+  double* x0 = nullptr;   // unrefined solution (FP64)
+};
+bool mixed_supported(int n);
+void mixed_plan_create(MixedPlan& mp, const DctPlan& p);
+void mixed_plan_destroy(MixedPlan& mp);
+void launch_mixed_solve(const MixedPlan& mp, const DctPlan& p, const Fields& f, const double* r, double* out,
+                        int refine, hipStream_t s);
+void launch_mixed_gemm_stage(const MixedPlan& mp, const DctPlan& p, hipStream_t s);
 
 }  // namespace otm

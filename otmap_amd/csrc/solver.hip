@@ -37,6 +37,7 @@ struct otm_solver {
   int n = 0;
   Fields f;
   DctPlan plan;
+  MixedPlan mplan;             // created on the first mixed-precision solve
   double* d_in = nullptr;      // staging of host-API inputs (n*n)
   double* d_xy = nullptr;      // staging of host-API vertex outputs (2V)
   double* h_sum = nullptr;     // pinned: host copy of partials
@@ -62,8 +63,9 @@ void release_fields(otm_solver* s) {
   Fields& f = s->f;
   dfree(f.rho); dfree(f.psi); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
   dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
-  dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.counters); dfree(f.trace);
+  dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.trace);
   dfree(s->d_in); dfree(s->d_xy);
+  mixed_plan_destroy(s->mplan);
   dct_plan_destroy(s->plan);
   f = Fields();
   s->n = 0;
@@ -90,11 +92,11 @@ void allocate(otm_solver* s, int n) {
   const size_t F = f.F, V = f.V;
   dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 2 * V); dalloc(f.r, F); dalloc(f.jd, F);
   dalloc(f.dhat, F); dalloc(f.d[0], F); dalloc(f.d[1], F); dalloc(f.dpair, 2);
-  dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, 8 * size_t(partial_slots(n)));
-  dalloc(f.tmp0, F); dalloc(f.tmp1, F); dalloc(f.state, 1); dalloc(f.counters, CNT_COUNT);
+  f.pslots = partial_slots(n);
+  dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, size_t(SITE_COUNT) * SITE_VALUES * size_t(f.pslots));
+  dalloc(f.tmp0, F); dalloc(f.tmp1, F); dalloc(f.state, 1);
   dalloc(s->d_in, F); dalloc(s->d_xy, 2 * V);
   OTM_HIP(hipMemcpy(f.dpair, f.d, 2 * sizeof(double*), hipMemcpyHostToDevice));
-  OTM_HIP(hipMemset(f.counters, 0, CNT_COUNT * sizeof(unsigned)));
   OTM_HIP(hipMemset(f.state, 0, sizeof(SolveState)));
   ensure_trace(s, 1024);
   dct_plan_create(s->plan, n);
@@ -123,8 +125,11 @@ int check_ready(otm_solver* s) {
   return OTM_OK;
 }
 
-void enqueue_iteration(otm_solver* s) {
-  launch_dct_solve(s->plan, s->f, s->f.r, s->f.dhat, s->stream);
+void enqueue_iteration(otm_solver* s, const otm_options& opt) {
+  if (opt.precision == OTM_PRECISION_MIXED)
+    launch_mixed_solve(s->mplan, s->plan, s->f, s->f.r, s->f.dhat, opt.refine_steps, s->stream);
+  else
+    launch_dct_solve(s->plan, s->f, s->f.r, s->f.dhat, s->stream);
   launch_beta(s->f, s->stream);
   launch_dir_coef(s->f, s->stream);
   launch_update(s->f, s->stream);
@@ -139,6 +144,17 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
   if (opt.precision != OTM_PRECISION_FP64 && opt.precision != OTM_PRECISION_MIXED) {
     set_error("unknown precision mode");
     return OTM_ERR_ARG;
+  }
+  if (opt.precision == OTM_PRECISION_MIXED) {
+    if (!mixed_supported(s->n)) {
+      set_error("mixed precision needs a grid size that is a multiple of 64");
+      return OTM_ERR_ARG;
+    }
+    if (opt.refine_steps < 0 || opt.refine_steps > 8) {
+      set_error("refine_steps must be in [0, 8]");
+      return OTM_ERR_ARG;
+    }
+    if (!s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
   }
   Fields& f = s->f;
   hipStream_t st = s->stream;
@@ -155,7 +171,7 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
   long enqueued = 0;
   int chunk = 4;
   for (;;) {
-    for (int c = 0; c < chunk && enqueued <= long(opt.max_iter); ++c, ++enqueued) enqueue_iteration(s);
+    for (int c = 0; c < chunk && enqueued <= long(opt.max_iter); ++c, ++enqueued) enqueue_iteration(s, opt);
     OTM_HIP(hipMemcpyAsync(s->h_state, f.state, sizeof(SolveState), hipMemcpyDeviceToHost, st));
     OTM_HIP(hipStreamSynchronize(st));
     if (s->h_state->done) break;
@@ -194,7 +210,7 @@ void otm_options_init(otm_options* opt) {
   opt->threshold = 1e-7;
   opt->max_ratio = DBL_MAX;
   opt->precision = OTM_PRECISION_FP64;
-  opt->reserved = 0;
+  opt->refine_steps = 1;
 }
 
 const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -300,12 +316,18 @@ int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, doubl
 
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch) {
   if (int rc = check_ready(s)) return rc;
-  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_DCT_SOLVE) {
+  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_MIXED_SOLVE) {
     set_error("invalid profile arguments");
+    return OTM_ERR_ARG;
+  }
+  const bool mixed = kernel == OTM_KERNEL_MIXED_GEMM || kernel == OTM_KERNEL_MIXED_SOLVE;
+  if (mixed && !mixed_supported(s->n)) {
+    set_error("mixed precision needs a grid size that is a multiple of 64");
     return OTM_ERR_ARG;
   }
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
+  if (mixed && !s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
   // A scratch state keeps every gate open: iteration 1, conjugate-Jacobian
   // beta, update pending.  Intermediate buffers of the last solve are reused
   // as inputs and overwritten.
@@ -320,27 +342,35 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   hipEvent_t a, b;
   OTM_HIP(hipEventCreate(&a));
   OTM_HIP(hipEventCreate(&b));
-  double total = 0.;
-  for (int k = 0; k < reps + 1; ++k) {  // first launch is a warm-up
-    OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->stream));
-    OTM_HIP(hipEventRecord(a, s->stream));
+  // The gates stay open across repeated launches (the scalar kernels only
+  // advance the iteration count), so every kernel runs back-to-back between
+  // one event pair: the figure is the kernel duration with launch latency
+  // overlapped, as rocprof reports it.  BETA and DIR_COEF include their
+  // single-workgroup scalar kernel.
+  if (kernel == OTM_KERNEL_DCT_SOLVE || kernel == OTM_KERNEL_MIXED_SOLVE) tpl.update = 0;
+  OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->stream));
+  auto launch = [&]() {
     switch (kernel) {
       case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(s->plan, f, s->stream); break;
       case OTM_KERNEL_BETA: launch_beta(f, s->stream); break;
       case OTM_KERNEL_DIR_COEF: launch_dir_coef(f, s->stream); break;
       case OTM_KERNEL_UPDATE: launch_update(f, s->stream); break;
       case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(s->plan, f, f.r, f.dhat, s->stream); break;
+      case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(s->mplan, s->plan, s->stream); break;
+      default: launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, 1, s->stream); break;
     }
-    OTM_HIP(hipEventRecord(b, s->stream));
-    OTM_HIP(hipEventSynchronize(b));
-    float ms = 0.f;
-    OTM_HIP(hipEventElapsedTime(&ms, a, b));
-    if (k > 0) total += ms;
-  }
+  };
+  launch();  // warm-up
+  OTM_HIP(hipEventRecord(a, s->stream));
+  for (int k = 0; k < reps; ++k) launch();
+  OTM_HIP(hipEventRecord(b, s->stream));
+  OTM_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  OTM_HIP(hipEventElapsedTime(&ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   (void)hipFree(scratch);
-  *ms_per_launch = total / reps;
+  *ms_per_launch = double(ms) / reps;
   return OTM_OK;
   OTM_GUARD_END
 }
@@ -394,6 +424,29 @@ int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat) {
   OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->stream));
   OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
   launch_dct_solve(s->plan, f, f.r, f.dhat, s->stream);
+  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipStreamSynchronize(s->stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int refine_steps) {
+  if (int rc = check_ready(s)) return rc;
+  if (!r || !d_hat) { set_error("null pointer"); return OTM_ERR_ARG; }
+  if (!mixed_supported(s->n)) { set_error("mixed precision needs a grid size that is a multiple of 64"); return OTM_ERR_ARG; }
+  if (refine_steps < 0 || refine_steps > 8) { set_error("refine_steps must be in [0, 8]"); return OTM_ERR_ARG; }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  if (!s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
+  const Fields& f = s->f;
+  double sum = 0.;
+  for (long k = 0; k < f.F; ++k) sum += r[k];
+  SolveState st;
+  std::memset(&st, 0, sizeof(st));
+  st.sum_r = sum;
+  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->stream));
+  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, refine_steps, s->stream);
   OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   OTM_HIP(hipStreamSynchronize(s->stream));
   return OTM_OK;

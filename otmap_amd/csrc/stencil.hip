@@ -3,10 +3,11 @@
 // Every stencil kernel works on a TI x TJ tile of faces (TJ along the
 // contiguous index j) with a one-face halo staged in LDS, derives the
 // (TI+1) x (TJ+1) vertex gradients of the tile in LDS, and then evaluates its
-// per-face quantity.  Reductions write one partial per workgroup; the last
-// workgroup to finish (atomic ticket) folds the partials in a fixed order, so
-// results are deterministic and the scalar logic of an iteration (beta, the
-// quartic line search, convergence) runs on the device without host syncs.
+// per-face quantity.  Reductions write one partial per workgroup; a
+// single-workgroup scalar kernel launched next folds them in a fixed order and
+// takes the iteration's scalar decisions (beta, the quartic line search,
+// convergence) on the device, so results are deterministic and the host never
+// waits inside an iteration.
 //
 // Arithmetic follows the reference's operation order (the library is built
 // with -ffp-contract=off):
@@ -172,80 +173,60 @@ __global__ __launch_bounds__(NT) void k_fill(double* p, long count, double v) {
   for (long i = long(blockIdx.x) * NT + threadIdx.x; i < count; i += long(gridDim.x) * NT) p[i] = v;
 }
 
-// density adjustment: sum / min / max, then scale + ratio cap (otsolver_2dgrid.cpp:47-73)
-__global__ __launch_bounds__(NT) void k_adjust_reduce(const double* in, long F, double A, double* partials,
-                                                      unsigned* counter, SolveState* st) {
-  double s = 0., mn = DBL_MAX, mx = -DBL_MAX;
+// density adjustment (otsolver_2dgrid.cpp:47-73), pass 1: sum / min / max
+__global__ __launch_bounds__(NT) void k_adjust_reduce(const double* in, long F, double* partials) {
+  double s[1] = {0.};
+  double mn = DBL_MAX, mx = -DBL_MAX;
   for (long i = long(blockIdx.x) * NT + threadIdx.x; i < F; i += long(gridDim.x) * NT) {
     const double v = in[i];
-    s += v;
+    s[0] += v;
     mn = fmin(mn, v);
     mx = fmax(mx, v);
   }
-  // min/max through the sum machinery on negated/duplicated slots is not
-  // order-free; do them with explicit shuffles instead.
   for (int off = 32; off > 0; off >>= 1) {
     mn = fmin(mn, __shfl_down(mn, off));
     mx = fmax(mx, __shfl_down(mx, off));
   }
   __shared__ double mm[2][NT / 64];
-  __shared__ int is_last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) { mm[0][w] = mn; mm[1][w] = mx; }
-  double v1[1] = {s};
-  block_sum<1>(v1);  // contains a __syncthreads after mm writes
-  const int nb = gridDim.x;
+  block_sum<1>(s);  // synchronises after the mm writes
   if (threadIdx.x == 0) {
-    double bmn = mm[0][0], bmx = mm[1][0];
-    for (int k = 1; k < NT / 64; ++k) { bmn = fmin(bmn, mm[0][k]); bmx = fmax(bmx, mm[1][k]); }
-    store_agent(&partials[blockIdx.x], v1[0]);
-    store_agent(&partials[nb + blockIdx.x], bmn);
-    store_agent(&partials[2 * nb + blockIdx.x], bmx);
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == unsigned(nb - 1)) ? 1 : 0;
-    if (is_last) __threadfence();
+    for (int k = 1; k < NT / 64; ++k) { mm[0][0] = fmin(mm[0][0], mm[0][k]); mm[1][0] = fmax(mm[1][0], mm[1][k]); }
+    const int nb = gridDim.x;
+    partials[blockIdx.x] = s[0];
+    partials[nb + blockIdx.x] = mm[0][0];
+    partials[2 * nb + blockIdx.x] = mm[1][0];
   }
-  __syncthreads();
-  if (!is_last) return;
-  // fold: every thread takes a fixed strided subset (independent loads in flight)
+}
+
+// pass 2 (one workgroup): integral, ratio cap -> scale and shift
+__global__ __launch_bounds__(NT) void k_adjust_scalar(const double* partials, int nb, double A, double max_ratio,
+                                                     SolveState* st) {
   double tot[1] = {0.};
   double gmn = DBL_MAX, gmx = -DBL_MAX;
-  for (int base = 0; base < nb; base += NT * FOLD_BATCH) {
-    double ts[FOLD_BATCH], tn[FOLD_BATCH], tx[FOLD_BATCH];
-#pragma unroll
-    for (int u = 0; u < FOLD_BATCH; ++u) {
-      const int j = base + u * NT + threadIdx.x;
-      const bool in = j < nb;
-      ts[u] = in ? load_agent(&partials[j]) : 0.;
-      tn[u] = in ? load_agent(&partials[nb + j]) : DBL_MAX;
-      tx[u] = in ? load_agent(&partials[2 * nb + j]) : -DBL_MAX;
-    }
-#pragma unroll
-    for (int u = 0; u < FOLD_BATCH; ++u) {
-      tot[0] += ts[u];
-      gmn = fmin(gmn, tn[u]);
-      gmx = fmax(gmx, tx[u]);
-    }
+  for (int j = threadIdx.x; j < nb; j += NT) {
+    tot[0] += partials[j];
+    gmn = fmin(gmn, partials[nb + j]);
+    gmx = fmax(gmx, partials[2 * nb + j]);
   }
   for (int off = 32; off > 0; off >>= 1) {
     gmn = fmin(gmn, __shfl_down(gmn, off));
     gmx = fmax(gmx, __shfl_down(gmx, off));
   }
-  __syncthreads();
+  __shared__ double mm[2][NT / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) { mm[0][w] = gmn; mm[1][w] = gmx; }
-  block_sum<1>(tot);  // synchronises after the mm writes
+  block_sum<1>(tot);
   if (threadIdx.x == 0) {
     for (int k = 1; k < NT / 64; ++k) { mm[0][0] = fmin(mm[0][0], mm[0][k]); mm[1][0] = fmax(mm[1][0], mm[1][k]); }
-    gmn = mm[0][0];
-    gmx = mm[1][0];
     const double I = tot[0] * A;
-    const double nmn = gmn / I, nmx = gmx / I;  // division by I > 0 is monotone
-    const double ratio = fmin(st->max_ratio, nmx / nmn);
+    const double nmn = mm[0][0] / I, nmx = mm[1][0] / I;  // division by I > 0 is monotone
+    const double ratio = fmin(max_ratio, nmx / nmn);
     const double cand = (ratio * nmn - nmx) / (1. - ratio);
     st->dens_scale = I;
     st->dens_shift = (0. < cand) ? cand : 0.;  // std::max(0., cand), NaN -> 0
-    *counter = 0u;
+    st->max_ratio = max_ratio;
   }
 }
 
@@ -255,10 +236,8 @@ __global__ __launch_bounds__(NT) void k_adjust_apply(const double* in, double* r
     rho[i] = (in[i] / I + b) / (1. + b);
 }
 
-// initial residual at psi = 0 and solver state initialisation
-__global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, double* r, int n, double* partials,
-                                                        unsigned* counter, SolveState* st, int max_iter,
-                                                        double threshold, int cj) {
+// initial residual at psi = 0: r_0 and the partials of |r|^2, sum r
+__global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, double* r, int n, double* partials) {
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
@@ -279,9 +258,16 @@ __global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, doub
       acc[1] += rv;
     }
   }
-  if (!publish_partials<2>(acc, partials, counter)) return;
-  fold_partials<2>(acc, partials, counter, block_count());
+  write_partials<2>(acc, partials);
+}
+
+// solver state initialisation (otsolver_2dgrid.cpp:126-160)
+__global__ __launch_bounds__(NT) void k_scalar_init(const double* partials, int nb, int n, SolveState* st,
+                                                   int max_iter, double threshold, int cj) {
+  double acc[2];
+  fold_partials<2>(acc, partials, nb);
   if (threadIdx.x == 0) {
+    const double A = 1.0 / (double(n) * double(n));
     const double e0 = acc[0] / A;
     st->e = e0;
     st->e0 = e0;
@@ -301,16 +287,11 @@ __global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, doub
   }
 }
 
-// conjugate-Jacobian beta: residual at psi - eps*d_hat (eps = alpha/2),
-// beta = max(-1, Jd.(r_eps - r_k) / |Jd|^2 / eps * alpha)   (otsolver_2dgrid.cpp:248-261)
+// conjugate-Jacobian beta, pass 1: residual at psi - eps*d_hat (eps = alpha/2)
+// and the partials of Jd.(r_eps - r_k), Jd.Jd   (otsolver_2dgrid.cpp:248-261)
 __global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dhat, const double* rho, const double* r,
-                                            const double* jd, int n, double* partials, unsigned* counter,
-                                            SolveState* st) {
-  if (st->done) return;
-  if (st->it < 1 || !st->cj) {
-    if (block_id() == 0 && threadIdx.x == 0) st->use_prev = 0;  // d = d_hat, beta unchanged
-    return;
-  }
+                                            const double* jd, int n, double* partials, const SolveState* st) {
+  if (st->done || st->it < 1 || !st->cj) return;
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
   const double eps = st->alpha / 2.;
@@ -320,7 +301,7 @@ __global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dh
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
   const double e = 1. / double(n), A = 1.0 / (double(n) * double(n));
-  double acc[2] = {0., 0.};  // Jd.(r_eps - r), Jd.Jd
+  double acc[2] = {0., 0.};
   for (int t = threadIdx.x; t < TI * TJ; t += NT) {
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
@@ -332,9 +313,44 @@ __global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dh
       acc[1] += j * j;
     }
   }
-  if (!publish_partials<2>(acc, partials, counter)) return;
-  fold_partials<2>(acc, partials, counter, block_count());
+  write_partials<2>(acc, partials);
+}
+
+// pass 2 (one workgroup): early stop on |d_hat| (otsolver_2dgrid.cpp:182-183),
+// then beta = max(-1, Jd.(r_eps - r_k) / |Jd|^2 / eps * alpha) and the choice
+// d = d_hat + beta d_prev (it >= 1, conjugate-Jacobian) or d = d_hat.
+__global__ __launch_bounds__(NT) void k_scalar_beta(const double* dnorm_partials, int dnorm_nb,
+                                                   const double* beta_partials, int beta_nb, SolveState* st) {
+  __shared__ int stop;
   if (threadIdx.x == 0) {
+    st->update = 0;  // the previous iteration's update has run
+    stop = st->done;
+  }
+  __syncthreads();
+  if (stop) return;
+  double dn[1];
+  fold_partials<1>(dn, dnorm_partials, dnorm_nb);
+  __shared__ int early;
+  if (threadIdx.x == 0) {
+    const double dnorm = sqrt(dn[0]);
+    st->dnorm = dnorm;
+    early = dnorm <= 2 * DBL_MIN;
+    if (early) {
+      st->done = 1;
+      st->exit_reason = 2;
+    }
+  }
+  __syncthreads();
+  if (early) return;
+  const bool conj = st->it >= 1 && st->cj;
+  if (!conj) {
+    if (threadIdx.x == 0) st->use_prev = 0;  // beta keeps its value (printed as is)
+    return;
+  }
+  double acc[2];
+  fold_partials<2>(acc, beta_partials, beta_nb);
+  if (threadIdx.x == 0) {
+    const double eps = st->alpha / 2.;
     const double cand = acc[0] / acc[1] / eps * st->alpha;
     st->beta = (-1. < cand) ? cand : -1.;  // std::max(-1., cand)
     st->use_prev = 1;
@@ -342,11 +358,11 @@ __global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dh
 }
 
 // direction d = d_hat (+ beta d_prev), vertex gradients of d, quartic
-// coefficients a, b per quad, and the four dot products of the line search;
-// the last workgroup picks alpha and tests convergence.
+// coefficients a, b per quad (r(psi + t d) = r + b t + a t^2), and the
+// partials of b.r, a.r, b.b, a.b, a.a, sum a, sum b   (otsolver_2dgrid.cpp:467-561)
 __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* const* dbuf, const double* g0,
                                                 const double* r, double* aout, double* bout, int n, double* partials,
-                                                unsigned* counter, SolveState* st, double* trace, int trace_cap) {
+                                                const SolveState* st) {
   if (st->done) return;
   __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
   __shared__ double s[NF];
@@ -373,7 +389,7 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
   const double e = 1. / double(n);
-  double acc[5] = {0., 0., 0., 0., 0.};  // b.r, a.r, b.b, a.b, a.a
+  double acc[7] = {0., 0., 0., 0., 0., 0., 0.};
   for (int t = threadIdx.x; t < TI * TJ; t += NT) {
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
@@ -399,10 +415,24 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
       acc[2] += bv * bv;
       acc[3] += av * bv;
       acc[4] += av * av;
+      acc[5] += av;
+      acc[6] += bv;
     }
   }
-  if (!publish_partials<5>(acc, partials, counter)) return;
-  fold_partials<5>(acc, partials, counter, block_count());
+  write_partials<7>(acc, partials);
+}
+
+// pass 2 (one workgroup): z = [e A, 2 b.r, b.b + 2 a.r, 2 a.b, a.a], the step
+// alpha, e(alpha)/A, traces, the iteration count and the stop test
+// (otsolver_2dgrid.cpp:556-605, :160); sum r_{k+1} = alpha^2 sum a + alpha sum b + sum r_k.
+__global__ __launch_bounds__(NT) void k_scalar_step(const double* partials, int nb, int n, SolveState* st,
+                                                   double* trace, int trace_cap) {
+  __shared__ int stop;
+  if (threadIdx.x == 0) stop = st->done;
+  __syncthreads();
+  if (stop) return;
+  double acc[7];
+  fold_partials<7>(acc, partials, nb);
   if (threadIdx.x == 0) {
     const double A = 1.0 / (double(n) * double(n));
     double z[5];
@@ -416,6 +446,8 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
     for (int q = 0; q < 5; ++q) st->z[q] = z[q];
     st->alpha = alpha;
     st->e = emin / A;
+    st->sum_r = acc[5] * (alpha * alpha) + acc[6] * alpha + st->sum_r;
+    const int k = st->it;
     if (k < trace_cap) {
       trace[k] = st->e;
       trace[trace_cap + k] = alpha;
@@ -432,10 +464,9 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
 }
 
 // line-search update: psi += alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
-// (otsolver_2dgrid.cpp:599-604); Jd = r_{k+1} - r_k; sum of r_{k+1} for the next solve.
+// (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta.
 __global__ __launch_bounds__(NT) void k_update(double* psi, double* const* dbuf, double* g0, const double* a,
-                                              const double* b, double* r, double* jd, int n, double* partials,
-                                              unsigned* counter, SolveState* st) {
+                                              const double* b, double* r, double* jd, int n, const SolveState* st) {
   if (!st->update) return;
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
@@ -461,7 +492,6 @@ __global__ __launch_bounds__(NT) void k_update(double* psi, double* const* dbuf,
       g0[V + v] = g0[V + v] + alpha * gy[t];
     }
   }
-  double acc[1] = {0.};
   for (int t = threadIdx.x; t < TI * TJ; t += NT) {
     const int fa = t / TJ, fb = t % TJ;
     const int fi = i0 + fa, fj = j0 + fb;
@@ -472,18 +502,12 @@ __global__ __launch_bounds__(NT) void k_update(double* psi, double* const* dbuf,
       const double rn = a[f] * (alpha * alpha) + b[f] * alpha + rk;
       r[f] = rn;
       jd[f] = rn - rk;
-      acc[0] += rn;
     }
-  }
-  if (!publish_partials<1>(acc, partials, counter)) return;
-  fold_partials<1>(acc, partials, counter, block_count());
-  if (threadIdx.x == 0) {
-    st->sum_r = acc[0];
-    st->update = 0;
   }
 }
 
-// vertex gradients of a face field into a (2V SoA) array
+// vertex gradients of a face field into a (2V SoA) array; add_origin: forward
+// positions = origin point double(i)*(1/n) + gradient (otsolver_2dgrid.cpp:223-227)
 __global__ __launch_bounds__(NT) void k_vertex_gradients(const double* psi, double* G, int n, int add_origin) {
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
@@ -502,7 +526,7 @@ __global__ __launch_bounds__(NT) void k_vertex_gradients(const double* psi, doub
     const bool own_j = (vb < TJ) || (vj == n);
     if (own_i && own_j && vi <= n && vj <= n) {
       const long v = vi * V1 + vj;
-      if (add_origin) {  // forward positions: origin point double(i)*(1/n) + gradient
+      if (add_origin) {
         G[v] = double(vi) * dx + gx[t];
         G[V + v] = double(vj) * dx + gy[t];
       } else {
@@ -535,8 +559,7 @@ __global__ __launch_bounds__(NT) void k_residual_only(const double* psi, const d
       acc[0] += rv * rv;
     }
   }
-  block_sum<1>(acc);
-  if (threadIdx.x == 0) partials[block_id()] = acc[0];
+  write_partials<1>(acc, partials);
 }
 
 // quartic coefficients only (component entry point)
@@ -586,6 +609,8 @@ inline dim3 tile_grid(int n) { return dim3(tiles_j(n), tiles_i(n)); }
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+int dnorm_blocks(int n, bool) { return ((n + 63) / 64) * ((n + 63) / 64); }
+
 void launch_fill(double* p, long count, double v, hipStream_t s) {
   if (count <= 0) return;
   const long blocks = std::min<long>((count + NT - 1) / NT, 4096);
@@ -595,11 +620,12 @@ void launch_fill(double* p, long count, double v, hipStream_t s) {
 
 void launch_adjust_density(const double* in, const Fields& f, double max_ratio, hipStream_t s) {
   const long blocks = std::min<long>((f.F + NT * 4 - 1) / (NT * 4), 1024);
-  // the ratio cap is read by the reduction's last workgroup
-  OTM_HIP(hipMemcpyAsync(&f.state->max_ratio, &max_ratio, sizeof(double), hipMemcpyHostToDevice, s));
   const double A = 1.0 / (double(f.n) * double(f.n));
-  hipLaunchKernelGGL(k_adjust_reduce, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.F, A, f.partials,
-                     f.counters + CNT_ADJUST, f.state);
+  double* part = f.site(SITE_ADJUST);
+  hipLaunchKernelGGL(k_adjust_reduce, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.F, part);
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_adjust_scalar, dim3(1), dim3(NT), 0, s, (const double*)part, int(blocks), A, max_ratio,
+                     f.state);
   OTM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_adjust_apply, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.rho, f.F,
                      (const SolveState*)f.state);
@@ -607,15 +633,40 @@ void launch_adjust_density(const double* in, const Fields& f, double max_ratio, 
 }
 
 void launch_initial_residual(const Fields& f, int max_iter, double threshold, int cj, hipStream_t s) {
-  hipLaunchKernelGGL(k_initial_residual, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.rho, f.r, f.n, f.partials,
-                     f.counters + CNT_RESID, f.state, max_iter, threshold, cj);
+  double* part = f.site(SITE_RESID);
+  hipLaunchKernelGGL(k_initial_residual, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.rho, f.r, f.n, part);
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(NT), 0, s, (const double*)part, num_tiles(f.n), f.n, f.state,
+                     max_iter, threshold, cj);
   OTM_HIP(hipGetLastError());
 }
 
 void launch_beta(const Fields& f, hipStream_t s) {
+  double* part = f.site(SITE_BETA);
   hipLaunchKernelGGL(k_beta, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.psi, (const double*)f.dhat,
-                     (const double*)f.rho, (const double*)f.r, (const double*)f.jd, f.n, f.partials,
-                     f.counters + CNT_BETA, f.state);
+                     (const double*)f.rho, (const double*)f.r, (const double*)f.jd, f.n, part,
+                     (const SolveState*)f.state);
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scalar_beta, dim3(1), dim3(NT), 0, s, (const double*)f.site(SITE_DNORM),
+                     dnorm_blocks(f.n, false), (const double*)part, num_tiles(f.n), f.state);
+  OTM_HIP(hipGetLastError());
+}
+
+// The direction / update kernels take the d ping-pong pair through a small
+// device array so that one captured iteration serves every iteration.
+void launch_dir_coef(const Fields& f, hipStream_t s) {
+  double* part = f.site(SITE_COEF);
+  hipLaunchKernelGGL(k_dir_coef, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.dhat, (double* const*)f.dpair,
+                     (const double*)f.g0, (const double*)f.r, f.a, f.b, f.n, part, (const SolveState*)f.state);
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scalar_step, dim3(1), dim3(NT), 0, s, (const double*)part, num_tiles(f.n), f.n, f.state,
+                     f.trace, f.trace_cap);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_update(const Fields& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_update, tile_grid(f.n), dim3(NT), 0, s, f.psi, (double* const*)f.dpair, f.g0,
+                     (const double*)f.a, (const double*)f.b, f.r, f.jd, f.n, (const SolveState*)f.state);
   OTM_HIP(hipGetLastError());
 }
 
@@ -639,20 +690,4 @@ void launch_params_1d(const double* g0, const double* dir, double* a, double* b,
   OTM_HIP(hipGetLastError());
 }
 
-// The direction / update kernels take the d ping-pong pair through a small
-// device array so that one captured iteration serves every iteration.
-void launch_dir_coef(const Fields& f, hipStream_t s) {
-  hipLaunchKernelGGL(k_dir_coef, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.dhat,
-                     (double* const*)f.dpair, (const double*)f.g0,
-                     (const double*)f.r, f.a, f.b, f.n, f.partials, f.counters + CNT_COEF, f.state, f.trace,
-                     f.trace_cap);
-  OTM_HIP(hipGetLastError());
-}
-
-void launch_update(const Fields& f, hipStream_t s) {
-  hipLaunchKernelGGL(k_update, tile_grid(f.n), dim3(NT), 0, s, f.psi,
-                     (double* const*)f.dpair, f.g0, (const double*)f.a,
-                     (const double*)f.b, f.r, f.jd, f.n, f.partials, f.counters + CNT_UPDATE, f.state);
-  OTM_HIP(hipGetLastError());
-}
 }  // namespace otm

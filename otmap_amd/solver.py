@@ -40,6 +40,7 @@ class SolverOptions:
     threshold: float = 1e-7
     max_ratio: float = DBL_MAX
     precision: Precision = Precision.FP64
+    refine_steps: int = 1  # mixed precision: FP64 refinement steps per Laplacian solve
 
     def to_c(self) -> OtmOptions:
         o = OtmOptions()
@@ -49,6 +50,7 @@ class SolverOptions:
         o.threshold = float(self.threshold)
         o.max_ratio = float(self.max_ratio)
         o.precision = int(self.precision)
+        o.refine_steps = int(self.refine_steps)
         return o
 
 
@@ -152,6 +154,13 @@ class GridBasedTransportSolver:
         n = self._n
         d = np.empty(n * n)
         check(_lib.lib().otm_laplacian_solve(self._h, ptr(_f64(r, n * n)), ptr(d)), "laplacian_solve")
+        return d
+
+    def laplacian_solve_mixed(self, r, refine_steps: int = 1):
+        n = self._n
+        d = np.empty(n * n)
+        check(_lib.lib().otm_laplacian_solve_mixed(self._h, ptr(_f64(r, n * n)), ptr(d), int(refine_steps)),
+              "laplacian_solve_mixed")
         return d
 
     def line_search_coefficients(self, g0, direction):

@@ -1,0 +1,69 @@
+"""GPU parity of the mixed-precision form (3-term BF16-split DCT GEMMs on MFMA
+with FP32 accumulation, FP64 refinement; stencils, norms and the quartic in FP64).
+
+Bars (the drop-in bar for every result):
+  * full solves: forward vertex positions within 1e-6 (unit-square
+    coordinates) of the FP64 oracle, iteration count within +-1;
+  * the Laplacian solve itself, against the closed form (FFT DCT), reported
+    per refinement depth: unrefined ~1e-6 relative, refined <= 1e-9.
+"""
+import numpy as np
+import pytest
+
+from otmap_amd import Precision, SolverOptions
+from otmap_amd import densities as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _fft_dct_solve(n, r):
+    import scipy.fft
+    k = np.arange(n)
+    lam = 2 * (1 - np.outer(np.cos(np.pi * k / n), np.cos(np.pi * k / n)))
+    lam[0, 0] = np.inf
+    rp = r.copy()
+    rp[0] -= r.sum()
+    return scipy.fft.idctn(scipy.fft.dctn(rp.reshape(n, n), type=2, norm="ortho") / lam, type=2,
+                           norm="ortho").reshape(-1)
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_mixed_laplacian_accuracy(n, solver_factory):
+    s = solver_factory(n)
+    rng = np.random.default_rng(77 + n)
+    # a white right-hand side (worst case for the DCT error amplification)
+    # and a smooth one (the residuals of the Newton iteration are smooth-ish)
+    white = rng.standard_normal(n * n)
+    smooth = D.gaussian_mixture(n, 3) - 1.0
+    errs = {}
+    for name, r in (("white", white), ("smooth", smooth)):
+        x = _fft_dct_solve(n, r)
+        for refine in (0, 1, 2):
+            d = s.laplacian_solve_mixed(r, refine)
+            d = d - d.mean()
+            errs[(name, refine)] = np.abs(d - x).max() / np.abs(x).max()
+    print(n, errs)
+    assert errs[("white", 0)] < 1e-4 and errs[("smooth", 0)] < 1e-4
+    assert errs[("white", 1)] < 1e-9 and errs[("smooth", 1)] < 1e-9
+    assert errs[("white", 2)] < 1e-11 and errs[("smooth", 2)] < 1e-11
+
+
+@pytest.mark.parametrize("n,seed", [(64, 9), (128, 10), (256, 1000)])
+def test_mixed_solve_matches_oracle(n, seed, oracle, solver_factory):
+    rho = D.gaussian_mixture(n, seed)
+    s = solver_factory(n)
+    o = oracle.solve(n, rho)
+    tm = s.solve(rho, SolverOptions(precision=Precision.MIXED))
+    st = s.last_stats
+    assert abs(st.iterations - o["iters"]) <= 1, (st.iterations, o["iters"])
+    assert st.final_residual <= 1e-7
+    err = np.abs(tm.fwd_mesh().xy - o["fwd"]).max()
+    print(n, "mixed vs oracle", err, st.iterations, o["iters"])
+    assert err < 1e-6, err
+
+
+def test_mixed_rejects_unsupported_size(solver_factory):
+    from otmap_amd._lib import OtmError
+    s = solver_factory(100)
+    with pytest.raises(OtmError):
+        s.solve(np.ones(100 * 100) + 0.1 * np.arange(10000) / 10000, SolverOptions(precision=Precision.MIXED))
