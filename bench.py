@@ -33,6 +33,16 @@ PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the two DCT GEMM stage kernels at n = 1024, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by
+# what the counter reads for a 256 MiB device copy (FETCH_SIZE x2.0, WRITE_SIZE
+# x1.0); the runs and their commands are in profiles/gemm_1024_hbm_traffic.md.
+TRAFFIC_SOURCE = "profiles/gemm_1024_hbm_traffic.md"
+TRAFFIC_BYTES_1024 = {
+    "k_gemm_x3": (11318.1 * 2.0 + 3750.2) * 1024,
+    "k_gemm_f64": (34021.3 * 2.0 + 7555.7) * 1024,
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -42,7 +52,45 @@ def parse():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--kernel", default=None,
+                    choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
+                             "mixed_solve", "copy"],
+                    help="only time this kernel (after one solve that fills the buffers); for profilers")
     return ap.parse_args()
+
+
+KERNEL_IDS = {"dct_gemm_f64": 0, "beta": 1, "dir_coef": 2, "update": 3, "dct_solve_f64": 4, "mixed_gemm": 5,
+              "mixed_solve": 6}
+
+
+def kernel_only(args):
+    """One solve to fill the buffers, then `profile_reps` back-to-back launches
+    of one kernel between HIP events on the solver's stream."""
+    import torch
+
+    if args.kernel == "copy":  # calibration of the HBM traffic counters: 256 MiB read + 256 MiB written
+        a = torch.ones(1 << 25, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(args.profile_reps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+        return
+
+    from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions
+    from otmap_amd import _lib
+    from otmap_amd import densities as D
+
+    n = args.n
+    s = GridBasedTransportSolver(0)
+    s.set_verbose_level(0)
+    s.init(n)
+    prec = Precision.MIXED if args.kernel.startswith("mixed") else Precision.FP64
+    s.solve(D.smoothed_shapes(n) if n == 1024 else D.gaussian_mixture(n), SolverOptions(precision=prec))
+    ms = _lib.C.c_double()
+    _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, KERNEL_IDS[args.kernel], args.profile_reps,
+                                                    _lib.C.byref(ms)), "profile")
+    print(json.dumps({"kernel": args.kernel, "n": n, "reps": args.profile_reps, "ms_per_launch": ms.value}))
 
 
 def cpu_baseline(n, density):
@@ -68,6 +116,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
               "the batched and single solves are measured per GPU", file=sys.stderr)
         sys.exit(2)
+    if args.kernel:
+        kernel_only(args)
+        return
     import torch
 
     from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions
@@ -120,23 +171,50 @@ def main():
             errs[name] = float(max(np.abs(f[idx] - g["fx"]).max(), np.abs(f[V + idx] - g["fy"]).max()))
         err = dict(errs, reference_iterations=int(g["iters"]))
 
-    # roofline of the dominant kernel: the FP64-MFMA DCT GEMM stage (2 n^3 flops per launch)
+    # per-kernel device time: back-to-back launches between HIP events recorded
+    # on the solver's own stream (the stream every kernel of a solve runs on)
     ms = _lib.C.c_double()
-    _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, _lib.OTM_KERNEL_DCT_GEMM, args.profile_reps,
-                                                    _lib.C.byref(ms)), "profile")
-    gemm_ms = ms.value
-    flops = 2.0 * n ** 3
-    achieved = flops / (gemm_ms * 1e-3) / 1e12
     kernel_ms = {}
-    kernels = [("beta", _lib.OTM_KERNEL_BETA), ("dir_coef", _lib.OTM_KERNEL_DIR_COEF),
-               ("update", _lib.OTM_KERNEL_UPDATE), ("dct_solve_f64", _lib.OTM_KERNEL_DCT_SOLVE)]
+    kernels = [("dct_gemm_f64", _lib.OTM_KERNEL_DCT_GEMM), ("beta", _lib.OTM_KERNEL_BETA),
+               ("dir_coef", _lib.OTM_KERNEL_DIR_COEF), ("update", _lib.OTM_KERNEL_UPDATE),
+               ("dct_solve_f64", _lib.OTM_KERNEL_DCT_SOLVE)]
     if "mixed" in results:
         kernels += [("mixed_gemm_bf16x3", _lib.OTM_KERNEL_MIXED_GEMM), ("mixed_solve", _lib.OTM_KERNEL_MIXED_SOLVE)]
     for kname, kid in kernels:
         _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, kid, args.profile_reps, _lib.C.byref(ms)), "profile")
-        kernel_ms[kname] = round(ms.value, 4)
+        kernel_ms[kname] = ms.value
 
     head = "mixed" if "mixed" in results else "fp64"
+
+    # Roofline of the dominant kernel of the headline mode: one DCT GEMM stage
+    # (k_gemm_x3 for mixed, k_gemm_f64 for FP64).  Algorithmic work per launch
+    # is one n x n x n product, 2 n^3 flops; the BF16 split multiplicity (6
+    # products) is implementation overhead and is not counted.
+    def gemm_roofline(kname, t_ms, peak, label):
+        achieved = 2.0 * n ** 3 / (t_ms * 1e-3) / 1e12
+        traffic = TRAFFIC_BYTES_1024.get(kname) if n == 1024 else None
+        return {"kernel": label, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": round(traffic) if traffic else None,
+                "traffic_source": TRAFFIC_SOURCE if traffic else None, "ms_per_launch": round(t_ms, 5)}
+
+    roof_f64 = gemm_roofline("k_gemm_f64", kernel_ms["dct_gemm_f64"], PEAK_FP64_MFMA_TFLOPS,
+                             "k_gemm_f64: one DCT stage, n^3 FP64 MFMA")
+    roof_x3 = (gemm_roofline("k_gemm_x3", kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
+                             "k_gemm_x3: one DCT stage, 3-term BF16 split, FP32 accumulate")
+               if "mixed" in results else None)
+
+    # Whole-iteration roofline: 26 FP64 field touches per face (218 MB at
+    # 1024^2) at HBM peak plus the 8 n^3 flops of the four GEMM stages at the
+    # MFMA peak of the mode, over the measured device time per iteration.
+    def iteration_roofline(name, peak_tflops):
+        t_roof = 26 * 8 * n * n / (PEAK_HBM_GBS * 1e9) + 8.0 * n ** 3 / (peak_tflops * 1e12)
+        t_iter = results[name]["dev_ms"] * 1e-3 / max(results[name]["iters"], 1)
+        return {"t_roof_us": round(t_roof * 1e6, 2), "t_iter_us": round(t_iter * 1e6, 2),
+                "frac": round(t_roof / t_iter, 4)}
+
+    iter_roof = {"fp64": iteration_roofline("fp64", PEAK_FP64_MFMA_TFLOPS)}
+    if "mixed" in results:
+        iter_roof["mixed"] = iteration_roofline("mixed", PEAK_BF16_TFLOPS)
     line = {
         "metric": METRIC,
         "value": round(results[head]["ms"], 3),
@@ -161,10 +239,10 @@ def main():
         "final_residual": {k: v["res"] for k, v in results.items()},
         "init_s": round(t_init, 4),
         "max_vertex_err_vs_reference": err,
-        "kernel_ms": dict(kernel_ms, dct_gemm_f64=round(gemm_ms, 4)),
-        "roofline": {"kernel": "k_gemm_f64 (one DCT stage, n^3 FP64 MFMA)", "bound": "mfma",
-                     "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": None},
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "roofline": roof_x3 if head == "mixed" else roof_f64,
+        "roofline_fp64_gemm": roof_f64,
+        "iteration_roofline": iter_roof,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, rho)

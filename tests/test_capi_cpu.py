@@ -40,3 +40,15 @@ def test_errors_without_device_are_reported():
     # null handles are rejected before any HIP call
     assert _lib.lib().otm_solver_init(None, 16) != 0
     assert b"null" in _lib.lib().otm_last_error()
+
+
+def test_cpp_dropin_headers_build_and_link():
+    # the C++ caller of include/otmap/*.h links against the library; without
+    # arguments it prints its usage before touching a device
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp", "_build",
+                       "dropin_example")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 2 and "usage" in r.stderr
