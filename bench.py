@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--inverse-points", type=int, default=1 << 20)
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
                              "mixed_solve", "copy"],
@@ -91,6 +92,41 @@ def kernel_only(args):
     _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, KERNEL_IDS[args.kernel], args.profile_reps,
                                                     _lib.C.byref(ms)), "profile")
     print(json.dumps({"kernel": args.kernel, "n": n, "reps": args.profile_reps, "ms_per_launch": ms.value}))
+
+
+def inverse_throughput(n, d_fwd, d_adj, count, reps):
+    """TransportMap::inv (otlib/transport_map.cpp:50-102) of `count` seeded
+    uniform points against the forward mesh already in HBM."""
+    import torch
+
+    from otmap_amd import _lib
+    from otmap_amd import densities as D
+
+    L = _lib.lib()
+    h = _lib.C.c_void_p()
+    _lib.check(L.otm_map_create_device(_lib.C.byref(h), 0, n, d_fwd.data_ptr(), d_adj.data_ptr()), "map")
+    try:
+        t0 = time.perf_counter()
+        _lib.check(L.otm_map_init_inverse(h), "init_inverse")
+        t_bvh = time.perf_counter() - t0
+        dev = d_fwd.device
+        pts = torch.from_numpy(D.uniform_points(count, seed=2000 + n).reshape(-1)).to(dev)
+        out = torch.empty(2 * count, dtype=torch.float64, device=dev)
+        face = torch.empty(count, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        _lib.check(L.otm_map_inverse_device(h, pts.data_ptr(), count, 0, out.data_ptr(), face.data_ptr(), None),
+                   "inverse")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(L.otm_map_inverse_device(h, pts.data_ptr(), count, 0, out.data_ptr(), face.data_ptr(), None),
+                       "inverse")
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        located = int((face >= 0).sum().item())
+    finally:
+        L.otm_map_destroy(h)
+    return {"queries": count, "ms": round(ms, 4), "queries_per_s": round(count / (ms * 1e-3)),
+            "algorithmic_GB_per_s": round(36.0 * count / (ms * 1e-3) / 1e9, 1), "located": located,
+            "bvh_build_ms": round(t_bvh * 1e3, 2), "mode": "inv (multi-hit selection), FP64"}
 
 
 def cpu_baseline(n, density):
@@ -158,6 +194,10 @@ def main():
         fwd = d_fwd.cpu().numpy()
         results[name] = dict(ms=wall, iters=stats[-1].iterations, res=stats[-1].final_residual, fwd=fwd,
                              dev_ms=float(np.mean([st.ms_solve for st in stats])))
+
+    # inverse map at 1M uniform query points (device buffers): BVH build on the
+    # host (once per map, as init_inverse), then one query per lane
+    inv_stats = inverse_throughput(n, d_fwd, d_adj, args.inverse_points, args.steps)
 
     # parity against the reference's golden positions when they exist for this workload
     err = None
@@ -239,6 +279,7 @@ def main():
         "final_residual": {k: v["res"] for k, v in results.items()},
         "init_s": round(t_init, 4),
         "max_vertex_err_vs_reference": err,
+        "inverse_1M": inv_stats,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "roofline": roof_x3 if head == "mixed" else roof_f64,
         "roofline_fp64_gemm": roof_f64,
