@@ -230,31 +230,38 @@ def main():
     # (k_gemm_x3 for mixed, k_gemm_f64 for FP64).  Algorithmic work per launch
     # is one n x n x n product, 2 n^3 flops; the BF16 split multiplicity (6
     # products) is implementation overhead and is not counted.
-    def gemm_roofline(kname, t_ms, peak, label):
-        achieved = 2.0 * n ** 3 / (t_ms * 1e-3) / 1e12
+    def gemm_roofline(kname, t_ms, peak, label, flops):
+        achieved = flops / (t_ms * 1e-3) / 1e12
         traffic = TRAFFIC_BYTES_1024.get(kname) if n == 1024 else None
         return {"kernel": label, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": round(traffic) if traffic else None,
                 "traffic_source": TRAFFIC_SOURCE if traffic else None, "ms_per_launch": round(t_ms, 5)}
 
-    roof_f64 = gemm_roofline("k_gemm_f64", kernel_ms["dct_gemm_f64"], PEAK_FP64_MFMA_TFLOPS,
-                             "k_gemm_f64: one DCT stage, n^3 FP64 MFMA")
+    tiles, n_tiles = s.gemm_config()
+    folded = n_tiles > 0
+    stage_flops_f64 = float(n) ** 3 if folded else 2.0 * n ** 3
+    roof_f64 = gemm_roofline("k_gemm_fold" if folded else "k_gemm_f64", kernel_ms["dct_gemm_f64"],
+                             PEAK_FP64_MFMA_TFLOPS,
+                             "k_gemm_fold<F1>: one even/odd-folded DCT stage (n^3 flops), FP64 MFMA" if folded else
+                             "k_gemm_f64: one DCT stage (2 n^3 flops), FP64 MFMA", stage_flops_f64)
     roof_x3 = (gemm_roofline("k_gemm_x3", kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
-                             "k_gemm_x3: one DCT stage, 3-term BF16 split, FP32 accumulate")
+                             "k_gemm_x3: one DCT stage (2 n^3 flops), 3-term BF16 split, FP32 accumulate",
+                             2.0 * n ** 3)
                if "mixed" in results else None)
 
     # Whole-iteration roofline: 26 FP64 field touches per face (218 MB at
-    # 1024^2) at HBM peak plus the 8 n^3 flops of the four GEMM stages at the
-    # MFMA peak of the mode, over the measured device time per iteration.
-    def iteration_roofline(name, peak_tflops):
-        t_roof = 26 * 8 * n * n / (PEAK_HBM_GBS * 1e9) + 8.0 * n ** 3 / (peak_tflops * 1e12)
+    # 1024^2) at HBM peak plus the flops of the four GEMM stages at the MFMA
+    # peak of the mode (8 n^3, or 4 n^3 for the folded FP64 transform), over
+    # the measured device time per iteration.
+    def iteration_roofline(name, peak_tflops, gemm_flops):
+        t_roof = 26 * 8 * n * n / (PEAK_HBM_GBS * 1e9) + gemm_flops / (peak_tflops * 1e12)
         t_iter = results[name]["dev_ms"] * 1e-3 / max(results[name]["iters"], 1)
         return {"t_roof_us": round(t_roof * 1e6, 2), "t_iter_us": round(t_iter * 1e6, 2),
                 "frac": round(t_roof / t_iter, 4)}
 
-    iter_roof = {"fp64": iteration_roofline("fp64", PEAK_FP64_MFMA_TFLOPS)}
+    iter_roof = {"fp64": iteration_roofline("fp64", PEAK_FP64_MFMA_TFLOPS, 4 * stage_flops_f64)}
     if "mixed" in results:
-        iter_roof["mixed"] = iteration_roofline("mixed", PEAK_BF16_TFLOPS)
+        iter_roof["mixed"] = iteration_roofline("mixed", PEAK_BF16_TFLOPS, 8.0 * n ** 3)
     line = {
         "metric": METRIC,
         "value": round(results[head]["ms"], 3),
@@ -284,6 +291,7 @@ def main():
         "roofline": roof_x3 if head == "mixed" else roof_f64,
         "roofline_fp64_gemm": roof_f64,
         "iteration_roofline": iter_roof,
+        "dct_tiles": {"folded": folded, "stage_config": tiles, "candidates": n_tiles},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, rho)

@@ -114,6 +114,18 @@ enum {
 };
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
 
+/* Tile shapes of the four FP64 DCT stages (no reference counterpart).  For
+ * n % 128 == 0 the Laplacian solve runs the even/odd folded transform and
+ * otm_solver_init times every candidate tile shape of each stage on the
+ * device, keeping the fastest (cached per device and n).  The shapes differ in
+ * speed only: every shape accumulates in the same order and gives
+ * bit-identical results.
+ *   otm_solver_gemm_config: cfg[0..3] = chosen shape per stage, returns the
+ *     number of candidate shapes (0 when the plan does not fold);
+ *   otm_solver_set_gemm_config: force shape `config` on `stage` (0..3). */
+int otm_solver_gemm_config(const otm_solver* s, int* cfg);
+int otm_solver_set_gemm_config(otm_solver* s, int stage, int config);
+
 /* Component entry points: the reference's protected stencils, on host
  * buffers, for parity testing and for callers that drive the iteration.
  *   compute_vertex_gradients (otlib/otsolver_2dgrid.cpp:362-423): grads 2*(n+1)^2

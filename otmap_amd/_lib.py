@@ -51,6 +51,8 @@ _SIGNATURES = {
     "otm_solve_device": (C.c_int, [_vp, _vp, C.POINTER(OtmOptions), _vp, _vp, C.POINTER(OtmStats)]),
     "otm_solver_trace": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int]),
     "otm_solver_profile_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "otm_solver_gemm_config": (C.c_int, [_vp, _ip]),
+    "otm_solver_set_gemm_config": (C.c_int, [_vp, C.c_int, C.c_int]),
     "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
     "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),

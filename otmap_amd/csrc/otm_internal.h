@@ -113,11 +113,22 @@ struct DctPlan {
   double* C = nullptr;    // orthonormal DCT-II matrix, row-major C[k][i]
   double* Ct = nullptr;   // its transpose
   double* sq = nullptr;   // sin^2(pi m / (2n)), m = 0..2n
+  // even/odd folded form (n % 128 == 0): with h = n/2,
+  //   P[r][i] = C[2r][i] (r < h), C[2(r-h)+1][i] (r >= h), i < h   (n x h)
+  //   Pt = P^T                                                      (h x n)
+  bool fold = false;
+  double* P = nullptr;
+  double* Pt = nullptr;
+  int cfg[4] = {0, 0, 0, 0};  // tile shape per folded stage, picked by dct_plan_tune
 };
 void dct_plan_create(DctPlan& p, int n);
 void dct_plan_destroy(DctPlan& p);
 void launch_dct_solve(const DctPlan& p, const Fields& f, const double* r, double* out, hipStream_t s);
 void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s);
+// times every candidate tile shape of each folded stage on the handle's stream
+// (once per device and n, cached for the process) and stores the fastest
+void dct_plan_tune(DctPlan& p, int device, const Fields& f, hipStream_t s);
+int dct_fold_configs();
 
 // Mixed-precision Laplacian solve (dct_mixed.hip): 3-term BF16 split GEMMs on
 // MFMA + FP64 refinement.  Requires n % 64 == 0.

@@ -100,6 +100,7 @@ void allocate(otm_solver* s, int n) {
   OTM_HIP(hipMemset(f.state, 0, sizeof(SolveState)));
   ensure_trace(s, 1024);
   dct_plan_create(s->plan, n);
+  dct_plan_tune(s->plan, s->device, f, s->stream);
   s->n = n;
 }
 
@@ -271,6 +272,24 @@ int otm_solver_init(otm_solver* s, int n) {
 }
 
 int otm_solver_grid_size(const otm_solver* s) { return s ? s->n : 0; }
+
+int otm_solver_gemm_config(const otm_solver* s, int* cfg) {
+  if (int rc = check_ready(const_cast<otm_solver*>(s))) return rc;
+  if (!cfg) { set_error("null config array"); return OTM_ERR_ARG; }
+  for (int st = 0; st < 4; ++st) cfg[st] = s->plan.cfg[st];
+  return s->plan.fold ? dct_fold_configs() : 0;
+}
+
+int otm_solver_set_gemm_config(otm_solver* s, int stage, int config) {
+  if (int rc = check_ready(s)) return rc;
+  if (!s->plan.fold) { set_error("the DCT plan of this grid size does not fold (n % 128 != 0)"); return OTM_ERR_STATE; }
+  if (stage < 0 || stage > 3 || config < 0 || config >= dct_fold_configs()) {
+    set_error("stage must be 0..3 and config 0..otm_solver_gemm_config()-1");
+    return OTM_ERR_ARG;
+  }
+  s->plan.cfg[stage] = config;
+  return OTM_OK;
+}
 
 int otm_solve_device(otm_solver* s, const double* d_density, const otm_options* opt, double* d_fwd_xy,
                      double* d_adjusted_density, otm_stats* stats) {

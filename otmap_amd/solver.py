@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import OtmOptions, OtmStats, check, ptr
+from ._lib import OtmOptions, OtmStats, check, iptr, ptr
 
 DBL_MAX = float(np.finfo(np.float64).max)
 
@@ -149,6 +149,18 @@ class GridBasedTransportSolver:
         check(_lib.lib().otm_residual(self._h, ptr(_f64(adjusted_density, n * n)), ptr(_f64(psi, n * n)), ptr(r),
                                       _lib.C.byref(e)), "residual")
         return e.value, r
+
+    def gemm_config(self):
+        """(tile shape per folded DCT stage, number of candidate shapes); the
+        count is 0 when the grid size does not fold (n % 128 != 0)."""
+        cfg = np.zeros(4, dtype=np.int32)
+        count = _lib.lib().otm_solver_gemm_config(self._h, iptr(cfg))
+        if count < 0:
+            check(count, "gemm_config")
+        return [int(c) for c in cfg], count
+
+    def set_gemm_config(self, stage: int, config: int) -> None:
+        check(_lib.lib().otm_solver_set_gemm_config(self._h, stage, config), "set_gemm_config")
 
     def laplacian_solve(self, r):
         n = self._n

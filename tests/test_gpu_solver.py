@@ -150,3 +150,25 @@ def test_max_ratio_adjustment(oracle, solver_factory):
     assert np.allclose(tm.density(), o["density"], rtol=1e-14, atol=0)
     assert abs(tm.density().max() / tm.density().min() - 50.0) < 1e-9
     assert np.abs(tm.fwd_mesh().xy - o["fwd"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_folded_dct_tile_shapes_bit_identical(n, solver_factory):
+    """Every candidate tile shape of the folded FP64 stages gives the same bits
+    (the autotuner may pick any of them)."""
+    s = solver_factory(n)
+    chosen, count = s.gemm_config()
+    assert count >= 2 and all(0 <= c < count for c in chosen)
+    r = _rng(n).standard_normal(n * n)
+    ref = s.laplacian_solve(r)
+    for c in range(count):
+        for st in range(4):
+            s.set_gemm_config(st, c)
+        assert np.array_equal(s.laplacian_solve(r), ref), c
+    for st in range(4):
+        s.set_gemm_config(st, chosen[st])
+
+
+def test_unfolded_sizes_report_no_tile_choice(solver_factory):
+    s = solver_factory(100)
+    assert s.gemm_config()[1] == 0
