@@ -17,6 +17,7 @@
 // (The mean of d_hat is zero up to rounding and no observable of the solver
 // depends on it: every use of psi or d goes through vertex gradients.)
 
+#include <algorithm>
 #include <array>
 #include <cfloat>
 #include <cmath>
@@ -32,6 +33,7 @@ namespace otm {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double dv2 __attribute__((ext_vector_type(2)));
 
 constexpr int BM = 64, BN = 64, BK = 16;
 
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, c
 }
 
 // ---------------------------------------------------------------------------
-// Even/odd folded transform (n % 128 == 0, h = n/2).  Row k of C is symmetric
+// Even/odd folded transform (n % 64 == 0, h = n/2).  Row k of C is symmetric
 // (k even) or antisymmetric (k odd) under i -> n-1-i, so with P = [E; O]
 // (E, O = even / odd rows of C restricted to i < h) the four stages become
 //   F1  T  = P  . fold_rows(R')           rows of T in (even k, odd k) order
@@ -152,30 +154,43 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, c
 // where fold(x)[i] = x[i] + s x[n-1-i] with s = +1 for the even half of the
 // output and -1 for the odd half.  Each stage is n^3 flops instead of 2 n^3.
 // Workgroup tile FM x FN (four waves, 2 x 2), K step KB, double-buffered LDS.
-// The tile shape is picked per stage by dct_plan_tune (all shapes accumulate
+// The tile shape is picked per stage by dct_tune (all shapes accumulate
 // over k in the same order, so every shape gives bit-identical results).
 enum FoldStage { F1 = 0, F2 = 1, F3 = 2, F4 = 3 };
 
 template <int ST, int FM, int FN, int KB>
-__global__ __launch_bounds__(NT) void k_gemm_fold(const double* __restrict__ A, int lda,
-                                                 const double* __restrict__ B, int ldb, double* __restrict__ D,
-                                                 int n, int K, SolveState* st, const double* __restrict__ sq,
-                                                 double* partials, bool rhs) {
-  if (st && st->done) return;
+__global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                 int count, bool gate, bool rhs) {
+  const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
+  const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
+  const int nb = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
+  const Job& J = jobs[p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int n = J.f.n, h = n >> 1;
+  // stage operands:  F1: P (n x h) . fold_rows(r), K = h     F2: fold_cols(T) . Pt (h x n), K = h
+  //                  F3: Pt rows < h . Y, K = n                F4: T3 . P cols < h, K = n
+  const auto A = gp(ST == F1 ? J.P : ST == F3 ? J.Pt : (const double*)J.f.tmp0);
+  const auto B = gp(ST == F1 ? (const double*)J.f.r : ST == F2 ? J.Pt : ST == F3 ? (const double*)J.f.tmp1 : J.P);
+  const auto D = gp(ST == F1 ? J.f.tmp0 : ST == F2 ? J.f.tmp1 : ST == F3 ? J.f.tmp0 : J.f.dhat);
+  const int lda = ST == F1 ? h : n, ldb = ST == F4 ? h : n;
+  const int K = (ST == F1 || ST == F2) ? h : n;
+  const int cols = ST == F4 ? h : n;
+  const auto sq = gp(J.sq);
   constexpr int TI = FM / 32, TJ = FN / 32;  // each wave: FM/2 x FN/2 = TI x TJ MFMA 16x16 tiles
   constexpr int ACHUNKS = FM * KB / 4, BCHUNKS = KB * FN / 4;  // chunks of 4 consecutive values
   constexpr int ACH = (ACHUNKS + NT - 1) / NT, BCH = (BCHUNKS + NT - 1) / NT;
   constexpr bool SPLIT = ST == F3 || ST == F4;
   __shared__ double As[2][FM][KB + 1];
   __shared__ double Bs[2][KB][FN + 2];
-  const int h = n >> 1;
-  const int m0 = blockIdx.y * FM, n0 = blockIdx.x * FN;
+  const int ntn = cols / FN;
+  const int m0 = (lt / ntn) * FM, n0 = (lt % ntn) * FN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * (FM / 2), wn = (w & 1) * (FN / 2);
   const int lr = lane & 15, lk = lane >> 4;
   const double sgnA = (ST == F2 && n0 >= h) ? -1. : 1.;
   const double sgnB = (ST == F1 && m0 >= h) ? -1. : 1.;
-  const double S = (ST == F1 && rhs && st) ? st->sum_r : 0.;
+  const double S = (ST == F1 && rhs) ? st->sum_r : 0.;
   double ra[ACH][4], rb[BCH][4];
 
   auto gload = [&](int k0) {
@@ -184,12 +199,12 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const double* __restrict__ A, 
       const int c = t + u * NT;
       if (ACHUNKS % NT == 0 || c < ACHUNKS) {
         const int ar = c / (KB / 4), ak = (c % (KB / 4)) * 4;
-        const double2* pa = reinterpret_cast<const double2*>(A + long(m0 + ar) * lda + k0 + ak);
-        const double2 x0 = pa[0], x1 = pa[1];
+        const OTM_GLOBAL dv2* pa = reinterpret_cast<const OTM_GLOBAL dv2*>(A + long(m0 + ar) * lda + k0 + ak);
+        const dv2 x0 = pa[0], x1 = pa[1];
         ra[u][0] = x0.x; ra[u][1] = x0.y; ra[u][2] = x1.x; ra[u][3] = x1.y;
         if (ST == F2) {  // mirrored columns n-1-(k0+ak+q), q = 0..3, read as one ascending run
-          const double2* pm = reinterpret_cast<const double2*>(A + long(m0 + ar) * lda + (n - 4 - k0 - ak));
-          const double2 y0 = pm[0], y1 = pm[1];
+          const OTM_GLOBAL dv2* pm = reinterpret_cast<const OTM_GLOBAL dv2*>(A + long(m0 + ar) * lda + (n - 4 - k0 - ak));
+          const dv2 y0 = pm[0], y1 = pm[1];
           ra[u][0] += sgnA * y1.y; ra[u][1] += sgnA * y1.x; ra[u][2] += sgnA * y0.y; ra[u][3] += sgnA * y0.x;
         }
       }
@@ -200,12 +215,12 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const double* __restrict__ A, 
       if (BCHUNKS % NT == 0 || c < BCHUNKS) {
         const int bk = c / (FN / 4), bc = (c % (FN / 4)) * 4;
         const int gk = k0 + bk;
-        const double2* pb = reinterpret_cast<const double2*>(B + long(gk) * ldb + n0 + bc);
-        const double2 y0 = pb[0], y1 = pb[1];
+        const OTM_GLOBAL dv2* pb = reinterpret_cast<const OTM_GLOBAL dv2*>(B + long(gk) * ldb + n0 + bc);
+        const dv2 y0 = pb[0], y1 = pb[1];
         rb[u][0] = y0.x; rb[u][1] = y0.y; rb[u][2] = y1.x; rb[u][3] = y1.y;
         if (ST == F1) {  // mirrored row n-1-gk
-          const double2* pm = reinterpret_cast<const double2*>(B + long(n - 1 - gk) * ldb + n0 + bc);
-          const double2 z0 = pm[0], z1 = pm[1];
+          const OTM_GLOBAL dv2* pm = reinterpret_cast<const OTM_GLOBAL dv2*>(B + long(n - 1 - gk) * ldb + n0 + bc);
+          const dv2 z0 = pm[0], z1 = pm[1];
           rb[u][0] += sgnB * z0.x; rb[u][1] += sgnB * z0.y; rb[u][2] += sgnB * z1.x; rb[u][3] += sgnB * z1.y;
           if (gk == 0 && n0 + bc == 0) rb[u][0] -= S;  // R' = R - (sum r) e0 e0^T
         }
@@ -293,8 +308,8 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const double* __restrict__ A, 
           const double lam = 2. * (sq[dm] + sq[k + l]);
           D[long(row) * n + col] = (k == 0 && l == 0) ? 0. : acc[0][i][j][v] / lam;
         } else {
-          const double p = acc[0][i][j][v], q = acc[SPLIT ? 1 : 0][i][j][v];
-          const double s0 = p + q, s1 = p - q;
+          const double pv = acc[0][i][j][v], qv = acc[SPLIT ? 1 : 0][i][j][v];
+          const double s0 = pv + qv, s1 = pv - qv;
           if (ST == F3) {
             D[long(row) * n + col] = s0;
             D[long(n - 1 - row) * n + col] = s1;
@@ -305,50 +320,29 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const double* __restrict__ A, 
           }
         }
       }
-  if (ST == F4 && partials) write_partials<1>(nrm, partials);
+  if (ST == F4) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nb);
 }
 
-// candidate tile shapes (FM, FN, KB) of the folded stages
-constexpr int FOLD_CFG[][3] = {{64, 64, 16}, {64, 32, 16}, {32, 64, 16}, {32, 32, 16},
-                               {64, 64, 32}, {64, 32, 32}, {32, 64, 32}, {32, 32, 32}};
-constexpr int NFOLD_CFG = int(sizeof(FOLD_CFG) / sizeof(FOLD_CFG[0]));
-
 template <int ST, int CFG>
-void launch_fold_cfg(const DctPlan& p, const double* A, const double* B, double* D, SolveState* st,
-                     double* partials, bool rhs, hipStream_t s) {
+void launch_fold_cfg(const JobSet& js, bool gate, bool rhs, hipStream_t s) {
   constexpr int FM = FOLD_CFG[CFG][0], FN = FOLD_CFG[CFG][1], KB = FOLD_CFG[CFG][2];
-  const int n = p.n, h = n / 2;
-  // F1: P (n x h) . fold_rows(X), K = h      F2: fold_cols(T) . Pt (h x n), K = h
-  // F3: Pt rows < h . Y, K = n               F4: T3 . P cols < h, K = n
-  const int rows = ST == F3 ? h : n, cols = ST == F4 ? h : n;
-  const int K = (ST == F1 || ST == F2) ? h : n;
-  const int lda = ST == F1 ? h : n;
-  const int ldb = ST == F4 ? h : n;
-  const dim3 grid(cols / FN, rows / FM);
-  hipLaunchKernelGGL((k_gemm_fold<ST, FM, FN, KB>), grid, dim3(NT), 0, s, A, lda, B, ldb, D, n, K, st, p.sq,
-                     partials, rhs);
+  const int kind = fold_kind(ST, CFG);
+  if (js.total[kind] == 0) return;
+  hipLaunchKernelGGL((k_gemm_fold<ST, FM, FN, KB>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s,
+                     (const Job*)js.jobs, (const int*)js.starts_of(kind), js.count, gate, rhs);
   OTM_HIP(hipGetLastError());
 }
 
-typedef void (*FoldLauncher)(const DctPlan&, const double*, const double*, double*, SolveState*, double*, bool,
-                             hipStream_t);
+typedef void (*FoldLauncher)(const JobSet&, bool, bool, hipStream_t);
 
 template <int ST, int... C>
 constexpr std::array<FoldLauncher, sizeof...(C)> fold_row(std::integer_sequence<int, C...>) {
   return {&launch_fold_cfg<ST, C>...};
 }
 
-const std::array<std::array<FoldLauncher, NFOLD_CFG>, 4> FOLD_TABLE = {
-    fold_row<F1>(std::make_integer_sequence<int, NFOLD_CFG>{}),
-    fold_row<F2>(std::make_integer_sequence<int, NFOLD_CFG>{}),
-    fold_row<F3>(std::make_integer_sequence<int, NFOLD_CFG>{}),
-    fold_row<F4>(std::make_integer_sequence<int, NFOLD_CFG>{})};
-
-template <int ST>
-void gemm_fold(const DctPlan& p, const double* A, const double* B, double* D, SolveState* st, double* partials,
-               bool rhs, hipStream_t s) {
-  FOLD_TABLE[ST][p.cfg[ST]](p, A, B, D, st, partials, rhs, s);
-}
+const std::array<std::array<FoldLauncher, NFOLD>, 4> FOLD_TABLE = {
+    fold_row<F1>(std::make_integer_sequence<int, NFOLD>{}), fold_row<F2>(std::make_integer_sequence<int, NFOLD>{}),
+    fold_row<F3>(std::make_integer_sequence<int, NFOLD>{}), fold_row<F4>(std::make_integer_sequence<int, NFOLD>{})};
 
 template <int MODE>
 void gemm(const double* A, const double* B, double* D, int n, SolveState* st, const double* sq, double* partials,
@@ -389,7 +383,7 @@ void dct_plan_create(DctPlan& p, int n) {
   OTM_HIP(hipMemcpy(p.Ct, Ct.data(), nn * sizeof(double), hipMemcpyHostToDevice));
   OTM_HIP(hipMemcpy(p.sq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
 
-  p.fold = n % 128 == 0;
+  p.fold = n % 64 == 0;
   if (p.fold) {
     const int h = n / 2;
     std::vector<double> P(size_t(n) * h), Pt(size_t(h) * n);
@@ -416,22 +410,24 @@ void dct_plan_destroy(DctPlan& p) {
   p = DctPlan();
 }
 
-int dct_fold_configs() { return NFOLD_CFG; }
-
-// Autotuner of the folded stages: every candidate shape runs twice untimed,
-// then three times between events on the handle's stream; the fastest mean
-// wins.  The buffers are the solver's own scratch fields (values irrelevant to
-// timing; the last stage writes no partials).  Results are cached per device
-// and n for the life of the process.
-void dct_plan_tune(DctPlan& p, int device, const Fields& f, hipStream_t s) {
-  if (!p.fold) return;
+// Autotuner of the folded stages over a job table: every candidate shape that
+// divides n/2 of every job runs twice untimed, then three times between events
+// on stream s; the fastest wins.  Launches are ungated and work on the jobs'
+// scratch fields (values irrelevant to timing).  Results are cached per device
+// and multiset of grid sizes for the life of the process.
+void dct_tune(JobSet& js, int device, hipStream_t s) {
+  if (!js.fold) return;
   static std::mutex mu;
-  static std::map<std::pair<int, int>, std::array<int, 4>> cache;
+  static std::map<std::pair<int, std::vector<int>>, std::array<int, 4>> cache;
+  std::vector<int> sizes;
+  for (const Job& J : js.host) sizes.push_back(J.f.n);
+  std::sort(sizes.begin(), sizes.end());
+  const auto key = std::make_pair(device, sizes);
   {
     std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find({device, p.n});
+    auto it = cache.find(key);
     if (it != cache.end()) {
-      for (int st = 0; st < 4; ++st) p.cfg[st] = it->second[st];
+      for (int st = 0; st < 4; ++st) js.cfg[st] = it->second[st];
       return;
     }
   }
@@ -440,15 +436,16 @@ void dct_plan_tune(DctPlan& p, int device, const Fields& f, hipStream_t s) {
   OTM_HIP(hipEventCreate(&e1));
   std::array<int, 4> best{};
   for (int st = 0; st < 4; ++st) {
-    const double* A = st == F1 ? p.P : st == F3 ? p.Pt : f.tmp0;
-    const double* B = st == F2 ? p.Pt : st == F4 ? p.P : f.tmp1;
-    double* D = f.dhat;
     float best_ms = 1e30f;
-    for (int c = 0; c < NFOLD_CFG; ++c) {
+    best[st] = -1;
+    for (int c = 0; c < NFOLD; ++c) {
+      bool ok = true;
+      for (int n : sizes) ok = ok && fold_cfg_valid(n, c);
+      if (!ok) continue;
       auto fn = FOLD_TABLE[st][c];
-      for (int rep = 0; rep < 2; ++rep) fn(p, A, B, D, nullptr, nullptr, false, s);
+      for (int rep = 0; rep < 2; ++rep) fn(js, false, false, s);
       OTM_HIP(hipEventRecord(e0, s));
-      for (int rep = 0; rep < 3; ++rep) fn(p, A, B, D, nullptr, nullptr, false, s);
+      for (int rep = 0; rep < 3; ++rep) fn(js, false, false, s);
       OTM_HIP(hipEventRecord(e1, s));
       OTM_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -458,39 +455,44 @@ void dct_plan_tune(DctPlan& p, int device, const Fields& f, hipStream_t s) {
         best[st] = c;
       }
     }
-    p.cfg[st] = best[st];
+    js.cfg[st] = best[st];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lock(mu);
-  cache[{device, p.n}] = best;
+  cache[key] = best;
 }
 
-// one GEMM stage as the solve runs it, without gating, for kernel timing:
-// folded F1 (n^3 flops) when the plan folds, else the plain T3 = C^T Y (2 n^3)
-void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s) {
-  if (p.fold)
-    gemm_fold<F1>(p, p.P, f.tmp1, f.tmp0, nullptr, nullptr, false, s);
-  else
-    gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, p.n, nullptr, p.sq, nullptr, s);
+// one GEMM stage as the solve runs it, ungated, for kernel timing: folded F1
+// (n^3 flops) when the table folds, else the unfolded T3 = C^T Y (2 n^3)
+void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s) {
+  if (js.fold) {
+    FOLD_TABLE[F1][js.cfg[F1]](js, false, false, s);
+  } else {
+    const Fields& f = js.host[0].f;
+    gemm<G_PLAIN>(p->Ct, f.tmp1, f.tmp0, p->n, nullptr, p->sq, nullptr, s);
+  }
 }
 
-// d_hat = C^T [(C R' C^T) / lambda] C.  Every GEMM is skipped once state->done
-// is set; the last stage leaves the |d_hat|^2 partials in SITE_DNORM.
-void launch_dct_solve(const DctPlan& p, const Fields& f, const double* r, double* out, hipStream_t s) {
-  const int n = p.n;
-  SolveState* st = f.state;
-  if (p.fold) {
-    gemm_fold<F1>(p, p.P, r, f.tmp0, st, nullptr, true, s);
-    gemm_fold<F2>(p, f.tmp0, p.Pt, f.tmp1, st, nullptr, false, s);
-    gemm_fold<F3>(p, p.Pt, f.tmp1, f.tmp0, st, nullptr, false, s);
-    gemm_fold<F4>(p, f.tmp0, p.P, out, st, f.site(SITE_DNORM), false, s);
+// d_hat = C^T [(C R' C^T) / lambda] C of every job, R' = r - (sum r) e0.
+// Every GEMM is skipped for jobs whose state is done; the last stage leaves
+// the |d_hat|^2 partials in SITE_DNORM.  Without folding (n % 64 != 0) the
+// table holds one job and p is its plan.
+void launch_dct_solve(const JobSet& js, const DctPlan* p, hipStream_t s) {
+  if (js.fold) {
+    FOLD_TABLE[F1][js.cfg[F1]](js, true, true, s);
+    FOLD_TABLE[F2][js.cfg[F2]](js, true, false, s);
+    FOLD_TABLE[F3][js.cfg[F3]](js, true, false, s);
+    FOLD_TABLE[F4][js.cfg[F4]](js, true, false, s);
     return;
   }
-  gemm<G_RHS>(p.C, r, f.tmp0, n, st, p.sq, nullptr, s);
-  gemm<G_SPECTRAL>(f.tmp0, p.Ct, f.tmp1, n, st, p.sq, nullptr, s);
-  gemm<G_PLAIN>(p.Ct, f.tmp1, f.tmp0, n, st, p.sq, nullptr, s);
-  gemm<G_FINAL>(f.tmp0, p.C, out, n, st, p.sq, f.site(SITE_DNORM), s);
+  const Fields& f = js.host[0].f;
+  const int n = p->n;
+  SolveState* st = f.state;
+  gemm<G_RHS>(p->C, f.r, f.tmp0, n, st, p->sq, nullptr, s);
+  gemm<G_SPECTRAL>(f.tmp0, p->Ct, f.tmp1, n, st, p->sq, nullptr, s);
+  gemm<G_PLAIN>(p->Ct, f.tmp1, f.tmp0, n, st, p->sq, nullptr, s);
+  gemm<G_FINAL>(f.tmp0, p->C, f.dhat, n, st, p->sq, f.site(SITE_DNORM), s);
 }
 
 }  // namespace otm

@@ -3,8 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace otm {
 
@@ -68,6 +71,15 @@ struct SolveState {
   int pad;
 };
 
+// Pointers read from device memory (job tables) are generic to the compiler;
+// re-qualifying them as global lets it emit global instead of flat memory
+// instructions (kernel pointer arguments are global already).
+#define OTM_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline OTM_GLOBAL T* gp(T* p) {
+  return (OTM_GLOBAL T*)p;
+}
+
 // Per-grid device buffers.
 struct Fields {
   int n = 0;
@@ -89,19 +101,78 @@ struct Fields {
   SolveState* state = nullptr;
   double* trace = nullptr;   // 3 * trace_cap: residual, alpha, beta per iteration
   int trace_cap = 0;
-  double* site(int s) const { return partials + long(s) * SITE_VALUES * pslots; }
+  __host__ __device__ double* site(int s) const { return partials + long(s) * SITE_VALUES * pslots; }
 };
 
-// Stencil kernel launchers (stencil.hip)
+// ---------------------------------------------------------------------------
+// Job tables.  Every kernel of the solve runs over a set of independent grids
+// ("jobs") in one launch: workgroup b of a launch of kind K belongs to job p
+// with starts[K][p] <= b < starts[K][p+1] and works on local tile
+// b - starts[K][p] of that grid.  A single solve is a table of one job; a
+// batch of uneven grids is one table, so each step of an iteration is one
+// launch for the whole batch.
+// ---------------------------------------------------------------------------
+struct Job {
+  Fields f;
+  const double* P = nullptr;   // folded DCT plan of f.n (see DctPlan)
+  const double* Pt = nullptr;
+  const double* sq = nullptr;
+  const double* dens_in = nullptr;  // input density of the current solve (F)
+  double* fwd_out = nullptr;        // forward vertex positions (2V, SoA)
+  double* adj_out = nullptr;        // copy of the adjusted density (F, nullable)
+};
+
+// Tile shapes (FM, FN, KB) of the folded FP64 DCT stages, picked per stage by
+// the autotuner among those that divide n/2 for every job of the table.
+constexpr int NFOLD = 8;
+constexpr int FOLD_CFG[NFOLD][3] = {{64, 64, 16}, {64, 32, 16}, {32, 64, 16}, {32, 32, 16},
+                                    {64, 64, 32}, {64, 32, 32}, {32, 64, 32}, {32, 32, 32}};
+inline bool fold_cfg_valid(int n, int c) {
+  const int h = n / 2;
+  return n % 64 == 0 && h % FOLD_CFG[c][0] == 0 && h % FOLD_CFG[c][1] == 0 && h % FOLD_CFG[c][2] == 0;
+}
+// workgroups of folded stage st (0..3) with shape c for grid size n
+inline int fold_tiles(int n, int st, int c) {
+  const int h = n / 2;
+  const int rows = st == 2 ? h : n, cols = st == 3 ? h : n;
+  return (rows / FOLD_CFG[c][0]) * (cols / FOLD_CFG[c][1]);
+}
+// workgroups of the grid-stride ("flat") kernels per job
+inline int flat_blocks(long F) { return int(std::min<long>((F + NT * 4 - 1) / (NT * 4), 1024)); }
+
+// KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 and mixed GEMMs)
+enum LaunchKind {
+  KIND_STENCIL = 0,
+  KIND_FLAT = 1,
+  KIND_PLAIN64 = 2,
+  KIND_GEMM0 = 3,
+  KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
+};
+inline int fold_kind(int stage, int c) { return KIND_GEMM0 + stage * NFOLD + c; }
+
+struct JobSet {
+  int count = 0;
+  Job* jobs = nullptr;     // device copy of the table
+  int* starts = nullptr;   // device: KIND_COUNT x (count + 1) workgroup prefix sums
+  std::vector<Job> host;   // host copy
+  std::vector<int> total;  // workgroups per kind
+  bool fold = false;       // every job uses the folded transform
+  int cfg[4] = {NFOLD - 1, NFOLD - 1, NFOLD - 1, NFOLD - 1};
+  const int* starts_of(int kind) const { return starts + long(kind) * (count + 1); }
+};
+void jobset_create(JobSet& js, const std::vector<Job>& jobs);
+void jobset_upload(JobSet& js, hipStream_t s);  // after changing js.host (same grid sizes)
+void jobset_destroy(JobSet& js);
+
+// Stencil kernel launchers (stencil.hip), one launch per step for all jobs
+void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s);
+// k_beta + scalar kernel; dkind: launch kind whose workgroups wrote the
+// |d_hat|^2 partials (the last stage of the Laplacian solve)
+void launch_beta(const JobSet& js, int dkind, hipStream_t s);
+void launch_dir_coef(const JobSet& js, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
+void launch_update(const JobSet& js, hipStream_t s);
+void launch_finish(const JobSet& js, hipStream_t s);     // forward positions, adjusted density copy
 void launch_fill(double* p, long count, double v, hipStream_t s);
-void launch_adjust_density(const double* in, const Fields& f, double max_ratio, hipStream_t s);
-void launch_initial_residual(const Fields& f, int max_iter, double threshold, int cj, hipStream_t s);
-void launch_beta(const Fields& f, hipStream_t s);       // k_beta + scalar kernel
-void launch_dir_coef(const Fields& f, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
-void launch_update(const Fields& f, hipStream_t s);
-// number of workgroups of the final DCT stage (its |d_hat|^2 partials go to SITE_DNORM)
-int dnorm_blocks(int n, bool mixed);
-void launch_forward_positions(const double* psi, double* xy, int n, hipStream_t s);
 // component kernels (single evaluations)
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s);
 void launch_residual_only(const double* psi, const double* rho, double* r, double* partials, int n, hipStream_t s);
@@ -113,22 +184,23 @@ struct DctPlan {
   double* C = nullptr;    // orthonormal DCT-II matrix, row-major C[k][i]
   double* Ct = nullptr;   // its transpose
   double* sq = nullptr;   // sin^2(pi m / (2n)), m = 0..2n
-  // even/odd folded form (n % 128 == 0): with h = n/2,
+  // even/odd folded form (n % 64 == 0): with h = n/2,
   //   P[r][i] = C[2r][i] (r < h), C[2(r-h)+1][i] (r >= h), i < h   (n x h)
   //   Pt = P^T                                                      (h x n)
   bool fold = false;
   double* P = nullptr;
   double* Pt = nullptr;
-  int cfg[4] = {0, 0, 0, 0};  // tile shape per folded stage, picked by dct_plan_tune
 };
 void dct_plan_create(DctPlan& p, int n);
 void dct_plan_destroy(DctPlan& p);
-void launch_dct_solve(const DctPlan& p, const Fields& f, const double* r, double* out, hipStream_t s);
-void launch_dct_gemm_stage(const DctPlan& p, const Fields& f, hipStream_t s);
-// times every candidate tile shape of each folded stage on the handle's stream
-// (once per device and n, cached for the process) and stores the fastest
-void dct_plan_tune(DctPlan& p, int device, const Fields& f, hipStream_t s);
-int dct_fold_configs();
+// FP64 Laplacian solve of every job (folded when js.fold, else the unfolded
+// transform of the single job with plan p)
+void launch_dct_solve(const JobSet& js, const DctPlan* p, hipStream_t s);
+void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s);
+// times every valid candidate tile shape of each folded stage over the table's
+// jobs on stream s (cached per device and set of grid sizes) and stores the
+// fastest in js.cfg
+void dct_tune(JobSet& js, int device, hipStream_t s);
 
 // Mixed-precision Laplacian solve (dct_mixed.hip): 3-term BF16 split GEMMs on
 // MFMA + FP64 refinement.  Requires n % 64 == 0.

@@ -34,14 +34,28 @@ __device__ static inline void block_sum(double (&v)[K]) {
     for (int k = 0; k < K; ++k) v[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
 }
 
-// pass 1: K workgroup partials
+// pass 1: K workgroup partials of workgroup bid among nb
+template <int K>
+__device__ static inline void write_partials_at(double (&v)[K], double* partials, int bid, int nb) {
+  block_sum<K>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; ++k) gp(partials)[long(k) * nb + bid] = v[k];
+}
+
 template <int K>
 __device__ static inline void write_partials(double (&v)[K], double* partials) {
-  block_sum<K>(v);
-  if (threadIdx.x == 0) {
-    const int nb = block_count(), bid = block_id();
-    for (int k = 0; k < K; ++k) partials[long(k) * nb + bid] = v[k];
+  write_partials_at<K>(v, partials, block_id(), block_count());
+}
+
+// Job of workgroup b in a launch over a job table (starts: count + 1 prefix
+// sums of the workgroups per job); uniform across the workgroup.
+__device__ static inline int job_of(const int* __restrict__ starts, int count, int b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= b) lo = mid; else hi = mid - 1;
   }
+  return lo;
 }
 
 // pass 2 (single workgroup): fold nb partials of each of K values; valid in
@@ -57,7 +71,7 @@ __device__ static inline void fold_partials(double (&v)[K], const double* partia
 #pragma unroll
       for (int u = 0; u < FOLD_BATCH; ++u) {
         const int j = base + u * NT + threadIdx.x;
-        tmp[u] = (j < nb) ? partials[long(k) * nb + j] : 0.;
+        tmp[u] = (j < nb) ? gp(partials)[long(k) * nb + j] : 0.;
       }
 #pragma unroll
       for (int u = 0; u < FOLD_BATCH; ++u) s += tmp[u];

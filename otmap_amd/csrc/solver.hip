@@ -2,11 +2,12 @@
 // host-side orchestration of GridBasedTransportSolver::init()/solve()
 // (otlib/otsolver_2dgrid.cpp:75-245).
 //
-// One Newton iteration is seven kernels on the handle's stream:
-//   4 DCT GEMMs (d_hat = M^+ (r - S e0))            otsolver_2dgrid.cpp:174-183
-//   k_beta      (conjugate-Jacobian beta)            :191-200, :248-261
-//   k_dir_coef  (d, quartic coefficients, alpha)      :467-598
-//   k_update    (psi, G0, r)                          :599-605
+// One Newton iteration is nine kernels on the handle's stream, each launched
+// over the handle's one-job table:
+//   4 DCT GEMM stages (d_hat = M^+ (r - S e0))                otsolver_2dgrid.cpp:174-183
+//   k_beta + k_scalar_beta (conjugate-Jacobian beta)           :191-200, :248-261
+//   k_dir_coef + k_scalar_step (d, quartic coefficients, alpha) :467-598
+//   k_update (psi, G0, r)                                      :599-605
 // All scalar decisions are taken on the device; the host only checks the
 // stop flag once per chunk of iterations, and kernels enqueued past the stop
 // return immediately.
@@ -36,6 +37,7 @@ struct otm_solver {
   int verbose = 1;
   int n = 0;
   Fields f;
+  JobSet js;                   // the one-job table every kernel of a solve runs over
   DctPlan plan;
   MixedPlan mplan;             // created on the first mixed-precision solve
   double* d_in = nullptr;      // staging of host-API inputs (n*n)
@@ -65,6 +67,7 @@ void release_fields(otm_solver* s) {
   dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
   dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.trace);
   dfree(s->d_in); dfree(s->d_xy);
+  jobset_destroy(s->js);
   mixed_plan_destroy(s->mplan);
   dct_plan_destroy(s->plan);
   f = Fields();
@@ -76,12 +79,14 @@ long partial_slots(int n) {
   return std::max<long>(std::max<long>(num_tiles(n), gemm_tiles), 3 * 1024);
 }
 
-void ensure_trace(otm_solver* s, int cap) {
+// grows the trace buffer; true when it was reallocated
+bool ensure_trace(otm_solver* s, int cap) {
   Fields& f = s->f;
-  if (cap <= f.trace_cap) return;
+  if (cap <= f.trace_cap) return false;
   dfree(f.trace);
   dalloc(f.trace, 3 * size_t(cap));
   f.trace_cap = cap;
+  return true;
 }
 
 void allocate(otm_solver* s, int n) {
@@ -100,7 +105,13 @@ void allocate(otm_solver* s, int n) {
   OTM_HIP(hipMemset(f.state, 0, sizeof(SolveState)));
   ensure_trace(s, 1024);
   dct_plan_create(s->plan, n);
-  dct_plan_tune(s->plan, s->device, f, s->stream);
+  Job J;
+  J.f = f;
+  J.P = s->plan.P;
+  J.Pt = s->plan.Pt;
+  J.sq = s->plan.sq;
+  jobset_create(s->js, {J});
+  dct_tune(s->js, s->device, s->stream);
   s->n = n;
 }
 
@@ -126,14 +137,19 @@ int check_ready(otm_solver* s) {
   return OTM_OK;
 }
 
-void enqueue_iteration(otm_solver* s, const otm_options& opt) {
-  if (opt.precision == OTM_PRECISION_MIXED)
-    launch_mixed_solve(s->mplan, s->plan, s->f, s->f.r, s->f.dhat, opt.refine_steps, s->stream);
+// launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
+int dnorm_kind(const JobSet& js, bool mixed) { return (mixed || !js.fold) ? KIND_PLAIN64 : fold_kind(3, js.cfg[3]); }
+
+void enqueue_iteration(otm_solver* s, const JobSet& js, const otm_options& opt) {
+  const bool mixed = opt.precision == OTM_PRECISION_MIXED;
+  if (mixed)
+    launch_mixed_solve(s->mplan, s->plan, js.host[0].f, js.host[0].f.r, js.host[0].f.dhat, opt.refine_steps,
+                       s->stream);
   else
-    launch_dct_solve(s->plan, s->f, s->f.r, s->f.dhat, s->stream);
-  launch_beta(s->f, s->stream);
-  launch_dir_coef(s->f, s->stream);
-  launch_update(s->f, s->stream);
+    launch_dct_solve(js, &s->plan, s->stream);
+  launch_beta(js, dnorm_kind(js, mixed), s->stream);
+  launch_dir_coef(js, s->stream);
+  launch_update(js, s->stream);
 }
 
 int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_in, double* d_fwd, double* d_adj,
@@ -159,20 +175,21 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
   }
   Fields& f = s->f;
   hipStream_t st = s->stream;
-  ensure_trace(s, std::max(opt.max_iter, 1));
+  if (ensure_trace(s, std::max(opt.max_iter, 1))) s->js.host[0].f = f;
+  Job& J = s->js.host[0];
+  J.dens_in = d_density;
+  J.fwd_out = d_fwd;
+  J.adj_out = d_adj;
+  jobset_upload(s->js, st);
   OTM_HIP(hipEventRecord(s->ev0, st));
-  launch_adjust_density(d_density, f, opt.max_ratio, st);
-  launch_fill(f.psi, f.F, 0., st);
-  launch_fill(f.g0, 2 * f.V, 0., st);
-  launch_fill(f.d[0], f.F, 0., st);
-  launch_fill(f.d[1], f.F, 0., st);
-  launch_initial_residual(f, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, st);
+  launch_start(s->js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0,
+               st);
   // Iterations are enqueued in chunks; the stop flag is read back once per
   // chunk.  Iterations enqueued after the stop are no-ops on the device.
   long enqueued = 0;
   int chunk = 4;
   for (;;) {
-    for (int c = 0; c < chunk && enqueued <= long(opt.max_iter); ++c, ++enqueued) enqueue_iteration(s, opt);
+    for (int c = 0; c < chunk && enqueued <= long(opt.max_iter); ++c, ++enqueued) enqueue_iteration(s, s->js, opt);
     OTM_HIP(hipMemcpyAsync(s->h_state, f.state, sizeof(SolveState), hipMemcpyDeviceToHost, st));
     OTM_HIP(hipStreamSynchronize(st));
     if (s->h_state->done) break;
@@ -182,8 +199,7 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
     }
     chunk = std::min(chunk * 2, 16);
   }
-  launch_forward_positions(f.psi, d_fwd, s->n, st);
-  if (d_adj) OTM_HIP(hipMemcpyAsync(d_adj, f.rho, f.F * sizeof(double), hipMemcpyDeviceToDevice, st));
+  launch_finish(s->js, st);
   OTM_HIP(hipEventRecord(s->ev1, st));
   OTM_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
@@ -276,18 +292,19 @@ int otm_solver_grid_size(const otm_solver* s) { return s ? s->n : 0; }
 int otm_solver_gemm_config(const otm_solver* s, int* cfg) {
   if (int rc = check_ready(const_cast<otm_solver*>(s))) return rc;
   if (!cfg) { set_error("null config array"); return OTM_ERR_ARG; }
-  for (int st = 0; st < 4; ++st) cfg[st] = s->plan.cfg[st];
-  return s->plan.fold ? dct_fold_configs() : 0;
+  for (int st = 0; st < 4; ++st) cfg[st] = s->js.cfg[st];
+  return s->js.fold ? NFOLD : 0;
 }
 
 int otm_solver_set_gemm_config(otm_solver* s, int stage, int config) {
   if (int rc = check_ready(s)) return rc;
-  if (!s->plan.fold) { set_error("the DCT plan of this grid size does not fold (n % 128 != 0)"); return OTM_ERR_STATE; }
-  if (stage < 0 || stage > 3 || config < 0 || config >= dct_fold_configs()) {
+  if (!s->js.fold) { set_error("the DCT of this grid size does not fold (n % 64 != 0)"); return OTM_ERR_STATE; }
+  if (stage < 0 || stage > 3 || config < 0 || config >= NFOLD) {
     set_error("stage must be 0..3 and config 0..otm_solver_gemm_config()-1");
     return OTM_ERR_ARG;
   }
-  s->plan.cfg[stage] = config;
+  if (!fold_cfg_valid(s->n, config)) { set_error("tile shape does not divide n/2"); return OTM_ERR_ARG; }
+  s->js.cfg[stage] = config;
   return OTM_OK;
 }
 
@@ -350,10 +367,14 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   // A scratch state keeps every gate open: iteration 1, conjugate-Jacobian
   // beta, update pending.  Intermediate buffers of the last solve are reused
   // as inputs and overwritten.
-  Fields f = s->f;
   SolveState* scratch = nullptr;
   OTM_HIP(hipMalloc(&scratch, sizeof(SolveState)));
-  f.state = scratch;
+  Job J = s->js.host[0];
+  J.f.state = scratch;
+  Fields& f = J.f;
+  JobSet pj;
+  jobset_create(pj, {J});
+  for (int st = 0; st < 4; ++st) pj.cfg[st] = s->js.cfg[st];
   SolveState tpl;
   std::memset(&tpl, 0, sizeof(tpl));
   tpl.it = 1; tpl.cj = 1; tpl.max_iter = 1 << 30; tpl.threshold = 0.; tpl.alpha = 0.5; tpl.beta = 0.1;
@@ -370,11 +391,11 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->stream));
   auto launch = [&]() {
     switch (kernel) {
-      case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(s->plan, f, s->stream); break;
-      case OTM_KERNEL_BETA: launch_beta(f, s->stream); break;
-      case OTM_KERNEL_DIR_COEF: launch_dir_coef(f, s->stream); break;
-      case OTM_KERNEL_UPDATE: launch_update(f, s->stream); break;
-      case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(s->plan, f, f.r, f.dhat, s->stream); break;
+      case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(pj, &s->plan, s->stream); break;
+      case OTM_KERNEL_BETA: launch_beta(pj, dnorm_kind(pj, false), s->stream); break;
+      case OTM_KERNEL_DIR_COEF: launch_dir_coef(pj, s->stream); break;
+      case OTM_KERNEL_UPDATE: launch_update(pj, s->stream); break;
+      case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(pj, &s->plan, s->stream); break;
       case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(s->mplan, s->plan, s->stream); break;
       default: launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, 1, s->stream); break;
     }
@@ -388,6 +409,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   OTM_HIP(hipEventElapsedTime(&ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  jobset_destroy(pj);
   (void)hipFree(scratch);
   *ms_per_launch = double(ms) / reps;
   return OTM_OK;
@@ -442,7 +464,7 @@ int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat) {
   st.sum_r = sum;
   OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->stream));
   OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  launch_dct_solve(s->plan, f, f.r, f.dhat, s->stream);
+  launch_dct_solve(s->js, &s->plan, s->stream);
   OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   OTM_HIP(hipStreamSynchronize(s->stream));
   return OTM_OK;

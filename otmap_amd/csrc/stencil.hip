@@ -167,17 +167,57 @@ __device__ inline double pick_step(const double* z, double& emin) {
 }
 
 // ---------------------------------------------------------------------------
-// kernels
+// kernels of the solve, over a job table (see Job / JobSet)
 // ---------------------------------------------------------------------------
+#define JOB_ARGS const Job* __restrict__ jobs, const int* __restrict__ starts, int count
+
+struct Loc {
+  int p;   // job
+  int t;   // workgroup within the job
+  int nb;  // workgroups of the job in this launch
+};
+
+__device__ inline Loc locate(const int* __restrict__ starts, int count) {
+  const int b = blockIdx.x;
+  if (count == 1) return {0, b, int(gridDim.x)};  // one job: no table lookup
+  const int p = job_of(starts, count, b);
+  return {p, b - starts[p], starts[p + 1] - starts[p]};
+}
+
+// tile origin of local stencil tile t (row-major over the tiles_j(n) x tiles_i(n) grid)
+__device__ inline void tile_origin(int t, int n, int& i0, int& j0) {
+  const int tj = (n + TJ - 1) / TJ;
+  i0 = (t / tj) * TI;
+  j0 = (t % tj) * TJ;
+}
+
 __global__ __launch_bounds__(NT) void k_fill(double* p, long count, double v) {
   for (long i = long(blockIdx.x) * NT + threadIdx.x; i < count; i += long(gridDim.x) * NT) p[i] = v;
 }
 
+// psi = 0, G0 = 0, both search directions = 0 (otsolver_2dgrid.cpp:126-140)
+__global__ __launch_bounds__(NT) void k_init_fields(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Fields& f = jobs[L.p].f;
+  const long stride = long(L.nb) * NT;
+  const auto psi = gp(f.psi), d0 = gp(f.d[0]), d1 = gp(f.d[1]), g0 = gp(f.g0);
+  for (long i = long(L.t) * NT + threadIdx.x; i < f.F; i += stride) {
+    psi[i] = 0.;
+    d0[i] = 0.;
+    d1[i] = 0.;
+  }
+  for (long i = long(L.t) * NT + threadIdx.x; i < 2 * f.V; i += stride) g0[i] = 0.;
+}
+
 // density adjustment (otsolver_2dgrid.cpp:47-73), pass 1: sum / min / max
-__global__ __launch_bounds__(NT) void k_adjust_reduce(const double* in, long F, double* partials) {
+__global__ __launch_bounds__(NT) void k_adjust_reduce(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto in = gp(J.dens_in);
+  const long F = J.f.F;
   double s[1] = {0.};
   double mn = DBL_MAX, mx = -DBL_MAX;
-  for (long i = long(blockIdx.x) * NT + threadIdx.x; i < F; i += long(gridDim.x) * NT) {
+  for (long i = long(L.t) * NT + threadIdx.x; i < F; i += long(L.nb) * NT) {
     const double v = in[i];
     s[0] += v;
     mn = fmin(mn, v);
@@ -193,16 +233,20 @@ __global__ __launch_bounds__(NT) void k_adjust_reduce(const double* in, long F, 
   block_sum<1>(s);  // synchronises after the mm writes
   if (threadIdx.x == 0) {
     for (int k = 1; k < NT / 64; ++k) { mm[0][0] = fmin(mm[0][0], mm[0][k]); mm[1][0] = fmax(mm[1][0], mm[1][k]); }
-    const int nb = gridDim.x;
-    partials[blockIdx.x] = s[0];
-    partials[nb + blockIdx.x] = mm[0][0];
-    partials[2 * nb + blockIdx.x] = mm[1][0];
+    const auto partials = gp(J.f.site(SITE_ADJUST));
+    partials[L.t] = s[0];
+    partials[L.nb + L.t] = mm[0][0];
+    partials[2 * L.nb + L.t] = mm[1][0];
   }
 }
 
-// pass 2 (one workgroup): integral, ratio cap -> scale and shift
-__global__ __launch_bounds__(NT) void k_adjust_scalar(const double* partials, int nb, double A, double max_ratio,
-                                                     SolveState* st) {
+// pass 2 (one workgroup per job): integral, ratio cap -> scale and shift
+__global__ __launch_bounds__(NT) void k_adjust_scalar(JOB_ARGS, double max_ratio) {
+  const int p = blockIdx.x;
+  const Fields& f = jobs[p].f;
+  const int nb = starts[p + 1] - starts[p];
+  const auto partials = gp(f.site(SITE_ADJUST));
+  const double A = 1.0 / (double(f.n) * double(f.n));
   double tot[1] = {0.};
   double gmn = DBL_MAX, gmx = -DBL_MAX;
   for (int j = threadIdx.x; j < nb; j += NT) {
@@ -224,23 +268,32 @@ __global__ __launch_bounds__(NT) void k_adjust_scalar(const double* partials, in
     const double nmn = mm[0][0] / I, nmx = mm[1][0] / I;  // division by I > 0 is monotone
     const double ratio = fmin(max_ratio, nmx / nmn);
     const double cand = (ratio * nmn - nmx) / (1. - ratio);
+    const auto st = gp(f.state);
     st->dens_scale = I;
     st->dens_shift = (0. < cand) ? cand : 0.;  // std::max(0., cand), NaN -> 0
     st->max_ratio = max_ratio;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_adjust_apply(const double* in, double* rho, long F, const SolveState* st) {
+__global__ __launch_bounds__(NT) void k_adjust_apply(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto in = gp(J.dens_in);
+  const auto rho = gp(J.f.rho);
+  const auto st = gp((const SolveState*)J.f.state);
   const double I = st->dens_scale, b = st->dens_shift;
-  for (long i = long(blockIdx.x) * NT + threadIdx.x; i < F; i += long(gridDim.x) * NT)
-    rho[i] = (in[i] / I + b) / (1. + b);
+  for (long i = long(L.t) * NT + threadIdx.x; i < J.f.F; i += long(L.nb) * NT) rho[i] = (in[i] / I + b) / (1. + b);
 }
 
 // initial residual at psi = 0: r_0 and the partials of |r|^2, sum r
-__global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, double* r, int n, double* partials) {
+__global__ __launch_bounds__(NT) void k_initial_residual(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Fields& f = jobs[L.p].f;
+  const int n = f.n;
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
-  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  int i0, j0;
+  tile_origin(L.t, n, i0, j0);
   load_halo(s, n, i0, j0, [](long) { return 0.0; });
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
@@ -251,23 +304,25 @@ __global__ __launch_bounds__(NT) void k_initial_residual(const double* rho, doub
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
-      const long f = long(fi) * n + fj;
-      const double rv = quad_area(gx, gy, a, b, e) - A * rho[f];
-      r[f] = rv;
+      const long fc = long(fi) * n + fj;
+      const double rv = quad_area(gx, gy, a, b, e) - A * gp(f.rho)[fc];
+      gp(f.r)[fc] = rv;
       acc[0] += rv * rv;
       acc[1] += rv;
     }
   }
-  write_partials<2>(acc, partials);
+  write_partials_at<2>(acc, f.site(SITE_RESID), L.t, L.nb);
 }
 
-// solver state initialisation (otsolver_2dgrid.cpp:126-160)
-__global__ __launch_bounds__(NT) void k_scalar_init(const double* partials, int nb, int n, SolveState* st,
-                                                   int max_iter, double threshold, int cj) {
+// solver state initialisation (otsolver_2dgrid.cpp:126-160), one workgroup per job
+__global__ __launch_bounds__(NT) void k_scalar_init(JOB_ARGS, int max_iter, double threshold, int cj) {
+  const int p = blockIdx.x;
+  const Fields& f = jobs[p].f;
   double acc[2];
-  fold_partials<2>(acc, partials, nb);
+  fold_partials<2>(acc, f.site(SITE_RESID), starts[p + 1] - starts[p]);
   if (threadIdx.x == 0) {
-    const double A = 1.0 / (double(n) * double(n));
+    const auto st = gp(f.state);
+    const double A = 1.0 / (double(f.n) * double(f.n));
     const double e0 = acc[0] / A;
     st->e = e0;
     st->e0 = e0;
@@ -289,14 +344,21 @@ __global__ __launch_bounds__(NT) void k_scalar_init(const double* partials, int 
 
 // conjugate-Jacobian beta, pass 1: residual at psi - eps*d_hat (eps = alpha/2)
 // and the partials of Jd.(r_eps - r_k), Jd.Jd   (otsolver_2dgrid.cpp:248-261)
-__global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dhat, const double* rho, const double* r,
-                                            const double* jd, int n, double* partials, const SolveState* st) {
+__global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Fields& f = jobs[L.p].f;
+  const auto st = gp((const SolveState*)f.state);
   if (st->done || st->it < 1 || !st->cj) return;
+  const int n = f.n;
+  const auto psi = gp(f.psi);
+  const auto dhat = gp(f.dhat);
+  const auto rho = gp(f.rho), r = gp(f.r), jd = gp(f.jd);
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
   const double eps = st->alpha / 2.;
-  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
-  load_halo(s, n, i0, j0, [&](long f) { return psi[f] - eps * dhat[f]; });
+  int i0, j0;
+  tile_origin(L.t, n, i0, j0);
+  load_halo(s, n, i0, j0, [&](long fc) { return psi[fc] - eps * dhat[fc]; });
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
@@ -306,21 +368,24 @@ __global__ __launch_bounds__(NT) void k_beta(const double* psi, const double* dh
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
-      const long f = long(fi) * n + fj;
-      const double reps = quad_area(gx, gy, a, b, e) - A * rho[f];
-      const double j = jd[f];
-      acc[0] += j * (reps - r[f]);
+      const long fc = long(fi) * n + fj;
+      const double reps = quad_area(gx, gy, a, b, e) - A * rho[fc];
+      const double j = jd[fc];
+      acc[0] += j * (reps - r[fc]);
       acc[1] += j * j;
     }
   }
-  write_partials<2>(acc, partials);
+  write_partials_at<2>(acc, f.site(SITE_BETA), L.t, L.nb);
 }
 
-// pass 2 (one workgroup): early stop on |d_hat| (otsolver_2dgrid.cpp:182-183),
+// pass 2 (one workgroup per job): early stop on |d_hat| (otsolver_2dgrid.cpp:182-183),
 // then beta = max(-1, Jd.(r_eps - r_k) / |Jd|^2 / eps * alpha) and the choice
 // d = d_hat + beta d_prev (it >= 1, conjugate-Jacobian) or d = d_hat.
-__global__ __launch_bounds__(NT) void k_scalar_beta(const double* dnorm_partials, int dnorm_nb,
-                                                   const double* beta_partials, int beta_nb, SolveState* st) {
+// dstarts: workgroup prefix of the launch that wrote the |d_hat|^2 partials.
+__global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restrict__ dstarts) {
+  const int p = blockIdx.x;
+  const Fields& f = jobs[p].f;
+  const auto st = gp(f.state);
   __shared__ int stop;
   if (threadIdx.x == 0) {
     st->update = 0;  // the previous iteration's update has run
@@ -329,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(const double* dnorm_partials
   __syncthreads();
   if (stop) return;
   double dn[1];
-  fold_partials<1>(dn, dnorm_partials, dnorm_nb);
+  fold_partials<1>(dn, f.site(SITE_DNORM), dstarts[p + 1] - dstarts[p]);
   __shared__ int early;
   if (threadIdx.x == 0) {
     const double dnorm = sqrt(dn[0]);
@@ -348,7 +413,7 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(const double* dnorm_partials
     return;
   }
   double acc[2];
-  fold_partials<2>(acc, beta_partials, beta_nb);
+  fold_partials<2>(acc, f.site(SITE_BETA), starts[p + 1] - starts[p]);
   if (threadIdx.x == 0) {
     const double eps = st->alpha / 2.;
     const double cand = acc[0] / acc[1] / eps * st->alpha;
@@ -360,22 +425,28 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(const double* dnorm_partials
 // direction d = d_hat (+ beta d_prev), vertex gradients of d, quartic
 // coefficients a, b per quad (r(psi + t d) = r + b t + a t^2), and the
 // partials of b.r, a.r, b.b, a.b, a.a, sum a, sum b   (otsolver_2dgrid.cpp:467-561)
-__global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* const* dbuf, const double* g0,
-                                                const double* r, double* aout, double* bout, int n, double* partials,
-                                                const SolveState* st) {
+__global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Fields& f = jobs[L.p].f;
+  const auto st = gp((const SolveState*)f.state);
   if (st->done) return;
+  const int n = f.n;
   __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
   __shared__ double s[NF];
   const int k = st->it;
-  const double* dprev = dbuf[k & 1];
-  double* dnew = dbuf[(k + 1) & 1];
+  const auto dhat = gp(f.dhat);
+  const auto dprev = gp(f.d[k & 1]);
+  const auto dnew = gp(f.d[(k + 1) & 1]);
+  const auto g0 = gp(f.g0);
+  const auto aout = gp(f.a), bout = gp(f.b), r = gp(f.r);
   const bool use_prev = st->use_prev != 0;
   const double beta = st->beta;
-  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+  int i0, j0;
+  tile_origin(L.t, n, i0, j0);
   if (use_prev)
-    load_halo(s, n, i0, j0, [&](long f) { return dhat[f] + beta * dprev[f]; });
+    load_halo(s, n, i0, j0, [&](long fc) { return dhat[fc] + beta * dprev[fc]; });
   else
-    load_halo(s, n, i0, j0, [&](long f) { return dhat[f]; });
+    load_halo(s, n, i0, j0, [&](long fc) { return dhat[fc]; });
   const long V1 = n + 1;
   const long V = V1 * V1;
   for (int t = threadIdx.x; t < NV; t += NT) {
@@ -394,8 +465,8 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
-      const long f = long(fi) * n + fj;
-      dnew[f] = s[(a + 1) * LDF + b + 1];
+      const long fc = long(fi) * n + fj;
+      dnew[fc] = s[(a + 1) * LDF + b + 1];
       const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
       const double d0ax = hx[v11] - hx[v00] + e;
       const double d0ay = hy[v11] - hy[v00] + e;
@@ -407,9 +478,9 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
       const double ddby = gy[v01] - gy[v10];
       const double av = 0.5 * (ddax * ddby - dday * ddbx);
       const double bv = 0.5 * (ddax * d0by - dday * d0bx + d0ax * ddby - d0ay * ddbx);
-      aout[f] = av;
-      bout[f] = bv;
-      const double rv = r[f];
+      aout[fc] = av;
+      bout[fc] = bv;
+      const double rv = r[fc];
       acc[0] += bv * rv;
       acc[1] += av * rv;
       acc[2] += bv * bv;
@@ -419,22 +490,24 @@ __global__ __launch_bounds__(NT) void k_dir_coef(const double* dhat, double* con
       acc[6] += bv;
     }
   }
-  write_partials<7>(acc, partials);
+  write_partials_at<7>(acc, f.site(SITE_COEF), L.t, L.nb);
 }
 
-// pass 2 (one workgroup): z = [e A, 2 b.r, b.b + 2 a.r, 2 a.b, a.a], the step
-// alpha, e(alpha)/A, traces, the iteration count and the stop test
+// pass 2 (one workgroup per job): z = [e A, 2 b.r, b.b + 2 a.r, 2 a.b, a.a], the
+// step alpha, e(alpha)/A, traces, the iteration count and the stop test
 // (otsolver_2dgrid.cpp:556-605, :160); sum r_{k+1} = alpha^2 sum a + alpha sum b + sum r_k.
-__global__ __launch_bounds__(NT) void k_scalar_step(const double* partials, int nb, int n, SolveState* st,
-                                                   double* trace, int trace_cap) {
+__global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
+  const int p = blockIdx.x;
+  const Fields& f = jobs[p].f;
+  const auto st = gp(f.state);
   __shared__ int stop;
   if (threadIdx.x == 0) stop = st->done;
   __syncthreads();
   if (stop) return;
   double acc[7];
-  fold_partials<7>(acc, partials, nb);
+  fold_partials<7>(acc, f.site(SITE_COEF), starts[p + 1] - starts[p]);
   if (threadIdx.x == 0) {
-    const double A = 1.0 / (double(n) * double(n));
+    const double A = 1.0 / (double(f.n) * double(f.n));
     double z[5];
     z[0] = st->e * A;
     z[1] = 2. * acc[0];
@@ -448,10 +521,11 @@ __global__ __launch_bounds__(NT) void k_scalar_step(const double* partials, int 
     st->e = emin / A;
     st->sum_r = acc[5] * (alpha * alpha) + acc[6] * alpha + st->sum_r;
     const int k = st->it;
-    if (k < trace_cap) {
-      trace[k] = st->e;
-      trace[trace_cap + k] = alpha;
-      trace[2 * trace_cap + k] = st->beta;
+    if (k < f.trace_cap) {
+      const auto tr = gp(f.trace);
+      tr[k] = st->e;
+      tr[f.trace_cap + k] = alpha;
+      tr[2 * f.trace_cap + k] = st->beta;
     }
     const int it = k + 1;
     st->it = it;
@@ -465,16 +539,22 @@ __global__ __launch_bounds__(NT) void k_scalar_step(const double* partials, int 
 
 // line-search update: psi += alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
 // (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta.
-__global__ __launch_bounds__(NT) void k_update(double* psi, double* const* dbuf, double* g0, const double* a,
-                                              const double* b, double* r, double* jd, int n, const SolveState* st) {
+__global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Fields& f = jobs[L.p].f;
+  const auto st = gp((const SolveState*)f.state);
   if (!st->update) return;
+  const int n = f.n;
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
   const int k = st->it - 1;
-  const double* d = dbuf[(k + 1) & 1];
+  const auto d = gp(f.d[(k + 1) & 1]);
+  const auto g0 = gp(f.g0);
+  const auto psi = gp(f.psi), a = gp(f.a), b = gp(f.b), r = gp(f.r), jd = gp(f.jd);
   const double alpha = st->alpha;
-  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
-  load_halo(s, n, i0, j0, [&](long f) { return d[f]; });
+  int i0, j0;
+  tile_origin(L.t, n, i0, j0);
+  load_halo(s, n, i0, j0, [&](long fc) { return d[fc]; });
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
@@ -496,14 +576,55 @@ __global__ __launch_bounds__(NT) void k_update(double* psi, double* const* dbuf,
     const int fa = t / TJ, fb = t % TJ;
     const int fi = i0 + fa, fj = j0 + fb;
     if (fi < n && fj < n) {
-      const long f = long(fi) * n + fj;
-      psi[f] = psi[f] + alpha * s[(fa + 1) * LDF + fb + 1];
-      const double rk = r[f];
-      const double rn = a[f] * (alpha * alpha) + b[f] * alpha + rk;
-      r[f] = rn;
-      jd[f] = rn - rk;
+      const long fc = long(fi) * n + fj;
+      psi[fc] = psi[fc] + alpha * s[(fa + 1) * LDF + fb + 1];
+      const double rk = r[fc];
+      const double rn = a[fc] * (alpha * alpha) + b[fc] * alpha + rk;
+      r[fc] = rn;
+      jd[fc] = rn - rk;
     }
   }
+}
+
+// forward positions = origin point double(i)*(1/n) + vertex gradient of psi
+// (otsolver_2dgrid.cpp:223-227)
+__global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Job& J = jobs[L.p];
+  const int n = J.f.n;
+  const auto psi = gp(J.f.psi);
+  const auto G = gp(J.fwd_out);
+  __shared__ double gx[NV], gy[NV];
+  __shared__ double s[NF];
+  int i0, j0;
+  tile_origin(L.t, n, i0, j0);
+  load_halo(s, n, i0, j0, [&](long fc) { return psi[fc]; });
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+  const long V1 = n + 1;
+  const long V = V1 * V1;
+  const double dx = 1. / double(n);
+  for (int t = threadIdx.x; t < NV; t += NT) {
+    const int va = t / LDV, vb = t - va * LDV;
+    const int vi = i0 + va, vj = j0 + vb;
+    const bool own_i = (va < TI) || (vi == n);
+    const bool own_j = (vb < TJ) || (vj == n);
+    if (own_i && own_j && vi <= n && vj <= n) {
+      const long v = vi * V1 + vj;
+      G[v] = double(vi) * dx + gx[t];
+      G[V + v] = double(vj) * dx + gy[t];
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_copy_adjusted(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Job& J = jobs[L.p];
+  if (!J.adj_out) return;
+  const auto out = gp(J.adj_out);
+  const auto rho = gp(J.f.rho);
+  for (long i = long(L.t) * NT + threadIdx.x; i < J.f.F; i += long(L.nb) * NT) out[i] = rho[i];
 }
 
 // vertex gradients of a face field into a (2V SoA) array; add_origin: forward
@@ -609,8 +730,6 @@ inline dim3 tile_grid(int n) { return dim3(tiles_j(n), tiles_i(n)); }
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int dnorm_blocks(int n, bool) { return ((n + 63) / 64) * ((n + 63) / 64); }
-
 void launch_fill(double* p, long count, double v, hipStream_t s) {
   if (count <= 0) return;
   const long blocks = std::min<long>((count + NT - 1) / NT, 4096);
@@ -618,61 +737,98 @@ void launch_fill(double* p, long count, double v, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
-void launch_adjust_density(const double* in, const Fields& f, double max_ratio, hipStream_t s) {
-  const long blocks = std::min<long>((f.F + NT * 4 - 1) / (NT * 4), 1024);
-  const double A = 1.0 / (double(f.n) * double(f.n));
-  double* part = f.site(SITE_ADJUST);
-  hipLaunchKernelGGL(k_adjust_reduce, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.F, part);
+namespace {
+#define JOB_PASS(kind) (const Job*)js.jobs, (const int*)js.starts_of(kind), js.count
+inline dim3 grid_of(const JobSet& js, int kind) { return dim3(unsigned(js.total[kind])); }
+}  // namespace
+
+// density adjustment, field initialisation, initial residual and solver state
+void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s) {
+  hipLaunchKernelGGL(k_adjust_reduce, grid_of(js, KIND_FLAT), dim3(NT), 0, s, JOB_PASS(KIND_FLAT));
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_adjust_scalar, dim3(1), dim3(NT), 0, s, (const double*)part, int(blocks), A, max_ratio,
-                     f.state);
+  hipLaunchKernelGGL(k_adjust_scalar, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_FLAT), max_ratio);
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_adjust_apply, dim3(unsigned(blocks)), dim3(NT), 0, s, in, f.rho, f.F,
-                     (const SolveState*)f.state);
+  hipLaunchKernelGGL(k_adjust_apply, grid_of(js, KIND_FLAT), dim3(NT), 0, s, JOB_PASS(KIND_FLAT));
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_init_fields, grid_of(js, KIND_FLAT), dim3(NT), 0, s, JOB_PASS(KIND_FLAT));
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_initial_residual, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
+  OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scalar_init, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL), max_iter, threshold, cj);
   OTM_HIP(hipGetLastError());
 }
 
-void launch_initial_residual(const Fields& f, int max_iter, double threshold, int cj, hipStream_t s) {
-  double* part = f.site(SITE_RESID);
-  hipLaunchKernelGGL(k_initial_residual, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.rho, f.r, f.n, part);
+// dkind: launch kind whose workgroups wrote the |d_hat|^2 partials
+void launch_beta(const JobSet& js, int dkind, hipStream_t s) {
+  hipLaunchKernelGGL(k_beta, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(NT), 0, s, (const double*)part, num_tiles(f.n), f.n, f.state,
-                     max_iter, threshold, cj);
-  OTM_HIP(hipGetLastError());
-}
-
-void launch_beta(const Fields& f, hipStream_t s) {
-  double* part = f.site(SITE_BETA);
-  hipLaunchKernelGGL(k_beta, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.psi, (const double*)f.dhat,
-                     (const double*)f.rho, (const double*)f.r, (const double*)f.jd, f.n, part,
-                     (const SolveState*)f.state);
-  OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_beta, dim3(1), dim3(NT), 0, s, (const double*)f.site(SITE_DNORM),
-                     dnorm_blocks(f.n, false), (const double*)part, num_tiles(f.n), f.state);
+  hipLaunchKernelGGL(k_scalar_beta, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
+                     (const int*)js.starts_of(dkind));
   OTM_HIP(hipGetLastError());
 }
 
-// The direction / update kernels take the d ping-pong pair through a small
-// device array so that one captured iteration serves every iteration.
-void launch_dir_coef(const Fields& f, hipStream_t s) {
-  double* part = f.site(SITE_COEF);
-  hipLaunchKernelGGL(k_dir_coef, tile_grid(f.n), dim3(NT), 0, s, (const double*)f.dhat, (double* const*)f.dpair,
-                     (const double*)f.g0, (const double*)f.r, f.a, f.b, f.n, part, (const SolveState*)f.state);
+void launch_dir_coef(const JobSet& js, hipStream_t s) {
+  hipLaunchKernelGGL(k_dir_coef, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_step, dim3(1), dim3(NT), 0, s, (const double*)part, num_tiles(f.n), f.n, f.state,
-                     f.trace, f.trace_cap);
+  hipLaunchKernelGGL(k_scalar_step, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
 }
 
-void launch_update(const Fields& f, hipStream_t s) {
-  hipLaunchKernelGGL(k_update, tile_grid(f.n), dim3(NT), 0, s, f.psi, (double* const*)f.dpair, f.g0,
-                     (const double*)f.a, (const double*)f.b, f.r, f.jd, f.n, (const SolveState*)f.state);
+void launch_update(const JobSet& js, hipStream_t s) {
+  hipLaunchKernelGGL(k_update, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
 }
 
-void launch_forward_positions(const double* psi, double* xy, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_vertex_gradients, tile_grid(n), dim3(NT), 0, s, psi, xy, n, 1);
+void launch_finish(const JobSet& js, hipStream_t s) {
+  hipLaunchKernelGGL(k_forward, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_adjusted, grid_of(js, KIND_FLAT), dim3(NT), 0, s, JOB_PASS(KIND_FLAT));
+  OTM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// job tables
+// ---------------------------------------------------------------------------
+void jobset_create(JobSet& js, const std::vector<Job>& jobs) {
+  js.count = int(jobs.size());
+  js.host = jobs;
+  js.fold = true;
+  for (const Job& J : jobs) js.fold = js.fold && J.P != nullptr;
+  const int C1 = js.count + 1;
+  std::vector<int> st(size_t(KIND_COUNT) * C1, 0);
+  js.total.assign(KIND_COUNT, 0);
+  for (int kind = 0; kind < KIND_COUNT; ++kind) {
+    int acc = 0;
+    for (int p = 0; p < js.count; ++p) {
+      st[size_t(kind) * C1 + p] = acc;
+      const int n = jobs[p].f.n;
+      int w = 0;
+      if (kind == KIND_STENCIL) w = num_tiles(n);
+      else if (kind == KIND_FLAT) w = flat_blocks(jobs[p].f.F);
+      else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
+      else {
+        const int g = kind - KIND_GEMM0, stage = g / NFOLD, c = g % NFOLD;
+        w = fold_cfg_valid(n, c) ? fold_tiles(n, stage, c) : 0;
+      }
+      acc += w;
+    }
+    st[size_t(kind) * C1 + js.count] = acc;
+    js.total[kind] = acc;
+  }
+  OTM_HIP(hipMalloc(&js.jobs, js.count * sizeof(Job)));
+  OTM_HIP(hipMalloc(&js.starts, st.size() * sizeof(int)));
+  OTM_HIP(hipMemcpy(js.jobs, jobs.data(), js.count * sizeof(Job), hipMemcpyHostToDevice));
+  OTM_HIP(hipMemcpy(js.starts, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void jobset_upload(JobSet& js, hipStream_t s) {
+  OTM_HIP(hipMemcpyAsync(js.jobs, js.host.data(), js.count * sizeof(Job), hipMemcpyHostToDevice, s));
+}
+
+void jobset_destroy(JobSet& js) {
+  if (js.jobs) (void)hipFree(js.jobs);
+  if (js.starts) (void)hipFree(js.starts);
+  js = JobSet();
 }
 
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s) {
