@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--inverse-points", type=int, default=1 << 20)
+    ap.add_argument("--batch", type=int, default=64, help="densities of the batched workload (0: skip)")
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
                              "mixed_solve", "copy"],
@@ -129,6 +130,61 @@ def inverse_throughput(n, d_fwd, d_adj, count, reps):
             "bvh_build_ms": round(t_bvh * 1e3, 2), "mode": "inv (multi-hit selection), FP64"}
 
 
+def batched_throughput(count, steps, warmup):
+    """64 seeded Gaussian-mixture densities with grid sizes drawn from
+    {128, 192, 256, 384, 512, 768, 1024}, solved together by one BatchSolver
+    (device-resident inputs), against the same densities solved one after the
+    other by single-grid solvers."""
+    import torch
+
+    from otmap_amd import BatchSolver, GridBasedTransportSolver
+    from otmap_amd import densities as D
+
+    specs = D.batch_specs(count)
+    dens = D.batch_densities(count)
+    sizes = [n for n, _, _ in specs]
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(d).to(dev) for d in dens]
+    d_out = [torch.empty(2 * (n + 1) ** 2, dtype=torch.float64, device=dev) for n in sizes]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = BatchSolver(sizes)
+    t_create = time.perf_counter() - t0
+    ip = [t.data_ptr() for t in d_in]
+    op = [t.data_ptr() for t in d_out]
+    for _ in range(warmup):
+        b.solve_device(ip, op)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = b.solve_device(ip, op)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    iters = [s.iterations for s in st]
+
+    # the same densities through single-grid solvers, one handle per size
+    solvers = {}
+    for n in sorted(set(sizes)):
+        s = GridBasedTransportSolver(0)
+        s.set_verbose_level(0)
+        s.init(n)
+        solvers[n] = s
+    for k in range(len(sizes)):  # warm every handle once (graph capture)
+        solvers[sizes[k]].solve_device(ip[k], op[k])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(len(sizes)):
+        solvers[sizes[k]].solve_device(ip[k], op[k])
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    return {"densities": count, "faces": int(sum(n * n for n in sizes)), "sizes": sorted(set(sizes)),
+            "solves_per_s": round(count / t_batch, 2), "ms_per_batch": round(t_batch * 1e3, 2),
+            "sequential_single_solves_per_s": round(count / t_seq, 2),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "iterations": {"min": min(iters), "max": max(iters), "mean": round(float(np.mean(iters)), 1)},
+            "create_s": round(t_create, 3), "precision": "fp64", "gemm_tiles": b.gemm_config()[0]}
+
+
 def cpu_baseline(n, density):
     """The compiled reference (oracle/_ref/libotref.so), one solve, 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -198,6 +254,7 @@ def main():
     # inverse map at 1M uniform query points (device buffers): BVH build on the
     # host (once per map, as init_inverse), then one query per lane
     inv_stats = inverse_throughput(n, d_fwd, d_adj, args.inverse_points, args.steps)
+    batch_stats = batched_throughput(args.batch, args.steps, args.warmup) if args.batch > 0 else None
 
     # parity against the reference's golden positions when they exist for this workload
     err = None
@@ -287,6 +344,7 @@ def main():
         "init_s": round(t_init, 4),
         "max_vertex_err_vs_reference": err,
         "inverse_1M": inv_stats,
+        "batched": batch_stats,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "roofline": roof_x3 if head == "mixed" else roof_f64,
         "roofline_fp64_gemm": roof_f64,

@@ -63,6 +63,7 @@ typedef struct otm_stats {
 
 typedef struct otm_solver otm_solver;
 typedef struct otm_map otm_map;
+typedef struct otm_batch otm_batch;
 
 /* Fills *opt with the reference defaults (otlib/otsolver_2dgrid.h:37-43). */
 void otm_options_init(otm_options* opt);
@@ -115,7 +116,7 @@ enum {
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
 
 /* Tile shapes of the four FP64 DCT stages (no reference counterpart).  For
- * n % 128 == 0 the Laplacian solve runs the even/odd folded transform and
+ * n % 64 == 0 the Laplacian solve runs the even/odd folded transform and
  * otm_solver_init times every candidate tile shape of each stage on the
  * device, keeping the fastest (cached per device and n).  The shapes differ in
  * speed only: every shape accumulates in the same order and gives
@@ -180,6 +181,31 @@ int otm_rasterize(int device, int n, const double* mesh_xy, const double* densit
  * rows*cols int32 scratch (receives the winning face index, -1 if none). */
 int otm_rasterize_device(int device, int n, const double* d_mesh_xy, const double* d_density, int rows, int cols,
                          int img_f32, void* d_img, int32_t* d_winner);
+
+/* ---- Batched solves -----------------------------------------------------
+ * No reference counterpart: the reference solves one density per
+ * GridBasedTransportSolver (otlib/otsolver_2dgrid.cpp:106-245) and its apps
+ * loop over densities (apps/barycenters.cpp:167-197).  A batch owns `count`
+ * grids of sizes n[p] (multiples of 64) and solves all of them together: each
+ * step of a Newton iteration is one kernel launch for the whole batch, the
+ * iterations are captured in a HIP graph, and the DCT tile shapes are chosen
+ * by the autotuner for the batch's sizes.  Every grid gets exactly the result
+ * of a single-grid FP64 solve of its density (bit-identical); stats[p] holds
+ * grid p's iterations and residuals (ms_solve: the whole batch).
+ *   density / fwd_xy / adjusted_density: arrays of `count` pointers, each
+ *   sized for its grid as in otm_solve (adjusted_density and its entries
+ *   nullable).  Options apply to every grid; precision must be FP64. */
+int otm_batch_create(otm_batch** out, int device, int count, const int* n);
+int otm_batch_destroy(otm_batch* b);
+int otm_batch_size(const otm_batch* b);
+int otm_batch_solve(otm_batch* b, const double* const* density, const otm_options* opt, double* const* fwd_xy,
+                    double* const* adjusted_density, otm_stats* stats);
+int otm_batch_solve_device(otm_batch* b, const double* const* d_density, const otm_options* opt,
+                           double* const* d_fwd_xy, double* const* d_adjusted_density, otm_stats* stats);
+/* per-iteration residual / alpha / beta of grid `job` in the last solve */
+int otm_batch_trace(const otm_batch* b, int job, double* residual, double* alpha, double* beta, int cap);
+/* tile shape per folded DCT stage chosen for this batch; returns the number of candidates */
+int otm_batch_gemm_config(const otm_batch* b, int* cfg);
 
 #ifdef __cplusplus
 }

@@ -2,12 +2,14 @@
 
 Drop-in mirror of the reference's API (ggael/otmap):
   GridBasedTransportSolver, SolverOptions, BetaOpt      otlib/otsolver_2dgrid.h
-  TransportMap, apply_inverse_map, transport_cost        otlib/transport_map.h
+  TransportMap, apply_inverse_map                        otlib/transport_map.h
   rasterize_image                                        otlib/utils/rasterizer.h
-backed by hand-written HIP kernels in libotmap_amd.so (C ABI: include/otmap_capi.h).
+plus BatchSolver (many densities of uneven sizes solved together), backed by
+hand-written HIP kernels in libotmap_amd.so (C ABI: include/otmap_capi.h).
 """
+from .batch import BatchSolver
 from .solver import BetaOpt, GridBasedTransportSolver, Precision, SolverOptions, SolveStats
 from .transport_map import GridMesh, TransportMap, apply_inverse_map, rasterize_image
 
-__all__ = ["BetaOpt", "GridBasedTransportSolver", "Precision", "SolverOptions", "SolveStats", "GridMesh",
-           "TransportMap", "apply_inverse_map", "rasterize_image"]
+__all__ = ["BatchSolver", "BetaOpt", "GridBasedTransportSolver", "Precision", "SolverOptions", "SolveStats",
+           "GridMesh", "TransportMap", "apply_inverse_map", "rasterize_image"]

@@ -67,6 +67,15 @@ _SIGNATURES = {
     "otm_map_inverse_device": (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _vp, _vp, _vp]),
     "otm_rasterize": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]),
     "otm_rasterize_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "otm_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _ip]),
+    "otm_batch_destroy": (C.c_int, [_vp]),
+    "otm_batch_size": (C.c_int, [_vp]),
+    "otm_batch_solve": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(OtmOptions), C.POINTER(_dp), C.POINTER(_dp),
+                                  C.POINTER(OtmStats)]),
+    "otm_batch_solve_device": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(OtmOptions), C.POINTER(_vp),
+                                         C.POINTER(_vp), C.POINTER(OtmStats)]),
+    "otm_batch_trace": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
+    "otm_batch_gemm_config": (C.c_int, [_vp, _ip]),
 }
 
 _lib = None
@@ -90,6 +99,7 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise OtmError(f"{LIB_PATH} is missing: build it with `python -m otmap_amd.build` "
                        "(or __graft_entry__.build()); there is no CPU fallback")
+    _init_torch_runtime_first()
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(L, name)
@@ -97,6 +107,18 @@ def lib():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def _init_torch_runtime_first() -> None:
+    """PyTorch-ROCm wheels bundle their own HIP/HSA runtime, so a process using
+    both torch and this library holds two runtimes; torch's can only start if
+    it starts first.  When torch is already imported, start it before this
+    library touches the device."""
+    import sys
+
+    torch = sys.modules.get("torch")
+    if torch is not None and getattr(torch, "cuda", None) is not None and torch.cuda.device_count() > 0:
+        torch.cuda.init()
 
 
 def check(rc: int, what: str) -> None:

@@ -28,6 +28,8 @@ SOURCES = [
     "inverse.hip",
     "raster.hip",
     "dct_mixed.hip",
+    "driver.hip",
+    "batch.hip",
 ]
 
 FLAGS = [

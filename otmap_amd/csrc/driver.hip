@@ -1,0 +1,174 @@
+// Per-grid buffers and the iteration driver shared by the single-grid solver
+// handle and the batched handle.
+//
+// A solve of a job table is: density adjustment, field initialisation and the
+// initial residual (otsolver_2dgrid.cpp:106-160), then Newton iterations until
+// every job's device state says done (:160-221), then the forward positions
+// (:223-227).  Iterations are captured once into a HIP graph of ITERS_PER_GRAPH
+// iterations and replayed; after each batch of replays one small kernel counts
+// the jobs still running and the host reads that count.  Iterations replayed
+// after a job has stopped are no-ops for that job (every kernel checks its
+// gate), so results do not depend on how many replays are in flight.
+
+#include <cfloat>
+#include <cstring>
+
+#include "otm_internal.h"
+
+namespace otm {
+
+namespace {
+
+template <class T>
+void dalloc(T*& p, size_t count) {
+  OTM_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+long partial_slots(int n) {
+  const long gemm_tiles = long((n + 63) / 64) * ((n + 63) / 64);
+  return std::max<long>(std::max<long>(num_tiles(n), gemm_tiles), 3 * 1024);
+}
+
+constexpr int ITERS_PER_GRAPH = 4;
+
+}  // namespace
+
+void fields_create(Fields& f, int n, int trace_cap) {
+  f = Fields();
+  f.n = n;
+  f.F = long(n) * n;
+  f.V = long(n + 1) * (n + 1);
+  const size_t F = f.F, V = f.V;
+  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 2 * V); dalloc(f.r, F); dalloc(f.jd, F);
+  dalloc(f.dhat, F); dalloc(f.d[0], F); dalloc(f.d[1], F); dalloc(f.dpair, 2);
+  f.pslots = partial_slots(n);
+  dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, size_t(SITE_COUNT) * SITE_VALUES * size_t(f.pslots));
+  dalloc(f.tmp0, F); dalloc(f.tmp1, F); dalloc(f.state, 1);
+  OTM_HIP(hipMemcpy(f.dpair, f.d, 2 * sizeof(double*), hipMemcpyHostToDevice));
+  OTM_HIP(hipMemset(f.state, 0, sizeof(SolveState)));
+  dalloc(f.trace, 3 * size_t(trace_cap));
+  f.trace_cap = trace_cap;
+}
+
+void fields_destroy(Fields& f) {
+  dfree(f.rho); dfree(f.psi); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
+  dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
+  dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.trace);
+  f = Fields();
+}
+
+bool fields_grow_trace(Fields& f, int cap) {
+  if (cap <= f.trace_cap) return false;
+  dfree(f.trace);
+  dalloc(f.trace, 3 * size_t(cap));
+  f.trace_cap = cap;
+  return true;
+}
+
+void driver_create(Driver& d) {
+  OTM_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+  OTM_HIP(hipEventCreate(&d.ev0));
+  OTM_HIP(hipEventCreate(&d.ev1));
+  OTM_HIP(hipMalloc(&d.d_open, sizeof(int)));
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), sizeof(int), hipHostMallocDefault));
+}
+
+void driver_invalidate(Driver& d) {
+  if (d.exec) (void)hipGraphExecDestroy(d.exec);
+  if (d.graph) (void)hipGraphDestroy(d.graph);
+  d.exec = nullptr;
+  d.graph = nullptr;
+  d.key = -1;
+}
+
+void driver_destroy(Driver& d) {
+  if (d.stream) (void)hipStreamSynchronize(d.stream);
+  driver_invalidate(d);
+  if (d.d_open) (void)hipFree(d.d_open);
+  if (d.h_open) (void)hipHostFree(d.h_open);
+  if (d.ev0) (void)hipEventDestroy(d.ev0);
+  if (d.ev1) (void)hipEventDestroy(d.ev1);
+  if (d.stream) (void)hipStreamDestroy(d.stream);
+  d = Driver();
+}
+
+// launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
+int dnorm_kind(const JobSet& js, bool mixed) { return (mixed || !js.fold) ? KIND_PLAIN64 : fold_kind(3, js.cfg[3]); }
+
+void enqueue_iteration(const JobSet& js, const DctPlan* plan, const MixedPlan* mplan, const otm_options& opt,
+                       hipStream_t s) {
+  const bool mixed = opt.precision == OTM_PRECISION_MIXED;
+  if (mixed) {
+    const Fields& f = js.host[0].f;
+    launch_mixed_solve(*mplan, *plan, f, f.r, f.dhat, opt.refine_steps, s);
+  } else {
+    launch_dct_solve(js, plan, s);
+  }
+  launch_beta(js, dnorm_kind(js, mixed), s);
+  launch_dir_coef(js, s);
+  launch_update(js, s);
+}
+
+int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const MixedPlan* mplan, const otm_options& opt,
+                 float* ms) {
+  hipStream_t s = d.stream;
+  const bool mixed = opt.precision == OTM_PRECISION_MIXED;
+  if (mixed && js.count != 1) {
+    set_error("mixed precision runs on single-grid handles");
+    return OTM_ERR_ARG;
+  }
+  // the captured iteration depends on the precision mode and the tile shapes
+  const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (long(js.cfg[0]) << 8) |
+                   (long(js.cfg[1]) << 12) | (long(js.cfg[2]) << 16) | (long(js.cfg[3]) << 20);
+  if (d.exec && d.key != key) driver_invalidate(d);
+  jobset_upload(js, s);
+  OTM_HIP(hipEventRecord(d.ev0, s));
+  launch_start(js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, s);
+  if (!d.exec) {
+    OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < ITERS_PER_GRAPH; ++k) enqueue_iteration(js, plan, mplan, opt, s);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    OTM_HIP(hipStreamEndCapture(s, &d.graph));
+    OTM_HIP(hipGraphInstantiate(&d.exec, d.graph, nullptr, nullptr, 0));
+    d.key = key;
+  }
+  // replays: 1, 2, 4 graphs between checks (4, 8, 16 iterations)
+  long launched = 0;
+  int chunk = 1;
+  for (;;) {
+    for (int c = 0; c < chunk && launched <= long(opt.max_iter); ++c) {
+      OTM_HIP(hipGraphLaunch(d.exec, s));
+      launched += ITERS_PER_GRAPH;
+    }
+    launch_count_open(js, d.d_open, s);
+    OTM_HIP(hipMemcpyAsync(d.h_open, d.d_open, sizeof(int), hipMemcpyDeviceToHost, s));
+    OTM_HIP(hipStreamSynchronize(s));
+    if (*d.h_open == 0) break;
+    if (launched > long(opt.max_iter)) {
+      set_error("internal error: solve did not reach its stop condition");
+      return OTM_ERR_STATE;
+    }
+    chunk = std::min(chunk * 2, 4);
+  }
+  launch_finish(js, s);
+  OTM_HIP(hipEventRecord(d.ev1, s));
+  OTM_HIP(hipStreamSynchronize(s));
+  float t = 0.f;
+  OTM_HIP(hipEventElapsedTime(&t, d.ev0, d.ev1));
+  if (ms) *ms = t;
+  return OTM_OK;
+}
+
+}  // namespace otm

@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/otmap_capi.h"
+
 namespace otm {
 
 // ---------------------------------------------------------------------------
@@ -172,6 +174,7 @@ void launch_beta(const JobSet& js, int dkind, hipStream_t s);
 void launch_dir_coef(const JobSet& js, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
 void launch_update(const JobSet& js, hipStream_t s);
 void launch_finish(const JobSet& js, hipStream_t s);     // forward positions, adjusted density copy
+void launch_count_open(const JobSet& js, int* d_open, hipStream_t s);  // jobs whose state is not done
 void launch_fill(double* p, long count, double v, hipStream_t s);
 // component kernels (single evaluations)
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s);
@@ -218,5 +221,28 @@ void mixed_plan_destroy(MixedPlan& mp);
 void launch_mixed_solve(const MixedPlan& mp, const DctPlan& p, const Fields& f, const double* r, double* out,
                         int refine, hipStream_t s);
 void launch_mixed_gemm_stage(const MixedPlan& mp, const DctPlan& p, hipStream_t s);
+
+// Per-grid buffers and the iteration driver (driver.hip)
+void fields_create(Fields& f, int n, int trace_cap);
+void fields_destroy(Fields& f);
+bool fields_grow_trace(Fields& f, int cap);  // true when the buffer was reallocated
+
+struct Driver {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;     // captured iterations
+  hipGraphExec_t exec = nullptr;
+  long key = -1;                  // precision mode and tile shapes of the captured graph
+  int* d_open = nullptr;          // jobs still running (device)
+  int* h_open = nullptr;          // pinned copy
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+void driver_create(Driver& d);
+void driver_destroy(Driver& d);
+void driver_invalidate(Driver& d);  // drop the captured graph (tables or shapes changed)
+int dnorm_kind(const JobSet& js, bool mixed);
+// one solve of every job of the table (js.host[p].dens_in / fwd_out / adj_out
+// set by the caller); returns an OTM_* code, *ms = device time
+int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const MixedPlan* mplan, const otm_options& opt,
+                 float* ms);
 
 }  // namespace otm

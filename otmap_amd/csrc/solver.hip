@@ -33,18 +33,16 @@ using namespace otm;
 
 struct otm_solver {
   int device = 0;
-  hipStream_t stream = nullptr;
   int verbose = 1;
   int n = 0;
-  Fields f;
+  Fields f;                    // the grid's buffers (also js.host[0].f)
   JobSet js;                   // the one-job table every kernel of a solve runs over
+  Driver drv;                  // stream, captured iterations, events
   DctPlan plan;
   MixedPlan mplan;             // created on the first mixed-precision solve
   double* d_in = nullptr;      // staging of host-API inputs (n*n)
   double* d_xy = nullptr;      // staging of host-API vertex outputs (2V)
-  double* h_sum = nullptr;     // pinned: host copy of partials
   SolveState* h_state = nullptr;  // pinned
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int last_iters = 0;
 };
 
@@ -62,56 +60,27 @@ void dalloc(T*& p, size_t count) {
 }
 
 void release_fields(otm_solver* s) {
-  Fields& f = s->f;
-  dfree(f.rho); dfree(f.psi); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
-  dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
-  dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.trace);
+  driver_invalidate(s->drv);
+  fields_destroy(s->f);
   dfree(s->d_in); dfree(s->d_xy);
   jobset_destroy(s->js);
   mixed_plan_destroy(s->mplan);
   dct_plan_destroy(s->plan);
-  f = Fields();
   s->n = 0;
 }
 
-long partial_slots(int n) {
-  const long gemm_tiles = long((n + 63) / 64) * ((n + 63) / 64);
-  return std::max<long>(std::max<long>(num_tiles(n), gemm_tiles), 3 * 1024);
-}
-
-// grows the trace buffer; true when it was reallocated
-bool ensure_trace(otm_solver* s, int cap) {
-  Fields& f = s->f;
-  if (cap <= f.trace_cap) return false;
-  dfree(f.trace);
-  dalloc(f.trace, 3 * size_t(cap));
-  f.trace_cap = cap;
-  return true;
-}
-
 void allocate(otm_solver* s, int n) {
-  Fields& f = s->f;
-  f.n = n;
-  f.F = long(n) * n;
-  f.V = long(n + 1) * (n + 1);
-  const size_t F = f.F, V = f.V;
-  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 2 * V); dalloc(f.r, F); dalloc(f.jd, F);
-  dalloc(f.dhat, F); dalloc(f.d[0], F); dalloc(f.d[1], F); dalloc(f.dpair, 2);
-  f.pslots = partial_slots(n);
-  dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, size_t(SITE_COUNT) * SITE_VALUES * size_t(f.pslots));
-  dalloc(f.tmp0, F); dalloc(f.tmp1, F); dalloc(f.state, 1);
-  dalloc(s->d_in, F); dalloc(s->d_xy, 2 * V);
-  OTM_HIP(hipMemcpy(f.dpair, f.d, 2 * sizeof(double*), hipMemcpyHostToDevice));
-  OTM_HIP(hipMemset(f.state, 0, sizeof(SolveState)));
-  ensure_trace(s, 1024);
+  fields_create(s->f, n, 1024);
+  dalloc(s->d_in, s->f.F);
+  dalloc(s->d_xy, 2 * s->f.V);
   dct_plan_create(s->plan, n);
   Job J;
-  J.f = f;
+  J.f = s->f;
   J.P = s->plan.P;
   J.Pt = s->plan.Pt;
   J.sq = s->plan.sq;
   jobset_create(s->js, {J});
-  dct_tune(s->js, s->device, s->stream);
+  dct_tune(s->js, s->device, s->drv.stream);
   s->n = n;
 }
 
@@ -137,21 +106,6 @@ int check_ready(otm_solver* s) {
   return OTM_OK;
 }
 
-// launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
-int dnorm_kind(const JobSet& js, bool mixed) { return (mixed || !js.fold) ? KIND_PLAIN64 : fold_kind(3, js.cfg[3]); }
-
-void enqueue_iteration(otm_solver* s, const JobSet& js, const otm_options& opt) {
-  const bool mixed = opt.precision == OTM_PRECISION_MIXED;
-  if (mixed)
-    launch_mixed_solve(s->mplan, s->plan, js.host[0].f, js.host[0].f.r, js.host[0].f.dhat, opt.refine_steps,
-                       s->stream);
-  else
-    launch_dct_solve(js, &s->plan, s->stream);
-  launch_beta(js, dnorm_kind(js, mixed), s->stream);
-  launch_dir_coef(js, s->stream);
-  launch_update(js, s->stream);
-}
-
 int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_in, double* d_fwd, double* d_adj,
                  otm_stats* stats) {
   otm_options opt;
@@ -173,37 +127,15 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
     }
     if (!s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
   }
-  Fields& f = s->f;
-  hipStream_t st = s->stream;
-  if (ensure_trace(s, std::max(opt.max_iter, 1))) s->js.host[0].f = f;
+  if (fields_grow_trace(s->f, std::max(opt.max_iter, 1))) driver_invalidate(s->drv);
   Job& J = s->js.host[0];
+  J.f = s->f;
   J.dens_in = d_density;
   J.fwd_out = d_fwd;
   J.adj_out = d_adj;
-  jobset_upload(s->js, st);
-  OTM_HIP(hipEventRecord(s->ev0, st));
-  launch_start(s->js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0,
-               st);
-  // Iterations are enqueued in chunks; the stop flag is read back once per
-  // chunk.  Iterations enqueued after the stop are no-ops on the device.
-  long enqueued = 0;
-  int chunk = 4;
-  for (;;) {
-    for (int c = 0; c < chunk && enqueued <= long(opt.max_iter); ++c, ++enqueued) enqueue_iteration(s, s->js, opt);
-    OTM_HIP(hipMemcpyAsync(s->h_state, f.state, sizeof(SolveState), hipMemcpyDeviceToHost, st));
-    OTM_HIP(hipStreamSynchronize(st));
-    if (s->h_state->done) break;
-    if (enqueued > long(opt.max_iter)) {
-      set_error("internal error: solve did not reach its stop condition");
-      return OTM_ERR_STATE;
-    }
-    chunk = std::min(chunk * 2, 16);
-  }
-  launch_finish(s->js, st);
-  OTM_HIP(hipEventRecord(s->ev1, st));
-  OTM_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
-  OTM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  if (int rc = driver_solve(s->drv, s->js, &s->plan, &s->mplan, opt, &ms)) return rc;
+  OTM_HIP(hipMemcpy(s->h_state, s->f.state, sizeof(SolveState), hipMemcpyDeviceToHost));
   const SolveState& hs = *s->h_state;
   s->last_iters = hs.it;
   if (stats) {
@@ -242,11 +174,8 @@ int otm_solver_create(otm_solver** out, int device) {
   OTM_HIP(hipSetDevice(device));
   auto* s = new otm_solver;
   s->device = device;
-  OTM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  driver_create(s->drv);
   OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_state), sizeof(SolveState), hipHostMallocDefault));
-  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_sum), 8 * 4096 * sizeof(double), hipHostMallocDefault));
-  OTM_HIP(hipEventCreate(&s->ev0));
-  OTM_HIP(hipEventCreate(&s->ev1));
   *out = s;
   return OTM_OK;
   OTM_GUARD_END
@@ -256,13 +185,10 @@ int otm_solver_destroy(otm_solver* s) {
   if (!s) return OTM_OK;
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
-  if (s->stream) OTM_HIP(hipStreamSynchronize(s->stream));
+  if (s->drv.stream) OTM_HIP(hipStreamSynchronize(s->drv.stream));
   release_fields(s);
   if (s->h_state) (void)hipHostFree(s->h_state);
-  if (s->h_sum) (void)hipHostFree(s->h_sum);
-  if (s->ev0) (void)hipEventDestroy(s->ev0);
-  if (s->ev1) (void)hipEventDestroy(s->ev1);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  driver_destroy(s->drv);
   delete s;
   return OTM_OK;
   OTM_GUARD_END
@@ -280,7 +206,7 @@ int otm_solver_init(otm_solver* s, int n) {
   if (s->n == n) return OTM_OK;  // already set up (otsolver_2dgrid.cpp:79-83)
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   release_fields(s);
   allocate(s, n);
   return OTM_OK;
@@ -325,13 +251,13 @@ int otm_solve(otm_solver* s, const double* density, const otm_options* opt, doub
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
   const Fields& f = s->f;
-  OTM_HIP(hipMemcpyAsync(s->d_in, density, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  OTM_HIP(hipMemcpyAsync(s->d_in, density, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
   int rc = solve_device(s, s->d_in, opt, s->d_xy, nullptr, stats);
   if (rc) return rc;
-  OTM_HIP(hipMemcpyAsync(fwd_xy, s->d_xy, 2 * f.V * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipMemcpyAsync(fwd_xy, s->d_xy, 2 * f.V * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
   if (adjusted_density)
-    OTM_HIP(hipMemcpyAsync(adjusted_density, f.rho, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+    OTM_HIP(hipMemcpyAsync(adjusted_density, f.rho, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;
   OTM_GUARD_END
 }
@@ -388,22 +314,22 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   // overlapped, as rocprof reports it.  BETA and DIR_COEF include their
   // single-workgroup scalar kernel.
   if (kernel == OTM_KERNEL_DCT_SOLVE || kernel == OTM_KERNEL_MIXED_SOLVE) tpl.update = 0;
-  OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->stream));
+  OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->drv.stream));
   auto launch = [&]() {
     switch (kernel) {
-      case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(pj, &s->plan, s->stream); break;
-      case OTM_KERNEL_BETA: launch_beta(pj, dnorm_kind(pj, false), s->stream); break;
-      case OTM_KERNEL_DIR_COEF: launch_dir_coef(pj, s->stream); break;
-      case OTM_KERNEL_UPDATE: launch_update(pj, s->stream); break;
-      case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(pj, &s->plan, s->stream); break;
-      case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(s->mplan, s->plan, s->stream); break;
-      default: launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, 1, s->stream); break;
+      case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(pj, &s->plan, s->drv.stream); break;
+      case OTM_KERNEL_BETA: launch_beta(pj, dnorm_kind(pj, false), s->drv.stream); break;
+      case OTM_KERNEL_DIR_COEF: launch_dir_coef(pj, s->drv.stream); break;
+      case OTM_KERNEL_UPDATE: launch_update(pj, s->drv.stream); break;
+      case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(pj, &s->plan, s->drv.stream); break;
+      case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(s->mplan, s->plan, s->drv.stream); break;
+      default: launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, 1, s->drv.stream); break;
     }
   };
   launch();  // warm-up
-  OTM_HIP(hipEventRecord(a, s->stream));
+  OTM_HIP(hipEventRecord(a, s->drv.stream));
   for (int k = 0; k < reps; ++k) launch();
-  OTM_HIP(hipEventRecord(b, s->stream));
+  OTM_HIP(hipEventRecord(b, s->drv.stream));
   OTM_HIP(hipEventSynchronize(b));
   float ms = 0.f;
   OTM_HIP(hipEventElapsedTime(&ms, a, b));
@@ -422,10 +348,10 @@ int otm_vertex_gradients(otm_solver* s, const double* psi, double* grads) {
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
   const Fields& f = s->f;
-  OTM_HIP(hipMemcpyAsync(s->d_in, psi, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  launch_vertex_gradients(s->d_in, s->d_xy, s->n, s->stream);
-  OTM_HIP(hipMemcpyAsync(grads, s->d_xy, 2 * f.V * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+  OTM_HIP(hipMemcpyAsync(s->d_in, psi, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  launch_vertex_gradients(s->d_in, s->d_xy, s->n, s->drv.stream);
+  OTM_HIP(hipMemcpyAsync(grads, s->d_xy, 2 * f.V * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;
   OTM_GUARD_END
 }
@@ -437,13 +363,13 @@ int otm_residual(otm_solver* s, const double* adjusted_density, const double* ps
   OTM_HIP(hipSetDevice(s->device));
   const Fields& f = s->f;
   const int nb = num_tiles(s->n);
-  OTM_HIP(hipMemcpyAsync(f.rho, adjusted_density, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  OTM_HIP(hipMemcpyAsync(f.psi, psi, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  launch_residual_only(f.psi, f.rho, f.r, f.partials, s->n, s->stream);
-  OTM_HIP(hipMemcpyAsync(r, f.r, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  OTM_HIP(hipMemcpyAsync(f.rho, adjusted_density, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  OTM_HIP(hipMemcpyAsync(f.psi, psi, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  launch_residual_only(f.psi, f.rho, f.r, f.partials, s->n, s->drv.stream);
+  OTM_HIP(hipMemcpyAsync(r, f.r, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
   std::vector<double> part(nb);
-  OTM_HIP(hipMemcpyAsync(part.data(), f.partials, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+  OTM_HIP(hipMemcpyAsync(part.data(), f.partials, nb * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   double acc = 0.;
   for (double p : part) acc += p;
   if (e) *e = acc / (1.0 / (double(s->n) * double(s->n)));
@@ -462,11 +388,11 @@ int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat) {
   SolveState st;
   std::memset(&st, 0, sizeof(st));
   st.sum_r = sum;
-  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->stream));
-  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  launch_dct_solve(s->js, &s->plan, s->stream);
-  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->drv.stream));
+  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  launch_dct_solve(s->js, &s->plan, s->drv.stream);
+  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;
   OTM_GUARD_END
 }
@@ -485,11 +411,11 @@ int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int
   SolveState st;
   std::memset(&st, 0, sizeof(st));
   st.sum_r = sum;
-  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->stream));
-  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, refine_steps, s->stream);
-  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->drv.stream));
+  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, refine_steps, s->drv.stream);
+  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;
   OTM_GUARD_END
 }
@@ -500,12 +426,12 @@ int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* 
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
   const Fields& f = s->f;
-  OTM_HIP(hipMemcpyAsync(f.g0, g0, 2 * f.V * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  OTM_HIP(hipMemcpyAsync(f.dhat, dir, f.F * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  launch_params_1d(f.g0, f.dhat, f.a, f.b, s->n, s->stream);
-  OTM_HIP(hipMemcpyAsync(a, f.a, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipMemcpyAsync(b, f.b, f.F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  OTM_HIP(hipStreamSynchronize(s->stream));
+  OTM_HIP(hipMemcpyAsync(f.g0, g0, 2 * f.V * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  OTM_HIP(hipMemcpyAsync(f.dhat, dir, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  launch_params_1d(f.g0, f.dhat, f.a, f.b, s->n, s->drv.stream);
+  OTM_HIP(hipMemcpyAsync(a, f.a, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipMemcpyAsync(b, f.b, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;
   OTM_GUARD_END
 }

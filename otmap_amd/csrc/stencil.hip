@@ -618,6 +618,18 @@ __global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
   }
 }
 
+// number of jobs whose state is not done (one workgroup)
+__global__ __launch_bounds__(NT) void k_count_open(const Job* __restrict__ jobs, int count, int* out) {
+  __shared__ int total;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  int c = 0;
+  for (int p = threadIdx.x; p < count; p += NT) c += gp((const SolveState*)jobs[p].f.state)->done ? 0 : 1;
+  if (c) atomicAdd(&total, c);
+  __syncthreads();
+  if (threadIdx.x == 0) gp(out)[0] = total;
+}
+
 __global__ __launch_bounds__(NT) void k_copy_adjusted(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Job& J = jobs[L.p];
@@ -783,6 +795,11 @@ void launch_finish(const JobSet& js, hipStream_t s) {
   hipLaunchKernelGGL(k_forward, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_copy_adjusted, grid_of(js, KIND_FLAT), dim3(NT), 0, s, JOB_PASS(KIND_FLAT));
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_count_open(const JobSet& js, int* d_open, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_open, dim3(1), dim3(NT), 0, s, (const Job*)js.jobs, js.count, d_open);
   OTM_HIP(hipGetLastError());
 }
 

@@ -152,7 +152,7 @@ class GridBasedTransportSolver:
 
     def gemm_config(self):
         """(tile shape per folded DCT stage, number of candidate shapes); the
-        count is 0 when the grid size does not fold (n % 128 != 0)."""
+        count is 0 when the grid size does not fold (n % 64 != 0)."""
         cfg = np.zeros(4, dtype=np.int32)
         count = _lib.lib().otm_solver_gemm_config(self._h, iptr(cfg))
         if count < 0:

@@ -34,6 +34,11 @@ def solver_factory():
     return make
 
 
-def pytest_collection_modifyitems(config, items):
-    # unmarked tests run on the CPU; gpu tests carry the marker explicitly
-    pass
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """Some GPU tests hand torch device buffers to the library.  PyTorch-ROCm
+    bundles its own HIP runtime, which only starts if it starts before the
+    library's; import torch first in sessions that run GPU tests."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch  # noqa: F401  (otmap_amd._lib starts its runtime before loading the library)
+    yield
