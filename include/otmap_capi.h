@@ -190,11 +190,12 @@ int otm_rasterize_device(int device, int n, const double* d_mesh_xy, const doubl
  * step of a Newton iteration is one kernel launch for the whole batch, the
  * iterations are captured in a HIP graph, and the DCT tile shapes are chosen
  * by the autotuner for the batch's sizes.  Every grid gets exactly the result
- * of a single-grid FP64 solve of its density (bit-identical); stats[p] holds
+ * of a single-grid solve of its density (bit-identical); stats[p] holds
  * grid p's iterations and residuals (ms_solve: the whole batch).
  *   density / fwd_xy / adjusted_density: arrays of `count` pointers, each
  *   sized for its grid as in otm_solve (adjusted_density and its entries
- *   nullable).  Options apply to every grid; precision must be FP64. */
+ *   nullable).  Options apply to every grid, both precision modes included
+ *   (a mixed batch gives each grid the result of its single mixed solve). */
 int otm_batch_create(otm_batch** out, int device, int count, const int* n);
 int otm_batch_destroy(otm_batch* b);
 int otm_batch_size(const otm_batch* b);

@@ -6,7 +6,9 @@
 // shapes are picked by the autotuner for the batch's set of sizes.  Each job
 // runs exactly the single-grid computation (same kernels, same operation
 // order), so a batched result is bit-identical to the same density solved
-// alone; jobs that converge early skip the remaining iterations.
+// alone; jobs that converge early skip the remaining iterations.  Both
+// precision modes run batched (mixed: the folded BF16x3 transform with FP64
+// refinement for every grid).
 
 #include <cstring>
 #include <map>
@@ -21,7 +23,8 @@ struct otm_batch {
   int count = 0;
   std::vector<int> n;
   std::vector<Fields> f;
-  std::map<int, DctPlan> plans;  // one plan per distinct grid size
+  std::map<int, DctPlan> plans;     // one plan per distinct grid size
+  std::map<int, MixedPlan> mplans;  // BF16 factor planes, on the first mixed solve
   JobSet js;
   Driver drv;
   std::vector<double*> d_in, d_xy;  // staging of host-API buffers
@@ -40,6 +43,8 @@ void batch_release(otm_batch* b) {
   jobset_destroy(b->js);
   for (Fields& f : b->f) fields_destroy(f);
   for (auto& kv : b->plans) dct_plan_destroy(kv.second);
+  for (auto& kv : b->mplans) mixed_plan_destroy(kv.second);
+  b->mplans.clear();
   for (double* p : b->d_in) if (p) (void)hipFree(p);
   for (double* p : b->d_xy) if (p) (void)hipFree(p);
   b->f.clear();
@@ -56,6 +61,11 @@ void build_table(otm_batch* b) {
     jobs[p].P = pl.P;
     jobs[p].Pt = pl.Pt;
     jobs[p].sq = pl.sq;
+    auto it = b->mplans.find(b->n[p]);
+    if (it != b->mplans.end()) {
+      jobs[p].Pp = it->second.Pp;
+      jobs[p].Ptp = it->second.Ptp;
+    }
   }
   int cfg[4];
   const bool had = b->js.jobs != nullptr;
@@ -71,14 +81,27 @@ int solve_batch(otm_batch* b, const double* const* d_density, const otm_options*
   otm_options_init(&opt);
   if (opt_in) opt = *opt_in;
   if (opt.max_iter < 0) opt.max_iter = 0;
-  if (opt.precision != OTM_PRECISION_FP64) {
-    set_error("batched solves run in FP64 (OTM_PRECISION_FP64)");
+  if (opt.precision != OTM_PRECISION_FP64 && opt.precision != OTM_PRECISION_MIXED) {
+    set_error("unknown precision mode");
     return OTM_ERR_ARG;
   }
+  if (opt.precision == OTM_PRECISION_MIXED && (opt.refine_steps < 0 || opt.refine_steps > 8)) {
+    set_error("refine_steps must be in [0, 8]");
+    return OTM_ERR_ARG;
+  }
+  bool rebuild = false;
   const int cap = std::max(opt.max_iter, 1);
   if (cap > b->trace_cap) {
     for (Fields& f : b->f) fields_grow_trace(f, cap);
     b->trace_cap = cap;
+    rebuild = true;
+  }
+  if (opt.precision == OTM_PRECISION_MIXED && b->mplans.empty()) {
+    for (Fields& f : b->f) fields_enable_mixed(f);
+    for (auto& kv : b->plans) mixed_plan_create(b->mplans[kv.first], kv.second);
+    rebuild = true;
+  }
+  if (rebuild) {
     build_table(b);
     driver_invalidate(b->drv);
   }
@@ -89,7 +112,7 @@ int solve_batch(otm_batch* b, const double* const* d_density, const otm_options*
     J.adj_out = d_adj ? d_adj[p] : nullptr;
   }
   float ms = 0.f;
-  if (int rc = driver_solve(b->drv, b->js, nullptr, nullptr, opt, &ms)) return rc;
+  if (int rc = driver_solve(b->drv, b->js, nullptr, opt, &ms)) return rc;
   if (stats) {
     for (int p = 0; p < b->count; ++p) {
       SolveState hs;

@@ -183,30 +183,66 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
   constexpr bool SPLIT = ST == F3 || ST == F4;
   __shared__ double As[2][FM][KB + 1];
   __shared__ double Bs[2][KB][FN + 2];
-  const int ntn = cols / FN;
-  const int m0 = (lt / ntn) * FM, n0 = (lt % ntn) * FN;
+  const int ntn = cols / FN, ntm = (ST == F3 ? h : n) / FM;
+  if (lt >= ntm * ntn) {  // padding workgroup of the job: neutral partial only
+    if (ST == F4) {
+      double z[1] = {0.};
+      write_partials_at<1>(z, J.f.site(SITE_DNORM), lt, nb);
+    }
+    return;
+  }
+  int tmi, tni;
+  xcd_tile(lt, ntm, ntn, tmi, tni);
+  const int m0 = tmi * FM, n0 = tni * FN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * (FM / 2), wn = (w & 1) * (FN / 2);
   const int lr = lane & 15, lk = lane >> 4;
   const double sgnA = (ST == F2 && n0 >= h) ? -1. : 1.;
   const double sgnB = (ST == F1 && m0 >= h) ? -1. : 1.;
   const double S = (ST == F1 && rhs) ? st->sum_r : 0.;
-  double ra[ACH][4], rb[BCH][4];
+  // two register slots of K-step loads in flight (see the pipeline below);
+  // chunk indices past the tile reload an earlier chunk (never stored) so
+  // that every lane issues the same loads
+  double ra[2][ACH][4], rb[2][BCH][4];
 
-  auto gload = [&](int k0) {
+  auto gload = [&](double (&xa)[ACH][4], double (&xb)[BCH][4], int k0) {
+#pragma unroll
+    for (int u = 0; u < ACH; ++u) {
+      const int c0 = t + u * NT, c = c0 < ACHUNKS ? c0 : c0 - ACHUNKS;
+      const int ar = c / (KB / 4), ak = (c % (KB / 4)) * 4;
+      const OTM_GLOBAL dv2* pa = reinterpret_cast<const OTM_GLOBAL dv2*>(A + long(m0 + ar) * lda + k0 + ak);
+      const dv2 x0 = pa[0], x1 = pa[1];
+      xa[u][0] = x0.x; xa[u][1] = x0.y; xa[u][2] = x1.x; xa[u][3] = x1.y;
+      if (ST == F2) {  // mirrored columns n-1-(k0+ak+q), q = 0..3, read as one ascending run
+        const OTM_GLOBAL dv2* pm = reinterpret_cast<const OTM_GLOBAL dv2*>(A + long(m0 + ar) * lda + (n - 4 - k0 - ak));
+        const dv2 y0 = pm[0], y1 = pm[1];
+        xa[u][0] += sgnA * y1.y; xa[u][1] += sgnA * y1.x; xa[u][2] += sgnA * y0.y; xa[u][3] += sgnA * y0.x;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BCH; ++u) {
+      const int c0 = t + u * NT, c = c0 < BCHUNKS ? c0 : c0 - BCHUNKS;
+      const int bk = c / (FN / 4), bc = (c % (FN / 4)) * 4;
+      const int gk = k0 + bk;
+      const OTM_GLOBAL dv2* pb = reinterpret_cast<const OTM_GLOBAL dv2*>(B + long(gk) * ldb + n0 + bc);
+      const dv2 y0 = pb[0], y1 = pb[1];
+      xb[u][0] = y0.x; xb[u][1] = y0.y; xb[u][2] = y1.x; xb[u][3] = y1.y;
+      if (ST == F1) {  // mirrored row n-1-gk
+        const OTM_GLOBAL dv2* pm = reinterpret_cast<const OTM_GLOBAL dv2*>(B + long(n - 1 - gk) * ldb + n0 + bc);
+        const dv2 z0 = pm[0], z1 = pm[1];
+        xb[u][0] += sgnB * z0.x; xb[u][1] += sgnB * z0.y; xb[u][2] += sgnB * z1.x; xb[u][3] += sgnB * z1.y;
+        if (gk == 0 && n0 + bc == 0) xb[u][0] -= S;  // R' = R - (sum r) e0 e0^T
+      }
+    }
+  };
+  auto sstore = [&](const double (&xa)[ACH][4], const double (&xb)[BCH][4], int buf) {
 #pragma unroll
     for (int u = 0; u < ACH; ++u) {
       const int c = t + u * NT;
       if (ACHUNKS % NT == 0 || c < ACHUNKS) {
         const int ar = c / (KB / 4), ak = (c % (KB / 4)) * 4;
-        const OTM_GLOBAL dv2* pa = reinterpret_cast<const OTM_GLOBAL dv2*>(A + long(m0 + ar) * lda + k0 + ak);
-        const dv2 x0 = pa[0], x1 = pa[1];
-        ra[u][0] = x0.x; ra[u][1] = x0.y; ra[u][2] = x1.x; ra[u][3] = x1.y;
-        if (ST == F2) {  // mirrored columns n-1-(k0+ak+q), q = 0..3, read as one ascending run
-          const OTM_GLOBAL dv2* pm = reinterpret_cast<const OTM_GLOBAL dv2*>(A + long(m0 + ar) * lda + (n - 4 - k0 - ak));
-          const dv2 y0 = pm[0], y1 = pm[1];
-          ra[u][0] += sgnA * y1.y; ra[u][1] += sgnA * y1.x; ra[u][2] += sgnA * y0.y; ra[u][3] += sgnA * y0.x;
-        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) As[buf][ar][ak + q] = xa[u][q];
       }
     }
 #pragma unroll
@@ -214,80 +250,66 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
       const int c = t + u * NT;
       if (BCHUNKS % NT == 0 || c < BCHUNKS) {
         const int bk = c / (FN / 4), bc = (c % (FN / 4)) * 4;
-        const int gk = k0 + bk;
-        const OTM_GLOBAL dv2* pb = reinterpret_cast<const OTM_GLOBAL dv2*>(B + long(gk) * ldb + n0 + bc);
-        const dv2 y0 = pb[0], y1 = pb[1];
-        rb[u][0] = y0.x; rb[u][1] = y0.y; rb[u][2] = y1.x; rb[u][3] = y1.y;
-        if (ST == F1) {  // mirrored row n-1-gk
-          const OTM_GLOBAL dv2* pm = reinterpret_cast<const OTM_GLOBAL dv2*>(B + long(n - 1 - gk) * ldb + n0 + bc);
-          const dv2 z0 = pm[0], z1 = pm[1];
-          rb[u][0] += sgnB * z0.x; rb[u][1] += sgnB * z0.y; rb[u][2] += sgnB * z1.x; rb[u][3] += sgnB * z1.y;
-          if (gk == 0 && n0 + bc == 0) rb[u][0] -= S;  // R' = R - (sum r) e0 e0^T
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Bs[buf][bk][bc + q] = xb[u][q];
+      }
+    }
+  };
+  auto compute = [&](d4 (&c)[TI][TJ], int buf) {
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 4) {
+      double a[TI], b[TJ];
+#pragma unroll
+      for (int i = 0; i < TI; ++i) a[i] = As[buf][wm + 16 * i + lr][kk + lk];
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) b[j] = Bs[buf][kk + lk][wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) c[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], c[i][j], 0, 0, 0);
+    }
+  };
+
+  // Software pipeline, two K-steps in flight: step s stores register slot
+  // s & 1 (loaded two steps earlier) into LDS buffer s & 1, refills the slot
+  // with step s + 2, syncs and multiplies; the buffer written at step s was
+  // last read at step s - 2, before every wave passed the sync of step s - 1.
+  // Loads past the last step re-read the last K block (in bounds, never
+  // stored).  The split stages move the first-half sums to acc_lo when step
+  // h / KB starts (the k order of every accumulation is unchanged).
+  d4 acc[TI][TJ], acc_lo[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      acc[i][j] = d4{0., 0., 0., 0.};
+      acc_lo[i][j] = d4{0., 0., 0., 0.};
+    }
+  const int nsteps = K / KB, hsteps = SPLIT ? h / KB : -1;
+  const int klast = K - KB;
+  auto step = [&](double (&xa)[ACH][4], double (&xb)[BCH][4], int s, int buf) {
+    if (SPLIT && s == hsteps) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+          acc_lo[i][j] = acc[i][j];
+          acc[i][j] = d4{0., 0., 0., 0.};
         }
-      }
     }
+    sstore(xa, xb, buf);
+    gload(xa, xb, min((s + 2) * KB, klast));
+    __syncthreads();
+    compute(acc, buf);
   };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < ACH; ++u) {
-      const int c = t + u * NT;
-      if (ACHUNKS % NT == 0 || c < ACHUNKS) {
-        const int ar = c / (KB / 4), ak = (c % (KB / 4)) * 4;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) As[buf][ar][ak + q] = ra[u][q];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < BCH; ++u) {
-      const int c = t + u * NT;
-      if (BCHUNKS % NT == 0 || c < BCHUNKS) {
-        const int bk = c / (FN / 4), bc = (c % (FN / 4)) * 4;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Bs[buf][bk][bc + q] = rb[u][q];
-      }
-    }
-  };
-
-  d4 acc[SPLIT ? 2 : 1][TI][TJ];
-#pragma unroll
-  for (int x = 0; x < (SPLIT ? 2 : 1); ++x)
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int j = 0; j < TJ; ++j) acc[x][i][j] = d4{0., 0., 0., 0.};
-
-  int buf = 0;
-  auto run = [&](d4 (&c)[TI][TJ], int kbeg, int kend) {
-    for (int k0 = kbeg; k0 < kend; k0 += KB) {
-      const bool more = k0 + KB < K;
-      if (more) gload(k0 + KB);
-#pragma unroll
-      for (int kk = 0; kk < KB; kk += 4) {
-        double a[TI], b[TJ];
-#pragma unroll
-        for (int i = 0; i < TI; ++i) a[i] = As[buf][wm + 16 * i + lr][kk + lk];
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) b[j] = Bs[buf][kk + lk][wn + 16 * j + lr];
-#pragma unroll
-        for (int i = 0; i < TI; ++i)
-#pragma unroll
-          for (int j = 0; j < TJ; ++j) c[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], c[i][j], 0, 0, 0);
-      }
-      if (more) sstore(buf ^ 1);
-      __syncthreads();
-      buf ^= 1;
-    }
-  };
-
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  if (SPLIT) {
-    run(acc[0], 0, h);
-    run(acc[SPLIT ? 1 : 0], h, K);
-  } else {
-    run(acc[0], 0, K);
+  gload(ra[0], rb[0], 0);
+  gload(ra[1], rb[1], min(KB, klast));
+  int s = 0;
+  for (; s + 2 <= nsteps; s += 2) {
+    step(ra[0], rb[0], s, 0);
+    step(ra[1], rb[1], s + 1, 1);
   }
+  if (s < nsteps) step(ra[0], rb[0], s, 0);
 
   // lane l holds D[l/16 + 4v][l%16] of each 16x16 tile
   double nrm[1] = {0.};
@@ -300,15 +322,15 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
         const int row = m0 + wm + 16 * i + lk + 4 * v;
         const int col = n0 + wn + 16 * j + lr;
         if (ST == F1) {
-          D[long(row) * n + col] = acc[0][i][j][v];
+          D[long(row) * n + col] = acc[i][j][v];
         } else if (ST == F2) {
           const int k = row < h ? 2 * row : 2 * (row - h) + 1;
           const int l = col < h ? 2 * col : 2 * (col - h) + 1;
           const int dm = k > l ? k - l : l - k;
           const double lam = 2. * (sq[dm] + sq[k + l]);
-          D[long(row) * n + col] = (k == 0 && l == 0) ? 0. : acc[0][i][j][v] / lam;
+          D[long(row) * n + col] = (k == 0 && l == 0) ? 0. : acc[i][j][v] / lam;
         } else {
-          const double pv = acc[0][i][j][v], qv = acc[SPLIT ? 1 : 0][i][j][v];
+          const double pv = acc_lo[i][j][v], qv = acc[i][j][v];
           const double s0 = pv + qv, s1 = pv - qv;
           if (ST == F3) {
             D[long(row) * n + col] = s0;

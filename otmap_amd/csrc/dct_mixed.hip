@@ -1,35 +1,32 @@
 // Mixed-precision Laplacian solve: the same 2-D DCT-II diagonalisation as the
-// FP64 path, with every GEMM run as a 3-term BF16 split on MFMA
-// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), followed by FP64 iterative
-// refinement against the exact operator.
-//
-// Operands and intermediates live in HBM as FP32 (4 bytes per element).  A
-// workgroup splits each FP32 value exactly into three BF16 terms
-// x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1))
-// while staging it into LDS, and a product keeps the six terms a_i b_j with
-// i + j <= 2 (NP = 6; NP = 3 keeps i + j <= 1).
-//
-// Every stage is D[m][n] = sum_k A[m][k] B[n][k] with both operands stored
-// with k contiguous (what the MFMA fragments want) and writes its output
-// transposed through LDS, so the next stage again contracts its contiguous
-// index:
-//   stage 1  T1 = R C^T          A = R  [i][j]    B = C  [l][j]   -> T1^T [l][i]
-//   stage 2  Y  = (C T1) / lam   A = T1^T[l][i]   B = C  [k][i]   -> Y    [k][l]
-//   stage 3  U  = Y C            A = Y  [k][l]    B = C^T[j][l]   -> U^T  [j][k]
-//   stage 4  X  = C^T U          A = U^T[j][k]    B = C^T[i][k]   -> X    [i][j] (FP64)
-//
-// Kernel shape: 64x64 output tile per workgroup (four waves, 32x32 each as
-// 2x2 MFMA tiles), K in steps of 64 with a 3-deep register prefetch ring and
-// double-buffered LDS, and an XCD-aware tile order: workgroups are dealt to
-// the 8 XCDs round-robin, so consecutive ids that land on one XCD get a
-// compact 4 x 8 block of tiles whose operand panels fit that XCD's L2.
-//
-// Refinement (FP64): s = r' - M0 x with the exact stencil of the reference's
+// FP64 path (d_hat = C^T [(C R' C^T) / lambda] C, R' = r - (sum r) e0), with
+// every GEMM run as a 3-term BF16 split on MFMA (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation), followed by FP64 iterative refinement against the exact
 // operator (weight -1/2 at the four clamped diagonal neighbours, 2 on the
-// diagonal; otlib/otsolver_2dgrid.cpp:284-306), then x += solve(s).
+// diagonal; otlib/otsolver_2dgrid.cpp:284-306).
+//
+// Operands live in HBM already split: a value x is stored as three BF16 planes
+// x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)),
+// so the GEMM inner loop only moves bytes and issues MFMAs.  A product keeps
+// the six terms a_i b_j with i + j <= 2.  The constant DCT factors are split
+// once per grid size; every stage splits its own output once in its epilogue.
+//
+// The transform is even/odd folded (h = n/2; P = [E; O], the even and odd rows
+// of C restricted to the first h columns, Pt = P^T), each stage being
+// D[m][c] = sum_k A[m][k] B[c][k] with k contiguous in both operands:
+//   prep  U = R'(:, j) + R'(:, n-1-j),  W = R'(:, j) - R'(:, n-1-j)     (n x h, FP64 -> planes)
+//   M1    T[i][l] = sum_j {U|W}[i][j] P[l][j]            (U for l < h, W for l >= h)
+//         -> U2[l][i] = T[i][l] + T[n-1-i][l],  W2 = difference        (rows i and n-1-i in one tile)
+//   M2    Z[l][k] = sum_i {U2|W2}[l][i] P[k][i] / lambda  (U2 for k < h)   Z = Y^T, even/odd order
+//   M3    T3[i][l] = sum_k Z[l][k] Pt[i][k]: the halves k < h, k >= h give rows i and n-1-i
+//   M4    X[i][j]  = sum_l T3[i][l] Pt[j][l]: the halves give columns j and n-1-j (FP64 out)
+// Each stage is n^3 multiply-adds (x6 BF16 products) instead of 2 n^3.
+//
+// Every kernel runs over a job table (one launch for all grids of a batch).
 
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "otm_internal.h"
@@ -41,19 +38,20 @@ namespace {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int TM = 64, TN = 64, KB = 64;  // workgroup tile and K step
-constexpr int DEPTH = 3;                  // register prefetch ring
-constexpr int LDK = KB + 8;               // LDS row length (bf16) of operand tiles: 144 B, conflict-free reads
-constexpr int LDD = TM + 4;               // LDS row length (f32) of the transposed output tile
-constexpr int PLANE = TM * LDK;           // u16 per plane of one operand tile
+constexpr int MTM = 64;        // A rows per workgroup tile
+constexpr int MTN = 32;        // B rows (output columns) per workgroup tile
+constexpr int MKB = 32;        // K step
+constexpr int MLD = MKB + 8;   // LDS row length (u16): 80 B rows keep 16-B fragment reads aligned
+constexpr int DLD = MTN + 1;   // LDS row length (f32) of a staged output tile
 
-enum MixedMode { X_PLAIN = 0, X_SPECTRAL = 1, X_FINAL = 2, X_FINAL_ADD = 3 };
+enum MStage { M1 = 0, M2 = 1, M3 = 2, M4 = 3 };
+// M4 epilogue: x0 = D | d_hat = D (+partials) | d_hat = x0 + D (+partials) | x0 += D
+enum MFinal { FIN_X0 = 0, FIN_OUT = 1, FIN_ADD_OUT = 2, FIN_ADD_X0 = 3 };
 
-__device__ inline u16 bf16_bits(float x) {
-  __bf16 h = (__bf16)x;
-  return __builtin_bit_cast(u16, h);
-}
+__device__ inline u16 bf16_bits(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
 __device__ inline float bf16_float(u16 b) { return (float)__builtin_bit_cast(__bf16, b); }
 
 // exact 3-term split of an FP32 value
@@ -61,301 +59,427 @@ __device__ inline void split3(float x, u16& h0, u16& h1, u16& h2) {
   h0 = bf16_bits(x);
   const float r1 = x - bf16_float(h0);
   h1 = bf16_bits(r1);
-  const float r2 = r1 - bf16_float(h1);
-  h2 = bf16_bits(r2);
+  h2 = bf16_bits(r1 - bf16_float(h1));
 }
 
-// tile coordinates for workgroup id b of an nt x nt grid (XCD-aware when nt % 4 == 0)
-__device__ inline void tile_of(int b, int nt, int& tm, int& tn) {
-  if (nt % 4 == 0 && nt >= 8) {
-    const int per = nt * nt / 8;          // tiles per XCD
-    const int xcd = b % 8, local = b / 8;
-    const int cols = nt / 2, rows = per / cols;  // rows x cols block of tiles per XCD
-    const int bx = xcd % 2, by = xcd / 2;
-    tm = by * rows + local / cols;
-    tn = bx * cols + local % cols;
-  } else {
-    tm = b / nt;
-    tn = b % nt;
+// 3-term split of an FP64 value (24 significant bits)
+__device__ inline void split3d(double x, u16& h0, u16& h1, u16& h2) {
+  h0 = bf16_bits(float(x));
+  const double r1 = x - double(bf16_float(h0));
+  h1 = bf16_bits(float(r1));
+  h2 = bf16_bits(float(r1 - double(bf16_float(h1))));
+}
+
+__host__ __device__ inline int mtiles_n(int st, int n) {
+  const int h = n / 2;
+  return st == M1 ? n / MTN : (st == M2 ? n / MTN : h / MTN);
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side: fold the columns of r' (mode 0) or of the refinement
+// residual s = r' - M0 x0 (mode 1) and split them into U / W planes
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                             int count, bool gate) {
+  const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
+  const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
+  const int nb = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
+  const Job& J = jobs[p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int n = J.f.n, h = n / 2;
+  const long nh = long(n) * h;
+  const auto r = gp((const double*)J.f.r);
+  const auto x = gp((const double*)J.f.x0);
+  const auto out = gp(J.f.mrhs);
+  const double S = st->sum_r;
+  auto src = [&](int i, int j) {
+    const long f = long(i) * n + j;
+    double v = r[f];
+    if (f == 0) v -= S;
+    if (MODE == 1) {
+      const int i1 = i == 0 ? i : i - 1, j1 = j == 0 ? j : j - 1;
+      const int i2 = i == n - 1 ? i : i + 1, j2 = j == n - 1 ? j : j + 1;
+      const double nbs = (x[long(i1) * n + j1] + x[long(i1) * n + j2]) + (x[long(i2) * n + j1] + x[long(i2) * n + j2]);
+      v = v - (2.0 * x[f] - 0.5 * nbs);
+    }
+    return v;
+  };
+  for (long q = long(lt) * NT + threadIdx.x; q < nh; q += long(nb) * NT) {
+    const int i = int(q / h), j = int(q - long(i) * h);
+    const double a = src(i, j), b = src(i, n - 1 - j);
+    u16 u0, u1, u2, w0, w1, w2;
+    split3d(a + b, u0, u1, u2);
+    split3d(a - b, w0, w1, w2);
+    out[q] = u0; out[nh + q] = u1; out[2 * nh + q] = u2;
+    out[3 * nh + q] = w0; out[4 * nh + q] = w1; out[5 * nh + q] = w2;
   }
 }
 
-// D = A * B^T, A and B n x n FP32 row-major (k contiguous).
-template <int MODE, int NP>
-__global__ __launch_bounds__(NT) void k_gemm_x3(const float* __restrict__ A, const float* __restrict__ B, int n,
-                                               float* __restrict__ out32, double* __restrict__ out64,
-                                               const double* __restrict__ add64, const double* __restrict__ sq,
-                                               const SolveState* st, double* partials) {
-  if (st && st->done) return;
-  constexpr int NPL = (NP == 1) ? 1 : (NP == 3 ? 2 : 3);  // planes used
-  // [buf][operand][plane][row * LDK + k]; the output tile reuses the same storage
-  __shared__ __attribute__((aligned(16))) u16 lds[2 * 2 * 3 * PLANE];
-  const int nt = n / TM;
+// ---------------------------------------------------------------------------
+// one stage of the folded transform for every job
+// ---------------------------------------------------------------------------
+template <int ST>
+__global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, const int* __restrict__ starts, int count,
+                                             bool gate, int fin) {
+  const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
+  const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
+  const int nbk = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
+  const Job& J = jobs[p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int n = J.f.n, h = n / 2;
+  const long nh = long(n) * h, nn = long(n) * n;
+  constexpr bool SPLIT = ST == M3 || ST == M4;
+  const int ntn = mtiles_n(ST, n);
+  const int ntm = ST == M1 ? h / (MTM / 2) : n / MTM;
+  if (lt >= ntm * ntn) {  // padding workgroup of the job: neutral partial only
+    if (ST == M4 && (fin == FIN_OUT || fin == FIN_ADD_OUT)) {
+      double z[1] = {0.};
+      write_partials_at<1>(z, J.f.site(SITE_DNORM), lt, nbk);
+    }
+    return;
+  }
   int tm, tn;
-  tile_of(blockIdx.x, nt, tm, tn);
-  const int m0 = tm * TM, n0 = tn * TN;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  const int lr = lane & 15, lq = lane >> 4;
-  const int grow = t >> 2, gk = (t & 3) * 16;  // staging: row, 16 consecutive k
-  const float* pa = A + long(m0 + grow) * n + gk;
-  const float* pb = B + long(n0 + grow) * n + gk;
-  const int nsteps = n / KB;
-
-  float4 ra[DEPTH][4], rb[DEPTH][4];
-  auto gload = [&](int slot, int step) {
-    const float4* qa = reinterpret_cast<const float4*>(pa + step * KB);
-    const float4* qb = reinterpret_cast<const float4*>(pb + step * KB);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      ra[slot][q] = qa[q];
-      rb[slot][q] = qb[q];
-    }
-  };
-  auto sstore = [&](int slot, int buf) {
-    u16* la = lds + (buf * 2 + 0) * 3 * PLANE + grow * LDK + gk;
-    u16* lb = lds + (buf * 2 + 1) * 3 * PLANE + grow * LDK + gk;
-    u16 ha[3][16], hb[3][16];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float xa[4] = {ra[slot][q].x, ra[slot][q].y, ra[slot][q].z, ra[slot][q].w};
-      const float xb[4] = {rb[slot][q].x, rb[slot][q].y, rb[slot][q].z, rb[slot][q].w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        split3(xa[e], ha[0][4 * q + e], ha[1][4 * q + e], ha[2][4 * q + e]);
-        split3(xb[e], hb[0][4 * q + e], hb[1][4 * q + e], hb[2][4 * q + e]);
-      }
-    }
-#pragma unroll
-    for (int p = 0; p < NPL; ++p) {
-      uint4 va0, va1, vb0, vb1;
-      va0.x = ha[p][0] | (unsigned(ha[p][1]) << 16); va0.y = ha[p][2] | (unsigned(ha[p][3]) << 16);
-      va0.z = ha[p][4] | (unsigned(ha[p][5]) << 16); va0.w = ha[p][6] | (unsigned(ha[p][7]) << 16);
-      va1.x = ha[p][8] | (unsigned(ha[p][9]) << 16); va1.y = ha[p][10] | (unsigned(ha[p][11]) << 16);
-      va1.z = ha[p][12] | (unsigned(ha[p][13]) << 16); va1.w = ha[p][14] | (unsigned(ha[p][15]) << 16);
-      vb0.x = hb[p][0] | (unsigned(hb[p][1]) << 16); vb0.y = hb[p][2] | (unsigned(hb[p][3]) << 16);
-      vb0.z = hb[p][4] | (unsigned(hb[p][5]) << 16); vb0.w = hb[p][6] | (unsigned(hb[p][7]) << 16);
-      vb1.x = hb[p][8] | (unsigned(hb[p][9]) << 16); vb1.y = hb[p][10] | (unsigned(hb[p][11]) << 16);
-      vb1.z = hb[p][12] | (unsigned(hb[p][13]) << 16); vb1.w = hb[p][14] | (unsigned(hb[p][15]) << 16);
-      *reinterpret_cast<uint4*>(la + p * PLANE) = va0;
-      *reinterpret_cast<uint4*>(la + p * PLANE + 8) = va1;
-      *reinterpret_cast<uint4*>(lb + p * PLANE) = vb0;
-      *reinterpret_cast<uint4*>(lb + p * PLANE + 8) = vb1;
-    }
-  };
-
-  f4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-
-  auto compute = [&](int buf) {
-    const u16* la = lds + (buf * 2 + 0) * 3 * PLANE;
-    const u16* lb = lds + (buf * 2 + 1) * 3 * PLANE;
-#pragma unroll
-    for (int kk = 0; kk < KB; kk += 32) {
-      bf8 fa[2][3], fb[2][3];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int p = 0; p < NPL; ++p) {
-          fa[i][p] = *reinterpret_cast<const bf8*>(la + p * PLANE + (wm + i * 16 + lr) * LDK + kk + lq * 8);
-          fb[i][p] = *reinterpret_cast<const bf8*>(lb + p * PLANE + (wn + i * 16 + lr) * LDK + kk + lq * 8);
-        }
-      // product-major order: consecutive MFMAs update different accumulators;
-      // small terms first, the leading product last
-      constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
-      constexpr int PB[6] = {0, 1, 2, 0, 1, 0};
-#pragma unroll
-      for (int q = 6 - NP; q < 6; ++q)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[j][PB[q]], acc[i][j], 0, 0, 0);
-    }
-  };
-
-#pragma unroll
-  for (int s = 0; s < DEPTH; ++s)
-    if (s < nsteps) gload(s, s);
-  for (int s0 = 0; s0 < nsteps; s0 += DEPTH) {
-#pragma unroll
-    for (int u = 0; u < DEPTH; ++u) {
-      const int s = s0 + u;
-      if (s < nsteps) {
-        const int buf = s & 1;
-        sstore(u, buf);
-        if (s + DEPTH < nsteps) gload(u, s + DEPTH);
-        __syncthreads();
-        compute(buf);
-      }
-    }
-  }
-  __syncthreads();
-
-  // transposed output tile in LDS: Ds[n_local][m_local]; lane holds D[4*lq+v][lr]
-  float* Ds = reinterpret_cast<float*>(lds);
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int ml = wm + i * 16 + lq * 4 + v, nl = wn + j * 16 + lr;
-        float val = acc[i][j][v];
-        if (MODE == X_SPECTRAL) {  // D[m=l][n=k] = Y[k][l]
-          const int r = m0 + ml, c = n0 + nl;
-          const int dm = r > c ? r - c : c - r;
-          const double lam = 2. * (sq[dm] + sq[r + c]);
-          val = (r == 0 && c == 0) ? 0.f : float(double(val) / lam);
-        }
-        Ds[nl * LDD + ml] = val;
-      }
-  __syncthreads();
-  // each thread writes 16 consecutive m of one transposed row
-  const int nl = t >> 2, mc = (t & 3) * 16;
-  const long orow = long(n0 + nl) * n + m0 + mc;
-  if (MODE == X_PLAIN || MODE == X_SPECTRAL) {
-    float4* dst = reinterpret_cast<float4*>(out32 + orow);
-    const float4* src = reinterpret_cast<const float4*>(Ds + nl * LDD + mc);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dst[q] = src[q];
+  xcd_tile(lt, ntm, ntn, tm, tn);
+  const int m0 = tm * (ST == M1 ? MTM / 2 : MTM), n0 = tn * MTN;
+  // operands (u16 planes): A rows x lda, plane stride aps; B rows x ldb, plane stride bps
+  const u16* Ab;
+  const u16* Bb;
+  int lda, ldb;
+  long aps, bps;
+  int K;
+  if (ST == M1) {
+    Ab = J.f.mrhs + (n0 >= h ? 3 * nh : 0); lda = h; aps = nh;
+    Bb = J.Pp; ldb = h; bps = nh; K = h;
+  } else if (ST == M2) {
+    Ab = J.f.mwa + (n0 >= h ? 3 * nh : 0); lda = h; aps = nh;
+    Bb = J.Pp; ldb = h; bps = nh; K = h;
+  } else if (ST == M3) {
+    Ab = J.f.mwb; lda = n; aps = nn;
+    Bb = J.Ptp; ldb = n; bps = nh; K = n;
   } else {
+    Ab = J.f.mwa; lda = n; aps = nn;
+    Bb = J.Ptp; ldb = n; bps = nh; K = n;
+  }
+  const auto A = gp(Ab);
+  const auto B = gp(Bb);
+
+  __shared__ __attribute__((aligned(16))) u16 As[2][3][MTM][MLD];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][3][MTN][MLD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 16;
+  const int lr = lane & 15, lq = lane >> 4;
+
+  // global -> register staging: A 3 x 64 rows x 4 chunks of 8 (3 per thread), B 3 x 32 x 4 (1.5 per thread)
+  // two register slots of K-step loads in flight, double-buffered LDS
+  u32x4 ra[2][3], rb[2][2];
+  auto arow = [&](int row) {  // global row of tile row `row`
+    if (ST == M1) return row < MTM / 2 ? m0 + row : n - m0 - MTM + row;  // rows i and mirrored n-1-i
+    return m0 + row;
+  };
+  auto gload = [&](u32x4 (&xa)[3], u32x4 (&xb)[2], int k0) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int c = t + u * NT;
+      const int pl = c >> 8, rem = c & 255, row = rem >> 2, kc = (rem & 3) * 8;
+      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + pl * aps + long(arow(row)) * lda + k0 + kc);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      // 384 B chunks over 512 slots: the last 128 slots reload chunks 256..383
+      // (never stored), so every lane issues the same loads and the waits on
+      // the ring stay exact
+      const int c0 = t + u * NT, c = c0 < 3 * MTN * 4 ? c0 : c0 - MTN * 4;
+      const int pl = c >> 7, rem = c & 127, row = rem >> 2, kc = (rem & 3) * 8;
+      xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + long(n0 + row) * ldb + k0 + kc);
+    }
+  };
+  auto sstore = [&](const u32x4 (&xa)[3], const u32x4 (&xb)[2], int buf) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int c = t + u * NT;
+      const int pl = c >> 8, rem = c & 255, row = rem >> 2, kc = (rem & 3) * 8;
+      *reinterpret_cast<u32x4*>(&As[buf][pl][row][kc]) = xa[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int c = t + u * NT;
+      if (c < 3 * MTN * 4) {
+        const int pl = c >> 7, rem = c & 127, row = rem >> 2, kc = (rem & 3) * 8;
+        *reinterpret_cast<u32x4*>(&Bs[buf][pl][row][kc]) = xb[u];
+      }
+    }
+  };
+
+  auto compute = [&](f4 (&c)[2], int buf) {
+    bf8 fa[2][3], fb[3];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(&As[buf][pl][wm + 16 * i + lr][lq * 8]);
+      fb[pl] = *reinterpret_cast<const bf8*>(&Bs[buf][pl][wn + lr][lq * 8]);
+    }
+    // small products first, the leading one last
+    constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
+    constexpr int PB[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) c[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[PB[q]], c[i], 0, 0, 0);
+  };
+
+  // Software pipeline, two K-steps in flight: step s stores register slot
+  // s & 1 (loaded two steps earlier) into LDS buffer s & 1, refills the slot
+  // with step s + 2, syncs and multiplies.  The buffer written at step s was
+  // last read at step s - 2, before every wave passed the sync of step s - 1.
+  // The main loop body holds no conditional memory operation, so the waits on
+  // the loads stay exact.  Loads past the last step re-read the last K block
+  // (in bounds, never stored).  For the split stages the first-half sum moves
+  // to acc_lo when step h / MKB starts.
+  const int nsteps = K / MKB, hsteps = SPLIT ? h / MKB : -1;
+  const int klast = K - MKB;
+  f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+  f4 acc_lo[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+  auto step = [&](u32x4 (&xa)[3], u32x4 (&xb)[2], int s, int buf) {
+    if (SPLIT && s == hsteps) {
+      acc_lo[0] = acc[0];
+      acc_lo[1] = acc[1];
+      acc[0] = f4{0.f, 0.f, 0.f, 0.f};
+      acc[1] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    sstore(xa, xb, buf);
+    gload(xa, xb, min((s + 2) * MKB, klast));
+    __syncthreads();
+    compute(acc, buf);
+  };
+  gload(ra[0], rb[0], 0);
+  gload(ra[1], rb[1], min(MKB, klast));
+  int s = 0;
+  for (; s + 2 <= nsteps; s += 2) {
+    step(ra[0], rb[0], s, 0);
+    step(ra[1], rb[1], s + 1, 1);
+  }
+  if (s < nsteps) step(ra[0], rb[0], s, 0);
+  __syncthreads();
+  const f4 r0[2] = {SPLIT ? acc_lo[0] : acc[0], SPLIT ? acc_lo[1] : acc[1]};  // k < h (or all k)
+  const f4 r1[2] = {acc[0], acc[1]};                                       // k >= h
+
+  // lane (lr, lq) of MFMA tile i holds D[wm + 16i + 4 lq + v][wn + lr]
+  if (ST == M4) {
+    const auto x0 = gp(J.f.x0);
+    const auto dh = gp(J.f.dhat);
     double nrm[1] = {0.};
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      double v = double(Ds[nl * LDD + mc + e]);
-      if (MODE == X_FINAL_ADD) v = add64[orow + e] + v;
-      out64[orow + e] = v;
-      nrm[0] += v * v;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long row = m0 + wm + 16 * i + 4 * lq + v;
+        const int col = n0 + wn + lr;
+        const double p0 = double(r0[i][v]), p1 = double(r1[i][v]);
+        const long ia = row * n + col, ib = row * n + (n - 1 - col);
+        double va = p0 + p1, vb = p0 - p1;
+        if (fin == FIN_ADD_OUT || fin == FIN_ADD_X0) {
+          va = x0[ia] + va;
+          vb = x0[ib] + vb;
+        }
+        if (fin == FIN_X0 || fin == FIN_ADD_X0) {
+          x0[ia] = va;
+          x0[ib] = vb;
+        } else {
+          dh[ia] = va;
+          dh[ib] = vb;
+          nrm[0] += va * va + vb * vb;
+        }
+      }
+    if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nbk);
+    return;
+  }
+
+  // the other stages stage the FP32 tile(s) in LDS (reusing the operand
+  // buffers) and write BF16 planes in 8-16 byte runs
+  float* Ds0 = reinterpret_cast<float*>(&As[0][0][0][0]);
+  float* Ds1 = Ds0 + MTM * DLD;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = wm + 16 * i + 4 * lq + v, c = wn + lr;
+      if (ST == M3) {
+        Ds0[r * DLD + c] = r0[i][v] + r1[i][v];
+        Ds1[r * DLD + c] = r0[i][v] - r1[i][v];
+      } else {
+        Ds0[r * DLD + c] = r0[i][v];
+      }
     }
-    if (partials) write_partials<1>(nrm, partials);
+  __syncthreads();
+  if (ST == M1) {
+    // U2 / W2 [l = n0 + c][i = m0 + a .. a+3]: pair tile row a with its mirror 63 - a
+    const auto out = gp(J.f.mwa);
+    const int c = t >> 3, a0 = (t & 7) * 4;
+    u16 u[3][4], wv[3][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float top = Ds0[(a0 + q) * DLD + c], mir = Ds0[(MTM - 1 - a0 - q) * DLD + c];
+      split3(top + mir, u[0][q], u[1][q], u[2][q]);
+      split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
+    }
+    const long base = long(n0 + c) * h + m0 + a0;
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      u32x2 pu, pw;
+      pu.x = u[pl][0] | (unsigned(u[pl][1]) << 16); pu.y = u[pl][2] | (unsigned(u[pl][3]) << 16);
+      pw.x = wv[pl][0] | (unsigned(wv[pl][1]) << 16); pw.y = wv[pl][2] | (unsigned(wv[pl][3]) << 16);
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + base) = pu;
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + 3 * nh + pl * nh + base) = pw;
+    }
+  } else if (ST == M2) {
+    // Z[l = m0 + r][k = n0 + c0 .. c0+7] / lambda(perm k, perm l)
+    const auto out = gp(J.f.mwb);
+    const auto sq = gp(J.sq);
+    const int r = t >> 2, c0 = (t & 3) * 8;
+    const int lrow = m0 + r;
+    const int lperm = lrow < h ? 2 * lrow : 2 * (lrow - h) + 1;
+    u16 z[3][8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kc = n0 + c0 + q;
+      const int kperm = kc < h ? 2 * kc : 2 * (kc - h) + 1;
+      const int dm = kperm > lperm ? kperm - lperm : lperm - kperm;
+      const double lam = 2. * (sq[dm] + sq[kperm + lperm]);
+      const float val = (kperm == 0 && lperm == 0) ? 0.f : float(double(Ds0[r * DLD + c0 + q]) / lam);
+      split3(val, z[0][q], z[1][q], z[2][q]);
+    }
+    const long base = long(lrow) * n + n0 + c0;
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      u32x4 pz;
+      pz.x = z[pl][0] | (unsigned(z[pl][1]) << 16); pz.y = z[pl][2] | (unsigned(z[pl][3]) << 16);
+      pz.z = z[pl][4] | (unsigned(z[pl][5]) << 16); pz.w = z[pl][6] | (unsigned(z[pl][7]) << 16);
+      *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + base) = pz;
+    }
+  } else {
+    // T3[i = n0 + c][l = m0 + r0 .. r0+7] and T3[n-1-i][l]
+    const auto out = gp(J.f.mwa);
+    const int c = t >> 3, r0 = (t & 7) * 8;
+    u16 s0[3][8], s1[3][8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      split3(Ds0[(r0 + q) * DLD + c], s0[0][q], s0[1][q], s0[2][q]);
+      split3(Ds1[(r0 + q) * DLD + c], s1[0][q], s1[1][q], s1[2][q]);
+    }
+    const long b0 = long(n0 + c) * n + m0 + r0, b1 = long(n - 1 - n0 - c) * n + m0 + r0;
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      u32x4 p0, p1;
+      p0.x = s0[pl][0] | (unsigned(s0[pl][1]) << 16); p0.y = s0[pl][2] | (unsigned(s0[pl][3]) << 16);
+      p0.z = s0[pl][4] | (unsigned(s0[pl][5]) << 16); p0.w = s0[pl][6] | (unsigned(s0[pl][7]) << 16);
+      p1.x = s1[pl][0] | (unsigned(s1[pl][1]) << 16); p1.y = s1[pl][2] | (unsigned(s1[pl][3]) << 16);
+      p1.z = s1[pl][4] | (unsigned(s1[pl][5]) << 16); p1.w = s1[pl][6] | (unsigned(s1[pl][7]) << 16);
+      *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b0) = p0;
+      *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b1) = p1;
+    }
   }
 }
 
-// FP64 face field -> FP32 right-hand side; element 0 gets r[0] - S (rank-1 pin correction)
-__global__ __launch_bounds__(NT) void k_split_rhs(const double* __restrict__ r, float* __restrict__ out, long F,
-                                                 const SolveState* st) {
-  if (st && st->done) return;
-  const double S = st ? st->sum_r : 0.;
-  for (long i = long(blockIdx.x) * NT + threadIdx.x; i < F; i += long(gridDim.x) * NT) {
-    double v = r[i];
-    if (i == 0) v -= S;
-    out[i] = float(v);
-  }
-}
-
-// s = r' - M0 x  (FP64), stored as FP32; r' = r - S e0
-__global__ __launch_bounds__(NT) void k_refine_residual(const double* __restrict__ r, const double* __restrict__ x,
-                                                       float* __restrict__ out, int n, const SolveState* st) {
-  if (st && st->done) return;
-  const double S = st ? st->sum_r : 0.;
-  const long F = long(n) * n;
-  for (long f = long(blockIdx.x) * NT + threadIdx.x; f < F; f += long(gridDim.x) * NT) {
-    const int i = int(f / n), j = int(f - (f / n) * n);
-    const int i1 = i == 0 ? i : i - 1, j1 = j == 0 ? j : j - 1;
-    const int i2 = i == n - 1 ? i : i + 1, j2 = j == n - 1 ? j : j + 1;
-    const double nb = (x[long(i1) * n + j1] + x[long(i1) * n + j2]) + (x[long(i2) * n + j1] + x[long(i2) * n + j2]);
-    double rv = r[f];
-    if (f == 0) rv -= S;
-    out[f] = float(rv - (2.0 * x[f] - 0.5 * nb));
-  }
-}
-
-template <int MODE, int NP>
-void mgemm(const float* A, const float* B, int n, float* out32, double* out64, const double* add64, const double* sq,
-           const SolveState* st, double* partials, hipStream_t s) {
-  const int nt = n / TM;
-  hipLaunchKernelGGL((k_gemm_x3<MODE, NP>), dim3(nt * nt), dim3(NT), 0, s, A, B, n, out32, out64, add64, sq, st,
-                     partials);
+template <int ST>
+void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
+  const int kind = KIND_M0 + ST;
+  hipLaunchKernelGGL((k_mgemm<ST>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s, (const Job*)js.jobs,
+                     (const int*)js.starts_of(kind), js.count, gate, fin);
   OTM_HIP(hipGetLastError());
 }
 
-// out = (add +) transform(rhs); publish: the final stage leaves the |out|^2
-// partials in SITE_DNORM for the early-stop test (otsolver_2dgrid.cpp:182)
-template <int NP>
-void mixed_transform(const MixedPlan& mp, const DctPlan& p, const Fields& f, const float* rhs, double* out,
-                     const double* add, bool publish, hipStream_t s) {
-  const int n = p.n;
-  const long nn = long(n) * n;
-  const SolveState* st = f.state;
-  float* t0 = mp.work;
-  float* t1 = mp.work + nn;
-  double* part = publish ? f.site(SITE_DNORM) : nullptr;
-  mgemm<X_PLAIN, NP>(rhs, mp.C32, n, t0, nullptr, nullptr, p.sq, st, nullptr, s);
-  mgemm<X_SPECTRAL, NP>(t0, mp.C32, n, t1, nullptr, nullptr, p.sq, st, nullptr, s);
-  mgemm<X_PLAIN, NP>(t1, mp.Ct32, n, t0, nullptr, nullptr, p.sq, st, nullptr, s);
-  if (add)
-    mgemm<X_FINAL_ADD, NP>(t0, mp.Ct32, n, nullptr, out, add, p.sq, st, part, s);
-  else
-    mgemm<X_FINAL, NP>(t0, mp.Ct32, n, nullptr, out, nullptr, p.sq, st, part, s);
+template <int MODE>
+void launch_mprep(const JobSet& js, bool gate, hipStream_t s) {
+  hipLaunchKernelGGL((k_mprep<MODE>), dim3(unsigned(js.total[KIND_FLAT])), dim3(NT), 0, s, (const Job*)js.jobs,
+                     (const int*)js.starts_of(KIND_FLAT), js.count, gate);
+  OTM_HIP(hipGetLastError());
+}
+
+void mixed_transform(const JobSet& js, bool gate, int fin, hipStream_t s) {
+  launch_mstage<M1>(js, gate, 0, s);
+  launch_mstage<M2>(js, gate, 0, s);
+  launch_mstage<M3>(js, gate, 0, s);
+  launch_mstage<M4>(js, gate, fin, s);
 }
 
 }  // namespace
 
-bool mixed_supported(int n) { return n % 64 == 0; }
+int mixed_tiles(int st, int n) {
+  const int h = n / 2;
+  switch (st) {
+    case M1: return (h / (MTM / 2)) * (n / MTN);
+    case M2: return (n / MTM) * (n / MTN);
+    default: return (n / MTM) * (h / MTN);
+  }
+}
+
+bool mixed_supported(int n) { return n % 64 == 0 && n >= 64; }
 
 void mixed_plan_create(MixedPlan& mp, const DctPlan& p) {
-  const int n = p.n;
+  const int n = p.n, h = n / 2;
   mp.n = n;
-  if (!mixed_supported(n)) return;
-  const size_t nn = size_t(n) * n;
-  std::vector<double> C(nn), Ct(nn);
-  OTM_HIP(hipMemcpy(C.data(), p.C, nn * sizeof(double), hipMemcpyDeviceToHost));
-  OTM_HIP(hipMemcpy(Ct.data(), p.Ct, nn * sizeof(double), hipMemcpyDeviceToHost));
-  std::vector<float> c32(nn), ct32(nn);
-  for (size_t k = 0; k < nn; ++k) {
-    c32[k] = float(C[k]);
-    ct32[k] = float(Ct[k]);
-  }
-  OTM_HIP(hipMalloc(&mp.C32, nn * sizeof(float)));
-  OTM_HIP(hipMalloc(&mp.Ct32, nn * sizeof(float)));
-  OTM_HIP(hipMalloc(&mp.work, 2 * nn * sizeof(float)));
-  OTM_HIP(hipMalloc(&mp.rhs, nn * sizeof(float)));
-  OTM_HIP(hipMalloc(&mp.x0, nn * sizeof(double)));
-  OTM_HIP(hipMemcpy(mp.C32, c32.data(), nn * sizeof(float), hipMemcpyHostToDevice));
-  OTM_HIP(hipMemcpy(mp.Ct32, ct32.data(), nn * sizeof(float), hipMemcpyHostToDevice));
+  if (!mixed_supported(n) || !p.fold) return;
+  const size_t nh = size_t(n) * h;
+  std::vector<double> P(nh);
+  OTM_HIP(hipMemcpy(P.data(), p.P, nh * sizeof(double), hipMemcpyDeviceToHost));
+  // 3-term BF16 split of the FP64 factors (round to nearest even, as __bf16 does)
+  auto bf = [](float x) {
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u16(u >> 16);
+  };
+  auto fl = [](u16 b) {
+    const unsigned u = unsigned(b) << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+  };
+  std::vector<u16> Pp(3 * nh), Ptp(3 * nh);
+  for (int r = 0; r < n; ++r)
+    for (int i = 0; i < h; ++i) {
+      const double x = P[size_t(r) * h + i];
+      const u16 h0 = bf(float(x));
+      const double r1 = x - double(fl(h0));
+      const u16 h1 = bf(float(r1));
+      const u16 h2 = bf(float(r1 - double(fl(h1))));
+      const u16 hs[3] = {h0, h1, h2};
+      for (int pl = 0; pl < 3; ++pl) {
+        Pp[pl * nh + size_t(r) * h + i] = hs[pl];
+        Ptp[pl * nh + size_t(i) * n + r] = hs[pl];
+      }
+    }
+  OTM_HIP(hipMalloc(&mp.Pp, 3 * nh * sizeof(u16)));
+  OTM_HIP(hipMalloc(&mp.Ptp, 3 * nh * sizeof(u16)));
+  OTM_HIP(hipMemcpy(mp.Pp, Pp.data(), 3 * nh * sizeof(u16), hipMemcpyHostToDevice));
+  OTM_HIP(hipMemcpy(mp.Ptp, Ptp.data(), 3 * nh * sizeof(u16), hipMemcpyHostToDevice));
 }
 
 void mixed_plan_destroy(MixedPlan& mp) {
-  if (mp.C32) (void)hipFree(mp.C32);
-  if (mp.Ct32) (void)hipFree(mp.Ct32);
-  if (mp.work) (void)hipFree(mp.work);
-  if (mp.rhs) (void)hipFree(mp.rhs);
-  if (mp.x0) (void)hipFree(mp.x0);
+  if (mp.Pp) (void)hipFree(mp.Pp);
+  if (mp.Ptp) (void)hipFree(mp.Ptp);
   mp = MixedPlan();
 }
 
-// one ungated 6-product stage (stage 3 shape), for kernel timing
-void launch_mixed_gemm_stage(const MixedPlan& mp, const DctPlan& p, hipStream_t s) {
-  const long nn = long(p.n) * p.n;
-  mgemm<X_PLAIN, 6>(mp.work + nn, mp.Ct32, p.n, mp.work, nullptr, nullptr, p.sq, nullptr, nullptr, s);
-}
+// one ungated stage (M2, the plain n x n output stage) for kernel timing
+void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s) { launch_mstage<M2>(js, false, 0, s); }
 
-// d_hat = M^+ (r - S e0): mixed transform, then `refine` FP64 refinement steps
-// (the correction transforms keep NP_CORR products).
-void launch_mixed_solve(const MixedPlan& mp, const DctPlan& p, const Fields& f, const double* r, double* out,
-                        int refine, hipStream_t s) {
-  const int n = p.n;
-  const long F = long(n) * n;
-  const unsigned blocks = unsigned(std::min<long>((F + NT - 1) / NT, 8192));
-  hipLaunchKernelGGL(k_split_rhs, dim3(blocks), dim3(NT), 0, s, r, mp.rhs, F, (const SolveState*)f.state);
-  OTM_HIP(hipGetLastError());
+// d_hat = M^+ (r - S e0) of every job: mixed transform, then `refine` FP64
+// refinement steps x += transform(r' - M0 x); the last stage leaves the
+// |d_hat|^2 partials in SITE_DNORM (launch kind KIND_M0 + 3).
+void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s) {
+  launch_mprep<0>(js, gate, s);
   if (refine <= 0) {
-    mixed_transform<6>(mp, p, f, mp.rhs, out, nullptr, true, s);
+    mixed_transform(js, gate, FIN_OUT, s);
     return;
   }
-  mixed_transform<6>(mp, p, f, mp.rhs, mp.x0, nullptr, false, s);
+  mixed_transform(js, gate, FIN_X0, s);
   for (int k = 0; k < refine; ++k) {
-    hipLaunchKernelGGL(k_refine_residual, dim3(blocks), dim3(NT), 0, s, r, (const double*)mp.x0, mp.rhs, n,
-                       (const SolveState*)f.state);
-    OTM_HIP(hipGetLastError());
-    const bool last = (k == refine - 1);
-    // all six products: with the three leading ones only, one refinement step
-    // leaves ~1e-9 instead of ~2e-10 relative error at n = 1024
-    mixed_transform<6>(mp, p, f, mp.rhs, last ? out : mp.x0, mp.x0, last, s);
+    launch_mprep<1>(js, gate, s);
+    mixed_transform(js, gate, k == refine - 1 ? FIN_ADD_OUT : FIN_ADD_X0, s);
   }
 }
 

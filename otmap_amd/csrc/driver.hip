@@ -60,7 +60,17 @@ void fields_destroy(Fields& f) {
   dfree(f.rho); dfree(f.psi); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
   dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
   dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.trace);
+  dfree(f.mrhs); dfree(f.mwa); dfree(f.mwb); dfree(f.x0);
   f = Fields();
+}
+
+void fields_enable_mixed(Fields& f) {
+  if (f.x0) return;
+  const size_t F = f.F;
+  dalloc(f.mrhs, 3 * F);
+  dalloc(f.mwa, 3 * F);
+  dalloc(f.mwb, 3 * F);
+  dalloc(f.x0, F);
 }
 
 bool fields_grow_trace(Fields& f, int cap) {
@@ -99,30 +109,24 @@ void driver_destroy(Driver& d) {
 }
 
 // launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
-int dnorm_kind(const JobSet& js, bool mixed) { return (mixed || !js.fold) ? KIND_PLAIN64 : fold_kind(3, js.cfg[3]); }
+int dnorm_kind(const JobSet& js, bool mixed) {
+  if (mixed) return KIND_M0 + 3;
+  return js.fold ? fold_kind(3, js.cfg[3]) : KIND_PLAIN64;
+}
 
-void enqueue_iteration(const JobSet& js, const DctPlan* plan, const MixedPlan* mplan, const otm_options& opt,
-                       hipStream_t s) {
+void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s) {
   const bool mixed = opt.precision == OTM_PRECISION_MIXED;
-  if (mixed) {
-    const Fields& f = js.host[0].f;
-    launch_mixed_solve(*mplan, *plan, f, f.r, f.dhat, opt.refine_steps, s);
-  } else {
+  if (mixed)
+    launch_mixed_solve(js, opt.refine_steps, true, s);
+  else
     launch_dct_solve(js, plan, s);
-  }
   launch_beta(js, dnorm_kind(js, mixed), s);
   launch_dir_coef(js, s);
   launch_update(js, s);
 }
 
-int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const MixedPlan* mplan, const otm_options& opt,
-                 float* ms) {
+int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& opt, float* ms) {
   hipStream_t s = d.stream;
-  const bool mixed = opt.precision == OTM_PRECISION_MIXED;
-  if (mixed && js.count != 1) {
-    set_error("mixed precision runs on single-grid handles");
-    return OTM_ERR_ARG;
-  }
   // the captured iteration depends on the precision mode and the tile shapes
   const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (long(js.cfg[0]) << 8) |
                    (long(js.cfg[1]) << 12) | (long(js.cfg[2]) << 16) | (long(js.cfg[3]) << 20);
@@ -133,7 +137,7 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const MixedPlan* mp
   if (!d.exec) {
     OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-      for (int k = 0; k < ITERS_PER_GRAPH; ++k) enqueue_iteration(js, plan, mplan, opt, s);
+      for (int k = 0; k < ITERS_PER_GRAPH; ++k) enqueue_iteration(js, plan, opt, s);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);

@@ -103,6 +103,13 @@ struct Fields {
   SolveState* state = nullptr;
   double* trace = nullptr;   // 3 * trace_cap: residual, alpha, beta per iteration
   int trace_cap = 0;
+  // mixed-precision solve (allocated on first use): BF16 planes of the folded
+  // right-hand side (2 x 3 x n*h), two stage intermediates (3 x n*n each) and
+  // the FP64 unrefined solution
+  unsigned short* mrhs = nullptr;
+  unsigned short* mwa = nullptr;
+  unsigned short* mwb = nullptr;
+  double* x0 = nullptr;
   __host__ __device__ double* site(int s) const { return partials + long(s) * SITE_VALUES * pslots; }
 };
 
@@ -119,6 +126,8 @@ struct Job {
   const double* P = nullptr;   // folded DCT plan of f.n (see DctPlan)
   const double* Pt = nullptr;
   const double* sq = nullptr;
+  const unsigned short* Pp = nullptr;   // BF16 planes of P (3 x n*h) for the mixed solve
+  const unsigned short* Ptp = nullptr;  // BF16 planes of Pt (3 x h*n)
   const double* dens_in = nullptr;  // input density of the current solve (F)
   double* fwd_out = nullptr;        // forward vertex positions (2V, SoA)
   double* adj_out = nullptr;        // copy of the adjusted density (F, nullable)
@@ -142,12 +151,14 @@ inline int fold_tiles(int n, int st, int c) {
 // workgroups of the grid-stride ("flat") kernels per job
 inline int flat_blocks(long F) { return int(std::min<long>((F + NT * 4 - 1) / (NT * 4), 1024)); }
 
-// KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 and mixed GEMMs)
+// KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
+// KIND_M0 + s: stage s of the mixed transform
 enum LaunchKind {
   KIND_STENCIL = 0,
   KIND_FLAT = 1,
   KIND_PLAIN64 = 2,
-  KIND_GEMM0 = 3,
+  KIND_M0 = 3,
+  KIND_GEMM0 = KIND_M0 + 4,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
 inline int fold_kind(int stage, int c) { return KIND_GEMM0 + stage * NFOLD + c; }
@@ -205,22 +216,21 @@ void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s);
 // fastest in js.cfg
 void dct_tune(JobSet& js, int device, hipStream_t s);
 
-// Mixed-precision Laplacian solve (dct_mixed.hip): 3-term BF16 split GEMMs on
-// MFMA + FP64 refinement.  Requires n % 64 == 0.
+// Mixed-precision Laplacian solve (dct_mixed.hip): folded transform with
+// 3-term BF16 split GEMMs on MFMA + FP64 refinement.  Requires n % 64 == 0.
 struct MixedPlan {
   int n = 0;
-  float* C32 = nullptr;   // C   [k][i] in FP32 (split into BF16 terms on the fly)
-  float* Ct32 = nullptr;  // C^T [i][k]
-  float* work = nullptr;  // 2 intermediates (FP32)
-  float* rhs = nullptr;   // right-hand side of the current transform (FP32)
-  double* x0 = nullptr;   // unrefined solution (FP64)
+  unsigned short* Pp = nullptr;   // BF16 planes of P  (3 x n*h)
+  unsigned short* Ptp = nullptr;  // BF16 planes of Pt (3 x h*n)
 };
 bool mixed_supported(int n);
+int mixed_tiles(int stage, int n);  // workgroups of mixed stage 0..3
 void mixed_plan_create(MixedPlan& mp, const DctPlan& p);
 void mixed_plan_destroy(MixedPlan& mp);
-void launch_mixed_solve(const MixedPlan& mp, const DctPlan& p, const Fields& f, const double* r, double* out,
-                        int refine, hipStream_t s);
-void launch_mixed_gemm_stage(const MixedPlan& mp, const DctPlan& p, hipStream_t s);
+void fields_enable_mixed(Fields& f);  // allocates the mixed buffers (driver.hip)
+// d_hat = M^+ (r - S e0) of every job of the table (all with mixed buffers and plans)
+void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s);
+void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s);
 
 // Per-grid buffers and the iteration driver (driver.hip)
 void fields_create(Fields& f, int n, int trace_cap);
@@ -241,8 +251,8 @@ void driver_destroy(Driver& d);
 void driver_invalidate(Driver& d);  // drop the captured graph (tables or shapes changed)
 int dnorm_kind(const JobSet& js, bool mixed);
 // one solve of every job of the table (js.host[p].dens_in / fwd_out / adj_out
-// set by the caller); returns an OTM_* code, *ms = device time
-int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const MixedPlan* mplan, const otm_options& opt,
-                 float* ms);
+// set by the caller; for mixed precision every job has its mixed buffers and
+// plan); returns an OTM_* code, *ms = device time
+int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& opt, float* ms);
 
 }  // namespace otm

@@ -47,6 +47,24 @@ __device__ static inline void write_partials(double (&v)[K], double* partials) {
   write_partials_at<K>(v, partials, block_id(), block_count());
 }
 
+// XCD-aware tile order.  Workgroups are dealt to the 8 XCDs round-robin
+// (workgroup b runs on XCD b % 8), each with its own L2.  When the R x C tile
+// grid splits into 4 x 2 blocks, local tile lt goes to block lt % 8, so one
+// XCD's workgroups share R/4 tile rows of A and C/2 tile columns of B in its
+// L2 instead of streaming the whole operands.  Requires the job's first
+// workgroup index to be a multiple of 8 (job tables pad GEMM launches).
+__device__ static inline void xcd_tile(int lt, int R, int C, int& tm, int& tn) {
+  if (R % 4 == 0 && C % 2 == 0) {
+    const int xcd = lt & 7, local = lt >> 3;
+    const int br = R >> 2, bc = C >> 1;
+    tm = (xcd >> 1) * br + local / bc;
+    tn = (xcd & 1) * bc + local % bc;
+  } else {
+    tm = lt / C;
+    tn = lt % C;
+  }
+}
+
 // Job of workgroup b in a launch over a job table (starts: count + 1 prefix
 // sums of the workgroups per job); uniform across the workgroup.
 __device__ static inline int job_of(const int* __restrict__ starts, int count, int b) {

@@ -84,6 +84,19 @@ void allocate(otm_solver* s, int n) {
   s->n = n;
 }
 
+// mixed-precision buffers and BF16 factor planes, on first use
+void ensure_mixed(otm_solver* s) {
+  if (s->f.x0) return;
+  fields_enable_mixed(s->f);
+  mixed_plan_create(s->mplan, s->plan);
+  Job& J = s->js.host[0];
+  J.f = s->f;
+  J.Pp = s->mplan.Pp;
+  J.Ptp = s->mplan.Ptp;
+  jobset_upload(s->js, s->drv.stream);
+  driver_invalidate(s->drv);
+}
+
 int fail_hip(const HipError& e) {
   set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
   return OTM_ERR_HIP;
@@ -125,7 +138,7 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
       set_error("refine_steps must be in [0, 8]");
       return OTM_ERR_ARG;
     }
-    if (!s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
+    ensure_mixed(s);
   }
   if (fields_grow_trace(s->f, std::max(opt.max_iter, 1))) driver_invalidate(s->drv);
   Job& J = s->js.host[0];
@@ -134,7 +147,7 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
   J.fwd_out = d_fwd;
   J.adj_out = d_adj;
   float ms = 0.f;
-  if (int rc = driver_solve(s->drv, s->js, &s->plan, &s->mplan, opt, &ms)) return rc;
+  if (int rc = driver_solve(s->drv, s->js, &s->plan, opt, &ms)) return rc;
   OTM_HIP(hipMemcpy(s->h_state, s->f.state, sizeof(SolveState), hipMemcpyDeviceToHost));
   const SolveState& hs = *s->h_state;
   s->last_iters = hs.it;
@@ -289,7 +302,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   }
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
-  if (mixed && !s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
+  if (mixed) ensure_mixed(s);
   // A scratch state keeps every gate open: iteration 1, conjugate-Jacobian
   // beta, update pending.  Intermediate buffers of the last solve are reused
   // as inputs and overwritten.
@@ -322,8 +335,8 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
       case OTM_KERNEL_DIR_COEF: launch_dir_coef(pj, s->drv.stream); break;
       case OTM_KERNEL_UPDATE: launch_update(pj, s->drv.stream); break;
       case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(pj, &s->plan, s->drv.stream); break;
-      case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(s->mplan, s->plan, s->drv.stream); break;
-      default: launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, 1, s->drv.stream); break;
+      case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(pj, s->drv.stream); break;
+      default: launch_mixed_solve(pj, 1, true, s->drv.stream); break;
     }
   };
   launch();  // warm-up
@@ -404,7 +417,7 @@ int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int
   if (refine_steps < 0 || refine_steps > 8) { set_error("refine_steps must be in [0, 8]"); return OTM_ERR_ARG; }
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
-  if (!s->mplan.C32) mixed_plan_create(s->mplan, s->plan);
+  ensure_mixed(s);
   const Fields& f = s->f;
   double sum = 0.;
   for (long k = 0; k < f.F; ++k) sum += r[k];
@@ -413,7 +426,7 @@ int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int
   st.sum_r = sum;
   OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->drv.stream));
   OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
-  launch_mixed_solve(s->mplan, s->plan, f, f.r, f.dhat, refine_steps, s->drv.stream);
+  launch_mixed_solve(s->js, refine_steps, true, s->drv.stream);
   OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
   OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;

@@ -823,10 +823,13 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs) {
       if (kind == KIND_STENCIL) w = num_tiles(n);
       else if (kind == KIND_FLAT) w = flat_blocks(jobs[p].f.F);
       else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
+      else if (kind < KIND_GEMM0) w = mixed_supported(n) ? mixed_tiles(kind - KIND_M0, n) : 0;
       else {
         const int g = kind - KIND_GEMM0, stage = g / NFOLD, c = g % NFOLD;
         w = fold_cfg_valid(n, c) ? fold_tiles(n, stage, c) : 0;
       }
+      // GEMM launches start every job on a multiple of 8 workgroups (XCD-aware tile order)
+      if (kind >= KIND_M0) w = (w + 7) / 8 * 8;
       acc += w;
     }
     st[size_t(kind) * C1 + js.count] = acc;

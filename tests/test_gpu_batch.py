@@ -97,6 +97,22 @@ def test_batch_rejects_bad_input():
         BatchSolver([100])
     b = BatchSolver([64])
     with pytest.raises(OtmError):
-        b.solve([np.ones(64 * 64)], SolverOptions(precision=Precision.MIXED))
+        b.solve([np.ones(64 * 64)], SolverOptions(precision=Precision.MIXED, refine_steps=9))
     cfg, count = b.gemm_config()
     assert count > 0 and all(0 <= c < count for c in cfg)
+
+
+def test_batch_mixed_bit_identical_to_single_mixed(oracle):
+    sizes = [192, 128, 256]
+    dens = [D.gaussian_mixture(n, 70 + k) for k, n in enumerate(sizes)]
+    opt = SolverOptions(precision=Precision.MIXED)
+    print("mixed batch sizes", sizes, flush=True)
+    b = BatchSolver(sizes)
+    maps = b.solve(dens, opt)
+    for k, n in enumerate(sizes):
+        tm, st, _ = single(n, dens[k], opt)
+        assert b.last_stats[k].iterations == st.iterations, (k, n)
+        assert np.array_equal(maps[k].fwd_mesh().xy, tm.fwd_mesh().xy), (k, n)
+        o = oracle.solve(n, dens[k])
+        assert abs(st.iterations - o["iters"]) <= 1
+        assert np.abs(tm.fwd_mesh().xy - o["fwd"]).max() < 1e-6
