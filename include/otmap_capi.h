@@ -170,6 +170,13 @@ int otm_map_inverse(otm_map* m, const double* pts, int64_t count, int mode, doub
 int otm_map_inverse_device(otm_map* m, const double* d_pts, int64_t count, int mode, double* d_out,
                            int32_t* d_face, int32_t* d_nhits);
 
+/* Inverse images of the origin grid vertices (i/n, j/n), in vertex order,
+ * SoA: the inverse map as a mesh, as the reference builds it by applying
+ * apply_inverse_map to a copy of origin_mesh() (apps/barycenters.cpp:187-190).
+ * mode 0 = inv, 1 = inv_fast; NaN where no face holds the point. */
+int otm_map_inverse_vertices(otm_map* m, int mode, double* xy);
+int otm_map_inverse_vertices_device(otm_map* m, int mode, double* d_xy);
+
 /* ---- rasterize_image (otlib/utils/rasterizer.h:21-22, rasterizer.cpp:188-308)
  * Per-face density mode on the (n+1)^2-vertex quad mesh `mesh_xy` (SoA):
  * every pixel covered by a face gets float(1/n^2) * density(f) / |area(f)|,
@@ -181,6 +188,41 @@ int otm_rasterize(int device, int n, const double* mesh_xy, const double* densit
  * rows*cols int32 scratch (receives the winning face index, -1 if none). */
 int otm_rasterize_device(int device, int n, const double* d_mesh_xy, const double* d_density, int rows, int cols,
                          int img_f32, void* d_img, int32_t* d_winner);
+
+/* ---- Transport cost ----------------------------------------------------
+ * transport_cost(src_mesh, dst_mesh, density_per_face, cost_per_face)
+ * (otlib/transport_map.cpp:131-215) on (n+1)^2-vertex quad grids (SoA
+ * positions): *total = sum(cost) / integral; cost (nullable) per face,
+ * normalised by the same integral.  As in the reference, each quad is costed
+ * through the triangle of its first halfedge (vertices 3, 0, 1). */
+int otm_transport_cost(int device, int n, const double* src_xy, const double* dst_xy, const double* density,
+                       double* cost, double* total);
+int otm_transport_cost_device(int device, int n, const double* d_src_xy, const double* d_dst_xy,
+                              const double* d_density, double* d_cost, double* total);
+/* compute_transport_cost (otlib/otsolver_2dgrid.cpp:608-640): 2 x 2 Gauss
+ * points of |displacement|^2 per face times density * A, for vertex
+ * displacements disp_xy (SoA, 2*(n+1)^2); *total = sum of cost. */
+int otm_grid_transport_cost(int device, int n, const double* disp_xy, const double* density, double* cost,
+                            double* total);
+
+/* ---- Pipeline helpers (device buffers; no reference C++ API: the steps of
+ * the reference's barycenters and stippling apps) --------------------------
+ * otm_mesh_lerp_device: out = lerp (k = 2: (1-u) m0 + u m1) or bilerp (k = 4:
+ *   (1-u)(1-v) m0 + u(1-v) m1 + uv m2 + (1-u)v m3) of `count` doubles,
+ *   apps/barycenters.cpp:128-140, 254-277.
+ * otm_image_normalize_mean_device: out = img * (expected_mean / mean(img)),
+ *   written as float (out_f32 != 0) or double; *mean (nullable) = mean(img),
+ *   apps/barycenters.cpp:280-289.
+ * otm_prune_points_device: keeps, in order, the interleaved points whose
+ *   pixel (floor(p * n)) has non-zero density (face order), or that are not
+ *   NaN; *kept = number written to d_out (room for `count` points),
+ *   apps/stippling.cpp:188-196 (pixel indices are clamped to the grid). */
+int otm_mesh_lerp_device(int device, long count, int k, const double* const* d_meshes, double u, double v,
+                         double* d_out);
+int otm_image_normalize_mean_device(int device, long count, const double* d_img, double expected_mean, int out_f32,
+                                    void* d_out, double* mean);
+int otm_prune_points_device(int device, int n, const double* d_density, const double* d_pts, int64_t count,
+                            double* d_out, int64_t* kept);
 
 /* ---- Batched solves -----------------------------------------------------
  * No reference counterpart: the reference solves one density per

@@ -67,6 +67,14 @@ _SIGNATURES = {
     "otm_map_inverse_device": (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _vp, _vp, _vp]),
     "otm_rasterize": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]),
     "otm_rasterize_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "otm_map_inverse_vertices": (C.c_int, [_vp, C.c_int, _dp]),
+    "otm_map_inverse_vertices_device": (C.c_int, [_vp, C.c_int, _vp]),
+    "otm_transport_cost": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    "otm_transport_cost_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _dp]),
+    "otm_grid_transport_cost": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "otm_mesh_lerp_device": (C.c_int, [C.c_int, C.c_long, C.c_int, C.POINTER(_vp), C.c_double, C.c_double, _vp]),
+    "otm_image_normalize_mean_device": (C.c_int, [C.c_int, C.c_long, _vp, C.c_double, C.c_int, _vp, _dp]),
+    "otm_prune_points_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int64, _vp, C.POINTER(C.c_int64)]),
     "otm_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _ip]),
     "otm_batch_destroy": (C.c_int, [_vp]),
     "otm_batch_size": (C.c_int, [_vp]),

@@ -30,6 +30,7 @@ SOURCES = [
     "dct_mixed.hip",
     "driver.hip",
     "batch.hip",
+    "pipeline.hip",
 ]
 
 FLAGS = [

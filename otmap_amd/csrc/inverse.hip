@@ -154,20 +154,18 @@ __device__ inline bool box_contains(const BNode& b, P2 p) {
   return b.lox <= p.x && b.loy <= p.y && p.x <= b.hix && p.y <= b.hiy;
 }
 
-// One query per lane.  mode 0: inv (all hits, reference selection), 1: inv_fast (first hit).
-__global__ __launch_bounds__(256) void k_inverse(const BNode* __restrict__ nodes, const int* __restrict__ faces,
-                                                const double* __restrict__ xy, const double* __restrict__ density,
-                                                int n, const double* __restrict__ pts, long m, int fast,
-                                                double* __restrict__ out, int* __restrict__ face_out,
-                                                int* __restrict__ nhits_out) {
-  const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= m) return;
+// Inverse image of one point q (transport_map.cpp:50-102).  fast: first hit
+// (inv_fast), else every hit with the reference's selection.  Returns the
+// number of hits; *face the selected face (-1 if none); (rx, ry) its
+// bilinear interpolation of the origin positions (NaN if none).
+__device__ inline int inv_query(const BNode* __restrict__ nodes, const int* __restrict__ faces,
+                                const double* __restrict__ xy, const double* __restrict__ density, int n, P2 q,
+                                int fast, double& rx, double& ry, int& face) {
   const long V = long(n + 1) * (n + 1);
-  P2 q = {pts[2 * k], pts[2 * k + 1]};
   q.x = fmin(fmax(q.x, 0.), 1.);  // snap to [0,1]^2 (transport_map.cpp:53)
   q.y = fmin(fmax(q.y, 0.), 1.);
   int hits = 0;
-  int first_face = -1, chosen = -1;
+  int chosen = -1;
   double w0[4] = {0., 0., 0., 0.};
   double best = 0.;
   // invariant: `node` holds q (or is -1 = traversal finished)
@@ -191,7 +189,6 @@ __global__ __launch_bounds__(256) void k_inverse(const BNode* __restrict__ nodes
         if (!bilinear_weights(q, p, w)) continue;
         if (!weights_in_unit(w)) continue;
         if (hits == 0) {
-          first_face = f;
           chosen = f;
           for (int t = 0; t < 4; ++t) w0[t] = w[t];
           best = density[f];
@@ -209,18 +206,15 @@ __global__ __launch_bounds__(256) void k_inverse(const BNode* __restrict__ nodes
     while (next >= 0 && !box_contains(nodes[next], q)) next = nodes[next].escape;
     node = next;
   }
-  if (nhits_out) nhits_out[k] = hits;
-  if (face_out) face_out[k] = chosen;
+  face = chosen;
   if (hits == 0) {
-    out[2 * k] = __builtin_nan("");
-    out[2 * k + 1] = __builtin_nan("");
-    return;
+    rx = __builtin_nan("");
+    ry = __builtin_nan("");
+    return 0;
   }
-  (void)first_face;
   int v[4];
   quad_vertices(n, chosen, v);
   const double dx = 1. / double(n);
-  double rx = 0., ry = 0.;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int vi = v[t] / (n + 1), vj = v[t] - vi * (n + 1);
@@ -228,8 +222,43 @@ __global__ __launch_bounds__(256) void k_inverse(const BNode* __restrict__ nodes
     if (t == 0) { rx = w0[0] * ox; ry = w0[0] * oy; }
     else { rx += w0[t] * ox; ry += w0[t] * oy; }
   }
+  return hits;
+}
+
+// One query per lane over interleaved points.
+__global__ __launch_bounds__(256) void k_inverse(const BNode* __restrict__ nodes, const int* __restrict__ faces,
+                                                const double* __restrict__ xy, const double* __restrict__ density,
+                                                int n, const double* __restrict__ pts, long m, int fast,
+                                                double* __restrict__ out, int* __restrict__ face_out,
+                                                int* __restrict__ nhits_out) {
+  const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  double rx, ry;
+  int face;
+  const int hits = inv_query(nodes, faces, xy, density, n, P2{pts[2 * k], pts[2 * k + 1]}, fast, rx, ry, face);
+  if (nhits_out) nhits_out[k] = hits;
+  if (face_out) face_out[k] = face;
   out[2 * k] = rx;
   out[2 * k + 1] = ry;
+}
+
+// Inverse images of the origin grid vertices (i/n, j/n) in vertex order, SoA
+// output: the inverse map as a mesh (apps/barycenters.cpp:187-190).
+__global__ __launch_bounds__(256) void k_inverse_vertices(const BNode* __restrict__ nodes,
+                                                         const int* __restrict__ faces,
+                                                         const double* __restrict__ xy,
+                                                         const double* __restrict__ density, int n, int fast,
+                                                         double* __restrict__ out) {
+  const long V = long(n + 1) * (n + 1);
+  const long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const int vi = int(v / (n + 1)), vj = int(v - long(vi) * (n + 1));
+  const double dx = 1. / double(n);
+  double rx, ry;
+  int face;
+  inv_query(nodes, faces, xy, density, n, P2{double(vi) * dx, double(vj) * dx}, fast, rx, ry, face);
+  out[v] = rx;
+  out[V + v] = ry;
 }
 
 }  // namespace
@@ -406,6 +435,51 @@ int otm_map_inverse_device(otm_map* m, const double* d_pts, int64_t count, int m
     launch_inverse(m, d_pts, count, mode, d_out, d_face, d_nhits);
     OTM_HIP(hipStreamSynchronize(m->stream));
     return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+
+int otm_map_inverse_vertices_device(otm_map* m, int mode, double* d_xy) {
+  if (!m || !d_xy || (mode != 0 && mode != 1)) {
+    set_error("invalid inverse arguments");
+    return OTM_ERR_ARG;
+  }
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    map_build(m);
+    const long V = long(m->n + 1) * (m->n + 1);
+    hipLaunchKernelGGL(k_inverse_vertices, dim3(unsigned((V + 255) / 256)), dim3(256), 0, m->stream,
+                       (const BNode*)m->nodes, (const int*)m->faces, (const double*)m->xy, (const double*)m->density,
+                       m->n, mode, d_xy);
+    OTM_HIP(hipGetLastError());
+    OTM_HIP(hipStreamSynchronize(m->stream));
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
+
+int otm_map_inverse_vertices(otm_map* m, int mode, double* xy) {
+  if (!m || !xy) {
+    set_error("invalid inverse arguments");
+    return OTM_ERR_ARG;
+  }
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    const long V = long(m->n + 1) * (m->n + 1);
+    double* d = nullptr;
+    OTM_HIP(hipMalloc(&d, 2 * V * sizeof(double)));
+    int rc = otm_map_inverse_vertices_device(m, mode, d);
+    if (rc == OTM_OK) {
+      const hipError_t e = hipMemcpy(xy, d, 2 * V * sizeof(double), hipMemcpyDeviceToHost);
+      (void)hipFree(d);
+      OTM_HIP(e);
+    } else {
+      (void)hipFree(d);
+    }
+    return rc;
   } catch (const HipError& e) {
     return map_fail(e);
   }

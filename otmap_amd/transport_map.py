@@ -64,6 +64,28 @@ class GridMesh:
     def copy(self) -> "GridMesh":
         return GridMesh(self.n, self.xy.copy())
 
+    @classmethod
+    def read(cls, path: str) -> "GridMesh":
+        """Reads an OFF file written by write() (or by the reference for a
+        grid map, otlib/surface_mesh/IO_off.cpp): the faces must be the
+        regular grid's quads in face order."""
+        with open(path) as fh:
+            tokens = fh.read().split()
+        if not tokens or tokens[0] != "OFF":
+            raise ValueError(f"{path}: not an OFF file")
+        nv, nf = int(tokens[1]), int(tokens[2])
+        n = int(round(np.sqrt(nf)))
+        if n * n != nf or (n + 1) ** 2 != nv:
+            raise ValueError(f"{path}: {nv} vertices / {nf} faces is not an n x n quad grid")
+        pos = np.asarray(tokens[4:4 + 3 * nv], dtype=np.float64).reshape(nv, 3)
+        faces = np.asarray(tokens[4 + 3 * nv:4 + 3 * nv + 5 * nf], dtype=np.int64).reshape(nf, 5)
+        mesh = cls(n, np.concatenate([pos[:, 0], pos[:, 1]]))
+        i, j = np.divmod(np.arange(nf), n)
+        expect = np.stack([j + i * (n + 1), j + (i + 1) * (n + 1), j + 1 + (i + 1) * (n + 1), j + 1 + i * (n + 1)], 1)
+        if not (np.all(faces[:, 0] == 4) and np.array_equal(faces[:, 1:], expect)):
+            raise ValueError(f"{path}: faces are not the regular grid's quads")
+        return mesh
+
     def write(self, path: str) -> None:
         """OFF file with z = 0, "%.10f" coordinates (otlib/surface_mesh/IO_off.cpp)."""
         n = self.n
@@ -131,6 +153,16 @@ class TransportMap:
                                               _lib.iptr(face), _lib.iptr(nh)), "inverse")
         return out.reshape(m, 2), face, nh
 
+    def inverse_mesh(self, fast: bool = False) -> GridMesh:
+        """The inverse map as a mesh: inverse images of the origin grid
+        vertices (apply_inverse_map on origin_mesh().points(),
+        apps/barycenters.cpp:187-190)."""
+        n = self._fwd.n
+        xy = np.empty(2 * (n + 1) ** 2)
+        _lib.check(_lib.lib().otm_map_inverse_vertices(self._handle(), 1 if fast else 0, _lib.ptr(xy)),
+                   "inverse_vertices")
+        return GridMesh(n, xy)
+
     def inv(self, p):
         """otlib/transport_map.h:44"""
         return self.inverse(np.asarray(p, dtype=np.float64).reshape(1, 2))[0][0]
@@ -157,3 +189,29 @@ def rasterize_image(mesh: GridMesh, density=None, shape=None, device: int = 0):
     _lib.check(_lib.lib().otm_rasterize(device, mesh.n, _lib.ptr(mesh.xy), _lib.ptr(dens), int(rows), int(cols),
                                         _lib.ptr(img)), "rasterize_image")
     return img.reshape(rows, cols)
+
+
+def transport_cost(src_mesh: GridMesh, dst_mesh: GridMesh, density, per_face: bool = False, device: int = 0):
+    """otlib/transport_map.cpp:131-215 on quad grid meshes: the total cost, and
+    with per_face the per-face costs (normalised as in the reference)."""
+    n = src_mesh.n
+    if dst_mesh.n != n:
+        raise ValueError("meshes of different sizes")
+    dens = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+    cost = np.empty(n * n) if per_face else None
+    total = _lib.C.c_double()
+    _lib.check(_lib.lib().otm_transport_cost(device, n, _lib.ptr(src_mesh.xy), _lib.ptr(dst_mesh.xy), _lib.ptr(dens),
+                                             _lib.ptr(cost), _lib.C.byref(total)), "transport_cost")
+    return (total.value, cost) if per_face else total.value
+
+
+def grid_transport_cost(n: int, displacement, density, device: int = 0):
+    """compute_transport_cost (otlib/otsolver_2dgrid.cpp:608-640): per-face
+    cost and total for vertex displacements (SoA, 2 (n+1)^2)."""
+    disp = np.ascontiguousarray(displacement, dtype=np.float64).reshape(-1)
+    dens = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+    cost = np.empty(n * n)
+    total = _lib.C.c_double()
+    _lib.check(_lib.lib().otm_grid_transport_cost(device, n, _lib.ptr(disp), _lib.ptr(dens), _lib.ptr(cost),
+                                                  _lib.C.byref(total)), "grid_transport_cost")
+    return total.value, cost

@@ -52,3 +52,16 @@ def test_cpp_dropin_headers_build_and_link():
     assert os.path.exists(exe), "run __graft_entry__.build() first"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=30)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_off_round_trip(tmp_path):
+    import numpy as np
+
+    from otmap_amd import GridMesh
+
+    n = 5
+    m = GridMesh(n, np.random.default_rng(0).uniform(0, 1, 2 * (n + 1) ** 2))
+    path = str(tmp_path / "m.off")
+    m.write(path)
+    r = GridMesh.read(path)
+    assert r.n == n and np.abs(r.xy - m.xy).max() <= 5e-11
