@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--inverse-points", type=int, default=1 << 20)
     ap.add_argument("--batch", type=int, default=64, help="densities of the batched workload (0: skip)")
+    ap.add_argument("--no-pipelines", action="store_true", help="skip the barycentre and stippling workloads")
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
                              "mixed_solve", "copy"],
@@ -185,6 +186,79 @@ def batched_throughput(count, steps, warmup):
             "create_s": round(t_create, 3), "precision": "fp64", "gemm_tiles": b.gemm_config()[0]}
 
 
+def barycentre_pipeline(n=512, steps=4):
+    """Four seeded smoothed-shape densities at 512^2 solved as one batch
+    (FP64), the inverse map of each as a mesh, 5 x 5 bilinear interpolations,
+    each rasterised to 512^2 and mean-normalised to FP32."""
+    import torch
+
+    from otmap_amd import BatchSolver, pipelines
+    from otmap_amd import densities as D
+
+    dens = D.barycentre_inputs(n)
+    means = [float(d.mean()) for d in dens]
+    b = BatchSolver([n] * len(dens))
+    b.solve(dens)  # warm (graph capture, tuning)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    maps = b.solve(dens)
+    t_solve = time.perf_counter() - t0
+    pipelines.barycentres(maps, steps, n, means)  # warm the BVH builds
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    imgs = pipelines.barycentres(maps, steps, n, means)
+    torch.cuda.synchronize()
+    t_interp = time.perf_counter() - t0
+    return {"densities": len(dens), "n": n, "images": len(imgs), "solve_ms": round(t_solve * 1e3, 2),
+            "inverse_interpolate_raster_ms": round(t_interp * 1e3, 2),
+            "total_ms": round((t_solve + t_interp) * 1e3, 2),
+            "iterations": [s.iterations for s in b.last_stats], "image_dtype": "f32"}
+
+
+def stippling_pipeline(n=2048, points=1 << 20):
+    """2048^2 log-normal x Gaussian-mixture density (contrast capped at 1e4 by
+    max_ratio), mixed-precision solve, then 1M uniform points advected through
+    the inverse map and pruned on zero density; FP64 solve beside it for the
+    map difference."""
+    import torch
+
+    from otmap_amd import GridBasedTransportSolver, GridMesh, Precision, SolverOptions, TransportMap, pipelines
+    from otmap_amd import densities as D
+
+    rho = D.hdr_lognormal(n)
+    dev = torch.device("cuda:0")
+    d_rho = torch.from_numpy(rho).to(dev)
+    V = (n + 1) ** 2
+    d_fwd = {p: torch.empty(2 * V, dtype=torch.float64, device=dev) for p in ("fp64", "mixed")}
+    d_adj = torch.empty(n * n, dtype=torch.float64, device=dev)
+    s = GridBasedTransportSolver(0)
+    s.set_verbose_level(0)
+    s.init(n)
+    out = {"n": n, "density": "log-normal (sigma 2, corr 16 px) x Gaussian mixture, max_ratio 1e4, floor 1e-4"}
+    for name, prec in (("mixed", Precision.MIXED), ("fp64", Precision.FP64)):
+        opt = SolverOptions(precision=prec, max_ratio=1e4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = s.solve_device(d_rho.data_ptr(), d_fwd[name].data_ptr(), d_adj.data_ptr(), opt)
+        torch.cuda.synchronize()
+        out[f"{name}_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        out[f"{name}_iterations"] = st.iterations
+    out["max_vertex_diff_mixed_vs_fp64"] = float((d_fwd["mixed"] - d_fwd["fp64"]).abs().max().item())
+    tm = TransportMap(GridMesh.regular(n), GridMesh(n, d_fwd["mixed"].cpu().numpy()), d_adj.cpu().numpy())
+    pts = D.uniform_points(points, seed=2000 + n)
+    t0 = time.perf_counter()
+    tm.init_inverse()
+    t_bvh = time.perf_counter() - t0
+    pipelines.stippling(tm, rho, pts)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kept = pipelines.stippling(tm, rho, pts)
+    torch.cuda.synchronize()
+    out.update({"points": points, "kept": int(kept.shape[0]), "bvh_build_ms": round(t_bvh * 1e3, 2),
+                "advect_prune_ms": round((time.perf_counter() - t0) * 1e3, 2)})
+    return out
+
+
 def cpu_baseline(n, density):
     """The compiled reference (oracle/_ref/libotref.so), one solve, 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -255,6 +329,8 @@ def main():
     # host (once per map, as init_inverse), then one query per lane
     inv_stats = inverse_throughput(n, d_fwd, d_adj, args.inverse_points, args.steps)
     batch_stats = batched_throughput(args.batch, args.steps, args.warmup) if args.batch > 0 else None
+    bary = None if args.no_pipelines else barycentre_pipeline()
+    stipple = None if args.no_pipelines else stippling_pipeline()
 
     # parity against the reference's golden positions when they exist for this workload
     err = None
@@ -345,6 +421,8 @@ def main():
         "max_vertex_err_vs_reference": err,
         "inverse_1M": inv_stats,
         "batched": batch_stats,
+        "barycentre_4x512": bary,
+        "stippling_2048": stipple,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "roofline": roof_x3 if head == "mixed" else roof_f64,
         "roofline_fp64_gemm": roof_f64,
