@@ -127,6 +127,17 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
 int otm_solver_gemm_config(const otm_solver* s, int* cfg);
 int otm_solver_set_gemm_config(otm_solver* s, int stage, int config);
 
+/* Tile widths of the four mixed-precision DCT stages (no reference
+ * counterpart): config c uses workgroup tiles of 64 x (32 << c) outputs.  The
+ * first mixed solve times each valid width per stage and keeps the fastest;
+ * every width gives bit-identical results.
+ *   otm_solver_mixed_config: cfg[0..3] = chosen width per stage, returns the
+ *     number of candidate widths (0 before the first mixed solve or when n is
+ *     not a multiple of 64);
+ *   otm_solver_set_mixed_config: force width `config` on `stage` (0..3). */
+int otm_solver_mixed_config(const otm_solver* s, int* cfg);
+int otm_solver_set_mixed_config(otm_solver* s, int stage, int config);
+
 /* Component entry points: the reference's protected stencils, on host
  * buffers, for parity testing and for callers that drive the iteration.
  *   compute_vertex_gradients (otlib/otsolver_2dgrid.cpp:362-423): grads 2*(n+1)^2

@@ -53,6 +53,8 @@ _SIGNATURES = {
     "otm_solver_profile_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "otm_solver_gemm_config": (C.c_int, [_vp, _ip]),
     "otm_solver_set_gemm_config": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "otm_solver_mixed_config": (C.c_int, [_vp, _ip]),
+    "otm_solver_set_mixed_config": (C.c_int, [_vp, C.c_int, C.c_int]),
     "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
     "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),

@@ -29,6 +29,7 @@ struct otm_batch {
   Driver drv;
   std::vector<double*> d_in, d_xy;  // staging of host-API buffers
   int trace_cap = 0;
+  bool mixed_tuned = false;  // mixed tile widths picked (first mixed solve)
 };
 
 namespace {
@@ -67,12 +68,18 @@ void build_table(otm_batch* b) {
       jobs[p].Ptp = it->second.Ptp;
     }
   }
-  int cfg[4];
+  int cfg[4], mcfg[4];
   const bool had = b->js.jobs != nullptr;
-  if (had) std::memcpy(cfg, b->js.cfg, sizeof(cfg));
+  if (had) {
+    std::memcpy(cfg, b->js.cfg, sizeof(cfg));
+    std::memcpy(mcfg, b->js.mcfg, sizeof(mcfg));
+  }
   jobset_destroy(b->js);
   jobset_create(b->js, jobs);
-  if (had) std::memcpy(b->js.cfg, cfg, sizeof(cfg));
+  if (had) {
+    std::memcpy(b->js.cfg, cfg, sizeof(cfg));
+    std::memcpy(b->js.mcfg, mcfg, sizeof(mcfg));
+  }
 }
 
 int solve_batch(otm_batch* b, const double* const* d_density, const otm_options* opt_in, double* const* d_fwd,
@@ -101,8 +108,14 @@ int solve_batch(otm_batch* b, const double* const* d_density, const otm_options*
     for (auto& kv : b->plans) mixed_plan_create(b->mplans[kv.first], kv.second);
     rebuild = true;
   }
+  const bool tune_mixed = opt.precision == OTM_PRECISION_MIXED && b->mplans.size() && !b->mixed_tuned;
   if (rebuild) {
     build_table(b);
+    driver_invalidate(b->drv);
+  }
+  if (tune_mixed) {
+    mixed_tune(b->js, b->device, b->drv.stream);
+    b->mixed_tuned = true;
     driver_invalidate(b->drv);
   }
   for (int p = 0; p < b->count; ++p) {

@@ -26,7 +26,10 @@
 
 #include <cfloat>
 #include <cmath>
+#include <array>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "otm_internal.h"
@@ -42,10 +45,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int MTM = 64;        // A rows per workgroup tile
-constexpr int MTN = 32;        // B rows (output columns) per workgroup tile
 constexpr int MKB = 32;        // K step
 constexpr int MLD = MKB + 8;   // LDS row length (u16): 80 B rows keep 16-B fragment reads aligned
-constexpr int DLD = MTN + 1;   // LDS row length (f32) of a staged output tile
 
 enum MStage { M1 = 0, M2 = 1, M3 = 2, M4 = 3 };
 // M4 epilogue: x0 = D | d_hat = D (+partials) | d_hat = x0 + D (+partials) | x0 += D
@@ -68,11 +69,6 @@ __device__ inline void split3d(double x, u16& h0, u16& h1, u16& h2) {
   const double r1 = x - double(bf16_float(h0));
   h1 = bf16_bits(float(r1));
   h2 = bf16_bits(float(r1 - double(bf16_float(h1))));
-}
-
-__host__ __device__ inline int mtiles_n(int st, int n) {
-  const int h = n / 2;
-  return st == M1 ? n / MTN : (st == M2 ? n / MTN : h / MTN);
 }
 
 // ---------------------------------------------------------------------------
@@ -118,9 +114,12 @@ __global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, cons
 }
 
 // ---------------------------------------------------------------------------
-// one stage of the folded transform for every job
+// one stage of the folded transform for every job.  Workgroup tile: 64 rows
+// of A (M1: 32 rows and their 32 mirrors) by TN rows of B (TN = 32 or 64,
+// picked per stage by the autotuner), K steps of 32, four waves of
+// 32 x TN/2.  All tile widths accumulate in the same order (bit-identical).
 // ---------------------------------------------------------------------------
-template <int ST>
+template <int ST, int TN>
 __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, const int* __restrict__ starts, int count,
                                              bool gate, int fin) {
   const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
@@ -132,7 +131,10 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   const int n = J.f.n, h = n / 2;
   const long nh = long(n) * h, nn = long(n) * n;
   constexpr bool SPLIT = ST == M3 || ST == M4;
-  const int ntn = mtiles_n(ST, n);
+  constexpr int TJ = TN / 32;                   // MFMA tiles per wave along B
+  constexpr int BCHUNKS = 3 * TN * 4;           // 16-byte B chunks per K step
+  constexpr int BCH = (BCHUNKS + NT - 1) / NT;  // per thread
+  const int ntn = (ST == M1 || ST == M2 ? n : h) / TN;
   const int ntm = ST == M1 ? h / (MTM / 2) : n / MTM;
   if (lt >= ntm * ntn) {  // padding workgroup of the job: neutral partial only
     if (ST == M4 && (fin == FIN_OUT || fin == FIN_ADD_OUT)) {
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   }
   int tm, tn;
   xcd_tile(lt, ntm, ntn, tm, tn);
-  const int m0 = tm * (ST == M1 ? MTM / 2 : MTM), n0 = tn * MTN;
+  const int m0 = tm * (ST == M1 ? MTM / 2 : MTM), n0 = tn * TN;
   // operands (u16 planes): A rows x lda, plane stride aps; B rows x ldb, plane stride bps
   const u16* Ab;
   const u16* Bb;
@@ -166,20 +168,27 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  __shared__ __attribute__((aligned(16))) u16 As[2][3][MTM][MLD];
-  __shared__ __attribute__((aligned(16))) u16 Bs[2][3][MTN][MLD];
+  // LDS: A planes [2][3][64][MLD], B planes [2][3][TN][MLD]; the epilogue
+  // reuses the space for one or two FP32 tiles [64][TN + 1]
+  constexpr int A_U16 = 2 * 3 * MTM * MLD, B_U16 = 2 * 3 * TN * MLD;
+  __shared__ __attribute__((aligned(16))) u16 smem[A_U16 + B_U16];
+  static_assert((A_U16 + B_U16) * 2 >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
+  auto As = [&](int buf, int pl, int row) { return smem + ((buf * 3 + pl) * MTM + row) * MLD; };
+  auto Bs = [&](int buf, int pl, int row) { return smem + A_U16 + ((buf * 3 + pl) * TN + row) * MLD; };
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 16;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * (TN / 2);
   const int lr = lane & 15, lq = lane >> 4;
 
-  // global -> register staging: A 3 x 64 rows x 4 chunks of 8 (3 per thread), B 3 x 32 x 4 (1.5 per thread)
-  // two register slots of K-step loads in flight, double-buffered LDS
-  u32x4 ra[2][3], rb[2][2];
+  // global -> register staging, two slots of K steps in flight.  A: 3 x 64
+  // rows x 4 chunks of 8 (3 per thread); B: 3 x TN x 4 (chunk indices past the
+  // tile reload an earlier chunk, never stored, so every lane issues the same
+  // loads and the waits on the slots stay exact)
+  u32x4 ra[2][3], rb[2][BCH];
   auto arow = [&](int row) {  // global row of tile row `row`
     if (ST == M1) return row < MTM / 2 ? m0 + row : n - m0 - MTM + row;  // rows i and mirrored n-1-i
     return m0 + row;
   };
-  auto gload = [&](u32x4 (&xa)[3], u32x4 (&xb)[2], int k0) {
+  auto gload = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int k0) {
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
       const int c = t + u * NT;
@@ -187,39 +196,37 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
       xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + pl * aps + long(arow(row)) * lda + k0 + kc);
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      // 384 B chunks over 512 slots: the last 128 slots reload chunks 256..383
-      // (never stored), so every lane issues the same loads and the waits on
-      // the ring stay exact
-      const int c0 = t + u * NT, c = c0 < 3 * MTN * 4 ? c0 : c0 - MTN * 4;
-      const int pl = c >> 7, rem = c & 127, row = rem >> 2, kc = (rem & 3) * 8;
+    for (int u = 0; u < BCH; ++u) {
+      const int c0 = t + u * NT, c = c0 < BCHUNKS ? c0 : c0 - TN * 4;
+      const int pl = c / (TN * 4), rem = c % (TN * 4), row = rem >> 2, kc = (rem & 3) * 8;
       xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + long(n0 + row) * ldb + k0 + kc);
     }
   };
-  auto sstore = [&](const u32x4 (&xa)[3], const u32x4 (&xb)[2], int buf) {
+  auto sstore = [&](const u32x4 (&xa)[3], const u32x4 (&xb)[BCH], int buf) {
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
       const int c = t + u * NT;
       const int pl = c >> 8, rem = c & 255, row = rem >> 2, kc = (rem & 3) * 8;
-      *reinterpret_cast<u32x4*>(&As[buf][pl][row][kc]) = xa[u];
+      *reinterpret_cast<u32x4*>(As(buf, pl, row) + kc) = xa[u];
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < BCH; ++u) {
       const int c = t + u * NT;
-      if (c < 3 * MTN * 4) {
-        const int pl = c >> 7, rem = c & 127, row = rem >> 2, kc = (rem & 3) * 8;
-        *reinterpret_cast<u32x4*>(&Bs[buf][pl][row][kc]) = xb[u];
+      if (BCHUNKS % NT == 0 || c < BCHUNKS) {
+        const int pl = c / (TN * 4), rem = c % (TN * 4), row = rem >> 2, kc = (rem & 3) * 8;
+        *reinterpret_cast<u32x4*>(Bs(buf, pl, row) + kc) = xb[u];
       }
     }
   };
 
-  auto compute = [&](f4 (&c)[2], int buf) {
-    bf8 fa[2][3], fb[3];
+  auto compute = [&](f4 (&c)[2][TJ], int buf) {
+    bf8 fa[2][3], fb[TJ][3];
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(&As[buf][pl][wm + 16 * i + lr][lq * 8]);
-      fb[pl] = *reinterpret_cast<const bf8*>(&Bs[buf][pl][wn + lr][lq * 8]);
+      for (int i = 0; i < 2; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(As(buf, pl, wm + 16 * i + lr) + lq * 8);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) fb[j][pl] = *reinterpret_cast<const bf8*>(Bs(buf, pl, wn + 16 * j + lr) + lq * 8);
     }
     // small products first, the leading one last
     constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
@@ -227,7 +234,10 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
 #pragma unroll
     for (int q = 0; q < 6; ++q)
 #pragma unroll
-      for (int i = 0; i < 2; ++i) c[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[PB[q]], c[i], 0, 0, 0);
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+          c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[j][PB[q]], c[i][j], 0, 0, 0);
   };
 
   // Software pipeline, two K-steps in flight: step s stores register slot
@@ -240,14 +250,23 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   // to acc_lo when step h / MKB starts.
   const int nsteps = K / MKB, hsteps = SPLIT ? h / MKB : -1;
   const int klast = K - MKB;
-  f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
-  f4 acc_lo[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
-  auto step = [&](u32x4 (&xa)[3], u32x4 (&xb)[2], int s, int buf) {
+  f4 acc[2][TJ], acc_lo[2][TJ];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      acc_lo[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  auto step = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int s, int buf) {
     if (SPLIT && s == hsteps) {
-      acc_lo[0] = acc[0];
-      acc_lo[1] = acc[1];
-      acc[0] = f4{0.f, 0.f, 0.f, 0.f};
-      acc[1] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+          acc_lo[i][j] = acc[i][j];
+          acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
     }
     sstore(xa, xb, buf);
     gload(xa, xb, min((s + 2) * MKB, klast));
@@ -263,10 +282,9 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   }
   if (s < nsteps) step(ra[0], rb[0], s, 0);
   __syncthreads();
-  const f4 r0[2] = {SPLIT ? acc_lo[0] : acc[0], SPLIT ? acc_lo[1] : acc[1]};  // k < h (or all k)
-  const f4 r1[2] = {acc[0], acc[1]};                                       // k >= h
 
-  // lane (lr, lq) of MFMA tile i holds D[wm + 16i + 4 lq + v][wn + lr]
+  // lane (lr, lq) of MFMA tile (i, j) holds D[wm + 16i + 4 lq + v][wn + 16j + lr];
+  // r0 = k < h (or all k), r1 = k >= h
   if (ST == M4) {
     const auto x0 = gp(J.f.x0);
     const auto dh = gp(J.f.dhat);
@@ -274,121 +292,141 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const long row = m0 + wm + 16 * i + 4 * lq + v;
-        const int col = n0 + wn + lr;
-        const double p0 = double(r0[i][v]), p1 = double(r1[i][v]);
-        const long ia = row * n + col, ib = row * n + (n - 1 - col);
-        double va = p0 + p1, vb = p0 - p1;
-        if (fin == FIN_ADD_OUT || fin == FIN_ADD_X0) {
-          va = x0[ia] + va;
-          vb = x0[ib] + vb;
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const long row = m0 + wm + 16 * i + 4 * lq + v;
+          const int col = n0 + wn + 16 * j + lr;
+          const double p0 = double(acc_lo[i][j][v]), p1 = double(acc[i][j][v]);
+          const long ia = row * n + col, ib = row * n + (n - 1 - col);
+          double va = p0 + p1, vb = p0 - p1;
+          if (fin == FIN_ADD_OUT || fin == FIN_ADD_X0) {
+            va = x0[ia] + va;
+            vb = x0[ib] + vb;
+          }
+          if (fin == FIN_X0 || fin == FIN_ADD_X0) {
+            x0[ia] = va;
+            x0[ib] = vb;
+          } else {
+            dh[ia] = va;
+            dh[ib] = vb;
+            nrm[0] += va * va + vb * vb;
+          }
         }
-        if (fin == FIN_X0 || fin == FIN_ADD_X0) {
-          x0[ia] = va;
-          x0[ib] = vb;
-        } else {
-          dh[ia] = va;
-          dh[ib] = vb;
-          nrm[0] += va * va + vb * vb;
-        }
-      }
     if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nbk);
     return;
   }
 
-  // the other stages stage the FP32 tile(s) in LDS (reusing the operand
-  // buffers) and write BF16 planes in 8-16 byte runs
-  float* Ds0 = reinterpret_cast<float*>(&As[0][0][0][0]);
-  float* Ds1 = Ds0 + MTM * DLD;
+  // the other stages stage the FP32 tile(s) in LDS and write BF16 planes in
+  // 8-16 byte runs
+  constexpr int DL = TN + 1;
+  float* Ds0 = reinterpret_cast<float*>(smem);
+  float* Ds1 = Ds0 + MTM * DL;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = wm + 16 * i + 4 * lq + v, c = wn + lr;
-      if (ST == M3) {
-        Ds0[r * DLD + c] = r0[i][v] + r1[i][v];
-        Ds1[r * DLD + c] = r0[i][v] - r1[i][v];
-      } else {
-        Ds0[r * DLD + c] = r0[i][v];
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm + 16 * i + 4 * lq + v, c = wn + 16 * j + lr;
+        if (ST == M3) {
+          Ds0[r * DL + c] = acc_lo[i][j][v] + acc[i][j][v];
+          Ds1[r * DL + c] = acc_lo[i][j][v] - acc[i][j][v];
+        } else {
+          Ds0[r * DL + c] = acc[i][j][v];
+        }
       }
-    }
   __syncthreads();
   if (ST == M1) {
     // U2 / W2 [l = n0 + c][i = m0 + a .. a+3]: pair tile row a with its mirror 63 - a
     const auto out = gp(J.f.mwa);
-    const int c = t >> 3, a0 = (t & 7) * 4;
-    u16 u[3][4], wv[3][4];
+    for (int it = t; it < TN * 8; it += NT) {
+      const int c = it >> 3, a0 = (it & 7) * 4;
+      u16 u[3][4], wv[3][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float top = Ds0[(a0 + q) * DLD + c], mir = Ds0[(MTM - 1 - a0 - q) * DLD + c];
-      split3(top + mir, u[0][q], u[1][q], u[2][q]);
-      split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
-    }
-    const long base = long(n0 + c) * h + m0 + a0;
+      for (int q = 0; q < 4; ++q) {
+        const float top = Ds0[(a0 + q) * DL + c], mir = Ds0[(MTM - 1 - a0 - q) * DL + c];
+        split3(top + mir, u[0][q], u[1][q], u[2][q]);
+        split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
+      }
+      const long base = long(n0 + c) * h + m0 + a0;
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-      u32x2 pu, pw;
-      pu.x = u[pl][0] | (unsigned(u[pl][1]) << 16); pu.y = u[pl][2] | (unsigned(u[pl][3]) << 16);
-      pw.x = wv[pl][0] | (unsigned(wv[pl][1]) << 16); pw.y = wv[pl][2] | (unsigned(wv[pl][3]) << 16);
-      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + base) = pu;
-      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + 3 * nh + pl * nh + base) = pw;
+      for (int pl = 0; pl < 3; ++pl) {
+        u32x2 pu, pw;
+        pu.x = u[pl][0] | (unsigned(u[pl][1]) << 16); pu.y = u[pl][2] | (unsigned(u[pl][3]) << 16);
+        pw.x = wv[pl][0] | (unsigned(wv[pl][1]) << 16); pw.y = wv[pl][2] | (unsigned(wv[pl][3]) << 16);
+        *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + base) = pu;
+        *reinterpret_cast<OTM_GLOBAL u32x2*>(out + 3 * nh + pl * nh + base) = pw;
+      }
     }
   } else if (ST == M2) {
     // Z[l = m0 + r][k = n0 + c0 .. c0+7] / lambda(perm k, perm l)
     const auto out = gp(J.f.mwb);
     const auto sq = gp(J.sq);
-    const int r = t >> 2, c0 = (t & 3) * 8;
-    const int lrow = m0 + r;
-    const int lperm = lrow < h ? 2 * lrow : 2 * (lrow - h) + 1;
-    u16 z[3][8];
+    for (int it = t; it < MTM * TN / 8; it += NT) {
+      const int r = it / (TN / 8), c0 = (it % (TN / 8)) * 8;
+      const int lrow = m0 + r;
+      const int lperm = lrow < h ? 2 * lrow : 2 * (lrow - h) + 1;
+      u16 z[3][8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kc = n0 + c0 + q;
-      const int kperm = kc < h ? 2 * kc : 2 * (kc - h) + 1;
-      const int dm = kperm > lperm ? kperm - lperm : lperm - kperm;
-      const double lam = 2. * (sq[dm] + sq[kperm + lperm]);
-      const float val = (kperm == 0 && lperm == 0) ? 0.f : float(double(Ds0[r * DLD + c0 + q]) / lam);
-      split3(val, z[0][q], z[1][q], z[2][q]);
-    }
-    const long base = long(lrow) * n + n0 + c0;
+      for (int q = 0; q < 8; ++q) {
+        const int kc = n0 + c0 + q;
+        const int kperm = kc < h ? 2 * kc : 2 * (kc - h) + 1;
+        const int dm = kperm > lperm ? kperm - lperm : lperm - kperm;
+        const double lam = 2. * (sq[dm] + sq[kperm + lperm]);
+        const float val = (kperm == 0 && lperm == 0) ? 0.f : float(double(Ds0[r * DL + c0 + q]) / lam);
+        split3(val, z[0][q], z[1][q], z[2][q]);
+      }
+      const long base = long(lrow) * n + n0 + c0;
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-      u32x4 pz;
-      pz.x = z[pl][0] | (unsigned(z[pl][1]) << 16); pz.y = z[pl][2] | (unsigned(z[pl][3]) << 16);
-      pz.z = z[pl][4] | (unsigned(z[pl][5]) << 16); pz.w = z[pl][6] | (unsigned(z[pl][7]) << 16);
-      *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + base) = pz;
+      for (int pl = 0; pl < 3; ++pl) {
+        u32x4 pz;
+        pz.x = z[pl][0] | (unsigned(z[pl][1]) << 16); pz.y = z[pl][2] | (unsigned(z[pl][3]) << 16);
+        pz.z = z[pl][4] | (unsigned(z[pl][5]) << 16); pz.w = z[pl][6] | (unsigned(z[pl][7]) << 16);
+        *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + base) = pz;
+      }
     }
   } else {
     // T3[i = n0 + c][l = m0 + r0 .. r0+7] and T3[n-1-i][l]
     const auto out = gp(J.f.mwa);
-    const int c = t >> 3, r0 = (t & 7) * 8;
-    u16 s0[3][8], s1[3][8];
+    for (int it = t; it < TN * 8; it += NT) {
+      const int c = it >> 3, r0 = (it & 7) * 8;
+      u16 s0[3][8], s1[3][8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      split3(Ds0[(r0 + q) * DLD + c], s0[0][q], s0[1][q], s0[2][q]);
-      split3(Ds1[(r0 + q) * DLD + c], s1[0][q], s1[1][q], s1[2][q]);
-    }
-    const long b0 = long(n0 + c) * n + m0 + r0, b1 = long(n - 1 - n0 - c) * n + m0 + r0;
+      for (int q = 0; q < 8; ++q) {
+        split3(Ds0[(r0 + q) * DL + c], s0[0][q], s0[1][q], s0[2][q]);
+        split3(Ds1[(r0 + q) * DL + c], s1[0][q], s1[1][q], s1[2][q]);
+      }
+      const long b0 = long(n0 + c) * n + m0 + r0, b1 = long(n - 1 - n0 - c) * n + m0 + r0;
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-      u32x4 p0, p1;
-      p0.x = s0[pl][0] | (unsigned(s0[pl][1]) << 16); p0.y = s0[pl][2] | (unsigned(s0[pl][3]) << 16);
-      p0.z = s0[pl][4] | (unsigned(s0[pl][5]) << 16); p0.w = s0[pl][6] | (unsigned(s0[pl][7]) << 16);
-      p1.x = s1[pl][0] | (unsigned(s1[pl][1]) << 16); p1.y = s1[pl][2] | (unsigned(s1[pl][3]) << 16);
-      p1.z = s1[pl][4] | (unsigned(s1[pl][5]) << 16); p1.w = s1[pl][6] | (unsigned(s1[pl][7]) << 16);
-      *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b0) = p0;
-      *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b1) = p1;
+      for (int pl = 0; pl < 3; ++pl) {
+        u32x4 p0, p1;
+        p0.x = s0[pl][0] | (unsigned(s0[pl][1]) << 16); p0.y = s0[pl][2] | (unsigned(s0[pl][3]) << 16);
+        p0.z = s0[pl][4] | (unsigned(s0[pl][5]) << 16); p0.w = s0[pl][6] | (unsigned(s0[pl][7]) << 16);
+        p1.x = s1[pl][0] | (unsigned(s1[pl][1]) << 16); p1.y = s1[pl][2] | (unsigned(s1[pl][3]) << 16);
+        p1.z = s1[pl][4] | (unsigned(s1[pl][5]) << 16); p1.w = s1[pl][6] | (unsigned(s1[pl][7]) << 16);
+        *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b0) = p0;
+        *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b1) = p1;
+      }
     }
   }
 }
 
-template <int ST>
-void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
-  const int kind = KIND_M0 + ST;
-  hipLaunchKernelGGL((k_mgemm<ST>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s, (const Job*)js.jobs,
+template <int ST, int TN>
+void launch_mstage_tn(const JobSet& js, bool gate, int fin, hipStream_t s) {
+  const int kind = mixed_kind(ST, TN == 32 ? 0 : 1);
+  if (js.total[kind] == 0) return;
+  hipLaunchKernelGGL((k_mgemm<ST, TN>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(kind), js.count, gate, fin);
   OTM_HIP(hipGetLastError());
+}
+
+template <int ST>
+void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
+  if (js.mcfg[ST] == 0)
+    launch_mstage_tn<ST, 32>(js, gate, fin, s);
+  else
+    launch_mstage_tn<ST, 64>(js, gate, fin, s);
 }
 
 template <int MODE>
@@ -407,13 +445,18 @@ void mixed_transform(const JobSet& js, bool gate, int fin, hipStream_t s) {
 
 }  // namespace
 
-int mixed_tiles(int st, int n) {
-  const int h = n / 2;
+int mixed_tiles(int st, int n, int cfg) {
+  const int h = n / 2, tn = MIXED_TN[cfg];
   switch (st) {
-    case M1: return (h / (MTM / 2)) * (n / MTN);
-    case M2: return (n / MTM) * (n / MTN);
-    default: return (n / MTM) * (h / MTN);
+    case M1: return (h / (MTM / 2)) * (n / tn);
+    case M2: return (n / MTM) * (n / tn);
+    default: return (n / MTM) * (h / tn);
   }
+}
+
+bool mixed_cfg_valid(int st, int n, int cfg) {
+  const int tn = MIXED_TN[cfg];
+  return mixed_supported(n) && ((st == M1 || st == M2) ? n : n / 2) % tn == 0;
 }
 
 bool mixed_supported(int n) { return n % 64 == 0 && n >= 64; }
@@ -467,9 +510,72 @@ void mixed_plan_destroy(MixedPlan& mp) {
 // one ungated stage (M2, the plain n x n output stage) for kernel timing
 void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s) { launch_mstage<M2>(js, false, 0, s); }
 
+namespace {
+using MStageFn = void (*)(const JobSet&, bool, int, hipStream_t);
+const MStageFn MSTAGE_TABLE[4][NMIXED] = {
+    {launch_mstage_tn<M1, 32>, launch_mstage_tn<M1, 64>},
+    {launch_mstage_tn<M2, 32>, launch_mstage_tn<M2, 64>},
+    {launch_mstage_tn<M3, 32>, launch_mstage_tn<M3, 64>},
+    {launch_mstage_tn<M4, 32>, launch_mstage_tn<M4, 64>},
+};
+}  // namespace
+
+// Times each valid tile width of every mixed stage over the table's jobs
+// (ungated; M4 writes only the x0 scratch) and keeps the fastest.  Results
+// are cached per device and sorted set of grid sizes.
+void mixed_tune(JobSet& js, int device, hipStream_t s) {
+  for (const Job& J : js.host)
+    if (!J.Pp || !mixed_supported(J.f.n)) return;
+  static std::mutex mu;
+  static std::map<std::pair<int, std::vector<int>>, std::array<int, 4>> cache;
+  std::vector<int> sizes;
+  for (const Job& J : js.host) sizes.push_back(J.f.n);
+  std::sort(sizes.begin(), sizes.end());
+  const auto key = std::make_pair(device, sizes);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      for (int st = 0; st < 4; ++st) js.mcfg[st] = it->second[st];
+      return;
+    }
+  }
+  hipEvent_t e0, e1;
+  OTM_HIP(hipEventCreate(&e0));
+  OTM_HIP(hipEventCreate(&e1));
+  std::array<int, 4> best{};
+  for (int st = 0; st < 4; ++st) {
+    float best_ms = 1e30f;
+    best[st] = 0;
+    for (int c = 0; c < NMIXED; ++c) {
+      bool ok = true;
+      for (int n : sizes) ok = ok && mixed_cfg_valid(st, n, c);
+      if (!ok) continue;
+      const MStageFn fn = MSTAGE_TABLE[st][c];
+      const int fin = st == M4 ? FIN_X0 : 0;
+      for (int rep = 0; rep < 2; ++rep) fn(js, false, fin, s);
+      OTM_HIP(hipEventRecord(e0, s));
+      for (int rep = 0; rep < 3; ++rep) fn(js, false, fin, s);
+      OTM_HIP(hipEventRecord(e1, s));
+      OTM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      OTM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best[st] = c;
+      }
+    }
+    js.mcfg[st] = best[st];
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lock(mu);
+  cache[key] = best;
+}
+
 // d_hat = M^+ (r - S e0) of every job: mixed transform, then `refine` FP64
 // refinement steps x += transform(r' - M0 x); the last stage leaves the
-// |d_hat|^2 partials in SITE_DNORM (launch kind KIND_M0 + 3).
+// |d_hat|^2 partials in SITE_DNORM (launch kind mixed_kind(3, js.mcfg[3])).
 void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s) {
   launch_mprep<0>(js, gate, s);
   if (refine <= 0) {

@@ -110,7 +110,7 @@ void driver_destroy(Driver& d) {
 
 // launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
 int dnorm_kind(const JobSet& js, bool mixed) {
-  if (mixed) return KIND_M0 + 3;
+  if (mixed) return mixed_kind(3, js.mcfg[3]);
   return js.fold ? fold_kind(3, js.cfg[3]) : KIND_PLAIN64;
 }
 
@@ -129,7 +129,9 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
   hipStream_t s = d.stream;
   // the captured iteration depends on the precision mode and the tile shapes
   const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (long(js.cfg[0]) << 8) |
-                   (long(js.cfg[1]) << 12) | (long(js.cfg[2]) << 16) | (long(js.cfg[3]) << 20);
+                   (long(js.cfg[1]) << 12) | (long(js.cfg[2]) << 16) | (long(js.cfg[3]) << 20) |
+                   (long(js.mcfg[0]) << 24) | (long(js.mcfg[1]) << 26) | (long(js.mcfg[2]) << 28) |
+                   (long(js.mcfg[3]) << 30);
   if (d.exec && d.key != key) driver_invalidate(d);
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));

@@ -151,17 +151,22 @@ inline int fold_tiles(int n, int st, int c) {
 // workgroups of the grid-stride ("flat") kernels per job
 inline int flat_blocks(long F) { return int(std::min<long>((F + NT * 4 - 1) / (NT * 4), 1024)); }
 
+// mixed-transform tile widths (output columns per workgroup), one per config
+constexpr int NMIXED = 2;
+constexpr int MIXED_TN[NMIXED] = {32, 64};
+
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
-// KIND_M0 + s: stage s of the mixed transform
+// mixed_kind(s, c): stage s of the mixed transform with tile width MIXED_TN[c]
 enum LaunchKind {
   KIND_STENCIL = 0,
   KIND_FLAT = 1,
   KIND_PLAIN64 = 2,
   KIND_M0 = 3,
-  KIND_GEMM0 = KIND_M0 + 4,
+  KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
 inline int fold_kind(int stage, int c) { return KIND_GEMM0 + stage * NFOLD + c; }
+inline int mixed_kind(int stage, int c) { return KIND_M0 + stage * NMIXED + c; }
 
 struct JobSet {
   int count = 0;
@@ -170,7 +175,8 @@ struct JobSet {
   std::vector<Job> host;   // host copy
   std::vector<int> total;  // workgroups per kind
   bool fold = false;       // every job uses the folded transform
-  int cfg[4] = {NFOLD - 1, NFOLD - 1, NFOLD - 1, NFOLD - 1};
+  int cfg[4] = {NFOLD - 1, NFOLD - 1, NFOLD - 1, NFOLD - 1};  // folded FP64 tile shape per stage
+  int mcfg[4] = {0, 0, 0, 0};                                 // mixed tile width per stage
   const int* starts_of(int kind) const { return starts + long(kind) * (count + 1); }
 };
 void jobset_create(JobSet& js, const std::vector<Job>& jobs);
@@ -224,13 +230,17 @@ struct MixedPlan {
   unsigned short* Ptp = nullptr;  // BF16 planes of Pt (3 x h*n)
 };
 bool mixed_supported(int n);
-int mixed_tiles(int stage, int n);  // workgroups of mixed stage 0..3
+int mixed_tiles(int stage, int n, int cfg);      // workgroups of mixed stage 0..3 with config cfg
+bool mixed_cfg_valid(int stage, int n, int cfg);  // n supported and the tile width divides the stage
 void mixed_plan_create(MixedPlan& mp, const DctPlan& p);
 void mixed_plan_destroy(MixedPlan& mp);
 void fields_enable_mixed(Fields& f);  // allocates the mixed buffers (driver.hip)
 // d_hat = M^+ (r - S e0) of every job of the table (all with mixed buffers and plans)
 void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s);
 void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s);
+// times every valid tile width of each mixed stage (cached like dct_tune) and
+// stores the fastest in js.mcfg
+void mixed_tune(JobSet& js, int device, hipStream_t s);
 
 // Per-grid buffers and the iteration driver (driver.hip)
 void fields_create(Fields& f, int n, int trace_cap);

@@ -94,6 +94,7 @@ void ensure_mixed(otm_solver* s) {
   J.Pp = s->mplan.Pp;
   J.Ptp = s->mplan.Ptp;
   jobset_upload(s->js, s->drv.stream);
+  mixed_tune(s->js, s->device, s->drv.stream);
   driver_invalidate(s->drv);
 }
 
@@ -247,6 +248,25 @@ int otm_solver_set_gemm_config(otm_solver* s, int stage, int config) {
   return OTM_OK;
 }
 
+int otm_solver_mixed_config(const otm_solver* s, int* cfg) {
+  if (int rc = check_ready(const_cast<otm_solver*>(s))) return rc;
+  if (!cfg) { set_error("null config array"); return OTM_ERR_ARG; }
+  for (int st = 0; st < 4; ++st) cfg[st] = s->js.mcfg[st];
+  return s->f.x0 ? NMIXED : 0;
+}
+
+int otm_solver_set_mixed_config(otm_solver* s, int stage, int config) {
+  if (int rc = check_ready(s)) return rc;
+  if (!s->f.x0) { set_error("no mixed solve has run on this solver"); return OTM_ERR_STATE; }
+  if (stage < 0 || stage > 3 || config < 0 || config >= NMIXED) {
+    set_error("stage must be 0..3 and config 0..otm_solver_mixed_config()-1");
+    return OTM_ERR_ARG;
+  }
+  if (!mixed_cfg_valid(stage, s->n, config)) { set_error("tile width does not divide the stage"); return OTM_ERR_ARG; }
+  s->js.mcfg[stage] = config;
+  return OTM_OK;
+}
+
 int otm_solve_device(otm_solver* s, const double* d_density, const otm_options* opt, double* d_fwd_xy,
                      double* d_adjusted_density, otm_stats* stats) {
   if (int rc = check_ready(s)) return rc;
@@ -313,7 +333,10 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   Fields& f = J.f;
   JobSet pj;
   jobset_create(pj, {J});
-  for (int st = 0; st < 4; ++st) pj.cfg[st] = s->js.cfg[st];
+  for (int st = 0; st < 4; ++st) {
+    pj.cfg[st] = s->js.cfg[st];
+    pj.mcfg[st] = s->js.mcfg[st];
+  }
   SolveState tpl;
   std::memset(&tpl, 0, sizeof(tpl));
   tpl.it = 1; tpl.cj = 1; tpl.max_iter = 1 << 30; tpl.threshold = 0.; tpl.alpha = 0.5; tpl.beta = 0.1;

@@ -823,7 +823,10 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs) {
       if (kind == KIND_STENCIL) w = num_tiles(n);
       else if (kind == KIND_FLAT) w = flat_blocks(jobs[p].f.F);
       else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
-      else if (kind < KIND_GEMM0) w = mixed_supported(n) ? mixed_tiles(kind - KIND_M0, n) : 0;
+      else if (kind < KIND_GEMM0) {
+        const int g = kind - KIND_M0, stage = g / NMIXED, c = g % NMIXED;
+        w = mixed_cfg_valid(stage, n, c) ? mixed_tiles(stage, n, c) : 0;
+      }
       else {
         const int g = kind - KIND_GEMM0, stage = g / NFOLD, c = g % NFOLD;
         w = fold_cfg_valid(n, c) ? fold_tiles(n, stage, c) : 0;

@@ -162,6 +162,18 @@ class GridBasedTransportSolver:
     def set_gemm_config(self, stage: int, config: int) -> None:
         check(_lib.lib().otm_solver_set_gemm_config(self._h, stage, config), "set_gemm_config")
 
+    def mixed_config(self):
+        """(tile width config per mixed DCT stage, number of candidates); the
+        count is 0 before the first mixed solve."""
+        cfg = np.zeros(4, dtype=np.int32)
+        count = _lib.lib().otm_solver_mixed_config(self._h, iptr(cfg))
+        if count < 0:
+            check(count, "mixed_config")
+        return [int(c) for c in cfg], count
+
+    def set_mixed_config(self, stage: int, config: int) -> None:
+        check(_lib.lib().otm_solver_set_mixed_config(self._h, stage, config), "set_mixed_config")
+
     def laplacian_solve(self, r):
         n = self._n
         d = np.empty(n * n)

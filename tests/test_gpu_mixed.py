@@ -67,3 +67,36 @@ def test_mixed_rejects_unsupported_size(solver_factory):
     s = solver_factory(100)
     with pytest.raises(OtmError):
         s.solve(np.ones(100 * 100) + 0.1 * np.arange(10000) / 10000, SolverOptions(precision=Precision.MIXED))
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_mixed_tile_widths_bit_identical(n, solver_factory):
+    """Every tile width of the mixed stages accumulates in the same order and
+    gives the same bits (the autotuner may pick any of them)."""
+    s = solver_factory(n)
+    r = np.random.default_rng(5 + n).standard_normal(n * n)
+    ref = s.laplacian_solve_mixed(r, 1)
+    chosen, count = s.mixed_config()
+    assert count == 2 and all(0 <= c < count for c in chosen), (chosen, count)
+    for c in range(count):
+        for st in range(4):
+            s.set_mixed_config(st, c)
+        assert np.array_equal(s.laplacian_solve_mixed(r, 1), ref), c
+    for st, c in enumerate([1, 0, 1, 0]):
+        s.set_mixed_config(st, c)
+    assert np.array_equal(s.laplacian_solve_mixed(r, 1), ref)
+
+
+def test_mixed_tile_width_must_divide_stage():
+    from otmap_amd import GridBasedTransportSolver
+    from otmap_amd._lib import OtmError
+    s = GridBasedTransportSolver(0)
+    s.init(64)
+    assert s.mixed_config()[1] == 0
+    with pytest.raises(OtmError):
+        s.set_mixed_config(0, 0)  # no mixed solve yet
+    s.laplacian_solve_mixed(np.ones(64 * 64), 0)
+    assert s.mixed_config()[1] == 2
+    s.set_mixed_config(0, 1)      # 64-wide tiles over n = 64 columns
+    with pytest.raises(OtmError):
+        s.set_mixed_config(2, 1)  # but not over n/2 = 32
