@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--inverse-points", type=int, default=1 << 20)
     ap.add_argument("--batch", type=int, default=64, help="densities of the batched workload (0: skip)")
     ap.add_argument("--no-pipelines", action="store_true", help="skip the barycentre and stippling workloads")
+    ap.add_argument("--batch-1024", type=int, default=8,
+                    help="1024^2 densities of the batched mixed-vs-FP64 workload (0: skip)")
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
                              "mixed_solve", "copy"],
@@ -184,6 +186,44 @@ def batched_throughput(count, steps, warmup):
             "speedup_vs_sequential": round(t_seq / t_batch, 3),
             "iterations": {"min": min(iters), "max": max(iters), "mean": round(float(np.mean(iters)), 1)},
             "create_s": round(t_create, 3), "precision": "fp64", "gemm_tiles": b.gemm_config()[0]}
+
+
+def batched_1024(count, steps, warmup, fp64_single_ms):
+    """`count` seeded smoothed-shape densities at 1024^2 (seeds 1..count past
+    the single-solve seed) solved together by one BatchSolver, FP64 and mixed,
+    against the single FP64 solve: the per-solve time of each batched form."""
+    import torch
+
+    from otmap_amd import BatchSolver, Precision, SolverOptions
+    from otmap_amd import densities as D
+
+    n = 1024
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(D.smoothed_shapes(n, seed=D.SEED_SHAPES_1024 + 1 + k)).to(dev) for k in range(count)]
+    d_out = [torch.empty(2 * (n + 1) ** 2, dtype=torch.float64, device=dev) for _ in range(count)]
+    ip = [t.data_ptr() for t in d_in]
+    op = [t.data_ptr() for t in d_out]
+    b = BatchSolver([n] * count)
+    out = {"densities": count, "n": n}
+    fwd = {}
+    for name, prec in (("fp64", Precision.FP64), ("mixed", Precision.MIXED)):
+        opt = SolverOptions(precision=prec)
+        for _ in range(warmup):
+            b.solve_device(ip, op, opt=opt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = b.solve_device(ip, op, opt=opt)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        fwd[name] = torch.cat(d_out)
+        out[f"{name}_ms_per_batch"] = round(ms, 2)
+        out[f"{name}_ms_per_solve"] = round(ms / count, 3)
+        out[f"{name}_iterations"] = [x.iterations for x in st]
+    out["max_vertex_diff_mixed_vs_fp64"] = float((fwd["mixed"] - fwd["fp64"]).abs().max().item())
+    out["speedup_mixed_batched_vs_fp64_single"] = round(fp64_single_ms / out["mixed_ms_per_solve"], 3)
+    out["mixed_tiles"] = b.mixed_config()[0]
+    return out
 
 
 def barycentre_pipeline(n=512, steps=4):
@@ -329,6 +369,8 @@ def main():
     # host (once per map, as init_inverse), then one query per lane
     inv_stats = inverse_throughput(n, d_fwd, d_adj, args.inverse_points, args.steps)
     batch_stats = batched_throughput(args.batch, args.steps, args.warmup) if args.batch > 0 else None
+    b1024 = (batched_1024(args.batch_1024, args.steps, args.warmup, results["fp64"]["ms"])
+             if args.batch_1024 > 0 and n == 1024 else None)
     bary = None if args.no_pipelines else barycentre_pipeline()
     stipple = None if args.no_pipelines else stippling_pipeline()
 
@@ -421,6 +463,7 @@ def main():
         "max_vertex_err_vs_reference": err,
         "inverse_1M": inv_stats,
         "batched": batch_stats,
+        "batched_1024": b1024,
         "barycentre_4x512": bary,
         "stippling_2048": stipple,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},

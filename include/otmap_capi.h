@@ -260,6 +260,9 @@ int otm_batch_solve_device(otm_batch* b, const double* const* d_density, const o
 int otm_batch_trace(const otm_batch* b, int job, double* residual, double* alpha, double* beta, int cap);
 /* tile shape per folded DCT stage chosen for this batch; returns the number of candidates */
 int otm_batch_gemm_config(const otm_batch* b, int* cfg);
+/* tile width per mixed DCT stage (0: 32 columns, 1: 64) picked on the first
+ * mixed solve of the batch; returns the number of candidates (0 before it) */
+int otm_batch_mixed_config(const otm_batch* b, int* cfg);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,7 @@ _SIGNATURES = {
                                          C.POINTER(_vp), C.POINTER(OtmStats)]),
     "otm_batch_trace": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "otm_batch_gemm_config": (C.c_int, [_vp, _ip]),
+    "otm_batch_mixed_config": (C.c_int, [_vp, _ip]),
 }
 
 _lib = None

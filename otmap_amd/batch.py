@@ -88,3 +88,12 @@ class BatchSolver:
         if count < 0:
             check(count, "batch gemm_config")
         return [int(c) for c in cfg], count
+
+    def mixed_config(self):
+        """(tile width config per mixed DCT stage, number of candidates); the
+        count is 0 before the first mixed solve."""
+        cfg = np.zeros(4, dtype=np.int32)
+        count = _lib.lib().otm_batch_mixed_config(self._h, iptr(cfg))
+        if count < 0:
+            check(count, "batch mixed_config")
+        return [int(c) for c in cfg], count

@@ -66,6 +66,7 @@ void build_table(otm_batch* b) {
     if (it != b->mplans.end()) {
       jobs[p].Pp = it->second.Pp;
       jobs[p].Ptp = it->second.Ptp;
+      jobs[p].ck = it->second.ck;
     }
   }
   int cfg[4], mcfg[4];
@@ -201,6 +202,12 @@ int otm_batch_gemm_config(const otm_batch* b, int* cfg) {
   if (!b || !cfg) { set_error("null argument"); return OTM_ERR_ARG; }
   for (int st = 0; st < 4; ++st) cfg[st] = b->js.cfg[st];
   return NFOLD;
+}
+
+int otm_batch_mixed_config(const otm_batch* b, int* cfg) {
+  if (!b || !cfg) { set_error("null argument"); return OTM_ERR_ARG; }
+  for (int st = 0; st < 4; ++st) cfg[st] = b->js.mcfg[st];
+  return b->mixed_tuned ? NMIXED : 0;
 }
 
 int otm_batch_solve_device(otm_batch* b, const double* const* d_density, const otm_options* opt,

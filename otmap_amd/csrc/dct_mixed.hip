@@ -116,30 +116,38 @@ __global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, cons
 // ---------------------------------------------------------------------------
 // one stage of the folded transform for every job.  Workgroup tile: 64 rows
 // of A (M1: 32 rows and their 32 mirrors) by TN rows of B (TN = 32 or 64,
-// picked per stage by the autotuner), K steps of 32, four waves of
-// 32 x TN/2.  All tile widths accumulate in the same order (bit-identical).
+// picked per stage by the autotuner), K steps of 32.  The workgroup is two
+// K groups of four waves (32 x TN/2 each): group 0 accumulates the first
+// ceil(S/2) of the S = K/32 steps, group 1 the rest, each through its own LDS
+// buffers, both in lockstep on one barrier per step.  For M3/M4 the split is
+// exactly k < h / k >= h, the two halves the fold needs; for M1/M2 the
+// epilogue adds the two FP32 partials.  The order of every sum depends on n
+// only, so all tile widths give bit-identical results.
 // ---------------------------------------------------------------------------
+constexpr int MNT = 2 * NT;  // threads of a mixed GEMM workgroup
+
 template <int ST, int TN>
-__global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, const int* __restrict__ starts, int count,
-                                             bool gate, int fin) {
+__global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, const int* __restrict__ starts, int count,
+                                              bool gate, int fin) {
   const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
   const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
   const int nbk = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
   const Job& J = jobs[p];
   const auto st = gp((const SolveState*)J.f.state);
-  if (gate && st->done) return;
+  // the gate is tested after the first operand loads are issued (they are in
+  // bounds either way), so its latency overlaps theirs
+  const int done = gate ? st->done : 0;
   const int n = J.f.n, h = n / 2;
   const long nh = long(n) * h, nn = long(n) * n;
-  constexpr bool SPLIT = ST == M3 || ST == M4;
   constexpr int TJ = TN / 32;                   // MFMA tiles per wave along B
   constexpr int BCHUNKS = 3 * TN * 4;           // 16-byte B chunks per K step
-  constexpr int BCH = (BCHUNKS + NT - 1) / NT;  // per thread
+  constexpr int BCH = (BCHUNKS + NT - 1) / NT;  // per thread of a group
   const int ntn = (ST == M1 || ST == M2 ? n : h) / TN;
   const int ntm = ST == M1 ? h / (MTM / 2) : n / MTM;
   if (lt >= ntm * ntn) {  // padding workgroup of the job: neutral partial only
     if (ST == M4 && (fin == FIN_OUT || fin == FIN_ADD_OUT)) {
       double z[1] = {0.};
-      write_partials_at<1>(z, J.f.site(SITE_DNORM), lt, nbk);
+      write_partials_at<1, MNT>(z, J.f.site(SITE_DNORM), lt, nbk);
     }
     return;
   }
@@ -168,21 +176,22 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  // LDS: A planes [2][3][64][MLD], B planes [2][3][TN][MLD]; the epilogue
-  // reuses the space for one or two FP32 tiles [64][TN + 1]
-  constexpr int A_U16 = 2 * 3 * MTM * MLD, B_U16 = 2 * 3 * TN * MLD;
-  __shared__ __attribute__((aligned(16))) u16 smem[A_U16 + B_U16];
-  static_assert((A_U16 + B_U16) * 2 >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
-  auto As = [&](int buf, int pl, int row) { return smem + ((buf * 3 + pl) * MTM + row) * MLD; };
-  auto Bs = [&](int buf, int pl, int row) { return smem + A_U16 + ((buf * 3 + pl) * TN + row) * MLD; };
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // LDS per group: A planes [2][3][64][MLD], B planes [2][3][TN][MLD]; the
+  // epilogue reuses the space for the two groups' FP32 tiles [64][TN + 1]
+  constexpr int A_U16 = 2 * 3 * MTM * MLD, B_U16 = 2 * 3 * TN * MLD, G_U16 = A_U16 + B_U16;
+  __shared__ __attribute__((aligned(16))) u16 smem[2 * G_U16];
+  static_assert(2 * G_U16 * 2 >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
+  const int t = threadIdx.x, g = t / NT, tg = t % NT, lane = t & 63, w = tg >> 6;
+  u16* const gs = smem + g * G_U16;
+  auto As = [&](int buf, int pl, int row) { return gs + ((buf * 3 + pl) * MTM + row) * MLD; };
+  auto Bs = [&](int buf, int pl, int row) { return gs + A_U16 + ((buf * 3 + pl) * TN + row) * MLD; };
   const int wm = (w >> 1) * 32, wn = (w & 1) * (TN / 2);
   const int lr = lane & 15, lq = lane >> 4;
 
   // global -> register staging, two slots of K steps in flight.  A: 3 x 64
-  // rows x 4 chunks of 8 (3 per thread); B: 3 x TN x 4 (chunk indices past the
-  // tile reload an earlier chunk, never stored, so every lane issues the same
-  // loads and the waits on the slots stay exact)
+  // rows x 4 chunks of 8 (3 per thread of the group); B: 3 x TN x 4 (chunk
+  // indices past the tile reload an earlier chunk, never stored, so every
+  // lane issues the same loads and the waits on the slots stay exact)
   u32x4 ra[2][3], rb[2][BCH];
   auto arow = [&](int row) {  // global row of tile row `row`
     if (ST == M1) return row < MTM / 2 ? m0 + row : n - m0 - MTM + row;  // rows i and mirrored n-1-i
@@ -191,13 +200,12 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   auto gload = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int k0) {
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
-      const int c = t + u * NT;
-      const int pl = c >> 8, rem = c & 255, row = rem >> 2, kc = (rem & 3) * 8;
-      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + pl * aps + long(arow(row)) * lda + k0 + kc);
+      const int row = tg >> 2, kc = (tg & 3) * 8;
+      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + u * aps + long(arow(row)) * lda + k0 + kc);
     }
 #pragma unroll
     for (int u = 0; u < BCH; ++u) {
-      const int c0 = t + u * NT, c = c0 < BCHUNKS ? c0 : c0 - TN * 4;
+      const int c0 = tg + u * NT, c = c0 < BCHUNKS ? c0 : c0 - TN * 4;
       const int pl = c / (TN * 4), rem = c % (TN * 4), row = rem >> 2, kc = (rem & 3) * 8;
       xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + long(n0 + row) * ldb + k0 + kc);
     }
@@ -205,13 +213,12 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
   auto sstore = [&](const u32x4 (&xa)[3], const u32x4 (&xb)[BCH], int buf) {
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
-      const int c = t + u * NT;
-      const int pl = c >> 8, rem = c & 255, row = rem >> 2, kc = (rem & 3) * 8;
-      *reinterpret_cast<u32x4*>(As(buf, pl, row) + kc) = xa[u];
+      const int row = tg >> 2, kc = (tg & 3) * 8;
+      *reinterpret_cast<u32x4*>(As(buf, u, row) + kc) = xa[u];
     }
 #pragma unroll
     for (int u = 0; u < BCH; ++u) {
-      const int c = t + u * NT;
+      const int c = tg + u * NT;
       if (BCHUNKS % NT == 0 || c < BCHUNKS) {
         const int pl = c / (TN * 4), rem = c % (TN * 4), row = rem >> 2, kc = (rem & 3) * 8;
         *reinterpret_cast<u32x4*>(Bs(buf, pl, row) + kc) = xb[u];
@@ -240,112 +247,98 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
           c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[j][PB[q]], c[i][j], 0, 0, 0);
   };
 
-  // Software pipeline, two K-steps in flight: step s stores register slot
-  // s & 1 (loaded two steps earlier) into LDS buffer s & 1, refills the slot
-  // with step s + 2, syncs and multiplies.  The buffer written at step s was
-  // last read at step s - 2, before every wave passed the sync of step s - 1.
-  // The main loop body holds no conditional memory operation, so the waits on
-  // the loads stay exact.  Loads past the last step re-read the last K block
-  // (in bounds, never stored).  For the split stages the first-half sum moves
-  // to acc_lo when step h / MKB starts.
-  const int nsteps = K / MKB, hsteps = SPLIT ? h / MKB : -1;
-  const int klast = K - MKB;
-  f4 acc[2][TJ], acc_lo[2][TJ];
+  // Software pipeline of each group, two K steps in flight: step s stores
+  // register slot s & 1 (loaded two steps earlier) into LDS buffer s & 1,
+  // refills the slot with step s + 2, syncs and multiplies.  The buffer
+  // written at step s was last read at step s - 2, before every wave passed
+  // the sync of step s - 1.  Both groups run S0 steps (the same barriers);
+  // when S is odd the last step of group 1 multiplies nothing.  Loads past a
+  // group's last step re-read an in-bounds K block that is never used.
+  const int S = K / MKB, S0 = (S + 1) >> 1, mine = g == 0 ? S0 : S - S0;
+  const int kbeg = g * S0 * MKB, klast = K - MKB;
+  f4 acc[2][TJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < TJ; ++j) {
-      acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-      acc_lo[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    }
+    for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   auto step = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int s, int buf) {
-    if (SPLIT && s == hsteps) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-          acc_lo[i][j] = acc[i][j];
-          acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-        }
-    }
     sstore(xa, xb, buf);
-    gload(xa, xb, min((s + 2) * MKB, klast));
+    gload(xa, xb, min(kbeg + (s + 2) * MKB, klast));
     __syncthreads();
-    compute(acc, buf);
+    if (s < mine) compute(acc, buf);
   };
-  gload(ra[0], rb[0], 0);
-  gload(ra[1], rb[1], min(MKB, klast));
+  // M2: eigenvalue factors of the tile's rows and columns, staged in LDS for the epilogue
+  __shared__ double cfac[MTM + TN];
+  double cf = 0.;
+  if (ST == M2 && t < MTM + TN) cf = gp(J.ck)[t < MTM ? m0 + t : n0 + t - MTM];
+  gload(ra[0], rb[0], min(kbeg, klast));
+  gload(ra[1], rb[1], min(kbeg + MKB, klast));
+  if (done) return;
+  if (ST == M2 && t < MTM + TN) cfac[t] = cf;
   int s = 0;
-  for (; s + 2 <= nsteps; s += 2) {
+  for (; s + 2 <= S0; s += 2) {
     step(ra[0], rb[0], s, 0);
     step(ra[1], rb[1], s + 1, 1);
   }
-  if (s < nsteps) step(ra[0], rb[0], s, 0);
+  if (s < S0) step(ra[0], rb[0], s, 0);
   __syncthreads();
 
-  // lane (lr, lq) of MFMA tile (i, j) holds D[wm + 16i + 4 lq + v][wn + 16j + lr];
-  // r0 = k < h (or all k), r1 = k >= h
-  if (ST == M4) {
-    const auto x0 = gp(J.f.x0);
-    const auto dh = gp(J.f.dhat);
-    double nrm[1] = {0.};
+  // stage both groups' FP32 tiles: lane (lr, lq) of MFMA tile (i, j) holds
+  // D[wm + 16i + 4 lq + v][wn + 16j + lr]; D0 = group 0 (k < h for M3/M4), D1 = group 1
+  constexpr int DL = TN + 1;
+  float* const D0 = reinterpret_cast<float*>(smem);
+  float* const D1 = D0 + MTM * DL;
+  {
+    float* Dg = g == 0 ? D0 : D1;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < TJ; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const long row = m0 + wm + 16 * i + 4 * lq + v;
-          const int col = n0 + wn + 16 * j + lr;
-          const double p0 = double(acc_lo[i][j][v]), p1 = double(acc[i][j][v]);
-          const long ia = row * n + col, ib = row * n + (n - 1 - col);
-          double va = p0 + p1, vb = p0 - p1;
-          if (fin == FIN_ADD_OUT || fin == FIN_ADD_X0) {
-            va = x0[ia] + va;
-            vb = x0[ib] + vb;
-          }
-          if (fin == FIN_X0 || fin == FIN_ADD_X0) {
-            x0[ia] = va;
-            x0[ib] = vb;
-          } else {
-            dh[ia] = va;
-            dh[ib] = vb;
-            nrm[0] += va * va + vb * vb;
-          }
-        }
-    if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nbk);
+        for (int v = 0; v < 4; ++v) Dg[(wm + 16 * i + 4 * lq + v) * DL + wn + 16 * j + lr] = acc[i][j][v];
+  }
+  __syncthreads();
+
+  if (ST == M4) {
+    // X[row][col] and X[row][n-1-col] = p0 +- p1 in FP64
+    const auto x0 = gp(J.f.x0);
+    const auto dh = gp(J.f.dhat);
+    double nrm[1] = {0.};
+    for (int it = t; it < MTM * TN; it += MNT) {
+      const int r = it / TN, c = it % TN;
+      const long row = m0 + r;
+      const int col = n0 + c;
+      const double p0 = double(D0[r * DL + c]), p1 = double(D1[r * DL + c]);
+      const long ia = row * n + col, ib = row * n + (n - 1 - col);
+      double va = p0 + p1, vb = p0 - p1;
+      if (fin == FIN_ADD_OUT || fin == FIN_ADD_X0) {
+        va = x0[ia] + va;
+        vb = x0[ib] + vb;
+      }
+      if (fin == FIN_X0 || fin == FIN_ADD_X0) {
+        x0[ia] = va;
+        x0[ib] = vb;
+      } else {
+        dh[ia] = va;
+        dh[ib] = vb;
+        nrm[0] += va * va + vb * vb;
+      }
+    }
+    if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1, MNT>(nrm, J.f.site(SITE_DNORM), lt, nbk);
     return;
   }
 
-  // the other stages stage the FP32 tile(s) in LDS and write BF16 planes in
-  // 8-16 byte runs
-  constexpr int DL = TN + 1;
-  float* Ds0 = reinterpret_cast<float*>(smem);
-  float* Ds1 = Ds0 + MTM * DL;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < TJ; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = wm + 16 * i + 4 * lq + v, c = wn + 16 * j + lr;
-        if (ST == M3) {
-          Ds0[r * DL + c] = acc_lo[i][j][v] + acc[i][j][v];
-          Ds1[r * DL + c] = acc_lo[i][j][v] - acc[i][j][v];
-        } else {
-          Ds0[r * DL + c] = acc[i][j][v];
-        }
-      }
-  __syncthreads();
+  // the other stages write BF16 planes in 8-16 byte runs
   if (ST == M1) {
     // U2 / W2 [l = n0 + c][i = m0 + a .. a+3]: pair tile row a with its mirror 63 - a
     const auto out = gp(J.f.mwa);
-    for (int it = t; it < TN * 8; it += NT) {
+    for (int it = t; it < TN * 8; it += MNT) {
       const int c = it >> 3, a0 = (it & 7) * 4;
       u16 u[3][4], wv[3][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float top = Ds0[(a0 + q) * DL + c], mir = Ds0[(MTM - 1 - a0 - q) * DL + c];
+        const int rt = a0 + q, rm = MTM - 1 - a0 - q;
+        const float top = D0[rt * DL + c] + D1[rt * DL + c], mir = D0[rm * DL + c] + D1[rm * DL + c];
         split3(top + mir, u[0][q], u[1][q], u[2][q]);
         split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
       }
@@ -360,21 +353,21 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
       }
     }
   } else if (ST == M2) {
-    // Z[l = m0 + r][k = n0 + c0 .. c0+7] / lambda(perm k, perm l)
+    // Z[l = m0 + r][k = n0 + c0 .. c0+7] / lambda(perm k, perm l), with
+    // lambda = 2 (1 - cos(pi k'/n) cos(pi l'/n)) formed in FP64 (relative
+    // error ~1e-16 / (pi^2 / n^2)) and the division in FP32
     const auto out = gp(J.f.mwb);
-    const auto sq = gp(J.sq);
-    for (int it = t; it < MTM * TN / 8; it += NT) {
+    for (int it = t; it < MTM * TN / 8; it += MNT) {
       const int r = it / (TN / 8), c0 = (it % (TN / 8)) * 8;
       const int lrow = m0 + r;
-      const int lperm = lrow < h ? 2 * lrow : 2 * (lrow - h) + 1;
+      const double cl = cfac[r];
       u16 z[3][8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int kc = n0 + c0 + q;
-        const int kperm = kc < h ? 2 * kc : 2 * (kc - h) + 1;
-        const int dm = kperm > lperm ? kperm - lperm : lperm - kperm;
-        const double lam = 2. * (sq[dm] + sq[kperm + lperm]);
-        const float val = (kperm == 0 && lperm == 0) ? 0.f : float(double(Ds0[r * DL + c0 + q]) / lam);
+        const float lam = float(2. * (1. - cfac[MTM + c0 + q] * cl));
+        const float y = D0[r * DL + c0 + q] + D1[r * DL + c0 + q];
+        const float val = (kc == 0 && lrow == 0) ? 0.f : y / lam;
         split3(val, z[0][q], z[1][q], z[2][q]);
       }
       const long base = long(lrow) * n + n0 + c0;
@@ -387,15 +380,16 @@ __global__ __launch_bounds__(NT) void k_mgemm(const Job* __restrict__ jobs, cons
       }
     }
   } else {
-    // T3[i = n0 + c][l = m0 + r0 .. r0+7] and T3[n-1-i][l]
+    // T3[i = n0 + c][l = m0 + r0 .. r0+7] = p0 + p1 and T3[n-1-i][l] = p0 - p1
     const auto out = gp(J.f.mwa);
-    for (int it = t; it < TN * 8; it += NT) {
+    for (int it = t; it < TN * 8; it += MNT) {
       const int c = it >> 3, r0 = (it & 7) * 8;
       u16 s0[3][8], s1[3][8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        split3(Ds0[(r0 + q) * DL + c], s0[0][q], s0[1][q], s0[2][q]);
-        split3(Ds1[(r0 + q) * DL + c], s1[0][q], s1[1][q], s1[2][q]);
+        const float lo = D0[(r0 + q) * DL + c], hi = D1[(r0 + q) * DL + c];
+        split3(lo + hi, s0[0][q], s0[1][q], s0[2][q]);
+        split3(lo - hi, s1[0][q], s1[1][q], s1[2][q]);
       }
       const long b0 = long(n0 + c) * n + m0 + r0, b1 = long(n - 1 - n0 - c) * n + m0 + r0;
 #pragma unroll
@@ -416,7 +410,7 @@ template <int ST, int TN>
 void launch_mstage_tn(const JobSet& js, bool gate, int fin, hipStream_t s) {
   const int kind = mixed_kind(ST, TN == 32 ? 0 : 1);
   if (js.total[kind] == 0) return;
-  hipLaunchKernelGGL((k_mgemm<ST, TN>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s, (const Job*)js.jobs,
+  hipLaunchKernelGGL((k_mgemm<ST, TN>), dim3(unsigned(js.total[kind])), dim3(MNT), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(kind), js.count, gate, fin);
   OTM_HIP(hipGetLastError());
 }
@@ -454,9 +448,12 @@ int mixed_tiles(int st, int n, int cfg) {
   }
 }
 
+// Every stage needs h % TN == 0: M3/M4 tile n/2 columns, and an M1/M2 tile
+// must not straddle column h, where the A operand switches from the U to the
+// W planes.
 bool mixed_cfg_valid(int st, int n, int cfg) {
-  const int tn = MIXED_TN[cfg];
-  return mixed_supported(n) && ((st == M1 || st == M2) ? n : n / 2) % tn == 0;
+  (void)st;
+  return mixed_supported(n) && (n / 2) % MIXED_TN[cfg] == 0;
 }
 
 bool mixed_supported(int n) { return n % 64 == 0 && n >= 64; }
@@ -495,6 +492,10 @@ void mixed_plan_create(MixedPlan& mp, const DctPlan& p) {
         Ptp[pl * nh + size_t(i) * n + r] = hs[pl];
       }
     }
+  std::vector<double> ck(n);
+  for (int r = 0; r < n; ++r) ck[r] = std::cos(M_PI * double(r < h ? 2 * r : 2 * (r - h) + 1) / double(n));
+  OTM_HIP(hipMalloc(&mp.ck, n * sizeof(double)));
+  OTM_HIP(hipMemcpy(mp.ck, ck.data(), n * sizeof(double), hipMemcpyHostToDevice));
   OTM_HIP(hipMalloc(&mp.Pp, 3 * nh * sizeof(u16)));
   OTM_HIP(hipMalloc(&mp.Ptp, 3 * nh * sizeof(u16)));
   OTM_HIP(hipMemcpy(mp.Pp, Pp.data(), 3 * nh * sizeof(u16), hipMemcpyHostToDevice));
@@ -504,6 +505,7 @@ void mixed_plan_create(MixedPlan& mp, const DctPlan& p) {
 void mixed_plan_destroy(MixedPlan& mp) {
   if (mp.Pp) (void)hipFree(mp.Pp);
   if (mp.Ptp) (void)hipFree(mp.Ptp);
+  if (mp.ck) (void)hipFree(mp.ck);
   mp = MixedPlan();
 }
 

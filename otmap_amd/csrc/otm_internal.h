@@ -128,6 +128,7 @@ struct Job {
   const double* sq = nullptr;
   const unsigned short* Pp = nullptr;   // BF16 planes of P (3 x n*h) for the mixed solve
   const unsigned short* Ptp = nullptr;  // BF16 planes of Pt (3 x h*n)
+  const double* ck = nullptr;           // cos(pi perm(r) / n), r < n, perm = even/odd order (mixed solve)
   const double* dens_in = nullptr;  // input density of the current solve (F)
   double* fwd_out = nullptr;        // forward vertex positions (2V, SoA)
   double* adj_out = nullptr;        // copy of the adjusted density (F, nullable)
@@ -228,6 +229,7 @@ struct MixedPlan {
   int n = 0;
   unsigned short* Pp = nullptr;   // BF16 planes of P  (3 x n*h)
   unsigned short* Ptp = nullptr;  // BF16 planes of Pt (3 x h*n)
+  double* ck = nullptr;           // cos(pi perm(r) / n), r < n (eigenvalue factors, even/odd order)
 };
 bool mixed_supported(int n);
 int mixed_tiles(int stage, int n, int cfg);      // workgroups of mixed stage 0..3 with config cfg

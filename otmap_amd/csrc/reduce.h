@@ -16,10 +16,12 @@ namespace otm {
 __device__ static inline int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
 __device__ static inline int block_count() { return gridDim.x * gridDim.y; }
 
-// In-block sum of K values; the result is valid in thread 0.  Fixed order.
-template <int K>
+// In-block sum of K values over NTH threads (a multiple of 64); the result
+// is valid in thread 0.  Fixed order.
+template <int K, int NTH = NT>
 __device__ static inline void block_sum(double (&v)[K]) {
-  __shared__ double red[K][NT / 64];
+  constexpr int NW = NTH / 64;
+  __shared__ double red[K][NW];
   for (int k = 0; k < K; ++k) {
     double x = v[k];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
@@ -31,13 +33,18 @@ __device__ static inline void block_sum(double (&v)[K]) {
     for (int k = 0; k < K; ++k) red[k][w] = v[k];
   __syncthreads();
   if (threadIdx.x == 0)
-    for (int k = 0; k < K; ++k) v[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+    for (int k = 0; k < K; ++k) {
+      double x = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+      for (int q = 4; q < NW; q += 4) x += (red[k][q] + red[k][q + 1]) + (red[k][q + 2] + red[k][q + 3]);
+      v[k] = x;
+    }
 }
 
-// pass 1: K workgroup partials of workgroup bid among nb
-template <int K>
+// pass 1: K workgroup partials of workgroup bid among nb (NTH threads per workgroup)
+template <int K, int NTH = NT>
 __device__ static inline void write_partials_at(double (&v)[K], double* partials, int bid, int nb) {
-  block_sum<K>(v);
+  static_assert(NTH % 256 == 0, "workgroups of 4k waves");
+  block_sum<K, NTH>(v);
   if (threadIdx.x == 0)
     for (int k = 0; k < K; ++k) gp(partials)[long(k) * nb + bid] = v[k];
 }

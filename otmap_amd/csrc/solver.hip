@@ -93,6 +93,7 @@ void ensure_mixed(otm_solver* s) {
   J.f = s->f;
   J.Pp = s->mplan.Pp;
   J.Ptp = s->mplan.Ptp;
+  J.ck = s->mplan.ck;
   jobset_upload(s->js, s->drv.stream);
   mixed_tune(s->js, s->device, s->drv.stream);
   driver_invalidate(s->drv);

@@ -87,16 +87,21 @@ def test_mixed_tile_widths_bit_identical(n, solver_factory):
     assert np.array_equal(s.laplacian_solve_mixed(r, 1), ref)
 
 
-def test_mixed_tile_width_must_divide_stage():
+@pytest.mark.parametrize("n", [64, 192])
+def test_mixed_tile_width_must_divide_half(n):
+    """A 64-wide tile needs n/2 % 64 == 0 in every stage: M3/M4 tile n/2
+    columns, and an M1/M2 tile must not straddle column n/2, where the A
+    operand switches planes (n = 192 with 64-wide M1/M2 tiles did)."""
     from otmap_amd import GridBasedTransportSolver
     from otmap_amd._lib import OtmError
     s = GridBasedTransportSolver(0)
-    s.init(64)
+    s.init(n)
     assert s.mixed_config()[1] == 0
     with pytest.raises(OtmError):
         s.set_mixed_config(0, 0)  # no mixed solve yet
-    s.laplacian_solve_mixed(np.ones(64 * 64), 0)
-    assert s.mixed_config()[1] == 2
-    s.set_mixed_config(0, 1)      # 64-wide tiles over n = 64 columns
-    with pytest.raises(OtmError):
-        s.set_mixed_config(2, 1)  # but not over n/2 = 32
+    s.laplacian_solve_mixed(np.ones(n * n), 0)
+    assert s.mixed_config() == ([0, 0, 0, 0], 2)
+    for st in range(4):
+        s.set_mixed_config(st, 0)
+        with pytest.raises(OtmError):
+            s.set_mixed_config(st, 1)
