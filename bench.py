@@ -33,14 +33,17 @@ PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of the two DCT GEMM stage kernels at n = 1024, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by
-# what the counter reads for a 256 MiB device copy (FETCH_SIZE x2.0, WRITE_SIZE
-# x1.0); the runs and their commands are in profiles/gemm_1024_hbm_traffic.md.
+# HBM bytes per launch of the timed DCT stage kernels at n = 1024 (mixed M2
+# with 32- or 64-column tiles, FP64 F1 with tile shape c), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
+# counter reads for a 256 MiB device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
+# the runs and their commands are in profiles/gemm_1024_hbm_traffic.md.
 TRAFFIC_SOURCE = "profiles/gemm_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
-    "k_gemm_x3": (11318.1 * 2.0 + 3750.2) * 1024,
-    "k_gemm_f64": (34021.3 * 2.0 + 7555.7) * 1024,
+    "k_mgemm<M2,32>": (9282.5 * 2.0 + 6144.0) * 1024,
+    "k_mgemm<M2,64>": (8578.9 * 2.0 + 5673.6) * 1024,
+    **{f"k_gemm_fold<F1,cfg{c}>": (f * 2.0 + 8192.0) * 1024
+       for c, f in enumerate([20521.6, 20517.0, 20517.0, 20513.0, 20537.5, 20522.0, 20526.5, 20564.3])},
 }
 
 
@@ -61,6 +64,8 @@ def parse():
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
                              "mixed_solve", "copy"],
                     help="only time this kernel (after one solve that fills the buffers); for profilers")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="with --kernel mixed_gemm / dct_gemm_f64: force this tile config on the timed stage")
     return ap.parse_args()
 
 
@@ -92,10 +97,15 @@ def kernel_only(args):
     s.init(n)
     prec = Precision.MIXED if args.kernel.startswith("mixed") else Precision.FP64
     s.solve(D.smoothed_shapes(n) if n == 1024 else D.gaussian_mixture(n), SolverOptions(precision=prec))
+    if args.tile is not None and args.kernel == "mixed_gemm":
+        s.set_mixed_config(1, args.tile)
+    elif args.tile is not None and args.kernel == "dct_gemm_f64":
+        s.set_gemm_config(0, args.tile)
     ms = _lib.C.c_double()
     _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, KERNEL_IDS[args.kernel], args.profile_reps,
                                                     _lib.C.byref(ms)), "profile")
-    print(json.dumps({"kernel": args.kernel, "n": n, "reps": args.profile_reps, "ms_per_launch": ms.value}))
+    print(json.dumps({"kernel": args.kernel, "n": n, "reps": args.profile_reps, "ms_per_launch": ms.value,
+                      "mixed_tiles": s.mixed_config()[0], "fold_tiles": s.gemm_config()[0]}))
 
 
 def inverse_throughput(n, d_fwd, d_adj, count, reps):
@@ -401,10 +411,11 @@ def main():
 
     head = "mixed" if "mixed" in results else "fp64"
 
-    # Roofline of the dominant kernel of the headline mode: one DCT GEMM stage
-    # (k_gemm_x3 for mixed, k_gemm_f64 for FP64).  Algorithmic work per launch
-    # is one n x n x n product, 2 n^3 flops; the BF16 split multiplicity (6
-    # products) is implementation overhead and is not counted.
+    # Roofline of the dominant kernel of the headline mode: one stage of the
+    # even/odd-folded DCT (k_mgemm stage M2 for mixed, k_gemm_fold stage F1 for
+    # FP64).  Algorithmic work per launch is the folded product, n x n outputs
+    # over n/2 terms = n^3 flops; the BF16 split multiplicity (6 products) is
+    # implementation overhead and is not counted.
     def gemm_roofline(kname, t_ms, peak, label, flops):
         achieved = flops / (t_ms * 1e-3) / 1e12
         traffic = TRAFFIC_BYTES_1024.get(kname) if n == 1024 else None
@@ -415,14 +426,17 @@ def main():
     tiles, n_tiles = s.gemm_config()
     folded = n_tiles > 0
     stage_flops_f64 = float(n) ** 3 if folded else 2.0 * n ** 3
-    roof_f64 = gemm_roofline("k_gemm_fold" if folded else "k_gemm_f64", kernel_ms["dct_gemm_f64"],
-                             PEAK_FP64_MFMA_TFLOPS,
-                             "k_gemm_fold<F1>: one even/odd-folded DCT stage (n^3 flops), FP64 MFMA" if folded else
+    roof_f64 = gemm_roofline(f"k_gemm_fold<F1,cfg{tiles[0]}>" if folded else "k_gemm_f64",
+                             kernel_ms["dct_gemm_f64"], PEAK_FP64_MFMA_TFLOPS,
+                             f"k_gemm_fold<F1> (tile shape {tiles[0]}): one even/odd-folded DCT stage "
+                             "(n^3 flops), FP64 MFMA" if folded else
                              "k_gemm_f64: one DCT stage (2 n^3 flops), FP64 MFMA", stage_flops_f64)
-    roof_x3 = (gemm_roofline("k_gemm_x3", kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
-                             "k_gemm_x3: one DCT stage (2 n^3 flops), 3-term BF16 split, FP32 accumulate",
-                             2.0 * n ** 3)
-               if "mixed" in results else None)
+    roof_x3 = None
+    if "mixed" in results:
+        tn = 32 << s.mixed_config()[0][1]
+        roof_x3 = gemm_roofline(f"k_mgemm<M2,{tn}>", kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
+                                f"k_mgemm<M2,{tn}>: one even/odd-folded DCT stage (n^3 flops), 3-term BF16 "
+                                "split on MFMA, FP32 accumulate", float(n) ** 3)
 
     # Whole-iteration roofline: 26 FP64 field touches per face (218 MB at
     # 1024^2) at HBM peak plus the flops of the four GEMM stages at the MFMA

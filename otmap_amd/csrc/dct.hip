@@ -453,34 +453,16 @@ void dct_tune(JobSet& js, int device, hipStream_t s) {
       return;
     }
   }
-  hipEvent_t e0, e1;
-  OTM_HIP(hipEventCreate(&e0));
-  OTM_HIP(hipEventCreate(&e1));
   std::array<int, 4> best{};
   for (int st = 0; st < 4; ++st) {
-    float best_ms = 1e30f;
-    best[st] = -1;
-    for (int c = 0; c < NFOLD; ++c) {
+    auto valid = [&](int c) {
       bool ok = true;
       for (int n : sizes) ok = ok && fold_cfg_valid(n, c);
-      if (!ok) continue;
-      auto fn = FOLD_TABLE[st][c];
-      for (int rep = 0; rep < 2; ++rep) fn(js, false, false, s);
-      OTM_HIP(hipEventRecord(e0, s));
-      for (int rep = 0; rep < 3; ++rep) fn(js, false, false, s);
-      OTM_HIP(hipEventRecord(e1, s));
-      OTM_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      OTM_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best[st] = c;
-      }
-    }
+      return ok;
+    };
+    best[st] = tune_pick(NFOLD, valid, [&](int c) { FOLD_TABLE[st][c](js, false, false, s); }, s);
     js.cfg[st] = best[st];
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lock(mu);
   cache[key] = best;
 }

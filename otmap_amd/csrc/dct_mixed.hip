@@ -542,35 +542,17 @@ void mixed_tune(JobSet& js, int device, hipStream_t s) {
       return;
     }
   }
-  hipEvent_t e0, e1;
-  OTM_HIP(hipEventCreate(&e0));
-  OTM_HIP(hipEventCreate(&e1));
   std::array<int, 4> best{};
   for (int st = 0; st < 4; ++st) {
-    float best_ms = 1e30f;
-    best[st] = 0;
-    for (int c = 0; c < NMIXED; ++c) {
+    auto valid = [&](int c) {
       bool ok = true;
       for (int n : sizes) ok = ok && mixed_cfg_valid(st, n, c);
-      if (!ok) continue;
-      const MStageFn fn = MSTAGE_TABLE[st][c];
-      const int fin = st == M4 ? FIN_X0 : 0;
-      for (int rep = 0; rep < 2; ++rep) fn(js, false, fin, s);
-      OTM_HIP(hipEventRecord(e0, s));
-      for (int rep = 0; rep < 3; ++rep) fn(js, false, fin, s);
-      OTM_HIP(hipEventRecord(e1, s));
-      OTM_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      OTM_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best[st] = c;
-      }
-    }
+      return ok;
+    };
+    const int fin = st == M4 ? FIN_X0 : 0;
+    best[st] = std::max(0, tune_pick(NMIXED, valid, [&](int c) { MSTAGE_TABLE[st][c](js, false, fin, s); }, s));
     js.mcfg[st] = best[st];
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lock(mu);
   cache[key] = best;
 }

@@ -223,6 +223,40 @@ void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s);
 // fastest in js.cfg
 void dct_tune(JobSet& js, int device, hipStream_t s);
 
+// Picks the fastest of `count` launch configurations on stream s: after two
+// warm-up launches of each valid configuration, TUNE_TRIALS rounds time
+// TUNE_REPS back-to-back launches of every configuration in turn (so drift
+// affects all alike); a configuration's time is its best round.  Ties go to
+// the lower index; -1 when none is valid.
+constexpr int TUNE_TRIALS = 3, TUNE_REPS = 4;
+template <class Valid, class Launch>
+int tune_pick(int count, Valid valid, Launch launch, hipStream_t s) {
+  hipEvent_t e0, e1;
+  OTM_HIP(hipEventCreate(&e0));
+  OTM_HIP(hipEventCreate(&e1));
+  std::vector<float> best(count, 1e30f);
+  for (int c = 0; c < count; ++c)
+    if (valid(c))
+      for (int r = 0; r < 2; ++r) launch(c);
+  for (int trial = 0; trial < TUNE_TRIALS; ++trial)
+    for (int c = 0; c < count; ++c) {
+      if (!valid(c)) continue;
+      OTM_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < TUNE_REPS; ++r) launch(c);
+      OTM_HIP(hipEventRecord(e1, s));
+      OTM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      OTM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      best[c] = std::min(best[c], ms);
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int pick = -1;
+  for (int c = 0; c < count; ++c)
+    if (valid(c) && (pick < 0 || best[c] < best[pick])) pick = c;
+  return pick;
+}
+
 // Mixed-precision Laplacian solve (dct_mixed.hip): folded transform with
 // 3-term BF16 split GEMMs on MFMA + FP64 refinement.  Requires n % 64 == 0.
 struct MixedPlan {
