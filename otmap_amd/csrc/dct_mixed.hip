@@ -102,14 +102,25 @@ __global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, cons
     }
     return v;
   };
-  for (long q = long(lt) * NT + threadIdx.x; q < nh; q += long(nb) * NT) {
+  // four consecutive j per thread (h % 4 == 0): 8-byte stores per plane
+  for (long q4 = long(lt) * NT + threadIdx.x; q4 < nh / 4; q4 += long(nb) * NT) {
+    const long q = q4 * 4;
     const int i = int(q / h), j = int(q - long(i) * h);
-    const double a = src(i, j), b = src(i, n - 1 - j);
-    u16 u0, u1, u2, w0, w1, w2;
-    split3d(a + b, u0, u1, u2);
-    split3d(a - b, w0, w1, w2);
-    out[q] = u0; out[nh + q] = u1; out[2 * nh + q] = u2;
-    out[3 * nh + q] = w0; out[4 * nh + q] = w1; out[5 * nh + q] = w2;
+    u16 u[3][4], w[3][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double a = src(i, j + e), b = src(i, n - 1 - j - e);
+      split3d(a + b, u[0][e], u[1][e], u[2][e]);
+      split3d(a - b, w[0][e], w[1][e], w[2][e]);
+    }
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      u32x2 pu, pw;
+      pu.x = u[pl][0] | (unsigned(u[pl][1]) << 16); pu.y = u[pl][2] | (unsigned(u[pl][3]) << 16);
+      pw.x = w[pl][0] | (unsigned(w[pl][1]) << 16); pw.y = w[pl][2] | (unsigned(w[pl][3]) << 16);
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + q) = pu;
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + (3 + pl) * nh + q) = pw;
+    }
   }
 }
 

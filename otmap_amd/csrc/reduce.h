@@ -84,25 +84,31 @@ __device__ static inline int job_of(const int* __restrict__ starts, int count, i
 }
 
 // pass 2 (single workgroup): fold nb partials of each of K values; valid in
-// thread 0.  Each thread issues FOLD_BATCH independent loads before adding.
+// thread 0.  Each thread issues the FOLD_BATCH loads of all K values before
+// adding (one latency per batch, not one per value).
 constexpr int FOLD_BATCH = 8;
 
 template <int K>
 __device__ static inline void fold_partials(double (&v)[K], const double* partials, int nb) {
-  for (int k = 0; k < K; ++k) {
-    double s = 0.;
-    for (int base = 0; base < nb; base += NT * FOLD_BATCH) {
-      double tmp[FOLD_BATCH];
+  double s[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) s[k] = 0.;
+  for (int base = 0; base < nb; base += NT * FOLD_BATCH) {
+    double tmp[K][FOLD_BATCH];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
 #pragma unroll
       for (int u = 0; u < FOLD_BATCH; ++u) {
         const int j = base + u * NT + threadIdx.x;
-        tmp[u] = (j < nb) ? gp(partials)[long(k) * nb + j] : 0.;
+        tmp[k][u] = (j < nb) ? gp(partials)[long(k) * nb + j] : 0.;
       }
 #pragma unroll
-      for (int u = 0; u < FOLD_BATCH; ++u) s += tmp[u];
-    }
-    v[k] = s;
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int u = 0; u < FOLD_BATCH; ++u) s[k] += tmp[k][u];
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = s[k];
   block_sum<K>(v);
 }
 

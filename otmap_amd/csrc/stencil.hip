@@ -32,14 +32,39 @@ constexpr int NF = (TI + 2) * LDF;  // face-halo values per tile
 constexpr int LDV = TJ + 1;         // tile vertex row length
 constexpr int NV = (TI + 1) * LDV;  // tile vertices
 
-// Loads face values fn(face) on [i0-1, i0+TI] x [j0-1, j0+TJ] (0 outside the grid).
+// Face values fn(face) on [i0-1, i0+TI] x [j0-1, j0+TJ] (0 outside the grid),
+// in two phases so that a kernel can issue all its global loads before the
+// first LDS store: halo_load fills HR registers per thread, halo_store puts
+// them in the LDS tile.
+constexpr int HR = (NF + NT - 1) / NT;  // halo values per thread
+constexpr int VR = (NV + NT - 1) / NT;  // tile vertices per thread
+constexpr int FR = TI * TJ / NT;        // faces per thread
+static_assert(TI * TJ % NT == 0, "whole faces per thread");
+
 template <class Fn>
-__device__ inline void load_halo(double* s, int n, int i0, int j0, Fn fn) {
-  for (int t = threadIdx.x; t < NF; t += NT) {
+__device__ inline void halo_load(double (&v)[HR], int n, int i0, int j0, Fn fn) {
+#pragma unroll
+  for (int u = 0; u < HR; ++u) {
+    const int t = threadIdx.x + u * NT;
     const int a = t / LDF, b = t - a * LDF;
     const int fi = i0 - 1 + a, fj = j0 - 1 + b;
-    s[t] = (fi >= 0 && fi < n && fj >= 0 && fj < n) ? fn(long(fi) * n + fj) : 0.0;
+    v[u] = (t < NF && fi >= 0 && fi < n && fj >= 0 && fj < n) ? fn(long(fi) * n + fj) : 0.0;
   }
+}
+
+__device__ inline void halo_store(double* s, const double (&v)[HR]) {
+#pragma unroll
+  for (int u = 0; u < HR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (t < NF) s[t] = v[u];
+  }
+}
+
+template <class Fn>
+__device__ inline void load_halo(double* s, int n, int i0, int j0, Fn fn) {
+  double v[HR];
+  halo_load(v, n, i0, j0, fn);
+  halo_store(s, v);
 }
 
 // Gradient at vertex (vi, vj) = tile-local (a, b) from the face halo s.
@@ -358,20 +383,35 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
   const double eps = st->alpha / 2.;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
-  load_halo(s, n, i0, j0, [&](long fc) { return psi[fc] - eps * dhat[fc]; });
+  // all global loads of the tile first: halo, then the per-face inputs
+  double hv[HR];
+  halo_load(hv, n, i0, j0, [&](long fc) { return psi[fc] - eps * dhat[fc]; });
+  double rhov[FR], jdv[FR], rv[FR];
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int fi = i0 + t / TJ, fj = j0 + t % TJ;
+    const bool in = fi < n && fj < n;
+    const long fc = long(fi) * n + fj;
+    rhov[u] = in ? rho[fc] : 0.;
+    jdv[u] = in ? jd[fc] : 0.;
+    rv[u] = in ? r[fc] : 0.;
+  }
+  halo_store(s, hv);
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
   const double e = 1. / double(n), A = 1.0 / (double(n) * double(n));
   double acc[2] = {0., 0.};
-  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
-      const long fc = long(fi) * n + fj;
-      const double reps = quad_area(gx, gy, a, b, e) - A * rho[fc];
-      const double j = jd[fc];
-      acc[0] += j * (reps - r[fc]);
+      const double reps = quad_area(gx, gy, a, b, e) - A * rhov[u];
+      const double j = jdv[u];
+      acc[0] += j * (reps - rv[u]);
       acc[1] += j * j;
     }
   }
@@ -443,25 +483,49 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const double beta = st->beta;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
+  // all global loads of the tile first: direction halo, cached G0 of the
+  // tile's vertices, residual of its faces
+  double hv[HR];
   if (use_prev)
-    load_halo(s, n, i0, j0, [&](long fc) { return dhat[fc] + beta * dprev[fc]; });
+    halo_load(hv, n, i0, j0, [&](long fc) { return dhat[fc] + beta * dprev[fc]; });
   else
-    load_halo(s, n, i0, j0, [&](long fc) { return dhat[fc]; });
+    halo_load(hv, n, i0, j0, [&](long fc) { return dhat[fc]; });
   const long V1 = n + 1;
   const long V = V1 * V1;
-  for (int t = threadIdx.x; t < NV; t += NT) {
+  double hxv[VR], hyv[VR];
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
     const int a = t / LDV, b = t - a * LDV;
     const int vi = i0 + a, vj = j0 + b;
-    const bool in = vi <= n && vj <= n;
-    hx[t] = in ? g0[vi * V1 + vj] : 0.;
-    hy[t] = in ? g0[V + vi * V1 + vj] : 0.;
+    const bool in = t < NV && vi <= n && vj <= n;
+    hxv[u] = in ? g0[vi * V1 + vj] : 0.;
+    hyv[u] = in ? g0[V + vi * V1 + vj] : 0.;
+  }
+  double rvv[FR];
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int fi = i0 + t / TJ, fj = j0 + t % TJ;
+    rvv[u] = (fi < n && fj < n) ? r[long(fi) * n + fj] : 0.;
+  }
+  halo_store(s, hv);
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (t < NV) {
+      hx[t] = hxv[u];
+      hy[t] = hyv[u];
+    }
   }
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
   const double e = 1. / double(n);
   double acc[7] = {0., 0., 0., 0., 0., 0., 0.};
-  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
     const int a = t / TJ, b = t % TJ;
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
@@ -480,7 +544,7 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
       const double bv = 0.5 * (ddax * d0by - dday * d0bx + d0ax * ddby - d0ay * ddbx);
       aout[fc] = av;
       bout[fc] = bv;
-      const double rv = r[fc];
+      const double rv = rvv[u];
       acc[0] += bv * rv;
       acc[1] += av * rv;
       acc[2] += bv * bv;
@@ -554,32 +618,62 @@ __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
   const double alpha = st->alpha;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
-  load_halo(s, n, i0, j0, [&](long fc) { return d[fc]; });
-  __syncthreads();
-  tile_grads(s, gx, gy, n, i0, j0);
-  __syncthreads();
+  // all global loads of the tile first: direction halo, G0 of the owned
+  // vertices, psi / r / a / b of the faces (the stores below may alias them)
+  double hv[HR];
+  halo_load(hv, n, i0, j0, [&](long fc) { return d[fc]; });
   const long V1 = n + 1;
   const long V = V1 * V1;
   // vertices owned by this tile: [i0, i0+TI) x [j0, j0+TJ), plus the last row/column of the grid
-  for (int t = threadIdx.x; t < NV; t += NT) {
+  bool own[VR];
+  long vid[VR];
+  double gxo[VR], gyo[VR];
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
     const int va = t / LDV, vb = t - va * LDV;
     const int vi = i0 + va, vj = j0 + vb;
     const bool own_i = (va < TI && vi <= n) || (vi == n);
     const bool own_j = (vb < TJ && vj <= n) || (vj == n);
-    if (own_i && own_j && vi <= n && vj <= n) {
-      const long v = vi * V1 + vj;
-      g0[v] = g0[v] + alpha * gx[t];
-      g0[V + v] = g0[V + v] + alpha * gy[t];
+    own[u] = t < NV && own_i && own_j && vi <= n && vj <= n;
+    vid[u] = vi * V1 + vj;
+    gxo[u] = own[u] ? g0[vid[u]] : 0.;
+    gyo[u] = own[u] ? g0[V + vid[u]] : 0.;
+  }
+  double psv[FR], rkv[FR], av[FR], bv[FR];
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int fi = i0 + t / TJ, fj = j0 + t % TJ;
+    const bool in = fi < n && fj < n;
+    const long fc = long(fi) * n + fj;
+    psv[u] = in ? psi[fc] : 0.;
+    rkv[u] = in ? r[fc] : 0.;
+    av[u] = in ? a[fc] : 0.;
+    bv[u] = in ? b[fc] : 0.;
+  }
+  halo_store(s, hv);
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (own[u]) {
+      g0[vid[u]] = gxo[u] + alpha * gx[t];
+      g0[V + vid[u]] = gyo[u] + alpha * gy[t];
     }
   }
-  for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
     const int fa = t / TJ, fb = t % TJ;
     const int fi = i0 + fa, fj = j0 + fb;
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
-      psi[fc] = psi[fc] + alpha * s[(fa + 1) * LDF + fb + 1];
-      const double rk = r[fc];
-      const double rn = a[fc] * (alpha * alpha) + b[fc] * alpha + rk;
+      psi[fc] = psv[u] + alpha * s[(fa + 1) * LDF + fb + 1];
+      const double rk = rkv[u];
+      const double rn = av[u] * (alpha * alpha) + bv[u] * alpha + rk;
       r[fc] = rn;
       jd[fc] = rn - rk;
     }
