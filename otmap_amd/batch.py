@@ -33,8 +33,11 @@ class BatchSolver:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.lib().otm_batch_destroy(h)
             self._h = None
+            try:
+                _lib.lib().otm_batch_destroy(h)
+            except Exception:  # interpreter shutdown: module globals already cleared
+                pass
 
     def __len__(self):
         return len(self.sizes)

@@ -8,7 +8,11 @@
 // depth >= 24; otherwise split the longest axis at the float-rounded box
 // midpoint with the reference's in-place two-pointer partition.  The same
 // partition keeps the same face order inside leaves and the same child
-// order, so hits come out in the reference's traversal order.
+// order, so hits come out in the reference's traversal order.  Large second
+// children are built by worker threads into their own node lists (their face
+// ranges are disjoint) and appended after the first child's subtree, which is
+// exactly where the sequential build numbers them: the node array is the
+// same as a one-thread build.
 //
 // Device layout: nodes flattened to 48-byte records {box, first, count,
 // escape}, where `escape` is the next node in depth-first order once a
@@ -24,6 +28,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <future>
 #include <vector>
 
 #include "../../include/otmap_capi.h"
@@ -77,7 +82,13 @@ int split(HostBVH& B, int start, int end, int dim, float sv) {
   return c(l) <= sv ? std::min(end, l + 1) : l;
 }
 
-void build_node(HostBVH& B, int id, int start, int end, int level, int leaf_size, int max_depth) {
+// second children of at least PAR_FACES faces, at most PAR_LEVELS levels
+// down (up to 2^PAR_LEVELS - 1 worker threads), are built concurrently
+constexpr int PAR_FACES = 1 << 16;
+constexpr int PAR_LEVELS = 4;
+
+void build_node(HostBVH& B, std::vector<BNode>& nodes, int id, int start, int end, int level, int leaf_size,
+                int max_depth) {
   double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
   for (int k = start; k < end; ++k) {
     int v[4];
@@ -102,18 +113,34 @@ void build_node(HostBVH& B, int id, int start, int end, int level, int leaf_size
   nd.count = std::max(0, end - start);
   nd.escape = -1;
   nd.pad = 0;
-  B.nodes[id] = nd;
+  nodes[id] = nd;
   if (end - start <= leaf_size || level >= max_depth) return;
   const int dim = diag[1] > diag[0] ? 1 : 0;
   const float sv = float(0.5 * (bhi[dim] + blo[dim]));
   const int mid = split(B, start, end, dim, sv);
   if (mid == start || mid == end) return;
-  const int child = int(B.nodes.size());
-  B.nodes[id].first = child;
-  B.nodes[id].count = -1;
-  B.nodes.resize(B.nodes.size() + 2);
-  build_node(B, child, start, mid, level + 1, leaf_size, max_depth);
-  build_node(B, child + 1, mid, end, level + 1, leaf_size, max_depth);
+  const int child = int(nodes.size());
+  nodes[id].first = child;
+  nodes[id].count = -1;
+  nodes.resize(nodes.size() + 2);
+  if (level < PAR_LEVELS && end - mid >= PAR_FACES) {
+    // second child: its own list, local root 0, children numbered from 1
+    std::vector<BNode> sub(1);
+    auto task = std::async(std::launch::async, [&]() {
+      sub.reserve(size_t(end - mid) / 2 + 16);
+      build_node(B, sub, 0, mid, end, level + 1, leaf_size, max_depth);
+    });
+    build_node(B, nodes, child, start, mid, level + 1, leaf_size, max_depth);
+    task.get();
+    const int off = int(nodes.size()) - 1;  // local index k >= 1 -> off + k
+    for (BNode& s : sub)
+      if (s.count < 0) s.first += off;
+    nodes[child + 1] = sub[0];
+    nodes.insert(nodes.end(), sub.begin() + 1, sub.end());
+  } else {
+    build_node(B, nodes, child, start, mid, level + 1, leaf_size, max_depth);
+    build_node(B, nodes, child + 1, mid, end, level + 1, leaf_size, max_depth);
+  }
 }
 
 void build_bvh(HostBVH& B, int n, const double* xy, int leaf_size, int max_depth) {
@@ -134,7 +161,7 @@ void build_bvh(HostBVH& B, int n, const double* xy, int leaf_size, int max_depth
   }
   B.nodes.assign(1, BNode{});
   B.nodes.reserve(size_t(F / 2 + 16));
-  build_node(B, 0, 0, int(F), 0, leaf_size, max_depth);
+  build_node(B, B.nodes, 0, 0, int(F), 0, leaf_size, max_depth);
   // escape links in depth-first order
   std::vector<int> stack = {0};
   B.nodes[0].escape = -1;

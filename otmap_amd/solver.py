@@ -84,8 +84,11 @@ class GridBasedTransportSolver:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.lib().otm_solver_destroy(h)
             self._h = None
+            try:
+                _lib.lib().otm_solver_destroy(h)
+            except Exception:  # interpreter shutdown: module globals already cleared
+                pass
 
     def set_verbose_level(self, v: int) -> None:
         check(_lib.lib().otm_solver_set_verbose_level(self._h, int(v)), "set_verbose_level")

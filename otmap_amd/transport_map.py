@@ -115,8 +115,11 @@ class TransportMap:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.lib().otm_map_destroy(h)
             self._h = None
+            try:
+                _lib.lib().otm_map_destroy(h)
+            except Exception:  # interpreter shutdown: module globals already cleared
+                pass
 
     def origin_mesh(self) -> GridMesh:
         return self._origin

@@ -59,6 +59,36 @@ def test_inverse_vs_oracle(n, seed, oracle):
         assert np.array_equal(inv, oinv, equal_nan=True)
 
 
+def _warped_grid(n):
+    """Regular grid under a smooth displacement that fixes the square's
+    boundary (quads stay convex)."""
+    V = (n + 1) ** 2
+    xy = GridMesh.regular(n).xy.copy()
+    x, y = xy[:V].copy(), xy[V:].copy()
+    xy[:V] = x + 0.01 * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)
+    xy[V:] = y + 0.01 * np.sin(4 * np.pi * x) * np.sin(2 * np.pi * y)
+    return xy
+
+
+def test_inverse_parallel_bvh_build_vs_oracle(oracle):
+    """At n = 512 the BVH's large subtrees are built on worker threads; the
+    node array, hence every hit order and multi-hit selection, must be the
+    one-thread build's (the oracle builds sequentially)."""
+    n = 512
+    xy = _warped_grid(n)
+    dens = np.random.default_rng(3).uniform(0.5, 2.0, n * n)
+    V = (n + 1) ** 2
+    pts = np.concatenate([np.random.default_rng(4).uniform(0, 1, (30000, 2)),
+                          np.stack([xy[:V], xy[V:]], 1)[::37]])
+    tm = TransportMap(GridMesh.regular(n), GridMesh(n, xy), dens)
+    for fast in (False, True):
+        inv, face, nh = tm.inverse(pts, fast=fast)
+        oinv, oface, onh = oracle.inverse(n, xy, dens, pts, fast=fast)
+        assert np.array_equal(face, oface)
+        assert np.array_equal(nh, onh)
+        assert np.array_equal(inv, oinv, equal_nan=True)
+
+
 def test_inverse_cfg0_vs_reference_golden():
     g = load("solve_gaussian256.npz")
     # the map itself comes from the GPU solve (FP64 path); cell ids must match
