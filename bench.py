@@ -433,9 +433,9 @@ def main():
                              "k_gemm_f64: one DCT stage (2 n^3 flops), FP64 MFMA", stage_flops_f64)
     roof_x3 = None
     if "mixed" in results:
-        tn = 32 << s.mixed_config()[0][1]
-        roof_x3 = gemm_roofline(f"k_mgemm<M2,{tn}>", kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
-                                f"k_mgemm<M2,{tn}>: one even/odd-folded DCT stage (n^3 flops), 3-term BF16 "
+        kname = ["k_mgemm<M2,32>", "k_mgemm<M2,64>", "k_mgemm_w<M2>"][s.mixed_config()[0][1]]
+        roof_x3 = gemm_roofline(kname, kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
+                                f"{kname}: one even/odd-folded DCT stage (n^3 flops), 3-term BF16 "
                                 "split on MFMA, FP32 accumulate", float(n) ** 3)
 
     # Whole-iteration roofline: 26 FP64 field touches per face (218 MB at

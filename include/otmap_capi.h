@@ -127,10 +127,11 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
 int otm_solver_gemm_config(const otm_solver* s, int* cfg);
 int otm_solver_set_gemm_config(otm_solver* s, int stage, int config);
 
-/* Tile widths of the four mixed-precision DCT stages (no reference
- * counterpart): config c uses workgroup tiles of 64 x (32 << c) outputs.  The
- * first mixed solve times each valid width per stage and keeps the fastest;
- * every width gives bit-identical results.
+/* Tile configurations of the four mixed-precision DCT stages (no reference
+ * counterpart): config 0 / 1 use workgroup tiles of 64 x 32 / 64 x 64
+ * outputs, config 2 tiles of 128 x 128 (valid when n % 256 == 0).  The first
+ * mixed solve times each valid configuration per stage and keeps the
+ * fastest; every configuration gives bit-identical results.
  *   otm_solver_mixed_config: cfg[0..3] = chosen width per stage, returns the
  *     number of candidate widths (0 before the first mixed solve or when n is
  *     not a multiple of 64);
@@ -260,8 +261,9 @@ int otm_batch_solve_device(otm_batch* b, const double* const* d_density, const o
 int otm_batch_trace(const otm_batch* b, int job, double* residual, double* alpha, double* beta, int cap);
 /* tile shape per folded DCT stage chosen for this batch; returns the number of candidates */
 int otm_batch_gemm_config(const otm_batch* b, int* cfg);
-/* tile width per mixed DCT stage (0: 32 columns, 1: 64) picked on the first
- * mixed solve of the batch; returns the number of candidates (0 before it) */
+/* tile configuration per mixed DCT stage (see otm_solver_mixed_config) picked
+ * on the first mixed solve of the batch; returns the number of candidates (0
+ * before it) */
 int otm_batch_mixed_config(const otm_batch* b, int* cfg);
 
 #ifdef __cplusplus

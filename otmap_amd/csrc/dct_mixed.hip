@@ -417,6 +417,288 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide form of a stage for large launches (config 2): 128 x 128 workgroup
+// tiles (M1: 64 rows and their 64 mirrors), four waves of 64 x 64, so a wave
+// issues 96 MFMAs per 24 LDS fragment reads.  One K group: the steps below
+// ceil(S/2) accumulate into a register block that is parked when step
+// ceil(S/2) starts, and the epilogue adds (or subtracts) the two FP32 halves
+// exactly as the two-group form does, so results are bit-identical.
+// ---------------------------------------------------------------------------
+constexpr int WT = 128;  // wide tile rows / columns
+
+template <int ST>
+__global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                               int count, bool gate, int fin) {
+  const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
+  const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
+  const int nbk = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
+  const Job& J = jobs[p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const int done = gate ? st->done : 0;
+  const int n = J.f.n, h = n / 2;
+  const long nh = long(n) * h, nn = long(n) * n;
+  constexpr int TI = 4, TJ = 4;  // MFMA tiles per wave
+  constexpr int CH = 3 * WT * 4 / NT;  // 16-byte chunks per thread per operand and K step (6)
+  const int ntn = (ST == M1 || ST == M2 ? n : h) / WT;
+  const int ntm = ST == M1 ? h / (WT / 2) : n / WT;
+  if (lt >= ntm * ntn) {
+    if (ST == M4 && (fin == FIN_OUT || fin == FIN_ADD_OUT)) {
+      double z[1] = {0.};
+      write_partials_at<1>(z, J.f.site(SITE_DNORM), lt, nbk);
+    }
+    return;
+  }
+  int tm, tn;
+  xcd_tile(lt, ntm, ntn, tm, tn);
+  const int m0 = tm * (ST == M1 ? WT / 2 : WT), n0 = tn * WT;
+  const u16* Ab;
+  const u16* Bb;
+  int lda, ldb;
+  long aps, bps;
+  int K;
+  if (ST == M1) {
+    Ab = J.f.mrhs + (n0 >= h ? 3 * nh : 0); lda = h; aps = nh;
+    Bb = J.Pp; ldb = h; bps = nh; K = h;
+  } else if (ST == M2) {
+    Ab = J.f.mwa + (n0 >= h ? 3 * nh : 0); lda = h; aps = nh;
+    Bb = J.Pp; ldb = h; bps = nh; K = h;
+  } else if (ST == M3) {
+    Ab = J.f.mwb; lda = n; aps = nn;
+    Bb = J.Ptp; ldb = n; bps = nh; K = n;
+  } else {
+    Ab = J.f.mwa; lda = n; aps = nn;
+    Bb = J.Ptp; ldb = n; bps = nh; K = n;
+  }
+  const auto A = gp(Ab);
+  const auto B = gp(Bb);
+
+  constexpr int O_U16 = 2 * 3 * WT * MLD;  // one operand, two buffers
+  __shared__ __attribute__((aligned(16))) u16 smem[2 * O_U16];
+  auto As = [&](int buf, int pl, int row) { return smem + ((buf * 3 + pl) * WT + row) * MLD; };
+  auto Bs = [&](int buf, int pl, int row) { return smem + O_U16 + ((buf * 3 + pl) * WT + row) * MLD; };
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int lr = lane & 15, lq = lane >> 4;
+  auto arow = [&](int row) {
+    if (ST == M1) return row < WT / 2 ? m0 + row : n - m0 - WT + row;
+    return m0 + row;
+  };
+
+  __shared__ double cfac[2 * WT];
+  double cf = 0.;
+  if (ST == M2) cf = gp(J.ck)[t < WT ? m0 + t : n0 + t - WT];
+
+  // chunk c of an operand step: plane c / 512, row (c % 512) / 4, k offset (c % 4) * 8
+  u32x4 ra[2][CH], rb[2][CH];
+  auto gload = [&](u32x4 (&xa)[CH], u32x4 (&xb)[CH], int k0) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int c = t + u * NT, pl = c / (WT * 4), rem = c % (WT * 4), row = rem >> 2, kc = (rem & 3) * 8;
+      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + pl * aps + long(arow(row)) * lda + k0 + kc);
+      xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + long(n0 + row) * ldb + k0 + kc);
+    }
+  };
+  auto sstore = [&](const u32x4 (&xa)[CH], const u32x4 (&xb)[CH], int buf) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int c = t + u * NT, pl = c / (WT * 4), rem = c % (WT * 4), row = rem >> 2, kc = (rem & 3) * 8;
+      *reinterpret_cast<u32x4*>(As(buf, pl, row) + kc) = xa[u];
+      *reinterpret_cast<u32x4*>(Bs(buf, pl, row) + kc) = xb[u];
+    }
+  };
+  f4 acc[TI][TJ], lo[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      lo[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  auto compute = [&](int buf) {
+    bf8 fa[TI][3], fb[TJ][3];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(As(buf, pl, wm + 16 * i + lr) + lq * 8);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) fb[j][pl] = *reinterpret_cast<const bf8*>(Bs(buf, pl, wn + 16 * j + lr) + lq * 8);
+    }
+    constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
+    constexpr int PB[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[j][PB[q]], acc[i][j], 0, 0, 0);
+  };
+  // the two-group form's K split: steps [0, S0) then [S0, S)
+  const int S = K / MKB, S0 = (S + 1) >> 1, klast = K - MKB;
+  auto step = [&](u32x4 (&xa)[CH], u32x4 (&xb)[CH], int s, int buf) {
+    if (s == S0) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+          lo[i][j] = acc[i][j];
+          acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+    sstore(xa, xb, buf);
+    gload(xa, xb, min((s + 2) * MKB, klast));
+    __syncthreads();
+    compute(buf);
+  };
+  gload(ra[0], rb[0], 0);
+  gload(ra[1], rb[1], min(MKB, klast));
+  if (done) return;
+  if (ST == M2) cfac[t] = cf;
+  int s = 0;
+  for (; s + 2 <= S; s += 2) {
+    step(ra[0], rb[0], s, 0);
+    step(ra[1], rb[1], s + 1, 1);
+  }
+  if (s < S) step(ra[0], rb[0], s, 0);
+  if (S0 == S) {  // every step was in the first half (S == 1)
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        lo[i][j] = acc[i][j];
+        acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+  }
+  __syncthreads();
+
+  // lane (lr, lq) of tile (i, j) holds D[wm + 16i + 4lq + v][wn + 16j + lr]; lo = k-steps < S0, acc = the rest
+  if (ST == M4) {
+    const auto x0 = gp(J.f.x0);
+    const auto dh = gp(J.f.dhat);
+    double nrm[1] = {0.};
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const long row = m0 + wm + 16 * i + 4 * lq + v;
+          const int col = n0 + wn + 16 * j + lr;
+          const double p0 = double(lo[i][j][v]), p1 = double(acc[i][j][v]);
+          const long ia = row * n + col, ib = row * n + (n - 1 - col);
+          double va = p0 + p1, vb = p0 - p1;
+          if (fin == FIN_ADD_OUT || fin == FIN_ADD_X0) {
+            va = x0[ia] + va;
+            vb = x0[ib] + vb;
+          }
+          if (fin == FIN_X0 || fin == FIN_ADD_X0) {
+            x0[ia] = va;
+            x0[ib] = vb;
+          } else {
+            dh[ia] = va;
+            dh[ib] = vb;
+            nrm[0] += va * va + vb * vb;
+          }
+        }
+    if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nbk);
+    return;
+  }
+
+  // the other stages stage one FP32 tile [128][132] in LDS per round
+  constexpr int DL = WT + 4;
+  float* const Ds = reinterpret_cast<float*>(smem);
+  auto stage = [&](float sign) {
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          Ds[(wm + 16 * i + 4 * lq + v) * DL + wn + 16 * j + lr] =
+              sign > 0.f ? lo[i][j][v] + acc[i][j][v] : lo[i][j][v] - acc[i][j][v];
+    __syncthreads();
+  };
+  if (ST == M1) {
+    stage(1.f);
+    const auto out = gp(J.f.mwa);
+    for (int it = t; it < WT * (WT / 8); it += NT) {
+      const int c = it / (WT / 8), a0 = (it % (WT / 8)) * 4;
+      u16 u[3][4], wv[3][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float top = Ds[(a0 + q) * DL + c], mir = Ds[(WT - 1 - a0 - q) * DL + c];
+        split3(top + mir, u[0][q], u[1][q], u[2][q]);
+        split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
+      }
+      const long base = long(n0 + c) * h + m0 + a0;
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) {
+        u32x2 pu, pw;
+        pu.x = u[pl][0] | (unsigned(u[pl][1]) << 16); pu.y = u[pl][2] | (unsigned(u[pl][3]) << 16);
+        pw.x = wv[pl][0] | (unsigned(wv[pl][1]) << 16); pw.y = wv[pl][2] | (unsigned(wv[pl][3]) << 16);
+        *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + base) = pu;
+        *reinterpret_cast<OTM_GLOBAL u32x2*>(out + 3 * nh + pl * nh + base) = pw;
+      }
+    }
+  } else if (ST == M2) {
+    stage(1.f);
+    const auto out = gp(J.f.mwb);
+    for (int it = t; it < WT * WT / 8; it += NT) {
+      const int r = it / (WT / 8), c0 = (it % (WT / 8)) * 8;
+      const int lrow = m0 + r;
+      const double cl = cfac[r];
+      u16 z[3][8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kc = n0 + c0 + q;
+        const float lam = float(2. * (1. - cfac[WT + c0 + q] * cl));
+        const float y = Ds[r * DL + c0 + q];
+        const float val = (kc == 0 && lrow == 0) ? 0.f : y / lam;
+        split3(val, z[0][q], z[1][q], z[2][q]);
+      }
+      const long base = long(lrow) * n + n0 + c0;
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) {
+        u32x4 pz;
+        pz.x = z[pl][0] | (unsigned(z[pl][1]) << 16); pz.y = z[pl][2] | (unsigned(z[pl][3]) << 16);
+        pz.z = z[pl][4] | (unsigned(z[pl][5]) << 16); pz.w = z[pl][6] | (unsigned(z[pl][7]) << 16);
+        *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + base) = pz;
+      }
+    }
+  } else {
+    // T3[i = n0 + c][l = m0 + r0 .. r0+7] = lo + hi (round 0), T3[n-1-i][l] = lo - hi (round 1)
+    const auto out = gp(J.f.mwa);
+    for (int round = 0; round < 2; ++round) {
+      if (round == 1) __syncthreads();  // round 0's reads of the staged tile are done
+      stage(round == 0 ? 1.f : -1.f);
+      for (int it = t; it < WT * (WT / 8); it += NT) {
+        const int c = it / (WT / 8), r0 = (it % (WT / 8)) * 8;
+        u16 sv[3][8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) split3(Ds[(r0 + q) * DL + c], sv[0][q], sv[1][q], sv[2][q]);
+        const long b0 = round == 0 ? long(n0 + c) * n + m0 + r0 : long(n - 1 - n0 - c) * n + m0 + r0;
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+          u32x4 pv;
+          pv.x = sv[pl][0] | (unsigned(sv[pl][1]) << 16); pv.y = sv[pl][2] | (unsigned(sv[pl][3]) << 16);
+          pv.z = sv[pl][4] | (unsigned(sv[pl][5]) << 16); pv.w = sv[pl][6] | (unsigned(sv[pl][7]) << 16);
+          *reinterpret_cast<OTM_GLOBAL u32x4*>(out + pl * nn + b0) = pv;
+        }
+      }
+    }
+  }
+}
+
+template <int ST>
+void launch_mstage_w(const JobSet& js, bool gate, int fin, hipStream_t s) {
+  const int kind = mixed_kind(ST, 2);
+  if (js.total[kind] == 0) return;
+  hipLaunchKernelGGL((k_mgemm_w<ST>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s, (const Job*)js.jobs,
+                     (const int*)js.starts_of(kind), js.count, gate, fin);
+  OTM_HIP(hipGetLastError());
+}
+
 template <int ST, int TN>
 void launch_mstage_tn(const JobSet& js, bool gate, int fin, hipStream_t s) {
   const int kind = mixed_kind(ST, TN == 32 ? 0 : 1);
@@ -430,8 +712,10 @@ template <int ST>
 void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
   if (js.mcfg[ST] == 0)
     launch_mstage_tn<ST, 32>(js, gate, fin, s);
-  else
+  else if (js.mcfg[ST] == 1)
     launch_mstage_tn<ST, 64>(js, gate, fin, s);
+  else
+    launch_mstage_w<ST>(js, gate, fin, s);
 }
 
 template <int MODE>
@@ -451,11 +735,11 @@ void mixed_transform(const JobSet& js, bool gate, int fin, hipStream_t s) {
 }  // namespace
 
 int mixed_tiles(int st, int n, int cfg) {
-  const int h = n / 2, tn = MIXED_TN[cfg];
+  const int h = n / 2, tn = MIXED_TN[cfg], tm = MIXED_TM[cfg];
   switch (st) {
-    case M1: return (h / (MTM / 2)) * (n / tn);
-    case M2: return (n / MTM) * (n / tn);
-    default: return (n / MTM) * (h / tn);
+    case M1: return (h / (tm / 2)) * (n / tn);
+    case M2: return (n / tm) * (n / tn);
+    default: return (n / tm) * (h / tn);
   }
 }
 
@@ -526,10 +810,10 @@ void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s) { launch_mstage<M2
 namespace {
 using MStageFn = void (*)(const JobSet&, bool, int, hipStream_t);
 const MStageFn MSTAGE_TABLE[4][NMIXED] = {
-    {launch_mstage_tn<M1, 32>, launch_mstage_tn<M1, 64>},
-    {launch_mstage_tn<M2, 32>, launch_mstage_tn<M2, 64>},
-    {launch_mstage_tn<M3, 32>, launch_mstage_tn<M3, 64>},
-    {launch_mstage_tn<M4, 32>, launch_mstage_tn<M4, 64>},
+    {launch_mstage_tn<M1, 32>, launch_mstage_tn<M1, 64>, launch_mstage_w<M1>},
+    {launch_mstage_tn<M2, 32>, launch_mstage_tn<M2, 64>, launch_mstage_w<M2>},
+    {launch_mstage_tn<M3, 32>, launch_mstage_tn<M3, 64>, launch_mstage_w<M3>},
+    {launch_mstage_tn<M4, 32>, launch_mstage_tn<M4, 64>, launch_mstage_w<M4>},
 };
 }  // namespace
 

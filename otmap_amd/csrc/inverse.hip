@@ -29,6 +29,7 @@
 #include <cfloat>
 #include <cstring>
 #include <future>
+#include <thread>
 #include <vector>
 
 #include "../../include/otmap_capi.h"
@@ -86,18 +87,48 @@ int split(HostBVH& B, int start, int end, int dim, float sv) {
 // down (up to 2^PAR_LEVELS - 1 worker threads), are built concurrently
 constexpr int PAR_FACES = 1 << 16;
 constexpr int PAR_LEVELS = 4;
+constexpr int MAX_THREADS = 16;
+
+// fn(begin, end, t) over up to MAX_THREADS contiguous chunks of [0, count)
+// of at least min_chunk items, chunk 0 on the calling thread
+template <class Fn>
+int parallel_chunks(long count, long min_chunk, Fn fn) {
+  const long hw = std::max(1u, std::thread::hardware_concurrency());
+  const int T = int(std::max(1L, std::min({long(MAX_THREADS), hw, count / std::max(1L, min_chunk)})));
+  if (T <= 1) {
+    fn(0L, count, 0);
+    return 1;
+  }
+  const long per = (count + T - 1) / T;
+  std::vector<std::thread> th;
+  for (int t = 1; t < T; ++t) th.emplace_back(fn, std::min(count, t * per), std::min(count, (t + 1) * per), t);
+  fn(0L, std::min(count, per), 0);
+  for (auto& x : th) x.join();
+  return T;
+}
 
 void build_node(HostBVH& B, std::vector<BNode>& nodes, int id, int start, int end, int level, int leaf_size,
                 int max_depth) {
-  double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
-  for (int k = start; k < end; ++k) {
-    int v[4];
-    quad_vertices(B.n, B.faces[k], v);
-    for (int q = 0; q < 4; ++q) {
-      const P2 p = B.pts[v[q]];
-      lo[0] = std::min(lo[0], p.x); lo[1] = std::min(lo[1], p.y);
-      hi[0] = std::max(hi[0], p.x); hi[1] = std::max(hi[1], p.y);
+  // box of the faces' vertices (min / max are exact, so chunked is the same)
+  double part[MAX_THREADS][4];
+  auto box = [&](long b, long e, int t) {
+    double l0 = DBL_MAX, l1 = DBL_MAX, h0 = -DBL_MAX, h1 = -DBL_MAX;
+    for (long k = start + b; k < start + e; ++k) {
+      int v[4];
+      quad_vertices(B.n, B.faces[k], v);
+      for (int q = 0; q < 4; ++q) {
+        const P2 p = B.pts[v[q]];
+        l0 = std::min(l0, p.x); l1 = std::min(l1, p.y);
+        h0 = std::max(h0, p.x); h1 = std::max(h1, p.y);
+      }
     }
+    part[t][0] = l0; part[t][1] = l1; part[t][2] = h0; part[t][3] = h1;
+  };
+  const int nt = level == 0 ? parallel_chunks(end - start, 1L << 18, box) : (box(0, end - start, 0), 1);
+  double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
+  for (int t = 0; t < nt; ++t) {
+    lo[0] = std::min(lo[0], part[t][0]); lo[1] = std::min(lo[1], part[t][1]);
+    hi[0] = std::max(hi[0], part[t][2]); hi[1] = std::max(hi[1], part[t][3]);
   }
   const double eps = DBL_EPSILON;
   const double diag[2] = {hi[0] - lo[0], hi[1] - lo[1]};
@@ -147,18 +178,22 @@ void build_bvh(HostBVH& B, int n, const double* xy, int leaf_size, int max_depth
   B.n = n;
   const long V = long(n + 1) * (n + 1), F = long(n) * n;
   B.pts.resize(V);
-  for (long v = 0; v < V; ++v) B.pts[v] = {xy[v], xy[V + v]};
+  parallel_chunks(V, 1L << 18, [&](long b, long e, int) {
+    for (long v = b; v < e; ++v) B.pts[v] = {xy[v], xy[V + v]};
+  });
   B.faces.resize(F);
   B.cent.resize(F);
-  for (long f = 0; f < F; ++f) {
-    int v[4];
-    quad_vertices(n, int(f), v);
-    P2 c = {0., 0.};
-    for (int q = 0; q < 4; ++q) { c.x += B.pts[v[q]].x; c.y += B.pts[v[q]].y; }
-    c.x /= 4; c.y /= 4;
-    B.cent[f] = c;
-    B.faces[f] = int(f);
-  }
+  parallel_chunks(F, 1L << 18, [&](long b, long e, int) {
+    for (long f = b; f < e; ++f) {
+      int v[4];
+      quad_vertices(n, int(f), v);
+      P2 c = {0., 0.};
+      for (int q = 0; q < 4; ++q) { c.x += B.pts[v[q]].x; c.y += B.pts[v[q]].y; }
+      c.x /= 4; c.y /= 4;
+      B.cent[f] = c;
+      B.faces[f] = int(f);
+    }
+  });
   B.nodes.assign(1, BNode{});
   B.nodes.reserve(size_t(F / 2 + 16));
   build_node(B, B.nodes, 0, 0, int(F), 0, leaf_size, max_depth);

@@ -153,8 +153,9 @@ inline int fold_tiles(int n, int st, int c) {
 inline int flat_blocks(long F) { return int(std::min<long>((F + NT * 4 - 1) / (NT * 4), 1024)); }
 
 // mixed-transform tile widths (output columns per workgroup), one per config
-constexpr int NMIXED = 2;
-constexpr int MIXED_TN[NMIXED] = {32, 64};
+constexpr int NMIXED = 3;
+constexpr int MIXED_TN[NMIXED] = {32, 64, 128};  // config 2: 128 x 128 tiles, four 64 x 64 waves
+constexpr int MIXED_TM[NMIXED] = {64, 64, 128};
 
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
 // mixed_kind(s, c): stage s of the mixed transform with tile width MIXED_TN[c]
