@@ -420,12 +420,36 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
 // ---------------------------------------------------------------------------
 // Wide form of a stage for large launches (config 2): 128 x 128 workgroup
 // tiles (M1: 64 rows and their 64 mirrors), four waves of 64 x 64, so a wave
-// issues 96 MFMAs per 24 LDS fragment reads.  One K group: the steps below
-// ceil(S/2) accumulate into a register block that is parked when step
-// ceil(S/2) starts, and the epilogue adds (or subtracts) the two FP32 halves
-// exactly as the two-group form does, so results are bit-identical.
+// issues 96 MFMAs per 24 LDS fragment reads.  Operand tiles go from global
+// memory straight into LDS (global_load_lds_dwordx4: no register staging, no
+// LDS store instructions), three K-step buffers deep.  The K steps of the two
+// halves (k < K/2 into `lo`, k >= K/2 into `acc`) are processed alternately,
+// each half in increasing k, so every output accumulates in the same order as
+// the two-group form and the epilogue adds (or subtracts) the two FP32 halves
+// exactly as it does: results are bit-identical.
 // ---------------------------------------------------------------------------
 constexpr int WT = 128;  // wide tile rows / columns
+
+// One 16-byte global -> LDS transfer per lane (global_load_lds_dwordx4): lane
+// l's 16 bytes at sbase + voff land at LDS byte lds + 16 l (lds wave-uniform).
+// Issued from asm so that the compiler's wait analysis, which cannot tell the
+// three K-step buffers apart, does not drain every transfer before each LDS
+// read; the kernel waits for them explicitly (vmcnt) before the barrier that
+// publishes a buffer.  The instruction has no register results.
+__device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds) {
+  asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(sbase), "s"(lds) : "memory");
+}
+// Wait until at most N of this wave's vector memory operations are
+// outstanding and its LDS reads have completed, then the workgroup barrier
+// (also a compiler memory barrier: no memory access moves across it).
+template <int N>
+__device__ inline void wait_vm_barrier() {
+  static_assert(N == 0 || N == 12, "waits used by the wide kernel");
+  if (N == 0)
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else
+    asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 template <int ST>
 __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, const int* __restrict__ starts,
@@ -439,7 +463,6 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   const int n = J.f.n, h = n / 2;
   const long nh = long(n) * h, nn = long(n) * n;
   constexpr int TI = 4, TJ = 4;  // MFMA tiles per wave
-  constexpr int CH = 3 * WT * 4 / NT;  // 16-byte chunks per thread per operand and K step (6)
   const int ntn = (ST == M1 || ST == M2 ? n : h) / WT;
   const int ntm = ST == M1 ? h / (WT / 2) : n / WT;
   if (lt >= ntm * ntn) {
@@ -473,13 +496,17 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  constexpr int O_U16 = 2 * 3 * WT * MLD;  // one operand, two buffers
-  __shared__ __attribute__((aligned(16))) u16 smem[2 * O_U16];
-  auto As = [&](int buf, int pl, int row) { return smem + ((buf * 3 + pl) * WT + row) * MLD; };
-  auto Bs = [&](int buf, int pl, int row) { return smem + O_U16 + ((buf * 3 + pl) * WT + row) * MLD; };
+  // LDS: three K-step buffers, each [operand A, B][plane 3][row 128][4 chunks
+  // of 8 k values]; chunk c of row r sits at position c ^ swz(r), which puts
+  // the 16 rows of a fragment read on 16 distinct 16-byte bank groups
+  constexpr int SLOTS = 2 * 3 * WT * 4;  // 16-byte chunks per K step
+  constexpr int BUF = SLOTS * 16;        // bytes per buffer
+  constexpr int DMA = SLOTS / NT;        // transfers per thread per K step (12)
+  __shared__ __attribute__((aligned(16))) unsigned char smem[3 * BUF];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
   const int lr = lane & 15, lq = lane >> 4;
+  auto swz = [](int r) { return ((r >> 1) ^ (r >> 3)) & 3; };
   auto arow = [&](int row) {
     if (ST == M1) return row < WT / 2 ? m0 + row : n - m0 - WT + row;
     return m0 + row;
@@ -489,23 +516,33 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   double cf = 0.;
   if (ST == M2) cf = gp(J.ck)[t < WT ? m0 + t : n0 + t - WT];
 
-  // chunk c of an operand step: plane c / 512, row (c % 512) / 4, k offset (c % 4) * 8
-  u32x4 ra[2][CH], rb[2][CH];
-  auto gload = [&](u32x4 (&xa)[CH], u32x4 (&xb)[CH], int k0) {
+  // transfer u of this lane (u < 6: A, else B) covers chunk slot
+  // (4u + w) * 64 + lane of a step; its byte offset from the operand base
+  // without the K-step term, and the wave-uniform LDS offset of the transfer
+  unsigned voff[DMA];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int c = t + u * NT, pl = c / (WT * 4), rem = c % (WT * 4), row = rem >> 2, kc = (rem & 3) * 8;
-      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + pl * aps + long(arow(row)) * lda + k0 + kc);
-      xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + long(n0 + row) * ldb + k0 + kc);
-    }
+  for (int u = 0; u < DMA; ++u) {
+    const int slot = (4 * u + w) * 64 + lane, pl = (slot % (SLOTS / 2)) / (WT * 4);
+    const int r = (slot % (WT * 4)) >> 2, kc = (slot & 3) ^ swz(r);
+    const long e = u < DMA / 2 ? pl * aps + long(arow(r)) * lda + kc * 8 : pl * bps + long(n0 + r) * ldb + kc * 8;
+    voff[u] = unsigned(e * 2);
+  }
+  const unsigned lds0 = unsigned(size_t((__attribute__((address_space(3))) unsigned char*)smem));
+  const unsigned wlds = lds0 + unsigned(__builtin_amdgcn_readfirstlane(w)) * 1024;
+  const int S = K / MKB, S0 = S >> 1;
+  // K step of position q in the alternating order: 0, S0, 1, S0 + 1, ...
+  auto kstep = [&](int q) { return (q & 1) ? S0 + (q >> 1) : (q >> 1); };
+  auto issue = [&](int q, int buf) {
+    const int k0 = kstep(q) * MKB;
+    const u16* a = Ab + k0;
+    const u16* b = Bb + k0;
+#pragma unroll
+    for (int u = 0; u < DMA; ++u)
+      lds_dma16(u < DMA / 2 ? (const void*)a : (const void*)b, voff[u], wlds + buf * BUF + u * 4096);
   };
-  auto sstore = [&](const u32x4 (&xa)[CH], const u32x4 (&xb)[CH], int buf) {
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int c = t + u * NT, pl = c / (WT * 4), rem = c % (WT * 4), row = rem >> 2, kc = (rem & 3) * 8;
-      *reinterpret_cast<u32x4*>(As(buf, pl, row) + kc) = xa[u];
-      *reinterpret_cast<u32x4*>(Bs(buf, pl, row) + kc) = xb[u];
-    }
+  auto frag = [&](int buf, int op, int pl, int r) {
+    return *reinterpret_cast<const bf8*>(smem + buf * BUF + op * (BUF / 2) +
+                                         (((pl * WT + r) * 4 + (lq ^ swz(r))) << 4));
   };
   f4 acc[TI][TJ], lo[TI][TJ];
 #pragma unroll
@@ -515,62 +552,65 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
       acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
       lo[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     }
-  auto compute = [&](int buf) {
+  // Position q (buffer q % 3): wait for this wave's transfers of q (issued
+  // two positions earlier; those of q + 1 may stay in flight), barrier (every
+  // wave's transfers of q are in LDS, and every wave is done reading buffer
+  // (q + 2) % 3, last read at position q - 1), start the transfers of q + 2,
+  // read the fragments and issue the 96 MFMAs.
+  auto body = [&](f4 (&c)[TI][TJ], int q, int buf) {
+    if (q + 1 < S)
+      wait_vm_barrier<12>();
+    else
+      wait_vm_barrier<0>();
+    if (q + 2 < S) issue(q + 2, (buf + 2) % 3);
     bf8 fa[TI][3], fb[TJ][3];
+    constexpr int RA[3] = {2, 1, 0};
+    constexpr int RB[3] = {0, 1, 2};
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
+    for (int x = 0; x < 3; ++x) {
 #pragma unroll
-      for (int i = 0; i < TI; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(As(buf, pl, wm + 16 * i + lr) + lq * 8);
+      for (int i = 0; i < TI; ++i) fa[i][RA[x]] = frag(buf, 0, RA[x], wm + 16 * i + lr);
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) fb[j][pl] = *reinterpret_cast<const bf8*>(Bs(buf, pl, wn + 16 * j + lr) + lq * 8);
+      for (int j = 0; j < TJ; ++j) fb[j][RB[x]] = frag(buf, 1, RB[x], wn + 16 * j + lr);
     }
     constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
     constexpr int PB[6] = {0, 1, 2, 0, 1, 0};
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
+    for (int x = 0; x < 6; ++x)
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
         for (int j = 0; j < TJ; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[j][PB[q]], acc[i][j], 0, 0, 0);
+          c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[x]], fb[j][PB[x]], c[i][j], 0, 0, 0);
   };
-  // the two-group form's K split: steps [0, S0) then [S0, S)
-  const int S = K / MKB, S0 = (S + 1) >> 1, klast = K - MKB;
-  auto step = [&](u32x4 (&xa)[CH], u32x4 (&xb)[CH], int s, int buf) {
-    if (s == S0) {
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-          lo[i][j] = acc[i][j];
-          acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-        }
-    }
-    sstore(xa, xb, buf);
-    gload(xa, xb, min((s + 2) * MKB, klast));
-    __syncthreads();
-    compute(buf);
-  };
-  gload(ra[0], rb[0], 0);
-  gload(ra[1], rb[1], min(MKB, klast));
-  if (done) return;
+  issue(0, 0);
+  issue(1, 1);
+  if (done) {  // no transfer may outlive the workgroup
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    return;
+  }
   if (ST == M2) cfac[t] = cf;
-  int s = 0;
-  for (; s + 2 <= S; s += 2) {
-    step(ra[0], rb[0], s, 0);
-    step(ra[1], rb[1], s + 1, 1);
+  // S is a multiple of 4: six positions per round (buffers 0, 1, 2, 0, 1, 2),
+  // then 2 or 4 more (buffers 0, 1 and 2, 0)
+  int q = 0;
+  for (; q + 6 <= S; q += 6) {
+    body(lo, q, 0);
+    body(acc, q + 1, 1);
+    body(lo, q + 2, 2);
+    body(acc, q + 3, 0);
+    body(lo, q + 4, 1);
+    body(acc, q + 5, 2);
   }
-  if (s < S) step(ra[0], rb[0], s, 0);
-  if (S0 == S) {  // every step was in the first half (S == 1)
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int j = 0; j < TJ; ++j) {
-        lo[i][j] = acc[i][j];
-        acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-      }
+  if (q + 2 <= S) {
+    body(lo, q, 0);
+    body(acc, q + 1, 1);
+    q += 2;
   }
-  __syncthreads();
+  if (q + 2 <= S) {
+    body(lo, q, 2);
+    body(acc, q + 1, 0);
+  }
+  __syncthreads();  // every wave is done reading the buffers before the epilogue reuses them
 
   // lane (lr, lq) of tile (i, j) holds D[wm + 16i + 4lq + v][wn + 16j + lr]; lo = k-steps < S0, acc = the rest
   if (ST == M4) {
@@ -699,9 +739,11 @@ void launch_mstage_w(const JobSet& js, bool gate, int fin, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
-template <int ST, int TN>
+// config c of the two-group form: tile width MIXED_TN[c]
+template <int ST, int C>
 void launch_mstage_tn(const JobSet& js, bool gate, int fin, hipStream_t s) {
-  const int kind = mixed_kind(ST, TN == 32 ? 0 : 1);
+  constexpr int TN = MIXED_TN[C];
+  const int kind = mixed_kind(ST, C);
   if (js.total[kind] == 0) return;
   hipLaunchKernelGGL((k_mgemm<ST, TN>), dim3(unsigned(js.total[kind])), dim3(MNT), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(kind), js.count, gate, fin);
@@ -710,12 +752,11 @@ void launch_mstage_tn(const JobSet& js, bool gate, int fin, hipStream_t s) {
 
 template <int ST>
 void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
-  if (js.mcfg[ST] == 0)
-    launch_mstage_tn<ST, 32>(js, gate, fin, s);
-  else if (js.mcfg[ST] == 1)
-    launch_mstage_tn<ST, 64>(js, gate, fin, s);
-  else
-    launch_mstage_w<ST>(js, gate, fin, s);
+  switch (js.mcfg[ST]) {
+    case 0: launch_mstage_tn<ST, 0>(js, gate, fin, s); break;
+    case 1: launch_mstage_tn<ST, 1>(js, gate, fin, s); break;
+    default: launch_mstage_w<ST>(js, gate, fin, s); break;
+  }
 }
 
 template <int MODE>
@@ -809,12 +850,12 @@ void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s) { launch_mstage<M2
 
 namespace {
 using MStageFn = void (*)(const JobSet&, bool, int, hipStream_t);
-const MStageFn MSTAGE_TABLE[4][NMIXED] = {
-    {launch_mstage_tn<M1, 32>, launch_mstage_tn<M1, 64>, launch_mstage_w<M1>},
-    {launch_mstage_tn<M2, 32>, launch_mstage_tn<M2, 64>, launch_mstage_w<M2>},
-    {launch_mstage_tn<M3, 32>, launch_mstage_tn<M3, 64>, launch_mstage_w<M3>},
-    {launch_mstage_tn<M4, 32>, launch_mstage_tn<M4, 64>, launch_mstage_w<M4>},
-};
+template <int ST>
+constexpr std::array<MStageFn, NMIXED> mstage_row() {
+  return {launch_mstage_tn<ST, 0>, launch_mstage_tn<ST, 1>, launch_mstage_w<ST>};
+}
+const std::array<MStageFn, NMIXED> MSTAGE_TABLE[4] = {mstage_row<M1>(), mstage_row<M2>(), mstage_row<M3>(),
+                                                       mstage_row<M4>()};
 }  // namespace
 
 // Times each valid tile width of every mixed stage over the table's jobs

@@ -45,7 +45,7 @@ void fields_create(Fields& f, int n, int trace_cap) {
   f.F = long(n) * n;
   f.V = long(n + 1) * (n + 1);
   const size_t F = f.F, V = f.V;
-  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 2 * V); dalloc(f.r, F); dalloc(f.jd, F);
+  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 4 * V); dalloc(f.r, F); dalloc(f.jd, F);
   dalloc(f.dhat, F); dalloc(f.d[0], F); dalloc(f.d[1], F); dalloc(f.dpair, 2);
   f.pslots = partial_slots(n);
   dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, size_t(SITE_COUNT) * SITE_VALUES * size_t(f.pslots));
@@ -130,8 +130,8 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
   // the captured iteration depends on the precision mode and the tile shapes
   const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (long(js.cfg[0]) << 8) |
                    (long(js.cfg[1]) << 12) | (long(js.cfg[2]) << 16) | (long(js.cfg[3]) << 20) |
-                   (long(js.mcfg[0]) << 24) | (long(js.mcfg[1]) << 26) | (long(js.mcfg[2]) << 28) |
-                   (long(js.mcfg[3]) << 30);
+                   (long(js.mcfg[0]) << 24) | (long(js.mcfg[1]) << 28) | (long(js.mcfg[2]) << 32) |
+                   (long(js.mcfg[3]) << 36);
   if (d.exec && d.key != key) driver_invalidate(d);
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));

@@ -88,7 +88,7 @@ struct Fields {
   long F = 0, V = 0;
   double* rho = nullptr;     // adjusted density (F)
   double* psi = nullptr;     // potential (F)
-  double* g0 = nullptr;      // cached vertex gradients of psi (2V, SoA)
+  double* g0 = nullptr;      // cached vertex gradients of psi: two buffers of 2V (SoA), current = it & 1
   double* r = nullptr;       // residual r_k (F), updated in place
   double* jd = nullptr;      // r_k - r_{k-1} (F)
   double* dhat = nullptr;    // Laplacian-solve direction (F)
@@ -152,9 +152,12 @@ inline int fold_tiles(int n, int st, int c) {
 // workgroups of the grid-stride ("flat") kernels per job
 inline int flat_blocks(long F) { return int(std::min<long>((F + NT * 4 - 1) / (NT * 4), 1024)); }
 
-// mixed-transform tile widths (output columns per workgroup), one per config
+// mixed-transform launch configurations: tile width (output columns per
+// workgroup) and tile rows.  Configs 0, 1: the two-group form; config 2: the
+// wide form (128 x 128 tiles, four 64 x 64 waves, operands loaded straight
+// into LDS).
 constexpr int NMIXED = 3;
-constexpr int MIXED_TN[NMIXED] = {32, 64, 128};  // config 2: 128 x 128 tiles, four 64 x 64 waves
+constexpr int MIXED_TN[NMIXED] = {32, 64, 128};
 constexpr int MIXED_TM[NMIXED] = {64, 64, 128};
 
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);

@@ -116,6 +116,25 @@ __device__ inline double quad_area(const double* gx, const double* gy, int a, in
                 (gy[v11] - gy[v00] + e) * (gx[v01] - gx[v10] - e));
 }
 
+// Quartic coefficients of tile-local face (a, b): with cached gradients
+// G0 = (hx, hy) and direction gradients (gx, gy), r(psi + t d) = r + b t + a t^2
+// (otsolver_2dgrid.cpp:496-537).  Shared by the line-search pass and the
+// update pass so that both evaluate exactly the same expressions.
+__device__ inline void quartic_ab(const double* gx, const double* gy, const double* hx, const double* hy, int a,
+                                  int b, double e, double& av, double& bv) {
+  const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
+  const double d0ax = hx[v11] - hx[v00] + e;
+  const double d0ay = hy[v11] - hy[v00] + e;
+  const double d0bx = hx[v01] - hx[v10] - e;
+  const double d0by = hy[v01] - hy[v10] + e;
+  const double ddax = gx[v11] - gx[v00];
+  const double dday = gy[v11] - gy[v00];
+  const double ddbx = gx[v01] - gx[v10];
+  const double ddby = gy[v01] - gy[v10];
+  av = 0.5 * (ddax * ddby - dday * ddbx);
+  bv = 0.5 * (ddax * d0by - dday * d0bx + d0ax * ddby - d0ay * ddbx);
+}
+
 // ---------------------------------------------------------------------------
 // Exact quartic line search on the device.
 // e(t) = z0 + z1 t + z2 t^2 + z3 t^3 + z4 t^4; the stationary points are the
@@ -220,7 +239,7 @@ __global__ __launch_bounds__(NT) void k_fill(double* p, long count, double v) {
   for (long i = long(blockIdx.x) * NT + threadIdx.x; i < count; i += long(gridDim.x) * NT) p[i] = v;
 }
 
-// psi = 0, G0 = 0, both search directions = 0 (otsolver_2dgrid.cpp:126-140)
+// psi = 0, both G0 buffers = 0, both search directions = 0 (otsolver_2dgrid.cpp:126-140)
 __global__ __launch_bounds__(NT) void k_init_fields(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
@@ -231,7 +250,7 @@ __global__ __launch_bounds__(NT) void k_init_fields(JOB_ARGS) {
     d0[i] = 0.;
     d1[i] = 0.;
   }
-  for (long i = long(L.t) * NT + threadIdx.x; i < 2 * f.V; i += stride) g0[i] = 0.;
+  for (long i = long(L.t) * NT + threadIdx.x; i < 4 * f.V; i += stride) g0[i] = 0.;
 }
 
 // density adjustment (otsolver_2dgrid.cpp:47-73), pass 1: sum / min / max
@@ -464,7 +483,9 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restr
 
 // direction d = d_hat (+ beta d_prev), vertex gradients of d, quartic
 // coefficients a, b per quad (r(psi + t d) = r + b t + a t^2), and the
-// partials of b.r, a.r, b.b, a.b, a.a, sum a, sum b   (otsolver_2dgrid.cpp:467-561)
+// partials of b.r, a.r, b.b, a.b, a.a, sum a, sum b   (otsolver_2dgrid.cpp:467-561).
+// a and b are not stored: the update pass evaluates them again from the same
+// inputs (d and the not yet updated G0), which costs less than their HBM round trip.
 __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
@@ -477,8 +498,8 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const auto dhat = gp(f.dhat);
   const auto dprev = gp(f.d[k & 1]);
   const auto dnew = gp(f.d[(k + 1) & 1]);
-  const auto g0 = gp(f.g0);
-  const auto aout = gp(f.a), bout = gp(f.b), r = gp(f.r);
+  const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
+  const auto r = gp(f.r);
   const bool use_prev = st->use_prev != 0;
   const double beta = st->beta;
   int i0, j0;
@@ -531,19 +552,8 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
       dnew[fc] = s[(a + 1) * LDF + b + 1];
-      const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
-      const double d0ax = hx[v11] - hx[v00] + e;
-      const double d0ay = hy[v11] - hy[v00] + e;
-      const double d0bx = hx[v01] - hx[v10] - e;
-      const double d0by = hy[v01] - hy[v10] + e;
-      const double ddax = gx[v11] - gx[v00];
-      const double dday = gy[v11] - gy[v00];
-      const double ddbx = gx[v01] - gx[v10];
-      const double ddby = gy[v01] - gy[v10];
-      const double av = 0.5 * (ddax * ddby - dday * ddbx);
-      const double bv = 0.5 * (ddax * d0by - dday * d0bx + d0ax * ddby - d0ay * ddbx);
-      aout[fc] = av;
-      bout[fc] = bv;
+      double av, bv;
+      quartic_ab(gx, gy, hx, hy, a, b, e, av, bv);
       const double rv = rvv[u];
       acc[0] += bv * rv;
       acc[1] += av * rv;
@@ -602,24 +612,29 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
 }
 
 // line-search update: psi += alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
-// (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta.
+// (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta.  The
+// quartic coefficients a, b come from d and G0 before its update, as in the
+// line-search pass.
 __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
   const auto st = gp((const SolveState*)f.state);
   if (!st->update) return;
   const int n = f.n;
-  __shared__ double gx[NV], gy[NV];
+  __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
   __shared__ double s[NF];
   const int k = st->it - 1;
   const auto d = gp(f.d[(k + 1) & 1]);
-  const auto g0 = gp(f.g0);
-  const auto psi = gp(f.psi), a = gp(f.a), b = gp(f.b), r = gp(f.r), jd = gp(f.jd);
+  // G0 ping-pong: neighbouring tiles read the border vertices this tile
+  // updates, so the update goes to the other buffer
+  const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
+  const auto g0n = gp(f.g0 + ((k + 1) & 1) * 2 * f.V);
+  const auto psi = gp(f.psi), r = gp(f.r), jd = gp(f.jd);
   const double alpha = st->alpha;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
-  // all global loads of the tile first: direction halo, G0 of the owned
-  // vertices, psi / r / a / b of the faces (the stores below may alias them)
+  // all global loads of the tile first: direction halo, G0 of the tile's
+  // vertices, psi / r of the faces (the stores below overwrite them)
   double hv[HR];
   halo_load(hv, n, i0, j0, [&](long fc) { return d[fc]; });
   const long V1 = n + 1;
@@ -627,20 +642,21 @@ __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
   // vertices owned by this tile: [i0, i0+TI) x [j0, j0+TJ), plus the last row/column of the grid
   bool own[VR];
   long vid[VR];
-  double gxo[VR], gyo[VR];
+  double hxv[VR], hyv[VR];
 #pragma unroll
   for (int u = 0; u < VR; ++u) {
     const int t = threadIdx.x + u * NT;
     const int va = t / LDV, vb = t - va * LDV;
     const int vi = i0 + va, vj = j0 + vb;
-    const bool own_i = (va < TI && vi <= n) || (vi == n);
-    const bool own_j = (vb < TJ && vj <= n) || (vj == n);
-    own[u] = t < NV && own_i && own_j && vi <= n && vj <= n;
+    const bool in = t < NV && vi <= n && vj <= n;
+    const bool own_i = (va < TI) || (vi == n);
+    const bool own_j = (vb < TJ) || (vj == n);
+    own[u] = in && own_i && own_j;
     vid[u] = vi * V1 + vj;
-    gxo[u] = own[u] ? g0[vid[u]] : 0.;
-    gyo[u] = own[u] ? g0[V + vid[u]] : 0.;
+    hxv[u] = in ? g0[vid[u]] : 0.;
+    hyv[u] = in ? g0[V + vid[u]] : 0.;
   }
-  double psv[FR], rkv[FR], av[FR], bv[FR];
+  double psv[FR], rkv[FR];
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
     const int t = threadIdx.x + u * NT;
@@ -649,10 +665,16 @@ __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
     const long fc = long(fi) * n + fj;
     psv[u] = in ? psi[fc] : 0.;
     rkv[u] = in ? r[fc] : 0.;
-    av[u] = in ? a[fc] : 0.;
-    bv[u] = in ? b[fc] : 0.;
   }
   halo_store(s, hv);
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (t < NV) {
+      hx[t] = hxv[u];
+      hy[t] = hyv[u];
+    }
+  }
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
@@ -660,10 +682,11 @@ __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
   for (int u = 0; u < VR; ++u) {
     const int t = threadIdx.x + u * NT;
     if (own[u]) {
-      g0[vid[u]] = gxo[u] + alpha * gx[t];
-      g0[V + vid[u]] = gyo[u] + alpha * gy[t];
+      g0n[vid[u]] = hxv[u] + alpha * gx[t];
+      g0n[V + vid[u]] = hyv[u] + alpha * gy[t];
     }
   }
+  const double e = 1. / double(n);
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
     const int t = threadIdx.x + u * NT;
@@ -672,8 +695,10 @@ __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
       psi[fc] = psv[u] + alpha * s[(fa + 1) * LDF + fb + 1];
+      double av, bv;
+      quartic_ab(gx, gy, hx, hy, fa, fb, e, av, bv);
       const double rk = rkv[u];
-      const double rn = av[u] * (alpha * alpha) + bv[u] * alpha + rk;
+      const double rn = av * (alpha * alpha) + bv * alpha + rk;
       r[fc] = rn;
       jd[fc] = rn - rk;
     }

@@ -71,10 +71,10 @@ def test_mixed_rejects_unsupported_size(solver_factory):
 
 @pytest.mark.parametrize("n", [128, 256, 512])
 def test_mixed_tile_widths_bit_identical(n, solver_factory):
-    """Every tile configuration of the mixed stages (32- and 64-column tiles
-    of the two-group form, 128 x 128 tiles of the wide form where n/2 % 128
-    == 0) accumulates in the same order and gives the same bits (the
-    autotuner may pick any of them)."""
+    """Every launch configuration of the mixed stages (32- and 64-column
+    tiles of the two-group form, 128 x 128 tiles of the direct-to-LDS wide
+    form where n/2 % 128 == 0) accumulates in the same order and gives the
+    same bits (the autotuner may pick any of them)."""
     from otmap_amd._lib import OtmError
     s = solver_factory(n)
     r = np.random.default_rng(5 + n).standard_normal(n * n)
