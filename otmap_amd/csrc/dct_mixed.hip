@@ -72,12 +72,11 @@ __device__ inline void split3d(double x, u16& h0, u16& h1, u16& h2) {
 }
 
 // ---------------------------------------------------------------------------
-// right-hand side: fold the columns of r' (mode 0) or of the refinement
-// residual s = r' - M0 x0 (mode 1) and split them into U / W planes
+// right-hand side: fold the columns of r' = r - (sum r) e0 and split them
+// into the U / W planes
 // ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, const int* __restrict__ starts,
-                                             int count, bool gate) {
+__global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, const int* __restrict__ starts, int count,
+                                             bool gate) {
   const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
   const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
   const int nb = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
@@ -87,19 +86,12 @@ __global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, cons
   const int n = J.f.n, h = n / 2;
   const long nh = long(n) * h;
   const auto r = gp((const double*)J.f.r);
-  const auto x = gp((const double*)J.f.x0);
   const auto out = gp(J.f.mrhs);
   const double S = st->sum_r;
   auto src = [&](int i, int j) {
     const long f = long(i) * n + j;
     double v = r[f];
     if (f == 0) v -= S;
-    if (MODE == 1) {
-      const int i1 = i == 0 ? i : i - 1, j1 = j == 0 ? j : j - 1;
-      const int i2 = i == n - 1 ? i : i + 1, j2 = j == n - 1 ? j : j + 1;
-      const double nbs = (x[long(i1) * n + j1] + x[long(i1) * n + j2]) + (x[long(i2) * n + j1] + x[long(i2) * n + j2]);
-      v = v - (2.0 * x[f] - 0.5 * nbs);
-    }
     return v;
   };
   // four consecutive j per thread (h % 4 == 0): 8-byte stores per plane
@@ -121,6 +113,96 @@ __global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, cons
       *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + q) = pu;
       *reinterpret_cast<OTM_GLOBAL u32x2*>(out + (3 + pl) * nh + q) = pw;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// refinement right-hand side: s = r' - M0 x0 with the exact operator (2 on
+// the diagonal, -1/2 at the four clamped diagonal neighbours), folded over
+// columns (U = s(:, j) + s(:, n-1-j), W = difference) and split into the six
+// BF16 planes.  One workgroup per RT rows x RT folded columns: x0 of the rows
+// i0-1 .. i0+RT and of the columns j0-1 .. j0+RT and their mirrors is staged
+// in LDS with clamped indices; each thread folds four consecutive columns of
+// one row.  s(i, j) = (r(i, j) - [i = j = 0] sum r) - (2 x0(i, j) - 0.5
+// ((x0(i-1, j-1) + x0(i-1, j+1)) + (x0(i+1, j-1) + x0(i+1, j+1)))).
+// ---------------------------------------------------------------------------
+constexpr int RT = 32;
+static_assert(RT * RT / 4 == NT, "four folded columns per thread");
+
+__global__ __launch_bounds__(NT) void k_mresid(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                              int count, bool gate) {
+  const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
+  const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
+  const Job& J = jobs[p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int n = J.f.n, h = n / 2;
+  const long nh = long(n) * h;
+  const int tj = h / RT;
+  const int i0 = (lt / tj) * RT, j0 = (lt % tj) * RT;
+  const auto r = gp((const double*)J.f.r);
+  const auto x = gp((const double*)J.f.x0);
+  const auto out = gp(J.f.mrhs);
+  constexpr int HW = RT + 2;
+  // xl[a][b] = x0(i0-1+a, j0-1+b), xr[a][b] = x0(i0-1+a, n-1-(j0+RT)+b), indices clamped to the grid
+  __shared__ double xl[HW][HW + 1], xr[HW][HW + 1];
+  const int t = threadIdx.x;
+  const int a = t / (RT / 4), c0 = (t % (RT / 4)) * 4;
+  const int i = i0 + a;
+  // this thread's residuals, loaded before the halo so that both latencies overlap
+  double rl[4], rr[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    rl[e] = r[long(i) * n + j0 + c0 + e];
+    rr[e] = r[long(i) * n + (n - 1 - j0 - c0 - e)];
+  }
+  constexpr int HR2 = (HW * HW + NT - 1) / NT;
+  double vl[HR2], vr[HR2];
+#pragma unroll
+  for (int u = 0; u < HR2; ++u) {
+    const int e = t + u * NT;
+    const int ha = e / HW, hb = e % HW;
+    const int gi = min(max(i0 - 1 + ha, 0), n - 1);
+    const int gl = min(max(j0 - 1 + hb, 0), n - 1);
+    const int gr = min(max(n - 1 - (j0 + RT) + hb, 0), n - 1);
+    vl[u] = e < HW * HW ? x[long(gi) * n + gl] : 0.;
+    vr[u] = e < HW * HW ? x[long(gi) * n + gr] : 0.;
+  }
+  const double S = st->sum_r;
+#pragma unroll
+  for (int u = 0; u < HR2; ++u) {
+    const int e = t + u * NT;
+    if (e < HW * HW) {
+      xl[e / HW][e % HW] = vl[u];
+      xr[e / HW][e % HW] = vr[u];
+    }
+  }
+  __syncthreads();
+  u16 uu[3][4], ww[3][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int c = c0 + e, j = j0 + c;
+    // left value s(i, j): halo (a + 1, c + 1)
+    double va = rl[e];
+    if (i == 0 && j == 0) va -= S;
+    const double nl = (xl[a][c] + xl[a][c + 2]) + (xl[a + 2][c] + xl[a + 2][c + 2]);
+    va = va - (2.0 * xl[a + 1][c + 1] - 0.5 * nl);
+    // mirrored value s(i, n-1-j): halo column RT - c
+    const int m = RT - c;
+    double vb = rr[e];
+    const double nr = (xr[a][m - 1] + xr[a][m + 1]) + (xr[a + 2][m - 1] + xr[a + 2][m + 1]);
+    vb = vb - (2.0 * xr[a + 1][m] - 0.5 * nr);
+    split3d(va + vb, uu[0][e], uu[1][e], uu[2][e]);
+    split3d(va - vb, ww[0][e], ww[1][e], ww[2][e]);
+  }
+  const long q = long(i) * h + j0 + c0;
+#pragma unroll
+  for (int pl = 0; pl < 3; ++pl) {
+    u32x2 pu, pw;
+    pu.x = uu[pl][0] | (unsigned(uu[pl][1]) << 16); pu.y = uu[pl][2] | (unsigned(uu[pl][3]) << 16);
+    pw.x = ww[pl][0] | (unsigned(ww[pl][1]) << 16); pw.y = ww[pl][2] | (unsigned(ww[pl][3]) << 16);
+    *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + q) = pu;
+    *reinterpret_cast<OTM_GLOBAL u32x2*>(out + (3 + pl) * nh + q) = pw;
   }
 }
 
@@ -759,9 +841,14 @@ void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
   }
 }
 
-template <int MODE>
+void launch_mresid(const JobSet& js, bool gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_mresid, dim3(unsigned(js.total[KIND_MRES])), dim3(NT), 0, s, (const Job*)js.jobs,
+                     (const int*)js.starts_of(KIND_MRES), js.count, gate);
+  OTM_HIP(hipGetLastError());
+}
+
 void launch_mprep(const JobSet& js, bool gate, hipStream_t s) {
-  hipLaunchKernelGGL((k_mprep<MODE>), dim3(unsigned(js.total[KIND_FLAT])), dim3(NT), 0, s, (const Job*)js.jobs,
+  hipLaunchKernelGGL(k_mprep, dim3(unsigned(js.total[KIND_FLAT])), dim3(NT), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(KIND_FLAT), js.count, gate);
   OTM_HIP(hipGetLastError());
 }
@@ -897,14 +984,14 @@ void mixed_tune(JobSet& js, int device, hipStream_t s) {
 // refinement steps x += transform(r' - M0 x); the last stage leaves the
 // |d_hat|^2 partials in SITE_DNORM (launch kind mixed_kind(3, js.mcfg[3])).
 void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s) {
-  launch_mprep<0>(js, gate, s);
+  launch_mprep(js, gate, s);
   if (refine <= 0) {
     mixed_transform(js, gate, FIN_OUT, s);
     return;
   }
   mixed_transform(js, gate, FIN_X0, s);
   for (int k = 0; k < refine; ++k) {
-    launch_mprep<1>(js, gate, s);
+    launch_mresid(js, gate, s);
     mixed_transform(js, gate, k == refine - 1 ? FIN_ADD_OUT : FIN_ADD_X0, s);
   }
 }

@@ -160,13 +160,15 @@ constexpr int NMIXED = 3;
 constexpr int MIXED_TN[NMIXED] = {32, 64, 128};
 constexpr int MIXED_TM[NMIXED] = {64, 64, 128};
 
+inline int mres_tiles(int n) { return (n / 32) * (n / 64); }
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
 // mixed_kind(s, c): stage s of the mixed transform with tile width MIXED_TN[c]
 enum LaunchKind {
   KIND_STENCIL = 0,
   KIND_FLAT = 1,
   KIND_PLAIN64 = 2,
-  KIND_M0 = 3,
+  KIND_MRES = 3,  // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
+  KIND_M0 = 4,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };

@@ -942,6 +942,7 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs) {
       if (kind == KIND_STENCIL) w = num_tiles(n);
       else if (kind == KIND_FLAT) w = flat_blocks(jobs[p].f.F);
       else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
+      else if (kind == KIND_MRES) w = mixed_supported(n) ? mres_tiles(n) : 0;
       else if (kind < KIND_GEMM0) {
         const int g = kind - KIND_M0, stage = g / NMIXED, c = g % NMIXED;
         w = mixed_cfg_valid(stage, n, c) ? mixed_tiles(stage, n, c) : 0;
