@@ -146,11 +146,11 @@ def inverse_throughput(n, d_fwd, d_adj, count, reps):
 def batched_throughput(count, steps, warmup):
     """64 seeded Gaussian-mixture densities with grid sizes drawn from
     {128, 192, 256, 384, 512, 768, 1024}, solved together by one BatchSolver
-    (device-resident inputs), against the same densities solved one after the
-    other by single-grid solvers."""
+    (device-resident inputs) in FP64 and in the mixed form, against the same
+    densities solved one after the other by single-grid FP64 solvers."""
     import torch
 
-    from otmap_amd import BatchSolver, GridBasedTransportSolver
+    from otmap_amd import BatchSolver, GridBasedTransportSolver, Precision, SolverOptions
     from otmap_amd import densities as D
 
     specs = D.batch_specs(count)
@@ -165,15 +165,23 @@ def batched_throughput(count, steps, warmup):
     t_create = time.perf_counter() - t0
     ip = [t.data_ptr() for t in d_in]
     op = [t.data_ptr() for t in d_out]
-    for _ in range(warmup):
-        b.solve_device(ip, op)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st = b.solve_device(ip, op)
-    torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / steps
-    iters = [s.iterations for s in st]
+
+    def timed(opt):
+        for _ in range(warmup):
+            b.solve_device(ip, op, opt=opt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = b.solve_device(ip, op, opt=opt)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, [s.iterations for s in st]
+
+    t_mixed, iters_mixed = timed(SolverOptions(precision=Precision.MIXED))
+    fwd_mixed = torch.cat(d_out)
+    t_mixed2, iters_mixed2 = timed(SolverOptions(precision=Precision.MIXED, refine_steps=2))
+    fwd_mixed2 = torch.cat(d_out)
+    t_batch, iters = timed(SolverOptions())
+    fwd_fp64 = torch.cat(d_out)
 
     # the same densities through single-grid solvers, one handle per size
     solvers = {}
@@ -195,7 +203,14 @@ def batched_throughput(count, steps, warmup):
             "sequential_single_solves_per_s": round(count / t_seq, 2),
             "speedup_vs_sequential": round(t_seq / t_batch, 3),
             "iterations": {"min": min(iters), "max": max(iters), "mean": round(float(np.mean(iters)), 1)},
-            "create_s": round(t_create, 3), "precision": "fp64", "gemm_tiles": b.gemm_config()[0]}
+            "create_s": round(t_create, 3), "precision": "fp64", "gemm_tiles": b.gemm_config()[0],
+            "mixed": {"solves_per_s": round(count / t_mixed, 2), "ms_per_batch": round(t_mixed * 1e3, 2),
+                      "iterations_equal_fp64": iters_mixed == iters,
+                      "max_vertex_diff_vs_fp64": float((fwd_mixed - fwd_fp64).abs().max().item()),
+                      "speedup_vs_fp64_batch": round(t_batch / t_mixed, 3), "mixed_tiles": b.mixed_config()[0]},
+            "mixed_refine2": {"solves_per_s": round(count / t_mixed2, 2), "ms_per_batch": round(t_mixed2 * 1e3, 2),
+                              "iterations_equal_fp64": iters_mixed2 == iters,
+                              "max_vertex_diff_vs_fp64": float((fwd_mixed2 - fwd_fp64).abs().max().item())}}
 
 
 def batched_1024(count, steps, warmup, fp64_single_ms):
@@ -438,10 +453,13 @@ def main():
                                 f"{kname}: one even/odd-folded DCT stage (n^3 flops), 3-term BF16 "
                                 "split on MFMA, FP32 accumulate", float(n) ** 3)
 
-    # Whole-iteration roofline: 26 FP64 field touches per face (218 MB at
-    # 1024^2) at HBM peak plus the flops of the four GEMM stages at the MFMA
-    # peak of the mode (8 n^3, or 4 n^3 for the folded FP64 transform), over
-    # the measured device time per iteration.
+    # Whole-iteration roofline: the reference's 26 FP64 field touches per face
+    # (218 MB at 1024^2; this build moves 22: the quartic coefficients are
+    # recomputed instead of stored) at HBM peak, plus the algorithmic flops of
+    # one Laplacian solve (four transform stages: 4 n^3 folded, 8 n^3
+    # unfolded; the BF16 split multiplicity and the refinement transform are
+    # not counted) at the MFMA peak of the mode, over the measured device time
+    # per iteration.
     def iteration_roofline(name, peak_tflops, gemm_flops):
         t_roof = 26 * 8 * n * n / (PEAK_HBM_GBS * 1e9) + gemm_flops / (peak_tflops * 1e12)
         t_iter = results[name]["dev_ms"] * 1e-3 / max(results[name]["iters"], 1)
@@ -450,7 +468,7 @@ def main():
 
     iter_roof = {"fp64": iteration_roofline("fp64", PEAK_FP64_MFMA_TFLOPS, 4 * stage_flops_f64)}
     if "mixed" in results:
-        iter_roof["mixed"] = iteration_roofline("mixed", PEAK_BF16_TFLOPS, 8.0 * n ** 3)
+        iter_roof["mixed"] = iteration_roofline("mixed", PEAK_BF16_TFLOPS, 4.0 * float(n) ** 3)
     line = {
         "metric": METRIC,
         "value": round(results[head]["ms"], 3),

@@ -116,3 +116,27 @@ def test_mixed_tile_width_must_divide_half(n):
         s.set_mixed_config(st, 0)
         with pytest.raises(OtmError):
             s.set_mixed_config(st, 1)
+
+
+def test_mixed_sensitive_trajectory_needs_second_refinement(oracle, solver_factory):
+    """Density 29 of the 64-density batched workload (n = 192, 3 Gaussians)
+    has a trajectory whose step lengths amplify a perturbation about 3x per
+    iteration from iteration ~17 on: a Laplacian-solve error of 1e-12 (one
+    refinement) ends 1.3e-6 away from the oracle's positions, 4e-14 (two
+    refinements) ends below 1e-8.  The FP64 path stays within 1e-8.  Same
+    iteration count in every mode."""
+    specs = D.batch_specs(64)
+    n = specs[29][0]
+    rho = D.batch_densities(64)[29]
+    o = oracle.solve(n, rho)
+    s = solver_factory(n)
+    errs = {}
+    for name, opt in (("fp64", SolverOptions()), ("mixed1", SolverOptions(precision=Precision.MIXED)),
+                      ("mixed2", SolverOptions(precision=Precision.MIXED, refine_steps=2))):
+        tm = s.solve(rho, opt)
+        assert s.last_stats.iterations == o["iters"], (name, s.last_stats.iterations, o["iters"])
+        errs[name] = np.abs(tm.fwd_mesh().xy - o["fwd"]).max()
+    print(n, errs)
+    assert errs["fp64"] < 1e-7
+    assert errs["mixed2"] < 1e-7
+    assert errs["mixed1"] < 1e-5
