@@ -7,7 +7,10 @@
 //
 // Operands live in HBM already split: a value x is stored as three BF16 planes
 // x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)),
-// so the GEMM inner loop only moves bytes and issues MFMAs.  A product keeps
+// so the GEMM inner loop only moves bytes and issues MFMAs.  Every plane of R
+// rows by K columns (k contiguous in the products) is stored in 16 x 32
+// blocks of 1 KB in MFMA fragment order (bofs below): a wave moves one block
+// as one contiguous run, from HBM and through LDS.  A product keeps
 // the six terms a_i b_j with i + j <= 2.  The constant DCT factors are split
 // once per grid size; every stage splits its own output once in its epilogue.
 //
@@ -46,7 +49,6 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int MTM = 64;        // A rows per workgroup tile
 constexpr int MKB = 32;        // K step
-constexpr int MLD = MKB + 8;   // LDS row length (u16): 80 B rows keep 16-B fragment reads aligned
 
 enum MStage { M1 = 0, M2 = 1, M3 = 2, M4 = 3 };
 // M4 epilogue: x0 = D | d_hat = D (+partials) | d_hat = x0 + D (+partials) | x0 += D
@@ -54,6 +56,15 @@ enum MFinal { FIN_X0 = 0, FIN_OUT = 1, FIN_ADD_OUT = 2, FIN_ADD_X0 = 3 };
 
 __device__ inline u16 bf16_bits(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
 __device__ inline float bf16_float(u16 b) { return (float)__builtin_bit_cast(__bf16, b); }
+
+// Element offset of (r, k) in a BF16 plane of row length K (K % 32 == 0)
+// stored in blocks of 16 rows x 32 k (row blocks outer, k blocks inner); in a
+// block, lane position l = 16 (k / 8 % 4) + r % 16 holds the 8 values
+// k / 8 * 8 .. + 7 of row r, which is what lane l of v_mfma_f32_16x16x32_bf16
+// takes as its A or B fragment.
+__host__ __device__ inline long bofs(long r, long k, long K) {
+  return ((r >> 4) * (K >> 5) + (k >> 5)) * 512 + ((((k >> 3) & 3) << 4) + (r & 15)) * 8 + (k & 7);
+}
 
 // exact 3-term split of an FP32 value
 __device__ inline void split3(float x, u16& h0, u16& h1, u16& h2) {
@@ -110,8 +121,8 @@ __global__ __launch_bounds__(NT) void k_mprep(const Job* __restrict__ jobs, cons
       u32x2 pu, pw;
       pu.x = u[pl][0] | (unsigned(u[pl][1]) << 16); pu.y = u[pl][2] | (unsigned(u[pl][3]) << 16);
       pw.x = w[pl][0] | (unsigned(w[pl][1]) << 16); pw.y = w[pl][2] | (unsigned(w[pl][3]) << 16);
-      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + q) = pu;
-      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + (3 + pl) * nh + q) = pw;
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + bofs(i, j, h)) = pu;
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + (3 + pl) * nh + bofs(i, j, h)) = pw;
     }
   }
 }
@@ -195,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_mresid(const Job* __restrict__ jobs, con
     split3d(va + vb, uu[0][e], uu[1][e], uu[2][e]);
     split3d(va - vb, ww[0][e], ww[1][e], ww[2][e]);
   }
-  const long q = long(i) * h + j0 + c0;
+  const long q = bofs(i, j0 + c0, h);
 #pragma unroll
   for (int pl = 0; pl < 3; ++pl) {
     u32x2 pu, pw;
@@ -269,22 +280,27 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  // LDS per group: A planes [2][3][64][MLD], B planes [2][3][TN][MLD]; the
-  // epilogue reuses the space for the two groups' FP32 tiles [64][TN + 1]
-  constexpr int A_U16 = 2 * 3 * MTM * MLD, B_U16 = 2 * 3 * TN * MLD, G_U16 = A_U16 + B_U16;
+  // LDS per group, in the blocks' fragment order: A [2 bufs][3 planes][4 row
+  // blocks][64 lanes] x 16 B, B [2][3][TN / 16][64] x 16 B; the epilogue
+  // reuses the space for the two groups' FP32 tiles [64][TN + 1]
+  constexpr int ARB = MTM / 16, BRB = TN / 16;  // 16-row blocks of A and B
+  constexpr int A_U16 = 2 * 3 * ARB * 512, B_U16 = 2 * 3 * BRB * 512, G_U16 = A_U16 + B_U16;
   __shared__ __attribute__((aligned(16))) u16 smem[2 * G_U16];
   static_assert(2 * G_U16 * 2 >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
   const int t = threadIdx.x, g = t / NT, tg = t % NT, lane = t & 63, w = tg >> 6;
   u16* const gs = smem + g * G_U16;
-  auto As = [&](int buf, int pl, int row) { return gs + ((buf * 3 + pl) * MTM + row) * MLD; };
-  auto Bs = [&](int buf, int pl, int row) { return gs + A_U16 + ((buf * 3 + pl) * TN + row) * MLD; };
+  // lane position `l` of row block rb of plane pl
+  auto As = [&](int buf, int pl, int rb, int l) { return gs + ((buf * 3 + pl) * ARB + rb) * 512 + l * 8; };
+  auto Bs = [&](int buf, int pl, int rb, int l) { return gs + A_U16 + ((buf * 3 + pl) * BRB + rb) * 512 + l * 8; };
   const int wm = (w >> 1) * 32, wn = (w & 1) * (TN / 2);
-  const int lr = lane & 15, lq = lane >> 4;
+  const int lr = lane & 15, lq = lane >> 4;  // MFMA output layout: lane (lr, lq) holds rows 4 lq .. + 3 of column lr
 
-  // global -> register staging, two slots of K steps in flight.  A: 3 x 64
-  // rows x 4 chunks of 8 (3 per thread of the group); B: 3 x TN x 4 (chunk
-  // indices past the tile reload an earlier chunk, never stored, so every
-  // lane issues the same loads and the waits on the slots stay exact)
+  // global -> register staging, two slots of K steps in flight.  Chunk c =
+  // tg + u * NT of a K step is lane c % 64 of block c / 64 (A: plane u, row
+  // block w; B: plane (c / 64) / BRB, row block (c / 64) % BRB), so each wave
+  // loads and stores whole 1 KB blocks.  Chunk indices past B's blocks reload
+  // an earlier chunk, never stored, so every lane issues the same loads and
+  // the waits on the slots stay exact.
   u32x4 ra[2][3], rb[2][BCH];
   auto arow = [&](int row) {  // global row of tile row `row`
     if (ST == M1) return row < MTM / 2 ? m0 + row : n - m0 - MTM + row;  // rows i and mirrored n-1-i
@@ -292,29 +308,24 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   };
   auto gload = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int k0) {
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
-      const int row = tg >> 2, kc = (tg & 3) * 8;
-      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + u * aps + long(arow(row)) * lda + k0 + kc);
-    }
+    for (int u = 0; u < 3; ++u)
+      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + u * aps + bofs(arow(16 * w), k0, lda) + lane * 8);
 #pragma unroll
     for (int u = 0; u < BCH; ++u) {
       const int c0 = tg + u * NT, c = c0 < BCHUNKS ? c0 : c0 - TN * 4;
-      const int pl = c / (TN * 4), rem = c % (TN * 4), row = rem >> 2, kc = (rem & 3) * 8;
-      xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + long(n0 + row) * ldb + k0 + kc);
+      const int b = c >> 6, pl = b / BRB, bb = b % BRB;
+      xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + bofs(n0 + 16 * bb, k0, ldb) + lane * 8);
     }
   };
   auto sstore = [&](const u32x4 (&xa)[3], const u32x4 (&xb)[BCH], int buf) {
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
-      const int row = tg >> 2, kc = (tg & 3) * 8;
-      *reinterpret_cast<u32x4*>(As(buf, u, row) + kc) = xa[u];
-    }
+    for (int u = 0; u < 3; ++u) *reinterpret_cast<u32x4*>(As(buf, u, w, lane)) = xa[u];
 #pragma unroll
     for (int u = 0; u < BCH; ++u) {
       const int c = tg + u * NT;
       if (BCHUNKS % NT == 0 || c < BCHUNKS) {
-        const int pl = c / (TN * 4), rem = c % (TN * 4), row = rem >> 2, kc = (rem & 3) * 8;
-        *reinterpret_cast<u32x4*>(Bs(buf, pl, row) + kc) = xb[u];
+        const int b = c >> 6;
+        *reinterpret_cast<u32x4*>(Bs(buf, b / BRB, b % BRB, lane)) = xb[u];
       }
     }
   };
@@ -324,9 +335,9 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(As(buf, pl, wm + 16 * i + lr) + lq * 8);
+      for (int i = 0; i < 2; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(As(buf, pl, wm / 16 + i, lane));
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) fb[j][pl] = *reinterpret_cast<const bf8*>(Bs(buf, pl, wn + 16 * j + lr) + lq * 8);
+      for (int j = 0; j < TJ; ++j) fb[j][pl] = *reinterpret_cast<const bf8*>(Bs(buf, pl, wn / 16 + j, lane));
     }
     // small products first, the leading one last
     constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
@@ -421,7 +432,7 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
     return;
   }
 
-  // the other stages write BF16 planes in 8-16 byte runs
+  // the other stages write BF16 planes (blocked layout) in 8-16 byte runs
   if (ST == M1) {
     // U2 / W2 [l = n0 + c][i = m0 + a .. a+3]: pair tile row a with its mirror 63 - a
     const auto out = gp(J.f.mwa);
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
         split3(top + mir, u[0][q], u[1][q], u[2][q]);
         split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
       }
-      const long base = long(n0 + c) * h + m0 + a0;
+      const long base = bofs(n0 + c, m0 + a0, h);
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) {
         u32x2 pu, pw;
@@ -463,7 +474,7 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
         const float val = (kc == 0 && lrow == 0) ? 0.f : y / lam;
         split3(val, z[0][q], z[1][q], z[2][q]);
       }
-      const long base = long(lrow) * n + n0 + c0;
+      const long base = bofs(lrow, n0 + c0, n);
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) {
         u32x4 pz;
@@ -484,7 +495,7 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
         split3(lo + hi, s0[0][q], s0[1][q], s0[2][q]);
         split3(lo - hi, s1[0][q], s1[1][q], s1[2][q]);
       }
-      const long b0 = long(n0 + c) * n + m0 + r0, b1 = long(n - 1 - n0 - c) * n + m0 + r0;
+      const long b0 = bofs(n0 + c, m0 + r0, n), b1 = bofs(n - 1 - n0 - c, m0 + r0, n);
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) {
         u32x4 p0, p1;
@@ -504,7 +515,8 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
 // tiles (M1: 64 rows and their 64 mirrors), four waves of 64 x 64, so a wave
 // issues 96 MFMAs per 24 LDS fragment reads.  Operand tiles go from global
 // memory straight into LDS (global_load_lds_dwordx4: no register staging, no
-// LDS store instructions), three K-step buffers deep.  The K steps of the two
+// LDS store instructions), three K-step buffers deep; every transfer and every
+// fragment read is one contiguous 1 KB block.  The K steps of the two
 // halves (k < K/2 into `lo`, k >= K/2 into `acc`) are processed alternately,
 // each half in increasing k, so every output accumulates in the same order as
 // the two-group form and the epilogue adds (or subtracts) the two FP32 halves
@@ -578,17 +590,16 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  // LDS: three K-step buffers, each [operand A, B][plane 3][row 128][4 chunks
-  // of 8 k values]; chunk c of row r sits at position c ^ swz(r), which puts
-  // the 16 rows of a fragment read on 16 distinct 16-byte bank groups
+  // LDS: three K-step buffers, each [operand A, B][plane 3][16-row block 8]
+  // [lane 64] x 16 bytes: the blocks of the HBM layout, so a transfer moves
+  // one contiguous 1 KB block and a fragment read is one contiguous 1 KB run
   constexpr int SLOTS = 2 * 3 * WT * 4;  // 16-byte chunks per K step
   constexpr int BUF = SLOTS * 16;        // bytes per buffer
   constexpr int DMA = SLOTS / NT;        // transfers per thread per K step (12)
   __shared__ __attribute__((aligned(16))) unsigned char smem[3 * BUF];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
-  const int lr = lane & 15, lq = lane >> 4;
-  auto swz = [](int r) { return ((r >> 1) ^ (r >> 3)) & 3; };
+  const int lr = lane & 15, lq = lane >> 4;  // MFMA output layout: lane (lr, lq) holds rows 4 lq .. + 3 of column lr
   auto arow = [&](int row) {
     if (ST == M1) return row < WT / 2 ? m0 + row : n - m0 - WT + row;
     return m0 + row;
@@ -598,16 +609,15 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   double cf = 0.;
   if (ST == M2) cf = gp(J.ck)[t < WT ? m0 + t : n0 + t - WT];
 
-  // transfer u of this lane (u < 6: A, else B) covers chunk slot
-  // (4u + w) * 64 + lane of a step; its byte offset from the operand base
-  // without the K-step term, and the wave-uniform LDS offset of the transfer
+  // transfer u of this lane (u < 6: A, else B) fills LDS block 4u + w of the
+  // step (plane ((4u + w) % 24) / 8, row block (4u + w) % 8) at position
+  // `lane`: its byte offset from the operand base, K step 0
   unsigned voff[DMA];
 #pragma unroll
   for (int u = 0; u < DMA; ++u) {
-    const int slot = (4 * u + w) * 64 + lane, pl = (slot % (SLOTS / 2)) / (WT * 4);
-    const int r = (slot % (WT * 4)) >> 2, kc = (slot & 3) ^ swz(r);
-    const long e = u < DMA / 2 ? pl * aps + long(arow(r)) * lda + kc * 8 : pl * bps + long(n0 + r) * ldb + kc * 8;
-    voff[u] = unsigned(e * 2);
+    const int blk = (4 * u + w) % 24, pl = blk >> 3, rb = blk & 7;
+    const long e = u < DMA / 2 ? pl * aps + bofs(arow(16 * rb), 0, lda) : pl * bps + bofs(n0 + 16 * rb, 0, ldb);
+    voff[u] = unsigned((e + lane * 8) * 2);
   }
   const unsigned lds0 = unsigned(size_t((__attribute__((address_space(3))) unsigned char*)smem));
   const unsigned wlds = lds0 + unsigned(__builtin_amdgcn_readfirstlane(w)) * 1024;
@@ -615,16 +625,16 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   // K step of position q in the alternating order: 0, S0, 1, S0 + 1, ...
   auto kstep = [&](int q) { return (q & 1) ? S0 + (q >> 1) : (q >> 1); };
   auto issue = [&](int q, int buf) {
-    const int k0 = kstep(q) * MKB;
-    const u16* a = Ab + k0;
-    const u16* b = Bb + k0;
+    const int kb = kstep(q);  // K block: 512 elements further along every row block
+    const u16* a = Ab + kb * 512;
+    const u16* b = Bb + kb * 512;
 #pragma unroll
     for (int u = 0; u < DMA; ++u)
       lds_dma16(u < DMA / 2 ? (const void*)a : (const void*)b, voff[u], wlds + buf * BUF + u * 4096);
   };
-  auto frag = [&](int buf, int op, int pl, int r) {
-    return *reinterpret_cast<const bf8*>(smem + buf * BUF + op * (BUF / 2) +
-                                         (((pl * WT + r) * 4 + (lq ^ swz(r))) << 4));
+  // fragment of 16-row block rb of plane pl of operand op
+  auto frag = [&](int buf, int op, int pl, int rb) {
+    return *reinterpret_cast<const bf8*>(smem + buf * BUF + op * (BUF / 2) + (((pl * 8 + rb) * 64 + lane) << 4));
   };
   f4 acc[TI][TJ], lo[TI][TJ];
 #pragma unroll
@@ -651,9 +661,9 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
 #pragma unroll
-      for (int i = 0; i < TI; ++i) fa[i][RA[x]] = frag(buf, 0, RA[x], wm + 16 * i + lr);
+      for (int i = 0; i < TI; ++i) fa[i][RA[x]] = frag(buf, 0, RA[x], wm / 16 + i);
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) fb[j][RB[x]] = frag(buf, 1, RB[x], wn + 16 * j + lr);
+      for (int j = 0; j < TJ; ++j) fb[j][RB[x]] = frag(buf, 1, RB[x], wn / 16 + j);
     }
     constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
     constexpr int PB[6] = {0, 1, 2, 0, 1, 0};
@@ -753,7 +763,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
         split3(top + mir, u[0][q], u[1][q], u[2][q]);
         split3(top - mir, wv[0][q], wv[1][q], wv[2][q]);
       }
-      const long base = long(n0 + c) * h + m0 + a0;
+      const long base = bofs(n0 + c, m0 + a0, h);
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) {
         u32x2 pu, pw;
@@ -779,7 +789,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
         const float val = (kc == 0 && lrow == 0) ? 0.f : y / lam;
         split3(val, z[0][q], z[1][q], z[2][q]);
       }
-      const long base = long(lrow) * n + n0 + c0;
+      const long base = bofs(lrow, n0 + c0, n);
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) {
         u32x4 pz;
@@ -799,7 +809,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
         u16 sv[3][8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) split3(Ds[(r0 + q) * DL + c], sv[0][q], sv[1][q], sv[2][q]);
-        const long b0 = round == 0 ? long(n0 + c) * n + m0 + r0 : long(n - 1 - n0 - c) * n + m0 + r0;
+        const long b0 = round == 0 ? bofs(n0 + c, m0 + r0, n) : bofs(n - 1 - n0 - c, m0 + r0, n);
 #pragma unroll
         for (int pl = 0; pl < 3; ++pl) {
           u32x4 pv;
@@ -911,8 +921,8 @@ void mixed_plan_create(MixedPlan& mp, const DctPlan& p) {
       const u16 h2 = bf(float(r1 - double(fl(h1))));
       const u16 hs[3] = {h0, h1, h2};
       for (int pl = 0; pl < 3; ++pl) {
-        Pp[pl * nh + size_t(r) * h + i] = hs[pl];
-        Ptp[pl * nh + size_t(i) * n + r] = hs[pl];
+        Pp[pl * nh + bofs(r, i, h)] = hs[pl];
+        Ptp[pl * nh + bofs(i, r, n)] = hs[pl];
       }
     }
   std::vector<double> ck(n);
