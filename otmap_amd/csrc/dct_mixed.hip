@@ -538,15 +538,17 @@ __device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds)
 // (also a compiler memory barrier: no memory access moves across it).
 template <int N>
 __device__ inline void wait_vm_barrier() {
-  static_assert(N == 0 || N == 12, "waits used by the wide kernel");
+  static_assert(N == 0 || N == 6 || N == 12, "waits used by the wide kernel");
   if (N == 0)
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else if (N == 6)
+    asm volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
   else
     asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int ST>
-__global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, const int* __restrict__ starts,
+template <int ST, int NW>
+__global__ __launch_bounds__(NW * 64) void k_mgemm_w(const Job* __restrict__ jobs, const int* __restrict__ starts,
                                                int count, bool gate, int fin) {
   const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
   const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
@@ -556,13 +558,14 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   const int done = gate ? st->done : 0;
   const int n = J.f.n, h = n / 2;
   const long nh = long(n) * h, nn = long(n) * n;
-  constexpr int TI = 4, TJ = 4;  // MFMA tiles per wave
+  constexpr int WTH = NW * 64;                 // threads
+  constexpr int TI = 4, TJ = NW == 4 ? 4 : 2;  // MFMA tiles per wave (64 x 64 or 64 x 32)
   const int ntn = (ST == M1 || ST == M2 ? n : h) / WT;
   const int ntm = ST == M1 ? h / (WT / 2) : n / WT;
   if (lt >= ntm * ntn) {
     if (ST == M4 && (fin == FIN_OUT || fin == FIN_ADD_OUT)) {
       double z[1] = {0.};
-      write_partials_at<1>(z, J.f.site(SITE_DNORM), lt, nbk);
+      write_partials_at<1, WTH>(z, J.f.site(SITE_DNORM), lt, nbk);
     }
     return;
   }
@@ -595,10 +598,10 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   // one contiguous 1 KB block and a fragment read is one contiguous 1 KB run
   constexpr int SLOTS = 2 * 3 * WT * 4;  // 16-byte chunks per K step
   constexpr int BUF = SLOTS * 16;        // bytes per buffer
-  constexpr int DMA = SLOTS / NT;        // transfers per thread per K step (12)
+  constexpr int DMA = SLOTS / WTH;       // transfers per thread per K step (12 or 6)
   __shared__ __attribute__((aligned(16))) unsigned char smem[3 * BUF];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int wm = NW == 4 ? (w >> 1) * 64 : (w >> 2) * 64, wn = NW == 4 ? (w & 1) * 64 : (w & 3) * 32;
   const int lr = lane & 15, lq = lane >> 4;  // MFMA output layout: lane (lr, lq) holds rows 4 lq .. + 3 of column lr
   auto arow = [&](int row) {
     if (ST == M1) return row < WT / 2 ? m0 + row : n - m0 - WT + row;
@@ -607,15 +610,15 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
 
   __shared__ double cfac[2 * WT];
   double cf = 0.;
-  if (ST == M2) cf = gp(J.ck)[t < WT ? m0 + t : n0 + t - WT];
+  if (ST == M2 && t < 2 * WT) cf = gp(J.ck)[t < WT ? m0 + t : n0 + t - WT];
 
-  // transfer u of this lane (u < 6: A, else B) fills LDS block 4u + w of the
-  // step (plane ((4u + w) % 24) / 8, row block (4u + w) % 8) at position
-  // `lane`: its byte offset from the operand base, K step 0
+  // transfer u of this lane (u < DMA / 2: A, else B) fills LDS block NW u + w
+  // of the step (plane ((NW u + w) % 24) / 8, row block (NW u + w) % 8) at
+  // position `lane`: its byte offset from the operand base, K step 0
   unsigned voff[DMA];
 #pragma unroll
   for (int u = 0; u < DMA; ++u) {
-    const int blk = (4 * u + w) % 24, pl = blk >> 3, rb = blk & 7;
+    const int blk = (NW * u + w) % 24, pl = blk >> 3, rb = blk & 7;
     const long e = u < DMA / 2 ? pl * aps + bofs(arow(16 * rb), 0, lda) : pl * bps + bofs(n0 + 16 * rb, 0, ldb);
     voff[u] = unsigned((e + lane * 8) * 2);
   }
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
     const u16* b = Bb + kb * 512;
 #pragma unroll
     for (int u = 0; u < DMA; ++u)
-      lds_dma16(u < DMA / 2 ? (const void*)a : (const void*)b, voff[u], wlds + buf * BUF + u * 4096);
+      lds_dma16(u < DMA / 2 ? (const void*)a : (const void*)b, voff[u], wlds + buf * BUF + u * NW * 1024);
   };
   // fragment of 16-row block rb of plane pl of operand op
   auto frag = [&](int buf, int op, int pl, int rb) {
@@ -651,7 +654,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   // read the fragments and issue the 96 MFMAs.
   auto body = [&](f4 (&c)[TI][TJ], int q, int buf) {
     if (q + 1 < S)
-      wait_vm_barrier<12>();
+      wait_vm_barrier<DMA>();
     else
       wait_vm_barrier<0>();
     if (q + 2 < S) issue(q + 2, (buf + 2) % 3);
@@ -681,7 +684,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     return;
   }
-  if (ST == M2) cfac[t] = cf;
+  if (ST == M2 && t < 2 * WT) cfac[t] = cf;
   // S is a multiple of 4: six positions per round (buffers 0, 1, 2, 0, 1, 2),
   // then 2 or 4 more (buffers 0, 1 and 2, 0)
   int q = 0;
@@ -733,7 +736,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
             nrm[0] += va * va + vb * vb;
           }
         }
-    if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nbk);
+    if (fin == FIN_OUT || fin == FIN_ADD_OUT) write_partials_at<1, WTH>(nrm, J.f.site(SITE_DNORM), lt, nbk);
     return;
   }
 
@@ -754,7 +757,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   if (ST == M1) {
     stage(1.f);
     const auto out = gp(J.f.mwa);
-    for (int it = t; it < WT * (WT / 8); it += NT) {
+    for (int it = t; it < WT * (WT / 8); it += WTH) {
       const int c = it / (WT / 8), a0 = (it % (WT / 8)) * 4;
       u16 u[3][4], wv[3][4];
 #pragma unroll
@@ -776,7 +779,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   } else if (ST == M2) {
     stage(1.f);
     const auto out = gp(J.f.mwb);
-    for (int it = t; it < WT * WT / 8; it += NT) {
+    for (int it = t; it < WT * WT / 8; it += WTH) {
       const int r = it / (WT / 8), c0 = (it % (WT / 8)) * 8;
       const int lrow = m0 + r;
       const double cl = cfac[r];
@@ -804,7 +807,7 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
     for (int round = 0; round < 2; ++round) {
       if (round == 1) __syncthreads();  // round 0's reads of the staged tile are done
       stage(round == 0 ? 1.f : -1.f);
-      for (int it = t; it < WT * (WT / 8); it += NT) {
+      for (int it = t; it < WT * (WT / 8); it += WTH) {
         const int c = it / (WT / 8), r0 = (it % (WT / 8)) * 8;
         u16 sv[3][8];
 #pragma unroll
@@ -822,11 +825,13 @@ __global__ __launch_bounds__(NT) void k_mgemm_w(const Job* __restrict__ jobs, co
   }
 }
 
-template <int ST>
+// config 2: four waves of 64 x 64; config 3: eight waves of 64 x 32 (two per SIMD)
+template <int ST, int C>
 void launch_mstage_w(const JobSet& js, bool gate, int fin, hipStream_t s) {
-  const int kind = mixed_kind(ST, 2);
+  constexpr int NW = C == 2 ? 4 : 8;
+  const int kind = mixed_kind(ST, C);
   if (js.total[kind] == 0) return;
-  hipLaunchKernelGGL((k_mgemm_w<ST>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s, (const Job*)js.jobs,
+  hipLaunchKernelGGL((k_mgemm_w<ST, NW>), dim3(unsigned(js.total[kind])), dim3(NW * 64), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(kind), js.count, gate, fin);
   OTM_HIP(hipGetLastError());
 }
@@ -847,7 +852,8 @@ void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
   switch (js.mcfg[ST]) {
     case 0: launch_mstage_tn<ST, 0>(js, gate, fin, s); break;
     case 1: launch_mstage_tn<ST, 1>(js, gate, fin, s); break;
-    default: launch_mstage_w<ST>(js, gate, fin, s); break;
+    case 2: launch_mstage_w<ST, 2>(js, gate, fin, s); break;
+    default: launch_mstage_w<ST, 3>(js, gate, fin, s); break;
   }
 }
 
@@ -949,7 +955,7 @@ namespace {
 using MStageFn = void (*)(const JobSet&, bool, int, hipStream_t);
 template <int ST>
 constexpr std::array<MStageFn, NMIXED> mstage_row() {
-  return {launch_mstage_tn<ST, 0>, launch_mstage_tn<ST, 1>, launch_mstage_w<ST>};
+  return {launch_mstage_tn<ST, 0>, launch_mstage_tn<ST, 1>, launch_mstage_w<ST, 2>, launch_mstage_w<ST, 3>};
 }
 const std::array<MStageFn, NMIXED> MSTAGE_TABLE[4] = {mstage_row<M1>(), mstage_row<M2>(), mstage_row<M3>(),
                                                        mstage_row<M4>()};

@@ -153,12 +153,12 @@ inline int fold_tiles(int n, int st, int c) {
 inline int flat_blocks(long F) { return int(std::min<long>((F + NT * 4 - 1) / (NT * 4), 1024)); }
 
 // mixed-transform launch configurations: tile width (output columns per
-// workgroup) and tile rows.  Configs 0, 1: the two-group form; config 2: the
-// wide form (128 x 128 tiles, four 64 x 64 waves, operands loaded straight
-// into LDS).
-constexpr int NMIXED = 3;
-constexpr int MIXED_TN[NMIXED] = {32, 64, 128};
-constexpr int MIXED_TM[NMIXED] = {64, 64, 128};
+// workgroup) and tile rows.  Configs 0, 1: the two-group form; configs 2, 3:
+// the wide form (128 x 128 tiles, operands loaded straight into LDS) with
+// four 64 x 64 waves or eight 64 x 32 waves.
+constexpr int NMIXED = 4;
+constexpr int MIXED_TN[NMIXED] = {32, 64, 128, 128};
+constexpr int MIXED_TM[NMIXED] = {64, 64, 128, 128};
 
 inline int mres_tiles(int n) { return (n / 32) * (n / 64); }
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
