@@ -154,13 +154,20 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(const double* __restrict__ A, c
 // where fold(x)[i] = x[i] + s x[n-1-i] with s = +1 for the even half of the
 // output and -1 for the odd half.  Each stage is n^3 flops instead of 2 n^3.
 // Workgroup tile FM x FN (four waves, 2 x 2), K step KB, double-buffered LDS.
-// The tile shape is picked per stage by dct_tune (all shapes accumulate
-// over k in the same order, so every shape gives bit-identical results).
+// Every sum is taken in two halves of K, each in increasing k, added (F1, F2,
+// F3 rows i / F4 columns j) or subtracted (F3 rows n-1-i / F4 columns n-1-j)
+// at the end: F3 and F4 split at k = h (the fold needs both halves), F1 and
+// F2 at k = h/2 when n % 128 == 0 (else in one piece).  With G = 1 one group
+// of four waves runs both halves in turn; with G = 2 (K halves split between
+// two groups of four waves, 512 threads) each group runs one half and the
+// second group's sums cross over LDS, so a 64 x 64 tile keeps eight waves
+// busy.  The tile shape is picked per stage by dct_tune (every shape sums in
+// the same order, so every shape gives bit-identical results).
 enum FoldStage { F1 = 0, F2 = 1, F3 = 2, F4 = 3 };
 
-template <int ST, int FM, int FN, int KB>
-__global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, const int* __restrict__ starts,
-                                                 int count, bool gate, bool rhs) {
+template <int ST, int FM, int FN, int KB, int G>
+__global__ __launch_bounds__(G * NT) void k_gemm_fold(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                     int count, bool gate, bool rhs) {
   const int p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
   const int lt = count == 1 ? int(blockIdx.x) : blockIdx.x - starts[p];
   const int nb = count == 1 ? int(gridDim.x) : starts[p + 1] - starts[p];
@@ -181,20 +188,27 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
   constexpr int ACHUNKS = FM * KB / 4, BCHUNKS = KB * FN / 4;  // chunks of 4 consecutive values
   constexpr int ACH = (ACHUNKS + NT - 1) / NT, BCH = (BCHUNKS + NT - 1) / NT;
   constexpr bool SPLIT = ST == F3 || ST == F4;
-  __shared__ double As[2][FM][KB + 1];
-  __shared__ double Bs[2][KB][FN + 2];
+  // LDS per group: A [2][FM][KB + 1], B [2][KB][FN + 2]; with G = 2 the
+  // epilogue reuses the space for the second group's FP64 tile [FM][FN + 1]
+  constexpr int A_D = 2 * FM * (KB + 1), B_D = 2 * KB * (FN + 2), G_D = A_D + B_D;
+  static_assert(G == 1 || G * G_D >= FM * (FN + 1), "epilogue staging must fit");
+  __shared__ double smem[G * G_D];
   const int ntn = cols / FN, ntm = (ST == F3 ? h : n) / FM;
   if (lt >= ntm * ntn) {  // padding workgroup of the job: neutral partial only
     if (ST == F4) {
       double z[1] = {0.};
-      write_partials_at<1>(z, J.f.site(SITE_DNORM), lt, nb);
+      write_partials_at<1, G * NT>(z, J.f.site(SITE_DNORM), lt, nb);
     }
     return;
   }
   int tmi, tni;
   xcd_tile(lt, ntm, ntn, tmi, tni);
   const int m0 = tmi * FM, n0 = tni * FN;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int g = G == 2 ? int(threadIdx.x) / NT : 0;
+  const int t = threadIdx.x % NT, lane = t & 63, w = t >> 6;
+  double* const gs = smem + g * G_D;
+  auto As = [&](int buf, int r, int c) -> double& { return gs[(buf * FM + r) * (KB + 1) + c]; };
+  auto Bs = [&](int buf, int r, int c) -> double& { return gs[A_D + (buf * KB + r) * (FN + 2) + c]; };
   const int wm = (w >> 1) * (FM / 2), wn = (w & 1) * (FN / 2);
   const int lr = lane & 15, lk = lane >> 4;
   const double sgnA = (ST == F2 && n0 >= h) ? -1. : 1.;
@@ -242,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
       if (ACHUNKS % NT == 0 || c < ACHUNKS) {
         const int ar = c / (KB / 4), ak = (c % (KB / 4)) * 4;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) As[buf][ar][ak + q] = xa[u][q];
+        for (int q = 0; q < 4; ++q) As(buf, ar, ak + q) = xa[u][q];
       }
     }
 #pragma unroll
@@ -251,7 +265,7 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
       if (BCHUNKS % NT == 0 || c < BCHUNKS) {
         const int bk = c / (FN / 4), bc = (c % (FN / 4)) * 4;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Bs[buf][bk][bc + q] = xb[u][q];
+        for (int q = 0; q < 4; ++q) Bs(buf, bk, bc + q) = xb[u][q];
       }
     }
   };
@@ -260,9 +274,9 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
     for (int kk = 0; kk < KB; kk += 4) {
       double a[TI], b[TJ];
 #pragma unroll
-      for (int i = 0; i < TI; ++i) a[i] = As[buf][wm + 16 * i + lr][kk + lk];
+      for (int i = 0; i < TI; ++i) a[i] = As(buf, wm + 16 * i + lr, kk + lk);
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) b[j] = Bs[buf][kk + lk][wn + 16 * j + lr];
+      for (int j = 0; j < TJ; ++j) b[j] = Bs(buf, kk + lk, wn + 16 * j + lr);
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -274,9 +288,10 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
   // s & 1 (loaded two steps earlier) into LDS buffer s & 1, refills the slot
   // with step s + 2, syncs and multiplies; the buffer written at step s was
   // last read at step s - 2, before every wave passed the sync of step s - 1.
-  // Loads past the last step re-read the last K block (in bounds, never
-  // stored).  The split stages move the first-half sums to acc_lo when step
-  // h / KB starts (the k order of every accumulation is unchanged).
+  // Loads past the group's last step re-read its last K block (in bounds,
+  // never stored).  With G = 1 the first-half sums move to acc_lo when step
+  // ksplit / KB starts (the k order of every accumulation is unchanged); with
+  // G = 2 group g runs the steps of half g into acc.
   d4 acc[TI][TJ], acc_lo[TI][TJ];
 #pragma unroll
   for (int i = 0; i < TI; ++i)
@@ -285,10 +300,13 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
       acc[i][j] = d4{0., 0., 0., 0.};
       acc_lo[i][j] = d4{0., 0., 0., 0.};
     }
-  const int nsteps = K / KB, hsteps = SPLIT ? h / KB : -1;
-  const int klast = K - KB;
+  const int ksplit = SPLIT ? h : (n % 128 == 0 ? h / 2 : K);
+  const int hsteps = G == 1 ? ksplit / KB : -1;
+  const int kbeg = G == 2 && g == 1 ? ksplit : 0, kend = G == 2 && g == 0 ? ksplit : K;
+  const int nsteps = (kend - kbeg) / KB;
+  const int klast = kend - KB;
   auto step = [&](double (&xa)[ACH][4], double (&xb)[BCH][4], int s, int buf) {
-    if (SPLIT && s == hsteps) {
+    if (G == 1 && s == hsteps) {
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -298,12 +316,12 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
         }
     }
     sstore(xa, xb, buf);
-    gload(xa, xb, min((s + 2) * KB, klast));
+    gload(xa, xb, min(kbeg + (s + 2) * KB, klast));
     __syncthreads();
     compute(acc, buf);
   };
-  gload(ra[0], rb[0], 0);
-  gload(ra[1], rb[1], min(KB, klast));
+  gload(ra[0], rb[0], kbeg);
+  gload(ra[1], rb[1], min(kbeg + KB, klast));
   int s = 0;
   for (; s + 2 <= nsteps; s += 2) {
     step(ra[0], rb[0], s, 0);
@@ -311,46 +329,61 @@ __global__ __launch_bounds__(NT) void k_gemm_fold(const Job* __restrict__ jobs, 
   }
   if (s < nsteps) step(ra[0], rb[0], s, 0);
 
-  // lane l holds D[l/16 + 4v][l%16] of each 16x16 tile
+  // lane l holds D[l/16 + 4v][l%16] of each 16x16 tile.  The two halves of
+  // every sum: lo (k < ksplit) and hi (k >= ksplit); with G = 2 the second
+  // group hands its sums (hi) to the first over LDS
+  if (G == 2) {
+    __syncthreads();  // every wave is done with its operand buffers
+    if (g == 1)
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) smem[(wm + 16 * i + lk + 4 * v) * (FN + 1) + wn + 16 * j + lr] = acc[i][j][v];
+    __syncthreads();
+  }
   double nrm[1] = {0.};
+  if (G == 1 || g == 0)
 #pragma unroll
-  for (int i = 0; i < TI; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < TJ; ++j)
+      for (int j = 0; j < TJ; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm + 16 * i + lk + 4 * v;
-        const int col = n0 + wn + 16 * j + lr;
-        if (ST == F1) {
-          D[long(row) * n + col] = acc[i][j][v];
-        } else if (ST == F2) {
-          const int k = row < h ? 2 * row : 2 * (row - h) + 1;
-          const int l = col < h ? 2 * col : 2 * (col - h) + 1;
-          const int dm = k > l ? k - l : l - k;
-          const double lam = 2. * (sq[dm] + sq[k + l]);
-          D[long(row) * n + col] = (k == 0 && l == 0) ? 0. : acc[i][j][v] / lam;
-        } else {
-          const double pv = acc_lo[i][j][v], qv = acc[i][j][v];
-          const double s0 = pv + qv, s1 = pv - qv;
-          if (ST == F3) {
-            D[long(row) * n + col] = s0;
-            D[long(n - 1 - row) * n + col] = s1;
+        for (int v = 0; v < 4; ++v) {
+          const int tr = wm + 16 * i + lk + 4 * v, tc = wn + 16 * j + lr;
+          const int row = m0 + tr, col = n0 + tc;
+          const double lo = G == 2 ? acc[i][j][v] : acc_lo[i][j][v];
+          const double hi = G == 2 ? smem[tr * (FN + 1) + tc] : acc[i][j][v];
+          if (ST == F1) {
+            D[long(row) * n + col] = ksplit < K ? lo + hi : hi;
+          } else if (ST == F2) {
+            const int k = row < h ? 2 * row : 2 * (row - h) + 1;
+            const int l = col < h ? 2 * col : 2 * (col - h) + 1;
+            const int dm = k > l ? k - l : l - k;
+            const double lam = 2. * (sq[dm] + sq[k + l]);
+            D[long(row) * n + col] = (k == 0 && l == 0) ? 0. : (ksplit < K ? lo + hi : hi) / lam;
           } else {
-            D[long(row) * n + col] = s0;
-            D[long(row) * n + (n - 1 - col)] = s1;
-            nrm[0] += s0 * s0 + s1 * s1;
+            const double s0 = lo + hi, s1 = lo - hi;
+            if (ST == F3) {
+              D[long(row) * n + col] = s0;
+              D[long(n - 1 - row) * n + col] = s1;
+            } else {
+              D[long(row) * n + col] = s0;
+              D[long(row) * n + (n - 1 - col)] = s1;
+              nrm[0] += s0 * s0 + s1 * s1;
+            }
           }
         }
-      }
-  if (ST == F4) write_partials_at<1>(nrm, J.f.site(SITE_DNORM), lt, nb);
+  if (ST == F4) write_partials_at<1, G * NT>(nrm, J.f.site(SITE_DNORM), lt, nb);
 }
 
 template <int ST, int CFG>
 void launch_fold_cfg(const JobSet& js, bool gate, bool rhs, hipStream_t s) {
-  constexpr int FM = FOLD_CFG[CFG][0], FN = FOLD_CFG[CFG][1], KB = FOLD_CFG[CFG][2];
+  constexpr int FM = FOLD_CFG[CFG][0], FN = FOLD_CFG[CFG][1], KB = FOLD_CFG[CFG][2], G = FOLD_CFG[CFG][3];
   const int kind = fold_kind(ST, CFG);
   if (js.total[kind] == 0) return;
-  hipLaunchKernelGGL((k_gemm_fold<ST, FM, FN, KB>), dim3(unsigned(js.total[kind])), dim3(NT), 0, s,
+  hipLaunchKernelGGL((k_gemm_fold<ST, FM, FN, KB, G>), dim3(unsigned(js.total[kind])), dim3(G * NT), 0, s,
                      (const Job*)js.jobs, (const int*)js.starts_of(kind), js.count, gate, rhs);
   OTM_HIP(hipGetLastError());
 }

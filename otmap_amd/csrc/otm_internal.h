@@ -134,14 +134,19 @@ struct Job {
   double* adj_out = nullptr;        // copy of the adjusted density (F, nullable)
 };
 
-// Tile shapes (FM, FN, KB) of the folded FP64 DCT stages, picked per stage by
-// the autotuner among those that divide n/2 for every job of the table.
-constexpr int NFOLD = 8;
-constexpr int FOLD_CFG[NFOLD][3] = {{64, 64, 16}, {64, 32, 16}, {32, 64, 16}, {32, 32, 16},
-                                    {64, 64, 32}, {64, 32, 32}, {32, 64, 32}, {32, 32, 32}};
+// Tile shapes (FM, FN, KB, G) of the folded FP64 DCT stages, picked per stage
+// by the autotuner among those that divide n/2 for every job of the table:
+// FM x FN outputs per workgroup, K steps of KB, G groups of four waves (G = 2:
+// the two halves of K run on two groups, which needs n % 128 == 0 so that
+// F1/F2 split at h/2 on a K-step boundary).
+constexpr int NFOLD = 12;
+constexpr int FOLD_CFG[NFOLD][4] = {{64, 64, 16, 1}, {64, 32, 16, 1}, {32, 64, 16, 1}, {32, 32, 16, 1},
+                                    {64, 64, 32, 1}, {64, 32, 32, 1}, {32, 64, 32, 1}, {32, 32, 32, 1},
+                                    {64, 64, 16, 2}, {64, 32, 16, 2}, {64, 64, 32, 2}, {64, 32, 32, 2}};
 inline bool fold_cfg_valid(int n, int c) {
   const int h = n / 2;
-  return n % 64 == 0 && h % FOLD_CFG[c][0] == 0 && h % FOLD_CFG[c][1] == 0 && h % FOLD_CFG[c][2] == 0;
+  return n % 64 == 0 && h % FOLD_CFG[c][0] == 0 && h % FOLD_CFG[c][1] == 0 && h % FOLD_CFG[c][2] == 0 &&
+         (FOLD_CFG[c][3] == 1 || n % 128 == 0);
 }
 // workgroups of folded stage st (0..3) with shape c for grid size n
 inline int fold_tiles(int n, int st, int c) {
@@ -182,7 +187,7 @@ struct JobSet {
   std::vector<Job> host;   // host copy
   std::vector<int> total;  // workgroups per kind
   bool fold = false;       // every job uses the folded transform
-  int cfg[4] = {NFOLD - 1, NFOLD - 1, NFOLD - 1, NFOLD - 1};  // folded FP64 tile shape per stage
+  int cfg[4] = {7, 7, 7, 7};  // folded FP64 tile shape per stage (7: valid for every folded n; tuned at init)
   int mcfg[4] = {0, 0, 0, 0};                                 // mixed tile width per stage
   const int* starts_of(int kind) const { return starts + long(kind) * (count + 1); }
 };
