@@ -66,6 +66,29 @@ __host__ __device__ inline long bofs(long r, long k, long K) {
   return ((r >> 4) * (K >> 5) + (k >> 5)) * 512 + ((((k >> 3) & 3) << 4) + (r & 15)) * 8 + (k & 7);
 }
 
+// One 16-byte global -> LDS transfer per lane (global_load_lds_dwordx4): lane
+// l's 16 bytes at sbase + voff land at LDS byte lds + 16 l (lds wave-uniform).
+// Issued from asm so that the compiler's wait analysis, which cannot tell the
+// K-step buffers apart, does not drain every transfer before each LDS read;
+// the kernels wait for them explicitly (vmcnt) before the barrier that
+// publishes a buffer.  The instruction has no register results.
+__device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds) {
+  asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(sbase), "s"(lds) : "memory");
+}
+// Wait until at most N of this wave's vector memory operations are
+// outstanding and its LDS reads have completed, then the workgroup barrier
+// (also a compiler memory barrier: no memory access moves across it).
+template <int N>
+__device__ inline void wait_vm_barrier() {
+  static_assert(N == 0 || N == 6 || N == 12, "waits used by the mixed stages");
+  if (N == 0)
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else if (N == 6)
+    asm volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else
+    asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // exact 3-term split of an FP32 value
 __device__ inline void split3(float x, u16& h0, u16& h1, u16& h2) {
   h0 = bf16_bits(x);
@@ -280,54 +303,52 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  // LDS per group, in the blocks' fragment order: A [2 bufs][3 planes][4 row
-  // blocks][64 lanes] x 16 B, B [2][3][TN / 16][64] x 16 B; the epilogue
-  // reuses the space for the two groups' FP32 tiles [64][TN + 1]
-  constexpr int ARB = MTM / 16, BRB = TN / 16;  // 16-row blocks of A and B
-  constexpr int A_U16 = 2 * 3 * ARB * 512, B_U16 = 2 * 3 * BRB * 512, G_U16 = A_U16 + B_U16;
-  __shared__ __attribute__((aligned(16))) u16 smem[2 * G_U16];
-  static_assert(2 * G_U16 * 2 >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
-  const int t = threadIdx.x, g = t / NT, tg = t % NT, lane = t & 63, w = tg >> 6;
-  u16* const gs = smem + g * G_U16;
-  // lane position `l` of row block rb of plane pl
-  auto As = [&](int buf, int pl, int rb, int l) { return gs + ((buf * 3 + pl) * ARB + rb) * 512 + l * 8; };
-  auto Bs = [&](int buf, int pl, int rb, int l) { return gs + A_U16 + ((buf * 3 + pl) * BRB + rb) * 512 + l * 8; };
+  // LDS per group: two K-step buffers of [A: 3 planes x 4 row blocks][B: 3
+  // planes x TN/16 row blocks] blocks of 1 KB (the blocks of the HBM layout);
+  // the epilogue reuses the space for the two groups' FP32 tiles [64][TN + 1]
+  constexpr int BRB = TN / 16;               // 16-row blocks of B
+  constexpr int NBLK = 3 * 4 + 3 * BRB;      // blocks per K step (18 or 24)
+  constexpr int DMAW = (NBLK + 3) / 4;       // transfers per wave per K step
+  constexpr int GBUF = NBLK * 1024;          // bytes per buffer
+  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * 2 * GBUF];
+  static_assert(2 * 2 * GBUF >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
+  // group and wave indices are wave-uniform (readfirstlane tells the compiler)
+  const int t = threadIdx.x, g = __builtin_amdgcn_readfirstlane(t / NT), tg = t % NT, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tg >> 6);
   const int wm = (w >> 1) * 32, wn = (w & 1) * (TN / 2);
   const int lr = lane & 15, lq = lane >> 4;  // MFMA output layout: lane (lr, lq) holds rows 4 lq .. + 3 of column lr
-
-  // global -> register staging, two slots of K steps in flight.  Chunk c =
-  // tg + u * NT of a K step is lane c % 64 of block c / 64 (A: plane u, row
-  // block w; B: plane (c / 64) / BRB, row block (c / 64) % BRB), so each wave
-  // loads and stores whole 1 KB blocks.  Chunk indices past B's blocks reload
-  // an earlier chunk, never stored, so every lane issues the same loads and
-  // the waits on the slots stay exact.
-  u32x4 ra[2][3], rb[2][BCH];
   auto arow = [&](int row) {  // global row of tile row `row`
     if (ST == M1) return row < MTM / 2 ? m0 + row : n - m0 - MTM + row;  // rows i and mirrored n-1-i
     return m0 + row;
   };
-  auto gload = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int k0) {
+  // transfer u of this lane: block b = 4u + w of the group's step (b < 12: A
+  // plane b / 4, row block b % 4; else B plane (b - 12) / BRB, row block
+  // (b - 12) % BRB), at position `lane`: byte offset from the operand base, K block 0
+  unsigned voff[DMAW];
 #pragma unroll
-    for (int u = 0; u < 3; ++u)
-      xa[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(A + u * aps + bofs(arow(16 * w), k0, lda) + lane * 8);
+  for (int u = 0; u < DMAW; ++u) {
+    const int b = min(4 * u + w, NBLK - 1);
+    long e;
+    if (b < 12)
+      e = (b >> 2) * aps + bofs(arow(16 * (b & 3)), 0, lda);
+    else
+      e = ((b - 12) / BRB) * bps + bofs(n0 + 16 * ((b - 12) % BRB), 0, ldb);
+    voff[u] = unsigned((e + lane * 8) * 2);
+  }
+  const unsigned lds_g = unsigned(size_t((__attribute__((address_space(3))) unsigned char*)smem)) + g * 2 * GBUF;
+  const unsigned wlds = lds_g + unsigned(w) * 1024;
+  // the group's K steps: [kb0, kb0 + mine) in blocks of 32 k
+  const int S = K / MKB, S0 = (S + 1) >> 1, mine = g == 0 ? S0 : S - S0;
+  const int kb0 = g * S0;
+  auto issue = [&](int s, int buf) {
+    const u16* a = Ab + (kb0 + s) * 512;
+    const u16* bp = Bb + (kb0 + s) * 512;
 #pragma unroll
-    for (int u = 0; u < BCH; ++u) {
-      const int c0 = tg + u * NT, c = c0 < BCHUNKS ? c0 : c0 - TN * 4;
-      const int b = c >> 6, pl = b / BRB, bb = b % BRB;
-      xb[u] = *reinterpret_cast<const OTM_GLOBAL u32x4*>(B + pl * bps + bofs(n0 + 16 * bb, k0, ldb) + lane * 8);
-    }
+    for (int u = 0; u < DMAW; ++u)
+      if (4 * u + w < NBLK) lds_dma16(4 * u < 12 ? (const void*)a : (const void*)bp, voff[u], wlds + buf * GBUF + u * 4096);
   };
-  auto sstore = [&](const u32x4 (&xa)[3], const u32x4 (&xb)[BCH], int buf) {
-#pragma unroll
-    for (int u = 0; u < 3; ++u) *reinterpret_cast<u32x4*>(As(buf, u, w, lane)) = xa[u];
-#pragma unroll
-    for (int u = 0; u < BCH; ++u) {
-      const int c = tg + u * NT;
-      if (BCHUNKS % NT == 0 || c < BCHUNKS) {
-        const int b = c >> 6;
-        *reinterpret_cast<u32x4*>(Bs(buf, b / BRB, b % BRB, lane)) = xb[u];
-      }
-    }
+  auto frag = [&](int buf, int blk) {
+    return *reinterpret_cast<const bf8*>(smem + (g * 2 + buf) * GBUF + ((blk * 64 + lane) << 4));
   };
 
   auto compute = [&](f4 (&c)[2][TJ], int buf) {
@@ -335,9 +356,9 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) fa[i][pl] = *reinterpret_cast<const bf8*>(As(buf, pl, wm / 16 + i, lane));
+      for (int i = 0; i < 2; ++i) fa[i][pl] = frag(buf, pl * 4 + wm / 16 + i);
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) fb[j][pl] = *reinterpret_cast<const bf8*>(Bs(buf, pl, wn / 16 + j, lane));
+      for (int j = 0; j < TJ; ++j) fb[j][pl] = frag(buf, 12 + pl * BRB + wn / 16 + j);
     }
     // small products first, the leading one last
     constexpr int PA[6] = {2, 1, 0, 1, 0, 0};
@@ -351,40 +372,38 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
           c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][PA[q]], fb[j][PB[q]], c[i][j], 0, 0, 0);
   };
 
-  // Software pipeline of each group, two K steps in flight: step s stores
-  // register slot s & 1 (loaded two steps earlier) into LDS buffer s & 1,
-  // refills the slot with step s + 2, syncs and multiplies.  The buffer
-  // written at step s was last read at step s - 2, before every wave passed
-  // the sync of step s - 1.  Both groups run S0 steps (the same barriers);
-  // when S is odd the last step of group 1 multiplies nothing.  Loads past a
-  // group's last step re-read an in-bounds K block that is never used.
-  const int S = K / MKB, S0 = (S + 1) >> 1, mine = g == 0 ? S0 : S - S0;
-  const int kbeg = g * S0 * MKB, klast = K - MKB;
+  // Each group runs S0 bodies in lockstep with the other (the same
+  // barriers); body s: wait for this wave's transfers of step s and its LDS
+  // reads, barrier (every wave's transfers of step s are in LDS, and every
+  // wave is done reading buffer (s + 1) & 1, last read at body s - 1), start
+  // the transfers of step s + 1 into that buffer, multiply step s.  When S is
+  // odd the last body of group 1 multiplies nothing.
   f4 acc[2][TJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](u32x4 (&xa)[3], u32x4 (&xb)[BCH], int s, int buf) {
-    sstore(xa, xb, buf);
-    gload(xa, xb, min(kbeg + (s + 2) * MKB, klast));
-    __syncthreads();
-    if (s < mine) compute(acc, buf);
-  };
   // M2: eigenvalue factors of the tile's rows and columns, staged in LDS for the epilogue
   __shared__ double cfac[MTM + TN];
   double cf = 0.;
   if (ST == M2 && t < MTM + TN) cf = gp(J.ck)[t < MTM ? m0 + t : n0 + t - MTM];
-  gload(ra[0], rb[0], min(kbeg, klast));
-  gload(ra[1], rb[1], min(kbeg + MKB, klast));
-  if (done) return;
+  if (mine > 0) issue(0, 0);
+  if (done) {  // no transfer may outlive the workgroup
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    return;
+  }
   if (ST == M2 && t < MTM + TN) cfac[t] = cf;
+  auto body = [&](int s, int buf) {
+    wait_vm_barrier<0>();
+    if (s + 1 < mine) issue(s + 1, buf ^ 1);
+    if (s < mine) compute(acc, buf);
+  };
   int s = 0;
   for (; s + 2 <= S0; s += 2) {
-    step(ra[0], rb[0], s, 0);
-    step(ra[1], rb[1], s + 1, 1);
+    body(s, 0);
+    body(s + 1, 1);
   }
-  if (s < S0) step(ra[0], rb[0], s, 0);
+  if (s < S0) body(s, 0);
   __syncthreads();
 
   // stage both groups' FP32 tiles: lane (lr, lq) of MFMA tile (i, j) holds
@@ -524,28 +543,6 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
 // ---------------------------------------------------------------------------
 constexpr int WT = 128;  // wide tile rows / columns
 
-// One 16-byte global -> LDS transfer per lane (global_load_lds_dwordx4): lane
-// l's 16 bytes at sbase + voff land at LDS byte lds + 16 l (lds wave-uniform).
-// Issued from asm so that the compiler's wait analysis, which cannot tell the
-// three K-step buffers apart, does not drain every transfer before each LDS
-// read; the kernel waits for them explicitly (vmcnt) before the barrier that
-// publishes a buffer.  The instruction has no register results.
-__device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds) {
-  asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(sbase), "s"(lds) : "memory");
-}
-// Wait until at most N of this wave's vector memory operations are
-// outstanding and its LDS reads have completed, then the workgroup barrier
-// (also a compiler memory barrier: no memory access moves across it).
-template <int N>
-__device__ inline void wait_vm_barrier() {
-  static_assert(N == 0 || N == 6 || N == 12, "waits used by the wide kernel");
-  if (N == 0)
-    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-  else if (N == 6)
-    asm volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-  else
-    asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 template <int ST, int NW>
 __global__ __launch_bounds__(NW * 64) void k_mgemm_w(const Job* __restrict__ jobs, const int* __restrict__ starts,
