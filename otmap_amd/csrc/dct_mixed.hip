@@ -80,9 +80,13 @@ __device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds)
 // (also a compiler memory barrier: no memory access moves across it).
 template <int N>
 __device__ inline void wait_vm_barrier() {
-  static_assert(N == 0 || N == 6 || N == 12, "waits used by the mixed stages");
+  static_assert(N == 0 || N == 4 || N == 5 || N == 6 || N == 12, "waits used by the mixed stages");
   if (N == 0)
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else if (N == 4)
+    asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else if (N == 5)
+    asm volatile("s_waitcnt vmcnt(5) lgkmcnt(0)\n\ts_barrier" ::: "memory");
   else if (N == 6)
     asm volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
   else
@@ -303,15 +307,15 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   const auto A = gp(Ab);
   const auto B = gp(Bb);
 
-  // LDS per group: two K-step buffers of [A: 3 planes x 4 row blocks][B: 3
+  // LDS per group: three K-step buffers of [A: 3 planes x 4 row blocks][B: 3
   // planes x TN/16 row blocks] blocks of 1 KB (the blocks of the HBM layout);
   // the epilogue reuses the space for the two groups' FP32 tiles [64][TN + 1]
   constexpr int BRB = TN / 16;               // 16-row blocks of B
   constexpr int NBLK = 3 * 4 + 3 * BRB;      // blocks per K step (18 or 24)
-  constexpr int DMAW = (NBLK + 3) / 4;       // transfers per wave per K step
+  constexpr int DMAW = (NBLK + 3) / 4;       // transfers per wave per K step (at most)
   constexpr int GBUF = NBLK * 1024;          // bytes per buffer
-  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * 2 * GBUF];
-  static_assert(2 * 2 * GBUF >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
+  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * 3 * GBUF];
+  static_assert(2 * 3 * GBUF >= 2 * MTM * (TN + 1) * 4, "epilogue staging must fit");
   // group and wave indices are wave-uniform (readfirstlane tells the compiler)
   const int t = threadIdx.x, g = __builtin_amdgcn_readfirstlane(t / NT), tg = t % NT, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(tg >> 6);
@@ -335,7 +339,7 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
       e = ((b - 12) / BRB) * bps + bofs(n0 + 16 * ((b - 12) % BRB), 0, ldb);
     voff[u] = unsigned((e + lane * 8) * 2);
   }
-  const unsigned lds_g = unsigned(size_t((__attribute__((address_space(3))) unsigned char*)smem)) + g * 2 * GBUF;
+  const unsigned lds_g = unsigned(size_t((__attribute__((address_space(3))) unsigned char*)smem)) + g * 3 * GBUF;
   const unsigned wlds = lds_g + unsigned(w) * 1024;
   // the group's K steps: [kb0, kb0 + mine) in blocks of 32 k
   const int S = K / MKB, S0 = (S + 1) >> 1, mine = g == 0 ? S0 : S - S0;
@@ -348,7 +352,7 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
       if (4 * u + w < NBLK) lds_dma16(4 * u < 12 ? (const void*)a : (const void*)bp, voff[u], wlds + buf * GBUF + u * 4096);
   };
   auto frag = [&](int buf, int blk) {
-    return *reinterpret_cast<const bf8*>(smem + (g * 2 + buf) * GBUF + ((blk * 64 + lane) << 4));
+    return *reinterpret_cast<const bf8*>(smem + (g * 3 + buf) * GBUF + ((blk * 64 + lane) << 4));
   };
 
   auto compute = [&](f4 (&c)[2][TJ], int buf) {
@@ -373,11 +377,12 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   };
 
   // Each group runs S0 bodies in lockstep with the other (the same
-  // barriers); body s: wait for this wave's transfers of step s and its LDS
-  // reads, barrier (every wave's transfers of step s are in LDS, and every
-  // wave is done reading buffer (s + 1) & 1, last read at body s - 1), start
-  // the transfers of step s + 1 into that buffer, multiply step s.  When S is
-  // odd the last body of group 1 multiplies nothing.
+  // barriers); body s (buffer s % 3): wait for this wave's transfers of step
+  // s (those of step s + 1 may stay in flight) and its LDS reads, barrier
+  // (every wave's transfers of step s are in LDS, and every wave is done
+  // reading buffer (s + 2) % 3, last read at body s - 1), start the
+  // transfers of step s + 2 into that buffer, multiply step s.  When S is odd
+  // the last body of group 1 multiplies nothing.
   f4 acc[2][TJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -388,22 +393,33 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
   double cf = 0.;
   if (ST == M2 && t < MTM + TN) cf = gp(J.ck)[t < MTM ? m0 + t : n0 + t - MTM];
   if (mine > 0) issue(0, 0);
+  if (mine > 1) issue(1, 1);
   if (done) {  // no transfer may outlive the workgroup
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     return;
   }
   if (ST == M2 && t < MTM + TN) cfac[t] = cf;
+  const int mycount = (NBLK - w + 3) / 4;  // this wave's transfers per step
   auto body = [&](int s, int buf) {
-    wait_vm_barrier<0>();
-    if (s + 1 < mine) issue(s + 1, buf ^ 1);
+    if (s + 1 >= mine)
+      wait_vm_barrier<0>();
+    else if (mycount == 6)
+      wait_vm_barrier<6>();
+    else if (mycount == 5)
+      wait_vm_barrier<5>();
+    else
+      wait_vm_barrier<4>();
+    if (s + 2 < mine) issue(s + 2, (buf + 2) % 3);
     if (s < mine) compute(acc, buf);
   };
   int s = 0;
-  for (; s + 2 <= S0; s += 2) {
+  for (; s + 3 <= S0; s += 3) {
     body(s, 0);
     body(s + 1, 1);
+    body(s + 2, 2);
   }
   if (s < S0) body(s, 0);
+  if (s + 1 < S0) body(s + 1, 1);
   __syncthreads();
 
   // stage both groups' FP32 tiles: lane (lr, lq) of MFMA tile (i, j) holds
