@@ -37,15 +37,14 @@
 
 #include "otm_internal.h"
 #include "reduce.h"
+#include "split.h"
 
 namespace otm {
 namespace {
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef unsigned short u16;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int MTM = 64;        // A rows per workgroup tile
 constexpr int MKB = 32;        // K step
@@ -54,17 +53,7 @@ enum MStage { M1 = 0, M2 = 1, M3 = 2, M4 = 3 };
 // M4 epilogue: x0 = D | d_hat = D (+partials) | d_hat = x0 + D (+partials) | x0 += D
 enum MFinal { FIN_X0 = 0, FIN_OUT = 1, FIN_ADD_OUT = 2, FIN_ADD_X0 = 3 };
 
-__device__ inline u16 bf16_bits(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
-__device__ inline float bf16_float(u16 b) { return (float)__builtin_bit_cast(__bf16, b); }
 
-// Element offset of (r, k) in a BF16 plane of row length K (K % 32 == 0)
-// stored in blocks of 16 rows x 32 k (row blocks outer, k blocks inner); in a
-// block, lane position l = 16 (k / 8 % 4) + r % 16 holds the 8 values
-// k / 8 * 8 .. + 7 of row r, which is what lane l of v_mfma_f32_16x16x32_bf16
-// takes as its A or B fragment.
-__host__ __device__ inline long bofs(long r, long k, long K) {
-  return ((r >> 4) * (K >> 5) + (k >> 5)) * 512 + ((((k >> 3) & 3) << 4) + (r & 15)) * 8 + (k & 7);
-}
 
 // One 16-byte global -> LDS transfer per lane (global_load_lds_dwordx4): lane
 // l's 16 bytes at sbase + voff land at LDS byte lds + 16 l (lds wave-uniform).
@@ -93,21 +82,6 @@ __device__ inline void wait_vm_barrier() {
     asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// exact 3-term split of an FP32 value
-__device__ inline void split3(float x, u16& h0, u16& h1, u16& h2) {
-  h0 = bf16_bits(x);
-  const float r1 = x - bf16_float(h0);
-  h1 = bf16_bits(r1);
-  h2 = bf16_bits(r1 - bf16_float(h1));
-}
-
-// 3-term split of an FP64 value (24 significant bits)
-__device__ inline void split3d(double x, u16& h0, u16& h1, u16& h2) {
-  h0 = bf16_bits(float(x));
-  const double r1 = x - double(bf16_float(h0));
-  h1 = bf16_bits(float(r1));
-  h2 = bf16_bits(float(r1 - double(bf16_float(h1))));
-}
 
 // ---------------------------------------------------------------------------
 // right-hand side: fold the columns of r' = r - (sum r) e0 and split them
@@ -876,11 +850,15 @@ void launch_mresid(const JobSet& js, bool gate, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
+}  // namespace
+
 void launch_mprep(const JobSet& js, bool gate, hipStream_t s) {
   hipLaunchKernelGGL(k_mprep, dim3(unsigned(js.total[KIND_FLAT])), dim3(NT), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(KIND_FLAT), js.count, gate);
   OTM_HIP(hipGetLastError());
 }
+
+namespace {
 
 void mixed_transform(const JobSet& js, bool gate, int fin, hipStream_t s) {
   launch_mstage<M1>(js, gate, 0, s);
@@ -1012,8 +990,10 @@ void mixed_tune(JobSet& js, int device, hipStream_t s) {
 // d_hat = M^+ (r - S e0) of every job: mixed transform, then `refine` FP64
 // refinement steps x += transform(r' - M0 x); the last stage leaves the
 // |d_hat|^2 partials in SITE_DNORM (launch kind mixed_kind(3, js.mcfg[3])).
-void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s) {
-  launch_mprep(js, gate, s);
+// prep = false: the folded planes of r' are already in place (the update
+// pass of the previous iteration wrote them).
+void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s, bool prep) {
+  if (prep) launch_mprep(js, gate, s);
   if (refine <= 0) {
     mixed_transform(js, gate, FIN_OUT, s);
     return;

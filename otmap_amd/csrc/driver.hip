@@ -116,13 +116,14 @@ int dnorm_kind(const JobSet& js, bool mixed) {
 
 void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s) {
   const bool mixed = opt.precision == OTM_PRECISION_MIXED;
+  // mixed: the update pass writes the next iteration's folded right-hand side
   if (mixed)
-    launch_mixed_solve(js, opt.refine_steps, true, s);
+    launch_mixed_solve(js, opt.refine_steps, true, s, false);
   else
     launch_dct_solve(js, plan, s);
   launch_beta(js, dnorm_kind(js, mixed), s);
   launch_dir_coef(js, s);
-  launch_update(js, s);
+  launch_update(js, s, mixed);
 }
 
 int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& opt, float* ms) {
@@ -136,6 +137,7 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));
   launch_start(js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, s);
+  if (opt.precision == OTM_PRECISION_MIXED) launch_mprep(js, true, s);  // right-hand side of the first iteration
   if (!d.exec) {
     OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {

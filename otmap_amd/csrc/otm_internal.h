@@ -166,14 +166,16 @@ constexpr int MIXED_TN[NMIXED] = {32, 64, 128, 128};
 constexpr int MIXED_TM[NMIXED] = {64, 64, 128, 128};
 
 inline int mres_tiles(int n) { return (n / 32) * (n / 64); }
+inline int upair_tiles(int n) { return tiles_i(n) * ((tiles_j(n) + 1) / 2); }
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
 // mixed_kind(s, c): stage s of the mixed transform with tile width MIXED_TN[c]
 enum LaunchKind {
   KIND_STENCIL = 0,
   KIND_FLAT = 1,
   KIND_PLAIN64 = 2,
-  KIND_MRES = 3,  // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
-  KIND_M0 = 4,
+  KIND_MRES = 3,   // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
+  KIND_UPAIR = 4,  // update pass of the mixed form: one workgroup per stencil tile and its column mirror
+  KIND_M0 = 5,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
@@ -201,7 +203,9 @@ void launch_start(const JobSet& js, double max_ratio, int max_iter, double thres
 // |d_hat|^2 partials (the last stage of the Laplacian solve)
 void launch_beta(const JobSet& js, int dkind, hipStream_t s);
 void launch_dir_coef(const JobSet& js, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
-void launch_update(const JobSet& js, hipStream_t s);
+// fold = true (mixed form, every n % 64 == 0): the update pass also writes
+// the folded BF16 planes of the next right-hand side
+void launch_update(const JobSet& js, hipStream_t s, bool fold = false);
 void launch_finish(const JobSet& js, hipStream_t s);     // forward positions, adjusted density copy
 void launch_count_open(const JobSet& js, int* d_open, hipStream_t s);  // jobs whose state is not done
 void launch_fill(double* p, long count, double v, hipStream_t s);
@@ -283,7 +287,8 @@ void mixed_plan_create(MixedPlan& mp, const DctPlan& p);
 void mixed_plan_destroy(MixedPlan& mp);
 void fields_enable_mixed(Fields& f);  // allocates the mixed buffers (driver.hip)
 // d_hat = M^+ (r - S e0) of every job of the table (all with mixed buffers and plans)
-void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s);
+void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s, bool prep = true);
+void launch_mprep(const JobSet& js, bool gate, hipStream_t s);  // folded planes of r' (first iteration)
 void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s);
 // times every valid tile width of each mixed stage (cached like dct_tune) and
 // stores the fastest in js.mcfg

@@ -23,6 +23,7 @@
 
 #include "otm_internal.h"
 #include "reduce.h"
+#include "split.h"
 
 namespace otm {
 namespace {
@@ -611,28 +612,21 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
   }
 }
 
-// line-search update: psi += alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
-// (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta.  The
+// line-search update of the tile at (i0, j0): psi += alpha d, g0 += alpha
+// grad(d), r <- a alpha^2 + b alpha + r (otsolver_2dgrid.cpp:599-604) and
+// Jd = r_{k+1} - r_k for the next beta; k = completed iterations - 1.  The
 // quartic coefficients a, b come from d and G0 before its update, as in the
-// line-search pass.
-__global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
-  const Loc L = locate(starts, count);
-  const Fields& f = jobs[L.p].f;
-  const auto st = gp((const SolveState*)f.state);
-  if (!st->update) return;
+// line-search pass.  rn: the new residual of this thread's faces (0 outside).
+__device__ inline void update_tile(const Fields& f, int k, double alpha, int i0, int j0, double (&rn)[FR]) {
   const int n = f.n;
   __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
   __shared__ double s[NF];
-  const int k = st->it - 1;
   const auto d = gp(f.d[(k + 1) & 1]);
   // G0 ping-pong: neighbouring tiles read the border vertices this tile
   // updates, so the update goes to the other buffer
   const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
   const auto g0n = gp(f.g0 + ((k + 1) & 1) * 2 * f.V);
   const auto psi = gp(f.psi), r = gp(f.r), jd = gp(f.jd);
-  const double alpha = st->alpha;
-  int i0, j0;
-  tile_origin(L.t, n, i0, j0);
   // all global loads of the tile first: direction halo, G0 of the tile's
   // vertices, psi / r of the faces (the stores below overwrite them)
   double hv[HR];
@@ -692,15 +686,88 @@ __global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
     const int t = threadIdx.x + u * NT;
     const int fa = t / TJ, fb = t % TJ;
     const int fi = i0 + fa, fj = j0 + fb;
+    rn[u] = 0.;
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
       psi[fc] = psv[u] + alpha * s[(fa + 1) * LDF + fb + 1];
       double av, bv;
       quartic_ab(gx, gy, hx, hy, fa, fb, e, av, bv);
       const double rk = rkv[u];
-      const double rn = av * (alpha * alpha) + bv * alpha + rk;
-      r[fc] = rn;
-      jd[fc] = rn - rk;
+      const double rnv = av * (alpha * alpha) + bv * alpha + rk;
+      r[fc] = rnv;
+      jd[fc] = rnv - rk;
+      rn[u] = rnv;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Fields& f = jobs[L.p].f;
+  const auto st = gp((const SolveState*)f.state);
+  if (!st->update) return;
+  int i0, j0;
+  tile_origin(L.t, f.n, i0, j0);
+  double rn[FR];
+  update_tile(f, st->it - 1, st->alpha, i0, j0, rn);
+}
+
+// The update pass of the mixed form (n % 64 == 0): one workgroup updates a
+// tile and its mirror across column n/2 (tile columns c and T-1-c, T = n/TJ;
+// the middle tile column alone when T is odd), then writes the next Laplacian
+// solve's right-hand side: the columns of r' = r - (sum r) e0 folded (U =
+// r'(:, j) + r'(:, n-1-j), W = difference) and split into the six BF16 planes
+// (blocked layout), the same values the separate fold kernel writes from r.
+__global__ __launch_bounds__(NT) void k_update_fold(JOB_ARGS) {
+  const Loc L = locate(starts, count);
+  const Job& J = jobs[L.p];
+  const Fields& f = J.f;
+  const auto st = gp((const SolveState*)f.state);
+  if (!st->update) return;
+  const int n = f.n, h = n / 2, T = n / TJ, P = (T + 1) / 2;
+  const int i0 = (L.t / P) * TI, cl = L.t % P, cr = T - 1 - cl;
+  const int k = st->it - 1;
+  const double alpha = st->alpha;
+  __shared__ double rl[TI * TJ], rr[TI * TJ];
+  double rn[FR];
+  update_tile(f, k, alpha, i0, cl * TJ, rn);
+#pragma unroll
+  for (int u = 0; u < FR; ++u) rl[threadIdx.x + u * NT] = rn[u];
+  if (cr != cl) {
+    __syncthreads();  // the tile's LDS is reused
+    update_tile(f, k, alpha, i0, cr * TJ, rn);
+#pragma unroll
+    for (int u = 0; u < FR; ++u) rr[threadIdx.x + u * NT] = rn[u];
+  }
+  __syncthreads();
+  // fold: face (a, b) of the left tile, column j = cl*TJ + b < h, pairs with
+  // column n-1-j: face (a, TJ-1-b) of the right tile (of the same tile when
+  // it is the middle one); four consecutive columns per thread
+  const double* mir = cr != cl ? rr : rl;
+  const int cols = cr != cl ? TJ : TJ / 2;
+  const int t = threadIdx.x, a = t / (TJ / 4), b0 = (t % (TJ / 4)) * 4;
+  if (a < TI && b0 < cols) {
+    const double S = st->sum_r;
+    const long nh = long(n) * h;
+    const auto out = gp(J.f.mrhs);
+    const int i = i0 + a, j = cl * TJ + b0;
+    u16 uu[3][4], ww[3][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double va = rl[a * TJ + b0 + e];
+      if (i == 0 && j + e == 0) va -= S;
+      const double vb = mir[a * TJ + TJ - 1 - b0 - e];
+      split3d(va + vb, uu[0][e], uu[1][e], uu[2][e]);
+      split3d(va - vb, ww[0][e], ww[1][e], ww[2][e]);
+    }
+    const long q = bofs(i, j, h);
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      u32x2 pu, pw;
+      pu.x = uu[pl][0] | (unsigned(uu[pl][1]) << 16); pu.y = uu[pl][2] | (unsigned(uu[pl][3]) << 16);
+      pw.x = ww[pl][0] | (unsigned(ww[pl][1]) << 16); pw.y = ww[pl][2] | (unsigned(ww[pl][3]) << 16);
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + pl * nh + q) = pu;
+      *reinterpret_cast<OTM_GLOBAL u32x2*>(out + (3 + pl) * nh + q) = pw;
     }
   }
 }
@@ -905,8 +972,11 @@ void launch_dir_coef(const JobSet& js, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
-void launch_update(const JobSet& js, hipStream_t s) {
-  hipLaunchKernelGGL(k_update, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
+void launch_update(const JobSet& js, hipStream_t s, bool fold) {
+  if (fold)
+    hipLaunchKernelGGL(k_update_fold, grid_of(js, KIND_UPAIR), dim3(NT), 0, s, JOB_PASS(KIND_UPAIR));
+  else
+    hipLaunchKernelGGL(k_update, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
 }
 
@@ -943,6 +1013,7 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs) {
       else if (kind == KIND_FLAT) w = flat_blocks(jobs[p].f.F);
       else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
       else if (kind == KIND_MRES) w = mixed_supported(n) ? mres_tiles(n) : 0;
+      else if (kind == KIND_UPAIR) w = mixed_supported(n) ? upair_tiles(n) : 0;
       else if (kind < KIND_GEMM0) {
         const int g = kind - KIND_M0, stage = g / NMIXED, c = g % NMIXED;
         w = mixed_cfg_valid(stage, n, c) ? mixed_tiles(stage, n, c) : 0;
