@@ -22,10 +22,15 @@ template <int K, int NTH = NT>
 __device__ static inline void block_sum(double (&v)[K]) {
   constexpr int NW = NTH / 64;
   __shared__ double red[K][NW];
-  for (int k = 0; k < K; ++k) {
-    double x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    v[k] = x;
+  // one shuffle step for all K values at a time: K independent exchanges in
+  // flight per step instead of K dependent chains of six (same sums)
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double y[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) y[k] = __shfl_down(v[k], off);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += y[k];
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();

@@ -493,8 +493,16 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const auto st = gp((const SolveState*)f.state);
   if (st->done) return;
   const int n = f.n;
-  __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
-  __shared__ double s[NF];
+  // LDS: direction gradients gx, gy and cached G0 (hx, hy); the direction
+  // halo s lives where hx, hy go once the gradients are formed (18.7 KB per
+  // workgroup: eight resident per CU, every tile of a 1024^2 grid at once)
+  __shared__ double lds[4 * NV];
+  static_assert(NF <= 2 * NV, "the halo fits in the G0 region");
+  double* const gx = lds;
+  double* const gy = lds + NV;
+  double* const hx = lds + 2 * NV;
+  double* const hy = lds + 3 * NV;
+  double* const s = hx;
   const int k = st->it;
   const auto dhat = gp(f.dhat);
   const auto dprev = gp(f.d[k & 1]);
@@ -532,6 +540,15 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
     rvv[u] = (fi < n && fj < n) ? r[long(fi) * n + fj] : 0.;
   }
   halo_store(s, hv);
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  double dv[FR];  // direction at the tile's own faces
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    dv[u] = s[(t / TJ + 1) * LDF + t % TJ + 1];
+  }
+  __syncthreads();  // the halo is dead: G0 takes its place
 #pragma unroll
   for (int u = 0; u < VR; ++u) {
     const int t = threadIdx.x + u * NT;
@@ -540,8 +557,6 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
       hy[t] = hyv[u];
     }
   }
-  __syncthreads();
-  tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
   const double e = 1. / double(n);
   double acc[7] = {0., 0., 0., 0., 0., 0., 0.};
@@ -552,7 +567,7 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
-      dnew[fc] = s[(a + 1) * LDF + b + 1];
+      dnew[fc] = dv[u];
       double av, bv;
       quartic_ab(gx, gy, hx, hy, a, b, e, av, bv);
       const double rv = rvv[u];
@@ -619,8 +634,13 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
 // line-search pass.  rn: the new residual of this thread's faces (0 outside).
 __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0, int j0, double (&rn)[FR]) {
   const int n = f.n;
-  __shared__ double gx[NV], gy[NV], hx[NV], hy[NV];
-  __shared__ double s[NF];
+  // LDS as in the line-search pass: gx, gy, hx, hy, the halo s in hx, hy's place
+  __shared__ double lds[4 * NV];
+  double* const gx = lds;
+  double* const gy = lds + NV;
+  double* const hx = lds + 2 * NV;
+  double* const hy = lds + 3 * NV;
+  double* const s = hx;
   const auto d = gp(f.d[(k + 1) & 1]);
   // G0 ping-pong: neighbouring tiles read the border vertices this tile
   // updates, so the update goes to the other buffer
@@ -661,6 +681,15 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
     rkv[u] = in ? r[fc] : 0.;
   }
   halo_store(s, hv);
+  __syncthreads();
+  tile_grads(s, gx, gy, n, i0, j0);
+  double dv[FR];  // direction at the tile's own faces
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    dv[u] = s[(t / TJ + 1) * LDF + t % TJ + 1];
+  }
+  __syncthreads();  // the halo is dead: G0 takes its place
 #pragma unroll
   for (int u = 0; u < VR; ++u) {
     const int t = threadIdx.x + u * NT;
@@ -669,8 +698,6 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
       hy[t] = hyv[u];
     }
   }
-  __syncthreads();
-  tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < VR; ++u) {
@@ -689,7 +716,7 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
     rn[u] = 0.;
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
-      psi[fc] = psv[u] + alpha * s[(fa + 1) * LDF + fb + 1];
+      psi[fc] = psv[u] + alpha * dv[u];
       double av, bv;
       quartic_ab(gx, gy, hx, hy, fa, fb, e, av, bv);
       const double rk = rkv[u];
@@ -701,7 +728,8 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
   }
 }
 
-__global__ __launch_bounds__(NT) void k_update(JOB_ARGS) {
+// eight waves per SIMD: every tile of a 1024^2 grid resident at once
+__global__ __launch_bounds__(NT, 8) void k_update(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
   const auto st = gp((const SolveState*)f.state);
