@@ -40,10 +40,10 @@ PEAK_HBM_GBS = 8000.0
 # the runs and their commands are in profiles/gemm_1024_hbm_traffic.md.
 TRAFFIC_SOURCE = "profiles/gemm_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
-    "k_mgemm<M2,32>": (9282.5 * 2.0 + 6144.0) * 1024,
-    "k_mgemm<M2,64>": (8578.9 * 2.0 + 5673.6) * 1024,
+    "k_mgemm<M2,32>": (9288.0 * 2.0 + 6144.0) * 1024,
+    "k_mgemm<M2,64>": (9294.0 * 2.0 + 6144.0) * 1024,
     **{f"k_gemm_fold<F1,cfg{c}>": (f * 2.0 + 8192.0) * 1024
-       for c, f in enumerate([20521.6, 20517.0, 20517.0, 20513.0, 20537.5, 20522.0, 20526.5, 20564.3])},
+       for c, f in enumerate([20521.6, 20517.0, 20517.0, 20513.0, 20537.5, 20522.0, 20526.5, 20564.3, 20520.0])},
 }
 
 

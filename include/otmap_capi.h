@@ -129,8 +129,8 @@ int otm_solver_set_gemm_config(otm_solver* s, int stage, int config);
 
 /* Tile configurations of the four mixed-precision DCT stages (no reference
  * counterpart): config 0 / 1 use workgroup tiles of 64 x 32 / 64 x 64
- * outputs, config 2 tiles of 128 x 128 with operands loaded straight into
- * LDS (valid when n % 256 == 0).  The first
+ * outputs, config 2 / 3 tiles of 128 x 128 computed by four / eight waves
+ * (valid when n % 256 == 0).  The first
  * mixed solve times each valid configuration per stage and keeps the
  * fastest; every configuration gives bit-identical results.
  *   otm_solver_mixed_config: cfg[0..3] = chosen width per stage, returns the

@@ -223,8 +223,9 @@ __global__ __launch_bounds__(NT) void k_mresid(const Job* __restrict__ jobs, con
 // of A (M1: 32 rows and their 32 mirrors) by TN rows of B (TN = 32 or 64,
 // picked per stage by the autotuner), K steps of 32.  The workgroup is two
 // K groups of four waves (32 x TN/2 each): group 0 accumulates the first
-// ceil(S/2) of the S = K/32 steps, group 1 the rest, each through its own LDS
-// buffers, both in lockstep on one barrier per step.  For M3/M4 the split is
+// ceil(S/2) of the S = K/32 steps, group 1 the rest, each through its own
+// three LDS buffers filled straight from global memory (global_load_lds, two
+// K steps ahead), both in lockstep on one barrier per step.  For M3/M4 the split is
 // exactly k < h / k >= h, the two halves the fold needs; for M1/M2 the
 // epilogue adds the two FP32 partials.  The order of every sum depends on n
 // only, so all tile widths give bit-identical results.
@@ -520,9 +521,10 @@ __global__ __launch_bounds__(MNT) void k_mgemm(const Job* __restrict__ jobs, con
 }
 
 // ---------------------------------------------------------------------------
-// Wide form of a stage for large launches (config 2): 128 x 128 workgroup
-// tiles (M1: 64 rows and their 64 mirrors), four waves of 64 x 64, so a wave
-// issues 96 MFMAs per 24 LDS fragment reads.  Operand tiles go from global
+// Wide form of a stage for large launches (configs 2 and 3): 128 x 128
+// workgroup tiles (M1: 64 rows and their 64 mirrors), four waves of 64 x 64
+// (96 MFMAs per 24 LDS fragment reads) or eight waves of 64 x 32 (two waves
+// per SIMD, 48 MFMAs per 18 fragment reads).  Operand tiles go from global
 // memory straight into LDS (global_load_lds_dwordx4: no register staging, no
 // LDS store instructions), three K-step buffers deep; every transfer and every
 // fragment read is one contiguous 1 KB block.  The K steps of the two
