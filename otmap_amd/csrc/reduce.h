@@ -16,25 +16,46 @@ namespace otm {
 __device__ static inline int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
 __device__ static inline int block_count() { return gridDim.x * gridDim.y; }
 
+// v moved across lanes by one DPP pattern (CTRL) in the rows of ROWS; lanes
+// without a source (and rows outside ROWS) get 0.  Two 32-bit moves.
+template <int CTRL, int ROWS>
+__device__ static inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Wave sums of K values on the VALU (no LDS traffic): row shifts by 1, 2, 4,
+// 8 leave each 16-lane row's sum in its lane 15, the row broadcasts 15 and 31
+// add rows 0 + 1 into lane 31 and rows 0 + 1 + 2 + 3 into lane 63.  Lane 63
+// holds the totals.  Fixed order; every step moves all K values at once.
+template <int K>
+__device__ static inline void wave_sum(double (&v)[K]) {
+#define OTM_DPP_STEP(CTRL, ROWS)                                  \
+  {                                                               \
+    double y[K];                                                  \
+    _Pragma("unroll") for (int k = 0; k < K; ++k) y[k] = dpp_move<CTRL, ROWS>(v[k]); \
+    _Pragma("unroll") for (int k = 0; k < K; ++k) v[k] += y[k];   \
+  }
+  OTM_DPP_STEP(0x111, 0xf)  // row_shr:1
+  OTM_DPP_STEP(0x112, 0xf)  // row_shr:2
+  OTM_DPP_STEP(0x114, 0xf)  // row_shr:4
+  OTM_DPP_STEP(0x118, 0xf)  // row_shr:8
+  OTM_DPP_STEP(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+  OTM_DPP_STEP(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
+#undef OTM_DPP_STEP
+}
+
 // In-block sum of K values over NTH threads (a multiple of 64); the result
 // is valid in thread 0.  Fixed order.
 template <int K, int NTH = NT>
 __device__ static inline void block_sum(double (&v)[K]) {
   constexpr int NW = NTH / 64;
   __shared__ double red[K][NW];
-  // one shuffle step for all K values at a time: K independent exchanges in
-  // flight per step instead of K dependent chains of six (same sums)
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    double y[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) y[k] = __shfl_down(v[k], off);
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += y[k];
-  }
+  wave_sum<K>(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
-  if (lane == 0)
+  if (lane == 63)
     for (int k = 0; k < K; ++k) red[k][w] = v[k];
   __syncthreads();
   if (threadIdx.x == 0)

@@ -403,9 +403,20 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
   const double eps = st->alpha / 2.;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
-  // all global loads of the tile first: halo, then the per-face inputs
-  double hv[HR];
-  halo_load(hv, n, i0, j0, [&](long fc) { return psi[fc] - eps * dhat[fc]; });
+  // all global loads of the tile first: halo (psi and d_hat kept apart,
+  // combined at the LDS store), then the per-face inputs
+  double pa[HR], pb[HR];
+  bool hin[HR];
+#pragma unroll
+  for (int u = 0; u < HR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int a = t / LDF, b = t - a * LDF;
+    const int fi = i0 - 1 + a, fj = j0 - 1 + b;
+    hin[u] = t < NF && fi >= 0 && fi < n && fj >= 0 && fj < n;
+    const long fc = long(fi) * n + fj;
+    pa[u] = hin[u] ? psi[fc] : 0.;
+    pb[u] = hin[u] ? dhat[fc] : 0.;
+  }
   double rhov[FR], jdv[FR], rv[FR];
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
@@ -417,7 +428,11 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
     jdv[u] = in ? jd[fc] : 0.;
     rv[u] = in ? r[fc] : 0.;
   }
-  halo_store(s, hv);
+#pragma unroll
+  for (int u = 0; u < HR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (t < NF) s[t] = hin[u] ? pa[u] - eps * pb[u] : 0.;
+  }
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
@@ -513,13 +528,22 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const double beta = st->beta;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
-  // all global loads of the tile first: direction halo, cached G0 of the
-  // tile's vertices, residual of its faces
-  double hv[HR];
-  if (use_prev)
-    halo_load(hv, n, i0, j0, [&](long fc) { return dhat[fc] + beta * dprev[fc]; });
-  else
-    halo_load(hv, n, i0, j0, [&](long fc) { return dhat[fc]; });
+  // all global loads of the tile first: direction halo (d_hat and d_prev
+  // kept apart, combined at the LDS store, so no arithmetic waits on a load
+  // before the others are issued), cached G0 of the tile's vertices,
+  // residual of its faces
+  double da[HR], db[HR];
+  bool hin[HR];
+#pragma unroll
+  for (int u = 0; u < HR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int a = t / LDF, b = t - a * LDF;
+    const int fi = i0 - 1 + a, fj = j0 - 1 + b;
+    hin[u] = t < NF && fi >= 0 && fi < n && fj >= 0 && fj < n;
+    const long fc = long(fi) * n + fj;
+    da[u] = hin[u] ? dhat[fc] : 0.;
+    db[u] = (hin[u] && use_prev) ? dprev[fc] : 0.;
+  }
   const long V1 = n + 1;
   const long V = V1 * V1;
   double hxv[VR], hyv[VR];
@@ -539,7 +563,11 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
     const int fi = i0 + t / TJ, fj = j0 + t % TJ;
     rvv[u] = (fi < n && fj < n) ? r[long(fi) * n + fj] : 0.;
   }
-  halo_store(s, hv);
+#pragma unroll
+  for (int u = 0; u < HR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (t < NF) s[t] = !hin[u] ? 0. : use_prev ? da[u] + beta * db[u] : da[u];
+  }
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   double dv[FR];  // direction at the tile's own faces
