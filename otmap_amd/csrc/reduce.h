@@ -114,11 +114,9 @@ __device__ static inline int job_of(const int* __restrict__ starts, int count, i
 // adding (one latency per batch, not one per value).
 constexpr int FOLD_BATCH = 8;
 
+// this thread's share of the fold (added to s; block_sum completes it)
 template <int K>
-__device__ static inline void fold_partials(double (&v)[K], const double* partials, int nb) {
-  double s[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) s[k] = 0.;
+__device__ static inline void fold_share(double (&s)[K], const double* partials, int nb) {
   for (int base = 0; base < nb; base += NT * FOLD_BATCH) {
     double tmp[K][FOLD_BATCH];
 #pragma unroll
@@ -133,8 +131,13 @@ __device__ static inline void fold_partials(double (&v)[K], const double* partia
 #pragma unroll
       for (int u = 0; u < FOLD_BATCH; ++u) s[k] += tmp[k][u];
   }
+}
+
+template <int K>
+__device__ static inline void fold_partials(double (&v)[K], const double* partials, int nb) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = s[k];
+  for (int k = 0; k < K; ++k) v[k] = 0.;
+  fold_share<K>(v, partials, nb);
   block_sum<K>(v);
 }
 

@@ -461,40 +461,39 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restr
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   const auto st = gp(f.state);
-  __shared__ int stop;
-  if (threadIdx.x == 0) {
-    st->update = 0;  // the previous iteration's update has run
-    stop = st->done;
+  // the state is read together with the partials (one memory round trip);
+  // partials that no pass wrote this iteration are folded and discarded
+  const int done = st->done, it = st->it, cj = st->cj;
+  const double alpha = st->alpha;
+  double v[3];  // |d_hat|^2, Jd.(r_eps - r_k), Jd.Jd: all loads issued before the sum
+  {
+    double dn[1] = {0.}, bd[2] = {0., 0.};
+    fold_share<1>(dn, f.site(SITE_DNORM), dstarts[p + 1] - dstarts[p]);
+    fold_share<2>(bd, f.site(SITE_BETA), starts[p + 1] - starts[p]);
+    v[0] = dn[0];
+    v[1] = bd[0];
+    v[2] = bd[1];
   }
-  __syncthreads();
-  if (stop) return;
-  double dn[1];
-  fold_partials<1>(dn, f.site(SITE_DNORM), dstarts[p + 1] - dstarts[p]);
-  __shared__ int early;
-  if (threadIdx.x == 0) {
-    const double dnorm = sqrt(dn[0]);
-    st->dnorm = dnorm;
-    early = dnorm <= 2 * DBL_MIN;
-    if (early) {
-      st->done = 1;
-      st->exit_reason = 2;
-    }
-  }
-  __syncthreads();
-  if (early) return;
-  const bool conj = st->it >= 1 && st->cj;
-  if (!conj) {
-    if (threadIdx.x == 0) st->use_prev = 0;  // beta keeps its value (printed as is)
+  block_sum<3>(v);
+  if (threadIdx.x != 0) return;
+  st->update = 0;  // the previous iteration's update has run
+  if (done) return;
+  const double dnorm = sqrt(v[0]);
+  st->dnorm = dnorm;
+  if (dnorm <= 2 * DBL_MIN) {
+    st->done = 1;
+    st->exit_reason = 2;
     return;
   }
-  double acc[2];
-  fold_partials<2>(acc, f.site(SITE_BETA), starts[p + 1] - starts[p]);
-  if (threadIdx.x == 0) {
-    const double eps = st->alpha / 2.;
-    const double cand = acc[0] / acc[1] / eps * st->alpha;
-    st->beta = (-1. < cand) ? cand : -1.;  // std::max(-1., cand)
-    st->use_prev = 1;
+  const bool conj = it >= 1 && cj;
+  if (!conj) {
+    st->use_prev = 0;  // beta keeps its value (printed as is)
+    return;
   }
+  const double eps = alpha / 2.;
+  const double cand = v[1] / v[2] / eps * alpha;
+  st->beta = (-1. < cand) ? cand : -1.;  // std::max(-1., cand)
+  st->use_prev = 1;
 }
 
 // direction d = d_hat (+ beta d_prev), vertex gradients of d, quartic
@@ -618,37 +617,37 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   const auto st = gp(f.state);
-  __shared__ int stop;
-  if (threadIdx.x == 0) stop = st->done;
-  __syncthreads();
-  if (stop) return;
+  // the state is read together with the partials (one memory round trip);
+  // a finished job folds them too and discards the result
+  const int done = st->done, k = st->it, max_iter = st->max_iter;
+  const double e_prev = st->e, sum_r = st->sum_r, beta = st->beta, threshold = st->threshold;
   double acc[7];
   fold_partials<7>(acc, f.site(SITE_COEF), starts[p + 1] - starts[p]);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !done) {
     const double A = 1.0 / (double(f.n) * double(f.n));
     double z[5];
-    z[0] = st->e * A;
+    z[0] = e_prev * A;
     z[1] = 2. * acc[0];
     z[2] = acc[2] + 2 * acc[1];
     z[3] = 2. * acc[3];
     z[4] = acc[4];
     double emin;
     const double alpha = pick_step(z, emin);
+    const double e = emin / A;
     for (int q = 0; q < 5; ++q) st->z[q] = z[q];
     st->alpha = alpha;
-    st->e = emin / A;
-    st->sum_r = acc[5] * (alpha * alpha) + acc[6] * alpha + st->sum_r;
-    const int k = st->it;
+    st->e = e;
+    st->sum_r = acc[5] * (alpha * alpha) + acc[6] * alpha + sum_r;
     if (k < f.trace_cap) {
       const auto tr = gp(f.trace);
-      tr[k] = st->e;
+      tr[k] = e;
       tr[f.trace_cap + k] = alpha;
-      tr[2 * f.trace_cap + k] = st->beta;
+      tr[2 * f.trace_cap + k] = beta;
     }
     const int it = k + 1;
     st->it = it;
     st->update = 1;
-    if (!(it < st->max_iter && st->e > st->threshold)) {
+    if (!(it < max_iter && e > threshold)) {
       st->done = 1;
       st->exit_reason = 1;
     }
