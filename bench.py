@@ -203,11 +203,13 @@ def batched_throughput(count, steps, warmup):
             "sequential_single_solves_per_s": round(count / t_seq, 2),
             "speedup_vs_sequential": round(t_seq / t_batch, 3),
             "iterations": {"min": min(iters), "max": max(iters), "mean": round(float(np.mean(iters)), 1)},
-            "create_s": round(t_create, 3), "precision": "fp64", "gemm_tiles": b.gemm_config()[0],
+            "create_s": round(t_create, 3), "precision": "fp64",
+            "gemm_tiles_by_size": {str(n): b.gemm_config(sizes.index(n))[0] for n in sorted(set(sizes))},
             "mixed": {"solves_per_s": round(count / t_mixed, 2), "ms_per_batch": round(t_mixed * 1e3, 2),
                       "iterations_equal_fp64": iters_mixed == iters,
                       "max_vertex_diff_vs_fp64": float((fwd_mixed - fwd_fp64).abs().max().item()),
-                      "speedup_vs_fp64_batch": round(t_batch / t_mixed, 3), "mixed_tiles": b.mixed_config()[0]},
+                      "speedup_vs_fp64_batch": round(t_batch / t_mixed, 3),
+                      "mixed_tiles_by_size": {str(n): b.mixed_config(sizes.index(n))[0] for n in sorted(set(sizes))}},
             "mixed_refine2": {"solves_per_s": round(count / t_mixed2, 2), "ms_per_batch": round(t_mixed2 * 1e3, 2),
                               "iterations_equal_fp64": iters_mixed2 == iters,
                               "max_vertex_diff_vs_fp64": float((fwd_mixed2 - fwd_fp64).abs().max().item())}}

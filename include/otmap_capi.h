@@ -260,12 +260,15 @@ int otm_batch_solve_device(otm_batch* b, const double* const* d_density, const o
                            double* const* d_fwd_xy, double* const* d_adjusted_density, otm_stats* stats);
 /* per-iteration residual / alpha / beta of grid `job` in the last solve */
 int otm_batch_trace(const otm_batch* b, int job, double* residual, double* alpha, double* beta, int cap);
-/* tile shape per folded DCT stage chosen for this batch; returns the number of candidates */
-int otm_batch_gemm_config(const otm_batch* b, int* cfg);
-/* tile configuration per mixed DCT stage (see otm_solver_mixed_config) picked
- * on the first mixed solve of the batch; returns the number of candidates (0
- * before it) */
-int otm_batch_mixed_config(const otm_batch* b, int* cfg);
+/* Tile shapes are tuned per grid size: the jobs of each size run the shape
+ * the autotuner picked for that size (one launch per shape in use per stage).
+ * otm_batch_gemm_config: folded FP64 tile shape per stage of grid `job` (-1:
+ * of the largest grid); returns the number of candidates. */
+int otm_batch_gemm_config(const otm_batch* b, int job, int* cfg);
+/* tile configuration per mixed DCT stage (see otm_solver_mixed_config) of grid
+ * `job` (-1: of the largest grid), picked on the first mixed solve of the
+ * batch; returns the number of candidates (0 before it) */
+int otm_batch_mixed_config(const otm_batch* b, int job, int* cfg);
 
 #ifdef __cplusplus
 }

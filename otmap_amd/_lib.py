@@ -85,8 +85,8 @@ _SIGNATURES = {
     "otm_batch_solve_device": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(OtmOptions), C.POINTER(_vp),
                                          C.POINTER(_vp), C.POINTER(OtmStats)]),
     "otm_batch_trace": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
-    "otm_batch_gemm_config": (C.c_int, [_vp, _ip]),
-    "otm_batch_mixed_config": (C.c_int, [_vp, _ip]),
+    "otm_batch_gemm_config": (C.c_int, [_vp, C.c_int, _ip]),
+    "otm_batch_mixed_config": (C.c_int, [_vp, C.c_int, _ip]),
 }
 
 _lib = None

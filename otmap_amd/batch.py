@@ -85,18 +85,21 @@ class BatchSolver:
             check(k, "batch trace")
         return r[:k], a[:k], b[:k]
 
-    def gemm_config(self):
+    def gemm_config(self, job: int = -1):
+        """(folded FP64 tile shape per DCT stage of grid `job` (-1: the
+        largest grid; shapes are tuned per grid size), number of candidates)."""
         cfg = np.zeros(4, dtype=np.int32)
-        count = _lib.lib().otm_batch_gemm_config(self._h, iptr(cfg))
+        count = _lib.lib().otm_batch_gemm_config(self._h, int(job), iptr(cfg))
         if count < 0:
             check(count, "batch gemm_config")
         return [int(c) for c in cfg], count
 
-    def mixed_config(self):
-        """(tile width config per mixed DCT stage, number of candidates); the
-        count is 0 before the first mixed solve."""
+    def mixed_config(self, job: int = -1):
+        """(tile configuration per mixed DCT stage of grid `job` (-1: the
+        largest grid), number of candidates); the count is 0 before the first
+        mixed solve."""
         cfg = np.zeros(4, dtype=np.int32)
-        count = _lib.lib().otm_batch_mixed_config(self._h, iptr(cfg))
+        count = _lib.lib().otm_batch_mixed_config(self._h, int(job), iptr(cfg))
         if count < 0:
             check(count, "batch mixed_config")
         return [int(c) for c in cfg], count

@@ -3,7 +3,8 @@
 // (otlib/otsolver_2dgrid.cpp:106-245); here every grid of the batch is one job
 // of a job table, so each step of a Newton iteration is one launch for the
 // whole batch, the iterations are captured in a HIP graph, and the DCT tile
-// shapes are picked by the autotuner for the batch's set of sizes.  Each job
+// shapes are picked by the autotuner per grid size (each size class runs its
+// own tile configuration, one launch per configuration in use).  Each job
 // runs exactly the single-grid computation (same kernels, same operation
 // order), so a batched result is bit-identical to the same density solved
 // alone; jobs that converge early skip the remaining iterations.  Both
@@ -69,17 +70,16 @@ void build_table(otm_batch* b) {
       jobs[p].ck = it->second.ck;
     }
   }
-  int cfg[4], mcfg[4];
+  // the tile assignment survives a rebuild (same jobs, same order)
   const bool had = b->js.jobs != nullptr;
-  if (had) {
-    std::memcpy(cfg, b->js.cfg, sizeof(cfg));
-    std::memcpy(mcfg, b->js.mcfg, sizeof(mcfg));
-  }
+  const auto jcfg = b->js.jcfg;
+  const auto jmcfg = b->js.jmcfg;
   jobset_destroy(b->js);
   jobset_create(b->js, jobs);
   if (had) {
-    std::memcpy(b->js.cfg, cfg, sizeof(cfg));
-    std::memcpy(b->js.mcfg, mcfg, sizeof(mcfg));
+    b->js.jcfg = jcfg;
+    b->js.jmcfg = jmcfg;
+    jobset_assign(b->js);
   }
 }
 
@@ -198,15 +198,17 @@ int otm_batch_destroy(otm_batch* b) {
 
 int otm_batch_size(const otm_batch* b) { return b ? b->count : 0; }
 
-int otm_batch_gemm_config(const otm_batch* b, int* cfg) {
+int otm_batch_gemm_config(const otm_batch* b, int job, int* cfg) {
   if (!b || !cfg) { set_error("null argument"); return OTM_ERR_ARG; }
-  for (int st = 0; st < 4; ++st) cfg[st] = b->js.cfg[st];
+  if (job < -1 || job >= b->count) { set_error("job index out of range"); return OTM_ERR_ARG; }
+  for (int st = 0; st < 4; ++st) cfg[st] = job < 0 ? b->js.cfg[st] : b->js.jcfg[job][st];
   return NFOLD;
 }
 
-int otm_batch_mixed_config(const otm_batch* b, int* cfg) {
+int otm_batch_mixed_config(const otm_batch* b, int job, int* cfg) {
   if (!b || !cfg) { set_error("null argument"); return OTM_ERR_ARG; }
-  for (int st = 0; st < 4; ++st) cfg[st] = b->js.mcfg[st];
+  if (job < -1 || job >= b->count) { set_error("job index out of range"); return OTM_ERR_ARG; }
+  for (int st = 0; st < 4; ++st) cfg[st] = job < 0 ? b->js.mcfg[st] : b->js.jmcfg[job][st];
   return b->mixed_tuned ? NMIXED : 0;
 }
 

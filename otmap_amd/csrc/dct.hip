@@ -465,46 +465,54 @@ void dct_plan_destroy(DctPlan& p) {
   p = DctPlan();
 }
 
-// Autotuner of the folded stages over a job table: every candidate shape that
-// divides n/2 of every job runs twice untimed, then three times between events
-// on stream s; the fastest wins.  Launches are ungated and work on the jobs'
-// scratch fields (values irrelevant to timing).  Results are cached per device
-// and multiset of grid sizes for the life of the process.
+// Autotuner of the folded stages, per grid size: over a table of the jobs of
+// that size, every candidate shape that divides n/2 runs twice untimed, then
+// three times between events on stream s; the fastest wins and is assigned
+// to those jobs.  Launches are ungated and work on the jobs' scratch fields
+// (values irrelevant to timing).  Results are cached per device, grid size and
+// number of jobs of that size for the life of the process.
 void dct_tune(JobSet& js, int device, hipStream_t s) {
   if (!js.fold) return;
   static std::mutex mu;
-  static std::map<std::pair<int, std::vector<int>>, std::array<int, 4>> cache;
-  std::vector<int> sizes;
-  for (const Job& J : js.host) sizes.push_back(J.f.n);
-  std::sort(sizes.begin(), sizes.end());
-  const auto key = std::make_pair(device, sizes);
-  {
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-      for (int st = 0; st < 4; ++st) js.cfg[st] = it->second[st];
-      return;
+  static std::map<std::array<int, 3>, std::array<int, 4>> cache;
+  std::map<int, std::vector<int>> by_size;
+  for (int p = 0; p < js.count; ++p) by_size[js.host[p].f.n].push_back(p);
+  for (const auto& kv : by_size) {
+    const int n = kv.first;
+    const std::array<int, 3> key = {device, n, int(kv.second.size())};
+    std::array<int, 4> best{};
+    bool hit = false;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      auto it = cache.find(key);
+      if (it != cache.end()) {
+        best = it->second;
+        hit = true;
+      }
     }
+    if (!hit) {
+      std::vector<Job> jobs;
+      for (int p : kv.second) jobs.push_back(js.host[p]);
+      JobSet t;
+      jobset_create(t, jobs, true);
+      for (int st = 0; st < 4; ++st)
+        best[st] = tune_pick(NFOLD, [&](int c) { return fold_cfg_valid(n, c); },
+                             [&](int c) { FOLD_TABLE[st][c](t, false, false, s); }, s);
+      OTM_HIP(hipStreamSynchronize(s));
+      jobset_destroy(t);
+      std::lock_guard<std::mutex> lock(mu);
+      cache[key] = best;
+    }
+    for (int p : kv.second) js.jcfg[p] = best;
   }
-  std::array<int, 4> best{};
-  for (int st = 0; st < 4; ++st) {
-    auto valid = [&](int c) {
-      bool ok = true;
-      for (int n : sizes) ok = ok && fold_cfg_valid(n, c);
-      return ok;
-    };
-    best[st] = tune_pick(NFOLD, valid, [&](int c) { FOLD_TABLE[st][c](js, false, false, s); }, s);
-    js.cfg[st] = best[st];
-  }
-  std::lock_guard<std::mutex> lock(mu);
-  cache[key] = best;
+  jobset_assign(js);
 }
 
 // one GEMM stage as the solve runs it, ungated, for kernel timing: folded F1
 // (n^3 flops) when the table folds, else the unfolded T3 = C^T Y (2 n^3)
 void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s) {
   if (js.fold) {
-    FOLD_TABLE[F1][js.cfg[F1]](js, false, false, s);
+    for (int c = 0; c < NFOLD; ++c) FOLD_TABLE[F1][c](js, false, false, s);
   } else {
     const Fields& f = js.host[0].f;
     gemm<G_PLAIN>(p->Ct, f.tmp1, f.tmp0, p->n, nullptr, p->sq, nullptr, s);
@@ -517,10 +525,9 @@ void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s) {
 // table holds one job and p is its plan.
 void launch_dct_solve(const JobSet& js, const DctPlan* p, hipStream_t s) {
   if (js.fold) {
-    FOLD_TABLE[F1][js.cfg[F1]](js, true, true, s);
-    FOLD_TABLE[F2][js.cfg[F2]](js, true, false, s);
-    FOLD_TABLE[F3][js.cfg[F3]](js, true, false, s);
-    FOLD_TABLE[F4][js.cfg[F4]](js, true, false, s);
+    // one launch per tile shape in use (shapes with no job return at once)
+    for (int st = 0; st < 4; ++st)
+      for (int c = 0; c < NFOLD; ++c) FOLD_TABLE[st][c](js, true, st == F1, s);
     return;
   }
   const Fields& f = js.host[0].f;

@@ -836,14 +836,13 @@ void launch_mstage_tn(const JobSet& js, bool gate, int fin, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
+// one launch per tile configuration in use (configurations with no job return at once)
 template <int ST>
 void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
-  switch (js.mcfg[ST]) {
-    case 0: launch_mstage_tn<ST, 0>(js, gate, fin, s); break;
-    case 1: launch_mstage_tn<ST, 1>(js, gate, fin, s); break;
-    case 2: launch_mstage_w<ST, 2>(js, gate, fin, s); break;
-    default: launch_mstage_w<ST, 3>(js, gate, fin, s); break;
-  }
+  launch_mstage_tn<ST, 0>(js, gate, fin, s);
+  launch_mstage_tn<ST, 1>(js, gate, fin, s);
+  launch_mstage_w<ST, 2>(js, gate, fin, s);
+  launch_mstage_w<ST, 3>(js, gate, fin, s);
 }
 
 void launch_mresid(const JobSet& js, bool gate, hipStream_t s) {
@@ -954,44 +953,53 @@ const std::array<MStageFn, NMIXED> MSTAGE_TABLE[4] = {mstage_row<M1>(), mstage_r
                                                        mstage_row<M4>()};
 }  // namespace
 
-// Times each valid tile width of every mixed stage over the table's jobs
-// (ungated; M4 writes only the x0 scratch) and keeps the fastest.  Results
-// are cached per device and sorted set of grid sizes.
+// Times each valid tile configuration of every mixed stage over a table of
+// the jobs of each grid size (ungated; M4 writes only the x0 scratch) and
+// assigns the fastest to those jobs.  Results are cached per device, grid
+// size and number of jobs of that size.
 void mixed_tune(JobSet& js, int device, hipStream_t s) {
   for (const Job& J : js.host)
     if (!J.Pp || !mixed_supported(J.f.n)) return;
   static std::mutex mu;
-  static std::map<std::pair<int, std::vector<int>>, std::array<int, 4>> cache;
-  std::vector<int> sizes;
-  for (const Job& J : js.host) sizes.push_back(J.f.n);
-  std::sort(sizes.begin(), sizes.end());
-  const auto key = std::make_pair(device, sizes);
-  {
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-      for (int st = 0; st < 4; ++st) js.mcfg[st] = it->second[st];
-      return;
+  static std::map<std::array<int, 3>, std::array<int, 4>> cache;
+  std::map<int, std::vector<int>> by_size;
+  for (int p = 0; p < js.count; ++p) by_size[js.host[p].f.n].push_back(p);
+  for (const auto& kv : by_size) {
+    const int n = kv.first;
+    const std::array<int, 3> key = {device, n, int(kv.second.size())};
+    std::array<int, 4> best{};
+    bool hit = false;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      auto it = cache.find(key);
+      if (it != cache.end()) {
+        best = it->second;
+        hit = true;
+      }
     }
+    if (!hit) {
+      std::vector<Job> jobs;
+      for (int p : kv.second) jobs.push_back(js.host[p]);
+      JobSet t;
+      jobset_create(t, jobs, true);
+      for (int st = 0; st < 4; ++st) {
+        const int fin = st == M4 ? FIN_X0 : 0;
+        best[st] = std::max(0, tune_pick(NMIXED, [&](int c) { return mixed_cfg_valid(st, n, c); },
+                                         [&](int c) { MSTAGE_TABLE[st][c](t, false, fin, s); }, s));
+      }
+      OTM_HIP(hipStreamSynchronize(s));
+      jobset_destroy(t);
+      std::lock_guard<std::mutex> lock(mu);
+      cache[key] = best;
+    }
+    for (int p : kv.second) js.jmcfg[p] = best;
   }
-  std::array<int, 4> best{};
-  for (int st = 0; st < 4; ++st) {
-    auto valid = [&](int c) {
-      bool ok = true;
-      for (int n : sizes) ok = ok && mixed_cfg_valid(st, n, c);
-      return ok;
-    };
-    const int fin = st == M4 ? FIN_X0 : 0;
-    best[st] = std::max(0, tune_pick(NMIXED, valid, [&](int c) { MSTAGE_TABLE[st][c](js, false, fin, s); }, s));
-    js.mcfg[st] = best[st];
-  }
-  std::lock_guard<std::mutex> lock(mu);
-  cache[key] = best;
+  jobset_assign(js);
 }
 
 // d_hat = M^+ (r - S e0) of every job: mixed transform, then `refine` FP64
 // refinement steps x += transform(r' - M0 x); the last stage leaves the
-// |d_hat|^2 partials in SITE_DNORM (launch kind mixed_kind(3, js.mcfg[3])).
+// |d_hat|^2 partials in SITE_DNORM (launch kind KIND_DNORMMX).
 // prep = false: the folded planes of r' are already in place (the update
 // pass of the previous iteration wrote them).
 void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s, bool prep) {

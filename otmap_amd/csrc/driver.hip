@@ -110,8 +110,8 @@ void driver_destroy(Driver& d) {
 
 // launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
 int dnorm_kind(const JobSet& js, bool mixed) {
-  if (mixed) return mixed_kind(3, js.mcfg[3]);
-  return js.fold ? fold_kind(3, js.cfg[3]) : KIND_PLAIN64;
+  if (mixed) return KIND_DNORMMX;
+  return js.fold ? KIND_DNORM64 : KIND_PLAIN64;
 }
 
 void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s) {
@@ -129,10 +129,7 @@ void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options&
 int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& opt, float* ms) {
   hipStream_t s = d.stream;
   // the captured iteration depends on the precision mode and the tile shapes
-  const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (long(js.cfg[0]) << 8) |
-                   (long(js.cfg[1]) << 12) | (long(js.cfg[2]) << 16) | (long(js.cfg[3]) << 20) |
-                   (long(js.mcfg[0]) << 24) | (long(js.mcfg[1]) << 28) | (long(js.mcfg[2]) << 32) |
-                   (long(js.mcfg[3]) << 36);
+  const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (js.version << 8);
   if (d.exec && d.key != key) driver_invalidate(d);
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 
 #include <cstdint>
 #include <string>
@@ -168,20 +169,30 @@ constexpr int MIXED_TM[NMIXED] = {64, 64, 128, 128};
 inline int mres_tiles(int n) { return (n / 32) * (n / 64); }
 inline int upair_tiles(int n) { return tiles_i(n) * ((tiles_j(n) + 1) / 2); }
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
-// mixed_kind(s, c): stage s of the mixed transform with tile width MIXED_TN[c]
+// fold_kind(s, c) / mixed_kind(s, c): stage s of the folded FP64 / mixed
+// transform with tile configuration c, over the jobs assigned configuration
+// c for stage s (over every job for which c is valid in an autotuner table).
 enum LaunchKind {
   KIND_STENCIL = 0,
   KIND_FLAT = 1,
   KIND_PLAIN64 = 2,
-  KIND_MRES = 3,   // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
-  KIND_UPAIR = 4,  // update pass of the mixed form: one workgroup per stencil tile and its column mirror
-  KIND_M0 = 5,
+  KIND_MRES = 3,     // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
+  KIND_UPAIR = 4,    // update pass of the mixed form: one workgroup per stencil tile and its column mirror
+  KIND_DNORM64 = 5,  // each job's workgroups in its last folded FP64 stage (writers of the |d_hat|^2 partials)
+  KIND_DNORMMX = 6,  // the same for the last mixed stage
+  KIND_M0 = 7,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
 inline int fold_kind(int stage, int c) { return KIND_GEMM0 + stage * NFOLD + c; }
 inline int mixed_kind(int stage, int c) { return KIND_M0 + stage * NMIXED + c; }
 
+// Tile configurations are assigned per job and stage (jcfg, jmcfg): jobs of
+// different grid sizes in one batch each run the configuration tuned for
+// their size, one launch per configuration in use.  Every configuration sums
+// in the same order, so the assignment never changes results.
+constexpr int FOLD_DEFAULT = 7;  // valid for every folded n
+constexpr int MIXED_DEFAULT = 0;  // valid for every mixed n
 struct JobSet {
   int count = 0;
   Job* jobs = nullptr;     // device copy of the table
@@ -189,11 +200,17 @@ struct JobSet {
   std::vector<Job> host;   // host copy
   std::vector<int> total;  // workgroups per kind
   bool fold = false;       // every job uses the folded transform
-  int cfg[4] = {7, 7, 7, 7};  // folded FP64 tile shape per stage (7: valid for every folded n; tuned at init)
-  int mcfg[4] = {0, 0, 0, 0};                                 // mixed tile width per stage
+  bool tuning = false;     // autotuner table: every valid configuration covers every job
+  std::vector<std::array<int, 4>> jcfg;   // per job: folded FP64 tile shape per stage
+  std::vector<std::array<int, 4>> jmcfg;  // per job: mixed tile configuration per stage
+  int cfg[4] = {FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT};  // of the largest grid (reported)
+  int mcfg[4] = {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT};
+  long version = 0;        // unique per table and assignment (captured graphs depend on it)
   const int* starts_of(int kind) const { return starts + long(kind) * (count + 1); }
 };
-void jobset_create(JobSet& js, const std::vector<Job>& jobs);
+// tuning = false: every job starts with the default configurations
+void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning = false);
+void jobset_assign(JobSet& js);  // after changing jcfg / jmcfg: new prefix sums, uploaded (synchronous)
 void jobset_upload(JobSet& js, hipStream_t s);  // after changing js.host (same grid sizes)
 void jobset_destroy(JobSet& js);
 
@@ -233,9 +250,9 @@ void dct_plan_destroy(DctPlan& p);
 // transform of the single job with plan p)
 void launch_dct_solve(const JobSet& js, const DctPlan* p, hipStream_t s);
 void launch_dct_gemm_stage(const JobSet& js, const DctPlan* p, hipStream_t s);
-// times every valid candidate tile shape of each folded stage over the table's
-// jobs on stream s (cached per device and set of grid sizes) and stores the
-// fastest in js.cfg
+// times every valid candidate tile shape of each folded stage over the jobs
+// of each grid size of the table on stream s (cached per device, grid size
+// and number of jobs of that size) and assigns the fastest to those jobs
 void dct_tune(JobSet& js, int device, hipStream_t s);
 
 // Picks the fastest of `count` launch configurations on stream s: after two
@@ -290,8 +307,8 @@ void fields_enable_mixed(Fields& f);  // allocates the mixed buffers (driver.hip
 void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s, bool prep = true);
 void launch_mprep(const JobSet& js, bool gate, hipStream_t s);  // folded planes of r' (first iteration)
 void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s);
-// times every valid tile width of each mixed stage (cached like dct_tune) and
-// stores the fastest in js.mcfg
+// times every valid tile configuration of each mixed stage per grid size
+// (cached and assigned like dct_tune)
 void mixed_tune(JobSet& js, int device, hipStream_t s);
 
 // Per-grid buffers and the iteration driver (driver.hip)

@@ -245,8 +245,12 @@ int otm_solver_set_gemm_config(otm_solver* s, int stage, int config) {
     return OTM_ERR_ARG;
   }
   if (!fold_cfg_valid(s->n, config)) { set_error("tile shape does not divide n/2"); return OTM_ERR_ARG; }
-  s->js.cfg[stage] = config;
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  s->js.jcfg[0][stage] = config;
+  jobset_assign(s->js);
   return OTM_OK;
+  OTM_GUARD_END
 }
 
 int otm_solver_mixed_config(const otm_solver* s, int* cfg) {
@@ -264,8 +268,12 @@ int otm_solver_set_mixed_config(otm_solver* s, int stage, int config) {
     return OTM_ERR_ARG;
   }
   if (!mixed_cfg_valid(stage, s->n, config)) { set_error("tile width does not divide the stage"); return OTM_ERR_ARG; }
-  s->js.mcfg[stage] = config;
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  s->js.jmcfg[0][stage] = config;
+  jobset_assign(s->js);
   return OTM_OK;
+  OTM_GUARD_END
 }
 
 int otm_solve_device(otm_solver* s, const double* d_density, const otm_options* opt, double* d_fwd_xy,
@@ -334,10 +342,9 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   Fields& f = J.f;
   JobSet pj;
   jobset_create(pj, {J});
-  for (int st = 0; st < 4; ++st) {
-    pj.cfg[st] = s->js.cfg[st];
-    pj.mcfg[st] = s->js.mcfg[st];
-  }
+  pj.jcfg = s->js.jcfg;
+  pj.jmcfg = s->js.jmcfg;
+  jobset_assign(pj);
   SolveState tpl;
   std::memset(&tpl, 0, sizeof(tpl));
   tpl.it = 1; tpl.cj = 1; tpl.max_iter = 1 << 30; tpl.threshold = 0.; tpl.alpha = 0.5; tpl.beta = 0.1;

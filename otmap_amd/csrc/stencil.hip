@@ -19,6 +19,7 @@
 //   quartic line search    otlib/otsolver_2dgrid.cpp:541-606
 //   density adjustment     otlib/otsolver_2dgrid.cpp:47-73
 
+#include <atomic>
 #include <cfloat>
 
 #include "otm_internal.h"
@@ -1050,44 +1051,93 @@ void launch_count_open(const JobSet& js, int* d_open, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // job tables
 // ---------------------------------------------------------------------------
-void jobset_create(JobSet& js, const std::vector<Job>& jobs) {
-  js.count = int(jobs.size());
-  js.host = jobs;
-  js.fold = true;
-  for (const Job& J : jobs) js.fold = js.fold && J.P != nullptr;
+namespace {
+
+// workgroup prefix sums of every launch kind for the table's jobs and their
+// assigned tile configurations (every valid configuration in a tuning table)
+std::vector<int> job_starts(JobSet& js) {
   const int C1 = js.count + 1;
   std::vector<int> st(size_t(KIND_COUNT) * C1, 0);
   js.total.assign(KIND_COUNT, 0);
+  // GEMM launches start every job on a multiple of 8 workgroups (XCD-aware tile order)
+  auto pad8 = [](int w) { return (w + 7) / 8 * 8; };
   for (int kind = 0; kind < KIND_COUNT; ++kind) {
     int acc = 0;
     for (int p = 0; p < js.count; ++p) {
       st[size_t(kind) * C1 + p] = acc;
-      const int n = jobs[p].f.n;
+      const Job& J = js.host[p];
+      const int n = J.f.n;
+      const auto& fc = js.jcfg[p];
+      const auto& mc = js.jmcfg[p];
       int w = 0;
       if (kind == KIND_STENCIL) w = num_tiles(n);
-      else if (kind == KIND_FLAT) w = flat_blocks(jobs[p].f.F);
+      else if (kind == KIND_FLAT) w = flat_blocks(J.f.F);
       else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
       else if (kind == KIND_MRES) w = mixed_supported(n) ? mres_tiles(n) : 0;
       else if (kind == KIND_UPAIR) w = mixed_supported(n) ? upair_tiles(n) : 0;
+      else if (kind == KIND_DNORM64) w = !js.tuning && fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
+      else if (kind == KIND_DNORMMX)
+        w = !js.tuning && mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
       else if (kind < KIND_GEMM0) {
         const int g = kind - KIND_M0, stage = g / NMIXED, c = g % NMIXED;
-        w = mixed_cfg_valid(stage, n, c) ? mixed_tiles(stage, n, c) : 0;
-      }
-      else {
+        const bool on = mixed_cfg_valid(stage, n, c) && (js.tuning || mc[stage] == c);
+        w = on ? pad8(mixed_tiles(stage, n, c)) : 0;
+      } else {
         const int g = kind - KIND_GEMM0, stage = g / NFOLD, c = g % NFOLD;
-        w = fold_cfg_valid(n, c) ? fold_tiles(n, stage, c) : 0;
+        const bool on = fold_cfg_valid(n, c) && (js.tuning || fc[stage] == c);
+        w = on ? pad8(fold_tiles(n, stage, c)) : 0;
       }
-      // GEMM launches start every job on a multiple of 8 workgroups (XCD-aware tile order)
-      if (kind >= KIND_M0) w = (w + 7) / 8 * 8;
       acc += w;
     }
     st[size_t(kind) * C1 + js.count] = acc;
     js.total[kind] = acc;
   }
+  return st;
+}
+
+// a process-wide unique stamp per table assignment
+long next_version() {
+  static std::atomic<long> counter{0};
+  return ++counter;
+}
+
+// configurations of the largest grid, reported by the configuration queries
+void report_configs(JobSet& js) {
+  int big = 0;
+  for (int p = 1; p < js.count; ++p)
+    if (js.host[p].f.n > js.host[big].f.n) big = p;
+  for (int k = 0; k < 4; ++k) {
+    js.cfg[k] = js.jcfg[big][k];
+    js.mcfg[k] = js.jmcfg[big][k];
+  }
+}
+
+}  // namespace
+
+void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning) {
+  js.count = int(jobs.size());
+  js.host = jobs;
+  js.tuning = tuning;
+  js.fold = true;
+  for (const Job& J : jobs) js.fold = js.fold && J.P != nullptr;
+  js.jcfg.assign(js.count, {FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT});
+  js.jmcfg.assign(js.count, {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT});
+  report_configs(js);
+  js.version = next_version();
+  const std::vector<int> st = job_starts(js);
   OTM_HIP(hipMalloc(&js.jobs, js.count * sizeof(Job)));
   OTM_HIP(hipMalloc(&js.starts, st.size() * sizeof(int)));
   OTM_HIP(hipMemcpy(js.jobs, jobs.data(), js.count * sizeof(Job), hipMemcpyHostToDevice));
   OTM_HIP(hipMemcpy(js.starts, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void jobset_assign(JobSet& js) {
+  // no launch of the device may still read the old prefix sums
+  OTM_HIP(hipDeviceSynchronize());
+  report_configs(js);
+  const std::vector<int> st = job_starts(js);
+  OTM_HIP(hipMemcpy(js.starts, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+  js.version = next_version();
 }
 
 void jobset_upload(JobSet& js, hipStream_t s) {
