@@ -655,12 +655,53 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
   }
 }
 
-// line-search update of the tile at (i0, j0): psi += alpha d, g0 += alpha
-// grad(d), r <- a alpha^2 + b alpha + r (otsolver_2dgrid.cpp:599-604) and
-// Jd = r_{k+1} - r_k for the next beta; k = completed iterations - 1.  The
-// quartic coefficients a, b come from d and G0 before its update, as in the
-// line-search pass.  rn: the new residual of this thread's faces (0 outside).
-__device__ inline void update_tile(const Fields& f, int k, double alpha, int i0, int j0, double (&rn)[FR]) {
+// Inputs of the update of one tile, loaded into registers: direction halo,
+// G0 of the tile's vertices, psi / r of its faces (the update overwrites
+// psi and r, and writes G0 to the other buffer, so a workgroup may load the
+// inputs of its next tile before it updates the current one).
+struct UpdateIn {
+  double hv[HR];
+  double hxv[VR], hyv[VR];
+  double psv[FR], rkv[FR];
+};
+
+__device__ inline void update_load(const Fields& f, int k, int i0, int j0, UpdateIn& R) {
+  const int n = f.n;
+  const auto d = gp(f.d[(k + 1) & 1]);
+  const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
+  const auto psi = gp(f.psi), r = gp(f.r);
+  halo_load(R.hv, n, i0, j0, [&](long fc) { return d[fc]; });
+  const long V1 = n + 1;
+  const long V = V1 * V1;
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int va = t / LDV, vb = t - va * LDV;
+    const int vi = i0 + va, vj = j0 + vb;
+    const bool in = t < NV && vi <= n && vj <= n;
+    R.hxv[u] = in ? g0[vi * V1 + vj] : 0.;
+    R.hyv[u] = in ? g0[V + vi * V1 + vj] : 0.;
+  }
+#pragma unroll
+  for (int u = 0; u < FR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int fi = i0 + t / TJ, fj = j0 + t % TJ;
+    const bool in = fi < n && fj < n;
+    const long fc = long(fi) * n + fj;
+    R.psv[u] = in ? psi[fc] : 0.;
+    R.rkv[u] = in ? r[fc] : 0.;
+  }
+}
+
+// line-search update of the tile at (i0, j0) from its loaded inputs: psi +=
+// alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
+// (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta;
+// k = completed iterations - 1.  The quartic coefficients a, b come from d
+// and G0 before its update, as in the line-search pass.  rn: the new residual
+// of this thread's faces (0 outside).  Starts by writing the workgroup's LDS:
+// a second call needs a barrier after the first.
+__device__ inline void update_apply(const Fields& f, int k, double alpha, int i0, int j0, const UpdateIn& R,
+                                    double (&rn)[FR]) {
   const int n = f.n;
   // LDS as in the line-search pass: gx, gy, hx, hy, the halo s in hx, hy's place
   __shared__ double lds[4 * NV];
@@ -669,46 +710,13 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
   double* const hx = lds + 2 * NV;
   double* const hy = lds + 3 * NV;
   double* const s = hx;
-  const auto d = gp(f.d[(k + 1) & 1]);
   // G0 ping-pong: neighbouring tiles read the border vertices this tile
   // updates, so the update goes to the other buffer
-  const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
   const auto g0n = gp(f.g0 + ((k + 1) & 1) * 2 * f.V);
   const auto psi = gp(f.psi), r = gp(f.r), jd = gp(f.jd);
-  // all global loads of the tile first: direction halo, G0 of the tile's
-  // vertices, psi / r of the faces (the stores below overwrite them)
-  double hv[HR];
-  halo_load(hv, n, i0, j0, [&](long fc) { return d[fc]; });
   const long V1 = n + 1;
   const long V = V1 * V1;
-  // vertices owned by this tile: [i0, i0+TI) x [j0, j0+TJ), plus the last row/column of the grid
-  bool own[VR];
-  long vid[VR];
-  double hxv[VR], hyv[VR];
-#pragma unroll
-  for (int u = 0; u < VR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    const int va = t / LDV, vb = t - va * LDV;
-    const int vi = i0 + va, vj = j0 + vb;
-    const bool in = t < NV && vi <= n && vj <= n;
-    const bool own_i = (va < TI) || (vi == n);
-    const bool own_j = (vb < TJ) || (vj == n);
-    own[u] = in && own_i && own_j;
-    vid[u] = vi * V1 + vj;
-    hxv[u] = in ? g0[vid[u]] : 0.;
-    hyv[u] = in ? g0[V + vid[u]] : 0.;
-  }
-  double psv[FR], rkv[FR];
-#pragma unroll
-  for (int u = 0; u < FR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    const int fi = i0 + t / TJ, fj = j0 + t % TJ;
-    const bool in = fi < n && fj < n;
-    const long fc = long(fi) * n + fj;
-    psv[u] = in ? psi[fc] : 0.;
-    rkv[u] = in ? r[fc] : 0.;
-  }
-  halo_store(s, hv);
+  halo_store(s, R.hv);
   __syncthreads();
   tile_grads(s, gx, gy, n, i0, j0);
   double dv[FR];  // direction at the tile's own faces
@@ -722,17 +730,23 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
   for (int u = 0; u < VR; ++u) {
     const int t = threadIdx.x + u * NT;
     if (t < NV) {
-      hx[t] = hxv[u];
-      hy[t] = hyv[u];
+      hx[t] = R.hxv[u];
+      hy[t] = R.hyv[u];
     }
   }
   __syncthreads();
+  // vertices owned by this tile: [i0, i0+TI) x [j0, j0+TJ), plus the last row/column of the grid
 #pragma unroll
   for (int u = 0; u < VR; ++u) {
     const int t = threadIdx.x + u * NT;
-    if (own[u]) {
-      g0n[vid[u]] = hxv[u] + alpha * gx[t];
-      g0n[V + vid[u]] = hyv[u] + alpha * gy[t];
+    const int va = t / LDV, vb = t - va * LDV;
+    const int vi = i0 + va, vj = j0 + vb;
+    const bool in = t < NV && vi <= n && vj <= n;
+    const bool own = in && ((va < TI) || (vi == n)) && ((vb < TJ) || (vj == n));
+    if (own) {
+      const long vid = vi * V1 + vj;
+      g0n[vid] = R.hxv[u] + alpha * gx[t];
+      g0n[V + vid] = R.hyv[u] + alpha * gy[t];
     }
   }
   const double e = 1. / double(n);
@@ -744,10 +758,10 @@ __device__ inline void update_tile(const Fields& f, int k, double alpha, int i0,
     rn[u] = 0.;
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
-      psi[fc] = psv[u] + alpha * dv[u];
+      psi[fc] = R.psv[u] + alpha * dv[u];
       double av, bv;
       quartic_ab(gx, gy, hx, hy, fa, fb, e, av, bv);
-      const double rk = rkv[u];
+      const double rk = R.rkv[u];
       const double rnv = av * (alpha * alpha) + bv * alpha + rk;
       r[fc] = rnv;
       jd[fc] = rnv - rk;
@@ -764,8 +778,11 @@ __global__ __launch_bounds__(NT, 8) void k_update(JOB_ARGS) {
   if (!st->update) return;
   int i0, j0;
   tile_origin(L.t, f.n, i0, j0);
+  const int k = st->it - 1;
+  UpdateIn R;
+  update_load(f, k, i0, j0, R);
   double rn[FR];
-  update_tile(f, st->it - 1, st->alpha, i0, j0, rn);
+  update_apply(f, k, st->alpha, i0, j0, R, rn);
 }
 
 // The update pass of the mixed form (n % 64 == 0): one workgroup updates a
@@ -785,13 +802,18 @@ __global__ __launch_bounds__(NT) void k_update_fold(JOB_ARGS) {
   const int k = st->it - 1;
   const double alpha = st->alpha;
   __shared__ double rl[TI * TJ], rr[TI * TJ];
+  // both tiles' inputs are loaded before the first update (the mirror's
+  // loads overlap the first tile's work)
+  UpdateIn RL, RR;
+  update_load(f, k, i0, cl * TJ, RL);
+  if (cr != cl) update_load(f, k, i0, cr * TJ, RR);
   double rn[FR];
-  update_tile(f, k, alpha, i0, cl * TJ, rn);
+  update_apply(f, k, alpha, i0, cl * TJ, RL, rn);
 #pragma unroll
   for (int u = 0; u < FR; ++u) rl[threadIdx.x + u * NT] = rn[u];
   if (cr != cl) {
     __syncthreads();  // the tile's LDS is reused
-    update_tile(f, k, alpha, i0, cr * TJ, rn);
+    update_apply(f, k, alpha, i0, cr * TJ, RR, rn);
 #pragma unroll
     for (int u = 0; u < FR; ++u) rr[threadIdx.x + u * NT] = rn[u];
   }
