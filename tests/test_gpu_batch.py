@@ -116,3 +116,28 @@ def test_batch_mixed_bit_identical_to_single_mixed(oracle):
         o = oracle.solve(n, dens[k])
         assert abs(st.iterations - o["iters"]) <= 1
         assert np.abs(tm.fwd_mesh().xy - o["fwd"]).max() < 1e-6
+
+
+def test_batch_tiles_tuned_per_grid_size():
+    """Each grid size of a batch runs the tile configurations tuned for it
+    (valid for that size, shared by every job of that size), and the results
+    stay bit-identical to single solves, which tune their own size alone."""
+    sizes = [192, 512, 192, 512]
+    dens = [D.gaussian_mixture(n, 90 + k) for k, n in enumerate(sizes)]
+    b = BatchSolver(sizes)
+    for prec in (Precision.FP64, Precision.MIXED):
+        opt = SolverOptions(precision=prec)
+        maps = b.solve(dens, opt)
+        for k, n in enumerate(sizes):
+            tm, st, _ = single(n, dens[k], opt)
+            assert b.last_stats[k].iterations == st.iterations, (prec, k)
+            assert np.array_equal(maps[k].fwd_mesh().xy, tm.fwd_mesh().xy), (prec, k)
+    h = {n: n // 2 for n in sizes}
+    g = [b.gemm_config(k)[0] for k in range(len(sizes))]
+    m = [b.mixed_config(k)[0] for k in range(len(sizes))]
+    print("fp64 tiles", g, "mixed tiles", m, flush=True)
+    assert g[0] == g[2] and g[1] == g[3] and m[0] == m[2] and m[1] == m[3]
+    widths = [32, 64, 128, 128]
+    for k, n in enumerate(sizes):
+        assert all(h[n] % widths[c] == 0 for c in m[k]), (n, m[k])
+    assert b.gemm_config()[0] == g[1] and b.mixed_config()[0] == m[1]  # -1: the largest grid
