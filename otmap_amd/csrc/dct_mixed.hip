@@ -162,11 +162,15 @@ __global__ __launch_bounds__(NT) void k_mresid(const Job* __restrict__ jobs, con
   const int a = t / (RT / 4), c0 = (t % (RT / 4)) * 4;
   const int i = i0 + a;
   // this thread's residuals, loaded before the halo so that both latencies overlap
+  // (32-byte aligned runs: j0 + c0 and n - 4 - j0 - c0 are multiples of 4)
   double rl[4], rr[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    rl[e] = r[long(i) * n + j0 + c0 + e];
-    rr[e] = r[long(i) * n + (n - 1 - j0 - c0 - e)];
+  {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const auto pl = reinterpret_cast<OTM_GLOBAL const d2*>(r + long(i) * n + j0 + c0);
+    const auto pr = reinterpret_cast<OTM_GLOBAL const d2*>(r + long(i) * n + (n - 4 - j0 - c0));
+    const d2 l0 = pl[0], l1 = pl[1], r0 = pr[0], r1 = pr[1];
+    rl[0] = l0.x; rl[1] = l0.y; rl[2] = l1.x; rl[3] = l1.y;
+    rr[0] = r1.y; rr[1] = r1.x; rr[2] = r0.y; rr[3] = r0.x;  // columns n-1-j0-c0-e, e = 0..3
   }
   constexpr int HR2 = (HW * HW + NT - 1) / NT;
   double vl[HR2], vr[HR2];
