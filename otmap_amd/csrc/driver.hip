@@ -5,10 +5,12 @@
 // initial residual (otsolver_2dgrid.cpp:106-160), then Newton iterations until
 // every job's device state says done (:160-221), then the forward positions
 // (:223-227).  Iterations are captured once into a HIP graph of ITERS_PER_GRAPH
-// iterations and replayed; after each batch of replays one small kernel counts
-// the jobs still running and the host reads that count.  Iterations replayed
-// after a job has stopped are no-ops for that job (every kernel checks its
-// gate), so results do not depend on how many replays are in flight.
+// iterations and replayed; after each replay one small kernel counts the jobs
+// still running into one of two pinned words, and the host reads the count of
+// replay k while replay k + 1 runs (the device never waits for the host).
+// Iterations replayed after a job has stopped are no-ops for that job (every
+// kernel checks its gate), so results do not depend on how many replays are
+// in flight.
 
 #include <cfloat>
 #include <cstring>
@@ -85,8 +87,10 @@ void driver_create(Driver& d) {
   OTM_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
   OTM_HIP(hipEventCreate(&d.ev0));
   OTM_HIP(hipEventCreate(&d.ev1));
-  OTM_HIP(hipMalloc(&d.d_open, sizeof(int)));
-  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), sizeof(int), hipHostMallocDefault));
+  OTM_HIP(hipEventCreate(&d.evc[0]));
+  OTM_HIP(hipEventCreate(&d.evc[1]));
+  OTM_HIP(hipMalloc(&d.d_open, 2 * sizeof(int)));
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), 2 * sizeof(int), hipHostMallocDefault));
 }
 
 void driver_invalidate(Driver& d) {
@@ -104,6 +108,8 @@ void driver_destroy(Driver& d) {
   if (d.h_open) (void)hipHostFree(d.h_open);
   if (d.ev0) (void)hipEventDestroy(d.ev0);
   if (d.ev1) (void)hipEventDestroy(d.ev1);
+  for (hipEvent_t e : d.evc)
+    if (e) (void)hipEventDestroy(e);
   if (d.stream) (void)hipStreamDestroy(d.stream);
   d = Driver();
 }
@@ -149,23 +155,28 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
     OTM_HIP(hipGraphInstantiate(&d.exec, d.graph, nullptr, nullptr, 0));
     d.key = key;
   }
-  // replays: 1, 2, 4 graphs between checks (4, 8, 16 iterations)
+  // one replay in flight ahead of the check: replay k + 1 is queued before
+  // the host waits for the open-job count of replay k (slot k % 2)
   long launched = 0;
-  int chunk = 1;
+  auto replay = [&](int slot) {
+    OTM_HIP(hipGraphLaunch(d.exec, s));
+    launched += ITERS_PER_GRAPH;
+    launch_count_open(js, d.d_open + slot, s);
+    OTM_HIP(hipMemcpyAsync(d.h_open + slot, d.d_open + slot, sizeof(int), hipMemcpyDeviceToHost, s));
+    OTM_HIP(hipEventRecord(d.evc[slot], s));
+  };
+  int slot = 0;
+  replay(slot);
   for (;;) {
-    for (int c = 0; c < chunk && launched <= long(opt.max_iter); ++c) {
-      OTM_HIP(hipGraphLaunch(d.exec, s));
-      launched += ITERS_PER_GRAPH;
-    }
-    launch_count_open(js, d.d_open, s);
-    OTM_HIP(hipMemcpyAsync(d.h_open, d.d_open, sizeof(int), hipMemcpyDeviceToHost, s));
-    OTM_HIP(hipStreamSynchronize(s));
-    if (*d.h_open == 0) break;
-    if (launched > long(opt.max_iter)) {
+    const bool more = launched <= long(opt.max_iter);
+    if (more) replay(slot ^ 1);
+    OTM_HIP(hipEventSynchronize(d.evc[slot]));
+    if (d.h_open[slot] == 0) break;
+    if (!more) {
       set_error("internal error: solve did not reach its stop condition");
       return OTM_ERR_STATE;
     }
-    chunk = std::min(chunk * 2, 4);
+    slot ^= 1;
   }
   launch_finish(js, s);
   OTM_HIP(hipEventRecord(d.ev1, s));

@@ -321,9 +321,10 @@ struct Driver {
   hipGraph_t graph = nullptr;     // captured iterations
   hipGraphExec_t exec = nullptr;
   long key = -1;                  // precision mode and tile shapes of the captured graph
-  int* d_open = nullptr;          // jobs still running (device)
-  int* h_open = nullptr;          // pinned copy
+  int* d_open = nullptr;          // jobs still running after a replay (device, two slots)
+  int* h_open = nullptr;          // pinned copies
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evc[2] = {nullptr, nullptr};  // the copy of each slot is complete
 };
 void driver_create(Driver& d);
 void driver_destroy(Driver& d);
