@@ -117,7 +117,8 @@ def test_constant_density_zero_iterations(solver_factory):
     assert np.abs(tm.fwd_mesh().xy - ref).max() <= 1e-15
 
 
-@pytest.mark.parametrize("max_iter", [0, 1, 3])
+# 4, 5, 8, 9: the cap at and just past whole replays of the 4-iteration graph
+@pytest.mark.parametrize("max_iter", [0, 1, 3, 4, 5, 8, 9])
 def test_max_iter_cap(max_iter, oracle, solver_factory):
     from otmap_amd import SolverOptions
     n = 32
