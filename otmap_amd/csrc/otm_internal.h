@@ -196,7 +196,7 @@ constexpr int MIXED_DEFAULT = 0;  // valid for every mixed n
 struct JobSet {
   int count = 0;
   Job* jobs = nullptr;     // device copy of the table
-  int* starts = nullptr;   // device: KIND_COUNT x (count + 1) workgroup prefix sums
+  int* starts = nullptr;   // device, per kind: count + 1 workgroup prefix sums, then the job of each workgroup
   std::vector<Job> host;   // host copy
   std::vector<int> total;  // workgroups per kind
   bool fold = false;       // every job uses the folded transform
@@ -206,7 +206,8 @@ struct JobSet {
   int cfg[4] = {FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT};  // of the largest grid (reported)
   int mcfg[4] = {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT};
   long version = 0;        // unique per table and assignment (captured graphs depend on it)
-  const int* starts_of(int kind) const { return starts + long(kind) * (count + 1); }
+  std::vector<long> kofs;  // offset of each kind's block in `starts`
+  const int* starts_of(int kind) const { return starts + kofs[kind]; }
 };
 // tuning = false: every job starts with the default configurations
 void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning = false);

@@ -98,15 +98,11 @@ __device__ static inline void xcd_tile(int lt, int R, int C, int& tm, int& tn) {
   }
 }
 
-// Job of workgroup b in a launch over a job table (starts: count + 1 prefix
-// sums of the workgroups per job); uniform across the workgroup.
+// Job of workgroup b in a launch over a job table: the kind's block holds
+// count + 1 prefix sums of the workgroups per job, then the job of every
+// workgroup (one load instead of a search); uniform across the workgroup.
 __device__ static inline int job_of(const int* __restrict__ starts, int count, int b) {
-  int lo = 0, hi = count - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (starts[mid] <= b) lo = mid; else hi = mid - 1;
-  }
-  return lo;
+  return starts[count + 1 + b];
 }
 
 // pass 2 (single workgroup): fold nb partials of each of K values; valid in

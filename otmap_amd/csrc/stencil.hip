@@ -1075,46 +1075,54 @@ void launch_count_open(const JobSet& js, int* d_open, hipStream_t s) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// workgroup prefix sums of every launch kind for the table's jobs and their
-// assigned tile configurations (every valid configuration in a tuning table)
+// Device image of the launch kinds for the table's jobs and their assigned
+// tile configurations (every valid configuration in a tuning table): per
+// kind, count + 1 workgroup prefix sums, then the job of each workgroup.
+// Fills js.total and js.kofs.
 std::vector<int> job_starts(JobSet& js) {
-  const int C1 = js.count + 1;
-  std::vector<int> st(size_t(KIND_COUNT) * C1, 0);
   js.total.assign(KIND_COUNT, 0);
+  js.kofs.assign(KIND_COUNT, 0);
+  std::vector<int> img;
   // GEMM launches start every job on a multiple of 8 workgroups (XCD-aware tile order)
   auto pad8 = [](int w) { return (w + 7) / 8 * 8; };
+  std::vector<int> w(js.count);
   for (int kind = 0; kind < KIND_COUNT; ++kind) {
-    int acc = 0;
     for (int p = 0; p < js.count; ++p) {
-      st[size_t(kind) * C1 + p] = acc;
       const Job& J = js.host[p];
       const int n = J.f.n;
       const auto& fc = js.jcfg[p];
       const auto& mc = js.jmcfg[p];
-      int w = 0;
-      if (kind == KIND_STENCIL) w = num_tiles(n);
-      else if (kind == KIND_FLAT) w = flat_blocks(J.f.F);
-      else if (kind == KIND_PLAIN64) w = ((n + 63) / 64) * ((n + 63) / 64);
-      else if (kind == KIND_MRES) w = mixed_supported(n) ? mres_tiles(n) : 0;
-      else if (kind == KIND_UPAIR) w = mixed_supported(n) ? upair_tiles(n) : 0;
-      else if (kind == KIND_DNORM64) w = !js.tuning && fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
+      int v = 0;
+      if (kind == KIND_STENCIL) v = num_tiles(n);
+      else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
+      else if (kind == KIND_PLAIN64) v = ((n + 63) / 64) * ((n + 63) / 64);
+      else if (kind == KIND_MRES) v = mixed_supported(n) ? mres_tiles(n) : 0;
+      else if (kind == KIND_UPAIR) v = mixed_supported(n) ? upair_tiles(n) : 0;
+      else if (kind == KIND_DNORM64) v = !js.tuning && fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
       else if (kind == KIND_DNORMMX)
-        w = !js.tuning && mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
+        v = !js.tuning && mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
       else if (kind < KIND_GEMM0) {
         const int g = kind - KIND_M0, stage = g / NMIXED, c = g % NMIXED;
         const bool on = mixed_cfg_valid(stage, n, c) && (js.tuning || mc[stage] == c);
-        w = on ? pad8(mixed_tiles(stage, n, c)) : 0;
+        v = on ? pad8(mixed_tiles(stage, n, c)) : 0;
       } else {
         const int g = kind - KIND_GEMM0, stage = g / NFOLD, c = g % NFOLD;
         const bool on = fold_cfg_valid(n, c) && (js.tuning || fc[stage] == c);
-        w = on ? pad8(fold_tiles(n, stage, c)) : 0;
+        v = on ? pad8(fold_tiles(n, stage, c)) : 0;
       }
-      acc += w;
+      w[p] = v;
     }
-    st[size_t(kind) * C1 + js.count] = acc;
+    js.kofs[kind] = long(img.size());
+    int acc = 0;
+    for (int p = 0; p < js.count; ++p) {
+      img.push_back(acc);
+      acc += w[p];
+    }
+    img.push_back(acc);
     js.total[kind] = acc;
+    for (int p = 0; p < js.count; ++p) img.insert(img.end(), size_t(w[p]), p);
   }
-  return st;
+  return img;
 }
 
 // a process-wide unique stamp per table assignment
@@ -1154,10 +1162,13 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning) {
 }
 
 void jobset_assign(JobSet& js) {
-  // no launch of the device may still read the old prefix sums
+  // no launch of the device may still read the old table image
   OTM_HIP(hipDeviceSynchronize());
   report_configs(js);
   const std::vector<int> st = job_starts(js);
+  if (js.starts) (void)hipFree(js.starts);
+  js.starts = nullptr;
+  OTM_HIP(hipMalloc(&js.starts, st.size() * sizeof(int)));
   OTM_HIP(hipMemcpy(js.starts, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
   js.version = next_version();
 }
