@@ -45,10 +45,12 @@ def test_inverse_identity():
     assert np.array_equal(inv, g["ident_inv"])
 
 
-@pytest.mark.parametrize("n,seed", [(64, 9), (256, 1000)])
-def test_inverse_vs_oracle(n, seed, oracle):
+# the 60000-point case adds points outside the unit square (the query clamps them)
+@pytest.mark.parametrize("n,seed,npts", [(64, 9, 20000), (256, 1000, 20000), (256, 1001, 60000)])
+def test_inverse_vs_oracle(n, seed, npts, oracle):
     o = oracle.solve(n, D.gaussian_mixture(n, seed))
-    pts = np.concatenate([np.random.default_rng(n).uniform(0, 1, (20000, 2)),
+    rng = np.random.default_rng(n + npts)
+    pts = np.concatenate([rng.uniform(0, 1, (npts, 2)), rng.uniform(-0.1, 1.1, (npts // 20, 2)),
                           np.stack([o["fwd"][: (n + 1) ** 2], o["fwd"][(n + 1) ** 2:]], 1)[::7]])
     tm = TransportMap(GridMesh.regular(n), GridMesh(n, o["fwd"]), o["density"])
     for fast in (False, True):
