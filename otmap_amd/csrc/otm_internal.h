@@ -261,7 +261,7 @@ void dct_tune(JobSet& js, int device, hipStream_t s);
 // TUNE_REPS back-to-back launches of every configuration in turn (so drift
 // affects all alike); a configuration's time is its best round.  Ties go to
 // the lower index; -1 when none is valid.
-constexpr int TUNE_TRIALS = 3, TUNE_REPS = 4;
+constexpr int TUNE_TRIALS = 5, TUNE_REPS = 6;
 template <class Valid, class Launch>
 int tune_pick(int count, Valid valid, Launch launch, hipStream_t s) {
   hipEvent_t e0, e1;
