@@ -453,7 +453,8 @@ def main():
                              "k_gemm_f64: one DCT stage (2 n^3 flops), FP64 MFMA", stage_flops_f64)
     roof_x3 = None
     if "mixed" in results:
-        kname = ["k_mgemm<M2,32>", "k_mgemm<M2,64>", "k_mgemm_w<M2>"][s.mixed_config()[0][1]]
+        kname = ["k_mgemm<M2,32>", "k_mgemm<M2,64>", "k_mgemm_w<M2,4 waves>", "k_mgemm_w<M2,8 waves>"][
+            s.mixed_config()[0][1]]
         roof_x3 = gemm_roofline(kname, kernel_ms["mixed_gemm_bf16x3"], PEAK_BF16_TFLOPS,
                                 f"{kname}: one even/odd-folded DCT stage (n^3 flops), 3-term BF16 "
                                 "split on MFMA, FP32 accumulate", float(n) ** 3)
