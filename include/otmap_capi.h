@@ -155,6 +155,12 @@ int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat);
  * FP64 refinement steps); n must be a multiple of 64 */
 int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int refine_steps);
 int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* dir, double* a, double* b);
+/* Step choice of solve_1D_problem (otlib/otsolver_2dgrid.cpp:563-595) on the
+ * device, for `count` coefficient vectors z (5 doubles each: e(t) = z0 + z1 t
+ * + ... + z4 t^4, the reference's z): alpha[k] = the real stationary point in
+ * [-10, 10] of least e, else 0.5; emin[k] = e(alpha[k]).  This is the choice
+ * every device line search makes (host buffers; synchronous). */
+int otm_line_search_steps(int device, const double* z, int count, double* alpha, double* emin);
 
 /* ---- TransportMap (otlib/transport_map.h:29-63) -------------------------
  * A map holds the forward mesh (2*(n+1)^2 SoA positions) and the adjusted

@@ -262,6 +262,7 @@ class Reference:
             "ref_apply_inverse_map": (_i, [_i, _dp, _dp, _dp, _l, P]),
             "ref_rasterize": (None, [_i, _dp, P, _i, _i, _dp]),
             "ref_transport_cost": (_d, [_i, _dp, _dp, _dp, P]),
+            "ref_solver_transport_cost": (_d, [_i, _dp, _dp, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -350,6 +351,13 @@ class Reference:
         ptr = cost.ctypes.data_as(C.POINTER(C.c_double)) if per_face else None
         r = self.L.ref_transport_cost(n, _f64(src), _f64(dst), _f64(density), ptr)
         return (r, cost) if per_face else r
+
+    def solver_transport_cost(self, n, G, rho_adj):
+        """GridBasedTransportSolver::compute_transport_cost on vertex
+        displacements G (SoA) with the adjusted density rho_adj: (sum, per face)."""
+        cost = np.zeros(n * n)
+        r = self.L.ref_solver_transport_cost(n, _f64(G), _f64(rho_adj), cost.ctypes.data_as(C.POINTER(C.c_double)))
+        return r, cost
 
 
 def reference_available() -> bool:

@@ -14,6 +14,7 @@
 //   TransportMap::inv / inv_fast, BVH2D         otlib/transport_map.cpp:50-102, otlib/utils/bvh2d.cpp
 //   rasterize_image                             otlib/utils/rasterizer.cpp:188-308
 //   transport_cost                              otlib/transport_map.cpp:131-215
+//   compute_transport_cost                      otlib/otsolver_2dgrid.cpp:608-640
 //   nestdiss_ordering                           otlib/details/nested_dissection.cpp:54-61
 
 #include "otsolver_2dgrid.h"
@@ -75,6 +76,7 @@ struct Probe : public otmap::GridBasedTransportSolver {
     return compute_conjugate_jacobian_beta(xk, rkm1, rk, dh, dp, alpha);
   }
   void adjust(VectorXd& d, double max_ratio) { adjust_density(d, max_ratio); }
+  void transport_cost(const MatrixX2d& g, VectorXd& cost) const { compute_transport_cost(g, cost); }
 };
 
 std::map<int, std::unique_ptr<Probe>>& probes() {
@@ -343,6 +345,21 @@ double ref_transport_cost(int n, const double* src_xy, const double* dst_xy, con
   double r = otmap::transport_cost(*src, *dst, Map<const VectorXd>(density, n * n), cost_per_face ? &c : nullptr);
   if (cost_per_face) Map<VectorXd>(cost_per_face, n * n) = c;
   return r;
+}
+
+// GridBasedTransportSolver::compute_transport_cost on vertex displacements
+// disp_xy (SoA, the gradients the solver passes) and the adjusted density the
+// solver would hold; returns the sum of the per-face costs.
+double ref_solver_transport_cost(int n, const double* disp_xy, const double* rho_adj, double* cost_per_face) {
+  MxcsrGuard guard;
+  Probe& p = probe(n);
+  VectorXd d = Map<const VectorXd>(rho_adj, n * n);
+  p.set_density(&d);
+  VectorXd c;
+  p.transport_cost(soa_to_grads(n, disp_xy), c);
+  p.set_density(nullptr);
+  if (cost_per_face) Map<VectorXd>(cost_per_face, n * n) = c;
+  return c.sum();
 }
 
 }  // extern "C"

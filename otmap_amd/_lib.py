@@ -18,6 +18,7 @@ HEADERS = [os.path.join(ROOT, "include", "otmap_capi.h")]
 OTM_OK = 0
 OTM_BETA_ZERO, OTM_BETA_CONJUGATE_JACOBIAN = 0, 1
 OTM_PRECISION_FP64, OTM_PRECISION_MIXED = 0, 1
+OTM_EXIT_NONE, OTM_EXIT_CONVERGED_OR_MAX_ITER, OTM_EXIT_ZERO_DIRECTION = 0, 1, 2
 (OTM_KERNEL_DCT_GEMM, OTM_KERNEL_BETA, OTM_KERNEL_DIR_COEF, OTM_KERNEL_UPDATE, OTM_KERNEL_DCT_SOLVE,
  OTM_KERNEL_MIXED_GEMM, OTM_KERNEL_MIXED_SOLVE) = range(7)
 
@@ -58,6 +59,7 @@ _SIGNATURES = {
     "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
     "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),
+    "otm_line_search_steps": (C.c_int, [C.c_int, _dp, C.c_int, _dp, _dp]),
     "otm_laplacian_solve_mixed": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "otm_line_search_coefficients": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_map_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _dp, _dp]),

@@ -45,12 +45,41 @@ FLAGS = [
 ]
 
 
+# Kernels whose hand-placed waits (s_waitcnt vmcnt(N) after a known number of
+# issued transfers, dct_mixed.hip) are only exact when the compiler adds no
+# vector memory operation of its own -- a register spill would add scratch
+# loads and stores.  The build fails if any of them reports scratch use.
+NO_SCRATCH = {"dct_mixed.hip": ("k_mgemm",)}
+
+
+def _resource_usage(stderr: str) -> dict:
+    """kernel (mangled) name -> scratch bytes per lane, from hipcc's
+    -Rpass-analysis=kernel-resource-usage remarks."""
+    usage, name = {}, None
+    for line in stderr.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split()[0]
+        elif "ScratchSize [bytes/lane]:" in line and name:
+            usage[name] = int(line.split("ScratchSize [bytes/lane]:")[1].split()[0])
+    return usage
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    if src in NO_SCRATCH:
+        cmd.append("-Rpass-analysis=kernel-resource-usage")
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
+    if src in NO_SCRATCH:
+        usage = _resource_usage(r.stderr)
+        checked = [k for k in usage if any(p in k for p in NO_SCRATCH[src])]
+        if not checked:
+            raise RuntimeError(f"{src}: no resource-usage remarks for {NO_SCRATCH[src]}")
+        spilled = {k: v for k, v in usage.items() if k in checked and v}
+        if spilled:
+            raise RuntimeError(f"{src}: kernels with hand-placed vmcnt waits use scratch: {spilled}")
     return obj
 
 

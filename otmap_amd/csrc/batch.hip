@@ -98,7 +98,7 @@ int solve_batch(otm_batch* b, const double* const* d_density, const otm_options*
     return OTM_ERR_ARG;
   }
   bool rebuild = false;
-  const int cap = std::max(opt.max_iter, 1);
+  const int cap = trace_cap_for(opt.max_iter);
   if (cap > b->trace_cap) {
     for (Fields& f : b->f) fields_grow_trace(f, cap);
     b->trace_cap = cap;

@@ -62,7 +62,7 @@ enum MFinal { FIN_X0 = 0, FIN_OUT = 1, FIN_ADD_OUT = 2, FIN_ADD_X0 = 3 };
 // the kernels wait for them explicitly (vmcnt) before the barrier that
 // publishes a buffer.  The instruction has no register results.
 __device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds) {
-  asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(sbase), "s"(lds) : "memory");
+  asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(sbase), "s"(lds) : "memory", "m0");
 }
 // Wait until at most N of this wave's vector memory operations are
 // outstanding and its LDS reads have completed, then the workgroup barrier

@@ -231,6 +231,7 @@ void launch_fill(double* p, long count, double v, hipStream_t s);
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s);
 void launch_residual_only(const double* psi, const double* rho, double* r, double* partials, int n, hipStream_t s);
 void launch_params_1d(const double* g0, const double* dir, double* a, double* b, int n, hipStream_t s);
+void launch_pick_steps(const double* z, int count, double* alpha, double* emin, hipStream_t s);
 
 // DCT Laplacian solve (dct.hip): d_hat = M^+ (r - S e0), S = state->sum_r
 struct DctPlan {
@@ -312,7 +313,12 @@ void launch_mixed_gemm_stage(const JobSet& js, hipStream_t s);
 // (cached and assigned like dct_tune)
 void mixed_tune(JobSet& js, int device, hipStream_t s);
 
-// Per-grid buffers and the iteration driver (driver.hip)
+// Per-grid buffers and the iteration driver (driver.hip).  The per-iteration
+// trace (residual, alpha, beta) keeps the first TRACE_MAX iterations whatever
+// max_iter is (a solve may be given INT_MAX as "unlimited"); later iterations
+// run but are not traced.
+constexpr int TRACE_MAX = 4096;
+inline int trace_cap_for(int max_iter) { return std::min(std::max(max_iter, 1), TRACE_MAX); }
 void fields_create(Fields& f, int n, int trace_cap);
 void fields_destroy(Fields& f);
 bool fields_grow_trace(Fields& f, int cap);  // true when the buffer was reallocated

@@ -142,7 +142,7 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
     }
     ensure_mixed(s);
   }
-  if (fields_grow_trace(s->f, std::max(opt.max_iter, 1))) driver_invalidate(s->drv);
+  if (fields_grow_trace(s->f, trace_cap_for(opt.max_iter))) driver_invalidate(s->drv);
   Job& J = s->js.host[0];
   J.f = s->f;
   J.dens_in = d_density;
@@ -476,6 +476,29 @@ int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* 
   OTM_HIP(hipMemcpyAsync(a, f.a, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
   OTM_HIP(hipMemcpyAsync(b, f.b, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
   OTM_HIP(hipStreamSynchronize(s->drv.stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_line_search_steps(int device, const double* z, int count, double* alpha, double* emin) {
+  if (count < 0 || (count > 0 && (!z || !alpha || !emin))) { set_error("null pointer or negative count"); return OTM_ERR_ARG; }
+  if (count == 0) return OTM_OK;
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(device));
+  double *dz = nullptr, *da = nullptr, *de = nullptr;
+  OTM_HIP(hipMalloc(&dz, 5 * size_t(count) * sizeof(double)));
+  hipError_t rc = hipMalloc(&da, size_t(count) * sizeof(double));
+  if (rc == hipSuccess) rc = hipMalloc(&de, size_t(count) * sizeof(double));
+  if (rc == hipSuccess) rc = hipMemcpy(dz, z, 5 * size_t(count) * sizeof(double), hipMemcpyHostToDevice);
+  if (rc == hipSuccess) {
+    launch_pick_steps(dz, count, da, de, nullptr);
+    rc = hipMemcpy(alpha, da, size_t(count) * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  if (rc == hipSuccess) rc = hipMemcpy(emin, de, size_t(count) * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dz);
+  (void)hipFree(da);
+  (void)hipFree(de);
+  OTM_HIP(rc);
   return OTM_OK;
   OTM_GUARD_END
 }

@@ -1183,6 +1183,25 @@ void jobset_destroy(JobSet& js) {
   js = JobSet();
 }
 
+// step choice of the line search for `count` coefficient vectors z (5 each):
+// the same pick_step the scalar kernel of a solve runs
+__global__ __launch_bounds__(64) void k_pick_steps(const double* __restrict__ z, int count, double* __restrict__ alpha,
+                                                  double* __restrict__ emin) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= count) return;
+  double zz[5];
+  for (int i = 0; i < 5; ++i) zz[i] = z[5 * long(k) + i];
+  double e = 0.;
+  alpha[k] = pick_step(zz, e);
+  emin[k] = e;
+}
+
+void launch_pick_steps(const double* z, int count, double* alpha, double* emin, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_pick_steps, dim3((count + 63) / 64), dim3(64), 0, s, z, count, alpha, emin);
+  OTM_HIP(hipGetLastError());
+}
+
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_vertex_gradients, tile_grid(n), dim3(NT), 0, s, psi, G, n, 0);
   OTM_HIP(hipGetLastError());

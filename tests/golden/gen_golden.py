@@ -171,8 +171,129 @@ def raster():
     save("raster.npz", **out)
 
 
+def costs():
+    """GridBasedTransportSolver::compute_transport_cost (reference
+    otlib/otsolver_2dgrid.cpp:608-640) on random displacements and on the
+    displacements of solved maps, with the adjusted densities."""
+    out = {}
+    for n in (5, 16, 33):
+        rng = np.random.default_rng(400 + n)
+        G = rng.standard_normal(2 * (n + 1) ** 2) * 1e-2
+        rho = R.adjust_density(n, rng.uniform(0.1, 3.0, n * n))
+        tot, cf = R.solver_transport_cost(n, G, rho)
+        out.update({f"rand{n}_disp": G, f"rand{n}_rho": rho, f"rand{n}_total": np.float64(tot),
+                    f"rand{n}_cost": cf})
+    for n, seed in ((16, 7), (64, 9)):
+        r = R.solve(n, D.gaussian_mixture(n, seed))
+        ident = np.concatenate([np.repeat(np.arange(n + 1.0), n + 1), np.tile(np.arange(n + 1.0), n + 1)]) / n
+        G = r["fwd"] - ident
+        tot, cf = R.solver_transport_cost(n, G, r["density"])
+        out.update({f"map{n}_disp": G, f"map{n}_rho": r["density"], f"map{n}_total": np.float64(tot),
+                    f"map{n}_cost": cf})
+    save("costs.npz", **out)
+
+
+def _dyadic(x, maxbits=48):
+    return x.denominator & (x.denominator - 1) == 0 and abs(x.numerator).bit_length() <= maxbits
+
+
+def line_search_roots():
+    """Step choices of the reference's solve_1D_problem (otlib/otsolver_2dgrid.cpp:
+    541-606: stationary points of the quartic e(t) from the EigenSolver of the
+    3 x 3 companion matrix, real (imag() == 0) roots in [-10, 10] of least e,
+    else 0.5) on constructed inputs whose derivative cubic has a double root
+    t0 (exact, or split by a relative perturbation of 2^-40 .. 2^-48 into two
+    close real roots or a complex pair of tiny imaginary part), the third root
+    inside or outside [-10, 10].  n = 4 grids with dyadic g0, direction and
+    residual, chosen so that every dot product of z is exact in FP64 (checked
+    with rationals): the recorded z is the one the reference used."""
+    from fractions import Fraction as Fr
+    import itertools
+
+    n, ek = 4, 2.0
+    F, V = n * n, (n + 1) ** 2
+    rng = np.random.default_rng(77)
+    cases = []
+    for trial in range(4000):
+        g0 = rng.integers(-4, 5, 2 * V) / 8.0 * float(2 ** rng.integers(0, 9)) * (-1) ** trial
+        d = rng.integers(-2, 3, F) / 8.0
+        a, b = R.params_1d(n, g0, d)
+        fa, fb = [Fr(x) for x in a], [Fr(x) for x in b]
+        z3, z4 = 2 * sum(x * y for x, y in zip(fa, fb)), sum(x * x for x in fa)
+        if z4 == 0:
+            continue
+        pair = next(((k, m) for k, m in itertools.combinations(range(F), 2) if fa[k] * fb[m] != fa[m] * fb[k]), None)
+        if pair is None:
+            continue
+        k, m = pair
+        det = fa[k] * fb[m] - fa[m] * fb[k]
+        bb = sum(x * x for x in fb)
+        for t0n in rng.permutation(np.arange(-40, 41))[:12]:
+            t0 = Fr(int(t0n), 8)
+            t1 = -3 * z3 / (4 * z4) - 2 * t0
+            z2, z1 = 2 * z4 * (t0 ** 2 + 2 * t0 * t1), -4 * z4 * t0 ** 2 * t1
+            ta, tb = (z2 - bb) / 2, z1 / 2
+            rk, rm = (ta * fb[m] - tb * fa[m]) / det, (tb * fa[k] - ta * fb[k]) / det
+            for bits, jit in ((None, 0), (40, 1), (44, -1), (48, 1), (48, -1)):
+                if bits is None:
+                    if not (_dyadic(rk) and _dyadic(rm)):
+                        continue
+                    vk, vm = rk, rm
+                else:
+                    def rnd(x):
+                        if x == 0:
+                            return Fr(0)
+                        q = Fr(2) ** (int(np.floor(np.log2(abs(float(x))))) - bits)
+                        return (round(x / q) + jit) * q
+                    vk, vm = rnd(rk), rnd(rm)
+                r = np.zeros(F)
+                r[k], r[m] = float(vk), float(vm)
+                zx = [Fr(ek) / F, 2 * sum(x * Fr(y) for x, y in zip(fb, r)),
+                      bb + 2 * sum(x * Fr(y) for x, y in zip(fa, r)), z3, z4]
+                zf = np.array([float(v) for v in zx])
+                if [Fr(v) for v in zf] != zx:
+                    continue
+                res = R.solve_1d(n, g0, np.zeros(F), d, r, ek)
+                w = [zx[1], 2 * zx[2], 3 * zx[3], 4 * zx[4]]
+                disc = (18 * w[3] * w[2] * w[1] * w[0] - 4 * w[2] ** 3 * w[0] + w[2] ** 2 * w[1] ** 2
+                        - 4 * w[3] * w[1] ** 3 - 27 * w[3] ** 2 * w[0] ** 2)
+                cases.append((zf, res["alpha"], res["e"], int(np.sign(disc)), float(t0), float(t1),
+                              np.concatenate([g0, d, r])))
+        if len(cases) >= 160:
+            break
+    out = dict(z=np.array([c[0] for c in cases]), alpha=np.array([c[1] for c in cases]),
+               e=np.array([c[2] for c in cases]), disc_sign=np.array([c[3] for c in cases]),
+               t0=np.array([c[4] for c in cases]), t1=np.array([c[5] for c in cases]),
+               inputs=np.array([c[6] for c in cases]), n=np.int64(n), ek=np.float64(ek))
+    save("line_search_roots.npz", **out)
+
+
+def edge_cases():
+    """The reference's input edge cases (otlib/otsolver_2dgrid.cpp):
+      zero64   zero-density cells with the default max_ratio (DBL_MAX): the
+               ratio is infinite and the shift beta' = max / DBL_MAX (:58-63)
+      nan16    a NaN density: the initial residual is NaN, `residual >
+               threshold` is false, 0 iterations and the identity map (:160)
+      zdir16   constant density with threshold -1: r = 0, d_hat = 0 and the
+               early break |d_hat| <= 2 DBL_MIN in the first iteration
+               (:182-183)"""
+    out = {}
+    n = 64
+    rho = D.gaussian_mixture(n, 21)
+    rho[(np.arange(n * n) % 7) == 0] = 0.0
+    cases = [("zero64", n, rho, {}),
+             ("nan16", 16, np.where(np.arange(256) == 5, np.nan, D.gaussian_mixture(16, 3)), {}),
+             ("zdir16", 16, np.ones(256), {"threshold": -1.0})]
+    for tag, n, rho, kw in cases:
+        r = R.solve(n, rho, **kw)
+        out.update({f"{tag}_density": rho, f"{tag}_iters": np.int64(r["iters"]), f"{tag}_res0": np.float64(r["res0"]),
+                    f"{tag}_fwd": r["fwd"], f"{tag}_adj": r["density"], f"{tag}_res": r["res"],
+                    f"{tag}_alpha": r["alpha"], f"{tag}_threshold": np.float64(kw.get("threshold", 1e-7))})
+    save("edge_cases.npz", **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["stencils", "small", "inverse", "raster", "cfg0", "cfg1"]
+    which = sys.argv[1:] or ["stencils", "small", "inverse", "raster", "costs", "edge", "roots", "cfg0", "cfg1"]
     if "stencils" in which:
         stencils()
     if "small" in which:
@@ -181,6 +302,12 @@ if __name__ == "__main__":
         inverse_small()
     if "raster" in which:
         raster()
+    if "costs" in which:
+        costs()
+    if "edge" in which:
+        edge_cases()
+    if "roots" in which:
+        line_search_roots()
     if "cfg0" in which:
         big_solve(256, "solve_gaussian256.npz", D.gaussian_mixture(256), 16384)
     if "cfg1" in which:

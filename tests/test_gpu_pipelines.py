@@ -37,6 +37,18 @@ def test_transport_cost_vs_reference_golden(n):
     assert np.allclose(cf, g[f"n{n}_cost_face"], rtol=1e-12, atol=1e-300)
 
 
+@pytest.mark.parametrize("tag", ["rand5", "rand16", "rand33", "map16", "map64"])
+def test_grid_transport_cost_vs_reference_golden(tag):
+    """compute_transport_cost (otlib/otsolver_2dgrid.cpp:608-640) on device
+    against the compiled reference's per-face costs: bit-exact (same
+    operation order), total to the sum order."""
+    g = np.load(os.path.join(GOLDEN, "costs.npz"))
+    n = int(g[f"{tag}_rho"].size ** 0.5)
+    t, cost = grid_transport_cost(n, g[f"{tag}_disp"], g[f"{tag}_rho"])
+    assert np.array_equal(cost, g[f"{tag}_cost"])
+    assert abs(t - float(g[f"{tag}_total"])) <= 1e-13 * abs(t)
+
+
 def test_transport_costs_vs_oracle(oracle):
     n = 64
     tm, o = solved(n, 9, oracle)

@@ -173,3 +173,78 @@ def test_folded_dct_tile_shapes_bit_identical(n, solver_factory):
 def test_unfolded_sizes_report_no_tile_choice(solver_factory):
     s = solver_factory(100)
     assert s.gemm_config()[1] == 0
+
+
+# ------------------------------------------------------------ input edge cases
+@pytest.mark.parametrize("tag,precision", [("zero64", 0), ("zero64", 1), ("nan16", 0), ("zdir16", 0)])
+def test_edge_cases_vs_reference_golden(tag, precision):
+    """otlib/otsolver_2dgrid.cpp edge cases against the compiled reference:
+    zero-density cells with the default max_ratio = DBL_MAX (:58-63; both
+    precision modes), a NaN density (NaN residual: 0 iterations, identity
+    map, :160) and constant density with threshold -1 (r = 0: the early break
+    |d_hat| <= 2 DBL_MIN before the first step, :182-183)."""
+    import os
+
+    from conftest import GOLDEN
+    from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions
+    from otmap_amd._lib import OTM_EXIT_ZERO_DIRECTION
+
+    g = np.load(os.path.join(GOLDEN, "edge_cases.npz"))
+    k = lambda s: g[f"{tag}_{s}"]  # noqa: E731
+    n = int(k("density").size ** 0.5)
+    s = GridBasedTransportSolver(0)
+    s.init(n)
+    tm = s.solve(k("density"), SolverOptions(threshold=float(k("threshold")), precision=Precision(precision)))
+    it = s.last_stats.iterations
+    assert abs(it - int(k("iters"))) <= (0 if tag != "zero64" else 1), (it, int(k("iters")))
+    assert np.abs(tm.fwd_mesh().xy - k("fwd")).max() <= (1e-9 if precision == 0 else 1e-6)
+    assert np.allclose(tm.density(), k("adj"), rtol=1e-14, atol=0, equal_nan=True)
+    if tag == "zdir16":
+        assert s.last_stats.exit_reason == OTM_EXIT_ZERO_DIRECTION
+    if tag == "nan16":
+        assert np.isnan(s.last_stats.initial_residual)
+
+
+# ------------------------------------------- line-search root classification
+def test_line_search_step_choice_vs_reference_golden():
+    """The device's step choice (otm_line_search_steps: the pick_step every
+    solve runs) against the reference's solve_1D_problem on constructed
+    quartics whose derivative cubic has a double root, exact or split by a
+    2^-40..2^-48 relative perturbation into two close real roots or a complex
+    pair of tiny imaginary part (reference otlib/otsolver_2dgrid.cpp:563-595;
+    EigenSolver of the companion matrix, roots with imag() == 0 exactly).
+
+    Bars: a pick of a simple, well separated root (the third root inside
+    [-10, 10]) agrees to 1e-11 relative (the reference's QR iteration loses a
+    few digits on such companion matrices); a pick inside a near-double pair
+    agrees to 1e-5 relative (such roots are determined only to ~sqrt(eps));
+    the classification (a root in [-10, 10] vs the 0.5 fallback) agrees
+    except where the pair is double to within 2^-40 and the third root lies
+    outside [-10, 10] -- there the reference's answer depends on the
+    rounding inside its Schur iteration (it reports one exactly double root
+    pair as complex and one as two reals); at most 3 of the 204 cases."""
+    import os
+
+    from conftest import GOLDEN
+    from otmap_amd import _lib
+
+    g = np.load(os.path.join(GOLDEN, "line_search_roots.npz"))
+    z = np.ascontiguousarray(g["z"])
+    m = len(z)
+    al, em = np.zeros(m), np.zeros(m)
+    assert _lib.lib().otm_line_search_steps(0, _lib.ptr(z), m, _lib.ptr(al), _lib.ptr(em)) == 0
+    ra = g["alpha"]
+    fb_dev, fb_ref = al == 0.5, ra == 0.5
+    outside = np.abs(g["t1"]) > 10
+    diverge = np.flatnonzero(fb_dev != fb_ref)
+    print("classification differs at", [(int(i), int(g["disc_sign"][i]), float(g["t0"][i]), float(ra[i]),
+                                          float(al[i])) for i in diverge])
+    assert np.all(outside[diverge]) and len(diverge) <= 3
+    both = ~fb_dev & ~fb_ref
+    simple = both & ~outside
+    assert np.all(np.abs(al[simple] - ra[simple]) <= 1e-11 * np.abs(ra[simple]))
+    assert np.all(np.abs(al[both] - ra[both]) <= 1e-5 * np.abs(ra[both]))
+    # emin = e(alpha) with the reference's Horner form
+    zz = z.T
+    e_al = zz[0] + al * (zz[1] + al * (zz[2] + al * (zz[3] + al * zz[4])))
+    assert np.array_equal(em, e_al)

@@ -318,3 +318,80 @@ def test_transport_cost_vs_reference(n, oracle):
     c, cf = oracle.transport_cost(n, ident, g[f"n{n}_fwd"], g[f"n{n}_density"], per_face=True)
     assert abs(c - float(g[f"n{n}_cost"])) <= 1e-13 * abs(c)
     assert np.allclose(cf, g[f"n{n}_cost_face"], rtol=1e-12, atol=1e-300)
+
+
+# ------------------------------------------------ compute_transport_cost (solver)
+def _vertex_ij(n):
+    i = np.repeat(np.arange(n + 1.0), n + 1)  # vertex j + i*(n+1)
+    j = np.tile(np.arange(n + 1.0), n + 1)
+    return i, j
+
+
+@pytest.mark.parametrize("tag", ["rand5", "rand16", "rand33", "map16", "map64"])
+def test_solver_transport_cost_vs_reference(tag, oracle):
+    g = load("costs.npz")
+    n = int(g[f"{tag}_rho"].size ** 0.5)
+    tot, cost = oracle.solver_transport_cost(n, g[f"{tag}_disp"], g[f"{tag}_rho"])
+    assert np.array_equal(cost, g[f"{tag}_cost"])
+    assert abs(tot - float(g[f"{tag}_total"])) <= 1e-13 * abs(tot)
+
+
+def test_solver_transport_cost_closed_forms(oracle):
+    """2 x 2 Gauss quadrature of |displacement|^2 over each cell, times rho * A
+    (otlib/otsolver_2dgrid.cpp:608-640): exact for displacements linear in the
+    vertex coordinates.  A uniform displacement c gives rho * A * |c|^2; the
+    field (alpha * x, gamma * y) gives rho * A * (alpha^2 (i^2 + i + 1/3) +
+    gamma^2 (j^2 + j + 1/3)) / n^2 on face (i, j) -- a midpoint rule (1/4 in
+    place of 1/3) or swapped face indices fail it."""
+    n = 12
+    A = 1.0 / (n * n)
+    rho = np.random.default_rng(3).uniform(0.5, 2.0, n * n)
+    V = (n + 1) ** 2
+    c = np.array([0.013, -0.021])
+    G = np.concatenate([np.full(V, c[0]), np.full(V, c[1])])
+    tot, cost = oracle.solver_transport_cost(n, G, rho)
+    assert np.allclose(cost, rho * A * (c @ c), rtol=1e-14, atol=0)
+    assert abs(tot - (rho * A).sum() * (c @ c)) <= 1e-13 * tot
+    vi, vj = _vertex_ij(n)
+    al, ga = 0.03, -0.07
+    G = np.concatenate([al * vi / n, ga * vj / n])
+    tot, cost = oracle.solver_transport_cost(n, G, rho)
+    fi = np.repeat(np.arange(n, dtype=float), n)  # face j + i*n
+    fj = np.tile(np.arange(n, dtype=float), n)
+    expect = rho * A * (al**2 * (fi**2 + fi + 1 / 3) + ga**2 * (fj**2 + fj + 1 / 3)) / n**2
+    assert np.allclose(cost, expect, rtol=1e-13, atol=0)
+
+
+# ------------------------------------------------------------ input edge cases
+@pytest.mark.parametrize("tag", ["zero64", "nan16", "zdir16"])
+def test_edge_cases_vs_reference(tag, oracle):
+    """Zero-density cells with the default max_ratio, a NaN density (0
+    iterations: NaN > threshold is false) and the |d_hat| <= 2 DBL_MIN early
+    break, against the compiled reference."""
+    g = load("edge_cases.npz")
+    k = lambda s: g[f"{tag}_{s}"]  # noqa: E731
+    n = int(k("density").size ** 0.5)
+    o = oracle.solve(n, k("density"), threshold=float(k("threshold")))
+    assert o["iters"] == int(k("iters"))
+    assert np.abs(o["fwd"] - k("fwd")).max() <= 1e-9
+    assert np.allclose(o["density"], k("adj"), rtol=1e-14, atol=0, equal_nan=True)
+    if tag != "zero64":
+        V = (n + 1) ** 2
+        ident = np.concatenate([np.repeat(np.arange(n + 1.0), n + 1), np.tile(np.arange(n + 1.0), n + 1)]) / n
+        assert o["iters"] == 0 and np.array_equal(o["fwd"], ident) and len(ident) == 2 * V
+
+
+def test_line_search_step_choice_vs_reference(oracle):
+    """The oracle's step choice against the reference on near-double-root
+    quartics (tests/golden/line_search_roots.npz): same bars as the device
+    test -- classification equal except near-double pairs whose third root
+    lies outside [-10, 10], simple roots to 1e-11, near-double picks to 1e-5."""
+    g = load("line_search_roots.npz")
+    al = np.array([oracle.pick_step(z)[0] for z in g["z"]])
+    ra = g["alpha"]
+    outside = np.abs(g["t1"]) > 10
+    diverge = np.flatnonzero((al == 0.5) != (ra == 0.5))
+    assert np.all(outside[diverge]) and len(diverge) <= 3
+    both = (al != 0.5) & (ra != 0.5)
+    assert np.all(np.abs(al[both & ~outside] - ra[both & ~outside]) <= 1e-11 * np.abs(ra[both & ~outside]))
+    assert np.all(np.abs(al[both] - ra[both]) <= 1e-5 * np.abs(ra[both]))
