@@ -6,7 +6,8 @@
 // Differences a caller can observe:
 //   * solve() takes the density as (pointer, size) or any container with
 //     data()/size() (std::vector<double>, Eigen::VectorXd); no Eigen needed;
-//   * SolverOptions carries two extra fields, `precision` and `refine_steps`;
+//   * SolverOptions carries three extra fields, `precision`, `refine_steps`
+//     and `mixed_min_n`;
 //   * errors of the HIP layer are reported on std::cerr and leave an empty
 //     map (the reference has no error channel: it prints and continues).
 #pragma once
@@ -35,6 +36,7 @@ struct SolverOptions {
   double max_ratio = std::numeric_limits<double>::max();
   Precision precision = Precision::FP64;
   int refine_steps = 1;
+  int mixed_min_n = 1024;  // mixed: grids from this size up run the BF16x3 transform
 };
 
 // otlib/otsolver_2dgrid.h:45-58
@@ -71,6 +73,7 @@ class GridBasedTransportSolver {
     o.max_ratio = opt.max_ratio;
     o.precision = opt.precision == Precision::Mixed ? OTM_PRECISION_MIXED : OTM_PRECISION_FP64;
     o.refine_steps = opt.refine_steps;
+    o.mixed_min_n = opt.mixed_min_n;
     auto fwd = std::make_shared<GridMesh>(GridMesh::regular(n));
     auto dens = std::make_shared<std::vector<double>>(size_t(n) * n);
     if (otm_solve(h_, density, &o, fwd->xy().data(), dens->data(), &last_) != OTM_OK) {

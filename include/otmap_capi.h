@@ -51,6 +51,9 @@ typedef struct otm_options {
   double max_ratio;  /* default DBL_MAX: cap of max/min of the target density */
   int precision;     /* OTM_PRECISION_FP64 (default, the drop-in) or OTM_PRECISION_MIXED */
   int refine_steps;  /* mixed only: FP64 refinement steps per Laplacian solve (default 1, 0..8) */
+  int mixed_min_n;   /* mixed only: grids with n >= mixed_min_n (and n % 64 == 0) run the BF16x3
+                        transform, smaller grids the FP64 transform (default 1024; 0 = BF16x3 for
+                        every supported size) */
 } otm_options;
 
 typedef struct otm_stats {

@@ -25,7 +25,8 @@ OTM_EXIT_NONE, OTM_EXIT_CONVERGED_OR_MAX_ITER, OTM_EXIT_ZERO_DIRECTION = 0, 1, 2
 
 class OtmOptions(C.Structure):
     _fields_ = [("beta", C.c_int), ("max_iter", C.c_int), ("threshold", C.c_double),
-                ("max_ratio", C.c_double), ("precision", C.c_int), ("refine_steps", C.c_int)]
+                ("max_ratio", C.c_double), ("precision", C.c_int), ("refine_steps", C.c_int),
+                ("mixed_min_n", C.c_int)]
 
 
 class OtmStats(C.Structure):

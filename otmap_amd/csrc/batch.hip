@@ -9,7 +9,8 @@
 // order), so a batched result is bit-identical to the same density solved
 // alone; jobs that converge early skip the remaining iterations.  Both
 // precision modes run batched (mixed: the folded BF16x3 transform with FP64
-// refinement for every grid).
+// refinement for the grids from mixed_min_n up, the FP64 transform for the
+// others, in the same launches).
 
 #include <cstring>
 #include <map>
@@ -30,7 +31,7 @@ struct otm_batch {
   Driver drv;
   std::vector<double*> d_in, d_xy;  // staging of host-API buffers
   int trace_cap = 0;
-  bool mixed_tuned = false;  // mixed tile widths picked (first mixed solve)
+  std::vector<int> mixed_tuned_route;  // routes for which the mixed tile widths were picked
 };
 
 namespace {
@@ -70,15 +71,19 @@ void build_table(otm_batch* b) {
       jobs[p].ck = it->second.ck;
     }
   }
-  // the tile assignment survives a rebuild (same jobs, same order)
+  // the tile assignment and the routes survive a rebuild (same jobs, same order)
   const bool had = b->js.jobs != nullptr;
   const auto jcfg = b->js.jcfg;
   const auto jmcfg = b->js.jmcfg;
+  const auto jroute = b->js.jroute;
+  const int routed = b->js.routed_mixed;
   jobset_destroy(b->js);
   jobset_create(b->js, jobs);
   if (had) {
     b->js.jcfg = jcfg;
     b->js.jmcfg = jmcfg;
+    b->js.jroute = jroute;
+    b->js.routed_mixed = routed;
     jobset_assign(b->js);
   }
 }
@@ -104,20 +109,25 @@ int solve_batch(otm_batch* b, const double* const* d_density, const otm_options*
     b->trace_cap = cap;
     rebuild = true;
   }
-  if (opt.precision == OTM_PRECISION_MIXED && b->mplans.empty()) {
-    for (Fields& f : b->f) fields_enable_mixed(f);
-    for (auto& kv : b->plans) mixed_plan_create(b->mplans[kv.first], kv.second);
-    rebuild = true;
-  }
-  const bool tune_mixed = opt.precision == OTM_PRECISION_MIXED && b->mplans.size() && !b->mixed_tuned;
+  // each grid's transform: mixed precision runs the BF16x3 transform on the
+  // grids from mixed_min_n up, the FP64 transform on the others
+  std::vector<int> route(b->count, 0);
+  if (opt.precision == OTM_PRECISION_MIXED)
+    for (int p = 0; p < b->count; ++p) route[p] = mixed_route(b->n[p], opt.mixed_min_n);
+  for (int p = 0; p < b->count; ++p)
+    if (route[p] && !b->f[p].x0) {  // mixed buffers and BF16 factor planes on first use
+      fields_enable_mixed(b->f[p]);
+      if (!b->mplans.count(b->n[p])) mixed_plan_create(b->mplans[b->n[p]], b->plans.at(b->n[p]));
+      rebuild = true;
+    }
   if (rebuild) {
     build_table(b);
     driver_invalidate(b->drv);
   }
-  if (tune_mixed) {
+  jobset_route(b->js, route);
+  if (b->js.routed_mixed > 0 && b->mixed_tuned_route != route) {
     mixed_tune(b->js, b->device, b->drv.stream);
-    b->mixed_tuned = true;
-    driver_invalidate(b->drv);
+    b->mixed_tuned_route = route;
   }
   for (int p = 0; p < b->count; ++p) {
     Job& J = b->js.host[p];
@@ -209,7 +219,7 @@ int otm_batch_mixed_config(const otm_batch* b, int job, int* cfg) {
   if (!b || !cfg) { set_error("null argument"); return OTM_ERR_ARG; }
   if (job < -1 || job >= b->count) { set_error("job index out of range"); return OTM_ERR_ARG; }
   for (int st = 0; st < 4; ++st) cfg[st] = job < 0 ? b->js.mcfg[st] : b->js.jmcfg[job][st];
-  return b->mixed_tuned ? NMIXED : 0;
+  return b->mixed_tuned_route.empty() ? 0 : NMIXED;
 }
 
 int otm_batch_solve_device(otm_batch* b, const double* const* d_density, const otm_options* opt,

@@ -850,6 +850,7 @@ void launch_mstage(const JobSet& js, bool gate, int fin, hipStream_t s) {
 }
 
 void launch_mresid(const JobSet& js, bool gate, hipStream_t s) {
+  if (js.total[KIND_MRES] == 0) return;
   hipLaunchKernelGGL(k_mresid, dim3(unsigned(js.total[KIND_MRES])), dim3(NT), 0, s, (const Job*)js.jobs,
                      (const int*)js.starts_of(KIND_MRES), js.count, gate);
   OTM_HIP(hipGetLastError());
@@ -858,8 +859,9 @@ void launch_mresid(const JobSet& js, bool gate, hipStream_t s) {
 }  // namespace
 
 void launch_mprep(const JobSet& js, bool gate, hipStream_t s) {
-  hipLaunchKernelGGL(k_mprep, dim3(unsigned(js.total[KIND_FLAT])), dim3(NT), 0, s, (const Job*)js.jobs,
-                     (const int*)js.starts_of(KIND_FLAT), js.count, gate);
+  if (js.total[KIND_FLATMX] == 0) return;
+  hipLaunchKernelGGL(k_mprep, dim3(unsigned(js.total[KIND_FLATMX])), dim3(NT), 0, s, (const Job*)js.jobs,
+                     (const int*)js.starts_of(KIND_FLATMX), js.count, gate);
   OTM_HIP(hipGetLastError());
 }
 
@@ -958,16 +960,19 @@ const std::array<MStageFn, NMIXED> MSTAGE_TABLE[4] = {mstage_row<M1>(), mstage_r
 }  // namespace
 
 // Times each valid tile configuration of every mixed stage over a table of
-// the jobs of each grid size (ungated; M4 writes only the x0 scratch) and
-// assigns the fastest to those jobs.  Results are cached per device, grid
+// the jobs of each grid size on the mixed transform (ungated; M4 writes only
+// the x0 scratch) and assigns the fastest to those jobs.  Results are cached per device, grid
 // size and number of jobs of that size.
 void mixed_tune(JobSet& js, int device, hipStream_t s) {
-  for (const Job& J : js.host)
-    if (!J.Pp || !mixed_supported(J.f.n)) return;
   static std::mutex mu;
   static std::map<std::array<int, 3>, std::array<int, 4>> cache;
-  std::map<int, std::vector<int>> by_size;
-  for (int p = 0; p < js.count; ++p) by_size[js.host[p].f.n].push_back(p);
+  std::map<int, std::vector<int>> by_size;  // jobs on the mixed transform, by grid size
+  for (int p = 0; p < js.count; ++p)
+    if (js.jroute[p] == 1) {
+      if (!js.host[p].Pp || !mixed_supported(js.host[p].f.n)) return;
+      by_size[js.host[p].f.n].push_back(p);
+    }
+  if (by_size.empty()) return;
   for (const auto& kv : by_size) {
     const int n = kv.first;
     const std::array<int, 3> key = {device, n, int(kv.second.size())};
@@ -1003,7 +1008,7 @@ void mixed_tune(JobSet& js, int device, hipStream_t s) {
 
 // d_hat = M^+ (r - S e0) of every job: mixed transform, then `refine` FP64
 // refinement steps x += transform(r' - M0 x); the last stage leaves the
-// |d_hat|^2 partials in SITE_DNORM (launch kind KIND_DNORMMX).
+// |d_hat|^2 partials in SITE_DNORM (launch kind KIND_DNORM of these jobs).
 // prep = false: the folded planes of r' are already in place (the update
 // pass of the previous iteration wrote them).
 void launch_mixed_solve(const JobSet& js, int refine, bool gate, hipStream_t s, bool prep) {

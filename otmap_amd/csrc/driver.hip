@@ -114,33 +114,26 @@ void driver_destroy(Driver& d) {
   d = Driver();
 }
 
-// launch kind of the last Laplacian-solve stage (writer of the |d_hat|^2 partials)
-int dnorm_kind(const JobSet& js, bool mixed) {
-  if (mixed) return KIND_DNORMMX;
-  return js.fold ? KIND_DNORM64 : KIND_PLAIN64;
-}
-
+// One iteration of every job: each job's Laplacian solve on its route (jobs
+// on the mixed transform find their folded right-hand side written by the
+// previous update pass), then the shared beta, line-search and update passes.
 void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s) {
-  const bool mixed = opt.precision == OTM_PRECISION_MIXED;
-  // mixed: the update pass writes the next iteration's folded right-hand side
-  if (mixed)
-    launch_mixed_solve(js, opt.refine_steps, true, s, false);
-  else
-    launch_dct_solve(js, plan, s);
-  launch_beta(js, dnorm_kind(js, mixed), s);
+  if (js.routed_mixed < js.count) launch_dct_solve(js, plan, s);
+  if (js.routed_mixed > 0) launch_mixed_solve(js, opt.refine_steps, true, s, false);
+  launch_beta(js, s);
   launch_dir_coef(js, s);
-  launch_update(js, s, mixed);
+  launch_update(js, s);
 }
 
 int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& opt, float* ms) {
   hipStream_t s = d.stream;
   // the captured iteration depends on the precision mode and the tile shapes
-  const long key = long(opt.precision) | (long(opt.refine_steps) << 4) | (js.version << 8);
+  const long key = long(opt.refine_steps) | (js.version << 8);
   if (d.exec && d.key != key) driver_invalidate(d);
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));
   launch_start(js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, s);
-  if (opt.precision == OTM_PRECISION_MIXED) launch_mprep(js, true, s);  // right-hand side of the first iteration
+  launch_mprep(js, true, s);  // jobs on the mixed transform: right-hand side of the first iteration
   if (!d.exec) {
     OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {

@@ -168,6 +168,10 @@ constexpr int MIXED_TM[NMIXED] = {64, 64, 128, 128};
 
 inline int mres_tiles(int n) { return (n / 32) * (n / 64); }
 inline int upair_tiles(int n) { return tiles_i(n) * ((tiles_j(n) + 1) / 2); }
+// Every job runs its Laplacian solve through one of two transforms, its
+// route: the FP64 transform (route 0) or the mixed BF16x3 transform with FP64
+// refinement (route 1).  Launch kinds of one transform cover only the jobs
+// routed to it.
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
 // fold_kind(s, c) / mixed_kind(s, c): stage s of the folded FP64 / mixed
 // transform with tile configuration c, over the jobs assigned configuration
@@ -176,11 +180,12 @@ enum LaunchKind {
   KIND_STENCIL = 0,
   KIND_FLAT = 1,
   KIND_PLAIN64 = 2,
-  KIND_MRES = 3,     // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
-  KIND_UPAIR = 4,    // update pass of the mixed form: one workgroup per stencil tile and its column mirror
-  KIND_DNORM64 = 5,  // each job's workgroups in its last folded FP64 stage (writers of the |d_hat|^2 partials)
-  KIND_DNORMMX = 6,  // the same for the last mixed stage
-  KIND_M0 = 7,
+  KIND_MRES = 3,       // refinement right-hand side: one workgroup per 32 rows x 32 folded columns
+  KIND_UPAIR = 4,      // update pass of the mixed form: one workgroup per stencil tile and its column mirror
+  KIND_DNORM = 5,      // each job's workgroups in the last stage of its transform (writers of the |d_hat|^2 partials)
+  KIND_STENCIL64 = 6,  // stencil tiles of the jobs on the FP64 transform (their plain update pass)
+  KIND_FLATMX = 7,     // grid-stride blocks of the jobs on the mixed transform (first right-hand side)
+  KIND_M0 = 8,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
@@ -203,6 +208,8 @@ struct JobSet {
   bool tuning = false;     // autotuner table: every valid configuration covers every job
   std::vector<std::array<int, 4>> jcfg;   // per job: folded FP64 tile shape per stage
   std::vector<std::array<int, 4>> jmcfg;  // per job: mixed tile configuration per stage
+  std::vector<int> jroute;                // per job: 0 FP64 transform, 1 mixed transform
+  int routed_mixed = 0;                   // jobs on the mixed transform
   int cfg[4] = {FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT};  // of the largest grid (reported)
   int mcfg[4] = {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT};
   long version = 0;        // unique per table and assignment (captured graphs depend on it)
@@ -212,18 +219,24 @@ struct JobSet {
 // tuning = false: every job starts with the default configurations
 void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning = false);
 void jobset_assign(JobSet& js);  // after changing jcfg / jmcfg: new prefix sums, uploaded (synchronous)
+// sets the transform route of every job (reassigns the table when it changes)
+void jobset_route(JobSet& js, const std::vector<int>& route);
+// Route of a grid of size n in the mixed precision mode: the BF16x3
+// transform from mixed_min_n up (and only where it is supported), the FP64
+// transform below, where it is both faster and exact.
+inline int mixed_route(int n, int mixed_min_n) { return n % 64 == 0 && n >= 64 && n >= mixed_min_n ? 1 : 0; }
 void jobset_upload(JobSet& js, hipStream_t s);  // after changing js.host (same grid sizes)
 void jobset_destroy(JobSet& js);
 
 // Stencil kernel launchers (stencil.hip), one launch per step for all jobs
 void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s);
-// k_beta + scalar kernel; dkind: launch kind whose workgroups wrote the
-// |d_hat|^2 partials (the last stage of the Laplacian solve)
-void launch_beta(const JobSet& js, int dkind, hipStream_t s);
+// k_beta + scalar kernel; the |d_hat|^2 partials of every job were written
+// by the workgroups of the last stage of its transform (KIND_DNORM)
+void launch_beta(const JobSet& js, hipStream_t s);
 void launch_dir_coef(const JobSet& js, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
-// fold = true (mixed form, every n % 64 == 0): the update pass also writes
-// the folded BF16 planes of the next right-hand side
-void launch_update(const JobSet& js, hipStream_t s, bool fold = false);
+// jobs on the mixed transform run the folding update pass, which also writes
+// the folded BF16 planes of their next right-hand side
+void launch_update(const JobSet& js, hipStream_t s);
 void launch_finish(const JobSet& js, hipStream_t s);     // forward positions, adjusted density copy
 void launch_count_open(const JobSet& js, int* d_open, hipStream_t s);  // jobs whose state is not done
 void launch_fill(double* p, long count, double v, hipStream_t s);
@@ -336,7 +349,6 @@ struct Driver {
 void driver_create(Driver& d);
 void driver_destroy(Driver& d);
 void driver_invalidate(Driver& d);  // drop the captured graph (tables or shapes changed)
-int dnorm_kind(const JobSet& js, bool mixed);
 // one solve of every job of the table (js.host[p].dens_in / fwd_out / adj_out
 // set by the caller; for mixed precision every job has its mixed buffers and
 // plan); returns an OTM_* code, *ms = device time

@@ -84,7 +84,8 @@ void allocate(otm_solver* s, int n) {
   s->n = n;
 }
 
-// mixed-precision buffers and BF16 factor planes, on first use
+// mixed-precision buffers and BF16 factor planes, on first use (the tile
+// configurations are tuned with the job on the mixed transform)
 void ensure_mixed(otm_solver* s) {
   if (s->f.x0) return;
   fields_enable_mixed(s->f);
@@ -95,7 +96,10 @@ void ensure_mixed(otm_solver* s) {
   J.Ptp = s->mplan.Ptp;
   J.ck = s->mplan.ck;
   jobset_upload(s->js, s->drv.stream);
+  const std::vector<int> route = s->js.jroute;
+  jobset_route(s->js, {1});
   mixed_tune(s->js, s->device, s->drv.stream);
+  jobset_route(s->js, route);
   driver_invalidate(s->drv);
 }
 
@@ -131,17 +135,16 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
     set_error("unknown precision mode");
     return OTM_ERR_ARG;
   }
+  int route = 0;
   if (opt.precision == OTM_PRECISION_MIXED) {
-    if (!mixed_supported(s->n)) {
-      set_error("mixed precision needs a grid size that is a multiple of 64");
-      return OTM_ERR_ARG;
-    }
     if (opt.refine_steps < 0 || opt.refine_steps > 8) {
       set_error("refine_steps must be in [0, 8]");
       return OTM_ERR_ARG;
     }
-    ensure_mixed(s);
+    route = mixed_route(s->n, opt.mixed_min_n);
+    if (route) ensure_mixed(s);
   }
+  jobset_route(s->js, {route});
   if (fields_grow_trace(s->f, trace_cap_for(opt.max_iter))) driver_invalidate(s->drv);
   Job& J = s->js.host[0];
   J.f = s->f;
@@ -175,6 +178,7 @@ void otm_options_init(otm_options* opt) {
   opt->max_ratio = DBL_MAX;
   opt->precision = OTM_PRECISION_FP64;
   opt->refine_steps = 1;
+  opt->mixed_min_n = 1024;
 }
 
 const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -344,6 +348,8 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   jobset_create(pj, {J});
   pj.jcfg = s->js.jcfg;
   pj.jmcfg = s->js.jmcfg;
+  pj.jroute = {mixed ? 1 : 0};
+  pj.routed_mixed = mixed ? 1 : 0;
   jobset_assign(pj);
   SolveState tpl;
   std::memset(&tpl, 0, sizeof(tpl));
@@ -362,7 +368,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   auto launch = [&]() {
     switch (kernel) {
       case OTM_KERNEL_DCT_GEMM: launch_dct_gemm_stage(pj, &s->plan, s->drv.stream); break;
-      case OTM_KERNEL_BETA: launch_beta(pj, dnorm_kind(pj, false), s->drv.stream); break;
+      case OTM_KERNEL_BETA: launch_beta(pj, s->drv.stream); break;
       case OTM_KERNEL_DIR_COEF: launch_dir_coef(pj, s->drv.stream); break;
       case OTM_KERNEL_UPDATE: launch_update(pj, s->drv.stream); break;
       case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(pj, &s->plan, s->drv.stream); break;
@@ -426,6 +432,7 @@ int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat) {
   if (!r || !d_hat) { set_error("null pointer"); return OTM_ERR_ARG; }
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
+  jobset_route(s->js, {0});
   const Fields& f = s->f;
   double sum = 0.;
   for (long k = 0; k < f.F; ++k) sum += r[k];
@@ -449,6 +456,7 @@ int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int
   OTM_GUARD_BEGIN
   OTM_HIP(hipSetDevice(s->device));
   ensure_mixed(s);
+  jobset_route(s->js, {1});
   const Fields& f = s->f;
   double sum = 0.;
   for (long k = 0; k < f.F; ++k) sum += r[k];

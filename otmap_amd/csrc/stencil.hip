@@ -1034,12 +1034,13 @@ void launch_start(const JobSet& js, double max_ratio, int max_iter, double thres
   OTM_HIP(hipGetLastError());
 }
 
-// dkind: launch kind whose workgroups wrote the |d_hat|^2 partials
-void launch_beta(const JobSet& js, int dkind, hipStream_t s) {
+// the |d_hat|^2 partials of each job were written by the workgroups of the
+// last stage of its transform (KIND_DNORM)
+void launch_beta(const JobSet& js, hipStream_t s) {
   hipLaunchKernelGGL(k_beta, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scalar_beta, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
-                     (const int*)js.starts_of(dkind));
+                     (const int*)js.starts_of(KIND_DNORM));
   OTM_HIP(hipGetLastError());
 }
 
@@ -1050,12 +1051,17 @@ void launch_dir_coef(const JobSet& js, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
-void launch_update(const JobSet& js, hipStream_t s, bool fold) {
-  if (fold)
+// jobs on the FP64 transform: plain update; jobs on the mixed transform: the
+// update that also folds and splits their next right-hand side
+void launch_update(const JobSet& js, hipStream_t s) {
+  if (js.total[KIND_STENCIL64] > 0) {
+    hipLaunchKernelGGL(k_update, grid_of(js, KIND_STENCIL64), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL64));
+    OTM_HIP(hipGetLastError());
+  }
+  if (js.total[KIND_UPAIR] > 0) {
     hipLaunchKernelGGL(k_update_fold, grid_of(js, KIND_UPAIR), dim3(NT), 0, s, JOB_PASS(KIND_UPAIR));
-  else
-    hipLaunchKernelGGL(k_update, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
-  OTM_HIP(hipGetLastError());
+    OTM_HIP(hipGetLastError());
+  }
 }
 
 void launch_finish(const JobSet& js, hipStream_t s) {
@@ -1092,22 +1098,28 @@ std::vector<int> job_starts(JobSet& js) {
       const int n = J.f.n;
       const auto& fc = js.jcfg[p];
       const auto& mc = js.jmcfg[p];
+      // an autotuner table covers every job with every transform
+      const bool r64 = js.tuning || js.jroute[p] == 0, rmx = js.tuning || js.jroute[p] == 1;
       int v = 0;
       if (kind == KIND_STENCIL) v = num_tiles(n);
       else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
-      else if (kind == KIND_PLAIN64) v = ((n + 63) / 64) * ((n + 63) / 64);
-      else if (kind == KIND_MRES) v = mixed_supported(n) ? mres_tiles(n) : 0;
-      else if (kind == KIND_UPAIR) v = mixed_supported(n) ? upair_tiles(n) : 0;
-      else if (kind == KIND_DNORM64) v = !js.tuning && fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
-      else if (kind == KIND_DNORMMX)
-        v = !js.tuning && mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
-      else if (kind < KIND_GEMM0) {
+      else if (kind == KIND_STENCIL64) v = r64 ? num_tiles(n) : 0;
+      else if (kind == KIND_FLATMX) v = rmx && mixed_supported(n) ? flat_blocks(J.f.F) : 0;
+      else if (kind == KIND_PLAIN64) v = r64 ? ((n + 63) / 64) * ((n + 63) / 64) : 0;
+      else if (kind == KIND_MRES) v = rmx && mixed_supported(n) ? mres_tiles(n) : 0;
+      else if (kind == KIND_UPAIR) v = rmx && mixed_supported(n) ? upair_tiles(n) : 0;
+      else if (kind == KIND_DNORM) {
+        if (js.tuning) v = 0;
+        else if (js.jroute[p] == 1) v = mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
+        else if (js.fold) v = fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
+        else v = ((n + 63) / 64) * ((n + 63) / 64);
+      } else if (kind < KIND_GEMM0) {
         const int g = kind - KIND_M0, stage = g / NMIXED, c = g % NMIXED;
-        const bool on = mixed_cfg_valid(stage, n, c) && (js.tuning || mc[stage] == c);
+        const bool on = rmx && mixed_cfg_valid(stage, n, c) && (js.tuning || mc[stage] == c);
         v = on ? pad8(mixed_tiles(stage, n, c)) : 0;
       } else {
         const int g = kind - KIND_GEMM0, stage = g / NFOLD, c = g % NFOLD;
-        const bool on = fold_cfg_valid(n, c) && (js.tuning || fc[stage] == c);
+        const bool on = r64 && fold_cfg_valid(n, c) && (js.tuning || fc[stage] == c);
         v = on ? pad8(fold_tiles(n, stage, c)) : 0;
       }
       w[p] = v;
@@ -1152,6 +1164,8 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning) {
   for (const Job& J : jobs) js.fold = js.fold && J.P != nullptr;
   js.jcfg.assign(js.count, {FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT});
   js.jmcfg.assign(js.count, {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT});
+  js.jroute.assign(js.count, 0);
+  js.routed_mixed = 0;
   report_configs(js);
   js.version = next_version();
   const std::vector<int> st = job_starts(js);
@@ -1171,6 +1185,14 @@ void jobset_assign(JobSet& js) {
   OTM_HIP(hipMalloc(&js.starts, st.size() * sizeof(int)));
   OTM_HIP(hipMemcpy(js.starts, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
   js.version = next_version();
+}
+
+void jobset_route(JobSet& js, const std::vector<int>& route) {
+  if (route == js.jroute) return;
+  js.jroute = route;
+  js.routed_mixed = 0;
+  for (int r : route) js.routed_mixed += r;
+  jobset_assign(js);
 }
 
 void jobset_upload(JobSet& js, hipStream_t s) {

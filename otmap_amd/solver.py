@@ -41,6 +41,9 @@ class SolverOptions:
     max_ratio: float = DBL_MAX
     precision: Precision = Precision.FP64
     refine_steps: int = 1  # mixed precision: FP64 refinement steps per Laplacian solve
+    # mixed precision: grids with n >= mixed_min_n (n % 64 == 0) run the BF16x3
+    # transform, smaller ones the FP64 transform (0: BF16x3 on every supported size)
+    mixed_min_n: int = 1024
 
     def to_c(self) -> OtmOptions:
         o = OtmOptions()
@@ -51,6 +54,7 @@ class SolverOptions:
         o.max_ratio = float(self.max_ratio)
         o.precision = int(self.precision)
         o.refine_steps = int(self.refine_steps)
+        o.mixed_min_n = int(self.mixed_min_n)
         return o
 
 

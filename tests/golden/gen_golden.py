@@ -268,6 +268,86 @@ def line_search_roots():
     save("line_search_roots.npz", **out)
 
 
+def _sampled_solve(r, n, count):
+    V = (n + 1) ** 2
+    idx = sample_vertices(V, count)
+    return dict(iters=np.int64(r["iters"]), res0=np.float64(r["res0"]), res=r["res"], alpha=r["alpha"],
+                beta=r["beta"], vidx=idx, fx=r["fwd"][idx], fy=r["fwd"][V + idx],
+                fwd_sum=np.float64(r["fwd"].sum()), fwd_sq=np.float64((r["fwd"] ** 2).sum()),
+                adj_sum=np.float64(r["density"].sum()))
+
+
+def _batch_job(args):
+    p, n = args
+    rho = D.batch_densities(64)[p]
+    r = Reference().solve(n, rho)
+    return p, _sampled_solve(r, n, 1024), r["t_solve"]
+
+
+def cfg2():
+    """The 64-density uneven batch (otmap_amd.densities.batch_densities): one
+    reference solve per density (otlib/otsolver_2dgrid.cpp:106-245)."""
+    from multiprocessing import Pool
+
+    specs = D.batch_specs(64)
+    out = {"n": np.array([n for n, _, _ in specs], dtype=np.int64)}
+    order = sorted(range(64), key=lambda p: -specs[p][0])  # largest first
+    with Pool(6) as pool:
+        for p, d, t in pool.imap_unordered(_batch_job, [(p, specs[p][0]) for p in order]):
+            print(f"  batch density {p} (n = {specs[p][0]}): {int(d['iters'])} iterations, {t:.1f} s", flush=True)
+            out.update({f"d{p}_{k}": v for k, v in d.items()})
+    save("batch64.npz", **out)
+
+
+def cfg3(count=4, n=512, steps=4):
+    """The bilinear barycentre pipeline of apps/barycenters.cpp:167-290 on the
+    four 512^2 smoothed-shape densities: forward solves, inverse meshes (the
+    origin grid through apply_inverse_map), bilerps of the inverse meshes at
+    (i/steps, j/steps) and their rasters (2-argument rasterize_image, before
+    the mean normalisation)."""
+    dens = D.barycentre_inputs(n, count=count)
+    V = (n + 1) ** 2
+    ident = np.concatenate([np.repeat(np.arange(n + 1.0), n + 1), np.tile(np.arange(n + 1.0), n + 1)]) / n
+    grid_pts = np.stack([ident[:V], ident[V:]], 1)
+    out = dict(n=np.int64(n), steps=np.int64(steps), means=np.array([d.mean() for d in dens]))
+    invs = []
+    for k, rho in enumerate(dens):
+        r = R.solve(n, rho)
+        out.update({f"s{k}_{key}": v for key, v in _sampled_solve(r, n, 8192).items()})
+        inv, _, _ = R.inverse(n, r["fwd"], r["density"], grid_pts)
+        inv_xy = np.concatenate([inv[:, 0], inv[:, 1]])
+        invs.append(inv_xy)
+        idx = sample_vertices(V, 8192)
+        out.update({f"inv{k}_x": inv_xy[idx], f"inv{k}_y": inv_xy[V + idx], f"inv{k}_sum": np.float64(inv_xy.sum())})
+        print(f"  barycentre density {k}: {r['iters']} iterations", flush=True)
+    rng = np.random.default_rng(3003)
+    pix = rng.choice(n * n, 4096, replace=False)
+    out["pix"] = pix
+    for i in range(steps + 1):
+        for j in range(steps + 1):
+            a, b = i / steps, j / steps
+            m = (1. - a) * (1. - b) * invs[0] + a * (1. - b) * invs[1] + a * b * invs[2] + (1. - a) * b * invs[3]
+            img = R.rasterize(n, m, n, n)
+            out[f"img{i}{j}_pix"] = img.reshape(-1)[pix]
+            out[f"img{i}{j}_sum"] = np.float64(img.sum())
+    save("barycentre4x512.npz", **out)
+
+
+def cfg4(n=2048, nq=16384):
+    """The 2048^2 high-contrast solve (max_ratio 1e4) and the inverse of a
+    sample of its 1M advected points (apps/stippling.cpp:164-197)."""
+    rho = D.hdr_lognormal(n)
+    r = R.solve(n, rho, max_ratio=1e4)
+    print(f"  2048^2: {r['iters']} iterations, init {r['t_init']:.1f} s, solve {r['t_solve']:.1f} s", flush=True)
+    out = _sampled_solve(r, n, 16384)
+    out.update(t_init=np.float64(r["t_init"]), t_solve=np.float64(r["t_solve"]), max_ratio=np.float64(1e4))
+    pts = D.uniform_points(1 << 20, seed=2000 + n)
+    qi = np.arange(0, 1 << 20, (1 << 20) // nq, dtype=np.int64)
+    inv, ff, nh = R.inverse(n, r["fwd"], r["density"], pts[qi])
+    out.update(qidx=qi, inv=inv, first_face=ff, nhits=nh)
+    save("stipple2048.npz", **out)
+
+
 def edge_cases():
     """The reference's input edge cases (otlib/otsolver_2dgrid.cpp):
       zero64   zero-density cells with the default max_ratio (DBL_MAX): the
@@ -293,7 +373,8 @@ def edge_cases():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["stencils", "small", "inverse", "raster", "costs", "edge", "roots", "cfg0", "cfg1"]
+    which = sys.argv[1:] or ["stencils", "small", "inverse", "raster", "costs", "edge", "roots", "cfg0", "cfg1",
+                             "cfg2", "cfg3", "cfg4"]
     if "stencils" in which:
         stencils()
     if "small" in which:
@@ -312,3 +393,9 @@ if __name__ == "__main__":
         big_solve(256, "solve_gaussian256.npz", D.gaussian_mixture(256), 16384)
     if "cfg1" in which:
         big_solve(1024, "solve_shapes1024.npz", D.smoothed_shapes(1024), 16384)
+    if "cfg2" in which:
+        cfg2()
+    if "cfg3" in which:
+        cfg3()
+    if "cfg4" in which:
+        cfg4()

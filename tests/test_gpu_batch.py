@@ -105,7 +105,7 @@ def test_batch_rejects_bad_input():
 def test_batch_mixed_bit_identical_to_single_mixed(oracle):
     sizes = [192, 128, 256]
     dens = [D.gaussian_mixture(n, 70 + k) for k, n in enumerate(sizes)]
-    opt = SolverOptions(precision=Precision.MIXED)
+    opt = SolverOptions(precision=Precision.MIXED, mixed_min_n=0)
     print("mixed batch sizes", sizes, flush=True)
     b = BatchSolver(sizes)
     maps = b.solve(dens, opt)
@@ -126,7 +126,7 @@ def test_batch_tiles_tuned_per_grid_size():
     dens = [D.gaussian_mixture(n, 90 + k) for k, n in enumerate(sizes)]
     b = BatchSolver(sizes)
     for prec in (Precision.FP64, Precision.MIXED):
-        opt = SolverOptions(precision=prec)
+        opt = SolverOptions(precision=prec, mixed_min_n=0)
         maps = b.solve(dens, opt)
         for k, n in enumerate(sizes):
             tm, st, _ = single(n, dens[k], opt)

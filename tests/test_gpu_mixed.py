@@ -1,5 +1,8 @@
 """GPU parity of the mixed-precision form (3-term BF16-split DCT GEMMs on MFMA
 with FP32 accumulation, FP64 refinement; stencils, norms and the quartic in FP64).
+Grids below mixed_min_n (default 1024) run the FP64 transform in the mixed
+mode; the tests that exercise the BF16x3 transform on small grids set
+mixed_min_n = 0 (BF16X3 below).
 
 Bars (the drop-in bar for every result):
   * full solves: forward vertex positions within 1e-6 (unit-square
@@ -14,6 +17,9 @@ from otmap_amd import Precision, SolverOptions
 from otmap_amd import densities as D
 
 pytestmark = pytest.mark.gpu
+
+# the BF16x3 transform at every supported grid size
+BF16X3 = SolverOptions(precision=Precision.MIXED, mixed_min_n=0)
 
 
 def _fft_dct_solve(n, r):
@@ -53,7 +59,7 @@ def test_mixed_solve_matches_oracle(n, seed, oracle, solver_factory):
     rho = D.gaussian_mixture(n, seed)
     s = solver_factory(n)
     o = oracle.solve(n, rho)
-    tm = s.solve(rho, SolverOptions(precision=Precision.MIXED))
+    tm = s.solve(rho, BF16X3)
     st = s.last_stats
     assert abs(st.iterations - o["iters"]) <= 1, (st.iterations, o["iters"])
     assert st.final_residual <= 1e-7
@@ -62,11 +68,18 @@ def test_mixed_solve_matches_oracle(n, seed, oracle, solver_factory):
     assert err < 1e-6, err
 
 
-def test_mixed_rejects_unsupported_size(solver_factory):
-    from otmap_amd._lib import OtmError
-    s = solver_factory(100)
-    with pytest.raises(OtmError):
-        s.solve(np.ones(100 * 100) + 0.1 * np.arange(10000) / 10000, SolverOptions(precision=Precision.MIXED))
+@pytest.mark.parametrize("n", [100, 192])
+def test_mixed_mode_routes_to_fp64_transform(n, solver_factory):
+    """Grids the BF16x3 transform does not support (n % 64 != 0) or that lie
+    below mixed_min_n run the FP64 transform in the mixed mode: the result is
+    the FP64 solve's, bit for bit."""
+    s = solver_factory(n)
+    rho = D.gaussian_mixture(n, 31)
+    ref = s.solve(rho, SolverOptions()).fwd_mesh().xy
+    it = s.last_stats.iterations
+    opt = SolverOptions(precision=Precision.MIXED, mixed_min_n=0 if n == 100 else 1024)
+    assert np.array_equal(s.solve(rho, opt).fwd_mesh().xy, ref)
+    assert s.last_stats.iterations == it
 
 
 @pytest.mark.parametrize("n", [128, 256, 512])
@@ -118,25 +131,28 @@ def test_mixed_tile_width_must_divide_half(n):
             s.set_mixed_config(st, 1)
 
 
-def test_mixed_sensitive_trajectory_needs_second_refinement(oracle, solver_factory):
+def test_mixed_sensitive_trajectory(oracle, solver_factory):
     """Density 29 of the 64-density batched workload (n = 192, 3 Gaussians)
     has a trajectory whose step lengths amplify a perturbation about 3x per
-    iteration from iteration ~17 on: a Laplacian-solve error of 1e-12 (one
-    refinement) ends 1.3e-6 away from the oracle's positions, 4e-14 (two
-    refinements) ends below 1e-8.  The FP64 path stays within 1e-8.  Same
-    iteration count in every mode."""
+    iteration from iteration ~17 on.  The default mixed mode runs the FP64
+    transform at this size and holds the 1e-6 bar; forced onto the BF16x3
+    transform (mixed_min_n = 0), a Laplacian-solve error of 1e-12 (one
+    refinement) ends ~1.3e-6 away from the oracle's positions and 4e-14 (two
+    refinements) below 1e-7.  Same iteration count in every mode."""
     specs = D.batch_specs(64)
     n = specs[29][0]
     rho = D.batch_densities(64)[29]
     o = oracle.solve(n, rho)
     s = solver_factory(n)
     errs = {}
-    for name, opt in (("fp64", SolverOptions()), ("mixed1", SolverOptions(precision=Precision.MIXED)),
-                      ("mixed2", SolverOptions(precision=Precision.MIXED, refine_steps=2))):
+    for name, opt in (("fp64", SolverOptions()), ("mixed", SolverOptions(precision=Precision.MIXED)),
+                      ("bf16x3_1", BF16X3),
+                      ("bf16x3_2", SolverOptions(precision=Precision.MIXED, mixed_min_n=0, refine_steps=2))):
         tm = s.solve(rho, opt)
         assert s.last_stats.iterations == o["iters"], (name, s.last_stats.iterations, o["iters"])
         errs[name] = np.abs(tm.fwd_mesh().xy - o["fwd"]).max()
     print(n, errs)
     assert errs["fp64"] < 1e-7
-    assert errs["mixed2"] < 1e-7
-    assert errs["mixed1"] < 1e-5
+    assert errs["mixed"] < 1e-6
+    assert errs["bf16x3_2"] < 1e-7
+    assert errs["bf16x3_1"] > 1e-7  # the BF16x3 trajectory with one refinement does drift
