@@ -176,12 +176,31 @@ def batched_throughput(count, steps, warmup):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, [s.iterations for s in st]
 
-    t_mixed, iters_mixed = timed(SolverOptions(precision=Precision.MIXED))
-    fwd_mixed = torch.cat(d_out)
-    t_mixed2, iters_mixed2 = timed(SolverOptions(precision=Precision.MIXED, refine_steps=2))
-    fwd_mixed2 = torch.cat(d_out)
-    t_batch, iters = timed(SolverOptions())
-    fwd_fp64 = torch.cat(d_out)
+    # largest sampled vertex error of each grid against the reference's solves
+    gpath = os.path.join(ROOT, "tests", "golden", "batch64.npz")
+    gold = np.load(gpath) if count == 64 and os.path.exists(gpath) else None
+
+    def ref_err():
+        if gold is None:
+            return None
+        worst = 0.0
+        for p, n in enumerate(sizes):
+            V = (n + 1) ** 2
+            idx = torch.from_numpy(gold[f"d{p}_vidx"]).to(dev)
+            xy = d_out[p]
+            worst = max(worst, float((xy[idx].cpu() - torch.from_numpy(gold[f"d{p}_fx"])).abs().max()),
+                        float((xy[V + idx].cpu() - torch.from_numpy(gold[f"d{p}_fy"])).abs().max()))
+        return worst
+
+    legs = {}
+    for name, opt in (("mixed", SolverOptions(precision=Precision.MIXED)),
+                      ("bf16x3_all_sizes", SolverOptions(precision=Precision.MIXED, mixed_min_n=0)),
+                      ("bf16x3_all_sizes_refine2", SolverOptions(precision=Precision.MIXED, mixed_min_n=0,
+                                                                 refine_steps=2)),
+                      ("fp64", SolverOptions())):
+        t, it = timed(opt)
+        legs[name] = dict(t=t, iters=it, fwd=torch.cat(d_out), err=ref_err())
+    t_batch, iters, fwd_fp64 = legs["fp64"]["t"], legs["fp64"]["iters"], legs["fp64"]["fwd"]
 
     # the same densities through single-grid solvers, one handle per size
     solvers = {}
@@ -204,15 +223,16 @@ def batched_throughput(count, steps, warmup):
             "speedup_vs_sequential": round(t_seq / t_batch, 3),
             "iterations": {"min": min(iters), "max": max(iters), "mean": round(float(np.mean(iters)), 1)},
             "create_s": round(t_create, 3), "precision": "fp64",
+            "max_vertex_err_vs_reference": legs["fp64"]["err"],
             "gemm_tiles_by_size": {str(n): b.gemm_config(sizes.index(n))[0] for n in sorted(set(sizes))},
-            "mixed": {"solves_per_s": round(count / t_mixed, 2), "ms_per_batch": round(t_mixed * 1e3, 2),
-                      "iterations_equal_fp64": iters_mixed == iters,
-                      "max_vertex_diff_vs_fp64": float((fwd_mixed - fwd_fp64).abs().max().item()),
-                      "speedup_vs_fp64_batch": round(t_batch / t_mixed, 3),
-                      "mixed_tiles_by_size": {str(n): b.mixed_config(sizes.index(n))[0] for n in sorted(set(sizes))}},
-            "mixed_refine2": {"solves_per_s": round(count / t_mixed2, 2), "ms_per_batch": round(t_mixed2 * 1e3, 2),
-                              "iterations_equal_fp64": iters_mixed2 == iters,
-                              "max_vertex_diff_vs_fp64": float((fwd_mixed2 - fwd_fp64).abs().max().item())}}
+            **{name: {"solves_per_s": round(count / L["t"], 2), "ms_per_batch": round(L["t"] * 1e3, 2),
+                      "iterations_equal_fp64": L["iters"] == iters,
+                      "max_vertex_err_vs_reference": L["err"],
+                      "max_vertex_diff_vs_fp64": float((L["fwd"] - fwd_fp64).abs().max().item()),
+                      "speedup_vs_fp64_batch": round(t_batch / L["t"], 3)}
+               for name, L in legs.items() if name != "fp64"},
+            "mixed_transform_from_n": SolverOptions().mixed_min_n,
+            "mixed_tiles_by_size": {str(n): b.mixed_config(sizes.index(n))[0] for n in sorted(set(sizes))}}
 
 
 def batched_1024(count, steps, warmup, fp64_single_ms):
@@ -329,8 +349,27 @@ def stippling_pipeline(n=2048, points=1 << 20):
     return out
 
 
-def cpu_baseline(n, density):
-    """The compiled reference (oracle/_ref/libotref.so), one solve, 1 thread."""
+def host_cpu():
+    """Model name of the host CPU and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        usable = os.cpu_count()
+    return {"model": model, "nproc": usable, "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(n, density, reps=3):
+    """The compiled reference (oracle/_ref/libotref.so), 1 thread: init(n)
+    once, then `reps` full solves of the same density; the median solve."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
         from oracle import Reference, reference_available
@@ -339,11 +378,11 @@ def cpu_baseline(n, density):
     if not reference_available():
         return {"value": None, "unit": "ms/solve", "cores": 1, "kind": "reference",
                 "sample": "oracle/_ref/libotref.so not built"}
-    R = Reference()
-    r = R.solve(n, density, traced=False)
-    return {"value": round(r["t_solve"] * 1e3, 1), "unit": "ms/solve", "cores": 1, "kind": "reference",
-            "sample": f"one full solve of the same {n}x{n} density (init {r['t_init']:.2f} s excluded), "
-                      f"reference built -O3 -DENABLE_SSE_MODE, SimplicialLDLT (no CHOLMOD), 1 thread"}
+    t_init, ts = Reference().solve_timed(n, density, reps)
+    return {"value": round(float(np.median(ts)) * 1e3, 1), "unit": "ms/solve", "cores": 1, "kind": "reference",
+            "solves_ms": [round(t * 1e3, 1) for t in ts], "host": host_cpu(),
+            "sample": f"median of {reps} full solves of the same {n}x{n} density (one init, {t_init:.2f} s, "
+                      f"excluded), reference built -O3 -DENABLE_SSE_MODE, SimplicialLDLT (no CHOLMOD), 1 thread"}
 
 
 def main():

@@ -4,8 +4,9 @@
 // libotmap_amd.so (include/otmap_capi.h); link with -lotmap_amd.
 //
 // Differences a caller can observe:
-//   * solve() takes the density as (pointer, size) or any container with
-//     data()/size() (std::vector<double>, Eigen::VectorXd); no Eigen needed;
+//   * with Eigen on the include path solve() takes Eigen::Ref<const
+//     Eigen::VectorXd>, as the reference; without it, any container with
+//     data()/size() (std::vector<double>), or (pointer, size);
 //   * SolverOptions carries three extra fields, `precision`, `refine_steps`
 //     and `mixed_min_n`;
 //   * errors of the HIP layer are reported on std::cerr and leave an empty
@@ -59,7 +60,7 @@ class GridBasedTransportSolver {
   }
 
   /** solve for the given density (n*n values, face order j + i*n) */
-  TransportMap solve(const double* density, size_t size, SolverOptions opt = SolverOptions()) {
+  TransportMap solve(const double* density, size_t size, SolverOptions opt) {
     const int n = otm_solver_grid_size(h_);
     if (n <= 0 || size != size_t(n) * size_t(n)) {
       std::cerr << "GridBasedTransportSolver::solve: density size does not match init(n)\n";
@@ -74,19 +75,28 @@ class GridBasedTransportSolver {
     o.precision = opt.precision == Precision::Mixed ? OTM_PRECISION_MIXED : OTM_PRECISION_FP64;
     o.refine_steps = opt.refine_steps;
     o.mixed_min_n = opt.mixed_min_n;
-    auto fwd = std::make_shared<GridMesh>(GridMesh::regular(n));
-    auto dens = std::make_shared<std::vector<double>>(size_t(n) * n);
-    if (otm_solve(h_, density, &o, fwd->xy().data(), dens->data(), &last_) != OTM_OK) {
+    const size_t V = size_t(n + 1) * (n + 1);
+    std::vector<double> xy(2 * V);
+    auto dens = std::make_shared<DensityVector>(long(n) * n);
+    if (otm_solve(h_, density, &o, xy.data(), dens->data(), &last_) != OTM_OK) {
       report("solve");
       return TransportMap();
     }
-    return TransportMap(origin_, fwd, dens);
+    // the origin mesh is shared with every returned map; the forward mesh is a copy
+    return TransportMap(origin_, std::make_shared<GridMesh>(n, xy), dens);
   }
 
+#if OTMAP_HAS_EIGEN
+  TransportMap solve(Eigen::Ref<const Eigen::VectorXd> density, SolverOptions opt = SolverOptions()) {
+    const Eigen::VectorXd d = density;  // contiguous copy of any expression
+    return solve(d.data(), size_t(d.size()), opt);
+  }
+#else
   template <class Vector>
   TransportMap solve(const Vector& density, SolverOptions opt = SolverOptions()) {
     return solve(density.data(), size_t(density.size()), opt);
   }
+#endif
 
   /** iterations, residuals and device time of the last solve */
   const otm_stats& last_stats() const { return last_; }

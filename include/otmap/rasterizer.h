@@ -1,7 +1,7 @@
 // Drop-in for the reference's otlib/utils/rasterizer.h (ggael/otmap):
-// rasterize_image in per-face density mode.  The image is row-major,
-// img[y * cols + x] == the reference's img(y, x).  Header-only over the C ABI
-// of libotmap_amd.so.
+// rasterize_image into an image of the caller's size (img.rows() x
+// img.cols(); Eigen::MatrixXd when Eigen is on the include path), per-face
+// density mode.  Header-only over the C ABI of libotmap_amd.so.
 #pragma once
 
 #include <iostream>
@@ -12,28 +12,43 @@
 
 namespace otmap {
 
+// otlib/utils/rasterizer.h:16-19
 enum RasterImageOption { RIO_PerVertexDensity, RIO_PerFaceDensity };
 
-// otlib/utils/rasterizer.h:21 (density per face; RIO_PerFaceDensity only)
-inline bool rasterize_image(const GridMesh& mesh, const std::vector<double>& density_per_face, int rows, int cols,
-                            std::vector<double>& img, int device = 0) {
-  img.assign(size_t(rows) * cols, 0.);
-  if (otm_rasterize(device, mesh.grid_size(), mesh.xy().data(), density_per_face.data(), rows, cols, img.data()) !=
-      OTM_OK) {
-    std::cerr << "otmap rasterize_image: " << otm_last_error() << "\n";
-    return false;
+namespace detail {
+inline void rasterize(const GridMesh& mesh, const double* density, Image& img, RasterImageOption opt) {
+  const long rows = img.rows(), cols = img.cols();
+  for (long x = 0; x < cols; ++x)
+    for (long y = 0; y < rows; ++y) img(y, x) = 0.;
+  if (opt != RIO_PerFaceDensity) {
+    std::cerr << "otmap rasterize_image: only RIO_PerFaceDensity is supported\n";
+    return;
   }
-  return true;
+  const std::vector<double> xy = mesh.xy();
+  std::vector<double> rm(size_t(rows) * size_t(cols));  // row-major, as the C ABI writes it
+  if (otm_rasterize(0, mesh.grid_size(), xy.data(), density, int(rows), int(cols), rm.data()) != OTM_OK) {
+    report("rasterize_image");
+    return;
+  }
+  for (long x = 0; x < cols; ++x)
+    for (long y = 0; y < rows; ++y) img(y, x) = rm[size_t(y) * size_t(cols) + size_t(x)];
+}
+}  // namespace detail
+
+// otlib/utils/rasterizer.h:21: every pixel covered by face f gets
+// float(1/nf) * density(f) / |area(f)|, the largest covering face wins
+inline void rasterize_image(const GridMesh& mesh, const DensityVector& density_per_face, Image& img,
+                            RasterImageOption opt = RIO_PerFaceDensity) {
+  if (size_t(density_per_face.size()) != size_t(mesh.faces_size())) {
+    std::cerr << "otmap rasterize_image: density size does not match the mesh\n";
+    return;
+  }
+  detail::rasterize(mesh, density_per_face.data(), img, opt);
 }
 
 // otlib/utils/rasterizer.h:22 (unit density per face)
-inline bool rasterize_image(const GridMesh& mesh, int rows, int cols, std::vector<double>& img, int device = 0) {
-  img.assign(size_t(rows) * cols, 0.);
-  if (otm_rasterize(device, mesh.grid_size(), mesh.xy().data(), nullptr, rows, cols, img.data()) != OTM_OK) {
-    std::cerr << "otmap rasterize_image: " << otm_last_error() << "\n";
-    return false;
-  }
-  return true;
+inline void rasterize_image(const GridMesh& mesh, Image& img, RasterImageOption opt = RIO_PerFaceDensity) {
+  detail::rasterize(mesh, nullptr, img, opt);
 }
 
 }  // namespace otmap

@@ -263,6 +263,7 @@ class Reference:
             "ref_rasterize": (None, [_i, _dp, P, _i, _i, _dp]),
             "ref_transport_cost": (_d, [_i, _dp, _dp, _dp, P]),
             "ref_solver_transport_cost": (_d, [_i, _dp, _dp, P]),
+            "ref_solve_timed": (_i, [_i, _dp, _i, P, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -283,6 +284,13 @@ class Reference:
         k = max(iters.value, 0)
         return dict(iters=iters.value, fwd=fwd, density=adj, res0=res0.value, res=rt[:k], alpha=at[:k],
                     beta=bt[:k], t_init=ti.value, t_solve=ts.value)
+
+    def solve_timed(self, n, density, reps):
+        """init(n) once, then `reps` solves of `density`: (init s, [solve s])."""
+        ti = C.c_double()
+        ts = np.zeros(reps)
+        self.L.ref_solve_timed(n, _f64(density), reps, C.byref(ti), ts)
+        return ti.value, ts
 
     def adjust_density(self, n, rho, max_ratio=np.finfo(np.float64).max):
         r = _f64(rho).copy()

@@ -185,6 +185,24 @@ int ref_solve(int n, const double* density, int beta_cj, int max_iter, double th
   return 0;
 }
 
+// Timing of the reference's solve(): one solver, init(n) once, then `reps`
+// solves of the same density (verbose 0); t_solve[k] = wall seconds of solve k.
+int ref_solve_timed(int n, const double* density, int reps, double* t_init, double* t_solve) {
+  MxcsrGuard guard;
+  otmap::GridBasedTransportSolver solver;
+  solver.set_verbose_level(0);
+  double t0 = now_s();
+  solver.init(n);
+  if (t_init) *t_init = now_s() - t0;
+  Map<const VectorXd> dens(density, n * n);
+  for (int k = 0; k < reps; ++k) {
+    t0 = now_s();
+    otmap::TransportMap tm = solver.solve(dens, otmap::SolverOptions());
+    t_solve[k] = now_s() - t0;
+  }
+  return 0;
+}
+
 void ref_adjust_density(int n, double* rho, double max_ratio) {
   MxcsrGuard guard;
   Probe& p = probe(n);

@@ -65,3 +65,33 @@ def test_off_round_trip(tmp_path):
     m.write(path)
     r = GridMesh.read(path)
     assert r.n == n and np.abs(r.xy - m.xy).max() <= 5e-11
+
+
+EIGEN_DIR = "/root/reference/extern"
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(EIGEN_DIR, "Eigen", "Core")),
+                    reason="Eigen headers (the reference's vendored copy) not mounted")
+@pytest.mark.parametrize("src", ["signatures.cpp", "refapi_example.cpp"])
+def test_reference_signatures_compile(src):
+    """The drop-in headers compile callers written against the reference's
+    own declarations (otlib/otsolver_2dgrid.h:52-58, otlib/transport_map.h:
+    29-70, otlib/utils/rasterizer.h:16-22, Surface_mesh points()/position()),
+    with the reference's include layout and Eigen; signatures.cpp binds every
+    entry point to a pointer of the reference's exact type."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-Wno-deprecated-copy",
+                        "-Wno-class-memaccess", "-Wno-int-in-bool-context", "-I", os.path.join(root, "include", "otmap"),
+                        "-isystem", EIGEN_DIR, os.path.join(root, "tests", "cpp", src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_dropin_headers_compile_without_eigen():
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-DOTMAP_NO_EIGEN",
+                        "-I", os.path.join(root, "include"), os.path.join(root, "tests", "cpp", "dropin_example.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
