@@ -245,6 +245,15 @@ int otm_image_normalize_mean_device(int device, long count, const double* d_img,
                                     void* d_out, double* mean);
 int otm_prune_points_device(int device, int n, const double* d_density, const double* d_pts, int64_t count,
                             double* d_out, int64_t* kept);
+/* otm_prune_points_device without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * results; its workgroup counts live in a fixed device array of the library
+ * (32 KB per GPU, held until the process ends), so count is limited to
+ * 16777216 points (larger counts return OTM_ERR_ARG), and calls that overlap
+ * on one GPU race.  Synchronous, like otm_prune_points_device. */
+int otm_prune_points_device_noalloc(int device, int n, const double* d_density, const double* d_pts, int64_t count,
+                                    double* d_out, int64_t* kept);
 
 /* ---- Batched solves -----------------------------------------------------
  * No reference counterpart: the reference solves one density per

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libotmap_amd.so")
 HEADERS = [os.path.join(ROOT, "include", "otmap_capi.h")]
 
 OTM_OK = 0
+OTM_ERR_ARG, OTM_ERR_HIP, OTM_ERR_STATE, OTM_ERR_ALLOC = -1, -2, -3, -4
 OTM_BETA_ZERO, OTM_BETA_CONJUGATE_JACOBIAN = 0, 1
 OTM_PRECISION_FP64, OTM_PRECISION_MIXED = 0, 1
 OTM_EXIT_NONE, OTM_EXIT_CONVERGED_OR_MAX_ITER, OTM_EXIT_ZERO_DIRECTION = 0, 1, 2
@@ -80,6 +81,8 @@ _SIGNATURES = {
     "otm_mesh_lerp_device": (C.c_int, [C.c_int, C.c_long, C.c_int, C.POINTER(_vp), C.c_double, C.c_double, _vp]),
     "otm_image_normalize_mean_device": (C.c_int, [C.c_int, C.c_long, _vp, C.c_double, C.c_int, _vp, _dp]),
     "otm_prune_points_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int64, _vp, C.POINTER(C.c_int64)]),
+    "otm_prune_points_device_noalloc": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int64, _vp,
+                                                  C.POINTER(C.c_int64)]),
     "otm_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _ip]),
     "otm_batch_destroy": (C.c_int, [_vp]),
     "otm_batch_size": (C.c_int, [_vp]),

@@ -12,6 +12,8 @@
 // contraction); sums are deterministic two-pass reductions.
 
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "geometry.h"
@@ -143,6 +145,15 @@ __global__ __launch_bounds__(NT) void k_img_scale(const double* __restrict__ img
 // of its pixel (floor(p * n), row = y, column = x, face order y + x * n) is
 // not zero; order preserved.  Pass 1: flags and per-workgroup counts.
 constexpr int PRUNE_CHUNK = NT * 8;  // points per workgroup
+// the prune predicate of point k (pixel floor(p n), clamped; NaN points dropped)
+__device__ inline bool prune_keep(int n, const double* __restrict__ density, const double* __restrict__ pts, long k) {
+  const double x = pts[2 * k], y = pts[2 * k + 1];
+  int ix = int(floor(x * double(n))), iy = int(floor(y * double(n)));
+  ix = ix < 0 ? 0 : (ix > n - 1 ? n - 1 : ix);
+  iy = iy < 0 ? 0 : (iy > n - 1 ? n - 1 : iy);
+  return !isnan(x) && !isnan(y) && density[long(iy) + long(ix) * n] != 0.;
+}
+
 __global__ __launch_bounds__(NT) void k_prune_count(int n, const double* __restrict__ density,
                                                    const double* __restrict__ pts, long count,
                                                    unsigned char* __restrict__ keep, int* __restrict__ block_count) {
@@ -154,14 +165,10 @@ __global__ __launch_bounds__(NT) void k_prune_count(int n, const double* __restr
   for (int u = threadIdx.x; u < PRUNE_CHUNK; u += NT) {
     const long k = base + u;
     if (k >= count) break;
-    const double x = pts[2 * k], y = pts[2 * k + 1];
     // the reference reads density(floor(y n), floor(x n)); indices are clamped
     // to the grid (a point at exactly 1.0 would read past the image there)
-    int ix = int(floor(x * double(n))), iy = int(floor(y * double(n)));
-    ix = ix < 0 ? 0 : (ix > n - 1 ? n - 1 : ix);
-    iy = iy < 0 ? 0 : (iy > n - 1 ? n - 1 : iy);
-    const bool k_ok = !isnan(x) && !isnan(y) && density[long(iy) + long(ix) * n] != 0.;
-    keep[k] = k_ok ? 1 : 0;
+    const bool k_ok = prune_keep(n, density, pts, k);
+    if (keep) keep[k] = k_ok ? 1 : 0;
     c += k_ok ? 1 : 0;
   }
   if (c) atomicAdd(&total, c);
@@ -181,18 +188,22 @@ __global__ __launch_bounds__(NT) void k_prune_scan(int* block_count, int nb, lon
   kept[0] = acc;
 }
 
-// pass 3: ordered scatter (each workgroup writes its chunk's kept points in order)
+// pass 3: ordered scatter (each workgroup writes its chunk's kept points in
+// order); keep == nullptr: the predicate is evaluated again instead of read
 __global__ __launch_bounds__(NT) void k_prune_scatter(const double* __restrict__ pts, long count,
                                                      const unsigned char* __restrict__ keep,
-                                                     const int* __restrict__ block_offset, double* __restrict__ out) {
+                                                     const int* __restrict__ block_offset, double* __restrict__ out,
+                                                     int n, const double* __restrict__ density) {
   __shared__ int off[NT + 1];
   const long base = long(blockIdx.x) * PRUNE_CHUNK;
   // thread t owns points [base + 8t, base + 8t + 8)
   constexpr int PER = PRUNE_CHUNK / NT;
+  bool mine[PER];
   int c = 0;
   for (int u = 0; u < PER; ++u) {
     const long k = base + long(threadIdx.x) * PER + u;
-    if (k < count && keep[k]) ++c;
+    mine[u] = k < count && (keep ? keep[k] != 0 : prune_keep(n, density, pts, k));
+    c += mine[u] ? 1 : 0;
   }
   off[threadIdx.x + 1] = c;
   __syncthreads();
@@ -204,13 +215,22 @@ __global__ __launch_bounds__(NT) void k_prune_scatter(const double* __restrict__
   long dst = long(block_offset[blockIdx.x]) + off[threadIdx.x];
   for (int u = 0; u < PER; ++u) {
     const long k = base + long(threadIdx.x) * PER + u;
-    if (k < count && keep[k]) {
+    if (mine[u]) {
       out[2 * dst] = pts[2 * k];
       out[2 * dst + 1] = pts[2 * k + 1];
       ++dst;
     }
   }
 }
+
+// Zero-allocation prune: the workgroup counts and the kept total live in a
+// file-scope device array (one copy per GPU, held until the process ends)
+// instead of buffers allocated per call.  Every call on a GPU shares that
+// GPU's array: calls that overlap on one GPU race.  Nothing in it needs to
+// start at zero (pass 1 writes every count it uses, pass 2 the total).
+constexpr long PRUNE_MAX_BLOCKS = 8192;                           // workgroups of one call at most
+constexpr long PRUNE_MAX_POINTS = PRUNE_MAX_BLOCKS * PRUNE_CHUNK;  // 16,777,216 points
+static __device__ __align__(256) long long scratch[PRUNE_MAX_BLOCKS / 2 + 1];  // 8192 int counts, then the total
 
 int red_blocks(long count) { return int(std::max<long>(1, std::min<long>((count + NT - 1) / NT, RB))); }
 
@@ -260,6 +280,20 @@ void grid_transport_cost_device(int n, const double* G, const double* rho, doubl
   OTM_HIP(hipMemcpyAsync(&hv, red.p, sizeof(hv), hipMemcpyDeviceToHost, s));
   OTM_HIP(hipStreamSynchronize(s));
   if (h_total) *h_total = hv;
+}
+
+// device address of this GPU's copy of `scratch` (looked up once per device,
+// with that device current)
+void* scratch_ptr(int device) {
+  static std::mutex mu;
+  static std::map<int, void*> ptrs;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(device);
+  if (it != ptrs.end()) return it->second;
+  void* p = nullptr;
+  OTM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  ptrs[device] = p;
+  return p;
 }
 
 }  // namespace
@@ -377,10 +411,43 @@ int otm_prune_points_device(int device, int n, const double* d_density, const do
     hipLaunchKernelGGL(k_prune_scan, dim3(1), dim3(NT), 0, nullptr, blocks.as<int>(), int(nb), total.as<long>());
     OTM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_prune_scatter, dim3(unsigned(nb)), dim3(NT), 0, nullptr, d_pts, long(count),
-                       (const unsigned char*)flags.as<unsigned char>(), (const int*)blocks.as<int>(), d_out);
+                       (const unsigned char*)flags.as<unsigned char>(), (const int*)blocks.as<int>(), d_out, n,
+                       d_density);
     OTM_HIP(hipGetLastError());
     long k = 0;
     OTM_HIP(hipMemcpy(&k, total.p, sizeof(long), hipMemcpyDeviceToHost));
+    *kept = k;
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return fail(e);
+  }
+}
+
+int otm_prune_points_device_noalloc(int device, int n, const double* d_density, const double* d_pts, int64_t count,
+                                    double* d_out, int64_t* kept) {
+  if (n < 1 || !d_density || !d_pts || !d_out || count < 0 || !kept) { set_error("invalid prune arguments"); return OTM_ERR_ARG; }
+  if (count > PRUNE_MAX_POINTS) {
+    set_error("otm_prune_points_device_noalloc: at most 16777216 points per call (its fixed device scratch); "
+              "use otm_prune_points_device for more");
+    return OTM_ERR_ARG;
+  }
+  if (count == 0) { *kept = 0; return OTM_OK; }
+  try {
+    OTM_HIP(hipSetDevice(device));
+    auto* base = static_cast<int*>(scratch_ptr(device));
+    int* blocks = base;
+    long* total = reinterpret_cast<long*>(base + PRUNE_MAX_BLOCKS);
+    const long nb = (count + PRUNE_CHUNK - 1) / PRUNE_CHUNK;
+    hipLaunchKernelGGL(k_prune_count, dim3(unsigned(nb)), dim3(NT), 0, nullptr, n, d_density, d_pts, long(count),
+                       (unsigned char*)nullptr, blocks);
+    OTM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_prune_scan, dim3(1), dim3(NT), 0, nullptr, blocks, int(nb), total);
+    OTM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_prune_scatter, dim3(unsigned(nb)), dim3(NT), 0, nullptr, d_pts, long(count),
+                       (const unsigned char*)nullptr, (const int*)blocks, d_out, n, d_density);
+    OTM_HIP(hipGetLastError());
+    long k = 0;
+    OTM_HIP(hipMemcpy(&k, total, sizeof(long), hipMemcpyDeviceToHost));
     *kept = k;
     return OTM_OK;
   } catch (const HipError& e) {
