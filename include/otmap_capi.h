@@ -245,6 +245,15 @@ int otm_image_normalize_mean_device(int device, long count, const double* d_img,
                                     void* d_out, double* mean);
 int otm_prune_points_device(int device, int n, const double* d_density, const double* d_pts, int64_t count,
                             double* d_out, int64_t* kept);
+/* generate_blue_noise_tile (apps/common/image_utils.cpp:148-173, used by
+ * apps/stippling.cpp:177-181) on device: the nb-point tile d_tile
+ * (interleaved, coordinates in [0,1)), N = floor(sqrt(nb)) per side,
+ * repeated over ceil(n/N)^2 cells scaled by N/n; the points with both
+ * coordinates <= 1 are written to d_out (room for cap points) in the
+ * reference's order (cell row i, cell column j, tile point k); *count = their
+ * number (also when cap is too small, which returns OTM_ERR_ARG). */
+int otm_blue_noise_tile_device(int device, int n, const double* d_tile, int64_t nb, double* d_out, int64_t cap,
+                               int64_t* count);
 /* otm_prune_points_device without any allocation, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same

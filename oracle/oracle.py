@@ -264,6 +264,8 @@ class Reference:
             "ref_transport_cost": (_d, [_i, _dp, _dp, _dp, P]),
             "ref_solver_transport_cost": (_d, [_i, _dp, _dp, P]),
             "ref_solve_timed": (_i, [_i, _dp, _i, P, _dp]),
+            "ref_load_point_cloud_dat": (_l, [C.c_char_p, _dp, _l]),
+            "ref_blue_noise_tile": (_l, [_i, _dp, _l, _dp, _l]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -284,6 +286,22 @@ class Reference:
         k = max(iters.value, 0)
         return dict(iters=iters.value, fwd=fwd, density=adj, res0=res0.value, res=rt[:k], alpha=at[:k],
                     beta=bt[:k], t_init=ti.value, t_solve=ts.value)
+
+    def load_point_cloud_dat(self, path, cap=1 << 20):
+        out = np.zeros(2 * cap)
+        k = self.L.ref_load_point_cloud_dat(str(path).encode(), out, cap)
+        if k < 0:
+            raise FileNotFoundError(path)
+        return out[: 2 * min(k, cap)].reshape(-1, 2)
+
+    def blue_noise_tile(self, n, tile):
+        t = _f64(tile).reshape(-1)
+        nb = t.size // 2
+        nt = (n + int(np.sqrt(nb)) - 1) // int(np.sqrt(nb))
+        cap = nt * nt * nb
+        out = np.zeros(2 * cap)
+        k = self.L.ref_blue_noise_tile(n, t, nb, out, cap)
+        return out[: 2 * k].reshape(-1, 2)
 
     def solve_timed(self, n, density, reps):
         """init(n) once, then `reps` solves of `density`: (init s, [solve s])."""

@@ -16,6 +16,7 @@
 //   transport_cost                              otlib/transport_map.cpp:131-215
 //   compute_transport_cost                      otlib/otsolver_2dgrid.cpp:608-640
 //   nestdiss_ordering                           otlib/details/nested_dissection.cpp:54-61
+//   load_point_cloud_dat, generate_blue_noise_tile apps/common/image_utils.cpp:148-173,209-231
 
 #include "otsolver_2dgrid.h"
 #include "transport_map.h"
@@ -23,6 +24,7 @@
 #include "utils/mesh_utils.h"
 #include "utils/rasterizer.h"
 #include "details/nested_dissection.h"
+#include "common/image_utils.h"
 
 #include <xmmintrin.h>
 #include <cstring>
@@ -378,6 +380,25 @@ double ref_solver_transport_cost(int n, const double* disp_xy, const double* rho
   p.set_density(nullptr);
   if (cost_per_face) Map<VectorXd>(cost_per_face, n * n) = c;
   return c.sum();
+}
+
+// The reference apps' blue-noise tile reader and tiler: the tile as read
+// from `path` (interleaved, up to cap points; returns the count), and the
+// points of generate_blue_noise_tile(n) from that tile (up to cap; returns
+// the count).
+long ref_load_point_cloud_dat(const char* path, double* out, long cap) {
+  std::vector<Vector2d> pts;
+  if (!load_point_cloud_dat(path, pts)) return -1;
+  for (long k = 0; k < long(pts.size()) && k < cap; ++k) { out[2 * k] = pts[k].x(); out[2 * k + 1] = pts[k].y(); }
+  return long(pts.size());
+}
+
+long ref_blue_noise_tile(int n, const double* tile, long nb, double* out, long cap) {
+  std::vector<Vector2d> t(nb), pts;
+  for (long k = 0; k < nb; ++k) t[k] = Vector2d(tile[2 * k], tile[2 * k + 1]);
+  generate_blue_noise_tile(n, pts, t);
+  for (long k = 0; k < long(pts.size()) && k < cap; ++k) { out[2 * k] = pts[k].x(); out[2 * k + 1] = pts[k].y(); }
+  return long(pts.size());
 }
 
 }  // extern "C"

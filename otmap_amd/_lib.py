@@ -81,6 +81,8 @@ _SIGNATURES = {
     "otm_mesh_lerp_device": (C.c_int, [C.c_int, C.c_long, C.c_int, C.POINTER(_vp), C.c_double, C.c_double, _vp]),
     "otm_image_normalize_mean_device": (C.c_int, [C.c_int, C.c_long, _vp, C.c_double, C.c_int, _vp, _dp]),
     "otm_prune_points_device": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int64, _vp, C.POINTER(C.c_int64)]),
+    "otm_blue_noise_tile_device": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int64, _vp, C.c_int64,
+                                             C.POINTER(C.c_int64)]),
     "otm_prune_points_device_noalloc": (C.c_int, [C.c_int, C.c_int, _vp, _vp, C.c_int64, _vp,
                                                   C.POINTER(C.c_int64)]),
     "otm_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _ip]),

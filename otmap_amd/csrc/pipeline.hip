@@ -232,6 +232,70 @@ constexpr long PRUNE_MAX_BLOCKS = 8192;                           // workgroups 
 constexpr long PRUNE_MAX_POINTS = PRUNE_MAX_BLOCKS * PRUNE_CHUNK;  // 16,777,216 points
 static __device__ __align__(256) long long scratch[PRUNE_MAX_BLOCKS / 2 + 1];  // 8192 int counts, then the total
 
+// generate_blue_noise_tile (apps/common/image_utils.cpp:148-173): the tile
+// of nb points (N = floor(sqrt(nb)) per side) repeated over nt x nt cells
+// of size N / n; candidate L = (i nt + j) nb + k is ((i + t_k.x) s, (j +
+// t_k.y) s), kept when both coordinates are <= 1, in candidate order.
+struct TileGrid {
+  long nb;    // tile points
+  int nt;     // cells per side
+  double s;   // N / n
+};
+__device__ inline bool tile_point(const TileGrid g, const double* __restrict__ tile, long L, double& x, double& y) {
+  const long cell = L / g.nb, k = L - cell * g.nb;
+  const int i = int(cell / g.nt), j = int(cell - long(i) * g.nt);
+  x = (double(i) + tile[2 * k]) * g.s;
+  y = (double(j) + tile[2 * k + 1]) * g.s;
+  return x <= 1. && y <= 1.;
+}
+
+__global__ __launch_bounds__(NT) void k_tile_count(const TileGrid g, const double* __restrict__ tile, long count,
+                                                  int* __restrict__ block_count) {
+  __shared__ int total;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  int c = 0;
+  const long base = long(blockIdx.x) * PRUNE_CHUNK;
+  for (int u = threadIdx.x; u < PRUNE_CHUNK; u += NT) {
+    const long L = base + u;
+    if (L >= count) break;
+    double x, y;
+    c += tile_point(g, tile, L, x, y) ? 1 : 0;
+  }
+  if (c) atomicAdd(&total, c);
+  __syncthreads();
+  if (threadIdx.x == 0) block_count[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(NT) void k_tile_scatter(const TileGrid g, const double* __restrict__ tile, long count,
+                                                    const int* __restrict__ block_offset, double* __restrict__ out) {
+  __shared__ int off[NT + 1];
+  const long base = long(blockIdx.x) * PRUNE_CHUNK;
+  constexpr int PER = PRUNE_CHUNK / NT;
+  double px[PER], py[PER];
+  bool mine[PER];
+  int c = 0;
+  for (int u = 0; u < PER; ++u) {
+    const long L = base + long(threadIdx.x) * PER + u;
+    mine[u] = L < count && tile_point(g, tile, L, px[u], py[u]);
+    c += mine[u] ? 1 : 0;
+  }
+  off[threadIdx.x + 1] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    off[0] = 0;
+    for (int t = 1; t <= NT; ++t) off[t] += off[t - 1];
+  }
+  __syncthreads();
+  long dst = long(block_offset[blockIdx.x]) + off[threadIdx.x];
+  for (int u = 0; u < PER; ++u)
+    if (mine[u]) {
+      out[2 * dst] = px[u];
+      out[2 * dst + 1] = py[u];
+      ++dst;
+    }
+}
+
 int red_blocks(long count) { return int(std::max<long>(1, std::min<long>((count + NT - 1) / NT, RB))); }
 
 struct DevBuf {
@@ -417,6 +481,42 @@ int otm_prune_points_device(int device, int n, const double* d_density, const do
     long k = 0;
     OTM_HIP(hipMemcpy(&k, total.p, sizeof(long), hipMemcpyDeviceToHost));
     *kept = k;
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return fail(e);
+  }
+}
+
+int otm_blue_noise_tile_device(int device, int n, const double* d_tile, int64_t nb, double* d_out, int64_t cap,
+                               int64_t* count) {
+  if (n < 1 || nb < 1 || !d_tile || !count || cap < 0 || (cap > 0 && !d_out)) {
+    set_error("invalid blue-noise tile arguments");
+    return OTM_ERR_ARG;
+  }
+  const int N = int(std::sqrt(double(nb)));
+  const int nt = (n + N - 1) / N;
+  const long total = long(nt) * nt * nb;
+  if (total > (1L << 31) - PRUNE_CHUNK) { set_error("blue-noise tiling too large"); return OTM_ERR_ARG; }
+  try {
+    OTM_HIP(hipSetDevice(device));
+    const TileGrid g{long(nb), nt, double(N) / double(n)};
+    const long nblk = (total + PRUNE_CHUNK - 1) / PRUNE_CHUNK;
+    DevBuf blocks{size_t(nblk) * sizeof(int)}, kept{sizeof(long)};
+    hipLaunchKernelGGL(k_tile_count, dim3(unsigned(nblk)), dim3(NT), 0, nullptr, g, d_tile, total, blocks.as<int>());
+    OTM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_prune_scan, dim3(1), dim3(NT), 0, nullptr, blocks.as<int>(), int(nblk), kept.as<long>());
+    OTM_HIP(hipGetLastError());
+    long k = 0;
+    OTM_HIP(hipMemcpy(&k, kept.p, sizeof(long), hipMemcpyDeviceToHost));
+    *count = k;
+    if (k > cap) {
+      set_error("blue-noise tiling: output holds fewer points than the tiling produces (see *count)");
+      return OTM_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_tile_scatter, dim3(unsigned(nblk)), dim3(NT), 0, nullptr, g, d_tile, total,
+                       (const int*)blocks.as<int>(), d_out);
+    OTM_HIP(hipGetLastError());
+    OTM_HIP(hipDeviceSynchronize());
     return OTM_OK;
   } catch (const HipError& e) {
     return fail(e);

@@ -5,8 +5,10 @@ barycentres   apps/barycenters.cpp:167-290: inverse map of each density as a
               interpolation of the inverse meshes, each interpolated mesh
               rasterised (unit density per face) and scaled to the
               interpolated mean of the input densities.
-stippling     apps/stippling.cpp:140-197: points advected through the inverse
-              map, then the points landing on zero density pruned.
+stippling     apps/stippling.cpp:140-197: points (given, or a blue-noise tile
+              repeated at a resolution, apps/common/image_utils.cpp:148-173)
+              advected through the inverse map, then the points landing on
+              zero density pruned.
 composite_map apps/otmap.cpp:119-132: map u -> v = inverse map of v applied to
               the forward mesh of u.
 
@@ -91,14 +93,61 @@ def barycentres(maps, steps: int, res: int, means, f32: bool = True, device: int
     return out
 
 
-def stippling(tmap: TransportMap, density, points, device: int = 0):
-    """Advects `points` ((m, 2), uniform in [0,1]^2) through the inverse map
-    and drops those landing on zero density of `density` (n x n, face order).
-    Returns the kept points as an (k, 2) device tensor."""
+def load_point_cloud_dat(path):
+    """A point tile as apps/common/image_utils.cpp:209-231 reads it: pairs of
+    numbers until the end of the file.  The reference's loop tests for the
+    end before reading, so a file that ends in whitespace after its last
+    number yields that last point twice (and an empty file one (0, 0)
+    point); the same points result here.  Returns (m, 2) float64."""
+    with open(path) as f:
+        text = f.read()
+    vals = [float(t) for t in text.split()]
+    if len(vals) % 2:
+        raise ValueError(f"{path}: odd number of coordinates")
+    pts = np.array(vals, dtype=np.float64).reshape(-1, 2)
+    if not text or text[-1].isspace():
+        last = pts[-1:] if len(pts) else np.zeros((1, 2))
+        pts = np.concatenate([pts, last])
+    return pts
+
+
+def blue_noise_tile(n: int, tile, device: int = 0):
+    """generate_blue_noise_tile(n, pts, tile) (apps/common/image_utils.cpp:
+    148-173) on the device: the tile repeated over ceil(n / N)^2 cells of size
+    N / n (N = floor(sqrt(len(tile)))), points with both coordinates <= 1, in
+    the reference's order.  Returns an (m, 2) float64 device tensor."""
+    torch = _torch()
+    dev = f"cuda:{device}"
+    t = torch.as_tensor(np.ascontiguousarray(tile, dtype=np.float64).reshape(-1), device=dev)
+    nb = t.numel() // 2
+    N = int(np.sqrt(nb))
+    nt = (n + N - 1) // N
+    cap = nt * nt * nb
+    out = torch.empty(2 * cap, dtype=torch.float64, device=dev)
+    count = C.c_int64()
+    _lib.check(_lib.lib().otm_blue_noise_tile_device(device, int(n), t.data_ptr(), nb, out.data_ptr(), cap,
+                                                     C.byref(count)), "blue_noise_tile")
+    return out[: 2 * count.value].view(-1, 2)
+
+
+def stippling(tmap: TransportMap, density, points=None, device: int = 0, tile=None, res: int | None = None):
+    """Advects points through the inverse map and drops those landing on
+    zero density of `density` (n x n, face order).  The points are `points`
+    ((m, 2) in [0,1]^2, host array or device tensor) or, as the reference app
+    makes them, the blue-noise `tile` repeated at resolution `res`
+    (apps/stippling.cpp:177-181).  Returns the kept points as an (k, 2) device
+    tensor."""
     torch = _torch()
     n = tmap.fwd_mesh().n
     dev = f"cuda:{device}"
-    pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64).reshape(-1), device=dev)
+    if points is None:
+        if tile is None or res is None:
+            raise ValueError("stippling needs points, or a tile and a resolution")
+        points = blue_noise_tile(res, tile, device)
+    if isinstance(points, torch.Tensor):
+        pts = points.to(device=dev, dtype=torch.float64).contiguous().reshape(-1)
+    else:
+        pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64).reshape(-1), device=dev)
     m = pts.numel() // 2
     adv = torch.empty_like(pts)
     L = _lib.lib()

@@ -348,6 +348,31 @@ def cfg4(n=2048, nq=16384):
     save("stipple2048.npz", **out)
 
 
+def blue_noise():
+    """The reference apps' blue-noise tile (data/uniform_pattern_sig2012.dat,
+    read by load_point_cloud_dat, apps/common/image_utils.cpp:209-231) and
+    generate_blue_noise_tile (:148-173) at three resolutions."""
+    path = "/root/reference/data/uniform_pattern_sig2012.dat"
+    tile = R.load_point_cloud_dat(path)
+    with open(path) as f:
+        text = f.read()
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".dat", delete=False) as f:
+        f.write(text + "\n")
+    tile_nl = R.load_point_cloud_dat(f.name)
+    os.unlink(f.name)
+    out = dict(tile=tile, tile_trailing_newline=tile_nl)
+    rng = np.random.default_rng(5005)
+    for n in (100, 333, 1000):
+        p = R.blue_noise_tile(n, tile)
+        out[f"n{n}_count"] = np.int64(len(p))
+        out[f"n{n}_sum"] = p.sum(0)
+        idx = np.sort(rng.choice(len(p), min(len(p), 4096), replace=False))
+        out[f"n{n}_idx"] = idx
+        out[f"n{n}_pts"] = p[idx]
+    save("blue_noise.npz", **out)
+
+
 def edge_cases():
     """The reference's input edge cases (otlib/otsolver_2dgrid.cpp):
       zero64   zero-density cells with the default max_ratio (DBL_MAX): the
@@ -373,7 +398,7 @@ def edge_cases():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["stencils", "small", "inverse", "raster", "costs", "edge", "roots", "cfg0", "cfg1",
+    which = sys.argv[1:] or ["stencils", "small", "inverse", "raster", "costs", "edge", "roots", "blue", "cfg0", "cfg1",
                              "cfg2", "cfg3", "cfg4"]
     if "stencils" in which:
         stencils()
@@ -389,6 +414,8 @@ if __name__ == "__main__":
         edge_cases()
     if "roots" in which:
         line_search_roots()
+    if "blue" in which:
+        blue_noise()
     if "cfg0" in which:
         big_solve(256, "solve_gaussian256.npz", D.gaussian_mixture(256), 16384)
     if "cfg1" in which:

@@ -126,3 +126,35 @@ def test_stippling_vs_oracle(oracle):
     keep = np.isfinite(adv).all(1) & (rho[iy + ix * n] != 0)
     assert 0 < keep.sum() < len(pts)
     assert np.array_equal(kept, adv[keep])
+
+
+@pytest.mark.parametrize("n", [100, 333, 1000])
+def test_blue_noise_tile_vs_reference_golden(n):
+    """generate_blue_noise_tile (apps/common/image_utils.cpp:148-173) on the
+    device from the reference's own tile: same count, sampled points
+    bit-exact in the reference's order, same coordinate sums."""
+    g = np.load(os.path.join(GOLDEN, "blue_noise.npz"))
+    pts = pipelines.blue_noise_tile(n, g["tile"]).cpu().numpy()
+    assert len(pts) == int(g[f"n{n}_count"])
+    assert np.array_equal(pts[g[f"n{n}_idx"]], g[f"n{n}_pts"])
+    assert np.allclose(pts.sum(0), g[f"n{n}_sum"], rtol=1e-13, atol=0)
+
+
+def test_stippling_from_blue_noise_tile_vs_oracle(oracle):
+    """apps/stippling.cpp:177-197 with the reference's tile: the tile
+    repeated at resolution 200, advected through the inverse map, pruned on
+    zero density; kept points identical, in order, to the oracle's."""
+    g = np.load(os.path.join(GOLDEN, "blue_noise.npz"))
+    n = 64
+    rho = D.gaussian_mixture(n, 13)
+    rho[(np.arange(n * n) % 11) == 0] = 0.0
+    o = oracle.solve(n, rho)
+    tm = TransportMap(GridMesh.regular(n), GridMesh(n, o["fwd"]), o["density"])
+    kept = pipelines.stippling(tm, rho, tile=g["tile"], res=200).cpu().numpy()
+    pts = pipelines.blue_noise_tile(200, g["tile"]).cpu().numpy()
+    adv, _, _ = oracle.inverse(n, o["fwd"], o["density"], pts)
+    ix = np.clip(np.floor(adv[:, 0] * n).astype(int), 0, n - 1)
+    iy = np.clip(np.floor(adv[:, 1] * n).astype(int), 0, n - 1)
+    keep = np.isfinite(adv).all(1) & (rho[iy + ix * n] != 0)
+    assert 0 < keep.sum() < len(pts)
+    assert np.array_equal(kept, adv[keep])

@@ -395,3 +395,18 @@ def test_line_search_step_choice_vs_reference(oracle):
     both = (al != 0.5) & (ra != 0.5)
     assert np.all(np.abs(al[both & ~outside] - ra[both & ~outside]) <= 1e-11 * np.abs(ra[both & ~outside]))
     assert np.all(np.abs(al[both] - ra[both]) <= 1e-5 * np.abs(ra[both]))
+
+
+def test_point_cloud_loader_vs_reference(tmp_path):
+    """load_point_cloud_dat (apps/common/image_utils.cpp:209-231), the
+    Python mirror: the reference's tile file as it reads it, and the same
+    text with a trailing newline (the reference then repeats the last point)."""
+    from otmap_amd import pipelines
+    g = load("blue_noise.npz")
+    text = "\n".join(f"{float(x)!r} {float(y)!r}" for x, y in g["tile"])
+    p0, p1 = tmp_path / "t0.dat", tmp_path / "t1.dat"
+    p0.write_text(text)
+    p1.write_text(text + "\n")
+    assert np.array_equal(pipelines.load_point_cloud_dat(p0), g["tile"])
+    assert np.array_equal(pipelines.load_point_cloud_dat(p1), g["tile_trailing_newline"])
+    assert len(g["tile_trailing_newline"]) == len(g["tile"]) + 1
