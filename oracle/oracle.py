@@ -76,6 +76,7 @@ class Oracle:
             "orc_rasterize": (None, [_i, _dp, C.POINTER(C.c_double), _i, _i, _dp]),
             "orc_transport_cost": (_d, [_i, _dp, _dp, _dp, C.POINTER(C.c_double)]),
             "orc_solver_transport_cost": (_d, [_i, _dp, _dp, C.POINTER(C.c_double)]),
+            "orc_set_area_precision": (None, [_i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

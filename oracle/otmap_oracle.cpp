@@ -101,11 +101,28 @@ void vertex_gradients(int n, const double* psi, double* G) {
 // ---------------------------------------------------------------------------
 // Quad areas from vertex displacements  (otlib/otsolver_2dgrid.cpp:425-447)
 // ---------------------------------------------------------------------------
+// Precision experiment (not the reference method): area_mode 1 evaluates
+// every quad area of the residual in FP32 arithmetic from the FP64 vertex
+// gradients rounded to FP32 (e in FP32 too), the result widened back to FP64;
+// 0 (default) is the reference's FP64.  Set by orc_set_area_precision.
+int g_area_mode = 0;
+
 void face_area(int n, const double* G, double* area) {
   const long V = long(n + 1) * (n + 1);
   const double* Gx = G;
   const double* Gy = G + V;
   const double e = 1. / double(n);
+  if (g_area_mode == 1) {
+    const float ef = float(e);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const int v00 = vidx(n, i, j), v10 = vidx(n, i + 1, j), v01 = vidx(n, i, j + 1), v11 = vidx(n, i + 1, j + 1);
+        const float a = float(Gx[v11]) - float(Gx[v00]) + ef, b = float(Gy[v01]) - float(Gy[v10]) + ef;
+        const float c = float(Gy[v11]) - float(Gy[v00]) + ef, d = float(Gx[v01]) - float(Gx[v10]) - ef;
+        area[fidx(n, i, j)] = double(0.5f * (a * b - c * d));
+      }
+    return;
+  }
   for (int i = 0; i < n; ++i) {
     for (int j = 0; j < n; ++j) {
       const int v00 = vidx(n, i, j), v10 = vidx(n, i + 1, j), v01 = vidx(n, i, j + 1), v11 = vidx(n, i + 1, j + 1);
@@ -683,6 +700,7 @@ void orc_adjust_density(int n, double* rho, double max_ratio) { adjust_density(n
 void orc_vertex_gradients(int n, const double* psi, double* G) { vertex_gradients(n, psi, G); }
 
 void orc_face_area(int n, const double* G, double* area) { face_area(n, G, area); }
+void orc_set_area_precision(int mode) { g_area_mode = mode; }
 
 double orc_residual(int n, const double* rho, const double* psi, double* r) {
   std::vector<double> G;

@@ -410,3 +410,32 @@ def test_point_cloud_loader_vs_reference(tmp_path):
     assert np.array_equal(pipelines.load_point_cloud_dat(p0), g["tile"])
     assert np.array_equal(pipelines.load_point_cloud_dat(p1), g["tile_trailing_newline"])
     assert len(g["tile_trailing_newline"]) == len(g["tile"]) + 1
+
+
+def test_fp32_quad_areas_break_the_map_bar_on_a_sensitive_density(oracle):
+    """Why the quad areas stay FP64 in both precision modes.  Areas are
+    evaluated per iteration only in the conjugate-Jacobian residual r(psi -
+    eps d_hat) (the line search updates r exactly from the quartic); with
+    those areas in FP32 (oracle switch orc_set_area_precision(1): FP32
+    arithmetic on the FP64 gradients) the 192^2 batch density 29, whose
+    trajectory amplifies perturbations ~3x per iteration, ends 1.2e-5 from the
+    reference's positions (FP64: 3e-10) -- past the 1e-6 drop-in bar, with
+    the same 37 iterations.  (cfg0 256^2: 1.3e-8, cfg1 1024^2: 9.2e-8 measured
+    the same way.)"""
+    g = load("batch64.npz")
+    p = 29
+    n = int(g["n"][p])
+    rho = D.batch_densities(64)[p]
+    V = (n + 1) ** 2
+    idx = g[f"d{p}_vidx"]
+    errs = {}
+    try:
+        for mode in (0, 1):
+            oracle.L.orc_set_area_precision(mode)
+            o = oracle.solve(n, rho)
+            assert o["iters"] == int(g[f"d{p}_iters"])
+            errs[mode] = max(np.abs(o["fwd"][idx] - g[f"d{p}_fx"]).max(), np.abs(o["fwd"][V + idx] - g[f"d{p}_fy"]).max())
+    finally:
+        oracle.L.orc_set_area_precision(0)
+    assert errs[0] < 1e-8
+    assert errs[1] > 1e-6
