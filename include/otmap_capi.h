@@ -180,6 +180,14 @@ int otm_map_destroy(otm_map* m);
 int otm_map_init_inverse(otm_map* m);
 /* number of BVH nodes (0 before the build) */
 int otm_map_bvh_size(const otm_map* m);
+/* The BVH as built (no reference counterpart: inspection and parity tests of
+ * the device build against BVH2D::build, otlib/utils/bvh2d.cpp:29-243), in
+ * the reference's node order: per node its inflated box (boxes[4k .. 4k+3] =
+ * min x, min y, max x, max y), first (inner: first child, leaf: first face
+ * slot), count (leaf: faces, inner: -1) and escape (next node in depth-first
+ * order once the subtree is done, -1 at the end); faces: the n^2 face slots.
+ * Builds the BVH if needed; arrays sized by otm_map_bvh_size. */
+int otm_map_bvh_export(otm_map* m, double* boxes, int32_t* first, int32_t* count, int32_t* escape, int32_t* faces);
 /* inv / inv_fast / apply_inverse_map (otlib/transport_map.cpp:50-128) for
  * `count` interleaved points.  mode 0 = inv (all overlapping faces, the
  * reference's selection), 1 = inv_fast (first hit).  out: interleaved

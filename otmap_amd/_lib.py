@@ -61,6 +61,7 @@ _SIGNATURES = {
     "otm_vertex_gradients": (C.c_int, [_vp, _dp, _dp]),
     "otm_residual": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_laplacian_solve": (C.c_int, [_vp, _dp, _dp]),
+    "otm_map_bvh_export": (C.c_int, [_vp, _dp, _ip, _ip, _ip, _ip]),
     "otm_line_search_steps": (C.c_int, [C.c_int, _dp, C.c_int, _dp, _dp]),
     "otm_laplacian_solve_mixed": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "otm_line_search_coefficients": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),

@@ -31,6 +31,7 @@ SOURCES = [
     "driver.hip",
     "batch.hip",
     "pipeline.hip",
+    "bvh_build.hip",
 ]
 
 FLAGS = [

@@ -1,0 +1,452 @@
+// BVH over the forward quads built on the device, node for node and face
+// slot for face slot the tree of the reference's BVH2D::build
+// (otlib/utils/bvh2d.cpp:29-58, 167-243) and of the host build in
+// inverse.hip: face centroids; per node the AABB of its faces' vertices
+// inflated by 2 eps diag + eps; leaf when <= leaf_size faces or depth >=
+// max_depth; otherwise the longest axis is split at the float-rounded box
+// midpoint by the reference's in-place two-pointer partition, and a split
+// that leaves one side empty makes a leaf (after the partition's swaps).
+//
+// The build is level-synchronous: every node of one depth is processed at
+// once, one thread per face slot.
+//   * Boxes: order-free min / max, as atomics on order-preserving 64-bit keys
+//     (one atomic per wave when the whole wave lies in one node).
+//   * Partition: with L = the faces left of the split value (c < sv) in
+//     [start, end), the two-pointer loop swaps the k-th right face in
+//     [start, start + L) (counted from the left) with the k-th left face in
+//     [start + L, end) (counted from the right) and nothing else; the ranks
+//     come from two exclusive scans over all slots.  Its returned middle is
+//     start + L, plus one when the face then at start + L lies exactly on
+//     the split value (the loop's final `<=` test).
+//   * Numbering: the reference appends a node's two children when it splits,
+//     in depth-first order (first child's subtree first), so the split node
+//     of preorder rank k (among split nodes) gets children 1 + 2k and 2 + 2k.
+//     Ranks follow from the split-node count of each first child's subtree
+//     (bottom-up), then top-down; escape links (the next node once a subtree
+//     is done: first child -> second child -> parent's escape) likewise.
+// Result: the same BNode array and face order as the host build (tested).
+
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <vector>
+
+#include "otm_internal.h"
+
+namespace otm {
+namespace {
+
+constexpr int BT = 256;  // threads per workgroup
+
+typedef unsigned long long u64;
+
+// order-preserving key of a double (min / max of keys = min / max of values)
+__device__ inline u64 dkey(double x) {
+  const u64 b = __double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+__device__ inline double dkey_inv(u64 k) {
+  const u64 b = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+  return __longlong_as_double(b);
+}
+
+struct Tmp {  // temporary node records, indexed by build order (level by level)
+  int* start;
+  int* end;
+  int* child;   // first child's record, -1 for a leaf
+  int* level;
+  u64* key;     // 4 per node: min x, min y, max x, max y
+  double* box;  // 4 per node: inflated box
+  int* dim;
+  float* sv;
+  int* split;   // 1: the node splits (set before the partition, cleared if a side stays empty)
+  int* nleft;   // L
+  int* mid;
+  int* nsplit;  // split nodes in the subtree (the node included)
+  int* rank;    // preorder rank among split nodes
+  int* fin;     // final index
+  int* esc;     // final escape index
+};
+
+__device__ inline void quad_v(int n, int f, int* v) {
+  const int i = f / n, j = f - (f / n) * n;
+  v[0] = j + i * (n + 1);
+  v[1] = j + (i + 1) * (n + 1);
+  v[2] = (j + 1) + (i + 1) * (n + 1);
+  v[3] = (j + 1) + i * (n + 1);
+}
+
+// centroids in the reference's order: ((((0 + p0) + p1) + p2) + p3) / 4
+__global__ __launch_bounds__(BT) void k_centroids(int n, const double* __restrict__ xy, int* __restrict__ fid,
+                                                 double* __restrict__ cx, double* __restrict__ cy,
+                                                 int* __restrict__ pn) {
+  const long F = long(n) * n, V = long(n + 1) * (n + 1);
+  for (long f = long(blockIdx.x) * BT + threadIdx.x; f < F; f += long(gridDim.x) * BT) {
+    int v[4];
+    quad_v(n, int(f), v);
+    double x = 0., y = 0.;
+    for (int q = 0; q < 4; ++q) {
+      x += xy[v[q]];
+      y += xy[V + v[q]];
+    }
+    cx[f] = x / 4;
+    cy[f] = y / 4;
+    fid[f] = int(f);
+    pn[f] = 0;
+  }
+}
+
+__global__ __launch_bounds__(BT) void k_root(Tmp T, int F) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  T.start[0] = 0;
+  T.end[0] = F;
+  T.level[0] = 0;
+}
+
+__global__ __launch_bounds__(BT) void k_lvl_init(Tmp T, int b, int K) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= K) return;
+  const int id = b + k;
+  T.key[4 * id + 0] = ~0ull;
+  T.key[4 * id + 1] = ~0ull;
+  T.key[4 * id + 2] = 0ull;
+  T.key[4 * id + 3] = 0ull;
+  T.nleft[id] = 0;
+  T.child[id] = -1;
+}
+
+// node boxes: min / max of the vertices of every face of the node
+__global__ __launch_bounds__(BT) void k_lvl_box(int n, const double* __restrict__ xy, const int* __restrict__ fid,
+                                               const int* __restrict__ pn, Tmp T) {
+  const long F = long(n) * n, V = long(n + 1) * (n + 1);
+  const long p = long(blockIdx.x) * BT + threadIdx.x;
+  const int nd = p < F ? pn[p] : -1;
+  double lx = DBL_MAX, ly = DBL_MAX, hx = -DBL_MAX, hy = -DBL_MAX;
+  if (nd >= 0) {
+    int v[4];
+    quad_v(n, fid[p], v);
+    for (int q = 0; q < 4; ++q) {
+      const double x = xy[v[q]], y = xy[V + v[q]];
+      lx = fmin(lx, x); ly = fmin(ly, y);
+      hx = fmax(hx, x); hy = fmax(hy, y);
+    }
+  }
+  const int first = __shfl(nd, 0);
+  if (__all(nd == first)) {
+    if (first < 0) return;
+    for (int o = 32; o > 0; o >>= 1) {
+      lx = fmin(lx, __shfl_xor(lx, o)); ly = fmin(ly, __shfl_xor(ly, o));
+      hx = fmax(hx, __shfl_xor(hx, o)); hy = fmax(hy, __shfl_xor(hy, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&T.key[4 * first + 0], dkey(lx));
+      atomicMin(&T.key[4 * first + 1], dkey(ly));
+      atomicMax(&T.key[4 * first + 2], dkey(hx));
+      atomicMax(&T.key[4 * first + 3], dkey(hy));
+    }
+  } else if (nd >= 0) {
+    atomicMin(&T.key[4 * nd + 0], dkey(lx));
+    atomicMin(&T.key[4 * nd + 1], dkey(ly));
+    atomicMax(&T.key[4 * nd + 2], dkey(hx));
+    atomicMax(&T.key[4 * nd + 3], dkey(hy));
+  }
+}
+
+// inflated box, leaf test, split axis and value (otlib/utils/bvh2d.cpp:193-221)
+__global__ __launch_bounds__(BT) void k_lvl_split(Tmp T, int b, int K, int leaf_size, int max_depth) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= K) return;
+  const int id = b + k;
+  const double lo[2] = {dkey_inv(T.key[4 * id + 0]), dkey_inv(T.key[4 * id + 1])};
+  const double hi[2] = {dkey_inv(T.key[4 * id + 2]), dkey_inv(T.key[4 * id + 3])};
+  const double eps = DBL_EPSILON;
+  const double diag[2] = {hi[0] - lo[0], hi[1] - lo[1]};
+  double blo[2], bhi[2];
+  for (int d = 0; d < 2; ++d) {
+    const double pad = diag[d] * (2. * eps) + eps;
+    blo[d] = lo[d] - pad;
+    bhi[d] = hi[d] + pad;
+  }
+  T.box[4 * id + 0] = blo[0];
+  T.box[4 * id + 1] = blo[1];
+  T.box[4 * id + 2] = bhi[0];
+  T.box[4 * id + 3] = bhi[1];
+  const int cnt = T.end[id] - T.start[id];
+  const bool leaf = cnt <= leaf_size || T.level[id] >= max_depth;
+  const int dim = diag[1] > diag[0] ? 1 : 0;
+  T.dim[id] = dim;
+  T.sv[id] = float(0.5 * (bhi[dim] + blo[dim]));
+  T.split[id] = leaf ? 0 : 1;
+}
+
+// left flags (centroid < split value) and L per splitting node
+__global__ __launch_bounds__(BT) void k_lvl_left(int F, const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const int* __restrict__ pn, Tmp T, unsigned char* __restrict__ left) {
+  const long p = long(blockIdx.x) * BT + threadIdx.x;
+  int nd = p < F ? pn[p] : -1;
+  if (nd >= 0 && !T.split[nd]) nd = -1;
+  int l = 0;
+  if (nd >= 0) {
+    const double c = T.dim[nd] ? cy[p] : cx[p];
+    l = c < double(T.sv[nd]) ? 1 : 0;
+  }
+  if (p < F) left[p] = (unsigned char)l;
+  const int first = __shfl(nd, 0);
+  if (__all(nd == first)) {
+    if (first < 0) return;
+    int s = l;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&T.nleft[first], s);
+  } else if (nd >= 0 && l) {
+    atomicAdd(&T.nleft[nd], 1);
+  }
+}
+
+// faces on the wrong side of start + L: right faces before it (bl), left faces from it on (br)
+__global__ __launch_bounds__(BT) void k_lvl_bad(int F, const int* __restrict__ pn, Tmp T,
+                                               const unsigned char* __restrict__ left, int* __restrict__ bl,
+                                               int* __restrict__ br) {
+  const long p = long(blockIdx.x) * BT + threadIdx.x;
+  if (p > F) return;
+  int a = 0, c = 0;
+  if (p < F) {
+    const int nd = pn[p];
+    if (nd >= 0 && T.split[nd]) {
+      const bool region = p < T.start[nd] + T.nleft[nd];
+      a = (region && !left[p]) ? 1 : 0;
+      c = (!region && left[p]) ? 1 : 0;
+    }
+  }
+  bl[p] = a;  // slot F stays 0: scans give totals at F
+  br[p] = c;
+}
+
+// slot of the k-th misplaced left face of each node, counted from the node's end
+__global__ __launch_bounds__(BT) void k_lvl_rpos(int F, const int* __restrict__ pn, Tmp T,
+                                                const int* __restrict__ br, const int* __restrict__ sr,
+                                                int* __restrict__ rpos) {
+  const long p = long(blockIdx.x) * BT + threadIdx.x;
+  if (p >= F || !br[p]) return;
+  const int nd = pn[p];
+  const int s0 = sr[T.start[nd]], e0 = sr[T.end[nd]];
+  const int r = e0 - sr[p] - 1;  // misplaced left faces after p in the node
+  rpos[s0 + r] = int(p);
+}
+
+// the two-pointer swaps: the k-th misplaced right face (from the left) with
+// the k-th misplaced left face (from the right)
+__global__ __launch_bounds__(BT) void k_lvl_swap(int F, const int* __restrict__ pn, Tmp T, const int* __restrict__ bl,
+                                                const int* __restrict__ sl, const int* __restrict__ sr,
+                                                const int* __restrict__ rpos, int* __restrict__ fid,
+                                                double* __restrict__ cx, double* __restrict__ cy) {
+  const long p = long(blockIdx.x) * BT + threadIdx.x;
+  if (p >= F || !bl[p]) return;
+  const int nd = pn[p];
+  const int r = sl[p] - sl[T.start[nd]];
+  const long q = rpos[sr[T.start[nd]] + r];
+  const int f = fid[p];
+  fid[p] = fid[q];
+  fid[q] = f;
+  const double x = cx[p], y = cy[p];
+  cx[p] = cx[q];
+  cy[p] = cy[q];
+  cx[q] = x;
+  cy[q] = y;
+}
+
+// the partition's middle; a split that leaves a side empty makes a leaf;
+// child counts for the next level
+__global__ __launch_bounds__(BT) void k_lvl_mid(Tmp T, int b, int K, const double* __restrict__ cx,
+                                               const double* __restrict__ cy, int* __restrict__ nchild) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= K) return;
+  const int id = b + k;
+  int two = 0;
+  if (T.split[id]) {
+    const int s = T.start[id], e = T.end[id];
+    int mid = s + T.nleft[id];
+    if (mid < e) {
+      const double c = T.dim[id] ? cy[mid] : cx[mid];
+      if (c == double(T.sv[id])) ++mid;
+    }
+    T.mid[id] = mid;
+    if (mid == s || mid == e)
+      T.split[id] = 0;
+    else
+      two = 2;
+  }
+  nchild[k] = two;
+}
+
+// child records of the next level (first children at b2 + exclusive scan)
+__global__ __launch_bounds__(BT) void k_lvl_children(Tmp T, int b, int K, int b2, const int* __restrict__ off) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= K) return;
+  const int id = b + k;
+  if (!T.split[id]) return;
+  const int c = b2 + off[k];
+  T.child[id] = c;
+  T.start[c] = T.start[id];
+  T.end[c] = T.mid[id];
+  T.start[c + 1] = T.mid[id];
+  T.end[c + 1] = T.end[id];
+  T.level[c] = T.level[c + 1] = T.level[id] + 1;
+}
+
+// node of every slot at the next level (-1: its leaf is final)
+__global__ __launch_bounds__(BT) void k_lvl_assign(int F, int* __restrict__ pn, Tmp T) {
+  const long p = long(blockIdx.x) * BT + threadIdx.x;
+  if (p >= F) return;
+  const int nd = pn[p];
+  if (nd < 0) return;
+  if (!T.split[nd]) {
+    pn[p] = -1;
+    return;
+  }
+  pn[p] = T.child[nd] + (p < T.mid[nd] ? 0 : 1);
+}
+
+// split nodes per subtree, one level (bottom-up)
+__global__ __launch_bounds__(BT) void k_count_split(Tmp T, int b, int K) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= K) return;
+  const int id = b + k;
+  const int c = T.child[id];
+  T.nsplit[id] = c < 0 ? 0 : 1 + T.nsplit[c] + T.nsplit[c + 1];
+}
+
+// preorder ranks, final indices and escape links, one level (top-down)
+__global__ __launch_bounds__(BT) void k_number(Tmp T, int b, int K) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= K) return;
+  const int id = b + k;
+  if (id == 0) {
+    T.rank[0] = 0;
+    T.fin[0] = 0;
+    T.esc[0] = -1;
+  }
+  const int c = T.child[id];
+  if (c < 0) return;
+  const int r = T.rank[id];
+  T.rank[c] = r + 1;
+  T.rank[c + 1] = r + 1 + T.nsplit[c];
+  T.fin[c] = 1 + 2 * r;
+  T.fin[c + 1] = 2 + 2 * r;
+  T.esc[c] = 1 + 2 * r + 1;
+  T.esc[c + 1] = T.esc[id];
+}
+
+__global__ __launch_bounds__(BT) void k_write_nodes(Tmp T, int total, BNode* __restrict__ nodes) {
+  const int id = blockIdx.x * BT + threadIdx.x;
+  if (id >= total) return;
+  BNode nd;
+  nd.lox = T.box[4 * id + 0];
+  nd.loy = T.box[4 * id + 1];
+  nd.hix = T.box[4 * id + 2];
+  nd.hiy = T.box[4 * id + 3];
+  const int c = T.child[id];
+  nd.first = c < 0 ? T.start[id] : T.fin[c];
+  nd.count = c < 0 ? max(0, T.end[id] - T.start[id]) : -1;
+  nd.escape = T.esc[id];
+  nd.pad = 0;
+  nodes[T.fin[id]] = nd;
+}
+
+inline unsigned blocks_for(long count) { return unsigned(std::max<long>(1, (count + BT - 1) / BT)); }
+
+template <class T>
+struct DArr {
+  T* p = nullptr;
+  explicit DArr(size_t count) { OTM_HIP(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T))); }
+  ~DArr() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+void bvh_build_device(int n, const double* d_xy, int leaf_size, int max_depth, hipStream_t s, BNode** nodes_out,
+                      int** faces_out, int* num_nodes) {
+  const long F = long(n) * n;
+  const long NMAX = 2 * F + 2;  // tree records: at most 2F - 1
+  DArr<double> cx(F), cy(F);
+  DArr<int> pn(F), bl(F + 1), br(F + 1), sl(F + 1), sr(F + 1), rpos(F);
+  DArr<unsigned char> left(F);
+  DArr<int> t_start(NMAX), t_end(NMAX), t_child(NMAX), t_level(NMAX), t_dim(NMAX), t_split(NMAX), t_nleft(NMAX),
+      t_mid(NMAX), t_nsplit(NMAX), t_rank(NMAX), t_fin(NMAX), t_esc(NMAX), nchild(NMAX), noff(NMAX + 1);
+  DArr<u64> t_key(4 * NMAX);
+  DArr<double> t_box(4 * NMAX);
+  DArr<float> t_sv(NMAX);
+  int* fid = nullptr;
+  OTM_HIP(hipMalloc(&fid, size_t(F) * sizeof(int)));
+  Tmp T{t_start.p, t_end.p, t_child.p, t_level.p, t_key.p, t_box.p, t_dim.p, t_sv.p, t_split.p, t_nleft.p,
+        t_mid.p, t_nsplit.p, t_rank.p, t_fin.p, t_esc.p};
+  // scan scratch, sized for the largest scan (F + 1 slots)
+  size_t scan_bytes = 0;
+  OTM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, bl.p, sl.p, int(F + 1), s));
+  DArr<unsigned char> scan_tmp(scan_bytes);
+  int* h_total = nullptr;
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_total), sizeof(int), hipHostMallocDefault));
+
+  const unsigned gF = blocks_for(F), gF1 = blocks_for(F + 1);
+  hipLaunchKernelGGL(k_centroids, dim3(std::min<unsigned>(gF, 8192)), dim3(BT), 0, s, n, d_xy, fid, cx.p, cy.p, pn.p);
+  hipLaunchKernelGGL(k_root, dim3(1), dim3(BT), 0, s, T, int(F));
+  std::vector<std::pair<int, int>> levels;  // (first record, count) per level
+  int b = 0, K = 1;
+  while (K > 0) {
+    levels.push_back({b, K});
+    const unsigned gK = blocks_for(K);
+    hipLaunchKernelGGL(k_lvl_init, dim3(gK), dim3(BT), 0, s, T, b, K);
+    hipLaunchKernelGGL(k_lvl_box, dim3(gF), dim3(BT), 0, s, n, d_xy, (const int*)fid, (const int*)pn.p, T);
+    hipLaunchKernelGGL(k_lvl_split, dim3(gK), dim3(BT), 0, s, T, b, K, leaf_size, max_depth);
+    hipLaunchKernelGGL(k_lvl_left, dim3(gF), dim3(BT), 0, s, int(F), (const double*)cx.p, (const double*)cy.p,
+                       (const int*)pn.p, T, left.p);
+    hipLaunchKernelGGL(k_lvl_bad, dim3(gF1), dim3(BT), 0, s, int(F), (const int*)pn.p, T,
+                       (const unsigned char*)left.p, bl.p, br.p);
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, bl.p, sl.p, int(F + 1), s));
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, br.p, sr.p, int(F + 1), s));
+    hipLaunchKernelGGL(k_lvl_rpos, dim3(gF), dim3(BT), 0, s, int(F), (const int*)pn.p, T, (const int*)br.p,
+                       (const int*)sr.p, rpos.p);
+    hipLaunchKernelGGL(k_lvl_swap, dim3(gF), dim3(BT), 0, s, int(F), (const int*)pn.p, T, (const int*)bl.p,
+                       (const int*)sl.p, (const int*)sr.p, (const int*)rpos.p, fid, cx.p, cy.p);
+    hipLaunchKernelGGL(k_lvl_mid, dim3(gK), dim3(BT), 0, s, T, b, K, (const double*)cx.p, (const double*)cy.p,
+                       nchild.p);
+    // next level: first children at b + K + exclusive scan of the child counts
+    size_t need = 0;
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, nchild.p, noff.p, K + 1, s));
+    if (need > scan_bytes) throw HipError{hipErrorOutOfMemory, "bvh_build_device scan scratch"};
+    OTM_HIP(hipMemsetAsync(nchild.p + K, 0, sizeof(int), s));
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, nchild.p, noff.p, K + 1, s));
+    OTM_HIP(hipMemcpyAsync(h_total, noff.p + K, sizeof(int), hipMemcpyDeviceToHost, s));
+    const int b2 = b + K;
+    hipLaunchKernelGGL(k_lvl_children, dim3(gK), dim3(BT), 0, s, T, b, K, b2, (const int*)noff.p);
+    hipLaunchKernelGGL(k_lvl_assign, dim3(gF), dim3(BT), 0, s, int(F), pn.p, T);
+    OTM_HIP(hipGetLastError());
+    OTM_HIP(hipStreamSynchronize(s));
+    const int K2 = *h_total;
+    if (b2 + long(K2) > NMAX) throw HipError{hipErrorInvalidValue, "bvh_build_device: tree larger than 2F"};
+    b = b2;
+    K = K2;
+  }
+  const int records = b;
+  for (int l = int(levels.size()) - 1; l >= 0; --l)
+    hipLaunchKernelGGL(k_count_split, dim3(blocks_for(levels[l].second)), dim3(BT), 0, s, T, levels[l].first,
+                       levels[l].second);
+  for (const auto& lv : levels)
+    hipLaunchKernelGGL(k_number, dim3(blocks_for(lv.second)), dim3(BT), 0, s, T, lv.first, lv.second);
+  OTM_HIP(hipGetLastError());
+  int nsplit_root = 0;
+  OTM_HIP(hipMemcpyAsync(&nsplit_root, t_nsplit.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  OTM_HIP(hipStreamSynchronize(s));
+  const int total = 1 + 2 * nsplit_root;
+  if (total != records) throw HipError{hipErrorInvalidValue, "bvh_build_device: node count mismatch"};
+  BNode* nodes = nullptr;
+  OTM_HIP(hipMalloc(&nodes, size_t(total) * sizeof(BNode)));
+  hipLaunchKernelGGL(k_write_nodes, dim3(blocks_for(total)), dim3(BT), 0, s, T, total, nodes);
+  OTM_HIP(hipGetLastError());
+  OTM_HIP(hipStreamSynchronize(s));
+  (void)hipHostFree(h_total);
+  *nodes_out = nodes;
+  *faces_out = fid;
+  *num_nodes = total;
+}
+
+}  // namespace otm

@@ -3,16 +3,13 @@
 // Reference: TransportMap::init_inverse / inv / inv_fast / apply_inverse_map
 // (otlib/transport_map.cpp:36-128) over BVH2D (otlib/utils/bvh2d.cpp:29-243).
 //
-// Build (host, the reference's algorithm): face centroids; per node the AABB of
-// its faces' vertices inflated by 2*eps*diag + eps; leaf when <= 4 faces or
-// depth >= 24; otherwise split the longest axis at the float-rounded box
-// midpoint with the reference's in-place two-pointer partition.  The same
-// partition keeps the same face order inside leaves and the same child
-// order, so hits come out in the reference's traversal order.  Large second
-// children are built by worker threads into their own node lists (their face
-// ranges are disjoint) and appended after the first child's subtree, which is
-// exactly where the sequential build numbers them: the node array is the
-// same as a one-thread build.
+// Build: on the device (bvh_build.hip), the reference's tree node for node:
+// face centroids; per node the AABB of its faces' vertices inflated by
+// 2*eps*diag + eps; leaf when <= 4 faces or depth >= 24; otherwise the longest
+// axis split at the float-rounded box midpoint with the reference's in-place
+// two-pointer partition.  The same partition keeps the same face order inside
+// leaves and the same child order, so hits come out in the reference's
+// traversal order.
 //
 // Device layout: nodes flattened to 48-byte records {box, first, count,
 // escape}, where `escape` is the next node in depth-first order once a
@@ -28,8 +25,6 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
-#include <future>
-#include <thread>
 #include <vector>
 
 #include "../../include/otmap_capi.h"
@@ -41,23 +36,7 @@ using namespace otm::geo;
 
 namespace otm {
 
-struct alignas(16) BNode {
-  double lox, loy, hix, hiy;
-  int first;   // inner: first child; leaf: first face slot
-  int count;   // leaf: number of faces; inner: -1
-  int escape;  // next node after this subtree (-1: end)
-  int pad;
-};
-
 namespace {
-
-struct HostBVH {
-  int n = 0;
-  std::vector<P2> pts;
-  std::vector<int> faces;
-  std::vector<P2> cent;
-  std::vector<BNode> nodes;
-};
 
 __host__ __device__ inline void quad_vertices(int n, int f, int* v) {
   const int i = f / n, j = f - (f / n) * n;
@@ -65,151 +44,6 @@ __host__ __device__ inline void quad_vertices(int n, int f, int* v) {
   v[1] = j + (i + 1) * (n + 1);
   v[2] = (j + 1) + (i + 1) * (n + 1);
   v[3] = (j + 1) + i * (n + 1);
-}
-
-int split(HostBVH& B, int start, int end, int dim, float sv) {
-  auto c = [&](int k) { return dim == 0 ? B.cent[k].x : B.cent[k].y; };
-  int l = start, r = end - 1;
-  while (l < r) {
-    while (l < end && c(l) < sv) ++l;
-    while (r >= start && c(r) >= sv) --r;
-    if (l >= r) break;
-    std::swap(B.cent[l], B.cent[r]);
-    std::swap(B.faces[l], B.faces[r]);
-    ++l;
-    --r;
-  }
-  if (l >= end) return end;
-  return c(l) <= sv ? std::min(end, l + 1) : l;
-}
-
-// second children of at least PAR_FACES faces, at most PAR_LEVELS levels
-// down (up to 2^PAR_LEVELS - 1 worker threads), are built concurrently
-constexpr int PAR_FACES = 1 << 16;
-constexpr int PAR_LEVELS = 4;
-constexpr int MAX_THREADS = 16;
-
-// fn(begin, end, t) over up to MAX_THREADS contiguous chunks of [0, count)
-// of at least min_chunk items, chunk 0 on the calling thread
-template <class Fn>
-int parallel_chunks(long count, long min_chunk, Fn fn) {
-  const long hw = std::max(1u, std::thread::hardware_concurrency());
-  const int T = int(std::max(1L, std::min({long(MAX_THREADS), hw, count / std::max(1L, min_chunk)})));
-  if (T <= 1) {
-    fn(0L, count, 0);
-    return 1;
-  }
-  const long per = (count + T - 1) / T;
-  std::vector<std::thread> th;
-  for (int t = 1; t < T; ++t) th.emplace_back(fn, std::min(count, t * per), std::min(count, (t + 1) * per), t);
-  fn(0L, std::min(count, per), 0);
-  for (auto& x : th) x.join();
-  return T;
-}
-
-void build_node(HostBVH& B, std::vector<BNode>& nodes, int id, int start, int end, int level, int leaf_size,
-                int max_depth) {
-  // box of the faces' vertices (min / max are exact, so chunked is the same)
-  double part[MAX_THREADS][4];
-  auto box = [&](long b, long e, int t) {
-    double l0 = DBL_MAX, l1 = DBL_MAX, h0 = -DBL_MAX, h1 = -DBL_MAX;
-    for (long k = start + b; k < start + e; ++k) {
-      int v[4];
-      quad_vertices(B.n, B.faces[k], v);
-      for (int q = 0; q < 4; ++q) {
-        const P2 p = B.pts[v[q]];
-        l0 = std::min(l0, p.x); l1 = std::min(l1, p.y);
-        h0 = std::max(h0, p.x); h1 = std::max(h1, p.y);
-      }
-    }
-    part[t][0] = l0; part[t][1] = l1; part[t][2] = h0; part[t][3] = h1;
-  };
-  const int nt = level == 0 ? parallel_chunks(end - start, 1L << 18, box) : (box(0, end - start, 0), 1);
-  double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
-  for (int t = 0; t < nt; ++t) {
-    lo[0] = std::min(lo[0], part[t][0]); lo[1] = std::min(lo[1], part[t][1]);
-    hi[0] = std::max(hi[0], part[t][2]); hi[1] = std::max(hi[1], part[t][3]);
-  }
-  const double eps = DBL_EPSILON;
-  const double diag[2] = {hi[0] - lo[0], hi[1] - lo[1]};
-  double blo[2], bhi[2];
-  for (int d = 0; d < 2; ++d) {
-    const double pad = diag[d] * (2. * eps) + eps;
-    blo[d] = lo[d] - pad;
-    bhi[d] = hi[d] + pad;
-  }
-  BNode nd;
-  nd.lox = blo[0]; nd.loy = blo[1]; nd.hix = bhi[0]; nd.hiy = bhi[1];
-  nd.first = start;
-  nd.count = std::max(0, end - start);
-  nd.escape = -1;
-  nd.pad = 0;
-  nodes[id] = nd;
-  if (end - start <= leaf_size || level >= max_depth) return;
-  const int dim = diag[1] > diag[0] ? 1 : 0;
-  const float sv = float(0.5 * (bhi[dim] + blo[dim]));
-  const int mid = split(B, start, end, dim, sv);
-  if (mid == start || mid == end) return;
-  const int child = int(nodes.size());
-  nodes[id].first = child;
-  nodes[id].count = -1;
-  nodes.resize(nodes.size() + 2);
-  if (level < PAR_LEVELS && end - mid >= PAR_FACES) {
-    // second child: its own list, local root 0, children numbered from 1
-    std::vector<BNode> sub(1);
-    auto task = std::async(std::launch::async, [&]() {
-      sub.reserve(size_t(end - mid) / 2 + 16);
-      build_node(B, sub, 0, mid, end, level + 1, leaf_size, max_depth);
-    });
-    build_node(B, nodes, child, start, mid, level + 1, leaf_size, max_depth);
-    task.get();
-    const int off = int(nodes.size()) - 1;  // local index k >= 1 -> off + k
-    for (BNode& s : sub)
-      if (s.count < 0) s.first += off;
-    nodes[child + 1] = sub[0];
-    nodes.insert(nodes.end(), sub.begin() + 1, sub.end());
-  } else {
-    build_node(B, nodes, child, start, mid, level + 1, leaf_size, max_depth);
-    build_node(B, nodes, child + 1, mid, end, level + 1, leaf_size, max_depth);
-  }
-}
-
-void build_bvh(HostBVH& B, int n, const double* xy, int leaf_size, int max_depth) {
-  B.n = n;
-  const long V = long(n + 1) * (n + 1), F = long(n) * n;
-  B.pts.resize(V);
-  parallel_chunks(V, 1L << 18, [&](long b, long e, int) {
-    for (long v = b; v < e; ++v) B.pts[v] = {xy[v], xy[V + v]};
-  });
-  B.faces.resize(F);
-  B.cent.resize(F);
-  parallel_chunks(F, 1L << 18, [&](long b, long e, int) {
-    for (long f = b; f < e; ++f) {
-      int v[4];
-      quad_vertices(n, int(f), v);
-      P2 c = {0., 0.};
-      for (int q = 0; q < 4; ++q) { c.x += B.pts[v[q]].x; c.y += B.pts[v[q]].y; }
-      c.x /= 4; c.y /= 4;
-      B.cent[f] = c;
-      B.faces[f] = int(f);
-    }
-  });
-  B.nodes.assign(1, BNode{});
-  B.nodes.reserve(size_t(F / 2 + 16));
-  build_node(B, B.nodes, 0, 0, int(F), 0, leaf_size, max_depth);
-  // escape links in depth-first order
-  std::vector<int> stack = {0};
-  B.nodes[0].escape = -1;
-  while (!stack.empty()) {
-    const int p = stack.back();
-    stack.pop_back();
-    if (B.nodes[p].count >= 0) continue;
-    const int c1 = B.nodes[p].first, c2 = c1 + 1;
-    B.nodes[c1].escape = c2;
-    B.nodes[c2].escape = B.nodes[p].escape;
-    stack.push_back(c2);
-    stack.push_back(c1);
-  }
 }
 
 __device__ inline bool box_contains(const BNode& b, P2 p) {
@@ -332,7 +166,6 @@ struct otm_map {
   hipStream_t stream = nullptr;
   double* xy = nullptr;       // forward positions (device, 2V)
   double* density = nullptr;  // adjusted density (device, F)
-  std::vector<double> h_xy;   // host copy for the build
   BNode* nodes = nullptr;
   int* faces = nullptr;
   int num_nodes = 0;
@@ -361,15 +194,10 @@ void map_release_staging(otm_map* m) {
   m->staged = 0;
 }
 
+// init_inverse(): BVH2D::build(fwd, 4, 24) on the device (otlib/transport_map.cpp:36-43)
 void map_build(otm_map* m) {
   if (m->nodes) return;
-  HostBVH B;
-  build_bvh(B, m->n, m->h_xy.data(), 4, 24);
-  m->num_nodes = int(B.nodes.size());
-  OTM_HIP(hipMalloc(&m->nodes, B.nodes.size() * sizeof(BNode)));
-  OTM_HIP(hipMalloc(&m->faces, B.faces.size() * sizeof(int)));
-  OTM_HIP(hipMemcpy(m->nodes, B.nodes.data(), B.nodes.size() * sizeof(BNode), hipMemcpyHostToDevice));
-  OTM_HIP(hipMemcpy(m->faces, B.faces.data(), B.faces.size() * sizeof(int), hipMemcpyHostToDevice));
+  bvh_build_device(m->n, m->xy, 4, 24, m->stream, &m->nodes, &m->faces, &m->num_nodes);
 }
 
 void launch_inverse(otm_map* m, const double* d_pts, long count, int mode, double* d_out, int* d_face, int* d_nh) {
@@ -394,7 +222,6 @@ int otm_map_create(otm_map** out, int device, int n, const double* fwd_xy, const
     m->device = device;
     m->n = n;
     const size_t V = size_t(n + 1) * (n + 1), F = size_t(n) * n;
-    m->h_xy.assign(fwd_xy, fwd_xy + 2 * V);
     OTM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     OTM_HIP(hipMalloc(&m->xy, 2 * V * sizeof(double)));
     OTM_HIP(hipMalloc(&m->density, F * sizeof(double)));
@@ -413,10 +240,22 @@ int otm_map_create_device(otm_map** out, int device, int n, const double* d_fwd_
   try {
     OTM_HIP(hipSetDevice(device));
     const size_t V = size_t(n + 1) * (n + 1), F = size_t(n) * n;
-    std::vector<double> hxy(2 * V), hd(F);
-    OTM_HIP(hipMemcpy(hxy.data(), d_fwd_xy, 2 * V * sizeof(double), hipMemcpyDeviceToHost));
-    OTM_HIP(hipMemcpy(hd.data(), d_density, F * sizeof(double), hipMemcpyDeviceToHost));
-    return otm_map_create(out, device, n, hxy.data(), hd.data());
+    auto* m = new otm_map;
+    m->device = device;
+    m->n = n;
+    try {
+      OTM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+      OTM_HIP(hipMalloc(&m->xy, 2 * V * sizeof(double)));
+      OTM_HIP(hipMalloc(&m->density, F * sizeof(double)));
+      OTM_HIP(hipMemcpyAsync(m->xy, d_fwd_xy, 2 * V * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
+      OTM_HIP(hipMemcpyAsync(m->density, d_density, F * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
+      OTM_HIP(hipStreamSynchronize(m->stream));
+    } catch (...) {
+      otm_map_destroy(m);
+      throw;
+    }
+    *out = m;
+    return OTM_OK;
   } catch (const HipError& e) {
     return map_fail(e);
   }
@@ -452,6 +291,29 @@ int otm_map_init_inverse(otm_map* m) {
 }
 
 int otm_map_bvh_size(const otm_map* m) { return m ? m->num_nodes : 0; }
+
+int otm_map_bvh_export(otm_map* m, double* boxes, int32_t* first, int32_t* count, int32_t* escape, int32_t* faces) {
+  if (!m || !boxes || !first || !count || !escape || !faces) { set_error("invalid export arguments"); return OTM_ERR_ARG; }
+  try {
+    OTM_HIP(hipSetDevice(m->device));
+    map_build(m);
+    std::vector<BNode> nodes(size_t(m->num_nodes));
+    OTM_HIP(hipMemcpy(nodes.data(), m->nodes, nodes.size() * sizeof(BNode), hipMemcpyDeviceToHost));
+    OTM_HIP(hipMemcpy(faces, m->faces, size_t(m->n) * m->n * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nodes.size(); ++k) {
+      boxes[4 * k + 0] = nodes[k].lox;
+      boxes[4 * k + 1] = nodes[k].loy;
+      boxes[4 * k + 2] = nodes[k].hix;
+      boxes[4 * k + 3] = nodes[k].hiy;
+      first[k] = nodes[k].first;
+      count[k] = nodes[k].count;
+      escape[k] = nodes[k].escape;
+    }
+    return OTM_OK;
+  } catch (const HipError& e) {
+    return map_fail(e);
+  }
+}
 
 int otm_map_inverse(otm_map* m, const double* pts, int64_t count, int mode, double* out, int32_t* face,
                     int32_t* nhits) {

@@ -354,4 +354,19 @@ void driver_invalidate(Driver& d);  // drop the captured graph (tables or shapes
 // plan); returns an OTM_* code, *ms = device time
 int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& opt, float* ms);
 
+// Flattened BVH node of the inverse map (inverse.hip, bvh_build.hip):
+// 48-byte records in the reference's node order.
+struct alignas(16) BNode {
+  double lox, loy, hix, hiy;
+  int first;   // inner: first child; leaf: first face slot
+  int count;   // leaf: number of faces; inner: -1
+  int escape;  // next node after this subtree in depth-first order (-1: end)
+  int pad;
+};
+// BVH2D::build(fwd, leaf_size, max_depth) of the (n+1)^2-vertex quad grid with
+// SoA positions d_xy (device) on stream s: *nodes_out (num_nodes records) and
+// *faces_out (n^2 face slots) are new device allocations owned by the caller.
+void bvh_build_device(int n, const double* d_xy, int leaf_size, int max_depth, hipStream_t s, BNode** nodes_out,
+                      int** faces_out, int* num_nodes);
+
 }  // namespace otm

@@ -72,10 +72,72 @@ def _warped_grid(n):
     return xy
 
 
+def _bvh_export(tm):
+    import ctypes as C
+
+    from otmap_amd import _lib
+    tm.init_inverse()
+    h = tm._handle()
+    k = _lib.lib().otm_map_bvh_size(h)
+    n = tm.fwd_mesh().n
+    boxes = np.zeros(4 * k)
+    first, count, esc = (np.zeros(k, dtype=np.int32) for _ in range(3))
+    faces = np.zeros(n * n, dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+    _lib.check(_lib.lib().otm_map_bvh_export(h, _lib.ptr(boxes), ip(first), ip(count), ip(esc), ip(faces)), "export")
+    return boxes.reshape(k, 4), first, count, esc, faces
+
+
+def _expected_escape(first, count):
+    """Depth-first escape links of a node array in the reference's order."""
+    esc = np.full(len(first), -1, dtype=np.int64)
+    stack = [0]
+    while stack:
+        p = stack.pop()
+        if count[p] >= 0:
+            continue
+        c1 = first[p]
+        esc[c1], esc[c1 + 1] = c1 + 1, esc[p]
+        stack += [c1 + 1, c1]
+    return esc
+
+
+@pytest.mark.parametrize("case", ["ident16", "ident33", "solved64", "warped100", "warped512", "solved1024"])
+def test_device_bvh_build_node_array_vs_oracle(case, oracle):
+    """BVH2D::build (otlib/utils/bvh2d.cpp:29-243) on the device: the node
+    array (inflated boxes bit-exact, children, leaf face ranges, escape links)
+    and the face order of the leaves equal the oracle's sequential
+    restatement of the reference build (itself pinned by the reference's
+    inverse goldens) -- on regular grids, where centroids fall exactly on
+    split values, on warped grids and on solved maps."""
+    if case.startswith("ident"):
+        n = int(case[5:])
+        xy = GridMesh.regular(n).xy
+    elif case.startswith("warped"):
+        n = int(case[6:])
+        xy = _warped_grid(n)
+    else:
+        n = int(case[6:])
+        rho = D.smoothed_shapes(n) if n == 1024 else D.gaussian_mixture(n, 9)
+        xy = oracle.solve(n, rho)["fwd"]
+    print(case, flush=True)
+    tm = TransportMap(GridMesh.regular(n), GridMesh(n, xy), np.ones(n * n))
+    boxes, first, count, esc, faces = _bvh_export(tm)
+    with oracle.bvh(n, xy) as b:
+        oboxes, ometa, ofaces = b.export()
+    assert len(boxes) == len(oboxes)
+    assert np.array_equal(boxes, oboxes)
+    leaf = ometa[:, 2] != 0
+    assert np.array_equal(count < 0, ~leaf)
+    assert np.array_equal(first, ometa[:, 0])
+    assert np.array_equal(count[leaf], ometa[leaf, 1])
+    assert np.array_equal(faces, ofaces)
+    assert np.array_equal(esc, _expected_escape(first, count))
+
+
 def test_inverse_parallel_bvh_build_vs_oracle(oracle):
-    """At n = 512 the BVH's large subtrees are built on worker threads; the
-    node array, hence every hit order and multi-hit selection, must be the
-    one-thread build's (the oracle builds sequentially)."""
+    """At n = 512 the queries against the device-built BVH give the hit
+    order and multi-hit selections of the oracle's sequential build."""
     n = 512
     xy = _warped_grid(n)
     dens = np.random.default_rng(3).uniform(0.5, 2.0, n * n)
