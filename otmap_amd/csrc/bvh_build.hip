@@ -1,7 +1,6 @@
 // BVH over the forward quads built on the device, node for node and face
 // slot for face slot the tree of the reference's BVH2D::build
-// (otlib/utils/bvh2d.cpp:29-58, 167-243) and of the host build in
-// inverse.hip: face centroids; per node the AABB of its faces' vertices
+// (otlib/utils/bvh2d.cpp:29-58, 167-243): face centroids; per node the AABB of its faces' vertices
 // inflated by 2 eps diag + eps; leaf when <= leaf_size faces or depth >=
 // max_depth; otherwise the longest axis is split at the float-rounded box
 // midpoint by the reference's in-place two-pointer partition, and a split
@@ -10,7 +9,8 @@
 // The build is level-synchronous: every node of one depth is processed at
 // once, one thread per face slot.
 //   * Boxes: order-free min / max, as atomics on order-preserving 64-bit keys
-//     (one atomic per wave when the whole wave lies in one node).
+//     (one atomic per workgroup chunk in one node, else per run of one
+//     node's slots in a wave).
 //   * Partition: with L = the faces left of the split value (c < sv) in
 //     [start, end), the two-pointer loop swaps the k-th right face in
 //     [start, start + L) (counted from the left) with the k-th left face in
@@ -24,7 +24,8 @@
 //     Ranks follow from the split-node count of each first child's subtree
 //     (bottom-up), then top-down; escape links (the next node once a subtree
 //     is done: first child -> second child -> parent's escape) likewise.
-// Result: the same BNode array and face order as the host build (tested).
+// Result: the same BNode array and face order as a sequential build of the
+// reference's algorithm (tested against the oracle's).
 
 #include <hipcub/hipcub.hpp>
 
@@ -115,40 +116,102 @@ __global__ __launch_bounds__(BT) void k_lvl_init(Tmp T, int b, int K) {
   T.child[id] = -1;
 }
 
+// A workgroup handles CHUNK consecutive face slots, PER per thread (slot
+// c0 + t + k BT).  Node ranges are contiguous, so when the chunk's first and
+// last slots belong to one node (not to finished leaves, which may enclose
+// active nodes) every slot between does: the workgroup then
+// reduces in registers and LDS and issues one atomic per value; otherwise
+// (chunks across node boundaries) each wave reduces the runs of slots of one
+// node by a segmented scan and issues one atomic per value per run.
+constexpr int PER = 4;
+constexpr int CHUNK = BT * PER;
+
+template <class Op>
+__device__ inline double block_reduce(double v, Op op, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int w = 1; w < BT / 64; ++w) r = op(r, sh[w]);
+  return r;
+}
+
+// inclusive segmented scan of op over the wave's lanes, segments = runs of
+// equal node ids (runs are contiguous: the lanes hold consecutive slots);
+// the last lane of a run ends with the run's total
+template <class T, class Op>
+__device__ inline T seg_scan(T v, int nd, Op op) {
+  const int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const T u = __shfl_up(v, o);
+    const int m = __shfl_up(nd, o);
+    if (lane >= o && m == nd) v = op(v, u);
+  }
+  return v;
+}
+__device__ inline bool run_end(int nd) {
+  const int lane = threadIdx.x & 63;
+  const int nx = __shfl_down(nd, 1);
+  return lane == 63 || nx != nd;
+}
+
+__device__ inline void face_box(int n, long V, const double* __restrict__ xy, int f, double& lx, double& ly,
+                                double& hx, double& hy) {
+  int v[4];
+  quad_v(n, f, v);
+  for (int q = 0; q < 4; ++q) {
+    const double x = xy[v[q]], y = xy[V + v[q]];
+    lx = fmin(lx, x); ly = fmin(ly, y);
+    hx = fmax(hx, x); hy = fmax(hy, y);
+  }
+}
+
 // node boxes: min / max of the vertices of every face of the node
 __global__ __launch_bounds__(BT) void k_lvl_box(int n, const double* __restrict__ xy, const int* __restrict__ fid,
                                                const int* __restrict__ pn, Tmp T) {
+  __shared__ double sh[BT / 64];
   const long F = long(n) * n, V = long(n + 1) * (n + 1);
-  const long p = long(blockIdx.x) * BT + threadIdx.x;
-  const int nd = p < F ? pn[p] : -1;
-  double lx = DBL_MAX, ly = DBL_MAX, hx = -DBL_MAX, hy = -DBL_MAX;
-  if (nd >= 0) {
-    int v[4];
-    quad_v(n, fid[p], v);
-    for (int q = 0; q < 4; ++q) {
-      const double x = xy[v[q]], y = xy[V + v[q]];
-      lx = fmin(lx, x); ly = fmin(ly, y);
-      hx = fmax(hx, x); hy = fmax(hy, y);
+  const long c0 = long(blockIdx.x) * CHUNK, c1 = c0 + CHUNK < F ? c0 + CHUNK : F;
+  const int n0 = pn[c0], n1 = pn[c1 - 1];
+  if (n0 == n1 && n0 >= 0) {  // one node (finished slots at both ends prove nothing)
+    double lx = DBL_MAX, ly = DBL_MAX, hx = -DBL_MAX, hy = -DBL_MAX;
+    for (int k = 0; k < PER; ++k) {
+      const long p = c0 + threadIdx.x + k * BT;
+      if (p < c1) face_box(n, V, xy, fid[p], lx, ly, hx, hy);
     }
+    auto mn = [](double a, double b) { return fmin(a, b); };
+    auto mx = [](double a, double b) { return fmax(a, b); };
+    lx = block_reduce(lx, mn, sh);
+    ly = block_reduce(ly, mn, sh);
+    hx = block_reduce(hx, mx, sh);
+    hy = block_reduce(hy, mx, sh);
+    if (threadIdx.x == 0) {
+      atomicMin(&T.key[4 * n0 + 0], dkey(lx));
+      atomicMin(&T.key[4 * n0 + 1], dkey(ly));
+      atomicMax(&T.key[4 * n0 + 2], dkey(hx));
+      atomicMax(&T.key[4 * n0 + 3], dkey(hy));
+    }
+    return;
   }
-  const int first = __shfl(nd, 0);
-  if (__all(nd == first)) {
-    if (first < 0) return;
-    for (int o = 32; o > 0; o >>= 1) {
-      lx = fmin(lx, __shfl_xor(lx, o)); ly = fmin(ly, __shfl_xor(ly, o));
-      hx = fmax(hx, __shfl_xor(hx, o)); hy = fmax(hy, __shfl_xor(hy, o));
+  auto mn = [](double a, double b) { return fmin(a, b); };
+  auto mx = [](double a, double b) { return fmax(a, b); };
+  for (int k = 0; k < PER; ++k) {
+    const long p = c0 + threadIdx.x + k * BT;
+    const int nd = p < c1 ? pn[p] : -1;
+    double lx = DBL_MAX, ly = DBL_MAX, hx = -DBL_MAX, hy = -DBL_MAX;
+    if (nd >= 0) face_box(n, V, xy, fid[p], lx, ly, hx, hy);
+    if (__all(nd < 0)) continue;
+    lx = seg_scan(lx, nd, mn);
+    ly = seg_scan(ly, nd, mn);
+    hx = seg_scan(hx, nd, mx);
+    hy = seg_scan(hy, nd, mx);
+    if (run_end(nd) && nd >= 0) {  // one atomic per value per run of the wave
+      atomicMin(&T.key[4 * nd + 0], dkey(lx));
+      atomicMin(&T.key[4 * nd + 1], dkey(ly));
+      atomicMax(&T.key[4 * nd + 2], dkey(hx));
+      atomicMax(&T.key[4 * nd + 3], dkey(hy));
     }
-    if ((threadIdx.x & 63) == 0) {
-      atomicMin(&T.key[4 * first + 0], dkey(lx));
-      atomicMin(&T.key[4 * first + 1], dkey(ly));
-      atomicMax(&T.key[4 * first + 2], dkey(hx));
-      atomicMax(&T.key[4 * first + 3], dkey(hy));
-    }
-  } else if (nd >= 0) {
-    atomicMin(&T.key[4 * nd + 0], dkey(lx));
-    atomicMin(&T.key[4 * nd + 1], dkey(ly));
-    atomicMax(&T.key[4 * nd + 2], dkey(hx));
-    atomicMax(&T.key[4 * nd + 3], dkey(hy));
   }
 }
 
@@ -179,26 +242,35 @@ __global__ __launch_bounds__(BT) void k_lvl_split(Tmp T, int b, int K, int leaf_
   T.split[id] = leaf ? 0 : 1;
 }
 
-// left flags (centroid < split value) and L per splitting node
+// left flags (centroid < split value) and L per splitting node (chunks as k_lvl_box)
 __global__ __launch_bounds__(BT) void k_lvl_left(int F, const double* __restrict__ cx, const double* __restrict__ cy,
                                                 const int* __restrict__ pn, Tmp T, unsigned char* __restrict__ left) {
-  const long p = long(blockIdx.x) * BT + threadIdx.x;
-  int nd = p < F ? pn[p] : -1;
-  if (nd >= 0 && !T.split[nd]) nd = -1;
-  int l = 0;
-  if (nd >= 0) {
-    const double c = T.dim[nd] ? cy[p] : cx[p];
-    l = c < double(T.sv[nd]) ? 1 : 0;
+  __shared__ double sh[BT / 64];
+  const long c0 = long(blockIdx.x) * CHUNK, c1 = c0 + CHUNK < F ? c0 + CHUNK : F;
+  const int n0 = pn[c0], n1 = pn[c1 - 1];
+  const bool one = n0 == n1 && n0 >= 0;
+  int tot = 0;
+  for (int k = 0; k < PER; ++k) {
+    const long p = c0 + threadIdx.x + k * BT;
+    int nd = p < c1 ? pn[p] : -1;
+    if (nd >= 0 && !T.split[nd]) nd = -1;
+    int l = 0;
+    if (nd >= 0) {
+      const double c = T.dim[nd] ? cy[p] : cx[p];
+      l = c < double(T.sv[nd]) ? 1 : 0;
+    }
+    if (p < c1) left[p] = (unsigned char)l;
+    if (one) {
+      tot += l;
+      continue;
+    }
+    if (__all(nd < 0)) continue;
+    const int s = seg_scan(l, nd, [](int a, int b) { return a + b; });
+    if (run_end(nd) && nd >= 0 && s) atomicAdd(&T.nleft[nd], s);
   }
-  if (p < F) left[p] = (unsigned char)l;
-  const int first = __shfl(nd, 0);
-  if (__all(nd == first)) {
-    if (first < 0) return;
-    int s = l;
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&T.nleft[first], s);
-  } else if (nd >= 0 && l) {
-    atomicAdd(&T.nleft[nd], 1);
+  if (one && T.split[n0]) {
+    const double s = block_reduce(double(tot), [](double a, double b) { return a + b; }, sh);
+    if (threadIdx.x == 0 && s > 0.) atomicAdd(&T.nleft[n0], int(s));
   }
 }
 
@@ -386,7 +458,7 @@ void bvh_build_device(int n, const double* d_xy, int leaf_size, int max_depth, h
   int* h_total = nullptr;
   OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_total), sizeof(int), hipHostMallocDefault));
 
-  const unsigned gF = blocks_for(F), gF1 = blocks_for(F + 1);
+  const unsigned gF = blocks_for(F), gF1 = blocks_for(F + 1), gC = unsigned((F + CHUNK - 1) / CHUNK);
   hipLaunchKernelGGL(k_centroids, dim3(std::min<unsigned>(gF, 8192)), dim3(BT), 0, s, n, d_xy, fid, cx.p, cy.p, pn.p);
   hipLaunchKernelGGL(k_root, dim3(1), dim3(BT), 0, s, T, int(F));
   std::vector<std::pair<int, int>> levels;  // (first record, count) per level
@@ -395,9 +467,9 @@ void bvh_build_device(int n, const double* d_xy, int leaf_size, int max_depth, h
     levels.push_back({b, K});
     const unsigned gK = blocks_for(K);
     hipLaunchKernelGGL(k_lvl_init, dim3(gK), dim3(BT), 0, s, T, b, K);
-    hipLaunchKernelGGL(k_lvl_box, dim3(gF), dim3(BT), 0, s, n, d_xy, (const int*)fid, (const int*)pn.p, T);
+    hipLaunchKernelGGL(k_lvl_box, dim3(gC), dim3(BT), 0, s, n, d_xy, (const int*)fid, (const int*)pn.p, T);
     hipLaunchKernelGGL(k_lvl_split, dim3(gK), dim3(BT), 0, s, T, b, K, leaf_size, max_depth);
-    hipLaunchKernelGGL(k_lvl_left, dim3(gF), dim3(BT), 0, s, int(F), (const double*)cx.p, (const double*)cy.p,
+    hipLaunchKernelGGL(k_lvl_left, dim3(gC), dim3(BT), 0, s, int(F), (const double*)cx.p, (const double*)cy.p,
                        (const int*)pn.p, T, left.p);
     hipLaunchKernelGGL(k_lvl_bad, dim3(gF1), dim3(BT), 0, s, int(F), (const int*)pn.p, T,
                        (const unsigned char*)left.p, bl.p, br.p);
