@@ -4,13 +4,18 @@
 // A solve of a job table is: density adjustment, field initialisation and the
 // initial residual (otsolver_2dgrid.cpp:106-160), then Newton iterations until
 // every job's device state says done (:160-221), then the forward positions
-// (:223-227).  Iterations are captured once into a HIP graph of ITERS_PER_GRAPH
-// iterations and replayed; after each replay one small kernel counts the jobs
-// still running into one of two pinned words, and the host reads the count of
-// replay k while replay k + 1 runs (the device never waits for the host).
-// Iterations replayed after a job has stopped are no-ops for that job (every
-// kernel checks its gate), so results do not depend on how many replays are
-// in flight.
+// (:223-227).  Iterations are captured once into two HIP graphs, one of
+// ITERS_PER_GRAPH iterations and one of a single iteration, and replayed;
+// after each replay one small kernel counts the jobs still running, and those
+// of them still far from their stop, into one of two pinned slots, and the
+// host reads the counts of replay k while replay k + 1 runs (the device never
+// waits for the host).  Long replays run while some job is far from its stop;
+// once every running job is near it (residual within NEAR_FACTOR of the
+// threshold, or within a long replay of max_iter) the host queues single
+// iterations, so at most about two iterations run gated after the last job
+// stops instead of up to 2 * ITERS_PER_GRAPH - 1.  Iterations replayed after a
+// job has stopped are no-ops for that job (every kernel checks its gate), so
+// results do not depend on how many replays are in flight or how long they are.
 
 #include <cfloat>
 #include <cstring>
@@ -38,6 +43,9 @@ long partial_slots(int n) {
 }
 
 constexpr int ITERS_PER_GRAPH = 4;
+// near the stop: the residual of the last iterations falls by 5-10% per
+// iteration on the benchmark inputs, so 2x the threshold is ~10 iterations out
+constexpr double NEAR_FACTOR = 2.0;
 
 }  // namespace
 
@@ -89,15 +97,17 @@ void driver_create(Driver& d) {
   OTM_HIP(hipEventCreate(&d.ev1));
   OTM_HIP(hipEventCreate(&d.evc[0]));
   OTM_HIP(hipEventCreate(&d.evc[1]));
-  OTM_HIP(hipMalloc(&d.d_open, 2 * sizeof(int)));
-  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), 2 * sizeof(int), hipHostMallocDefault));
+  OTM_HIP(hipMalloc(&d.d_open, 4 * sizeof(int)));
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), 4 * sizeof(int), hipHostMallocDefault));
 }
 
 void driver_invalidate(Driver& d) {
-  if (d.exec) (void)hipGraphExecDestroy(d.exec);
-  if (d.graph) (void)hipGraphDestroy(d.graph);
-  d.exec = nullptr;
-  d.graph = nullptr;
+  for (int g = 0; g < 2; ++g) {
+    if (d.exec[g]) (void)hipGraphExecDestroy(d.exec[g]);
+    if (d.graph[g]) (void)hipGraphDestroy(d.graph[g]);
+    d.exec[g] = nullptr;
+    d.graph[g] = nullptr;
+  }
   d.key = -1;
 }
 
@@ -129,46 +139,51 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
   hipStream_t s = d.stream;
   // the captured iteration depends on the precision mode and the tile shapes
   const long key = long(opt.refine_steps) | (js.version << 8);
-  if (d.exec && d.key != key) driver_invalidate(d);
+  if (d.exec[0] && d.key != key) driver_invalidate(d);
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));
   launch_start(js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, s);
   launch_mprep(js, true, s);  // jobs on the mixed transform: right-hand side of the first iteration
-  if (!d.exec) {
-    OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      for (int k = 0; k < ITERS_PER_GRAPH; ++k) enqueue_iteration(js, plan, opt, s);
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
+  if (!d.exec[0]) {
+    for (int g = 0; g < 2; ++g) {
+      OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int k = 0; k < (g == 0 ? ITERS_PER_GRAPH : 1); ++k) enqueue_iteration(js, plan, opt, s);
+      } catch (...) {
+        hipGraph_t gr = nullptr;
+        (void)hipStreamEndCapture(s, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      OTM_HIP(hipStreamEndCapture(s, &d.graph[g]));
+      OTM_HIP(hipGraphInstantiate(&d.exec[g], d.graph[g], nullptr, nullptr, 0));
     }
-    OTM_HIP(hipStreamEndCapture(s, &d.graph));
-    OTM_HIP(hipGraphInstantiate(&d.exec, d.graph, nullptr, nullptr, 0));
     d.key = key;
   }
   // one replay in flight ahead of the check: replay k + 1 is queued before
-  // the host waits for the open-job count of replay k (slot k % 2)
+  // the host waits for the counts of replay k (slot k % 2); its length follows
+  // the counts of replay k - 1
   long launched = 0;
-  auto replay = [&](int slot) {
-    OTM_HIP(hipGraphLaunch(d.exec, s));
-    launched += ITERS_PER_GRAPH;
-    launch_count_open(js, d.d_open + slot, s);
-    OTM_HIP(hipMemcpyAsync(d.h_open + slot, d.d_open + slot, sizeof(int), hipMemcpyDeviceToHost, s));
+  auto replay = [&](int slot, bool single) {
+    OTM_HIP(hipGraphLaunch(d.exec[single ? 1 : 0], s));
+    launched += single ? 1 : ITERS_PER_GRAPH;
+    launch_count_open(js, d.d_open + 2 * slot, NEAR_FACTOR, ITERS_PER_GRAPH, s);
+    OTM_HIP(hipMemcpyAsync(d.h_open + 2 * slot, d.d_open + 2 * slot, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     OTM_HIP(hipEventRecord(d.evc[slot], s));
   };
   int slot = 0;
-  replay(slot);
+  bool single = false;
+  replay(slot, single);
   for (;;) {
     const bool more = launched <= long(opt.max_iter);
-    if (more) replay(slot ^ 1);
+    if (more) replay(slot ^ 1, single);
     OTM_HIP(hipEventSynchronize(d.evc[slot]));
-    if (d.h_open[slot] == 0) break;
+    if (d.h_open[2 * slot] == 0) break;
     if (!more) {
       set_error("internal error: solve did not reach its stop condition");
       return OTM_ERR_STATE;
     }
+    single = d.h_open[2 * slot + 1] == 0;
     slot ^= 1;
   }
   launch_finish(js, s);

@@ -238,7 +238,10 @@ void launch_dir_coef(const JobSet& js, hipStream_t s);   // k_dir_coef + scalar 
 // the folded BF16 planes of their next right-hand side
 void launch_update(const JobSet& js, hipStream_t s);
 void launch_finish(const JobSet& js, hipStream_t s);     // forward positions, adjusted density copy
-void launch_count_open(const JobSet& js, int* d_open, hipStream_t s);  // jobs whose state is not done
+// d_open[0] = jobs whose state is not done, d_open[1] = those of them with a
+// residual above near_factor x threshold and more than near_iters iterations
+// left before max_iter
+void launch_count_open(const JobSet& js, int* d_open, double near_factor, int near_iters, hipStream_t s);
 void launch_fill(double* p, long count, double v, hipStream_t s);
 // component kernels (single evaluations)
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s);
@@ -338,10 +341,10 @@ bool fields_grow_trace(Fields& f, int cap);  // true when the buffer was realloc
 
 struct Driver {
   hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;     // captured iterations
-  hipGraphExec_t exec = nullptr;
-  long key = -1;                  // precision mode and tile shapes of the captured graph
-  int* d_open = nullptr;          // jobs still running after a replay (device, two slots)
+  hipGraph_t graph[2] = {nullptr, nullptr};     // captured iterations: a long and a one-iteration replay
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  long key = -1;                  // precision mode and tile shapes of the captured graphs
+  int* d_open = nullptr;          // open / far-from-stop job counts after a replay (device, two slots of two)
   int* h_open = nullptr;          // pinned copies
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evc[2] = {nullptr, nullptr};  // the copy of each slot is complete

@@ -883,15 +883,25 @@ __global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
 }
 
 // number of jobs whose state is not done (one workgroup)
-__global__ __launch_bounds__(NT) void k_count_open(const Job* __restrict__ jobs, int count, int* out) {
-  __shared__ int total;
-  if (threadIdx.x == 0) total = 0;
+// out[0]: jobs still running; out[1]: those of them that may run more than
+// `near_iters` further iterations (residual above near_factor x threshold
+// and more than near_iters iterations below max_iter).
+__global__ __launch_bounds__(NT) void k_count_open(const Job* __restrict__ jobs, int count, int* out,
+                                                   double near_factor, int near_iters) {
+  __shared__ int total[2];
+  if (threadIdx.x < 2) total[threadIdx.x] = 0;
   __syncthreads();
-  int c = 0;
-  for (int p = threadIdx.x; p < count; p += NT) c += gp((const SolveState*)jobs[p].f.state)->done ? 0 : 1;
-  if (c) atomicAdd(&total, c);
+  int c = 0, far = 0;
+  for (int p = threadIdx.x; p < count; p += NT) {
+    const auto st = gp((const SolveState*)jobs[p].f.state);
+    if (st->done) continue;
+    ++c;
+    far += (st->e > near_factor * st->threshold && st->it + near_iters < st->max_iter) ? 1 : 0;
+  }
+  if (c) atomicAdd(&total[0], c);
+  if (far) atomicAdd(&total[1], far);
   __syncthreads();
-  if (threadIdx.x == 0) gp(out)[0] = total;
+  if (threadIdx.x < 2) gp(out)[threadIdx.x] = total[threadIdx.x];
 }
 
 __global__ __launch_bounds__(NT) void k_copy_adjusted(JOB_ARGS) {
@@ -1071,8 +1081,9 @@ void launch_finish(const JobSet& js, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
-void launch_count_open(const JobSet& js, int* d_open, hipStream_t s) {
-  hipLaunchKernelGGL(k_count_open, dim3(1), dim3(NT), 0, s, (const Job*)js.jobs, js.count, d_open);
+void launch_count_open(const JobSet& js, int* d_open, double near_factor, int near_iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_open, dim3(1), dim3(NT), 0, s, (const Job*)js.jobs, js.count, d_open, near_factor,
+                     near_iters);
   OTM_HIP(hipGetLastError());
 }
 
