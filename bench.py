@@ -117,12 +117,19 @@ def inverse_throughput(n, d_fwd, d_adj, count, reps):
     from otmap_amd import densities as D
 
     L = _lib.lib()
-    h = _lib.C.c_void_p()
-    _lib.check(L.otm_map_create_device(_lib.C.byref(h), 0, n, d_fwd.data_ptr(), d_adj.data_ptr()), "map")
-    try:
+    # BVH build of a second map of the same mesh: the first build of a process
+    # also allocates the build's device workspace (kept for later builds)
+    t_again = None
+    for _ in range(2):
+        h = _lib.C.c_void_p()
+        _lib.check(L.otm_map_create_device(_lib.C.byref(h), 0, n, d_fwd.data_ptr(), d_adj.data_ptr()), "map")
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         _lib.check(L.otm_map_init_inverse(h), "init_inverse")
-        t_bvh = time.perf_counter() - t0
+        t_bvh, t_again = t_again, time.perf_counter() - t0
+        if t_bvh is None:
+            L.otm_map_destroy(h)
+    try:
         dev = d_fwd.device
         pts = torch.from_numpy(D.uniform_points(count, seed=2000 + n).reshape(-1)).to(dev)
         out = torch.empty(2 * count, dtype=torch.float64, device=dev)
@@ -140,7 +147,8 @@ def inverse_throughput(n, d_fwd, d_adj, count, reps):
         L.otm_map_destroy(h)
     return {"queries": count, "ms": round(ms, 4), "queries_per_s": round(count / (ms * 1e-3)),
             "algorithmic_GB_per_s": round(36.0 * count / (ms * 1e-3) / 1e9, 1), "located": located,
-            "bvh_build_ms": round(t_bvh * 1e3, 2), "mode": "inv (multi-hit selection), FP64"}
+            "bvh_build_ms": round(t_again * 1e3, 2), "bvh_build_first_ms": round(t_bvh * 1e3, 2),
+            "mode": "inv (multi-hit selection), FP64"}
 
 
 def batched_throughput(count, steps, warmup):

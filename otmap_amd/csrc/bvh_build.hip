@@ -26,10 +26,18 @@
 //     is done: first child -> second child -> parent's escape) likewise.
 // Result: the same BNode array and face order as a sequential build of the
 // reference's algorithm (tested against the oracle's).
+//
+// Every level up to the depth limit is enqueued at once: the range of each
+// level (first record, node count) is written on the device by the previous
+// level, node kernels run over the level's largest possible node count and
+// read the actual one, so the host waits once, for the final node count.
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cfloat>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "otm_internal.h"
@@ -97,17 +105,27 @@ __global__ __launch_bounds__(BT) void k_centroids(int n, const double* __restric
   }
 }
 
-__global__ __launch_bounds__(BT) void k_root(Tmp T, int F) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Level table lv (device): lv[2 l] = first record of level l, lv[2 l + 1] =
+// its node count, lv[LV_ERR] != 0 when a level would not fit the records.
+// Every level of the depth limit is enqueued at once; node kernels read
+// their level's range from the table (levels past the last one are empty).
+__device__ inline int lv_err_index(int levels) { return 2 * levels; }
+
+__global__ __launch_bounds__(BT) void k_root(Tmp T, int F, int* __restrict__ lv, int levels) {
+  if (blockIdx.x != 0) return;
+  for (int i = threadIdx.x; i <= 2 * levels; i += BT) lv[i] = 0;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  lv[1] = 1;  // level 0: the root alone
   T.start[0] = 0;
   T.end[0] = F;
   T.level[0] = 0;
 }
 
-__global__ __launch_bounds__(BT) void k_lvl_init(Tmp T, int b, int K) {
+__global__ __launch_bounds__(BT) void k_lvl_init(Tmp T, const int* __restrict__ lv, int l) {
   const int k = blockIdx.x * BT + threadIdx.x;
-  if (k >= K) return;
-  const int id = b + k;
+  if (k >= lv[2 * l + 1]) return;
+  const int id = lv[2 * l] + k;
   T.key[4 * id + 0] = ~0ull;
   T.key[4 * id + 1] = ~0ull;
   T.key[4 * id + 2] = 0ull;
@@ -216,10 +234,11 @@ __global__ __launch_bounds__(BT) void k_lvl_box(int n, const double* __restrict_
 }
 
 // inflated box, leaf test, split axis and value (otlib/utils/bvh2d.cpp:193-221)
-__global__ __launch_bounds__(BT) void k_lvl_split(Tmp T, int b, int K, int leaf_size, int max_depth) {
+__global__ __launch_bounds__(BT) void k_lvl_split(Tmp T, const int* __restrict__ lv, int l, int leaf_size,
+                                                 int max_depth) {
   const int k = blockIdx.x * BT + threadIdx.x;
-  if (k >= K) return;
-  const int id = b + k;
+  if (k >= lv[2 * l + 1]) return;
+  const int id = lv[2 * l] + k;
   const double lo[2] = {dkey_inv(T.key[4 * id + 0]), dkey_inv(T.key[4 * id + 1])};
   const double hi[2] = {dkey_inv(T.key[4 * id + 2]), dkey_inv(T.key[4 * id + 3])};
   const double eps = DBL_EPSILON;
@@ -327,12 +346,18 @@ __global__ __launch_bounds__(BT) void k_lvl_swap(int F, const int* __restrict__ 
 }
 
 // the partition's middle; a split that leaves a side empty makes a leaf;
-// child counts for the next level
-__global__ __launch_bounds__(BT) void k_lvl_mid(Tmp T, int b, int K, const double* __restrict__ cx,
-                                               const double* __restrict__ cy, int* __restrict__ nchild) {
+// child counts for the next level (zero from K to the level's bound, so a
+// scan over the bound + 1 entries gives the total at index K)
+__global__ __launch_bounds__(BT) void k_lvl_mid(Tmp T, const int* __restrict__ lv, int l, int bound,
+                                               const double* __restrict__ cx, const double* __restrict__ cy,
+                                               int* __restrict__ nchild) {
   const int k = blockIdx.x * BT + threadIdx.x;
-  if (k >= K) return;
-  const int id = b + k;
+  if (k > bound) return;
+  if (k >= lv[2 * l + 1]) {
+    nchild[k] = 0;
+    return;
+  }
+  const int id = lv[2 * l] + k;
   int two = 0;
   if (T.split[id]) {
     const int s = T.start[id], e = T.end[id];
@@ -350,11 +375,27 @@ __global__ __launch_bounds__(BT) void k_lvl_mid(Tmp T, int b, int K, const doubl
   nchild[k] = two;
 }
 
-// child records of the next level (first children at b2 + exclusive scan)
-__global__ __launch_bounds__(BT) void k_lvl_children(Tmp T, int b, int K, int b2, const int* __restrict__ off) {
+// child records of the next level (first children at b2 = b + K + exclusive
+// scan); the next level's range goes to the table.  A binary tree over F
+// faces with non-empty leaves has at most 2F - 1 records; a level that would
+// pass `records` is not written and raises the table's error flag.
+__global__ __launch_bounds__(BT) void k_lvl_children(Tmp T, int* __restrict__ lv, int l, int levels, long records,
+                                                    const int* __restrict__ off) {
   const int k = blockIdx.x * BT + threadIdx.x;
+  const int b = lv[2 * l], K = lv[2 * l + 1];
+  const int b2 = b + K;
+  const bool fits = long(b2) + off[K] <= records;
+  if (k == 0 && l + 1 < levels) {
+    lv[2 * (l + 1)] = b2;
+    lv[2 * (l + 1) + 1] = fits ? off[K] : 0;
+  }
+  if (k == 0 && (!fits || (l + 1 == levels && off[K] > 0))) lv[lv_err_index(levels)] = 1;
   if (k >= K) return;
   const int id = b + k;
+  if (!fits) {  // never for a well-formed tree: the level's nodes end as leaves
+    T.split[id] = 0;
+    return;
+  }
   if (!T.split[id]) return;
   const int c = b2 + off[k];
   T.child[id] = c;
@@ -379,19 +420,19 @@ __global__ __launch_bounds__(BT) void k_lvl_assign(int F, int* __restrict__ pn, 
 }
 
 // split nodes per subtree, one level (bottom-up)
-__global__ __launch_bounds__(BT) void k_count_split(Tmp T, int b, int K) {
+__global__ __launch_bounds__(BT) void k_count_split(Tmp T, const int* __restrict__ lv, int l) {
   const int k = blockIdx.x * BT + threadIdx.x;
-  if (k >= K) return;
-  const int id = b + k;
+  if (k >= lv[2 * l + 1]) return;
+  const int id = lv[2 * l] + k;
   const int c = T.child[id];
   T.nsplit[id] = c < 0 ? 0 : 1 + T.nsplit[c] + T.nsplit[c + 1];
 }
 
 // preorder ranks, final indices and escape links, one level (top-down)
-__global__ __launch_bounds__(BT) void k_number(Tmp T, int b, int K) {
+__global__ __launch_bounds__(BT) void k_number(Tmp T, const int* __restrict__ lv, int l) {
   const int k = blockIdx.x * BT + threadIdx.x;
-  if (k >= K) return;
-  const int id = b + k;
+  if (k >= lv[2 * l + 1]) return;
+  const int id = lv[2 * l] + k;
   if (id == 0) {
     T.rank[0] = 0;
     T.fin[0] = 0;
@@ -426,96 +467,130 @@ __global__ __launch_bounds__(BT) void k_write_nodes(Tmp T, int total, BNode* __r
 
 inline unsigned blocks_for(long count) { return unsigned(std::max<long>(1, (count + BT - 1) / BT)); }
 
-template <class T>
-struct DArr {
-  T* p = nullptr;
-  explicit DArr(size_t count) { OTM_HIP(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T))); }
-  ~DArr() { if (p) (void)hipFree(p); }
+// Temporaries of the build, kept per device and grown to the largest grid
+// built so far (a build on a 1024^2 grid needs ~0.3 GB, 2048^2 ~1.2 GB), so
+// repeated builds allocate nothing.  Builds on one device are serialised.
+struct Work {
+  long F = 0;
+  int levels = 0;
+  size_t scan_bytes = 0;
+  void* blob = nullptr;
+  double *cx, *cy;
+  int *pn, *bl, *br, *sl, *sr, *rpos, *nchild, *noff, *lv;
+  unsigned char *left, *scan_tmp;
+  Tmp T;
 };
+
+std::mutex g_work_mu;
+std::map<int, Work> g_work;
+
+Work& work_for(int device, long F, int levels) {
+  Work& w = g_work[device];
+  if (w.blob && w.F >= F && w.levels >= levels) return w;
+  if (w.blob) (void)hipFree(w.blob);
+  w = Work();
+  const long NMAX = 2 * F + 2;
+  size_t scan_bytes = 0;
+  OTM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int*)nullptr, (int*)nullptr, int(F + 1)));
+  // (byte offset, pointer) pairs filled in two passes: sizes, then addresses
+  std::vector<std::pair<void**, size_t>> pieces;
+  Tmp& T = w.T;
+  auto add = [&](auto*& p, size_t count) { pieces.push_back({reinterpret_cast<void**>(&p), count * sizeof(*p)}); };
+  add(w.cx, F); add(w.cy, F); add(w.pn, F); add(w.bl, F + 1); add(w.br, F + 1); add(w.sl, F + 1); add(w.sr, F + 1);
+  add(w.rpos, F); add(w.left, F); add(w.nchild, F + 1); add(w.noff, F + 2); add(w.lv, 2 * levels + 2);
+  add(w.scan_tmp, scan_bytes);
+  add(T.start, NMAX); add(T.end, NMAX); add(T.child, NMAX); add(T.level, NMAX); add(T.key, 4 * NMAX);
+  add(T.box, 4 * NMAX); add(T.dim, NMAX); add(T.sv, NMAX); add(T.split, NMAX); add(T.nleft, NMAX); add(T.mid, NMAX);
+  add(T.nsplit, NMAX); add(T.rank, NMAX); add(T.fin, NMAX); add(T.esc, NMAX);
+  size_t total = 0;
+  std::vector<size_t> offs;
+  for (const auto& pc : pieces) {
+    total = (total + 255) & ~size_t(255);
+    offs.push_back(total);
+    total += std::max<size_t>(pc.second, 1);
+  }
+  OTM_HIP(hipMalloc(&w.blob, total));
+  for (size_t i = 0; i < pieces.size(); ++i) *pieces[i].first = static_cast<char*>(w.blob) + offs[i];
+  w.F = F;
+  w.levels = levels;
+  w.scan_bytes = scan_bytes;
+  return w;
+}
 
 }  // namespace
 
+// The whole build is enqueued without waiting on the device: every level up
+// to the depth limit is launched (node kernels over the level's largest
+// possible node count, min(2^l, F)), the level ranges stay in the device
+// table, and the host reads the node count once at the end to size the
+// node array.
 void bvh_build_device(int n, const double* d_xy, int leaf_size, int max_depth, hipStream_t s, BNode** nodes_out,
                       int** faces_out, int* num_nodes) {
   const long F = long(n) * n;
   const long NMAX = 2 * F + 2;  // tree records: at most 2F - 1
-  DArr<double> cx(F), cy(F);
-  DArr<int> pn(F), bl(F + 1), br(F + 1), sl(F + 1), sr(F + 1), rpos(F);
-  DArr<unsigned char> left(F);
-  DArr<int> t_start(NMAX), t_end(NMAX), t_child(NMAX), t_level(NMAX), t_dim(NMAX), t_split(NMAX), t_nleft(NMAX),
-      t_mid(NMAX), t_nsplit(NMAX), t_rank(NMAX), t_fin(NMAX), t_esc(NMAX), nchild(NMAX), noff(NMAX + 1);
-  DArr<u64> t_key(4 * NMAX);
-  DArr<double> t_box(4 * NMAX);
-  DArr<float> t_sv(NMAX);
+  const int levels = max_depth + 1;  // a node at depth max_depth is a leaf
+  int device = 0;
+  OTM_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(g_work_mu);
+  Work& w = work_for(device, F, levels);
+  const Tmp& T = w.T;
   int* fid = nullptr;
   OTM_HIP(hipMalloc(&fid, size_t(F) * sizeof(int)));
-  Tmp T{t_start.p, t_end.p, t_child.p, t_level.p, t_key.p, t_box.p, t_dim.p, t_sv.p, t_split.p, t_nleft.p,
-        t_mid.p, t_nsplit.p, t_rank.p, t_fin.p, t_esc.p};
-  // scan scratch, sized for the largest scan (F + 1 slots)
-  size_t scan_bytes = 0;
-  OTM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, bl.p, sl.p, int(F + 1), s));
-  DArr<unsigned char> scan_tmp(scan_bytes);
-  int* h_total = nullptr;
-  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_total), sizeof(int), hipHostMallocDefault));
+  size_t scan_bytes = w.scan_bytes;
 
   const unsigned gF = blocks_for(F), gF1 = blocks_for(F + 1), gC = unsigned((F + CHUNK - 1) / CHUNK);
-  hipLaunchKernelGGL(k_centroids, dim3(std::min<unsigned>(gF, 8192)), dim3(BT), 0, s, n, d_xy, fid, cx.p, cy.p, pn.p);
-  hipLaunchKernelGGL(k_root, dim3(1), dim3(BT), 0, s, T, int(F));
-  std::vector<std::pair<int, int>> levels;  // (first record, count) per level
-  int b = 0, K = 1;
-  while (K > 0) {
-    levels.push_back({b, K});
+  hipLaunchKernelGGL(k_centroids, dim3(std::min<unsigned>(gF, 8192)), dim3(BT), 0, s, n, d_xy, fid, w.cx, w.cy,
+                     w.pn);
+  hipLaunchKernelGGL(k_root, dim3(1), dim3(BT), 0, s, T, int(F), w.lv, levels);
+  std::vector<int> bound(levels);
+  for (int l = 0; l < levels; ++l) {
+    const int K = bound[l] = int(std::min<long>(F, l < 31 ? (1l << l) : F));
     const unsigned gK = blocks_for(K);
-    hipLaunchKernelGGL(k_lvl_init, dim3(gK), dim3(BT), 0, s, T, b, K);
-    hipLaunchKernelGGL(k_lvl_box, dim3(gC), dim3(BT), 0, s, n, d_xy, (const int*)fid, (const int*)pn.p, T);
-    hipLaunchKernelGGL(k_lvl_split, dim3(gK), dim3(BT), 0, s, T, b, K, leaf_size, max_depth);
-    hipLaunchKernelGGL(k_lvl_left, dim3(gC), dim3(BT), 0, s, int(F), (const double*)cx.p, (const double*)cy.p,
-                       (const int*)pn.p, T, left.p);
-    hipLaunchKernelGGL(k_lvl_bad, dim3(gF1), dim3(BT), 0, s, int(F), (const int*)pn.p, T,
-                       (const unsigned char*)left.p, bl.p, br.p);
-    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, bl.p, sl.p, int(F + 1), s));
-    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, br.p, sr.p, int(F + 1), s));
-    hipLaunchKernelGGL(k_lvl_rpos, dim3(gF), dim3(BT), 0, s, int(F), (const int*)pn.p, T, (const int*)br.p,
-                       (const int*)sr.p, rpos.p);
-    hipLaunchKernelGGL(k_lvl_swap, dim3(gF), dim3(BT), 0, s, int(F), (const int*)pn.p, T, (const int*)bl.p,
-                       (const int*)sl.p, (const int*)sr.p, (const int*)rpos.p, fid, cx.p, cy.p);
-    hipLaunchKernelGGL(k_lvl_mid, dim3(gK), dim3(BT), 0, s, T, b, K, (const double*)cx.p, (const double*)cy.p,
-                       nchild.p);
-    // next level: first children at b + K + exclusive scan of the child counts
+    hipLaunchKernelGGL(k_lvl_init, dim3(gK), dim3(BT), 0, s, T, (const int*)w.lv, l);
+    hipLaunchKernelGGL(k_lvl_box, dim3(gC), dim3(BT), 0, s, n, d_xy, (const int*)fid, (const int*)w.pn, T);
+    hipLaunchKernelGGL(k_lvl_split, dim3(gK), dim3(BT), 0, s, T, (const int*)w.lv, l, leaf_size, max_depth);
+    hipLaunchKernelGGL(k_lvl_left, dim3(gC), dim3(BT), 0, s, int(F), (const double*)w.cx, (const double*)w.cy,
+                       (const int*)w.pn, T, w.left);
+    hipLaunchKernelGGL(k_lvl_bad, dim3(gF1), dim3(BT), 0, s, int(F), (const int*)w.pn, T,
+                       (const unsigned char*)w.left, w.bl, w.br);
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, scan_bytes, w.bl, w.sl, int(F + 1), s));
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, scan_bytes, w.br, w.sr, int(F + 1), s));
+    hipLaunchKernelGGL(k_lvl_rpos, dim3(gF), dim3(BT), 0, s, int(F), (const int*)w.pn, T, (const int*)w.br,
+                       (const int*)w.sr, w.rpos);
+    hipLaunchKernelGGL(k_lvl_swap, dim3(gF), dim3(BT), 0, s, int(F), (const int*)w.pn, T, (const int*)w.bl,
+                       (const int*)w.sl, (const int*)w.sr, (const int*)w.rpos, fid, w.cx, w.cy);
+    // child counts over the bound (zero past K), their scan, the next level
+    hipLaunchKernelGGL(k_lvl_mid, dim3(blocks_for(long(K) + 1)), dim3(BT), 0, s, T, (const int*)w.lv, l, K,
+                       (const double*)w.cx, (const double*)w.cy, w.nchild);
     size_t need = 0;
-    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, nchild.p, noff.p, K + 1, s));
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w.nchild, w.noff, K + 1, s));
     if (need > scan_bytes) throw HipError{hipErrorOutOfMemory, "bvh_build_device scan scratch"};
-    OTM_HIP(hipMemsetAsync(nchild.p + K, 0, sizeof(int), s));
-    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, nchild.p, noff.p, K + 1, s));
-    OTM_HIP(hipMemcpyAsync(h_total, noff.p + K, sizeof(int), hipMemcpyDeviceToHost, s));
-    const int b2 = b + K;
-    hipLaunchKernelGGL(k_lvl_children, dim3(gK), dim3(BT), 0, s, T, b, K, b2, (const int*)noff.p);
-    hipLaunchKernelGGL(k_lvl_assign, dim3(gF), dim3(BT), 0, s, int(F), pn.p, T);
+    OTM_HIP(hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, scan_bytes, w.nchild, w.noff, K + 1, s));
+    hipLaunchKernelGGL(k_lvl_children, dim3(gK), dim3(BT), 0, s, T, w.lv, l, levels, NMAX, (const int*)w.noff);
+    hipLaunchKernelGGL(k_lvl_assign, dim3(gF), dim3(BT), 0, s, int(F), w.pn, T);
     OTM_HIP(hipGetLastError());
-    OTM_HIP(hipStreamSynchronize(s));
-    const int K2 = *h_total;
-    if (b2 + long(K2) > NMAX) throw HipError{hipErrorInvalidValue, "bvh_build_device: tree larger than 2F"};
-    b = b2;
-    K = K2;
   }
-  const int records = b;
-  for (int l = int(levels.size()) - 1; l >= 0; --l)
-    hipLaunchKernelGGL(k_count_split, dim3(blocks_for(levels[l].second)), dim3(BT), 0, s, T, levels[l].first,
-                       levels[l].second);
-  for (const auto& lv : levels)
-    hipLaunchKernelGGL(k_number, dim3(blocks_for(lv.second)), dim3(BT), 0, s, T, lv.first, lv.second);
+  for (int l = levels - 1; l >= 0; --l)
+    hipLaunchKernelGGL(k_count_split, dim3(blocks_for(bound[l])), dim3(BT), 0, s, T, (const int*)w.lv, l);
+  for (int l = 0; l < levels; ++l)
+    hipLaunchKernelGGL(k_number, dim3(blocks_for(bound[l])), dim3(BT), 0, s, T, (const int*)w.lv, l);
   OTM_HIP(hipGetLastError());
-  int nsplit_root = 0;
-  OTM_HIP(hipMemcpyAsync(&nsplit_root, t_nsplit.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  // the one wait of the build: node count (1 + 2 x split nodes) and the
+  // level table's error flag
+  int h[2] = {0, 0};
+  OTM_HIP(hipMemcpyAsync(&h[0], T.nsplit, sizeof(int), hipMemcpyDeviceToHost, s));
+  OTM_HIP(hipMemcpyAsync(&h[1], w.lv + 2 * levels, sizeof(int), hipMemcpyDeviceToHost, s));
   OTM_HIP(hipStreamSynchronize(s));
-  const int total = 1 + 2 * nsplit_root;
-  if (total != records) throw HipError{hipErrorInvalidValue, "bvh_build_device: node count mismatch"};
+  if (h[1]) {
+    (void)hipFree(fid);
+    throw HipError{hipErrorInvalidValue, "bvh_build_device: tree larger than 2F records"};
+  }
+  const int total = 1 + 2 * h[0];
   BNode* nodes = nullptr;
   OTM_HIP(hipMalloc(&nodes, size_t(total) * sizeof(BNode)));
   hipLaunchKernelGGL(k_write_nodes, dim3(blocks_for(total)), dim3(BT), 0, s, T, total, nodes);
   OTM_HIP(hipGetLastError());
   OTM_HIP(hipStreamSynchronize(s));
-  (void)hipHostFree(h_total);
   *nodes_out = nodes;
   *faces_out = fid;
   *num_nodes = total;
