@@ -74,6 +74,7 @@ class Oracle:
             "orc_inverse": (None, [_vp, _dp, _dp, _l, _i, _dp, _ip, _ip]),
             "orc_locate_all": (_i, [_i, _dp, _d, _d, _ip, _i]),
             "orc_rasterize": (None, [_i, _dp, C.POINTER(C.c_double), _i, _i, _dp]),
+            "orc_raster_stats": (None, [_i, _dp, _i, _i, C.POINTER(C.c_longlong)]),
             "orc_transport_cost": (_d, [_i, _dp, _dp, _dp, C.POINTER(C.c_double)]),
             "orc_solver_transport_cost": (_d, [_i, _dp, _dp, C.POINTER(C.c_double)]),
             "orc_set_area_precision": (None, [_i]),
@@ -191,6 +192,14 @@ class Oracle:
         img = np.zeros(rows * cols)
         self.L.orc_rasterize(n, _f64(xy), _nullable(density), rows, cols, img)
         return img.reshape(rows, cols)
+
+    def raster_stats(self, n, xy, rows, cols):
+        """How often rasterize's carried weights (D < 0 pixels) decide a pixel."""
+        out = (C.c_longlong * 7)()
+        self.L.orc_raster_stats(n, _f64(xy), rows, cols, out)
+        keys = ("pixels", "failed", "carried_same_face", "carried_prev_face", "uncovered_no_weights",
+                "failed_but_covered", "max_faces_back")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def transport_cost(self, n, src, dst, density, per_face=False):
         cost = np.zeros(n * n) if per_face else None
@@ -372,6 +381,14 @@ class Reference:
         img = np.zeros(rows * cols)
         self.L.ref_rasterize(n, _f64(xy), _nullable(density), rows, cols, img)
         return img.reshape(rows, cols)
+
+    def raster_stats(self, n, xy, rows, cols):
+        """How often rasterize's carried weights (D < 0 pixels) decide a pixel."""
+        out = (C.c_longlong * 7)()
+        self.L.orc_raster_stats(n, _f64(xy), rows, cols, out)
+        keys = ("pixels", "failed", "carried_same_face", "carried_prev_face", "uncovered_no_weights",
+                "failed_but_covered", "max_faces_back")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def transport_cost(self, n, src, dst, density, per_face=False):
         cost = np.zeros(n * n) if per_face else None

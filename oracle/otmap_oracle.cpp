@@ -911,6 +911,57 @@ void orc_rasterize(int n, const double* xy, const double* density, int rows, int
   }
 }
 
+// How often the carried weights of orc_rasterize decide a pixel, on a given
+// mesh (same scan as orc_rasterize).  out[0] pixels scanned, out[1] pixels
+// whose bilinear solve fails (D < 0), out[2] of those, weights carried from
+// an earlier pixel of the same face, out[3] carried from a preceding face,
+// out[4] no successful solve before them (not covered), out[5] failing pixels
+// that end up covered (carried weights all in [0, 1]), out[6] the largest
+// number of faces between a failing pixel's face and the face whose solve
+// its weights come from (the device looks back at most 64 faces).
+void orc_raster_stats(int n, const double* xy, int rows, int cols, long long* out) {
+  const long V = long(n + 1) * (n + 1), F = long(n) * n;
+  for (int k = 0; k < 7; ++k) out[k] = 0;
+  double w[4] = {-1., -1., -1., -1.};
+  long last_ok_face = -1;
+  for (long f = 0; f < F; ++f) {
+    int v[4];
+    quad_vertices(n, int(f), v);
+    P2 p[4];
+    for (int t = 0; t < 4; ++t) p[t] = {xy[v[t]], xy[V + v[t]]};
+    double lo[2] = {p[0].x, p[0].y}, hi[2] = {p[0].x, p[0].y};
+    for (int t = 1; t < 4; ++t) {
+      lo[0] = std::min(lo[0], p[t].x); lo[1] = std::min(lo[1], p[t].y);
+      hi[0] = std::max(hi[0], p[t].x); hi[1] = std::max(hi[1], p[t].y);
+    }
+    const int x0 = std::max(int(lo[0] * double(cols)), 0), y0 = std::max(int(lo[1] * double(rows)), 0);
+    const int x1 = std::min(int(hi[0] * double(cols)) + 1, cols), y1 = std::min(int(hi[1] * double(rows)) + 1, rows);
+    for (int y = y0; y < y1; ++y)
+      for (int x = x0; x < x1; ++x) {
+        ++out[0];
+        const P2 c = {(double(x) + 0.5) / double(cols), (double(y) + 0.5) / double(rows)};
+        if (bilinear_weights(c, p, w)) {
+          last_ok_face = f;
+          continue;
+        }
+        ++out[1];
+        if (last_ok_face < 0) {
+          ++out[4];
+          continue;
+        }
+        if (last_ok_face == f) {
+          ++out[2];
+        } else {
+          ++out[3];
+          out[6] = std::max<long long>(out[6], f - last_ok_face);
+        }
+        bool ok = true;
+        for (int t = 0; t < 4; ++t) ok = ok && w[t] <= 1.0f && w[t] >= 0.0f;
+        if (ok) ++out[5];
+      }
+  }
+}
+
 // transport_cost on a quad grid mesh  (otlib/transport_map.cpp:131-215).
 // With one density value per quad the reference takes its triangle branch on
 // every quad: the triangle (v3, v0, v1) given by the face's first halfedge.

@@ -197,3 +197,34 @@ def test_raster_identity_closed_form():
     img = rasterize_image(GridMesh.regular(n))
     expect = float(np.float32(1.0 / (n * n))) / (1.0 / (n * n))
     assert np.allclose(img, expect, rtol=1e-12)
+
+
+def test_raster_carried_weights_on_barycentre_meshes(oracle):
+    """The rasteriser's carried weights (pixels whose bilinear solve fails,
+    otlib/utils/rasterizer.cpp:169-171) on the meshes the barycentre workload
+    rasterises: the bilerped inverse meshes of the four 512^2 shape densities
+    (apps/barycenters.cpp:187-245) at five of its 25 weights, 512 x 512
+    pixels.  Every pixel equals the oracle's raster of the same mesh, and the
+    carried weights never come from more than 64 faces back (the device's
+    look-back bound); the counts are printed."""
+    import ctypes as C
+
+    import torch
+
+    from otmap_amd import BatchSolver, SolverOptions, _lib, pipelines
+
+    n = 512
+    maps = BatchSolver([n] * 4).solve(D.barycentre_inputs(n, count=4), SolverOptions())
+    invs = pipelines.inverse_meshes_device(maps)
+    ptrs = (C.c_void_p * 4)(*[t.data_ptr() for t in invs])
+    count = 2 * (n + 1) ** 2
+    mesh = torch.empty(count, dtype=torch.float64, device="cuda:0")
+    L = _lib.lib()
+    for a, b in [(0.0, 0.0), (1.0, 0.0), (0.0, 1.0), (1.0, 1.0), (0.5, 0.5)]:
+        _lib.check(L.otm_mesh_lerp_device(0, count, 4, ptrs, a, b, mesh.data_ptr()), "mesh_lerp")
+        xy = mesh.cpu().numpy()
+        img = rasterize_image(GridMesh(n, xy), None, shape=(n, n))
+        assert np.array_equal(img, oracle.rasterize(n, xy, n, n)), (a, b)
+        st = oracle.raster_stats(n, xy, n, n)
+        print(f"carried weights at ({a}, {b}): {st}")
+        assert st["max_faces_back"] < 64, st

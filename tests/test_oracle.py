@@ -311,6 +311,27 @@ def test_raster_identity_closed_form(oracle):
     assert np.allclose(img, expect, rtol=1e-12)
 
 
+def test_raster_carried_weights_decide_reference_pixels(oracle):
+    """On the reference's own golden meshes the bilinear solve fails at some
+    pixel centres and the carried weights decide at least one pixel that the
+    reference's image covers (the images are matched bit for bit by
+    test_raster_vs_reference, so the carried-weights emulation is exercised);
+    the weights never come from more than a few faces back, far inside the
+    device's 64-face look-back.  An identity mesh never fails."""
+    g = load("raster.npz")
+    decided = 0
+    for n in (16, 32):
+        for rows, cols in ((n, n), (2 * n, 2 * n), (24, 40)):
+            st = oracle.raster_stats(n, g[f"n{n}_fwd"], rows, cols)
+            assert st["failed"] == st["carried_same_face"] + st["carried_prev_face"] + st["uncovered_no_weights"]
+            assert st["max_faces_back"] < 64
+            decided += st["failed_but_covered"]
+    assert decided >= 1
+    n = 16
+    ident = np.concatenate([np.repeat(np.arange(n + 1.0), n + 1), np.tile(np.arange(n + 1.0), n + 1)]) / n
+    assert oracle.raster_stats(n, ident, 37, 29)["failed"] == 0
+
+
 @pytest.mark.parametrize("n", [16, 32])
 def test_transport_cost_vs_reference(n, oracle):
     g = load("raster.npz")
