@@ -7,8 +7,8 @@
 //   * with Eigen on the include path solve() takes Eigen::Ref<const
 //     Eigen::VectorXd>, as the reference; without it, any container with
 //     data()/size() (std::vector<double>), or (pointer, size);
-//   * SolverOptions carries three extra fields, `precision`, `refine_steps`
-//     and `mixed_min_n`;
+//   * SolverOptions carries four extra fields, `precision`, `refine_steps`,
+//     `mixed_min_n` and `transform`;
 //   * errors of the HIP layer are reported on std::cerr and leave an empty
 //     map (the reference has no error channel: it prints and continues).
 #pragma once
@@ -29,6 +29,8 @@ enum struct BetaOpt { Zero, ConjugateJacobian };
 
 // otlib/otsolver_2dgrid.h:37-43 (+ arithmetic of the Laplacian solve)
 enum struct Precision { FP64, Mixed };
+// FP64 Laplacian solves: DCT as dense GEMMs on FP64 MFMA, or by fast transforms
+enum struct Transform { GEMM, FFT };
 
 struct SolverOptions {
   BetaOpt beta = BetaOpt::ConjugateJacobian;
@@ -38,6 +40,7 @@ struct SolverOptions {
   Precision precision = Precision::FP64;
   int refine_steps = 1;
   int mixed_min_n = 1024;  // mixed: grids from this size up run the BF16x3 transform
+  Transform transform = Transform::GEMM;
 };
 
 // otlib/otsolver_2dgrid.h:45-58
@@ -75,6 +78,7 @@ class GridBasedTransportSolver {
     o.precision = opt.precision == Precision::Mixed ? OTM_PRECISION_MIXED : OTM_PRECISION_FP64;
     o.refine_steps = opt.refine_steps;
     o.mixed_min_n = opt.mixed_min_n;
+    o.transform = opt.transform == Transform::FFT ? OTM_TRANSFORM_FFT : OTM_TRANSFORM_GEMM;
     const size_t V = size_t(n + 1) * (n + 1);
     std::vector<double> xy(2 * V);
     auto dens = std::make_shared<DensityVector>(long(n) * n);

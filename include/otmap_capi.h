@@ -40,6 +40,8 @@ extern "C" {
 enum { OTM_BETA_ZERO = 0, OTM_BETA_CONJUGATE_JACOBIAN = 1 };
 /* arithmetic of the Laplacian solve and stencils */
 enum { OTM_PRECISION_FP64 = 0, OTM_PRECISION_MIXED = 1 };
+/* how the FP64 Laplacian solves run (otm_options.transform) */
+enum { OTM_TRANSFORM_GEMM = 0, OTM_TRANSFORM_FFT = 1 };
 /* why a solve stopped */
 enum { OTM_EXIT_NONE = 0, OTM_EXIT_CONVERGED_OR_MAX_ITER = 1, OTM_EXIT_ZERO_DIRECTION = 2 };
 
@@ -54,6 +56,11 @@ typedef struct otm_options {
   int mixed_min_n;   /* mixed only: grids with n >= mixed_min_n (and n % 64 == 0) run the BF16x3
                         transform, smaller grids the FP64 transform (default 1024; 0 = BF16x3 for
                         every supported size) */
+  int transform;     /* grids on the FP64 transform: OTM_TRANSFORM_GEMM (default: the DCT as folded
+                        dense GEMMs on FP64 MFMA) or OTM_TRANSFORM_FFT (the same DCTs by fast
+                        transforms, for n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0; other sizes keep
+                        the GEMMs).  No counterpart in the reference, whose solve is a sparse
+                        Cholesky back-substitution. */
 } otm_options;
 
 typedef struct otm_stats {
@@ -157,6 +164,8 @@ int otm_laplacian_solve(otm_solver* s, const double* r, double* d_hat);
 /* same solve in the mixed-precision form (3-term BF16 split GEMMs + `refine_steps`
  * FP64 refinement steps); n must be a multiple of 64 */
 int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int refine_steps);
+/* same solve by fast DCTs (OTM_TRANSFORM_FFT); n must satisfy its size rule */
+int otm_laplacian_solve_fft(otm_solver* s, const double* r, double* d_hat);
 int otm_line_search_coefficients(otm_solver* s, const double* g0, const double* dir, double* a, double* b);
 /* Step choice of solve_1D_problem (otlib/otsolver_2dgrid.cpp:563-595) on the
  * device, for `count` coefficient vectors z (5 doubles each: e(t) = z0 + z1 t

@@ -8,10 +8,10 @@ plus BatchSolver (many densities of uneven sizes solved together), backed by
 hand-written HIP kernels in libotmap_amd.so (C ABI: include/otmap_capi.h).
 """
 from .batch import BatchSolver
-from .solver import BetaOpt, GridBasedTransportSolver, Precision, SolverOptions, SolveStats
+from .solver import BetaOpt, GridBasedTransportSolver, Precision, SolverOptions, SolveStats, Transform
 from .transport_map import (GridMesh, TransportMap, apply_inverse_map, grid_transport_cost, rasterize_image,
                             transport_cost)
 
-__all__ = ["BatchSolver", "BetaOpt", "GridBasedTransportSolver", "Precision", "SolverOptions", "SolveStats",
+__all__ = ["BatchSolver", "BetaOpt", "GridBasedTransportSolver", "Precision", "SolverOptions", "SolveStats", "Transform",
            "GridMesh", "TransportMap", "apply_inverse_map", "grid_transport_cost", "rasterize_image",
            "transport_cost"]

@@ -19,6 +19,7 @@ OTM_OK = 0
 OTM_ERR_ARG, OTM_ERR_HIP, OTM_ERR_STATE, OTM_ERR_ALLOC = -1, -2, -3, -4
 OTM_BETA_ZERO, OTM_BETA_CONJUGATE_JACOBIAN = 0, 1
 OTM_PRECISION_FP64, OTM_PRECISION_MIXED = 0, 1
+OTM_TRANSFORM_GEMM, OTM_TRANSFORM_FFT = 0, 1
 OTM_EXIT_NONE, OTM_EXIT_CONVERGED_OR_MAX_ITER, OTM_EXIT_ZERO_DIRECTION = 0, 1, 2
 (OTM_KERNEL_DCT_GEMM, OTM_KERNEL_BETA, OTM_KERNEL_DIR_COEF, OTM_KERNEL_UPDATE, OTM_KERNEL_DCT_SOLVE,
  OTM_KERNEL_MIXED_GEMM, OTM_KERNEL_MIXED_SOLVE) = range(7)
@@ -27,7 +28,7 @@ OTM_EXIT_NONE, OTM_EXIT_CONVERGED_OR_MAX_ITER, OTM_EXIT_ZERO_DIRECTION = 0, 1, 2
 class OtmOptions(C.Structure):
     _fields_ = [("beta", C.c_int), ("max_iter", C.c_int), ("threshold", C.c_double),
                 ("max_ratio", C.c_double), ("precision", C.c_int), ("refine_steps", C.c_int),
-                ("mixed_min_n", C.c_int)]
+                ("mixed_min_n", C.c_int), ("transform", C.c_int)]
 
 
 class OtmStats(C.Structure):
@@ -64,6 +65,7 @@ _SIGNATURES = {
     "otm_map_bvh_export": (C.c_int, [_vp, _dp, _ip, _ip, _ip, _ip]),
     "otm_line_search_steps": (C.c_int, [C.c_int, _dp, C.c_int, _dp, _dp]),
     "otm_laplacian_solve_mixed": (C.c_int, [_vp, _dp, _dp, C.c_int]),
+    "otm_laplacian_solve_fft": (C.c_int, [_vp, _dp, _dp]),
     "otm_line_search_coefficients": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "otm_map_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _dp, _dp]),
     "otm_map_create_device": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp]),

@@ -37,6 +37,7 @@ SOURCES = [
     "batch.hip",
     "pipeline.hip",
     "bvh_build.hip",
+    "fft.hip",
 ]
 
 FLAGS = [

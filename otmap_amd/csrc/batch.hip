@@ -64,6 +64,7 @@ void build_table(otm_batch* b) {
     jobs[p].P = pl.P;
     jobs[p].Pt = pl.Pt;
     jobs[p].sq = pl.sq;
+    jobs[p].fft = pl.fft;
     auto it = b->mplans.find(b->n[p]);
     if (it != b->mplans.end()) {
       jobs[p].Pp = it->second.Pp;
@@ -76,7 +77,7 @@ void build_table(otm_batch* b) {
   const auto jcfg = b->js.jcfg;
   const auto jmcfg = b->js.jmcfg;
   const auto jroute = b->js.jroute;
-  const int routed = b->js.routed_mixed;
+  const int routed = b->js.routed_mixed, routed_fft = b->js.routed_fft;
   jobset_destroy(b->js);
   jobset_create(b->js, jobs);
   if (had) {
@@ -84,6 +85,7 @@ void build_table(otm_batch* b) {
     b->js.jmcfg = jmcfg;
     b->js.jroute = jroute;
     b->js.routed_mixed = routed;
+    b->js.routed_fft = routed_fft;
     jobset_assign(b->js);
   }
 }
@@ -109,13 +111,20 @@ int solve_batch(otm_batch* b, const double* const* d_density, const otm_options*
     b->trace_cap = cap;
     rebuild = true;
   }
+  if (opt.transform != OTM_TRANSFORM_GEMM && opt.transform != OTM_TRANSFORM_FFT) {
+    set_error("unknown transform");
+    return OTM_ERR_ARG;
+  }
   // each grid's transform: mixed precision runs the BF16x3 transform on the
-  // grids from mixed_min_n up, the FP64 transform on the others
+  // grids from mixed_min_n up; the others run the FP64 transform, as GEMMs or
+  // as fast DCTs (opt.transform, where the size allows)
   std::vector<int> route(b->count, 0);
-  if (opt.precision == OTM_PRECISION_MIXED)
-    for (int p = 0; p < b->count; ++p) route[p] = mixed_route(b->n[p], opt.mixed_min_n);
+  for (int p = 0; p < b->count; ++p) {
+    if (opt.precision == OTM_PRECISION_MIXED) route[p] = mixed_route(b->n[p], opt.mixed_min_n);
+    if (route[p] == 0) route[p] = fp64_route(b->n[p], opt.transform);
+  }
   for (int p = 0; p < b->count; ++p)
-    if (route[p] && !b->f[p].x0) {  // mixed buffers and BF16 factor planes on first use
+    if (route[p] == 1 && !b->f[p].x0) {  // mixed buffers and BF16 factor planes on first use
       fields_enable_mixed(b->f[p]);
       if (!b->mplans.count(b->n[p])) mixed_plan_create(b->mplans[b->n[p]], b->plans.at(b->n[p]));
       rebuild = true;

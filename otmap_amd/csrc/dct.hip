@@ -454,9 +454,11 @@ void dct_plan_create(DctPlan& p, int n) {
     OTM_HIP(hipMemcpy(p.P, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice));
     OTM_HIP(hipMemcpy(p.Pt, Pt.data(), Pt.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  fft_plan_create(p.fft, n);
 }
 
 void dct_plan_destroy(DctPlan& p) {
+  if (p.fft) (void)hipFree(p.fft);
   if (p.C) (void)hipFree(p.C);
   if (p.Ct) (void)hipFree(p.Ct);
   if (p.sq) (void)hipFree(p.sq);

@@ -124,11 +124,13 @@ void driver_destroy(Driver& d) {
   d = Driver();
 }
 
-// One iteration of every job: each job's Laplacian solve on its route (jobs
-// on the mixed transform find their folded right-hand side written by the
-// previous update pass), then the shared beta, line-search and update passes.
+// One iteration of every job: each job's Laplacian solve on its route (FP64
+// GEMMs, fast DCTs, or the mixed transform, whose jobs find their folded
+// right-hand side written by the previous update pass), then the shared
+// beta, line-search and update passes.
 void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s) {
-  if (js.routed_mixed < js.count) launch_dct_solve(js, plan, s);
+  if (js.routed_mixed + js.routed_fft < js.count) launch_dct_solve(js, plan, s);
+  if (js.routed_fft > 0) launch_fft_solve(js, true, s);
   if (js.routed_mixed > 0) launch_mixed_solve(js, opt.refine_steps, true, s, false);
   launch_beta(js, s);
   launch_dir_coef(js, s);

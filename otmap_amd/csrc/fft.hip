@@ -1,0 +1,365 @@
+// FP64 Laplacian solve by fast DCTs (the transform route OTM_TRANSFORM_FFT).
+//
+// The same exact solve as the GEMM routes (dct.hip): with the orthonormal
+// DCT-II C of size n, the pinned Neumann Laplacian of the reference
+// (otlib/otsolver_2dgrid.cpp:284-321, solve and mean removal :174-183) is
+//   d_hat = C^T [ (C R' C^T) / lambda ] C,   R' = R - (sum r) e_0 e_0^T,
+// lambda_kl = 2 (sin^2(pi (k-l)/2n) + sin^2(pi (k+l)/2n)), d_hat(0,0)-mode = 0.
+// Here every 1-D DCT runs as an n-point complex FFT, O(n^2 log n) work per
+// solve instead of the 4 n^3 of the folded GEMMs, so the solve is bound by
+// memory traffic (three passes of one read and one write of the n x n field).
+//
+// 1-D DCT-II of a real row x (Makhoul): v[m] = x[2m], v[n-1-m] = x[2m+1];
+// V = FFT(v); X[k] = s_k Re(e^{-i pi k/2n} V[k]), s_0 = sqrt(1/n), s_k =
+// sqrt(2/n).  Two real rows a, b share one complex FFT of z = v_a + i v_b:
+// V_a[k] = (Z[k] + conj Z[n-k]) / 2, V_b[k] = (Z[k] - conj Z[n-k]) / 2i.
+// DCT-III (the inverse): with U[k] = X[k] / s_k (U[n] = 0),
+// V[k] = e^{i pi k/2n} (U[k] - i U[n-k]); v = IFFT(V); x[2m] = v[m],
+// x[2m+1] = v[n-1-m]; again two rows per complex sequence.
+//
+// Passes (one workgroup per 4 rows, i.e. two complex sequences):
+//   1  rows of R' -> DCT-II along j, written transposed (B^T, 32-byte runs)
+//   2  rows of B^T (columns of B) -> DCT-II along i, / lambda, DCT-III along
+//      i, written transposed back (spatial i, frequency l along j)
+//   3  rows -> DCT-III along j = d_hat, plus the |d_hat|^2 partials
+// The FFT is a self-sorting (Stockham) mixed-radix FFT, radices 4, 2, 3, in
+// two LDS buffers; the twiddle table exp(-2 pi i t/n) is staged in LDS.
+// Supported sizes: n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0.
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+#include "otm_internal.h"
+#include "reduce.h"
+
+namespace otm {
+
+bool fft_supported(int n) {
+  if (n < 8 || n > FFT_MAX_N || n % 4 != 0) return false;
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  while (m % 3 == 0) m /= 3;
+  return m == 1;
+}
+
+void fft_plan_create(double*& table, int n) {
+  table = nullptr;
+  if (!fft_supported(n)) return;
+  // twiddles exp(-2 pi i t / n), t < n, then phases exp(-i pi k / 2n), k < n
+  std::vector<double> h(4 * size_t(n));
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int t = 0; t < n; ++t) {
+    const long double a = 2.0L * pi * t / n, b = pi * t / (2.0L * n);
+    h[2 * t] = double(cosl(a));
+    h[2 * t + 1] = double(-sinl(a));
+    h[2 * n + 2 * t] = double(cosl(b));
+    h[2 * n + 2 * t + 1] = double(-sinl(b));
+  }
+  OTM_HIP(hipMalloc(&table, h.size() * sizeof(double)));
+  OTM_HIP(hipMemcpy(table, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+namespace {
+
+typedef double cd __attribute__((ext_vector_type(2)));  // complex (re, im)
+
+constexpr int FT = 256;  // threads per workgroup (4 rows of one grid)
+
+__device__ inline cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline cd conjc(cd a) { return {a.x, -a.y}; }
+__device__ inline cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ inline cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by -i (sign < 0) or +i (sign > 0)
+__device__ inline cd mul_i(cd a, int sign) { return sign < 0 ? cd{a.y, -a.x} : cd{-a.y, a.x}; }
+
+// One radix-R stage of the self-sorting FFT of the two sequences held in src
+// (seq q at src[q N .. q N + N)), written to dst: butterfly b of sequence q,
+// j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{sign 2 pi i
+// / (L R)}), and its DFT_R output q' goes to dst[m L R + j + q' L].
+template <int R>
+__device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
+                                 int L, int sign) {
+  const int NR = N / R, tstep = N / (L * R);
+  for (int b2 = threadIdx.x; b2 < 2 * NR; b2 += FT) {
+    const int q = b2 >= NR ? 1 : 0;
+    const int b = b2 - q * NR;
+    const int j = b % L, m = b / L;
+    const cd* s = src + q * N;
+    cd a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cd v = s[b + r * NR];
+      if (r > 0 && j > 0) {
+        const cd w = tw[j * r * tstep];
+        v = cmul(v, sign < 0 ? w : conjc(w));
+      }
+      a[r] = v;
+    }
+    cd c[R];
+    if (R == 2) {
+      c[0] = cadd(a[0], a[1]);
+      c[1] = csub(a[0], a[1]);
+    } else if (R == 4) {
+      const cd s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+      const cd s13 = cadd(a[1], a[3]), d13 = mul_i(csub(a[1], a[3]), sign);
+      c[0] = cadd(s02, s13);
+      c[1] = cadd(d02, d13);
+      c[2] = csub(s02, s13);
+      c[3] = csub(d02, d13);
+    } else {  // R == 3: w3 = e^{sign 2 pi i / 3} = (-1/2, sign sqrt(3)/2)
+      const double h3 = 0.86602540378443864676;  // sqrt(3) / 2
+      const cd s12 = cadd(a[1], a[2]), d12 = csub(a[1], a[2]);
+      const cd t = {a[0].x - 0.5 * s12.x, a[0].y - 0.5 * s12.y};
+      const cd u = mul_i(cd{h3 * d12.x, h3 * d12.y}, sign);  // sign i sqrt(3)/2 (a1 - a2)
+      c[0] = cadd(a[0], s12);
+      c[1] = cadd(t, u);
+      c[2] = csub(t, u);
+    }
+    cd* d = dst + q * N + m * L * R + j;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r * L] = c[r];
+  }
+}
+
+// FFT of both sequences of `a` (length N each), `b` scratch; returns the
+// buffer that holds the result.  Radices 4 first, then 2, then 3.
+__device__ inline cd* fft2(cd* a, cd* b, const cd* tw, int N, int sign) {
+  int rest = N, L = 1;
+  while (rest > 1) {
+    if (rest % 4 == 0) {
+      fft_stage<4>(a, b, tw, N, L, sign);
+      L *= 4;
+      rest /= 4;
+    } else if (rest % 2 == 0) {
+      fft_stage<2>(a, b, tw, N, L, sign);
+      L *= 2;
+      rest /= 2;
+    } else {
+      fft_stage<3>(a, b, tw, N, L, sign);
+      L *= 3;
+      rest /= 3;
+    }
+    __syncthreads();
+    cd* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// position of x[c] in the permuted sequence v (v[m] = x[2m], v[N-1-m] = x[2m+1])
+__device__ inline int perm_pos(int c, int N) { return (c & 1) ? N - 1 - (c >> 1) : (c >> 1); }
+
+__device__ inline double dct_scale(int k, int N) { return k == 0 ? sqrt(1.0 / N) : sqrt(2.0 / N); }
+
+// forward post-processing of sequence q at frequency k: the DCT-II
+// coefficients of its two rows (real part row 2q, imaginary part row 2q+1)
+__device__ inline void dct2_post(const cd* z, const cd* ph, int N, int k, double& xa, double& xb) {
+  const cd Z = z[k], Zc = conjc(z[k == 0 ? 0 : N - k]);
+  const cd va = {0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y)};
+  const cd dd = csub(Z, Zc);
+  const cd vb = {0.5 * dd.y, -0.5 * dd.x};  // (Z - Zc) / 2i
+  const cd p = ph[k];
+  const double s = dct_scale(k, N);
+  xa = s * (p.x * va.x - p.y * va.y);
+  xb = s * (p.x * vb.x - p.y * vb.y);
+}
+
+// inverse pre-processing at frequency k from the DCT-II coefficients of two
+// rows (a = row 2q, b = row 2q+1): Z[k] = V_a[k] + i V_b[k]
+__device__ inline cd dct3_pre(double xa, double xan, double xb, double xbn, const cd* ph, int N, int k) {
+  const double s = dct_scale(k, N), sn = dct_scale(N - k, N);
+  const double ua = xa / s, ub = xb / s;
+  const double uan = k == 0 ? 0. : xan / sn, ubn = k == 0 ? 0. : xbn / sn;
+  const cd e = conjc(ph[k]);                      // e^{i pi k / 2N}
+  const cd va = cmul(e, cd{ua, -uan}), vb = cmul(e, cd{ub, -ubn});
+  return {va.x - vb.y, va.y + vb.x};  // V_a + i V_b
+}
+
+struct FftLoc {
+  int p, lt, nb;
+};
+__device__ inline FftLoc fft_locate(const int* __restrict__ starts, int count) {
+  FftLoc L;
+  L.p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
+  L.lt = count == 1 ? int(blockIdx.x) : int(blockIdx.x) - starts[L.p];
+  L.nb = count == 1 ? int(gridDim.x) : starts[L.p + 1] - starts[L.p];
+  return L;
+}
+
+// LDS: buffers A and B of 2 sequences x N complex each, then the twiddles
+// (N complex): 80 N bytes.
+__device__ inline void stage_twiddles(cd* twl, const double* table, int N) {
+  const auto tw = gp(reinterpret_cast<const cd*>(table));
+  for (int t = threadIdx.x; t < N; t += FT) twl[t] = tw[t];
+}
+
+// Pass 1: rows of R' (R minus (sum r) at (0,0) when rhs), DCT-II along the
+// row, written transposed: out[k * n + row].
+__global__ __launch_bounds__(FT) void k_fft_rows_fwd(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                    int count, bool gate, bool rhs) {
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int N = J.f.n, r0 = 4 * L.lt;
+  cd* A = fsm;
+  cd* B = fsm + 2 * N;
+  cd* twl = fsm + 4 * N;
+  stage_twiddles(twl, J.fft, N);
+  const auto in = gp((const double*)J.f.r);
+  const double S = rhs ? st->sum_r : 0.;
+  for (int e = threadIdx.x; e < 4 * N; e += FT) {
+    const int rr = e / N, c = e - rr * N;
+    double v = in[long(r0 + rr) * N + c];
+    if (rhs && r0 + rr == 0 && c == 0) v -= S;
+    double* slot = reinterpret_cast<double*>(A + (rr >> 1) * N + perm_pos(c, N));
+    slot[rr & 1] = v;
+  }
+  __syncthreads();
+  const cd* z = fft2(A, B, twl, N, -1);
+  const cd* ph = reinterpret_cast<const cd*>(J.fft) + N;
+  const auto out = gp(J.f.tmp0);
+  for (int k = threadIdx.x; k < N; k += FT) {
+    double x0, x1, x2, x3;
+    dct2_post(z, ph, N, k, x0, x1);
+    dct2_post(z + N, ph, N, k, x2, x3);
+    auto o = reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + r0);
+    o[0] = cd{x0, x1};
+    o[1] = cd{x2, x3};
+  }
+}
+
+// Pass 2: rows of B^T (row l: DCT frequency l along j), DCT-II along i,
+// divided by lambda(k, l) (the (0,0) mode set to 0), DCT-III along i,
+// written transposed: out[i * n + l].
+__global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                int count, bool gate) {
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int N = J.f.n, l0 = 4 * L.lt;
+  cd* A = fsm;
+  cd* B = fsm + 2 * N;
+  cd* twl = fsm + 4 * N;
+  stage_twiddles(twl, J.fft, N);
+  const auto in = gp((const double*)J.f.tmp0);
+  for (int e = threadIdx.x; e < 4 * N; e += FT) {
+    const int rr = e / N, c = e - rr * N;
+    double* slot = reinterpret_cast<double*>(A + (rr >> 1) * N + perm_pos(c, N));
+    slot[rr & 1] = in[long(l0 + rr) * N + c];
+  }
+  __syncthreads();
+  cd* z = fft2(A, B, twl, N, -1);
+  cd* other = z == A ? B : A;
+  const cd* ph = reinterpret_cast<const cd*>(J.fft) + N;
+  const auto sq = gp(J.sq);
+  // spectrum / lambda as 4 real rows in the free buffer
+  double* X = reinterpret_cast<double*>(other);
+  for (int k = threadIdx.x; k < N; k += FT) {
+    double x[4];
+    dct2_post(z, ph, N, k, x[0], x[1]);
+    dct2_post(z + N, ph, N, k, x[2], x[3]);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int l = l0 + rr;
+      const int dm = k > l ? k - l : l - k;
+      const double lam = 2. * (sq[dm] + sq[k + l]);
+      X[rr * N + k] = (k == 0 && l == 0) ? 0. : x[rr] / lam;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < N; k += FT) {
+    const int kn = k == 0 ? 0 : N - k;
+    z[k] = dct3_pre(X[k], X[kn], X[N + k], X[N + kn], ph, N, k);
+    z[N + k] = dct3_pre(X[2 * N + k], X[2 * N + kn], X[3 * N + k], X[3 * N + kn], ph, N, k);
+  }
+  __syncthreads();
+  const cd* y = fft2(z, other, twl, N, +1);
+  const auto out = gp(J.f.tmp1);
+  const double inv = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += FT) {
+    const int m = perm_pos(i, N);
+    const cd u = y[m], v = y[N + m];
+    auto o = reinterpret_cast<OTM_GLOBAL cd*>(out + long(i) * N + l0);
+    o[0] = cd{u.x * inv, u.y * inv};
+    o[1] = cd{v.x * inv, v.y * inv};
+  }
+}
+
+// Pass 3: rows (spatial i, frequency l along j), DCT-III along j = d_hat,
+// and one |d_hat|^2 partial per workgroup.
+__global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                    int count, bool gate) {
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  if (gate && st->done) return;
+  const int N = J.f.n, r0 = 4 * L.lt;
+  cd* A = fsm;
+  cd* B = fsm + 2 * N;
+  cd* twl = fsm + 4 * N;
+  stage_twiddles(twl, J.fft, N);
+  const auto in = gp((const double*)J.f.tmp1);
+  const cd* ph = reinterpret_cast<const cd*>(J.fft) + N;
+  for (int k = threadIdx.x; k < N; k += FT) {
+    const int kn = k == 0 ? 0 : N - k;
+    const auto a = in + long(r0) * N;
+    A[k] = dct3_pre(a[k], a[kn], a[N + k], a[N + kn], ph, N, k);
+    A[N + k] = dct3_pre(a[2 * N + k], a[2 * N + kn], a[3 * N + k], a[3 * N + kn], ph, N, k);
+  }
+  __syncthreads();
+  const cd* y = fft2(A, B, twl, N, +1);
+  const auto out = gp(J.f.dhat);
+  const double inv = 1.0 / N;
+  double nrm[1] = {0.};
+  for (int e = threadIdx.x; e < 4 * N; e += FT) {
+    const int rr = e / N, c = e - rr * N;
+    const cd u = y[(rr >> 1) * N + perm_pos(c, N)];
+    const double v = ((rr & 1) ? u.y : u.x) * inv;
+    out[long(r0 + rr) * N + c] = v;
+    nrm[0] += v * v;
+  }
+  // workgroup partial: DPP wave sums, then the four wave totals in a fixed
+  // order through LDS (buffer A is dead once the result is in y = B or A is
+  // read: the barrier below orders it), so the kernel has no static LDS
+  wave_sum<1>(nrm);
+  double* red = reinterpret_cast<double*>(y == A ? B : A);
+  if ((threadIdx.x & 63) == 63) red[threadIdx.x >> 6] = nrm[0];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    gp(J.f.site(SITE_DNORM))[L.lt] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace
+
+// d_hat of every job on the FFT route (launch kind KIND_FFT, n/4 workgroups
+// per job); dynamic LDS sized for the largest such grid.
+void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s) {
+  if (js.total[KIND_FFT] == 0) return;
+  int nmax = 0;
+  for (int p = 0; p < js.count; ++p)
+    if (js.jroute[p] == 2) nmax = std::max(nmax, js.host[p].f.n);
+  const size_t lds = size_t(80) * nmax;
+  static bool attr = false;  // the kernels use no static LDS: up to 80 FFT_MAX_N bytes dynamic
+  if (!attr) {
+    const int cap = 80 * FFT_MAX_N;
+    for (const void* k : {reinterpret_cast<const void*>(&k_fft_rows_fwd), reinterpret_cast<const void*>(&k_fft_cols),
+                          reinterpret_cast<const void*>(&k_fft_rows_inv)})
+      OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+    attr = true;
+  }
+  const dim3 grid(unsigned(js.total[KIND_FFT]));
+  const int* st = js.starts_of(KIND_FFT);
+  hipLaunchKernelGGL(k_fft_rows_fwd, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate, true);
+  hipLaunchKernelGGL(k_fft_cols, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate);
+  hipLaunchKernelGGL(k_fft_rows_inv, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate);
+  OTM_HIP(hipGetLastError());
+}
+
+}  // namespace otm

@@ -130,6 +130,7 @@ struct Job {
   const unsigned short* Pp = nullptr;   // BF16 planes of P (3 x n*h) for the mixed solve
   const unsigned short* Ptp = nullptr;  // BF16 planes of Pt (3 x h*n)
   const double* ck = nullptr;           // cos(pi perm(r) / n), r < n, perm = even/odd order (mixed solve)
+  const double* fft = nullptr;          // FFT route tables of f.n: twiddles e^{-2 pi i t/n}, then DCT phases e^{-i pi k/2n} (4n doubles)
   const double* dens_in = nullptr;  // input density of the current solve (F)
   double* fwd_out = nullptr;        // forward vertex positions (2V, SoA)
   double* adj_out = nullptr;        // copy of the adjusted density (F, nullable)
@@ -168,10 +169,10 @@ constexpr int MIXED_TM[NMIXED] = {64, 64, 128, 128};
 
 inline int mres_tiles(int n) { return (n / 32) * (n / 64); }
 inline int upair_tiles(int n) { return tiles_i(n) * ((tiles_j(n) + 1) / 2); }
-// Every job runs its Laplacian solve through one of two transforms, its
-// route: the FP64 transform (route 0) or the mixed BF16x3 transform with FP64
-// refinement (route 1).  Launch kinds of one transform cover only the jobs
-// routed to it.
+// Every job runs its Laplacian solve through one of three transforms, its
+// route: the FP64 GEMM transform (route 0), the mixed BF16x3 transform with
+// FP64 refinement (route 1) or the FP64 fast-DCT transform (route 2,
+// fft.hip).  Launch kinds of one transform cover only the jobs routed to it.
 // KIND_PLAIN64: one workgroup per 64 x 64 output tile (unfolded FP64 GEMMs);
 // fold_kind(s, c) / mixed_kind(s, c): stage s of the folded FP64 / mixed
 // transform with tile configuration c, over the jobs assigned configuration
@@ -185,7 +186,8 @@ enum LaunchKind {
   KIND_DNORM = 5,      // each job's workgroups in the last stage of its transform (writers of the |d_hat|^2 partials)
   KIND_STENCIL64 = 6,  // stencil tiles of the jobs on the FP64 transform (their plain update pass)
   KIND_FLATMX = 7,     // grid-stride blocks of the jobs on the mixed transform (first right-hand side)
-  KIND_M0 = 8,
+  KIND_FFT = 8,        // fast-DCT passes: one workgroup per 4 rows of each job on the FFT route
+  KIND_M0 = 9,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
@@ -208,8 +210,9 @@ struct JobSet {
   bool tuning = false;     // autotuner table: every valid configuration covers every job
   std::vector<std::array<int, 4>> jcfg;   // per job: folded FP64 tile shape per stage
   std::vector<std::array<int, 4>> jmcfg;  // per job: mixed tile configuration per stage
-  std::vector<int> jroute;                // per job: 0 FP64 transform, 1 mixed transform
+  std::vector<int> jroute;                // per job: 0 FP64 GEMM transform, 1 mixed transform, 2 FP64 fast DCT
   int routed_mixed = 0;                   // jobs on the mixed transform
+  int routed_fft = 0;                     // jobs on the fast-DCT transform
   int cfg[4] = {FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT, FOLD_DEFAULT};  // of the largest grid (reported)
   int mcfg[4] = {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT};
   long version = 0;        // unique per table and assignment (captured graphs depend on it)
@@ -225,6 +228,11 @@ void jobset_route(JobSet& js, const std::vector<int>& route);
 // transform from mixed_min_n up (and only where it is supported), the FP64
 // transform below, where it is both faster and exact.
 inline int mixed_route(int n, int mixed_min_n) { return n % 64 == 0 && n >= 64 && n >= mixed_min_n ? 1 : 0; }
+// Fast-DCT transform (fft.hip): n = 2^a 3^b, 8 <= n <= FFT_MAX_N, n % 4 == 0.
+constexpr int FFT_MAX_N = 2048;
+bool fft_supported(int n);
+// FP64 route of a grid: the fast DCT when asked for and supported, else the GEMMs
+inline int fp64_route(int n, int transform) { return transform == 1 && fft_supported(n) ? 2 : 0; }
 void jobset_upload(JobSet& js, hipStream_t s);  // after changing js.host (same grid sizes)
 void jobset_destroy(JobSet& js);
 
@@ -261,7 +269,11 @@ struct DctPlan {
   bool fold = false;
   double* P = nullptr;
   double* Pt = nullptr;
+  double* fft = nullptr;  // fast-DCT tables (fft.hip), when fft_supported(n)
 };
+void fft_plan_create(double*& table, int n);  // the 4n-double table of Job::fft, or nullptr when unsupported
+// d_hat of every job on the fast-DCT route: three passes, n/4 workgroups per job
+void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s);
 void dct_plan_create(DctPlan& p, int n);
 void dct_plan_destroy(DctPlan& p);
 // FP64 Laplacian solve of every job (folded when js.fold, else the unfolded

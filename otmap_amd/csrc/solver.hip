@@ -79,6 +79,7 @@ void allocate(otm_solver* s, int n) {
   J.P = s->plan.P;
   J.Pt = s->plan.Pt;
   J.sq = s->plan.sq;
+  J.fft = s->plan.fft;
   jobset_create(s->js, {J});
   dct_tune(s->js, s->device, s->drv.stream);
   s->n = n;
@@ -135,6 +136,10 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
     set_error("unknown precision mode");
     return OTM_ERR_ARG;
   }
+  if (opt.transform != OTM_TRANSFORM_GEMM && opt.transform != OTM_TRANSFORM_FFT) {
+    set_error("unknown transform");
+    return OTM_ERR_ARG;
+  }
   int route = 0;
   if (opt.precision == OTM_PRECISION_MIXED) {
     if (opt.refine_steps < 0 || opt.refine_steps > 8) {
@@ -144,6 +149,7 @@ int solve_device(otm_solver* s, const double* d_density, const otm_options* opt_
     route = mixed_route(s->n, opt.mixed_min_n);
     if (route) ensure_mixed(s);
   }
+  if (route == 0) route = fp64_route(s->n, opt.transform);
   jobset_route(s->js, {route});
   if (fields_grow_trace(s->f, trace_cap_for(opt.max_iter))) driver_invalidate(s->drv);
   Job& J = s->js.host[0];
@@ -179,6 +185,7 @@ void otm_options_init(otm_options* opt) {
   opt->precision = OTM_PRECISION_FP64;
   opt->refine_steps = 1;
   opt->mixed_min_n = 1024;
+  opt->transform = OTM_TRANSFORM_GEMM;
 }
 
 const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -466,6 +473,31 @@ int otm_laplacian_solve_mixed(otm_solver* s, const double* r, double* d_hat, int
   OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->drv.stream));
   OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
   launch_mixed_solve(s->js, refine_steps, true, s->drv.stream);
+  OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
+  OTM_HIP(hipStreamSynchronize(s->drv.stream));
+  return OTM_OK;
+  OTM_GUARD_END
+}
+
+int otm_laplacian_solve_fft(otm_solver* s, const double* r, double* d_hat) {
+  if (int rc = check_ready(s)) return rc;
+  if (!r || !d_hat) { set_error("null pointer"); return OTM_ERR_ARG; }
+  if (!fft_supported(s->n)) {
+    set_error("the fast-DCT transform needs n = 2^a 3^b with 8 <= n <= 2048 and n % 4 == 0");
+    return OTM_ERR_ARG;
+  }
+  OTM_GUARD_BEGIN
+  OTM_HIP(hipSetDevice(s->device));
+  jobset_route(s->js, {2});
+  const Fields& f = s->f;
+  double sum = 0.;
+  for (long k = 0; k < f.F; ++k) sum += r[k];
+  SolveState st;
+  std::memset(&st, 0, sizeof(st));
+  st.sum_r = sum;
+  OTM_HIP(hipMemcpyAsync(f.state, &st, sizeof(st), hipMemcpyHostToDevice, s->drv.stream));
+  OTM_HIP(hipMemcpyAsync(f.r, r, f.F * sizeof(double), hipMemcpyHostToDevice, s->drv.stream));
+  launch_fft_solve(s->js, true, s->drv.stream);
   OTM_HIP(hipMemcpyAsync(d_hat, f.dhat, f.F * sizeof(double), hipMemcpyDeviceToHost, s->drv.stream));
   OTM_HIP(hipStreamSynchronize(s->drv.stream));
   return OTM_OK;

@@ -1109,18 +1109,21 @@ std::vector<int> job_starts(JobSet& js) {
       const int n = J.f.n;
       const auto& fc = js.jcfg[p];
       const auto& mc = js.jmcfg[p];
-      // an autotuner table covers every job with every transform
+      // an autotuner table covers every job with both GEMM transforms
       const bool r64 = js.tuning || js.jroute[p] == 0, rmx = js.tuning || js.jroute[p] == 1;
+      const bool rfft = !js.tuning && js.jroute[p] == 2;
       int v = 0;
       if (kind == KIND_STENCIL) v = num_tiles(n);
       else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
-      else if (kind == KIND_STENCIL64) v = r64 ? num_tiles(n) : 0;
+      else if (kind == KIND_STENCIL64) v = (r64 || rfft) ? num_tiles(n) : 0;  // the plain update pass
+      else if (kind == KIND_FFT) v = rfft ? n / 4 : 0;
       else if (kind == KIND_FLATMX) v = rmx && mixed_supported(n) ? flat_blocks(J.f.F) : 0;
       else if (kind == KIND_PLAIN64) v = r64 ? ((n + 63) / 64) * ((n + 63) / 64) : 0;
       else if (kind == KIND_MRES) v = rmx && mixed_supported(n) ? mres_tiles(n) : 0;
       else if (kind == KIND_UPAIR) v = rmx && mixed_supported(n) ? upair_tiles(n) : 0;
       else if (kind == KIND_DNORM) {
         if (js.tuning) v = 0;
+        else if (js.jroute[p] == 2) v = n / 4;
         else if (js.jroute[p] == 1) v = mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
         else if (js.fold) v = fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
         else v = ((n + 63) / 64) * ((n + 63) / 64);
@@ -1177,6 +1180,7 @@ void jobset_create(JobSet& js, const std::vector<Job>& jobs, bool tuning) {
   js.jmcfg.assign(js.count, {MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT, MIXED_DEFAULT});
   js.jroute.assign(js.count, 0);
   js.routed_mixed = 0;
+  js.routed_fft = 0;
   report_configs(js);
   js.version = next_version();
   const std::vector<int> st = job_starts(js);
@@ -1202,7 +1206,11 @@ void jobset_route(JobSet& js, const std::vector<int>& route) {
   if (route == js.jroute) return;
   js.jroute = route;
   js.routed_mixed = 0;
-  for (int r : route) js.routed_mixed += r;
+  js.routed_fft = 0;
+  for (int r : route) {
+    js.routed_mixed += r == 1 ? 1 : 0;
+    js.routed_fft += r == 2 ? 1 : 0;
+  }
   jobset_assign(js);
 }
 

@@ -32,6 +32,13 @@ class Precision(enum.IntEnum):
     MIXED = _lib.OTM_PRECISION_MIXED
 
 
+class Transform(enum.IntEnum):
+    """How the FP64 Laplacian solves run: the DCT as dense GEMMs on FP64 MFMA,
+    or the same DCTs by fast transforms (n = 2^a 3^b, 8 <= n <= 2048)."""
+    GEMM = _lib.OTM_TRANSFORM_GEMM
+    FFT = _lib.OTM_TRANSFORM_FFT
+
+
 @dataclass
 class SolverOptions:
     """otlib/otsolver_2dgrid.h:37-43, plus the arithmetic of the solve."""
@@ -44,6 +51,7 @@ class SolverOptions:
     # mixed precision: grids with n >= mixed_min_n (n % 64 == 0) run the BF16x3
     # transform, smaller ones the FP64 transform (0: BF16x3 on every supported size)
     mixed_min_n: int = 1024
+    transform: Transform = Transform.GEMM  # grids on the FP64 transform: GEMMs or fast DCTs
 
     def to_c(self) -> OtmOptions:
         o = OtmOptions()
@@ -55,6 +63,7 @@ class SolverOptions:
         o.precision = int(self.precision)
         o.refine_steps = int(self.refine_steps)
         o.mixed_min_n = int(self.mixed_min_n)
+        o.transform = int(self.transform)
         return o
 
 
@@ -192,6 +201,12 @@ class GridBasedTransportSolver:
         d = np.empty(n * n)
         check(_lib.lib().otm_laplacian_solve_mixed(self._h, ptr(_f64(r, n * n)), ptr(d), int(refine_steps)),
               "laplacian_solve_mixed")
+        return d
+
+    def laplacian_solve_fft(self, r):
+        n = self._n
+        d = np.empty(n * n)
+        check(_lib.lib().otm_laplacian_solve_fft(self._h, ptr(_f64(r, n * n)), ptr(d)), "laplacian_solve_fft")
         return d
 
     def line_search_coefficients(self, g0, direction):
