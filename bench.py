@@ -62,7 +62,7 @@ def parse():
                     help="1024^2 densities of the batched mixed-vs-FP64 workload (0: skip)")
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
-                             "mixed_solve", "copy"],
+                             "mixed_solve", "fft_solve", "copy"],
                     help="only time this kernel (after one solve that fills the buffers); for profilers")
     ap.add_argument("--tile", type=int, default=None,
                     help="with --kernel mixed_gemm / dct_gemm_f64: force this tile config on the timed stage")
@@ -70,7 +70,7 @@ def parse():
 
 
 KERNEL_IDS = {"dct_gemm_f64": 0, "beta": 1, "dir_coef": 2, "update": 3, "dct_solve_f64": 4, "mixed_gemm": 5,
-              "mixed_solve": 6}
+              "mixed_solve": 6, "fft_solve": 7}
 
 
 def kernel_only(args):

@@ -121,7 +121,8 @@ enum {
   OTM_KERNEL_UPDATE = 3,    /* psi / G0 / r update */
   OTM_KERNEL_DCT_SOLVE = 4, /* the whole FP64 Laplacian solve (4 GEMM stages) */
   OTM_KERNEL_MIXED_GEMM = 5, /* one 3-term BF16-split GEMM stage (6 MFMA products) */
-  OTM_KERNEL_MIXED_SOLVE = 6 /* the whole mixed Laplacian solve (split, 8 stages, refinement) */
+  OTM_KERNEL_MIXED_SOLVE = 6, /* the whole mixed Laplacian solve (split, 8 stages, refinement) */
+  OTM_KERNEL_FFT_SOLVE = 7    /* the whole fast-DCT Laplacian solve (3 passes) */
 };
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
 

@@ -64,7 +64,7 @@ namespace {
 
 typedef double cd __attribute__((ext_vector_type(2)));  // complex (re, im)
 
-constexpr int FT = 256;  // threads per workgroup (4 rows of one grid)
+constexpr int FT = 512;  // threads per workgroup (4 rows of one grid)
 
 __device__ inline cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline cd conjc(cd a) { return {a.x, -a.y}; }
@@ -81,10 +81,12 @@ template <int R>
 __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
                                  int L, int sign) {
   const int NR = N / R, tstep = N / (L * R);
+  const bool pow2 = (L & (L - 1)) == 0;
+  const int lshift = __ffs(L) - 1;
   for (int b2 = threadIdx.x; b2 < 2 * NR; b2 += FT) {
     const int q = b2 >= NR ? 1 : 0;
     const int b = b2 - q * NR;
-    const int j = b % L, m = b / L;
+    const int j = pow2 ? (b & (L - 1)) : b % L, m = pow2 ? (b >> lshift) : b / L;
     const cd* s = src + q * N;
     cd a[R];
 #pragma unroll
@@ -151,28 +153,35 @@ __device__ inline cd* fft2(cd* a, cd* b, const cd* tw, int N, int sign) {
 // position of x[c] in the permuted sequence v (v[m] = x[2m], v[N-1-m] = x[2m+1])
 __device__ inline int perm_pos(int c, int N) { return (c & 1) ? N - 1 - (c >> 1) : (c >> 1); }
 
-__device__ inline double dct_scale(int k, int N) { return k == 0 ? sqrt(1.0 / N) : sqrt(2.0 / N); }
+// orthonormal DCT scale s_k and its reciprocal (k == 0 or not)
+struct Scales {
+  double s0, s1, r0, r1;
+  __device__ explicit Scales(int N)
+      : s0(sqrt(1.0 / N)), s1(sqrt(2.0 / N)), r0(sqrt(double(N))), r1(sqrt(0.5 * N)) {}
+  __device__ double s(int k) const { return k == 0 ? s0 : s1; }
+  __device__ double r(int k) const { return k == 0 ? r0 : r1; }
+};
 
-// forward post-processing of sequence q at frequency k: the DCT-II
-// coefficients of its two rows (real part row 2q, imaginary part row 2q+1)
-__device__ inline void dct2_post(const cd* z, const cd* ph, int N, int k, double& xa, double& xb) {
+// forward post-processing of one sequence at frequency k (phase p =
+// e^{-i pi k/2N}): the DCT-II coefficients of its two rows (real part row
+// 2q, imaginary part row 2q+1)
+__device__ inline void dct2_post(const cd* z, cd p, const Scales& sc, int N, int k, double& xa, double& xb) {
   const cd Z = z[k], Zc = conjc(z[k == 0 ? 0 : N - k]);
   const cd va = {0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y)};
   const cd dd = csub(Z, Zc);
   const cd vb = {0.5 * dd.y, -0.5 * dd.x};  // (Z - Zc) / 2i
-  const cd p = ph[k];
-  const double s = dct_scale(k, N);
+  const double s = sc.s(k);
   xa = s * (p.x * va.x - p.y * va.y);
   xb = s * (p.x * vb.x - p.y * vb.y);
 }
 
 // inverse pre-processing at frequency k from the DCT-II coefficients of two
-// rows (a = row 2q, b = row 2q+1): Z[k] = V_a[k] + i V_b[k]
-__device__ inline cd dct3_pre(double xa, double xan, double xb, double xbn, const cd* ph, int N, int k) {
-  const double s = dct_scale(k, N), sn = dct_scale(N - k, N);
-  const double ua = xa / s, ub = xb / s;
-  const double uan = k == 0 ? 0. : xan / sn, ubn = k == 0 ? 0. : xbn / sn;
-  const cd e = conjc(ph[k]);                      // e^{i pi k / 2N}
+// rows (a = row 2q, b = row 2q+1; xan = X_a[N-k]): Z[k] = V_a[k] + i V_b[k]
+__device__ inline cd dct3_pre(double xa, double xan, double xb, double xbn, cd p, const Scales& sc, int k) {
+  const double r = sc.r(k);
+  const double ua = xa * r, ub = xb * r;
+  const double uan = k == 0 ? 0. : xan * sc.r1, ubn = k == 0 ? 0. : xbn * sc.r1;
+  const cd e = conjc(p);  // e^{i pi k / 2N}
   const cd va = cmul(e, cd{ua, -uan}), vb = cmul(e, cd{ub, -ubn});
   return {va.x - vb.y, va.y + vb.x};  // V_a + i V_b
 }
@@ -189,10 +198,51 @@ __device__ inline FftLoc fft_locate(const int* __restrict__ starts, int count) {
 }
 
 // LDS: buffers A and B of 2 sequences x N complex each, then the twiddles
-// (N complex): 80 N bytes.
-__device__ inline void stage_twiddles(cd* twl, const double* table, int N) {
+// (N complex): 80 N bytes.  Thread t handles the frequencies / columns
+// k = t + u FT, u < KPT; every global read of a pass is issued at its start.
+constexpr int KPT = FFT_MAX_N / FT;
+
+struct Prefetch {
+  cd tw[KPT];  // twiddles t + u FT
+  cd ph[KPT];  // DCT phases of k = t + u FT
+};
+__device__ inline void prefetch_tables(Prefetch& P, const double* table, int N) {
   const auto tw = gp(reinterpret_cast<const cd*>(table));
-  for (int t = threadIdx.x; t < N; t += FT) twl[t] = tw[t];
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int k = threadIdx.x + u * FT;
+    P.tw[u] = k < N ? tw[k] : cd{0., 0.};
+    P.ph[u] = k < N ? tw[N + k] : cd{0., 0.};
+  }
+}
+__device__ inline void store_twiddles(const Prefetch& P, cd* twl, int N) {
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int k = threadIdx.x + u * FT;
+    if (k < N) twl[k] = P.tw[u];
+  }
+}
+
+// rows r0 .. r0+3 of `in` (row length N) into registers, column k = t + u FT
+__device__ inline void load_rows(double (&v)[4][KPT], const double* in_, int N, int r0) {
+  const auto in = gp(in_);
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int c = threadIdx.x + u * FT;
+      v[rr][u] = c < N ? in[long(r0 + rr) * N + c] : 0.;
+    }
+}
+// the rows into the two complex sequences of A, permuted (v[m] = x[2m], v[N-1-m] = x[2m+1])
+__device__ inline void store_permuted(const double (&v)[4][KPT], cd* A, int N) {
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int c = threadIdx.x + u * FT;
+      if (c < N) reinterpret_cast<double*>(A + (rr >> 1) * N + perm_pos(c, N))[rr & 1] = v[rr][u];
+    }
 }
 
 // Pass 1: rows of R' (R minus (sum r) at (0,0) when rhs), DCT-II along the
@@ -208,24 +258,24 @@ __global__ __launch_bounds__(FT) void k_fft_rows_fwd(const Job* __restrict__ job
   cd* A = fsm;
   cd* B = fsm + 2 * N;
   cd* twl = fsm + 4 * N;
-  stage_twiddles(twl, J.fft, N);
-  const auto in = gp((const double*)J.f.r);
-  const double S = rhs ? st->sum_r : 0.;
-  for (int e = threadIdx.x; e < 4 * N; e += FT) {
-    const int rr = e / N, c = e - rr * N;
-    double v = in[long(r0 + rr) * N + c];
-    if (rhs && r0 + rr == 0 && c == 0) v -= S;
-    double* slot = reinterpret_cast<double*>(A + (rr >> 1) * N + perm_pos(c, N));
-    slot[rr & 1] = v;
-  }
+  Prefetch P;
+  prefetch_tables(P, J.fft, N);
+  double v[4][KPT];
+  load_rows(v, J.f.r, N, r0);
+  if (rhs && r0 == 0 && threadIdx.x == 0) v[0][0] -= st->sum_r;  // R' = R - (sum r) e0 e0^T
+  store_twiddles(P, twl, N);
+  store_permuted(v, A, N);
   __syncthreads();
   const cd* z = fft2(A, B, twl, N, -1);
-  const cd* ph = reinterpret_cast<const cd*>(J.fft) + N;
+  const Scales sc(N);
   const auto out = gp(J.f.tmp0);
-  for (int k = threadIdx.x; k < N; k += FT) {
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int k = threadIdx.x + u * FT;
+    if (k >= N) break;
     double x0, x1, x2, x3;
-    dct2_post(z, ph, N, k, x0, x1);
-    dct2_post(z + N, ph, N, k, x2, x3);
+    dct2_post(z, P.ph[u], sc, N, k, x0, x1);
+    dct2_post(z + N, P.ph[u], sc, N, k, x2, x3);
     auto o = reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + r0);
     o[0] = cd{x0, x1};
     o[1] = cd{x2, x3};
@@ -246,48 +296,62 @@ __global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, c
   cd* A = fsm;
   cd* B = fsm + 2 * N;
   cd* twl = fsm + 4 * N;
-  stage_twiddles(twl, J.fft, N);
-  const auto in = gp((const double*)J.f.tmp0);
-  for (int e = threadIdx.x; e < 4 * N; e += FT) {
-    const int rr = e / N, c = e - rr * N;
-    double* slot = reinterpret_cast<double*>(A + (rr >> 1) * N + perm_pos(c, N));
-    slot[rr & 1] = in[long(l0 + rr) * N + c];
-  }
+  Prefetch P;
+  prefetch_tables(P, J.fft, N);
+  double v[4][KPT];
+  load_rows(v, J.f.tmp0, N, l0);
+  // eigenvalue halves sin^2(pi |k-l| / 2N) + sin^2(pi (k+l) / 2N) of this thread's (k, l)
+  const auto sq = gp(J.sq);
+  double lam[4][KPT];
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = threadIdx.x + u * FT, l = l0 + rr;
+      const int dm = k > l ? k - l : l - k;
+      lam[rr][u] = k < N ? sq[dm] + sq[k + l] : 1.;
+    }
+  store_twiddles(P, twl, N);
+  store_permuted(v, A, N);
   __syncthreads();
   cd* z = fft2(A, B, twl, N, -1);
   cd* other = z == A ? B : A;
-  const cd* ph = reinterpret_cast<const cd*>(J.fft) + N;
-  const auto sq = gp(J.sq);
+  const Scales sc(N);
   // spectrum / lambda as 4 real rows in the free buffer
   double* X = reinterpret_cast<double*>(other);
-  for (int k = threadIdx.x; k < N; k += FT) {
-    double x[4];
-    dct2_post(z, ph, N, k, x[0], x[1]);
-    dct2_post(z + N, ph, N, k, x[2], x[3]);
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      const int l = l0 + rr;
-      const int dm = k > l ? k - l : l - k;
-      const double lam = 2. * (sq[dm] + sq[k + l]);
-      X[rr * N + k] = (k == 0 && l == 0) ? 0. : x[rr] / lam;
-    }
+  for (int u = 0; u < KPT; ++u) {
+    const int k = threadIdx.x + u * FT;
+    if (k >= N) break;
+    double x[4];
+    dct2_post(z, P.ph[u], sc, N, k, x[0], x[1]);
+    dct2_post(z + N, P.ph[u], sc, N, k, x[2], x[3]);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+      X[rr * N + k] = (k == 0 && l0 + rr == 0) ? 0. : x[rr] / (2. * lam[rr][u]);
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < N; k += FT) {
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int k = threadIdx.x + u * FT;
+    if (k >= N) break;
     const int kn = k == 0 ? 0 : N - k;
-    z[k] = dct3_pre(X[k], X[kn], X[N + k], X[N + kn], ph, N, k);
-    z[N + k] = dct3_pre(X[2 * N + k], X[2 * N + kn], X[3 * N + k], X[3 * N + kn], ph, N, k);
+    z[k] = dct3_pre(X[k], X[kn], X[N + k], X[N + kn], P.ph[u], sc, k);
+    z[N + k] = dct3_pre(X[2 * N + k], X[2 * N + kn], X[3 * N + k], X[3 * N + kn], P.ph[u], sc, k);
   }
   __syncthreads();
   const cd* y = fft2(z, other, twl, N, +1);
   const auto out = gp(J.f.tmp1);
   const double inv = 1.0 / N;
-  for (int i = threadIdx.x; i < N; i += FT) {
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int i = threadIdx.x + u * FT;
+    if (i >= N) break;
     const int m = perm_pos(i, N);
-    const cd u = y[m], v = y[N + m];
+    const cd a = y[m], b = y[N + m];
     auto o = reinterpret_cast<OTM_GLOBAL cd*>(out + long(i) * N + l0);
-    o[0] = cd{u.x * inv, u.y * inv};
-    o[1] = cd{v.x * inv, v.y * inv};
+    o[0] = cd{a.x * inv, a.y * inv};
+    o[1] = cd{b.x * inv, b.y * inv};
   }
 }
 
@@ -304,36 +368,57 @@ __global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ job
   cd* A = fsm;
   cd* B = fsm + 2 * N;
   cd* twl = fsm + 4 * N;
-  stage_twiddles(twl, J.fft, N);
-  const auto in = gp((const double*)J.f.tmp1);
-  const cd* ph = reinterpret_cast<const cd*>(J.fft) + N;
-  for (int k = threadIdx.x; k < N; k += FT) {
-    const int kn = k == 0 ? 0 : N - k;
-    const auto a = in + long(r0) * N;
-    A[k] = dct3_pre(a[k], a[kn], a[N + k], a[N + kn], ph, N, k);
-    A[N + k] = dct3_pre(a[2 * N + k], a[2 * N + kn], a[3 * N + k], a[3 * N + kn], ph, N, k);
+  Prefetch P;
+  prefetch_tables(P, J.fft, N);
+  // X[row][k] and X[row][N-k] of the 4 rows
+  const auto in = gp((const double*)J.f.tmp1) + long(r0) * N;
+  double xk[4][KPT], xn[4][KPT];
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = threadIdx.x + u * FT;
+      const int kn = k == 0 ? 0 : N - k;
+      xk[rr][u] = k < N ? in[rr * N + k] : 0.;
+      xn[rr][u] = k < N ? in[rr * N + kn] : 0.;
+    }
+  store_twiddles(P, twl, N);
+  const Scales sc(N);
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int k = threadIdx.x + u * FT;
+    if (k >= N) break;
+    A[k] = dct3_pre(xk[0][u], xn[0][u], xk[1][u], xn[1][u], P.ph[u], sc, k);
+    A[N + k] = dct3_pre(xk[2][u], xn[2][u], xk[3][u], xn[3][u], P.ph[u], sc, k);
   }
   __syncthreads();
   const cd* y = fft2(A, B, twl, N, +1);
   const auto out = gp(J.f.dhat);
   const double inv = 1.0 / N;
   double nrm[1] = {0.};
-  for (int e = threadIdx.x; e < 4 * N; e += FT) {
-    const int rr = e / N, c = e - rr * N;
-    const cd u = y[(rr >> 1) * N + perm_pos(c, N)];
-    const double v = ((rr & 1) ? u.y : u.x) * inv;
-    out[long(r0 + rr) * N + c] = v;
-    nrm[0] += v * v;
-  }
-  // workgroup partial: DPP wave sums, then the four wave totals in a fixed
-  // order through LDS (buffer A is dead once the result is in y = B or A is
-  // read: the barrier below orders it), so the kernel has no static LDS
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int c = threadIdx.x + u * FT;
+      if (c >= N) break;
+      const cd a = y[(rr >> 1) * N + perm_pos(c, N)];
+      const double x = ((rr & 1) ? a.y : a.x) * inv;
+      out[long(r0 + rr) * N + c] = x;
+      nrm[0] += x * x;
+    }
+  // workgroup partial: DPP wave sums, then the wave totals in a fixed order
+  // through LDS, in the buffer that does not hold y (the last FFT stage read
+  // it before its closing barrier), so the kernel has no static LDS
   wave_sum<1>(nrm);
   double* red = reinterpret_cast<double*>(y == A ? B : A);
   if ((threadIdx.x & 63) == 63) red[threadIdx.x >> 6] = nrm[0];
   __syncthreads();
-  if (threadIdx.x == 0)
-    gp(J.f.site(SITE_DNORM))[L.lt] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) {
+    double x = 0.;
+    for (int w = 0; w < FT / 64; w += 4) x += (red[w] + red[w + 1]) + (red[w + 2] + red[w + 3]);
+    gp(J.f.site(SITE_DNORM))[L.lt] = x;
+  }
 }
 
 }  // namespace
