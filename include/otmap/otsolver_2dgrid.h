@@ -40,7 +40,7 @@ struct SolverOptions {
   Precision precision = Precision::FP64;
   int refine_steps = 1;
   int mixed_min_n = 1024;  // mixed: grids from this size up run the BF16x3 transform
-  Transform transform = Transform::GEMM;
+  Transform transform = Transform::FFT;  // fast DCTs where the size allows, else the GEMMs
 };
 
 // otlib/otsolver_2dgrid.h:45-58

@@ -56,11 +56,11 @@ typedef struct otm_options {
   int mixed_min_n;   /* mixed only: grids with n >= mixed_min_n (and n % 64 == 0) run the BF16x3
                         transform, smaller grids the FP64 transform (default 1024; 0 = BF16x3 for
                         every supported size) */
-  int transform;     /* grids on the FP64 transform: OTM_TRANSFORM_GEMM (default: the DCT as folded
-                        dense GEMMs on FP64 MFMA) or OTM_TRANSFORM_FFT (the same DCTs by fast
-                        transforms, for n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0; other sizes keep
-                        the GEMMs).  No counterpart in the reference, whose solve is a sparse
-                        Cholesky back-substitution. */
+  int transform;     /* grids on the FP64 transform: OTM_TRANSFORM_FFT (default: the DCTs by fast
+                        transforms, for n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0; other sizes use
+                        the GEMMs) or OTM_TRANSFORM_GEMM (the DCT as folded dense GEMMs on FP64
+                        MFMA for every size).  No counterpart in the reference, whose solve is a
+                        sparse Cholesky back-substitution. */
 } otm_options;
 
 typedef struct otm_stats {

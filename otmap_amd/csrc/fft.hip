@@ -75,11 +75,13 @@ __device__ inline cd mul_i(cd a, int sign) { return sign < 0 ? cd{a.y, -a.x} : c
 
 // One radix-R stage of the self-sorting FFT of the two sequences held in src
 // (seq q at src[q N .. q N + N)), written to dst: butterfly b of sequence q,
-// j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{sign 2 pi i
-// / (L R)}), and its DFT_R output q' goes to dst[m L R + j + q' L].
-template <int R>
+// j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{SIGN 2 pi
+// i / (L R)}), and its DFT_R output q' goes to dst[m L R + j + q' L].  Every
+// input is multiplied by its twiddle, also w^0 = (1, 0), which is exact, so
+// the butterfly has no branches and all its LDS reads are issued together.
+template <int R, int SIGN>
 __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
-                                 int L, int sign) {
+                                 int L) {
   const int NR = N / R, tstep = N / (L * R);
   const bool pow2 = (L & (L - 1)) == 0;
   const int lshift = __ffs(L) - 1;
@@ -88,32 +90,29 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
     const int b = b2 - q * NR;
     const int j = pow2 ? (b & (L - 1)) : b % L, m = pow2 ? (b >> lshift) : b / L;
     const cd* s = src + q * N;
-    cd a[R];
+    cd a[R], w[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      cd v = s[b + r * NR];
-      if (r > 0 && j > 0) {
-        const cd w = tw[j * r * tstep];
-        v = cmul(v, sign < 0 ? w : conjc(w));
-      }
-      a[r] = v;
-    }
+    for (int r = 0; r < R; ++r) a[r] = s[b + r * NR];
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[r] = tw[j * r * tstep];
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], SIGN < 0 ? w[r] : conjc(w[r]));
     cd c[R];
     if (R == 2) {
       c[0] = cadd(a[0], a[1]);
       c[1] = csub(a[0], a[1]);
     } else if (R == 4) {
       const cd s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
-      const cd s13 = cadd(a[1], a[3]), d13 = mul_i(csub(a[1], a[3]), sign);
+      const cd s13 = cadd(a[1], a[3]), d13 = mul_i(csub(a[1], a[3]), SIGN);
       c[0] = cadd(s02, s13);
       c[1] = cadd(d02, d13);
       c[2] = csub(s02, s13);
       c[3] = csub(d02, d13);
-    } else {  // R == 3: w3 = e^{sign 2 pi i / 3} = (-1/2, sign sqrt(3)/2)
+    } else {  // R == 3: w3 = e^{SIGN 2 pi i / 3} = (-1/2, SIGN sqrt(3)/2)
       const double h3 = 0.86602540378443864676;  // sqrt(3) / 2
       const cd s12 = cadd(a[1], a[2]), d12 = csub(a[1], a[2]);
       const cd t = {a[0].x - 0.5 * s12.x, a[0].y - 0.5 * s12.y};
-      const cd u = mul_i(cd{h3 * d12.x, h3 * d12.y}, sign);  // sign i sqrt(3)/2 (a1 - a2)
+      const cd u = mul_i(cd{h3 * d12.x, h3 * d12.y}, SIGN);  // SIGN i sqrt(3)/2 (a1 - a2)
       c[0] = cadd(a[0], s12);
       c[1] = cadd(t, u);
       c[2] = csub(t, u);
@@ -126,19 +125,20 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
 
 // FFT of both sequences of `a` (length N each), `b` scratch; returns the
 // buffer that holds the result.  Radices 4 first, then 2, then 3.
-__device__ inline cd* fft2(cd* a, cd* b, const cd* tw, int N, int sign) {
+template <int SIGN>
+__device__ inline cd* fft2(cd* a, cd* b, const cd* tw, int N) {
   int rest = N, L = 1;
   while (rest > 1) {
     if (rest % 4 == 0) {
-      fft_stage<4>(a, b, tw, N, L, sign);
+      fft_stage<4, SIGN>(a, b, tw, N, L);
       L *= 4;
       rest /= 4;
     } else if (rest % 2 == 0) {
-      fft_stage<2>(a, b, tw, N, L, sign);
+      fft_stage<2, SIGN>(a, b, tw, N, L);
       L *= 2;
       rest /= 2;
     } else {
-      fft_stage<3>(a, b, tw, N, L, sign);
+      fft_stage<3, SIGN>(a, b, tw, N, L);
       L *= 3;
       rest /= 3;
     }
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(FT) void k_fft_rows_fwd(const Job* __restrict__ job
   store_twiddles(P, twl, N);
   store_permuted(v, A, N);
   __syncthreads();
-  const cd* z = fft2(A, B, twl, N, -1);
+  const cd* z = fft2<-1>(A, B, twl, N);
   const Scales sc(N);
   const auto out = gp(J.f.tmp0);
 #pragma unroll
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, c
   store_twiddles(P, twl, N);
   store_permuted(v, A, N);
   __syncthreads();
-  cd* z = fft2(A, B, twl, N, -1);
+  cd* z = fft2<-1>(A, B, twl, N);
   cd* other = z == A ? B : A;
   const Scales sc(N);
   // spectrum / lambda as 4 real rows in the free buffer
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, c
     z[N + k] = dct3_pre(X[2 * N + k], X[2 * N + kn], X[3 * N + k], X[3 * N + kn], P.ph[u], sc, k);
   }
   __syncthreads();
-  const cd* y = fft2(z, other, twl, N, +1);
+  const cd* y = fft2<1>(z, other, twl, N);
   const auto out = gp(J.f.tmp1);
   const double inv = 1.0 / N;
 #pragma unroll
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ job
     A[N + k] = dct3_pre(xk[2][u], xn[2][u], xk[3][u], xn[3][u], P.ph[u], sc, k);
   }
   __syncthreads();
-  const cd* y = fft2(A, B, twl, N, +1);
+  const cd* y = fft2<1>(A, B, twl, N);
   const auto out = gp(J.f.dhat);
   const double inv = 1.0 / N;
   double nrm[1] = {0.};

@@ -185,7 +185,7 @@ void otm_options_init(otm_options* opt) {
   opt->precision = OTM_PRECISION_FP64;
   opt->refine_steps = 1;
   opt->mixed_min_n = 1024;
-  opt->transform = OTM_TRANSFORM_GEMM;
+  opt->transform = OTM_TRANSFORM_FFT;
 }
 
 const char* otm_last_error(void) { return g_last_error.c_str(); }

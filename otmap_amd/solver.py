@@ -51,7 +51,7 @@ class SolverOptions:
     # mixed precision: grids with n >= mixed_min_n (n % 64 == 0) run the BF16x3
     # transform, smaller ones the FP64 transform (0: BF16x3 on every supported size)
     mixed_min_n: int = 1024
-    transform: Transform = Transform.GEMM  # grids on the FP64 transform: GEMMs or fast DCTs
+    transform: Transform = Transform.FFT  # grids on the FP64 transform: fast DCTs (where supported) or GEMMs
 
     def to_c(self) -> OtmOptions:
         o = OtmOptions()

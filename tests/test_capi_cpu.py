@@ -34,6 +34,8 @@ def test_options_defaults_match_reference():
     assert o.max_iter == 1000 and o.threshold == 1e-7
     assert o.max_ratio == pytest.approx(1.7976931348623157e308)
     assert o.precision == _lib.OTM_PRECISION_FP64
+    # beyond the reference: BF16x3 options and the FP64 transform (fast DCTs where supported)
+    assert o.refine_steps == 1 and o.mixed_min_n == 1024 and o.transform == _lib.OTM_TRANSFORM_FFT
 
 
 def test_errors_without_device_are_reported():

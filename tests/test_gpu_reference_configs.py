@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-from otmap_amd import Precision, SolverOptions
+from otmap_amd import Precision, SolverOptions, Transform
 from otmap_amd import densities as D
 
 pytestmark = pytest.mark.gpu
@@ -30,12 +30,14 @@ CASES = {
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.MIXED], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("cfg", sorted(CASES))
 def test_reference_workload(cfg, precision, solver_factory):
+    """FP64 here runs the GEMM transform (the fast-DCT transform, the
+    default, has the same test in test_gpu_fft.py)."""
     n, fname, make = CASES[cfg]
     g = np.load(os.path.join(GOLDEN, fname))
     V = (n + 1) ** 2
     print(f"{cfg} n={n} {precision.name}", flush=True)
     s = solver_factory(n)
-    tm = s.solve(make(), SolverOptions(precision=precision))
+    tm = s.solve(make(), SolverOptions(precision=precision, transform=Transform.GEMM))
     st = s.last_stats
     assert abs(st.iterations - int(g["iters"])) <= 1, (st.iterations, int(g["iters"]))
     assert st.final_residual <= 1e-7

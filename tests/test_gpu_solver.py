@@ -90,11 +90,13 @@ def test_line_search_coefficients_bitexact(n, oracle, solver_factory):
     assert np.array_equal(a, ao) and np.array_equal(b, bo)
 
 
+@pytest.mark.parametrize("transform", ["gemm", "fft"])
 @pytest.mark.parametrize("n,seed", [(16, 7), (32, 8), (64, 9), (128, 10), (256, 1000)])
-def test_full_solve_fp64_matches_oracle(n, seed, oracle, solver_factory):
+def test_full_solve_fp64_matches_oracle(n, seed, transform, oracle, solver_factory):
+    from otmap_amd import SolverOptions, Transform
     rho = D.gaussian_mixture(n, seed)
     s = solver_factory(n)
-    tm = s.solve(rho)
+    tm = s.solve(rho, SolverOptions(transform=Transform.GEMM if transform == "gemm" else Transform.FFT))
     o = oracle.solve(n, rho)
     st = s.last_stats
     assert abs(st.iterations - o["iters"]) <= 1, (st.iterations, o["iters"])
