@@ -7,11 +7,13 @@ the seeded smoothed-shapes density (3 discs/rings, Gaussian blur 5 px, floor
 solve() -- density already resident in HBM, forward vertex positions out --
 excluding init(n), which is reported separately, as the reference does.
 
-Prints ONE JSON line: value = ms per solve of the headline precision mode
-(mixed when available, else FP64), plus the FP64 figure, the roofline of the
-dominant kernel (measured live with HIP events on the solver's stream) and
-the CPU baseline (the compiled reference, 1 thread, one solve of the same
-density).
+Prints ONE JSON line: value = ms per solve with the default options (FP64,
+the Laplacian solve by fast DCTs), beside it the same solve with the DCTs as
+folded FP64 GEMMs on MFMA and the mixed form (3-term BF16-split GEMMs, FP64
+refinement), each with its error against the reference's positions; the
+roofline of the dominant kernel (measured live with HIP events on the
+solver's stream) and the CPU baseline (the compiled reference, 1 thread, one
+solve of the same density).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -62,7 +64,7 @@ def parse():
                     help="1024^2 densities of the batched mixed-vs-FP64 workload (0: skip)")
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
-                             "mixed_solve", "fft_solve", "copy"],
+                             "mixed_solve", "fft_solve", "fft_rows_fwd", "fft_cols", "fft_rows_inv", "copy"],
                     help="only time this kernel (after one solve that fills the buffers); for profilers")
     ap.add_argument("--tile", type=int, default=None,
                     help="with --kernel mixed_gemm / dct_gemm_f64: force this tile config on the timed stage")
@@ -70,7 +72,7 @@ def parse():
 
 
 KERNEL_IDS = {"dct_gemm_f64": 0, "beta": 1, "dir_coef": 2, "update": 3, "dct_solve_f64": 4, "mixed_gemm": 5,
-              "mixed_solve": 6, "fft_solve": 7}
+              "mixed_solve": 6, "fft_solve": 7, "fft_rows_fwd": 8, "fft_cols": 9, "fft_rows_inv": 10}
 
 
 def kernel_only(args):
@@ -87,7 +89,7 @@ def kernel_only(args):
         print(json.dumps({"kernel": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
         return
 
-    from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions
+    from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions, Transform
     from otmap_amd import _lib
     from otmap_amd import densities as D
 
@@ -95,8 +97,10 @@ def kernel_only(args):
     s = GridBasedTransportSolver(0)
     s.set_verbose_level(0)
     s.init(n)
-    prec = Precision.MIXED if args.kernel.startswith("mixed") else Precision.FP64
-    s.solve(D.smoothed_shapes(n) if n == 1024 else D.gaussian_mixture(n), SolverOptions(precision=prec))
+    # one solve on the route of the timed kernel fills its intermediate buffers
+    opt = (SolverOptions(precision=Precision.MIXED) if args.kernel.startswith("mixed") else
+           SolverOptions(transform=Transform.FFT if args.kernel.startswith("fft") else Transform.GEMM))
+    s.solve(D.smoothed_shapes(n) if n == 1024 else D.gaussian_mixture(n), opt)
     if args.tile is not None and args.kernel == "mixed_gemm":
         s.set_mixed_config(1, args.tile)
     elif args.tile is not None and args.kernel == "dct_gemm_f64":
@@ -154,11 +158,12 @@ def inverse_throughput(n, d_fwd, d_adj, count, reps):
 def batched_throughput(count, steps, warmup):
     """64 seeded Gaussian-mixture densities with grid sizes drawn from
     {128, 192, 256, 384, 512, 768, 1024}, solved together by one BatchSolver
-    (device-resident inputs) in FP64 and in the mixed form, against the same
+    (device-resident inputs) in FP64 (default options: fast-DCT transform;
+    and with the GEMM transform) and in the mixed form, against the same
     densities solved one after the other by single-grid FP64 solvers."""
     import torch
 
-    from otmap_amd import BatchSolver, GridBasedTransportSolver, Precision, SolverOptions
+    from otmap_amd import BatchSolver, GridBasedTransportSolver, Precision, SolverOptions, Transform
     from otmap_amd import densities as D
 
     specs = D.batch_specs(count)
@@ -205,6 +210,7 @@ def batched_throughput(count, steps, warmup):
                       ("bf16x3_all_sizes", SolverOptions(precision=Precision.MIXED, mixed_min_n=0)),
                       ("bf16x3_all_sizes_refine2", SolverOptions(precision=Precision.MIXED, mixed_min_n=0,
                                                                  refine_steps=2)),
+                      ("fp64_gemm", SolverOptions(transform=Transform.GEMM)),
                       ("fp64", SolverOptions())):
         t, it = timed(opt)
         legs[name] = dict(t=t, iters=it, fwd=torch.cat(d_out), err=ref_err())
@@ -230,7 +236,7 @@ def batched_throughput(count, steps, warmup):
             "sequential_single_solves_per_s": round(count / t_seq, 2),
             "speedup_vs_sequential": round(t_seq / t_batch, 3),
             "iterations": {"min": min(iters), "max": max(iters), "mean": round(float(np.mean(iters)), 1)},
-            "create_s": round(t_create, 3), "precision": "fp64",
+            "create_s": round(t_create, 3), "precision": "fp64", "transform": "fast DCT (FFT)",
             "max_vertex_err_vs_reference": legs["fp64"]["err"],
             "gemm_tiles_by_size": {str(n): b.gemm_config(sizes.index(n))[0] for n in sorted(set(sizes))},
             **{name: {"solves_per_s": round(count / L["t"], 2), "ms_per_batch": round(L["t"] * 1e3, 2),
@@ -245,11 +251,12 @@ def batched_throughput(count, steps, warmup):
 
 def batched_1024(count, steps, warmup, fp64_single_ms):
     """`count` seeded smoothed-shape densities at 1024^2 (seeds 1..count past
-    the single-solve seed) solved together by one BatchSolver, FP64 and mixed,
-    against the single FP64 solve: the per-solve time of each batched form."""
+    the single-solve seed) solved together by one BatchSolver, FP64 (fast-DCT
+    and GEMM transforms) and mixed, against the single FP64 solve: the
+    per-solve time of each batched form."""
     import torch
 
-    from otmap_amd import BatchSolver, Precision, SolverOptions
+    from otmap_amd import BatchSolver, Precision, SolverOptions, Transform
     from otmap_amd import densities as D
 
     n = 1024
@@ -261,8 +268,8 @@ def batched_1024(count, steps, warmup, fp64_single_ms):
     b = BatchSolver([n] * count)
     out = {"densities": count, "n": n}
     fwd = {}
-    for name, prec in (("fp64", Precision.FP64), ("mixed", Precision.MIXED)):
-        opt = SolverOptions(precision=prec)
+    for name, opt in (("fp64", SolverOptions()), ("fp64_gemm", SolverOptions(transform=Transform.GEMM)),
+                      ("mixed", SolverOptions(precision=Precision.MIXED))):
         for _ in range(warmup):
             b.solve_device(ip, op, opt=opt)
         torch.cuda.synchronize()
@@ -276,6 +283,8 @@ def batched_1024(count, steps, warmup, fp64_single_ms):
         out[f"{name}_ms_per_solve"] = round(ms / count, 3)
         out[f"{name}_iterations"] = [x.iterations for x in st]
     out["max_vertex_diff_mixed_vs_fp64"] = float((fwd["mixed"] - fwd["fp64"]).abs().max().item())
+    out["max_vertex_diff_fp64_gemm_vs_fp64"] = float((fwd["fp64_gemm"] - fwd["fp64"]).abs().max().item())
+    out["speedup_fp64_batched_vs_fp64_single"] = round(fp64_single_ms / out["fp64_ms_per_solve"], 3)
     out["speedup_mixed_batched_vs_fp64_single"] = round(fp64_single_ms / out["mixed_ms_per_solve"], 3)
     out["mixed_tiles"] = b.mixed_config()[0]
     return out
@@ -313,25 +322,30 @@ def barycentre_pipeline(n=512, steps=4):
 def stippling_pipeline(n=2048, points=1 << 20):
     """2048^2 log-normal x Gaussian-mixture density (contrast capped at 1e4 by
     max_ratio), mixed-precision solve, then 1M uniform points advected through
-    the inverse map and pruned on zero density; FP64 solve beside it for the
-    map difference.  Each mode's solve is timed after one warm solve."""
+    the inverse map and pruned on zero density; FP64 solves (fast-DCT and GEMM
+    transforms) beside it for the map difference.  Each mode's solve is timed
+    after one warm solve."""
     import torch
 
-    from otmap_amd import GridBasedTransportSolver, GridMesh, Precision, SolverOptions, TransportMap, pipelines
+    from otmap_amd import (GridBasedTransportSolver, GridMesh, Precision, SolverOptions, Transform, TransportMap,
+                           pipelines)
     from otmap_amd import densities as D
 
     rho = D.hdr_lognormal(n)
     dev = torch.device("cuda:0")
     d_rho = torch.from_numpy(rho).to(dev)
     V = (n + 1) ** 2
-    d_fwd = {p: torch.empty(2 * V, dtype=torch.float64, device=dev) for p in ("fp64", "mixed")}
+    d_fwd = {p: torch.empty(2 * V, dtype=torch.float64, device=dev) for p in ("fp64", "fp64_gemm", "mixed")}
     d_adj = torch.empty(n * n, dtype=torch.float64, device=dev)
     s = GridBasedTransportSolver(0)
     s.set_verbose_level(0)
     s.init(n)
     out = {"n": n, "density": "log-normal (sigma 2, corr 16 px) x Gaussian mixture, max_ratio 1e4, floor 1e-4"}
-    for name, prec in (("mixed", Precision.MIXED), ("fp64", Precision.FP64)):
-        opt = SolverOptions(precision=prec, max_ratio=1e4)
+    gpath = os.path.join(ROOT, "tests", "golden", "stipple2048.npz")
+    gold = np.load(gpath) if n == 2048 and os.path.exists(gpath) else None
+    for name, opt in (("fp64_gemm", SolverOptions(transform=Transform.GEMM, max_ratio=1e4)),
+                      ("fp64", SolverOptions(max_ratio=1e4)),
+                      ("mixed", SolverOptions(precision=Precision.MIXED, max_ratio=1e4))):
         # warm solve: the first mixed solve builds the BF16 factor planes and
         # tunes the mixed tiles, the first solve of each mode captures its graph
         s.solve_device(d_rho.data_ptr(), d_fwd[name].data_ptr(), d_adj.data_ptr(), opt)
@@ -341,7 +355,15 @@ def stippling_pipeline(n=2048, points=1 << 20):
         torch.cuda.synchronize()
         out[f"{name}_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
         out[f"{name}_iterations"] = st.iterations
+        if gold is not None:  # largest sampled vertex error against the reference's solve
+            xy = d_fwd[name].cpu().numpy()
+            idx = gold["vidx"]
+            out[f"{name}_max_vertex_err_vs_reference"] = float(max(np.abs(xy[idx] - gold["fx"]).max(),
+                                                                   np.abs(xy[V + idx] - gold["fy"]).max()))
+    if gold is not None:
+        out["reference_iterations"] = int(gold["iters"])
     out["max_vertex_diff_mixed_vs_fp64"] = float((d_fwd["mixed"] - d_fwd["fp64"]).abs().max().item())
+    out["max_vertex_diff_fp64_gemm_vs_fp64"] = float((d_fwd["fp64_gemm"] - d_fwd["fp64"]).abs().max().item())
     tm = TransportMap(GridMesh.regular(n), GridMesh(n, d_fwd["mixed"].cpu().numpy()), d_adj.cpu().numpy())
     pts = D.uniform_points(points, seed=2000 + n)
     t0 = time.perf_counter()
@@ -393,6 +415,17 @@ def cpu_baseline(n, density, reps=3):
                       f"excluded), reference built -O3 -DENABLE_SSE_MODE, SimplicialLDLT (no CHOLMOD), 1 thread"}
 
 
+def fft_supported(n):
+    """Sizes the fast-DCT transform takes: n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0."""
+    if n < 8 or n > 2048 or n % 4:
+        return False
+    while n % 2 == 0:
+        n //= 2
+    while n % 3 == 0:
+        n //= 3
+    return n == 1
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -404,7 +437,7 @@ def main():
         return
     import torch
 
-    from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions
+    from otmap_amd import GridBasedTransportSolver, Precision, SolverOptions, Transform
     from otmap_amd import _lib
     from otmap_amd import densities as D
 
@@ -423,12 +456,15 @@ def main():
     s.init(n)
     t_init = time.perf_counter() - t0
 
+    # fp64: the default options (FP64, fast-DCT transform where n allows it);
+    # fp64_gemm: the same solve with the DCTs as folded FP64 GEMMs on MFMA;
+    # mixed: 3-term BF16-split GEMMs + one FP64 refinement
+    fft = fft_supported(n)
     results = {}
-    modes = [("fp64", Precision.FP64, 1)]
+    modes = [("fp64", SolverOptions()), ("fp64_gemm", SolverOptions(transform=Transform.GEMM))]
     if n % 64 == 0:
-        modes.append(("mixed", Precision.MIXED, 1))
-    for name, prec, refine in modes:
-        opt = SolverOptions(precision=prec, refine_steps=refine)
+        modes.append(("mixed", SolverOptions(precision=Precision.MIXED)))
+    for name, opt in modes:
         for _ in range(args.warmup):
             s.solve_device(d_rho.data_ptr(), d_fwd.data_ptr(), d_adj.data_ptr(), opt)
         torch.cuda.synchronize()
@@ -441,9 +477,11 @@ def main():
         fwd = d_fwd.cpu().numpy()
         results[name] = dict(ms=wall, iters=stats[-1].iterations, res=stats[-1].final_residual, fwd=fwd,
                              dev_ms=float(np.mean([st.ms_solve for st in stats])))
+    # the inverse map below runs on the default mode's forward map
+    s.solve_device(d_rho.data_ptr(), d_fwd.data_ptr(), d_adj.data_ptr(), SolverOptions())
 
     # inverse map at 1M uniform query points (device buffers): BVH build on the
-    # host (once per map, as init_inverse), then one query per lane
+    # device (once per map, as init_inverse), then one query per lane
     inv_stats = inverse_throughput(n, d_fwd, d_adj, args.inverse_points, args.steps)
     batch_stats = batched_throughput(args.batch, args.steps, args.warmup) if args.batch > 0 else None
     b1024 = (batched_1024(args.batch_1024, args.steps, args.warmup, results["fp64"]["ms"])
@@ -467,28 +505,48 @@ def main():
     # on the solver's own stream (the stream every kernel of a solve runs on)
     ms = _lib.C.c_double()
     kernel_ms = {}
-    kernels = [("dct_gemm_f64", _lib.OTM_KERNEL_DCT_GEMM), ("beta", _lib.OTM_KERNEL_BETA),
-               ("dir_coef", _lib.OTM_KERNEL_DIR_COEF), ("update", _lib.OTM_KERNEL_UPDATE),
+    kernels = [("beta", _lib.OTM_KERNEL_BETA), ("dir_coef", _lib.OTM_KERNEL_DIR_COEF),
+               ("update", _lib.OTM_KERNEL_UPDATE), ("dct_gemm_f64", _lib.OTM_KERNEL_DCT_GEMM),
                ("dct_solve_f64", _lib.OTM_KERNEL_DCT_SOLVE)]
+    if fft:
+        kernels += [("fft_solve", _lib.OTM_KERNEL_FFT_SOLVE), ("fft_rows_fwd", _lib.OTM_KERNEL_FFT_ROWS_FWD),
+                    ("fft_cols", _lib.OTM_KERNEL_FFT_COLS), ("fft_rows_inv", _lib.OTM_KERNEL_FFT_ROWS_INV)]
     if "mixed" in results:
         kernels += [("mixed_gemm_bf16x3", _lib.OTM_KERNEL_MIXED_GEMM), ("mixed_solve", _lib.OTM_KERNEL_MIXED_SOLVE)]
     for kname, kid in kernels:
         _lib.check(_lib.lib().otm_solver_profile_kernel(s._h, kid, args.profile_reps, _lib.C.byref(ms)), "profile")
         kernel_ms[kname] = ms.value
 
-    head = "mixed" if "mixed" in results else "fp64"
+    head = "fp64"
 
-    # Roofline of the dominant kernel of the headline mode: one stage of the
-    # even/odd-folded DCT (k_mgemm stage M2 for mixed, k_gemm_fold stage F1 for
-    # FP64).  Algorithmic work per launch is the folded product, n x n outputs
-    # over n/2 terms = n^3 flops; the BF16 split multiplicity (6 products) is
+    def traffic_of(kname):
+        t = TRAFFIC_BYTES_1024.get(kname) if n == 1024 else None
+        return {"traffic": round(t) if t else None, "traffic_source": TRAFFIC_SOURCE if t else None}
+
+    # Roofline of the dominant kernel of the default path: the column pass of
+    # the fast-DCT solve (the longest kernel of an iteration), HBM-bound.  Its
+    # algorithmic bytes per launch are one read and one write of the n x n
+    # FP64 field (16 n^2 bytes; the twiddle, phase and eigenvalue tables add
+    # 64 n bytes and are not counted).
+    roof_fft = None
+    if fft:
+        t_ms = kernel_ms["fft_cols"]
+        achieved = 16.0 * n * n / (t_ms * 1e-3) / 1e9
+        roof_fft = {"kernel": "k_fft_cols: fast-DCT column pass (DCT-II, / lambda, DCT-III of n columns), "
+                              "one read + one write of the n x n FP64 field",
+                    "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(achieved / PEAK_HBM_GBS, 4), **traffic_of("k_fft_cols"),
+                    "ms_per_launch": round(t_ms, 5)}
+
+    # Rooflines of the GEMM routes: one stage of the even/odd-folded DCT
+    # (k_mgemm stage M2 for mixed, k_gemm_fold stage F1 for FP64).
+    # Algorithmic work per launch is the folded product, n x n outputs over
+    # n/2 terms = n^3 flops; the BF16 split multiplicity (6 products) is
     # implementation overhead and is not counted.
     def gemm_roofline(kname, t_ms, peak, label, flops):
         achieved = flops / (t_ms * 1e-3) / 1e12
-        traffic = TRAFFIC_BYTES_1024.get(kname) if n == 1024 else None
         return {"kernel": label, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": round(traffic) if traffic else None,
-                "traffic_source": TRAFFIC_SOURCE if traffic else None, "ms_per_launch": round(t_ms, 5)}
+                "frac": round(achieved / peak, 4), **traffic_of(kname), "ms_per_launch": round(t_ms, 5)}
 
     tiles, n_tiles = s.gemm_config()
     folded = n_tiles > 0
@@ -507,21 +565,24 @@ def main():
                                 "split on MFMA, FP32 accumulate", float(n) ** 3)
 
     # Whole-iteration roofline: the reference's 26 FP64 field touches per face
-    # (218 MB at 1024^2; this build moves 22: the quartic coefficients are
-    # recomputed instead of stored) at HBM peak, plus the algorithmic flops of
-    # one Laplacian solve (four transform stages: 4 n^3 folded, 8 n^3
-    # unfolded; the BF16 split multiplicity and the refinement transform are
-    # not counted) at the MFMA peak of the mode, over the measured device time
-    # per iteration.
-    def iteration_roofline(name, peak_tflops, gemm_flops):
-        t_roof = 26 * 8 * n * n / (PEAK_HBM_GBS * 1e9) + gemm_flops / (peak_tflops * 1e12)
+    # in the stencil passes (218 MB at 1024^2; this build moves 22: the quartic
+    # coefficients are recomputed instead of stored) at HBM peak, plus the
+    # Laplacian solve: for the fast-DCT route its three passes' 6 field
+    # touches at HBM peak, for the GEMM routes the algorithmic flops of the
+    # four transform stages (4 n^3 folded, 8 n^3 unfolded; the BF16 split
+    # multiplicity and the refinement transform are not counted) at the MFMA
+    # peak of the mode; over the measured device time per iteration.
+    def iteration_roofline(name, t_solve_roof):
+        t_roof = 26 * 8 * n * n / (PEAK_HBM_GBS * 1e9) + t_solve_roof
         t_iter = results[name]["dev_ms"] * 1e-3 / max(results[name]["iters"], 1)
         return {"t_roof_us": round(t_roof * 1e6, 2), "t_iter_us": round(t_iter * 1e6, 2),
                 "frac": round(t_roof / t_iter, 4)}
 
-    iter_roof = {"fp64": iteration_roofline("fp64", PEAK_FP64_MFMA_TFLOPS, 4 * stage_flops_f64)}
+    t_gemm64 = 4 * stage_flops_f64 / (PEAK_FP64_MFMA_TFLOPS * 1e12)
+    iter_roof = {"fp64": iteration_roofline("fp64", 6 * 8 * n * n / (PEAK_HBM_GBS * 1e9) if fft else t_gemm64),
+                 "fp64_gemm": iteration_roofline("fp64_gemm", t_gemm64)}
     if "mixed" in results:
-        iter_roof["mixed"] = iteration_roofline("mixed", PEAK_BF16_TFLOPS, 4.0 * float(n) ** 3)
+        iter_roof["mixed"] = iteration_roofline("mixed", 4.0 * float(n) ** 3 / (PEAK_BF16_TFLOPS * 1e12))
     line = {
         "metric": METRIC,
         "value": round(results[head]["ms"], 3),
@@ -533,15 +594,18 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3-split GEMM (f32 accumulate) + f64 refinement/stencils" if head == "mixed" else "f64",
+        "dtype": "f64",
         "data": f"synthetic: seeded smoothed shapes {n}x{n} (3 discs/rings, blur 5 px, floor 1e-3, mean 1)",
         "config": {"workload": f"single {n}x{n} grid transport solve to |r|^2/A <= 1e-7 (conjugate-Jacobian)",
-                   "n": n, "threshold": 1e-7, "precision": head},
+                   "n": n, "threshold": 1e-7, "precision": "fp64",
+                   "transform": "fast DCT (FFT)" if fft else "folded GEMMs"},
         "fp64_ms_per_solve": round(results["fp64"]["ms"], 3),
         "fp64_device_ms_per_solve": round(results["fp64"]["dev_ms"], 3),
+        "fp64_gemm_ms_per_solve": round(results["fp64_gemm"]["ms"], 3),
         "mixed_ms_per_solve": round(results["mixed"]["ms"], 3) if "mixed" in results else None,
-        "speedup_mixed_vs_fp64": (round(results["fp64"]["ms"] / results["mixed"]["ms"], 3)
-                                  if "mixed" in results else None),
+        "speedup_fp64_vs_fp64_gemm": round(results["fp64_gemm"]["ms"] / results["fp64"]["ms"], 3),
+        "speedup_mixed_vs_fp64_gemm": (round(results["fp64_gemm"]["ms"] / results["mixed"]["ms"], 3)
+                                       if "mixed" in results else None),
         "iterations": {k: v["iters"] for k, v in results.items()},
         "final_residual": {k: v["res"] for k, v in results.items()},
         "init_s": round(t_init, 4),
@@ -552,8 +616,9 @@ def main():
         "barycentre_4x512": bary,
         "stippling_2048": stipple,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
-        "roofline": roof_x3 if head == "mixed" else roof_f64,
+        "roofline": roof_fft if roof_fft else roof_f64,
         "roofline_fp64_gemm": roof_f64,
+        "roofline_mixed_gemm": roof_x3,
         "iteration_roofline": iter_roof,
         "dct_tiles": {"folded": folded, "stage_config": tiles, "candidates": n_tiles},
     }

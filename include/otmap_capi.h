@@ -122,7 +122,10 @@ enum {
   OTM_KERNEL_DCT_SOLVE = 4, /* the whole FP64 Laplacian solve (4 GEMM stages) */
   OTM_KERNEL_MIXED_GEMM = 5, /* one 3-term BF16-split GEMM stage (6 MFMA products) */
   OTM_KERNEL_MIXED_SOLVE = 6, /* the whole mixed Laplacian solve (split, 8 stages, refinement) */
-  OTM_KERNEL_FFT_SOLVE = 7    /* the whole fast-DCT Laplacian solve (3 passes) */
+  OTM_KERNEL_FFT_SOLVE = 7,   /* the whole fast-DCT Laplacian solve (3 passes) */
+  OTM_KERNEL_FFT_ROWS_FWD = 8, /* fast-DCT pass 1: DCT-II of the rows, transposed write */
+  OTM_KERNEL_FFT_COLS = 9,     /* fast-DCT pass 2: DCT-II, / lambda, DCT-III of the columns */
+  OTM_KERNEL_FFT_ROWS_INV = 10 /* fast-DCT pass 3: DCT-III of the rows + |d_hat|^2 partials */
 };
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
 

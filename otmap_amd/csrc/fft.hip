@@ -26,6 +26,7 @@
 // two LDS buffers; the twiddle table exp(-2 pi i t/n) is staged in LDS.
 // Supported sizes: n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0.
 
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -189,11 +190,19 @@ __device__ inline cd dct3_pre(double xa, double xan, double xb, double xbn, cd p
 struct FftLoc {
   int p, lt, nb;
 };
+// The dispatcher deals consecutive workgroups to the 8 XCDs in turn.  The
+// first 8 floor(G/8) workgroups are renumbered so that XCD x takes the
+// consecutive row groups x G/8 .. (x+1) G/8 - 1: the 32-byte column segments
+// of the transposed writes of neighbouring row groups then meet in the same
+// L2 and leave it as whole lines.
 __device__ inline FftLoc fft_locate(const int* __restrict__ starts, int count) {
+  const int g = int(gridDim.x), full = g & ~7;
+  int b = int(blockIdx.x);
+  if (b < full) b = (b & 7) * (full >> 3) + (b >> 3);
   FftLoc L;
-  L.p = count == 1 ? 0 : job_of(starts, count, blockIdx.x);
-  L.lt = count == 1 ? int(blockIdx.x) : int(blockIdx.x) - starts[L.p];
-  L.nb = count == 1 ? int(gridDim.x) : starts[L.p + 1] - starts[L.p];
+  L.p = count == 1 ? 0 : job_of(starts, count, b);
+  L.lt = count == 1 ? b : b - starts[L.p];
+  L.nb = count == 1 ? g : starts[L.p + 1] - starts[L.p];
   return L;
 }
 
@@ -425,26 +434,37 @@ __global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ job
 
 // d_hat of every job on the FFT route (launch kind KIND_FFT, n/4 workgroups
 // per job); dynamic LDS sized for the largest such grid.
-void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s) {
+void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
   if (js.total[KIND_FFT] == 0) return;
   int nmax = 0;
   for (int p = 0; p < js.count; ++p)
     if (js.jroute[p] == 2) nmax = std::max(nmax, js.host[p].f.n);
   const size_t lds = size_t(80) * nmax;
-  static bool attr = false;  // the kernels use no static LDS: up to 80 FFT_MAX_N bytes dynamic
-  if (!attr) {
+  // the kernels use no static LDS: up to 80 FFT_MAX_N bytes dynamic, allowed
+  // once per device (bit d of the mask)
+  static std::atomic<unsigned long long> attr{0};
+  int dev = 0;
+  OTM_HIP(hipGetDevice(&dev));
+  const unsigned long long bit = 1ull << (dev & 63);
+  if (!(attr.load() & bit)) {
     const int cap = 80 * FFT_MAX_N;
     for (const void* k : {reinterpret_cast<const void*>(&k_fft_rows_fwd), reinterpret_cast<const void*>(&k_fft_cols),
                           reinterpret_cast<const void*>(&k_fft_rows_inv)})
       OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, cap));
-    attr = true;
+    attr.fetch_or(bit);
   }
   const dim3 grid(unsigned(js.total[KIND_FFT]));
   const int* st = js.starts_of(KIND_FFT);
-  hipLaunchKernelGGL(k_fft_rows_fwd, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate, true);
-  hipLaunchKernelGGL(k_fft_cols, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate);
-  hipLaunchKernelGGL(k_fft_rows_inv, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate);
+  switch (pass) {
+    case 0: hipLaunchKernelGGL(k_fft_rows_fwd, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL(k_fft_cols, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL(k_fft_rows_inv, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate);
+  }
   OTM_HIP(hipGetLastError());
+}
+
+void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s) {
+  for (int pass = 0; pass < 3; ++pass) launch_fft_pass(js, pass, gate, s);
 }
 
 }  // namespace otm

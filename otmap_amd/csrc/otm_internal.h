@@ -274,6 +274,8 @@ struct DctPlan {
 void fft_plan_create(double*& table, int n);  // the 4n-double table of Job::fft, or nullptr when unsupported
 // d_hat of every job on the fast-DCT route: three passes, n/4 workgroups per job
 void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s);
+// one of the three passes (0 rows forward, 1 columns, 2 rows inverse)
+void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s);
 void dct_plan_create(DctPlan& p, int n);
 void dct_plan_destroy(DctPlan& p);
 // FP64 Laplacian solve of every job (folded when js.fold, else the unfolded

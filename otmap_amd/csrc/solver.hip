@@ -331,7 +331,7 @@ int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, doubl
 
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch) {
   if (int rc = check_ready(s)) return rc;
-  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_FFT_SOLVE) {
+  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_FFT_ROWS_INV) {
     set_error("invalid profile arguments");
     return OTM_ERR_ARG;
   }
@@ -340,7 +340,8 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
     set_error("mixed precision needs a grid size that is a multiple of 64");
     return OTM_ERR_ARG;
   }
-  if (kernel == OTM_KERNEL_FFT_SOLVE && !fft_supported(s->n)) {
+  const bool fft = kernel >= OTM_KERNEL_FFT_SOLVE;
+  if (fft && !fft_supported(s->n)) {
     set_error("the fast-DCT transform does not support this grid size");
     return OTM_ERR_ARG;
   }
@@ -359,9 +360,9 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   jobset_create(pj, {J});
   pj.jcfg = s->js.jcfg;
   pj.jmcfg = s->js.jmcfg;
-  pj.jroute = {mixed ? 1 : kernel == OTM_KERNEL_FFT_SOLVE ? 2 : 0};
+  pj.jroute = {mixed ? 1 : fft ? 2 : 0};
   pj.routed_mixed = mixed ? 1 : 0;
-  pj.routed_fft = kernel == OTM_KERNEL_FFT_SOLVE ? 1 : 0;
+  pj.routed_fft = fft ? 1 : 0;
   jobset_assign(pj);
   SolveState tpl;
   std::memset(&tpl, 0, sizeof(tpl));
@@ -375,7 +376,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   // one event pair: the figure is the kernel duration with launch latency
   // overlapped, as rocprof reports it.  BETA and DIR_COEF include their
   // single-workgroup scalar kernel.
-  if (kernel == OTM_KERNEL_DCT_SOLVE || kernel == OTM_KERNEL_MIXED_SOLVE || kernel == OTM_KERNEL_FFT_SOLVE)
+  if (kernel == OTM_KERNEL_DCT_SOLVE || kernel == OTM_KERNEL_MIXED_SOLVE || fft)
     tpl.update = 0;
   OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->drv.stream));
   auto launch = [&]() {
@@ -387,6 +388,9 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
       case OTM_KERNEL_DCT_SOLVE: launch_dct_solve(pj, &s->plan, s->drv.stream); break;
       case OTM_KERNEL_MIXED_GEMM: launch_mixed_gemm_stage(pj, s->drv.stream); break;
       case OTM_KERNEL_FFT_SOLVE: launch_fft_solve(pj, true, s->drv.stream); break;
+      case OTM_KERNEL_FFT_ROWS_FWD: launch_fft_pass(pj, 0, true, s->drv.stream); break;
+      case OTM_KERNEL_FFT_COLS: launch_fft_pass(pj, 1, true, s->drv.stream); break;
+      case OTM_KERNEL_FFT_ROWS_INV: launch_fft_pass(pj, 2, true, s->drv.stream); break;
       default: launch_mixed_solve(pj, 1, true, s->drv.stream); break;
     }
   };

@@ -163,3 +163,22 @@ def test_fft_batch_bit_identical_to_single_solves():
         tm = s.solve(rho, FFT)
         assert b.last_stats[k].iterations == s.last_stats.iterations, (k, n)
         assert np.array_equal(maps[k].fwd_mesh().xy, tm.fwd_mesh().xy), (k, n)
+
+
+def test_fft_pass_timing_entry_points(solver_factory):
+    """otm_solver_profile_kernel times the whole fast-DCT solve and each of its
+    three passes; sizes without the fast transform and unknown ids are
+    rejected."""
+    from otmap_amd import _lib
+
+    ms = _lib.C.c_double()
+    s = solver_factory(256)
+    s.laplacian_solve_fft(_rng(256).standard_normal(256 * 256))
+    for k in (_lib.OTM_KERNEL_FFT_SOLVE, _lib.OTM_KERNEL_FFT_ROWS_FWD, _lib.OTM_KERNEL_FFT_COLS,
+              _lib.OTM_KERNEL_FFT_ROWS_INV):
+        assert _lib.lib().otm_solver_profile_kernel(s._h, k, 3, _lib.C.byref(ms)) == 0
+        assert ms.value > 0
+    assert _lib.lib().otm_solver_profile_kernel(s._h, _lib.OTM_KERNEL_FFT_ROWS_INV + 1, 3, _lib.C.byref(ms)) != 0
+    u = solver_factory(100)
+    for k in (_lib.OTM_KERNEL_FFT_SOLVE, _lib.OTM_KERNEL_FFT_COLS):
+        assert _lib.lib().otm_solver_profile_kernel(u._h, k, 3, _lib.C.byref(ms)) != 0
