@@ -35,16 +35,19 @@ PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of the timed DCT stage kernels at n = 1024 (mixed M2
-# with 32- or 64-column tiles, FP64 F1 with tile shape c), from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
-# counter reads for a 256 MiB device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
-# the runs and their commands are in profiles/gemm_1024_hbm_traffic.md.
-TRAFFIC_SOURCE = "profiles/gemm_1024_hbm_traffic.md"
+# HBM bytes per launch of the timed transform kernels at n = 1024 (the
+# fast-DCT column pass; mixed M2 with 32- or 64-column tiles, FP64 F1 with
+# tile shape c), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
+# dispatch), each scaled by what the counter reads for a 256 MiB device copy
+# (FETCH_SIZE x2.0, WRITE_SIZE x1.0); the runs and their commands are in the
+# profile named beside each figure.
+GEMM_TRAFFIC = "profiles/gemm_1024_hbm_traffic.md"
+FFT_TRAFFIC = "profiles/fft_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
-    "k_mgemm<M2,32>": (9288.0 * 2.0 + 6144.0) * 1024,
-    "k_mgemm<M2,64>": (9294.0 * 2.0 + 6144.0) * 1024,
-    **{f"k_gemm_fold<F1,cfg{c}>": (f * 2.0 + 8192.0) * 1024
+    "k_fft_cols": ((4400.5 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
+    "k_mgemm<M2,32>": ((9288.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
+    "k_mgemm<M2,64>": ((9294.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
+    **{f"k_gemm_fold<F1,cfg{c}>": ((f * 2.0 + 8192.0) * 1024, GEMM_TRAFFIC)
        for c, f in enumerate([20521.6, 20517.0, 20517.0, 20513.0, 20537.5, 20522.0, 20526.5, 20564.3, 20520.0])},
 }
 
@@ -520,8 +523,8 @@ def main():
     head = "fp64"
 
     def traffic_of(kname):
-        t = TRAFFIC_BYTES_1024.get(kname) if n == 1024 else None
-        return {"traffic": round(t) if t else None, "traffic_source": TRAFFIC_SOURCE if t else None}
+        t, src = TRAFFIC_BYTES_1024.get(kname, (None, None)) if n == 1024 else (None, None)
+        return {"traffic": round(t) if t else None, "traffic_source": src}
 
     # Roofline of the dominant kernel of the default path: the column pass of
     # the fast-DCT solve (the longest kernel of an iteration), HBM-bound.  Its
