@@ -17,8 +17,9 @@
 // V[k] = e^{i pi k/2n} (U[k] - i U[n-k]); v = IFFT(V); x[2m] = v[m],
 // x[2m+1] = v[n-1-m]; again two rows per complex sequence.
 //
-// Passes (one workgroup per 4 rows, i.e. two complex sequences):
-//   1  rows of R' -> DCT-II along j, written transposed (B^T, 32-byte runs)
+// Passes (one workgroup per 2 rows, one complex sequence, for n <= 1024;
+// per 4 rows, two sequences, above):
+//   1  rows of R' -> DCT-II along j, written transposed (B^T, 16- or 32-byte runs)
 //   2  rows of B^T (columns of B) -> DCT-II along i, / lambda, DCT-III along
 //      i, written transposed back (spatial i, frequency l along j)
 //   3  rows -> DCT-III along j = d_hat, plus the |d_hat|^2 partials
@@ -65,7 +66,12 @@ namespace {
 
 typedef double cd __attribute__((ext_vector_type(2)));  // complex (re, im)
 
-constexpr int FT = 512;  // threads per workgroup (4 rows of one grid)
+// A workgroup of a pass holds FS complex sequences (2 FS grid rows) and has
+// 256 FS threads; thread t owns the frequencies / columns k = t + u 256 FS,
+// u < KPT, of every sequence: FS = 1 serves n <= 1024, FS = 2 n <= 2048.
+constexpr int KPT = 4;
+static_assert(FFT_SPLIT_N == 256 * KPT && FFT_MAX_N == 512 * KPT, "four columns per thread");
+constexpr int fft_lds(int fs) { return 16 * (2 * fs + 1); }  // LDS bytes per unit of n
 
 __device__ inline cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline cd conjc(cd a) { return {a.x, -a.y}; }
@@ -74,20 +80,21 @@ __device__ inline cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
 // multiply by -i (sign < 0) or +i (sign > 0)
 __device__ inline cd mul_i(cd a, int sign) { return sign < 0 ? cd{a.y, -a.x} : cd{-a.y, a.x}; }
 
-// One radix-R stage of the self-sorting FFT of the two sequences held in src
+// One radix-R stage of the self-sorting FFT of the FS sequences held in src
 // (seq q at src[q N .. q N + N)), written to dst: butterfly b of sequence q,
 // j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{SIGN 2 pi
 // i / (L R)}), and its DFT_R output q' goes to dst[m L R + j + q' L].  Every
 // input is multiplied by its twiddle, also w^0 = (1, 0), which is exact, so
 // the butterfly has no branches and all its LDS reads are issued together.
-template <int R, int SIGN>
+template <int FS, int R, int SIGN>
 __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
                                  int L) {
+  constexpr int FT = 256 * FS;
   const int NR = N / R, tstep = N / (L * R);
   const bool pow2 = (L & (L - 1)) == 0;
   const int lshift = __ffs(L) - 1;
-  for (int b2 = threadIdx.x; b2 < 2 * NR; b2 += FT) {
-    const int q = b2 >= NR ? 1 : 0;
+  for (int b2 = threadIdx.x; b2 < FS * NR; b2 += FT) {
+    const int q = FS == 1 ? 0 : b2 >= NR ? 1 : 0;
     const int b = b2 - q * NR;
     const int j = pow2 ? (b & (L - 1)) : b % L, m = pow2 ? (b >> lshift) : b / L;
     const cd* s = src + q * N;
@@ -124,22 +131,22 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
   }
 }
 
-// FFT of both sequences of `a` (length N each), `b` scratch; returns the
+// FFT of the FS sequences of `a` (length N each), `b` scratch; returns the
 // buffer that holds the result.  Radices 4 first, then 2, then 3.
-template <int SIGN>
-__device__ inline cd* fft2(cd* a, cd* b, const cd* tw, int N) {
+template <int FS, int SIGN>
+__device__ inline cd* fft_seq(cd* a, cd* b, const cd* tw, int N) {
   int rest = N, L = 1;
   while (rest > 1) {
     if (rest % 4 == 0) {
-      fft_stage<4, SIGN>(a, b, tw, N, L);
+      fft_stage<FS, 4, SIGN>(a, b, tw, N, L);
       L *= 4;
       rest /= 4;
     } else if (rest % 2 == 0) {
-      fft_stage<2, SIGN>(a, b, tw, N, L);
+      fft_stage<FS, 2, SIGN>(a, b, tw, N, L);
       L *= 2;
       rest /= 2;
     } else {
-      fft_stage<3, SIGN>(a, b, tw, N, L);
+      fft_stage<FS, 3, SIGN>(a, b, tw, N, L);
       L *= 3;
       rest /= 3;
     }
@@ -192,9 +199,9 @@ struct FftLoc {
 };
 // The dispatcher deals consecutive workgroups to the 8 XCDs in turn.  The
 // first 8 floor(G/8) workgroups are renumbered so that XCD x takes the
-// consecutive row groups x G/8 .. (x+1) G/8 - 1: the 32-byte column segments
-// of the transposed writes of neighbouring row groups then meet in the same
-// L2 and leave it as whole lines.
+// consecutive row groups x G/8 .. (x+1) G/8 - 1: the 16- or 32-byte column
+// segments of the transposed writes of neighbouring row groups then meet in
+// the same L2 and leave it as whole lines.
 __device__ inline FftLoc fft_locate(const int* __restrict__ starts, int count) {
   const int g = int(gridDim.x), full = g & ~7;
   int b = int(blockIdx.x);
@@ -206,16 +213,17 @@ __device__ inline FftLoc fft_locate(const int* __restrict__ starts, int count) {
   return L;
 }
 
-// LDS: buffers A and B of 2 sequences x N complex each, then the twiddles
-// (N complex): 80 N bytes.  Thread t handles the frequencies / columns
-// k = t + u FT, u < KPT; every global read of a pass is issued at its start.
-constexpr int KPT = FFT_MAX_N / FT;
+// LDS: buffers A and B of FS sequences x N complex each, then the twiddles
+// (N complex): fft_lds(FS) N bytes.  Every global read of a pass is issued
+// at its start.
 
 struct Prefetch {
   cd tw[KPT];  // twiddles t + u FT
   cd ph[KPT];  // DCT phases of k = t + u FT
 };
+template <int FS>
 __device__ inline void prefetch_tables(Prefetch& P, const double* table, int N) {
+  constexpr int FT = 256 * FS;
   const auto tw = gp(reinterpret_cast<const cd*>(table));
 #pragma unroll
   for (int u = 0; u < KPT; ++u) {
@@ -224,7 +232,9 @@ __device__ inline void prefetch_tables(Prefetch& P, const double* table, int N) 
     P.ph[u] = k < N ? tw[N + k] : cd{0., 0.};
   }
 }
+template <int FS>
 __device__ inline void store_twiddles(const Prefetch& P, cd* twl, int N) {
+  constexpr int FT = 256 * FS;
 #pragma unroll
   for (int u = 0; u < KPT; ++u) {
     const int k = threadIdx.x + u * FT;
@@ -232,21 +242,25 @@ __device__ inline void store_twiddles(const Prefetch& P, cd* twl, int N) {
   }
 }
 
-// rows r0 .. r0+3 of `in` (row length N) into registers, column k = t + u FT
-__device__ inline void load_rows(double (&v)[4][KPT], const double* in_, int N, int r0) {
+// rows r0 .. r0 + 2 FS - 1 of `in` (row length N) into registers, column k = t + u 256 FS
+template <int FS>
+__device__ inline void load_rows(double (&v)[2 * FS][KPT], const double* in_, int N, int r0) {
+  constexpr int FT = 256 * FS;
   const auto in = gp(in_);
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr)
+  for (int rr = 0; rr < 2 * FS; ++rr)
 #pragma unroll
     for (int u = 0; u < KPT; ++u) {
       const int c = threadIdx.x + u * FT;
       v[rr][u] = c < N ? in[long(r0 + rr) * N + c] : 0.;
     }
 }
-// the rows into the two complex sequences of A, permuted (v[m] = x[2m], v[N-1-m] = x[2m+1])
-__device__ inline void store_permuted(const double (&v)[4][KPT], cd* A, int N) {
+// the rows into the complex sequences of A, permuted (v[m] = x[2m], v[N-1-m] = x[2m+1])
+template <int FS>
+__device__ inline void store_permuted(const double (&v)[2 * FS][KPT], cd* A, int N) {
+  constexpr int FT = 256 * FS;
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr)
+  for (int rr = 0; rr < 2 * FS; ++rr)
 #pragma unroll
     for (int u = 0; u < KPT; ++u) {
       const int c = threadIdx.x + u * FT;
@@ -256,87 +270,92 @@ __device__ inline void store_permuted(const double (&v)[4][KPT], cd* A, int N) {
 
 // Pass 1: rows of R' (R minus (sum r) at (0,0) when rhs), DCT-II along the
 // row, written transposed: out[k * n + row].
-__global__ __launch_bounds__(FT) void k_fft_rows_fwd(const Job* __restrict__ jobs, const int* __restrict__ starts,
+template <int FS>
+__global__ __launch_bounds__(256 * FS) void k_fft_rows_fwd(const Job* __restrict__ jobs, const int* __restrict__ starts,
                                                     int count, bool gate, bool rhs) {
+  constexpr int FT = 256 * FS, ROWS = 2 * FS;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   if (gate && st->done) return;
-  const int N = J.f.n, r0 = 4 * L.lt;
+  const int N = J.f.n, r0 = ROWS * L.lt;
   cd* A = fsm;
-  cd* B = fsm + 2 * N;
-  cd* twl = fsm + 4 * N;
+  cd* B = fsm + FS * N;
+  cd* twl = fsm + 2 * FS * N;
   Prefetch P;
-  prefetch_tables(P, J.fft, N);
-  double v[4][KPT];
-  load_rows(v, J.f.r, N, r0);
+  prefetch_tables<FS>(P, J.fft, N);
+  double v[ROWS][KPT];
+  load_rows<FS>(v, J.f.r, N, r0);
   if (rhs && r0 == 0 && threadIdx.x == 0) v[0][0] -= st->sum_r;  // R' = R - (sum r) e0 e0^T
-  store_twiddles(P, twl, N);
-  store_permuted(v, A, N);
+  store_twiddles<FS>(P, twl, N);
+  store_permuted<FS>(v, A, N);
   __syncthreads();
-  const cd* z = fft2<-1>(A, B, twl, N);
+  const cd* z = fft_seq<FS, -1>(A, B, twl, N);
   const Scales sc(N);
   const auto out = gp(J.f.tmp0);
 #pragma unroll
   for (int u = 0; u < KPT; ++u) {
     const int k = threadIdx.x + u * FT;
     if (k >= N) break;
-    double x0, x1, x2, x3;
-    dct2_post(z, P.ph[u], sc, N, k, x0, x1);
-    dct2_post(z + N, P.ph[u], sc, N, k, x2, x3);
     auto o = reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + r0);
-    o[0] = cd{x0, x1};
-    o[1] = cd{x2, x3};
+#pragma unroll
+    for (int q = 0; q < FS; ++q) {
+      double x0, x1;
+      dct2_post(z + q * N, P.ph[u], sc, N, k, x0, x1);
+      o[q] = cd{x0, x1};
+    }
   }
 }
 
 // Pass 2: rows of B^T (row l: DCT frequency l along j), DCT-II along i,
 // divided by lambda(k, l) (the (0,0) mode set to 0), DCT-III along i,
 // written transposed: out[i * n + l].
-__global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
+template <int FS>
+__global__ __launch_bounds__(256 * FS) void k_fft_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
                                                 int count, bool gate) {
+  constexpr int FT = 256 * FS, ROWS = 2 * FS;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   if (gate && st->done) return;
-  const int N = J.f.n, l0 = 4 * L.lt;
+  const int N = J.f.n, l0 = ROWS * L.lt;
   cd* A = fsm;
-  cd* B = fsm + 2 * N;
-  cd* twl = fsm + 4 * N;
+  cd* B = fsm + FS * N;
+  cd* twl = fsm + 2 * FS * N;
   Prefetch P;
-  prefetch_tables(P, J.fft, N);
-  double v[4][KPT];
-  load_rows(v, J.f.tmp0, N, l0);
+  prefetch_tables<FS>(P, J.fft, N);
+  double v[ROWS][KPT];
+  load_rows<FS>(v, J.f.tmp0, N, l0);
   // eigenvalue halves sin^2(pi |k-l| / 2N) + sin^2(pi (k+l) / 2N) of this thread's (k, l)
   const auto sq = gp(J.sq);
-  double lam[4][KPT];
+  double lam[ROWS][KPT];
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr)
+  for (int rr = 0; rr < ROWS; ++rr)
 #pragma unroll
     for (int u = 0; u < KPT; ++u) {
       const int k = threadIdx.x + u * FT, l = l0 + rr;
       const int dm = k > l ? k - l : l - k;
       lam[rr][u] = k < N ? sq[dm] + sq[k + l] : 1.;
     }
-  store_twiddles(P, twl, N);
-  store_permuted(v, A, N);
+  store_twiddles<FS>(P, twl, N);
+  store_permuted<FS>(v, A, N);
   __syncthreads();
-  cd* z = fft2<-1>(A, B, twl, N);
+  cd* z = fft_seq<FS, -1>(A, B, twl, N);
   cd* other = z == A ? B : A;
   const Scales sc(N);
-  // spectrum / lambda as 4 real rows in the free buffer
+  // spectrum / lambda as ROWS real rows in the free buffer
   double* X = reinterpret_cast<double*>(other);
 #pragma unroll
   for (int u = 0; u < KPT; ++u) {
     const int k = threadIdx.x + u * FT;
     if (k >= N) break;
-    double x[4];
-    dct2_post(z, P.ph[u], sc, N, k, x[0], x[1]);
-    dct2_post(z + N, P.ph[u], sc, N, k, x[2], x[3]);
+    double x[ROWS];
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr)
+    for (int q = 0; q < FS; ++q) dct2_post(z + q * N, P.ph[u], sc, N, k, x[2 * q], x[2 * q + 1]);
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr)
       X[rr * N + k] = (k == 0 && l0 + rr == 0) ? 0. : x[rr] / (2. * lam[rr][u]);
   }
   __syncthreads();
@@ -345,11 +364,15 @@ __global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, c
     const int k = threadIdx.x + u * FT;
     if (k >= N) break;
     const int kn = k == 0 ? 0 : N - k;
-    z[k] = dct3_pre(X[k], X[kn], X[N + k], X[N + kn], P.ph[u], sc, k);
-    z[N + k] = dct3_pre(X[2 * N + k], X[2 * N + kn], X[3 * N + k], X[3 * N + kn], P.ph[u], sc, k);
+#pragma unroll
+    for (int q = 0; q < FS; ++q) {
+      const double* Xa = X + 2 * q * N;
+      const double* Xb = Xa + N;
+      z[q * N + k] = dct3_pre(Xa[k], Xa[kn], Xb[k], Xb[kn], P.ph[u], sc, k);
+    }
   }
   __syncthreads();
-  const cd* y = fft2<1>(z, other, twl, N);
+  const cd* y = fft_seq<FS, 1>(z, other, twl, N);
   const auto out = gp(J.f.tmp1);
   const double inv = 1.0 / N;
 #pragma unroll
@@ -357,33 +380,37 @@ __global__ __launch_bounds__(FT) void k_fft_cols(const Job* __restrict__ jobs, c
     const int i = threadIdx.x + u * FT;
     if (i >= N) break;
     const int m = perm_pos(i, N);
-    const cd a = y[m], b = y[N + m];
     auto o = reinterpret_cast<OTM_GLOBAL cd*>(out + long(i) * N + l0);
-    o[0] = cd{a.x * inv, a.y * inv};
-    o[1] = cd{b.x * inv, b.y * inv};
+#pragma unroll
+    for (int q = 0; q < FS; ++q) {
+      const cd a = y[q * N + m];
+      o[q] = cd{a.x * inv, a.y * inv};
+    }
   }
 }
 
 // Pass 3: rows (spatial i, frequency l along j), DCT-III along j = d_hat,
 // and one |d_hat|^2 partial per workgroup.
-__global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ jobs, const int* __restrict__ starts,
+template <int FS>
+__global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict__ jobs, const int* __restrict__ starts,
                                                     int count, bool gate) {
+  constexpr int FT = 256 * FS, ROWS = 2 * FS;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   if (gate && st->done) return;
-  const int N = J.f.n, r0 = 4 * L.lt;
+  const int N = J.f.n, r0 = ROWS * L.lt;
   cd* A = fsm;
-  cd* B = fsm + 2 * N;
-  cd* twl = fsm + 4 * N;
+  cd* B = fsm + FS * N;
+  cd* twl = fsm + 2 * FS * N;
   Prefetch P;
-  prefetch_tables(P, J.fft, N);
-  // X[row][k] and X[row][N-k] of the 4 rows
+  prefetch_tables<FS>(P, J.fft, N);
+  // X[row][k] and X[row][N-k] of the ROWS rows
   const auto in = gp((const double*)J.f.tmp1) + long(r0) * N;
-  double xk[4][KPT], xn[4][KPT];
+  double xk[ROWS][KPT], xn[ROWS][KPT];
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr)
+  for (int rr = 0; rr < ROWS; ++rr)
 #pragma unroll
     for (int u = 0; u < KPT; ++u) {
       const int k = threadIdx.x + u * FT;
@@ -391,22 +418,23 @@ __global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ job
       xk[rr][u] = k < N ? in[rr * N + k] : 0.;
       xn[rr][u] = k < N ? in[rr * N + kn] : 0.;
     }
-  store_twiddles(P, twl, N);
+  store_twiddles<FS>(P, twl, N);
   const Scales sc(N);
 #pragma unroll
   for (int u = 0; u < KPT; ++u) {
     const int k = threadIdx.x + u * FT;
     if (k >= N) break;
-    A[k] = dct3_pre(xk[0][u], xn[0][u], xk[1][u], xn[1][u], P.ph[u], sc, k);
-    A[N + k] = dct3_pre(xk[2][u], xn[2][u], xk[3][u], xn[3][u], P.ph[u], sc, k);
+#pragma unroll
+    for (int q = 0; q < FS; ++q)
+      A[q * N + k] = dct3_pre(xk[2 * q][u], xn[2 * q][u], xk[2 * q + 1][u], xn[2 * q + 1][u], P.ph[u], sc, k);
   }
   __syncthreads();
-  const cd* y = fft2<1>(A, B, twl, N);
+  const cd* y = fft_seq<FS, 1>(A, B, twl, N);
   const auto out = gp(J.f.dhat);
   const double inv = 1.0 / N;
   double nrm[1] = {0.};
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr)
+  for (int rr = 0; rr < ROWS; ++rr)
 #pragma unroll
     for (int u = 0; u < KPT; ++u) {
       const int c = threadIdx.x + u * FT;
@@ -430,37 +458,46 @@ __global__ __launch_bounds__(FT) void k_fft_rows_inv(const Job* __restrict__ job
   }
 }
 
-}  // namespace
-
-// d_hat of every job on the FFT route (launch kind KIND_FFT, n/4 workgroups
-// per job); dynamic LDS sized for the largest such grid.
-void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
-  if (js.total[KIND_FFT] == 0) return;
+template <int FS>
+void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  if (js.total[kind] == 0) return;
   int nmax = 0;
   for (int p = 0; p < js.count; ++p)
-    if (js.jroute[p] == 2) nmax = std::max(nmax, js.host[p].f.n);
-  const size_t lds = size_t(80) * nmax;
-  // the kernels use no static LDS: up to 80 FFT_MAX_N bytes dynamic, allowed
-  // once per device (bit d of the mask)
+    if (js.jroute[p] == 2 && fft_rows(js.host[p].f.n) == 2 * FS) nmax = std::max(nmax, js.host[p].f.n);
+  const size_t lds = size_t(fft_lds(FS)) * nmax;
+  // the kernels use no static LDS: up to fft_lds(FS) times the largest n of
+  // the kind dynamic, allowed once per device (bit d of the mask)
   static std::atomic<unsigned long long> attr{0};
   int dev = 0;
   OTM_HIP(hipGetDevice(&dev));
   const unsigned long long bit = 1ull << (dev & 63);
   if (!(attr.load() & bit)) {
-    const int cap = 80 * FFT_MAX_N;
-    for (const void* k : {reinterpret_cast<const void*>(&k_fft_rows_fwd), reinterpret_cast<const void*>(&k_fft_cols),
-                          reinterpret_cast<const void*>(&k_fft_rows_inv)})
+    const int cap = fft_lds(FS) * (FS == 1 ? FFT_SPLIT_N : FFT_MAX_N);
+    for (const void* k : {reinterpret_cast<const void*>(&k_fft_rows_fwd<FS>),
+                          reinterpret_cast<const void*>(&k_fft_cols<FS>),
+                          reinterpret_cast<const void*>(&k_fft_rows_inv<FS>)})
       OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, cap));
     attr.fetch_or(bit);
   }
-  const dim3 grid(unsigned(js.total[KIND_FFT]));
-  const int* st = js.starts_of(KIND_FFT);
+  const dim3 grid(unsigned(js.total[kind])), block(256 * FS);
+  const int* st = js.starts_of(kind);
+  const Job* jobs = js.jobs;
   switch (pass) {
-    case 0: hipLaunchKernelGGL(k_fft_rows_fwd, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate, true); break;
-    case 1: hipLaunchKernelGGL(k_fft_cols, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate); break;
-    default: hipLaunchKernelGGL(k_fft_rows_inv, grid, dim3(FT), lds, s, (const Job*)js.jobs, st, js.count, gate);
+    case 0: hipLaunchKernelGGL(k_fft_rows_fwd<FS>, grid, block, lds, s, jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL(k_fft_cols<FS>, grid, block, lds, s, jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL(k_fft_rows_inv<FS>, grid, block, lds, s, jobs, st, js.count, gate);
   }
   OTM_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// d_hat of every job on the FFT route: launch kind KIND_FFT (n <= FFT_SPLIT_N,
+// one workgroup per 2 rows) and KIND_FFT2 (one per 4 rows); dynamic LDS
+// sized for the largest grid of each kind.
+void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
+  launch_pass_kind<1>(js, KIND_FFT, pass, gate, s);
+  launch_pass_kind<2>(js, KIND_FFT2, pass, gate, s);
 }
 
 void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s) {

@@ -186,8 +186,9 @@ enum LaunchKind {
   KIND_DNORM = 5,      // each job's workgroups in the last stage of its transform (writers of the |d_hat|^2 partials)
   KIND_STENCIL64 = 6,  // stencil tiles of the jobs on the FP64 transform (their plain update pass)
   KIND_FLATMX = 7,     // grid-stride blocks of the jobs on the mixed transform (first right-hand side)
-  KIND_FFT = 8,        // fast-DCT passes: one workgroup per 4 rows of each job on the FFT route
-  KIND_M0 = 9,
+  KIND_FFT = 8,        // fast-DCT passes of the FFT-route jobs with n <= FFT_SPLIT_N: one workgroup per 2 rows
+  KIND_FFT2 = 9,       // fast-DCT passes of the FFT-route jobs with n > FFT_SPLIT_N: one workgroup per 4 rows
+  KIND_M0 = 10,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
@@ -230,6 +231,12 @@ void jobset_route(JobSet& js, const std::vector<int>& route);
 inline int mixed_route(int n, int mixed_min_n) { return n % 64 == 0 && n >= 64 && n >= mixed_min_n ? 1 : 0; }
 // Fast-DCT transform (fft.hip): n = 2^a 3^b, 8 <= n <= FFT_MAX_N, n % 4 == 0.
 constexpr int FFT_MAX_N = 2048;
+// Grid rows per workgroup of a fast-DCT pass (two per complex sequence): 2
+// (256 threads, 48 n bytes of LDS, two or more workgroups per CU) up to
+// FFT_SPLIT_N, 4 (512 threads, 80 n bytes) above, where two 2-row
+// workgroups would no longer fit one CU's LDS.
+constexpr int FFT_SPLIT_N = 1024;
+constexpr int fft_rows(int n) { return n <= FFT_SPLIT_N ? 2 : 4; }
 bool fft_supported(int n);
 // FP64 route of a grid: the fast DCT when asked for and supported, else the GEMMs
 inline int fp64_route(int n, int transform) { return transform == 1 && fft_supported(n) ? 2 : 0; }
@@ -272,7 +279,7 @@ struct DctPlan {
   double* fft = nullptr;  // fast-DCT tables (fft.hip), when fft_supported(n)
 };
 void fft_plan_create(double*& table, int n);  // the 4n-double table of Job::fft, or nullptr when unsupported
-// d_hat of every job on the fast-DCT route: three passes, n/4 workgroups per job
+// d_hat of every job on the fast-DCT route: three passes, n / fft_rows(n) workgroups per job
 void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s);
 // one of the three passes (0 rows forward, 1 columns, 2 rows inverse)
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s);

@@ -1116,14 +1116,15 @@ std::vector<int> job_starts(JobSet& js) {
       if (kind == KIND_STENCIL) v = num_tiles(n);
       else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
       else if (kind == KIND_STENCIL64) v = (r64 || rfft) ? num_tiles(n) : 0;  // the plain update pass
-      else if (kind == KIND_FFT) v = rfft ? n / 4 : 0;
+      else if (kind == KIND_FFT) v = rfft && fft_rows(n) == 2 ? n / 2 : 0;
+      else if (kind == KIND_FFT2) v = rfft && fft_rows(n) == 4 ? n / 4 : 0;
       else if (kind == KIND_FLATMX) v = rmx && mixed_supported(n) ? flat_blocks(J.f.F) : 0;
       else if (kind == KIND_PLAIN64) v = r64 ? ((n + 63) / 64) * ((n + 63) / 64) : 0;
       else if (kind == KIND_MRES) v = rmx && mixed_supported(n) ? mres_tiles(n) : 0;
       else if (kind == KIND_UPAIR) v = rmx && mixed_supported(n) ? upair_tiles(n) : 0;
       else if (kind == KIND_DNORM) {
         if (js.tuning) v = 0;
-        else if (js.jroute[p] == 2) v = n / 4;
+        else if (js.jroute[p] == 2) v = n / fft_rows(n);
         else if (js.jroute[p] == 1) v = mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
         else if (js.fold) v = fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
         else v = ((n + 63) / 64) * ((n + 63) / 64);
