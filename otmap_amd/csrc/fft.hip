@@ -278,7 +278,10 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_fwd(const Job* __restrict
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
-  if (gate && st->done) return;
+  // one job: its reads are issued before the gate is read (the two latencies
+  // overlap); in a batch the gate comes first, so converged jobs read nothing
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
   const int N = J.f.n, r0 = ROWS * L.lt;
   cd* A = fsm;
   cd* B = fsm + FS * N;
@@ -288,6 +291,7 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_fwd(const Job* __restrict
   double v[ROWS][KPT];
   load_rows<FS>(v, J.f.r, N, r0);
   if (rhs && r0 == 0 && threadIdx.x == 0) v[0][0] -= st->sum_r;  // R' = R - (sum r) e0 e0^T
+  if (gate && early && st->done) return;
   store_twiddles<FS>(P, twl, N);
   store_permuted<FS>(v, A, N);
   __syncthreads();
@@ -319,7 +323,10 @@ __global__ __launch_bounds__(256 * FS) void k_fft_cols(const Job* __restrict__ j
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
-  if (gate && st->done) return;
+  // one job: its reads are issued before the gate is read (the two latencies
+  // overlap); in a batch the gate comes first, so converged jobs read nothing
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
   const int N = J.f.n, l0 = ROWS * L.lt;
   cd* A = fsm;
   cd* B = fsm + FS * N;
@@ -339,6 +346,7 @@ __global__ __launch_bounds__(256 * FS) void k_fft_cols(const Job* __restrict__ j
       const int dm = k > l ? k - l : l - k;
       lam[rr][u] = k < N ? sq[dm] + sq[k + l] : 1.;
     }
+  if (gate && early && st->done) return;
   store_twiddles<FS>(P, twl, N);
   store_permuted<FS>(v, A, N);
   __syncthreads();
@@ -399,7 +407,10 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
-  if (gate && st->done) return;
+  // one job: its reads are issued before the gate is read (the two latencies
+  // overlap); in a batch the gate comes first, so converged jobs read nothing
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
   const int N = J.f.n, r0 = ROWS * L.lt;
   cd* A = fsm;
   cd* B = fsm + FS * N;
@@ -418,6 +429,7 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict
       xk[rr][u] = k < N ? in[rr * N + k] : 0.;
       xn[rr][u] = k < N ? in[rr * N + kn] : 0.;
     }
+  if (gate && early && st->done) return;
   store_twiddles<FS>(P, twl, N);
   const Scales sc(N);
 #pragma unroll

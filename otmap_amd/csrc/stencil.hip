@@ -394,7 +394,10 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
   const auto st = gp((const SolveState*)f.state);
-  if (st->done || st->it < 1 || !st->cj) return;
+  // one job: the tile's reads are issued before the gate is read (the two
+  // latencies overlap); in a batch the gate comes first
+  const bool early = count == 1;
+  if (!early && (st->done || st->it < 1 || !st->cj)) return;
   const int n = f.n;
   const auto psi = gp(f.psi);
   const auto dhat = gp(f.dhat);
@@ -429,6 +432,7 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
     jdv[u] = in ? jd[fc] : 0.;
     rv[u] = in ? r[fc] : 0.;
   }
+  if (early && (st->done || st->it < 1 || !st->cj)) return;
 #pragma unroll
   for (int u = 0; u < HR; ++u) {
     const int t = threadIdx.x + u * NT;
