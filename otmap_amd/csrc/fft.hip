@@ -23,7 +23,7 @@
 //   2  rows of B^T (columns of B) -> DCT-II along i, / lambda, DCT-III along
 //      i, written transposed back (spatial i, frequency l along j)
 //   3  rows -> DCT-III along j = d_hat, plus the |d_hat|^2 partials
-// The FFT is a self-sorting (Stockham) mixed-radix FFT, radices 4, 2, 3, in
+// The FFT is a self-sorting (Stockham) mixed-radix FFT, radices 8, 4, 2, 3, in
 // two LDS buffers; the twiddle table exp(-2 pi i t/n) is staged in LDS.
 // Supported sizes: n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0.
 
@@ -80,6 +80,17 @@ __device__ inline cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
 // multiply by -i (sign < 0) or +i (sign > 0)
 __device__ inline cd mul_i(cd a, int sign) { return sign < 0 ? cd{a.y, -a.x} : cd{-a.y, a.x}; }
 
+// DFT of 4 points with w4 = e^{SIGN 2 pi i / 4} = SIGN i
+template <int SIGN>
+__device__ inline void dft4(cd a0, cd a1, cd a2, cd a3, cd& x0, cd& x1, cd& x2, cd& x3) {
+  const cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
+  const cd s13 = cadd(a1, a3), d13 = mul_i(csub(a1, a3), SIGN);
+  x0 = cadd(s02, s13);
+  x1 = cadd(d02, d13);
+  x2 = csub(s02, s13);
+  x3 = csub(d02, d13);
+}
+
 // One radix-R stage of the self-sorting FFT of the FS sequences held in src
 // (seq q at src[q N .. q N + N)), written to dst: butterfly b of sequence q,
 // j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{SIGN 2 pi
@@ -110,12 +121,23 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
       c[0] = cadd(a[0], a[1]);
       c[1] = csub(a[0], a[1]);
     } else if (R == 4) {
-      const cd s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
-      const cd s13 = cadd(a[1], a[3]), d13 = mul_i(csub(a[1], a[3]), SIGN);
-      c[0] = cadd(s02, s13);
-      c[1] = cadd(d02, d13);
-      c[2] = csub(s02, s13);
-      c[3] = csub(d02, d13);
+      dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
+    } else if (R == 8) {
+      // X[2q] = DFT4(a_r + a_{r+4})[q], X[2q+1] = DFT4((a_r - a_{r+4}) w8^r)[q],
+      // w8 = e^{SIGN 2 pi i / 8}: w8^1 = (1 + SIGN i)/sqrt 2, w8^2 = SIGN i,
+      // w8^3 = (-1 + SIGN i)/sqrt 2
+      const double h = 0.70710678118654752440;  // 1 / sqrt(2)
+      cd s[4], d[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        s[r] = cadd(a[r], a[r + 4]);
+        d[r] = csub(a[r], a[r + 4]);
+      }
+      d[1] = cd{h * (d[1].x - SIGN * d[1].y), h * (d[1].y + SIGN * d[1].x)};
+      d[2] = mul_i(d[2], SIGN);
+      d[3] = cd{h * (-d[3].x - SIGN * d[3].y), h * (-d[3].y + SIGN * d[3].x)};
+      dft4<SIGN>(s[0], s[1], s[2], s[3], c[0], c[2], c[4], c[6]);
+      dft4<SIGN>(d[0], d[1], d[2], d[3], c[1], c[3], c[5], c[7]);
     } else {  // R == 3: w3 = e^{SIGN 2 pi i / 3} = (-1/2, SIGN sqrt(3)/2)
       const double h3 = 0.86602540378443864676;  // sqrt(3) / 2
       const cd s12 = cadd(a[1], a[2]), d12 = csub(a[1], a[2]);
@@ -132,12 +154,17 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
 }
 
 // FFT of the FS sequences of `a` (length N each), `b` scratch; returns the
-// buffer that holds the result.  Radices 4 first, then 2, then 3.
+// buffer that holds the result.  Radices 8 first, then 4, 2 and 3 (fewer
+// stages: every stage ends in a workgroup barrier).
 template <int FS, int SIGN>
 __device__ inline cd* fft_seq(cd* a, cd* b, const cd* tw, int N) {
   int rest = N, L = 1;
   while (rest > 1) {
-    if (rest % 4 == 0) {
+    if (rest % 8 == 0) {
+      fft_stage<FS, 8, SIGN>(a, b, tw, N, L);
+      L *= 8;
+      rest /= 8;
+    } else if (rest % 4 == 0) {
       fft_stage<FS, 4, SIGN>(a, b, tw, N, L);
       L *= 4;
       rest /= 4;
