@@ -26,7 +26,7 @@ from test_gpu_solver import _fft_dct_solve, _rng
 pytestmark = pytest.mark.gpu
 
 FFT = SolverOptions(transform=Transform.FFT)
-SIZES = [8, 12, 16, 24, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 2048]
+SIZES = [8, 12, 16, 24, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1152, 1536, 2048]
 
 
 def _apply_laplacian(n, d):
@@ -182,3 +182,22 @@ def test_fft_pass_timing_entry_points(solver_factory):
     u = solver_factory(100)
     for k in (_lib.OTM_KERNEL_FFT_SOLVE, _lib.OTM_KERNEL_FFT_COLS):
         assert _lib.lib().otm_solver_profile_kernel(u._h, k, 3, _lib.C.byref(ms)) != 0
+
+
+def test_fft_batch_mixing_both_workgroup_forms():
+    """One job table with grids on both fast-DCT workgroup forms (2 rows per
+    workgroup up to n = 1024, 4 rows above; 1152 = 2^7 3^2 needs a radix-3
+    stage) and one on the GEMM route (320 = 64 * 5): bit-identical to single
+    solves."""
+    sizes = [1152, 192, 2048, 320, 1024]
+    dens = [D.gaussian_mixture(n, 70 + k, K=3) for k, n in enumerate(sizes)]
+    print("fft batch of both forms", flush=True)
+    b = BatchSolver(sizes)
+    maps = b.solve(dens, SolverOptions(max_iter=12))
+    for k, (n, rho) in enumerate(zip(sizes, dens)):
+        s = GridBasedTransportSolver(0)
+        s.set_verbose_level(0)
+        s.init(n)
+        tm = s.solve(rho, SolverOptions(max_iter=12))
+        assert b.last_stats[k].iterations == s.last_stats.iterations, (k, n)
+        assert np.array_equal(maps[k].fwd_mesh().xy, tm.fwd_mesh().xy), (k, n)
