@@ -24,7 +24,8 @@
 //      i, written transposed back (spatial i, frequency l along j)
 //   3  rows -> DCT-III along j = d_hat, plus the |d_hat|^2 partials
 // The FFT is a self-sorting (Stockham) mixed-radix FFT, radices 8, 4, 2, 3, in
-// two LDS buffers; the twiddle table exp(-2 pi i t/n) is staged in LDS.
+// two LDS buffers; the stage twiddles (stage-major, n - 1 entries) are staged
+// in LDS.
 // Supported sizes: n = 2^a 3^b, 8 <= n <= 2048, n % 4 == 0.
 
 #include <atomic>
@@ -45,16 +46,34 @@ bool fft_supported(int n) {
   return m == 1;
 }
 
+// Radix of the FFT stage that follows span L with `rest` points left to
+// combine: 8 first, then 4, 2 and 3 (the device's fft_seq takes the same).
+static int stage_radix(int rest) { return rest % 8 == 0 ? 8 : rest % 4 == 0 ? 4 : rest % 2 == 0 ? 2 : 3; }
+
 void fft_plan_create(double*& table, int n) {
   table = nullptr;
   if (!fft_supported(n)) return;
-  // twiddles exp(-2 pi i t / n), t < n, then phases exp(-i pi k / 2n), k < n
-  std::vector<double> h(4 * size_t(n));
+  // Twiddles by stage: the stage of radix R after span L (L = product of the
+  // earlier radices) uses exp(-2 pi i j r / (L R)), j < L, 1 <= r < R, at
+  // entry (L - 1) + (r - 1) L + j -- consecutive j consecutive, so the lanes
+  // of a stage read consecutive LDS words; the stages fill entries 0 .. n-2.
+  // Then the DCT phases exp(-i pi k / 2n), k < n.
+  std::vector<double> h(4 * size_t(n), 0.);
   const long double pi = 3.141592653589793238462643383279502884L;
+  for (int rest = n, L = 1; rest > 1;) {
+    const int R = stage_radix(rest);
+    for (int r = 1; r < R; ++r)
+      for (int j = 0; j < L; ++j) {
+        const long double a = 2.0L * pi * (long double)(j * r) / (long double)(L * R);
+        const size_t e = size_t(L - 1 + (r - 1) * L + j);
+        h[2 * e] = double(cosl(a));
+        h[2 * e + 1] = double(-sinl(a));
+      }
+    L *= R;
+    rest /= R;
+  }
   for (int t = 0; t < n; ++t) {
-    const long double a = 2.0L * pi * t / n, b = pi * t / (2.0L * n);
-    h[2 * t] = double(cosl(a));
-    h[2 * t + 1] = double(-sinl(a));
+    const long double b = pi * t / (2.0L * n);
     h[2 * n + 2 * t] = double(cosl(b));
     h[2 * n + 2 * t + 1] = double(-sinl(b));
   }
@@ -94,14 +113,16 @@ __device__ inline void dft4(cd a0, cd a1, cd a2, cd a3, cd& x0, cd& x1, cd& x2, 
 // One radix-R stage of the self-sorting FFT of the FS sequences held in src
 // (seq q at src[q N .. q N + N)), written to dst: butterfly b of sequence q,
 // j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{SIGN 2 pi
-// i / (L R)}), and its DFT_R output q' goes to dst[m L R + j + q' L].  Every
-// input is multiplied by its twiddle, also w^0 = (1, 0), which is exact, so
-// the butterfly has no branches and all its LDS reads are issued together.
+// i / (L R)}, tw[(L - 1) + (r - 1) L + j] in the stage-major table), and its
+// DFT_R output q' goes to dst[m L R + j + q' L].  The first stage (L = 1)
+// has no twiddles; later stages multiply every input by its twiddle, also
+// w^0 = (1, 0), which is exact, so the butterfly has no branches and all its
+// LDS reads are issued together.
 template <int FS, int R, int SIGN>
 __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
                                  int L) {
   constexpr int FT = 256 * FS;
-  const int NR = N / R, tstep = N / (L * R);
+  const int NR = N / R;
   const bool pow2 = (L & (L - 1)) == 0;
   const int lshift = __ffs(L) - 1;
   for (int b2 = threadIdx.x; b2 < FS * NR; b2 += FT) {
@@ -112,10 +133,12 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
     cd a[R], w[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = s[b + r * NR];
+    if (L > 1) {
 #pragma unroll
-    for (int r = 1; r < R; ++r) w[r] = tw[j * r * tstep];
+      for (int r = 1; r < R; ++r) w[r] = tw[(L - 1) + (r - 1) * L + j];
 #pragma unroll
-    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], SIGN < 0 ? w[r] : conjc(w[r]));
+      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], SIGN < 0 ? w[r] : conjc(w[r]));
+    }
     cd c[R];
     if (R == 2) {
       c[0] = cadd(a[0], a[1]);
@@ -155,7 +178,8 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
 
 // FFT of the FS sequences of `a` (length N each), `b` scratch; returns the
 // buffer that holds the result.  Radices 8 first, then 4, 2 and 3 (fewer
-// stages: every stage ends in a workgroup barrier).
+// stages: every stage ends in a workgroup barrier), as stage_radix on the
+// host, which lays out the stage twiddles in this order.
 template <int FS, int SIGN>
 __device__ inline cd* fft_seq(cd* a, cd* b, const cd* tw, int N) {
   int rest = N, L = 1;
