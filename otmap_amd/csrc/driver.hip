@@ -7,9 +7,10 @@
 // (:223-227).  Iterations are captured once into two HIP graphs, one of
 // ITERS_PER_GRAPH iterations and one of a single iteration, and replayed;
 // after each replay one small kernel counts the jobs still running, and those
-// of them still far from their stop, into one of two pinned slots, and the
-// host reads the counts of replay k while replay k + 1 runs (the device never
-// waits for the host).  Long replays run while some job is far from its stop;
+// of them still far from their stop, straight into one of two slots of mapped
+// pinned host memory (no copy launch), and the host reads the counts of
+// replay k while replay k + 1 runs (the device never waits for the host).
+// Long replays run while some job is far from its stop;
 // once every running job is near it (residual within NEAR_FACTOR of the
 // threshold, or within a long replay of max_iter) the host queues single
 // iterations, so at most about two iterations run gated after the last job
@@ -97,8 +98,11 @@ void driver_create(Driver& d) {
   OTM_HIP(hipEventCreate(&d.ev1));
   OTM_HIP(hipEventCreate(&d.evc[0]));
   OTM_HIP(hipEventCreate(&d.evc[1]));
-  OTM_HIP(hipMalloc(&d.d_open, 4 * sizeof(int)));
-  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), 4 * sizeof(int), hipHostMallocDefault));
+  // the count kernel writes the host's slots directly (coherent: no host-side
+  // cache to flush; the event after the kernel orders the host's read)
+  OTM_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.h_open), 4 * sizeof(int),
+                        hipHostMallocMapped | hipHostMallocCoherent));
+  OTM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&d.d_open), d.h_open, 0));
 }
 
 void driver_invalidate(Driver& d) {
@@ -114,7 +118,6 @@ void driver_invalidate(Driver& d) {
 void driver_destroy(Driver& d) {
   if (d.stream) (void)hipStreamSynchronize(d.stream);
   driver_invalidate(d);
-  if (d.d_open) (void)hipFree(d.d_open);
   if (d.h_open) (void)hipHostFree(d.h_open);
   if (d.ev0) (void)hipEventDestroy(d.ev0);
   if (d.ev1) (void)hipEventDestroy(d.ev1);
@@ -170,7 +173,6 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
     OTM_HIP(hipGraphLaunch(d.exec[single ? 1 : 0], s));
     launched += single ? 1 : ITERS_PER_GRAPH;
     launch_count_open(js, d.d_open + 2 * slot, NEAR_FACTOR, ITERS_PER_GRAPH, s);
-    OTM_HIP(hipMemcpyAsync(d.h_open + 2 * slot, d.d_open + 2 * slot, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     OTM_HIP(hipEventRecord(d.evc[slot], s));
   };
   int slot = 0;

@@ -18,7 +18,8 @@
 // x[2m+1] = v[n-1-m]; again two rows per complex sequence.
 //
 // Passes (one workgroup per 2 rows, one complex sequence, for n <= 1024;
-// per 4 rows, two sequences, above):
+// per 4 rows, two sequences, above; for n = 512 and 1024 a 64-thread
+// workgroup per 2 rows in the register form below):
 //   1  rows of R' -> DCT-II along j, written transposed (B^T, 16- or 32-byte runs)
 //   2  rows of B^T (columns of B) -> DCT-II along i, / lambda, DCT-III along
 //      i, written transposed back (spatial i, frequency l along j)
@@ -60,8 +61,11 @@ void fft_plan_create(double*& table, int n) {
   // Then the DCT phases exp(-i pi k / 2n), k < n.
   std::vector<double> h(4 * size_t(n), 0.);
   const long double pi = 3.141592653589793238462643383279502884L;
-  for (int rest = n, L = 1; rest > 1;) {
-    const int R = stage_radix(rest);
+  // the register form (n = 1024: radices 16, 16, 4) or stage_radix's order
+  const int reg16[3] = {16, 16, 4};
+  int si = 0;
+  for (int rest = n, L = 1; rest > 1; ++si) {
+    const int R = fft_reg_e(n) == 16 ? reg16[si] : stage_radix(rest);
     for (int r = 1; r < R; ++r)
       for (int j = 0; j < L; ++j) {
         const long double a = 2.0L * pi * (long double)(j * r) / (long double)(L * R);
@@ -110,6 +114,26 @@ __device__ inline void dft4(cd a0, cd a1, cd a2, cd a3, cd& x0, cd& x1, cd& x2, 
   x3 = csub(d02, d13);
 }
 
+// DFT of 8 points: X[2q] = DFT4(a_r + a_{r+4})[q], X[2q+1] = DFT4((a_r -
+// a_{r+4}) w8^r)[q], w8 = e^{SIGN 2 pi i / 8}: w8^1 = (1 + SIGN i)/sqrt 2,
+// w8^2 = SIGN i, w8^3 = (-1 + SIGN i)/sqrt 2
+template <int SIGN, int RA>
+__device__ inline void dft8(const cd (&a)[RA], cd (&c)[RA]) {
+  static_assert(RA >= 8, "eight points");
+  const double h = 0.70710678118654752440;  // 1 / sqrt(2)
+  cd s[4], d[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    s[r] = cadd(a[r], a[r + 4]);
+    d[r] = csub(a[r], a[r + 4]);
+  }
+  d[1] = cd{h * (d[1].x - SIGN * d[1].y), h * (d[1].y + SIGN * d[1].x)};
+  d[2] = mul_i(d[2], SIGN);
+  d[3] = cd{h * (-d[3].x - SIGN * d[3].y), h * (-d[3].y + SIGN * d[3].x)};
+  dft4<SIGN>(s[0], s[1], s[2], s[3], c[0], c[2], c[4], c[6]);
+  dft4<SIGN>(d[0], d[1], d[2], d[3], c[1], c[3], c[5], c[7]);
+}
+
 // One radix-R stage of the self-sorting FFT of the FS sequences held in src
 // (seq q at src[q N .. q N + N)), written to dst: butterfly b of sequence q,
 // j = b % L, m = b / L, takes src[b + r N/R] times w^(j r) (w = e^{SIGN 2 pi
@@ -140,27 +164,13 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
       for (int r = 1; r < R; ++r) a[r] = cmul(a[r], SIGN < 0 ? w[r] : conjc(w[r]));
     }
     cd c[R];
-    if (R == 2) {
+    if constexpr (R == 2) {
       c[0] = cadd(a[0], a[1]);
       c[1] = csub(a[0], a[1]);
-    } else if (R == 4) {
+    } else if constexpr (R == 4) {
       dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
-    } else if (R == 8) {
-      // X[2q] = DFT4(a_r + a_{r+4})[q], X[2q+1] = DFT4((a_r - a_{r+4}) w8^r)[q],
-      // w8 = e^{SIGN 2 pi i / 8}: w8^1 = (1 + SIGN i)/sqrt 2, w8^2 = SIGN i,
-      // w8^3 = (-1 + SIGN i)/sqrt 2
-      const double h = 0.70710678118654752440;  // 1 / sqrt(2)
-      cd s[4], d[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        s[r] = cadd(a[r], a[r + 4]);
-        d[r] = csub(a[r], a[r + 4]);
-      }
-      d[1] = cd{h * (d[1].x - SIGN * d[1].y), h * (d[1].y + SIGN * d[1].x)};
-      d[2] = mul_i(d[2], SIGN);
-      d[3] = cd{h * (-d[3].x - SIGN * d[3].y), h * (-d[3].y + SIGN * d[3].x)};
-      dft4<SIGN>(s[0], s[1], s[2], s[3], c[0], c[2], c[4], c[6]);
-      dft4<SIGN>(d[0], d[1], d[2], d[3], c[1], c[3], c[5], c[7]);
+    } else if constexpr (R == 8) {
+      dft8<SIGN>(a, c);
     } else {  // R == 3: w3 = e^{SIGN 2 pi i / 3} = (-1/2, SIGN sqrt(3)/2)
       const double h3 = 0.86602540378443864676;  // sqrt(3) / 2
       const cd s12 = cadd(a[1], a[2]), d12 = csub(a[1], a[2]);
@@ -221,28 +231,34 @@ struct Scales {
   __device__ double r(int k) const { return k == 0 ? r0 : r1; }
 };
 
-// forward post-processing of one sequence at frequency k (phase p =
-// e^{-i pi k/2N}): the DCT-II coefficients of its two rows (real part row
-// 2q, imaginary part row 2q+1)
-__device__ inline void dct2_post(const cd* z, cd p, const Scales& sc, int N, int k, double& xa, double& xb) {
-  const cd Z = z[k], Zc = conjc(z[k == 0 ? 0 : N - k]);
+// forward post-processing of one sequence at frequency k from Z[k] and
+// Z[N-k] (Z[0] for k = 0), phase p = e^{-i pi k/2N}, scale s = s_k: the
+// DCT-II coefficients of its two rows (real part row 2q, imaginary part row 2q+1)
+__device__ inline void dct2_post_pair(cd Z, cd Zn, cd p, double s, double& xa, double& xb) {
+  const cd Zc = conjc(Zn);
   const cd va = {0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y)};
   const cd dd = csub(Z, Zc);
   const cd vb = {0.5 * dd.y, -0.5 * dd.x};  // (Z - Zc) / 2i
-  const double s = sc.s(k);
   xa = s * (p.x * va.x - p.y * va.y);
   xb = s * (p.x * vb.x - p.y * vb.y);
 }
+__device__ inline void dct2_post(const cd* z, cd p, const Scales& sc, int N, int k, double& xa, double& xb) {
+  dct2_post_pair(z[k], z[k == 0 ? 0 : N - k], p, sc.s(k), xa, xb);
+}
 
-// inverse pre-processing at frequency k from the DCT-II coefficients of two
-// rows (a = row 2q, b = row 2q+1; xan = X_a[N-k]): Z[k] = V_a[k] + i V_b[k]
-__device__ inline cd dct3_pre(double xa, double xan, double xb, double xbn, cd p, const Scales& sc, int k) {
-  const double r = sc.r(k);
-  const double ua = xa * r, ub = xb * r;
-  const double uan = k == 0 ? 0. : xan * sc.r1, ubn = k == 0 ? 0. : xbn * sc.r1;
+// inverse pre-processing at frequency k from the scaled coefficients U = X /
+// s_k of two rows (ua = U_a[k], uan = U_a[N-k], 0 for k = 0; likewise b):
+// Z[k] = V_a[k] + i V_b[k], V[k] = e^{i pi k/2N} (U[k] - i U[N-k])
+__device__ inline cd dct3_pre_u(double ua, double uan, double ub, double ubn, cd p) {
   const cd e = conjc(p);  // e^{i pi k / 2N}
   const cd va = cmul(e, cd{ua, -uan}), vb = cmul(e, cd{ub, -ubn});
   return {va.x - vb.y, va.y + vb.x};  // V_a + i V_b
+}
+// the same from the DCT-II coefficients (a = row 2q, b = row 2q+1; xan = X_a[N-k])
+__device__ inline cd dct3_pre(double xa, double xan, double xb, double xbn, cd p, const Scales& sc, int k) {
+  const double r = sc.r(k);
+  const double uan = k == 0 ? 0. : xan * sc.r1, ubn = k == 0 ? 0. : xbn * sc.r1;
+  return dct3_pre_u(xa * r, uan, xb * r, ubn, p);
 }
 
 struct FftLoc {
@@ -521,12 +537,343 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register form (n = 512 and 1024, fft_reg_e(n) = E = n / 64): one wave per
+// complex sequence (two grid rows), 64-thread workgroups, lane t holding the
+// E points t + 64 v (v < E) of its sequence in registers.  A stage of radix R
+// after span L gives lane t the butterflies b = t + 64 u (u < E / R), whose
+// inputs b + r n / R are its own slots u + r E / R; its outputs go through
+// the wave's LDS exchange buffer back into the same layout, except in the
+// last stage (L R = n), where they land in the lane's own slots.  Radices 8,
+// 8, 8 (n = 512) and 16, 16, 4 (n = 1024): two exchanges per FFT instead of
+// a workgroup barrier after each of five radix-8/4/2 stages.  The DCT
+// post/pre-processing pairs frequency k with n - k through one more
+// exchange.  The stage twiddles and DCT phases are staged in LDS at the
+// start, as are the exchange buffer's padded positions p + p / 16 (lanes of a
+// 16-lane group write distinct banks).
+// ---------------------------------------------------------------------------
+template <int E>
+struct RegRadix;
+template <>
+struct RegRadix<8> {
+  static constexpr int R0 = 8, R1 = 8, R2 = 8;
+};
+template <>
+struct RegRadix<16> {
+  static constexpr int R0 = 16, R1 = 16, R2 = 4;
+};
+
+__device__ inline int xpad(int p) { return p + (p >> 4); }
+
+// DFT of 16 points as 4 x 4: DFT4 over n1 of a[n2 + 4 n1], times w16^(n2 k1),
+// DFT4 over n2; X[k1 + 4 k2] lands in c[k1 + 4 k2]
+template <int SIGN>
+__device__ inline cd w16mul(cd y, int m) {
+  const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;  // cos, sin(pi/8)
+  const double h = 0.70710678118654752440;
+  switch (m) {
+    case 1: return cmul(y, cd{c1, SIGN * s1});
+    case 2: return cmul(y, cd{h, SIGN * h});
+    case 3: return cmul(y, cd{s1, SIGN * c1});
+    case 4: return mul_i(y, SIGN);
+    case 6: return cmul(y, cd{-h, SIGN * h});
+    case 9: return cmul(y, cd{-c1, -SIGN * s1});
+    default: return y;
+  }
+}
+template <int SIGN>
+__device__ inline void dft16(const cd (&a)[16], cd (&c)[16]) {
+  cd y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2)
+    dft4<SIGN>(a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12], y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+#pragma unroll
+  for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = w16mul<SIGN>(y[n2][k1], n2 * k1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) dft4<SIGN>(y[0][k1], y[1][k1], y[2][k1], y[3][k1], c[k1], c[k1 + 4], c[k1 + 8], c[k1 + 12]);
+}
+
+template <int E, int R, int L, int SIGN>
+__device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
+  constexpr int N = 64 * E, U = E / R;
+  constexpr bool LAST = L * R == N;
+  static_assert(E % R == 0 && (!LAST || L == 64 * U), "lane slots");
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int b = t + 64 * u;
+    const int j = b & (L - 1), m = b / L;
+    cd a[R], c[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = x[u + r * U];
+    if constexpr (L > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cd w = tw[(L - 1) + (r - 1) * L + j];
+        a[r] = cmul(a[r], SIGN < 0 ? w : conjc(w));
+      }
+    }
+    if constexpr (R == 4) dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
+    else if constexpr (R == 8) dft8<SIGN>(a, c);
+    else dft16<SIGN>(a, c);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) x[u + q * U] = c[q];  // position b + q L = t + 64 (u + q U)
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) xb[xpad(m * L * R + j + q * L)] = c[q];
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < E; ++v) x[v] = xb[xpad(t + 64 * v)];
+    __syncthreads();
+  }
+}
+
+template <int E, int SIGN>
+__device__ inline void reg_fft(cd (&x)[E], cd* xb, const cd* tw, int t) {
+  using RR = RegRadix<E>;
+  rstage<E, RR::R0, 1, SIGN>(x, xb, tw, t);
+  rstage<E, RR::R1, RR::R0, SIGN>(x, xb, tw, t);
+  rstage<E, RR::R2, RR::R0 * RR::R1, SIGN>(x, xb, tw, t);
+}
+
+// Common start of a register-form pass: the job, the gate, the table loads.
+// LDS: the exchange buffer (n + n/16 complex), then the stage twiddles (n
+// complex, n - 1 used) and the DCT phases (n complex).
+template <int E>
+struct RegPass {
+  static constexpr int N = 64 * E;
+  static constexpr size_t lds_bytes() { return size_t(3 * N + N / 16) * sizeof(cd); }
+};
+template <int E>
+__device__ inline void reg_load_table(cd (&tv)[2 * E], const double* table, int t) {
+  const auto tg = gp(reinterpret_cast<const cd*>(table));
+#pragma unroll
+  for (int q = 0; q < 2 * E; ++q) tv[q] = tg[t + 64 * q];
+}
+template <int E>
+__device__ inline void reg_store_table(const cd (&tv)[2 * E], cd* tws, int t) {
+#pragma unroll
+  for (int q = 0; q < 2 * E; ++q) tws[t + 64 * q] = tv[q];
+}
+// two real rows r0, r0 + 1 as the permuted complex sequence z = v_a + i v_b,
+// v[m] = x[2m], v[n-1-m] = x[2m+1]: lane t slot v holds z[t + 64 v]
+template <int E>
+__device__ inline void reg_load_rows(cd (&x)[E], const double* in_, int r0, int t) {
+  constexpr int N = 64 * E;
+  const auto in = gp(in_) + long(r0) * N;
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int m = t + 64 * v;
+    const int c = m < N / 2 ? 2 * m : 2 * N - 1 - 2 * m;
+    x[v] = cd{in[c], in[N + c]};
+  }
+}
+
+// Pass 1 (register form): rows of R' -> DCT-II along j, written transposed.
+template <int E>
+__global__ __launch_bounds__(64) void k_fftr_rows_fwd(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                      int count, bool gate, bool rhs) {
+  constexpr int N = 64 * E;
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const int t = threadIdx.x, r0 = 2 * L.lt;
+  cd* xb = fsm;
+  cd* tws = fsm + N + N / 16;
+  cd tv[2 * E];
+  reg_load_table<E>(tv, J.fft, t);
+  cd x[E];
+  reg_load_rows<E>(x, J.f.r, r0, t);
+  if (rhs && r0 == 0 && t == 0) x[0].x -= st->sum_r;  // R' = R - (sum r) e0 e0^T
+  if (gate && early && st->done) return;
+  reg_store_table<E>(tv, tws, t);
+  __syncthreads();
+  reg_fft<E, -1>(x, xb, tws, t);
+#pragma unroll
+  for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = x[v];
+  __syncthreads();
+  const Scales sc(N);
+  const auto out = gp(J.f.tmp0);
+  const cd* ph = tws + N;
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int k = t + 64 * v;
+    double xa, xo;
+    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], ph[k], sc.s(k), xa, xo);
+    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + r0) = cd{xa, xo};
+  }
+}
+
+// Pass 2 (register form): rows of B^T, DCT-II along i, / lambda, DCT-III
+// along i, written transposed back.
+template <int E>
+__global__ __launch_bounds__(64) void k_fftr_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                  int count, bool gate) {
+  constexpr int N = 64 * E;
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const int t = threadIdx.x, l0 = 2 * L.lt;
+  cd* xb = fsm;
+  cd* tws = fsm + N + N / 16;
+  cd tv[2 * E];
+  reg_load_table<E>(tv, J.fft, t);
+  cd x[E];
+  reg_load_rows<E>(x, J.f.tmp0, l0, t);
+  // eigenvalue halves sin^2(pi |k-l| / 2N) + sin^2(pi (k+l) / 2N) of this lane's (k, l)
+  const auto sq = gp(J.sq);
+  double lam[2][E];
+#pragma unroll
+  for (int v = 0; v < E; ++v)
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      const int k = t + 64 * v, l = l0 + rr;
+      const int dm = k > l ? k - l : l - k;
+      lam[rr][v] = sq[dm] + sq[k + l];
+    }
+  if (gate && early && st->done) return;
+  reg_store_table<E>(tv, tws, t);
+  __syncthreads();
+  reg_fft<E, -1>(x, xb, tws, t);
+#pragma unroll
+  for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = x[v];
+  __syncthreads();
+  const Scales sc(N);
+  const cd* ph = tws + N;
+  // spectrum / lambda, scaled by 1 / s_k for the inverse: (U_a[k], U_b[k])
+  cd w[E];
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int k = t + 64 * v;
+    double xa, xo;
+    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], ph[k], sc.s(k), xa, xo);
+    const double Xa = (k == 0 && l0 == 0) ? 0. : xa / (2. * lam[0][v]);
+    const double Xb = xo / (2. * lam[1][v]);
+    w[v] = cd{Xa * sc.r(k), Xb * sc.r(k)};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = w[v];
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int k = t + 64 * v;
+    const cd wn = k == 0 ? cd{0., 0.} : xb[xpad(N - k)];
+    x[v] = dct3_pre_u(w[v].x, wn.x, w[v].y, wn.y, ph[k]);
+  }
+  __syncthreads();
+  reg_fft<E, 1>(x, xb, tws, t);
+  const auto out = gp(J.f.tmp1);
+  const double inv = 1.0 / N;
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int m = t + 64 * v;
+    const int i = m < N / 2 ? 2 * m : 2 * (N - 1 - m) + 1;  // perm_pos(i) = m
+    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(i) * N + l0) = cd{x[v].x * inv, x[v].y * inv};
+  }
+}
+
+// Pass 3 (register form): DCT-III along j = d_hat, and the |d_hat|^2 partial.
+template <int E>
+__global__ __launch_bounds__(64) void k_fftr_rows_inv(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                      int count, bool gate) {
+  constexpr int N = 64 * E;
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const int t = threadIdx.x, r0 = 2 * L.lt;
+  cd* xb = fsm;
+  cd* tws = fsm + N + N / 16;
+  cd tv[2 * E];
+  reg_load_table<E>(tv, J.fft, t);
+  // X[row][k] and X[row][N-k] of the two rows
+  const auto in = gp((const double*)J.f.tmp1) + long(r0) * N;
+  double xk[2][E], xn[2][E];
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int k = t + 64 * v;
+    const int kn = k == 0 ? 0 : N - k;
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      xk[rr][v] = in[rr * N + k];
+      xn[rr][v] = in[rr * N + kn];
+    }
+  }
+  if (gate && early && st->done) return;
+  reg_store_table<E>(tv, tws, t);
+  const Scales sc(N);
+  cd x[E];
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int k = t + 64 * v;
+    x[v] = dct3_pre(xk[0][v], xn[0][v], xk[1][v], xn[1][v], tws[N + k], sc, k);
+  }
+  __syncthreads();
+  reg_fft<E, 1>(x, xb, tws, t);
+  const auto out = gp(J.f.dhat) + long(r0) * N;
+  const double inv = 1.0 / N;
+  double nrm[1] = {0.};
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int m = t + 64 * v;
+    const int c = m < N / 2 ? 2 * m : 2 * (N - 1 - m) + 1;
+    const double a = x[v].x * inv, b = x[v].y * inv;
+    out[c] = a;
+    out[N + c] = b;
+    nrm[0] += a * a;
+    nrm[0] += b * b;
+  }
+  wave_sum<1>(nrm);
+  if (t == 63) gp(J.f.site(SITE_DNORM))[L.lt] = nrm[0];
+}
+
+template <int E>
+void launch_reg_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  if (js.total[kind] == 0) return;
+  const size_t lds = RegPass<E>::lds_bytes();
+  static std::atomic<unsigned long long> attr{0};
+  int dev = 0;
+  OTM_HIP(hipGetDevice(&dev));
+  const unsigned long long bit = 1ull << (dev & 63);
+  if (!(attr.load() & bit)) {
+    for (const void* k : {reinterpret_cast<const void*>(&k_fftr_rows_fwd<E>),
+                          reinterpret_cast<const void*>(&k_fftr_cols<E>),
+                          reinterpret_cast<const void*>(&k_fftr_rows_inv<E>)})
+      OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    attr.fetch_or(bit);
+  }
+  const dim3 grid(unsigned(js.total[kind])), block(64);
+  const int* st = js.starts_of(kind);
+  const Job* jobs = js.jobs;
+  switch (pass) {
+    case 0: hipLaunchKernelGGL(k_fftr_rows_fwd<E>, grid, block, lds, s, jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL(k_fftr_cols<E>, grid, block, lds, s, jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL(k_fftr_rows_inv<E>, grid, block, lds, s, jobs, st, js.count, gate);
+  }
+  OTM_HIP(hipGetLastError());
+}
+
 template <int FS>
 void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
   if (js.total[kind] == 0) return;
   int nmax = 0;
   for (int p = 0; p < js.count; ++p)
-    if (js.jroute[p] == 2 && fft_rows(js.host[p].f.n) == 2 * FS) nmax = std::max(nmax, js.host[p].f.n);
+    if (js.jroute[p] == 2 && fft_rows(js.host[p].f.n) == 2 * FS && !fft_reg_e(js.host[p].f.n))
+      nmax = std::max(nmax, js.host[p].f.n);
   const size_t lds = size_t(fft_lds(FS)) * nmax;
   // the kernels use no static LDS: up to fft_lds(FS) times the largest n of
   // the kind dynamic, allowed once per device (bit d of the mask)
@@ -559,6 +906,8 @@ void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream
 // one workgroup per 2 rows) and KIND_FFT2 (one per 4 rows); dynamic LDS
 // sized for the largest grid of each kind.
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
+  launch_reg_kind<8>(js, KIND_FFTR8, pass, gate, s);
+  launch_reg_kind<16>(js, KIND_FFTR16, pass, gate, s);
   launch_pass_kind<1>(js, KIND_FFT, pass, gate, s);
   launch_pass_kind<2>(js, KIND_FFT2, pass, gate, s);
 }

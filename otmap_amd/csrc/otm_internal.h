@@ -188,7 +188,9 @@ enum LaunchKind {
   KIND_FLATMX = 7,     // grid-stride blocks of the jobs on the mixed transform (first right-hand side)
   KIND_FFT = 8,        // fast-DCT passes of the FFT-route jobs with n <= FFT_SPLIT_N: one workgroup per 2 rows
   KIND_FFT2 = 9,       // fast-DCT passes of the FFT-route jobs with n > FFT_SPLIT_N: one workgroup per 4 rows
-  KIND_M0 = 10,
+  KIND_FFTR8 = 10,     // fast-DCT passes in register form, n = 512: one 64-thread workgroup per 2 rows
+  KIND_FFTR16 = 11,    // the same for n = 1024
+  KIND_M0 = 12,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
@@ -237,6 +239,9 @@ constexpr int FFT_MAX_N = 2048;
 // workgroups would no longer fit one CU's LDS.
 constexpr int FFT_SPLIT_N = 1024;
 constexpr int fft_rows(int n) { return n <= FFT_SPLIT_N ? 2 : 4; }
+// Points per lane of the register form of the fast-DCT passes (one wave per
+// pair of rows, fft.hip), 0 for sizes that use the workgroup form.
+constexpr int fft_reg_e(int n) { return n == 512 ? 8 : n == 1024 ? 16 : 0; }
 bool fft_supported(int n);
 // FP64 route of a grid: the fast DCT when asked for and supported, else the GEMMs
 inline int fp64_route(int n, int transform) { return transform == 1 && fft_supported(n) ? 2 : 0; }
@@ -365,8 +370,8 @@ struct Driver {
   hipGraph_t graph[2] = {nullptr, nullptr};     // captured iterations: a long and a one-iteration replay
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   long key = -1;                  // precision mode and tile shapes of the captured graphs
-  int* d_open = nullptr;          // open / far-from-stop job counts after a replay (device, two slots of two)
-  int* h_open = nullptr;          // pinned copies
+  int* h_open = nullptr;          // open / far-from-stop job counts after a replay: two slots of two, mapped pinned host memory
+  int* d_open = nullptr;          // the device address of h_open
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evc[2] = {nullptr, nullptr};  // the copy of each slot is complete
 };
