@@ -6,11 +6,13 @@
 // every job's device state says done (:160-221), then the forward positions
 // (:223-227).  Iterations are captured once into two HIP graphs, one of
 // ITERS_PER_GRAPH iterations and one of a single iteration, and replayed;
-// after each replay one small kernel counts the jobs still running, and those
-// of them still far from their stop, straight into one of two slots of mapped
-// pinned host memory (no copy launch), and the host reads the counts of
-// replay k while replay k + 1 runs (the device never waits for the host).
-// Long replays run while some job is far from its stop;
+// after each replay of a batch one small kernel counts the jobs still
+// running, and those of them still far from their stop, straight into one of
+// two slots of mapped pinned host memory (no copy launch); a single job's
+// scalar kernels write its own two flags there when they change (no count
+// kernel).  The host reads the counts of replay k while replay k + 1 runs
+// (the device never waits for the host).  Long replays run while some job is
+// far from its stop;
 // once every running job is near it (residual within NEAR_FACTOR of the
 // threshold, or within a long replay of max_iter) the host queues single
 // iterations, so at most about two iterations run gated after the last job
@@ -42,11 +44,6 @@ long partial_slots(int n) {
   const long gemm_tiles = long((n + 63) / 64) * ((n + 63) / 64);
   return std::max<long>(std::max<long>(num_tiles(n), gemm_tiles), 3 * 1024);
 }
-
-constexpr int ITERS_PER_GRAPH = 4;
-// near the stop: the residual of the last iterations falls by 5-10% per
-// iteration on the benchmark inputs, so 2x the threshold is ~10 iterations out
-constexpr double NEAR_FACTOR = 2.0;
 
 }  // namespace
 
@@ -131,12 +128,12 @@ void driver_destroy(Driver& d) {
 // GEMMs, fast DCTs, or the mixed transform, whose jobs find their folded
 // right-hand side written by the previous update pass), then the shared
 // beta, line-search and update passes.
-void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s) {
+void enqueue_iteration(const JobSet& js, const DctPlan* plan, const otm_options& opt, hipStream_t s, int* flags) {
   if (js.routed_mixed + js.routed_fft < js.count) launch_dct_solve(js, plan, s);
   if (js.routed_fft > 0) launch_fft_solve(js, true, s);
   if (js.routed_mixed > 0) launch_mixed_solve(js, opt.refine_steps, true, s, false);
-  launch_beta(js, s);
-  launch_dir_coef(js, s);
+  launch_beta(js, s, flags);
+  launch_dir_coef(js, s, flags);
   launch_update(js, s);
 }
 
@@ -147,13 +144,17 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
   if (d.exec[0] && d.key != key) driver_invalidate(d);
   jobset_upload(js, s);
   OTM_HIP(hipEventRecord(d.ev0, s));
-  launch_start(js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, s);
+  // a single job reports through its own flags, a batch through the count kernel
+  const bool direct = js.count == 1;
+  int* flags = direct ? d.d_open : nullptr;
+  launch_start(js, opt.max_ratio, opt.max_iter, opt.threshold, opt.beta == OTM_BETA_CONJUGATE_JACOBIAN ? 1 : 0, s,
+               flags);
   launch_mprep(js, true, s);  // jobs on the mixed transform: right-hand side of the first iteration
   if (!d.exec[0]) {
     for (int g = 0; g < 2; ++g) {
       OTM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        for (int k = 0; k < (g == 0 ? ITERS_PER_GRAPH : 1); ++k) enqueue_iteration(js, plan, opt, s);
+        for (int k = 0; k < (g == 0 ? ITERS_PER_GRAPH : 1); ++k) enqueue_iteration(js, plan, opt, s, flags);
       } catch (...) {
         hipGraph_t gr = nullptr;
         (void)hipStreamEndCapture(s, &gr);
@@ -172,7 +173,7 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
   auto replay = [&](int slot, bool single) {
     OTM_HIP(hipGraphLaunch(d.exec[single ? 1 : 0], s));
     launched += single ? 1 : ITERS_PER_GRAPH;
-    launch_count_open(js, d.d_open + 2 * slot, NEAR_FACTOR, ITERS_PER_GRAPH, s);
+    if (!direct) launch_count_open(js, d.d_open + 2 * slot, s);
     OTM_HIP(hipEventRecord(d.evc[slot], s));
   };
   int slot = 0;
@@ -182,12 +183,13 @@ int driver_solve(Driver& d, JobSet& js, const DctPlan* plan, const otm_options& 
     const bool more = launched <= long(opt.max_iter);
     if (more) replay(slot ^ 1, single);
     OTM_HIP(hipEventSynchronize(d.evc[slot]));
-    if (d.h_open[2 * slot] == 0) break;
+    const volatile int* cnt = d.h_open + (direct ? 0 : 2 * slot);
+    if (cnt[0] == 0) break;
     if (!more) {
       set_error("internal error: solve did not reach its stop condition");
       return OTM_ERR_STATE;
     }
-    single = d.h_open[2 * slot + 1] == 0;
+    single = cnt[1] == 0;
     slot ^= 1;
   }
   launch_finish(js, s);

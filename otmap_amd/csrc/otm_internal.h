@@ -249,19 +249,29 @@ void jobset_upload(JobSet& js, hipStream_t s);  // after changing js.host (same 
 void jobset_destroy(JobSet& js);
 
 // Stencil kernel launchers (stencil.hip), one launch per step for all jobs
-void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s);
+// Iteration driver constants (driver.hip): iterations per long graph replay,
+// and the residual factor over the threshold below which a job is near its
+// stop (single-iteration replays from then on).
+constexpr int ITERS_PER_GRAPH = 4;
+// near the stop: the residual of the last iterations falls by 5-10% per
+// iteration on the benchmark inputs, so 2x the threshold is ~10 iterations out
+constexpr double NEAR_FACTOR = 2.0;
+// flags (one-job tables; null otherwise): the device address of the host's
+// {open, far} pair, which the scalar kernels update when either changes
+void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s,
+                  int* flags = nullptr);
 // k_beta + scalar kernel; the |d_hat|^2 partials of every job were written
 // by the workgroups of the last stage of its transform (KIND_DNORM)
-void launch_beta(const JobSet& js, hipStream_t s);
-void launch_dir_coef(const JobSet& js, hipStream_t s);   // k_dir_coef + scalar kernel (line search)
+void launch_beta(const JobSet& js, hipStream_t s, int* flags = nullptr);
+void launch_dir_coef(const JobSet& js, hipStream_t s, int* flags = nullptr);   // k_dir_coef + scalar kernel (line search)
 // jobs on the mixed transform run the folding update pass, which also writes
 // the folded BF16 planes of their next right-hand side
 void launch_update(const JobSet& js, hipStream_t s);
 void launch_finish(const JobSet& js, hipStream_t s);     // forward positions, adjusted density copy
 // d_open[0] = jobs whose state is not done, d_open[1] = those of them with a
-// residual above near_factor x threshold and more than near_iters iterations
-// left before max_iter
-void launch_count_open(const JobSet& js, int* d_open, double near_factor, int near_iters, hipStream_t s);
+// residual above NEAR_FACTOR x threshold and more than ITERS_PER_GRAPH
+// iterations left before max_iter
+void launch_count_open(const JobSet& js, int* d_open, hipStream_t s);
 void launch_fill(double* p, long count, double v, hipStream_t s);
 // component kernels (single evaluations)
 void launch_vertex_gradients(const double* psi, double* G, int n, hipStream_t s);
@@ -370,7 +380,11 @@ struct Driver {
   hipGraph_t graph[2] = {nullptr, nullptr};     // captured iterations: a long and a one-iteration replay
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   long key = -1;                  // precision mode and tile shapes of the captured graphs
-  int* h_open = nullptr;          // open / far-from-stop job counts after a replay: two slots of two, mapped pinned host memory
+  // open / far-from-stop job counts, mapped pinned host memory: two slots of
+  // two written by the count kernel after each replay of a batch; for a
+  // one-job table the job's own {open, far} flags in slot 0, written by its
+  // scalar kernels
+  int* h_open = nullptr;
   int* d_open = nullptr;          // the device address of h_open
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evc[2] = {nullptr, nullptr};  // the copy of each slot is complete

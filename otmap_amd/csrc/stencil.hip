@@ -360,8 +360,18 @@ __global__ __launch_bounds__(NT) void k_initial_residual(JOB_ARGS) {
   write_partials_at<2>(acc, f.site(SITE_RESID), L.t, L.nb);
 }
 
-// solver state initialisation (otsolver_2dgrid.cpp:126-160), one workgroup per job
-__global__ __launch_bounds__(NT) void k_scalar_init(JOB_ARGS, int max_iter, double threshold, int cj) {
+// A job still far from its stop: residual above NEAR_FACTOR x threshold and
+// more than one long replay of iterations below max_iter (the host queues
+// long replays while some job is far, single iterations after that).
+__device__ inline bool far_from_stop(double e, double threshold, int it, int max_iter) {
+  return e > NEAR_FACTOR * threshold && it + ITERS_PER_GRAPH < max_iter;
+}
+
+// solver state initialisation (otsolver_2dgrid.cpp:126-160), one workgroup per job.
+// flags (one-job tables, else null): the host's {open, far} pair in mapped
+// pinned memory; the scalar kernels write it where either changes, so the
+// host reads the job's progress without a count kernel per replay.
+__global__ __launch_bounds__(NT) void k_scalar_init(JOB_ARGS, int max_iter, double threshold, int cj, int* flags) {
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   double acc[2];
@@ -385,6 +395,10 @@ __global__ __launch_bounds__(NT) void k_scalar_init(JOB_ARGS, int max_iter, doub
     const bool run = (0 < max_iter) && (e0 > threshold);
     st->done = run ? 0 : 1;
     st->exit_reason = run ? 0 : 1;
+    if (flags) {
+      flags[0] = run ? 1 : 0;
+      flags[1] = run && far_from_stop(e0, threshold, 0, max_iter) ? 1 : 0;
+    }
   }
 }
 
@@ -462,7 +476,7 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
 // then beta = max(-1, Jd.(r_eps - r_k) / |Jd|^2 / eps * alpha) and the choice
 // d = d_hat + beta d_prev (it >= 1, conjugate-Jacobian) or d = d_hat.
 // dstarts: workgroup prefix of the launch that wrote the |d_hat|^2 partials.
-__global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restrict__ dstarts) {
+__global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restrict__ dstarts, int* flags) {
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   const auto st = gp(f.state);
@@ -488,6 +502,7 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restr
   if (dnorm <= 2 * DBL_MIN) {
     st->done = 1;
     st->exit_reason = 2;
+    if (flags) flags[0] = 0;
     return;
   }
   const bool conj = it >= 1 && cj;
@@ -618,7 +633,7 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
 // pass 2 (one workgroup per job): z = [e A, 2 b.r, b.b + 2 a.r, 2 a.b, a.a], the
 // step alpha, e(alpha)/A, traces, the iteration count and the stop test
 // (otsolver_2dgrid.cpp:556-605, :160); sum r_{k+1} = alpha^2 sum a + alpha sum b + sum r_k.
-__global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
+__global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS, int* flags) {
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   const auto st = gp(f.state);
@@ -652,9 +667,16 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS) {
     const int it = k + 1;
     st->it = it;
     st->update = 1;
-    if (!(it < max_iter && e > threshold)) {
+    const bool stop = !(it < max_iter && e > threshold);
+    if (stop) {
       st->done = 1;
       st->exit_reason = 1;
+    }
+    if (flags) {
+      if (stop) flags[0] = 0;
+      const bool far_prev = far_from_stop(e_prev, threshold, k, max_iter);
+      const bool far_now = !stop && far_from_stop(e, threshold, it, max_iter);
+      if (far_now != far_prev) flags[1] = far_now ? 1 : 0;
     }
   }
 }
@@ -887,11 +909,8 @@ __global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
 }
 
 // number of jobs whose state is not done (one workgroup)
-// out[0]: jobs still running; out[1]: those of them that may run more than
-// `near_iters` further iterations (residual above near_factor x threshold
-// and more than near_iters iterations below max_iter).
-__global__ __launch_bounds__(NT) void k_count_open(const Job* __restrict__ jobs, int count, int* out,
-                                                   double near_factor, int near_iters) {
+// out[0]: jobs still running; out[1]: those of them still far from their stop.
+__global__ __launch_bounds__(NT) void k_count_open(const Job* __restrict__ jobs, int count, int* out) {
   __shared__ int total[2];
   if (threadIdx.x < 2) total[threadIdx.x] = 0;
   __syncthreads();
@@ -900,7 +919,7 @@ __global__ __launch_bounds__(NT) void k_count_open(const Job* __restrict__ jobs,
     const auto st = gp((const SolveState*)jobs[p].f.state);
     if (st->done) continue;
     ++c;
-    far += (st->e > near_factor * st->threshold && st->it + near_iters < st->max_iter) ? 1 : 0;
+    far += far_from_stop(st->e, st->threshold, st->it, st->max_iter) ? 1 : 0;
   }
   if (c) atomicAdd(&total[0], c);
   if (far) atomicAdd(&total[1], far);
@@ -1033,7 +1052,8 @@ inline dim3 grid_of(const JobSet& js, int kind) { return dim3(unsigned(js.total[
 }  // namespace
 
 // density adjustment, field initialisation, initial residual and solver state
-void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s) {
+void launch_start(const JobSet& js, double max_ratio, int max_iter, double threshold, int cj, hipStream_t s,
+                  int* flags) {
   hipLaunchKernelGGL(k_adjust_reduce, grid_of(js, KIND_FLAT), dim3(NT), 0, s, JOB_PASS(KIND_FLAT));
   OTM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_adjust_scalar, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_FLAT), max_ratio);
@@ -1044,24 +1064,26 @@ void launch_start(const JobSet& js, double max_ratio, int max_iter, double thres
   OTM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_initial_residual, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_init, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL), max_iter, threshold, cj);
+  hipLaunchKernelGGL(k_scalar_init, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL), max_iter, threshold, cj,
+                     js.count == 1 ? flags : nullptr);
   OTM_HIP(hipGetLastError());
 }
 
 // the |d_hat|^2 partials of each job were written by the workgroups of the
 // last stage of its transform (KIND_DNORM)
-void launch_beta(const JobSet& js, hipStream_t s) {
+void launch_beta(const JobSet& js, hipStream_t s, int* flags) {
   hipLaunchKernelGGL(k_beta, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scalar_beta, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
-                     (const int*)js.starts_of(KIND_DNORM));
+                     (const int*)js.starts_of(KIND_DNORM), js.count == 1 ? flags : nullptr);
   OTM_HIP(hipGetLastError());
 }
 
-void launch_dir_coef(const JobSet& js, hipStream_t s) {
+void launch_dir_coef(const JobSet& js, hipStream_t s, int* flags) {
   hipLaunchKernelGGL(k_dir_coef, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_step, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
+  hipLaunchKernelGGL(k_scalar_step, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
+                     js.count == 1 ? flags : nullptr);
   OTM_HIP(hipGetLastError());
 }
 
@@ -1085,9 +1107,8 @@ void launch_finish(const JobSet& js, hipStream_t s) {
   OTM_HIP(hipGetLastError());
 }
 
-void launch_count_open(const JobSet& js, int* d_open, double near_factor, int near_iters, hipStream_t s) {
-  hipLaunchKernelGGL(k_count_open, dim3(1), dim3(NT), 0, s, (const Job*)js.jobs, js.count, d_open, near_factor,
-                     near_iters);
+void launch_count_open(const JobSet& js, int* d_open, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_open, dim3(1), dim3(NT), 0, s, (const Job*)js.jobs, js.count, d_open);
   OTM_HIP(hipGetLastError());
 }
 
