@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 GEMM_TRAFFIC = "profiles/gemm_1024_hbm_traffic.md"
 FFT_TRAFFIC = "profiles/fft_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
-    "k_fft_cols": ((4394.0 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
+    "k_fft_cols": ((4478.0 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
     "k_mgemm<M2,32>": ((9288.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
     "k_mgemm<M2,64>": ((9294.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
     **{f"k_gemm_fold<F1,cfg{c}>": ((f * 2.0 + 8192.0) * 1024, GEMM_TRAFFIC)
@@ -535,7 +535,10 @@ def main():
     if fft:
         t_ms = kernel_ms["fft_cols"]
         achieved = 16.0 * n * n / (t_ms * 1e-3) / 1e9
-        roof_fft = {"kernel": "k_fft_cols: fast-DCT column pass (DCT-II, / lambda, DCT-III of n columns), "
+        # the column pass runs in register form for n = 512 and 1024 (one wave
+        # per pair of columns), in the workgroup form at other sizes
+        cname = {512: "k_fftr_cols<8>", 1024: "k_fftr_cols<16>"}.get(n, "k_fft_cols<1>" if n <= 1024 else "k_fft_cols<2>")
+        roof_fft = {"kernel": f"{cname}: fast-DCT column pass (DCT-II, / lambda, DCT-III of n columns), "
                               "one read + one write of the n x n FP64 field",
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(achieved / PEAK_HBM_GBS, 4), **traffic_of("k_fft_cols"),
