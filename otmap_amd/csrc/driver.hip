@@ -53,7 +53,7 @@ void fields_create(Fields& f, int n, int trace_cap) {
   f.F = long(n) * n;
   f.V = long(n + 1) * (n + 1);
   const size_t F = f.F, V = f.V;
-  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.g0, 4 * V); dalloc(f.r, F); dalloc(f.jd, F);
+  dalloc(f.rho, F); dalloc(f.psi, F); dalloc(f.psi2, F); dalloc(f.g0, 2 * V); dalloc(f.r, F); dalloc(f.jd, F);
   dalloc(f.dhat, F); dalloc(f.d[0], F); dalloc(f.d[1], F); dalloc(f.dpair, 2);
   f.pslots = partial_slots(n);
   dalloc(f.a, F); dalloc(f.b, F); dalloc(f.partials, size_t(SITE_COUNT) * SITE_VALUES * size_t(f.pslots));
@@ -65,7 +65,7 @@ void fields_create(Fields& f, int n, int trace_cap) {
 }
 
 void fields_destroy(Fields& f) {
-  dfree(f.rho); dfree(f.psi); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
+  dfree(f.rho); dfree(f.psi); dfree(f.psi2); dfree(f.g0); dfree(f.r); dfree(f.jd); dfree(f.dhat);
   dfree(f.d[0]); dfree(f.d[1]); dfree(f.dpair); dfree(f.a); dfree(f.b); dfree(f.partials);
   dfree(f.tmp0); dfree(f.tmp1); dfree(f.state); dfree(f.trace);
   dfree(f.mrhs); dfree(f.mwa); dfree(f.mwb); dfree(f.x0);

@@ -88,8 +88,10 @@ struct Fields {
   int n = 0;
   long F = 0, V = 0;
   double* rho = nullptr;     // adjusted density (F)
-  double* psi = nullptr;     // potential (F)
-  double* g0 = nullptr;      // cached vertex gradients of psi: two buffers of 2V (SoA), current = it & 1
+  double* psi = nullptr;     // potential (F); holds the potential between solves
+  double* psi2 = nullptr;    // second buffer: iteration k reads psi_k from psi_buf(k), the update writes
+                             // psi_{k+1} to psi_buf(k + 1) (neighbouring tiles read a tile's faces as their halo)
+  double* g0 = nullptr;      // 2V (SoA): input staging of otm_line_search_coefficients (a solve forms grad psi from psi)
   double* r = nullptr;       // residual r_k (F), updated in place
   double* jd = nullptr;      // r_k - r_{k-1} (F)
   double* dhat = nullptr;    // Laplacian-solve direction (F)
@@ -112,6 +114,7 @@ struct Fields {
   unsigned short* mwb = nullptr;
   double* x0 = nullptr;
   __host__ __device__ double* site(int s) const { return partials + long(s) * SITE_VALUES * pslots; }
+  __host__ __device__ double* psi_buf(int k) const { return (k & 1) ? psi2 : psi; }
 };
 
 // ---------------------------------------------------------------------------

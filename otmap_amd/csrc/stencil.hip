@@ -111,6 +111,38 @@ __device__ inline void tile_grads(const double* s, double* gx, double* gy, int n
   }
 }
 
+// Vertex gradients of two face fields whose halos sa, sb sit in LDS, into
+// the four arrays gx, gy (of sa) and hx, hy (of sb) over the same LDS: the
+// gradients go through registers, so the halos may lie under the arrays.
+// Starts after a barrier that made both halos visible; ends with one.
+__device__ inline void two_tile_grads(const double* sa, const double* sb, double* gx, double* gy, double* hx,
+                                      double* hy, int n, int i0, int j0) {
+  double gv[VR][4];
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    const int a = t / LDV, b = t - a * LDV;
+    const int vi = i0 + a, vj = j0 + b;
+    gv[u][0] = gv[u][1] = gv[u][2] = gv[u][3] = 0.;
+    if (t < NV && vi <= n && vj <= n) {
+      vertex_grad(sa, n, a, b, vi, vj, gv[u][0], gv[u][1]);
+      vertex_grad(sb, n, a, b, vi, vj, gv[u][2], gv[u][3]);
+    }
+  }
+  __syncthreads();  // every halo read is done: the arrays take their place
+#pragma unroll
+  for (int u = 0; u < VR; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (t < NV) {
+      gx[t] = gv[u][0];
+      gy[t] = gv[u][1];
+      hx[t] = gv[u][2];
+      hy[t] = gv[u][3];
+    }
+  }
+  __syncthreads();
+}
+
 // half the cross product of the quad diagonals, tile-local face (a, b)
 __device__ inline double quad_area(const double* gx, const double* gy, int a, int b, double e) {
   const int v00 = a * LDV + b, v10 = (a + 1) * LDV + b, v01 = a * LDV + b + 1, v11 = (a + 1) * LDV + b + 1;
@@ -241,18 +273,19 @@ __global__ __launch_bounds__(NT) void k_fill(double* p, long count, double v) {
   for (long i = long(blockIdx.x) * NT + threadIdx.x; i < count; i += long(gridDim.x) * NT) p[i] = v;
 }
 
-// psi = 0, both G0 buffers = 0, both search directions = 0 (otsolver_2dgrid.cpp:126-140)
+// psi = 0, both search directions = 0 (otsolver_2dgrid.cpp:126-140); the
+// cached vertex gradients of psi (zero at psi = 0) are not stored: the
+// passes that use them form grad psi from psi
 __global__ __launch_bounds__(NT) void k_init_fields(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
   const long stride = long(L.nb) * NT;
-  const auto psi = gp(f.psi), d0 = gp(f.d[0]), d1 = gp(f.d[1]), g0 = gp(f.g0);
+  const auto psi = gp(f.psi), d0 = gp(f.d[0]), d1 = gp(f.d[1]);
   for (long i = long(L.t) * NT + threadIdx.x; i < f.F; i += stride) {
     psi[i] = 0.;
     d0[i] = 0.;
     d1[i] = 0.;
   }
-  for (long i = long(L.t) * NT + threadIdx.x; i < 4 * f.V; i += stride) g0[i] = 0.;
 }
 
 // density adjustment (otsolver_2dgrid.cpp:47-73), pass 1: sum / min / max
@@ -413,7 +446,7 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
   const bool early = count == 1;
   if (!early && (st->done || st->it < 1 || !st->cj)) return;
   const int n = f.n;
-  const auto psi = gp(f.psi);
+  const auto psi = gp(f.psi_buf(st->it));
   const auto dhat = gp(f.dhat);
   const auto rho = gp(f.rho), r = gp(f.r), jd = gp(f.jd);
   __shared__ double gx[NV], gy[NV];
@@ -527,21 +560,25 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const auto st = gp((const SolveState*)f.state);
   if (st->done) return;
   const int n = f.n;
-  // LDS: direction gradients gx, gy and cached G0 (hx, hy); the direction
-  // halo s lives where hx, hy go once the gradients are formed (18.7 KB per
-  // workgroup: eight resident per CU, every tile of a 1024^2 grid at once)
+  // LDS (18.7 KB per workgroup: eight resident per CU, every tile of a
+  // 1024^2 grid at once): the halos of psi (sp) and of the direction (s),
+  // then, over them, the vertex gradients of the direction (gx, gy) and of
+  // psi (hx, hy; the reference's cached G0, otsolver_2dgrid.cpp:473-478,
+  // which equals grad psi: it starts at 0 with psi = 0 and takes the same
+  // alpha-steps)
   __shared__ double lds[4 * NV];
-  static_assert(NF <= 2 * NV, "the halo fits in the G0 region");
+  static_assert(NF <= 2 * NV, "a halo fits under two gradient arrays");
   double* const gx = lds;
   double* const gy = lds + NV;
   double* const hx = lds + 2 * NV;
   double* const hy = lds + 3 * NV;
+  double* const sp = gx;
   double* const s = hx;
   const int k = st->it;
   const auto dhat = gp(f.dhat);
   const auto dprev = gp(f.d[k & 1]);
   const auto dnew = gp(f.d[(k + 1) & 1]);
-  const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
+  const auto psi = gp(f.psi_buf(k));
   const auto r = gp(f.r);
   const bool use_prev = st->use_prev != 0;
   const double beta = st->beta;
@@ -549,9 +586,8 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   tile_origin(L.t, n, i0, j0);
   // all global loads of the tile first: direction halo (d_hat and d_prev
   // kept apart, combined at the LDS store, so no arithmetic waits on a load
-  // before the others are issued), cached G0 of the tile's vertices,
-  // residual of its faces
-  double da[HR], db[HR];
+  // before the others are issued), psi halo, residual of its faces
+  double da[HR], db[HR], pa[HR];
   bool hin[HR];
 #pragma unroll
   for (int u = 0; u < HR; ++u) {
@@ -562,18 +598,7 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
     const long fc = long(fi) * n + fj;
     da[u] = hin[u] ? dhat[fc] : 0.;
     db[u] = (hin[u] && use_prev) ? dprev[fc] : 0.;
-  }
-  const long V1 = n + 1;
-  const long V = V1 * V1;
-  double hxv[VR], hyv[VR];
-#pragma unroll
-  for (int u = 0; u < VR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    const int a = t / LDV, b = t - a * LDV;
-    const int vi = i0 + a, vj = j0 + b;
-    const bool in = t < NV && vi <= n && vj <= n;
-    hxv[u] = in ? g0[vi * V1 + vj] : 0.;
-    hyv[u] = in ? g0[V + vi * V1 + vj] : 0.;
+    pa[u] = hin[u] ? psi[fc] : 0.;
   }
   double rvv[FR];
 #pragma unroll
@@ -585,26 +610,19 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
 #pragma unroll
   for (int u = 0; u < HR; ++u) {
     const int t = threadIdx.x + u * NT;
-    if (t < NF) s[t] = !hin[u] ? 0. : use_prev ? da[u] + beta * db[u] : da[u];
+    if (t < NF) {
+      s[t] = !hin[u] ? 0. : use_prev ? da[u] + beta * db[u] : da[u];
+      sp[t] = pa[u];
+    }
   }
   __syncthreads();
-  tile_grads(s, gx, gy, n, i0, j0);
   double dv[FR];  // direction at the tile's own faces
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
     const int t = threadIdx.x + u * NT;
     dv[u] = s[(t / TJ + 1) * LDF + t % TJ + 1];
   }
-  __syncthreads();  // the halo is dead: G0 takes its place
-#pragma unroll
-  for (int u = 0; u < VR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    if (t < NV) {
-      hx[t] = hxv[u];
-      hy[t] = hyv[u];
-    }
-  }
-  __syncthreads();
+  two_tile_grads(s, sp, gx, gy, hx, hy, n, i0, j0);
   const double e = 1. / double(n);
   double acc[7] = {0., 0., 0., 0., 0., 0., 0.};
 #pragma unroll
@@ -681,100 +699,64 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS, int* flags) {
   }
 }
 
-// Inputs of the update of one tile, loaded into registers: direction halo,
-// G0 of the tile's vertices, psi / r of its faces (the update overwrites
-// psi and r, and writes G0 to the other buffer, so a workgroup may load the
-// inputs of its next tile before it updates the current one).
+// Inputs of the update of one tile, loaded into registers: the direction's
+// and psi's halos, r of its faces (the update overwrites psi and r, so a
+// workgroup may load the inputs of its next tile before it updates the
+// current one).
 struct UpdateIn {
-  double hv[HR];
-  double hxv[VR], hyv[VR];
-  double psv[FR], rkv[FR];
+  double hv[HR];   // direction halo
+  double hp[HR];   // psi halo
+  double rkv[FR];  // r of the tile's faces
 };
 
 __device__ inline void update_load(const Fields& f, int k, int i0, int j0, UpdateIn& R) {
   const int n = f.n;
   const auto d = gp(f.d[(k + 1) & 1]);
-  const auto g0 = gp(f.g0 + (k & 1) * 2 * f.V);
-  const auto psi = gp(f.psi), r = gp(f.r);
+  const auto psi = gp(f.psi_buf(k)), r = gp(f.r);
   halo_load(R.hv, n, i0, j0, [&](long fc) { return d[fc]; });
-  const long V1 = n + 1;
-  const long V = V1 * V1;
-#pragma unroll
-  for (int u = 0; u < VR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    const int va = t / LDV, vb = t - va * LDV;
-    const int vi = i0 + va, vj = j0 + vb;
-    const bool in = t < NV && vi <= n && vj <= n;
-    R.hxv[u] = in ? g0[vi * V1 + vj] : 0.;
-    R.hyv[u] = in ? g0[V + vi * V1 + vj] : 0.;
-  }
+  halo_load(R.hp, n, i0, j0, [&](long fc) { return psi[fc]; });
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
     const int t = threadIdx.x + u * NT;
     const int fi = i0 + t / TJ, fj = j0 + t % TJ;
     const bool in = fi < n && fj < n;
-    const long fc = long(fi) * n + fj;
-    R.psv[u] = in ? psi[fc] : 0.;
-    R.rkv[u] = in ? r[fc] : 0.;
+    R.rkv[u] = in ? r[long(fi) * n + fj] : 0.;
   }
 }
 
 // line-search update of the tile at (i0, j0) from its loaded inputs: psi +=
-// alpha d, g0 += alpha grad(d), r <- a alpha^2 + b alpha + r
-// (otsolver_2dgrid.cpp:599-604) and Jd = r_{k+1} - r_k for the next beta;
-// k = completed iterations - 1.  The quartic coefficients a, b come from d
-// and G0 before its update, as in the line-search pass.  rn: the new residual
-// of this thread's faces (0 outside).  Starts by writing the workgroup's LDS:
-// a second call needs a barrier after the first.
+// alpha d, r <- a alpha^2 + b alpha + r (otsolver_2dgrid.cpp:599-604) and
+// Jd = r_{k+1} - r_k for the next beta; k = completed iterations - 1.  The
+// quartic coefficients a, b come from d and grad psi before the update, as
+// in the line-search pass (the reference's g0 += alpha grad d is the
+// gradient of the updated psi, which the next passes take from psi).
+// rn: the new residual of this thread's faces (0 outside).  Starts by
+// writing the workgroup's LDS: a second call needs a barrier after the first.
 __device__ inline void update_apply(const Fields& f, int k, double alpha, int i0, int j0, const UpdateIn& R,
                                     double (&rn)[FR]) {
   const int n = f.n;
-  // LDS as in the line-search pass: gx, gy, hx, hy, the halo s in hx, hy's place
+  // LDS as in the line-search pass: the halos of psi (sp) and of the
+  // direction (s) under the gradients gx, gy (direction) and hx, hy (psi)
   __shared__ double lds[4 * NV];
   double* const gx = lds;
   double* const gy = lds + NV;
   double* const hx = lds + 2 * NV;
   double* const hy = lds + 3 * NV;
+  double* const sp = gx;
   double* const s = hx;
-  // G0 ping-pong: neighbouring tiles read the border vertices this tile
-  // updates, so the update goes to the other buffer
-  const auto g0n = gp(f.g0 + ((k + 1) & 1) * 2 * f.V);
-  const auto psi = gp(f.psi), r = gp(f.r), jd = gp(f.jd);
-  const long V1 = n + 1;
-  const long V = V1 * V1;
+  const auto psin = gp(f.psi_buf(k + 1)), r = gp(f.r), jd = gp(f.jd);
   halo_store(s, R.hv);
+  halo_store(sp, R.hp);
   __syncthreads();
-  tile_grads(s, gx, gy, n, i0, j0);
-  double dv[FR];  // direction at the tile's own faces
+  double dv[FR], pv[FR];  // direction and psi at the tile's own faces
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
     const int t = threadIdx.x + u * NT;
-    dv[u] = s[(t / TJ + 1) * LDF + t % TJ + 1];
+    const int h = (t / TJ + 1) * LDF + t % TJ + 1;
+    dv[u] = s[h];
+    pv[u] = sp[h];
   }
-  __syncthreads();  // the halo is dead: G0 takes its place
-#pragma unroll
-  for (int u = 0; u < VR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    if (t < NV) {
-      hx[t] = R.hxv[u];
-      hy[t] = R.hyv[u];
-    }
-  }
-  __syncthreads();
-  // vertices owned by this tile: [i0, i0+TI) x [j0, j0+TJ), plus the last row/column of the grid
-#pragma unroll
-  for (int u = 0; u < VR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    const int va = t / LDV, vb = t - va * LDV;
-    const int vi = i0 + va, vj = j0 + vb;
-    const bool in = t < NV && vi <= n && vj <= n;
-    const bool own = in && ((va < TI) || (vi == n)) && ((vb < TJ) || (vj == n));
-    if (own) {
-      const long vid = vi * V1 + vj;
-      g0n[vid] = R.hxv[u] + alpha * gx[t];
-      g0n[V + vid] = R.hyv[u] + alpha * gy[t];
-    }
-  }
+  two_tile_grads(s, sp, gx, gy, hx, hy, n, i0, j0);
   const double e = 1. / double(n);
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
@@ -784,7 +766,7 @@ __device__ inline void update_apply(const Fields& f, int k, double alpha, int i0
     rn[u] = 0.;
     if (fi < n && fj < n) {
       const long fc = long(fi) * n + fj;
-      psi[fc] = R.psv[u] + alpha * dv[u];
+      psin[fc] = pv[u] + alpha * dv[u];
       double av, bv;
       quartic_ab(gx, gy, hx, hy, fa, fb, e, av, bv);
       const double rk = R.rkv[u];
@@ -877,12 +859,15 @@ __global__ __launch_bounds__(NT) void k_update_fold(JOB_ARGS) {
 }
 
 // forward positions = origin point double(i)*(1/n) + vertex gradient of psi
-// (otsolver_2dgrid.cpp:223-227)
+// (otsolver_2dgrid.cpp:223-227); the final potential, in the buffer of the
+// last completed iteration, goes back to its home buffer f.psi
 __global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Job& J = jobs[L.p];
   const int n = J.f.n;
-  const auto psi = gp(J.f.psi);
+  const auto st = gp((const SolveState*)J.f.state);
+  const double* fin = J.f.psi_buf(st->it);
+  const auto psi = gp(fin);
   const auto G = gp(J.fwd_out);
   __shared__ double gx[NV], gy[NV];
   __shared__ double s[NF];
@@ -890,6 +875,14 @@ __global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
   tile_origin(L.t, n, i0, j0);
   load_halo(s, n, i0, j0, [&](long fc) { return psi[fc]; });
   __syncthreads();
+  if (fin != J.f.psi) {
+    const auto home = gp(J.f.psi);
+    for (int t = threadIdx.x; t < TI * TJ; t += NT) {
+      const int a = t / TJ, b = t % TJ;
+      const int fi = i0 + a, fj = j0 + b;
+      if (fi < n && fj < n) home[long(fi) * n + fj] = s[(a + 1) * LDF + b + 1];
+    }
+  }
   tile_grads(s, gx, gy, n, i0, j0);
   __syncthreads();
   const long V1 = n + 1;
