@@ -435,8 +435,15 @@ __global__ __launch_bounds__(NT) void k_scalar_init(JOB_ARGS, int max_iter, doub
   }
 }
 
-// conjugate-Jacobian beta, pass 1: residual at psi - eps*d_hat (eps = alpha/2)
-// and the partials of Jd.(r_eps - r_k), Jd.Jd   (otsolver_2dgrid.cpp:248-261)
+// conjugate-Jacobian beta, pass 1: the partials of Jd.(r_eps - r_k) and
+// Jd.Jd (otsolver_2dgrid.cpp:248-261), r_eps the residual at psi - eps d_hat
+// (eps = alpha/2).  The area is quadratic in the vertex gradients, which are
+// linear in the potential, so with the quartic coefficients (a, b) of d_hat
+// at psi (the line search's, otsolver_2dgrid.cpp:467-539)
+// r(psi + t d_hat) - r(psi) = b t + a t^2 exactly, and r_eps - r_k is that at
+// t = -eps: the pass reads psi's and d_hat's halos and Jd, not rho and r.
+// (The reference evaluates r_eps afresh and subtracts the running r_k; the
+// two differ by the running residual's rounding drift.)
 __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
@@ -448,57 +455,56 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
   const int n = f.n;
   const auto psi = gp(f.psi_buf(st->it));
   const auto dhat = gp(f.dhat);
-  const auto rho = gp(f.rho), r = gp(f.r), jd = gp(f.jd);
-  __shared__ double gx[NV], gy[NV];
-  __shared__ double s[NF];
+  const auto jd = gp(f.jd);
+  // LDS as in the line-search pass: the halos of psi (sp) and d_hat (s)
+  // under the gradients gx, gy (d_hat) and hx, hy (psi)
+  __shared__ double lds[4 * NV];
+  double* const gx = lds;
+  double* const gy = lds + NV;
+  double* const hx = lds + 2 * NV;
+  double* const hy = lds + 3 * NV;
+  double* const sp = gx;
+  double* const s = hx;
   const double eps = st->alpha / 2.;
   int i0, j0;
   tile_origin(L.t, n, i0, j0);
-  // all global loads of the tile first: halo (psi and d_hat kept apart,
-  // combined at the LDS store), then the per-face inputs
+  // all global loads of the tile first: the two halos, then Jd of its faces
   double pa[HR], pb[HR];
-  bool hin[HR];
 #pragma unroll
   for (int u = 0; u < HR; ++u) {
     const int t = threadIdx.x + u * NT;
     const int a = t / LDF, b = t - a * LDF;
     const int fi = i0 - 1 + a, fj = j0 - 1 + b;
-    hin[u] = t < NF && fi >= 0 && fi < n && fj >= 0 && fj < n;
+    const bool in = t < NF && fi >= 0 && fi < n && fj >= 0 && fj < n;
     const long fc = long(fi) * n + fj;
-    pa[u] = hin[u] ? psi[fc] : 0.;
-    pb[u] = hin[u] ? dhat[fc] : 0.;
+    pa[u] = in ? psi[fc] : 0.;
+    pb[u] = in ? dhat[fc] : 0.;
   }
-  double rhov[FR], jdv[FR], rv[FR];
+  double jdv[FR];
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
     const int t = threadIdx.x + u * NT;
     const int fi = i0 + t / TJ, fj = j0 + t % TJ;
-    const bool in = fi < n && fj < n;
-    const long fc = long(fi) * n + fj;
-    rhov[u] = in ? rho[fc] : 0.;
-    jdv[u] = in ? jd[fc] : 0.;
-    rv[u] = in ? r[fc] : 0.;
+    jdv[u] = (fi < n && fj < n) ? jd[long(fi) * n + fj] : 0.;
   }
   if (early && (st->done || st->it < 1 || !st->cj)) return;
-#pragma unroll
-  for (int u = 0; u < HR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    if (t < NF) s[t] = hin[u] ? pa[u] - eps * pb[u] : 0.;
-  }
+  halo_store(s, pb);
+  halo_store(sp, pa);
   __syncthreads();
-  tile_grads(s, gx, gy, n, i0, j0);
-  __syncthreads();
-  const double e = 1. / double(n), A = 1.0 / (double(n) * double(n));
+  two_tile_grads(s, sp, gx, gy, hx, hy, n, i0, j0);
+  const double e = 1. / double(n);
+  const double t = -eps;
   double acc[2] = {0., 0.};
 #pragma unroll
   for (int u = 0; u < FR; ++u) {
-    const int t = threadIdx.x + u * NT;
-    const int a = t / TJ, b = t % TJ;
+    const int q = threadIdx.x + u * NT;
+    const int a = q / TJ, b = q % TJ;
     const int fi = i0 + a, fj = j0 + b;
     if (fi < n && fj < n) {
-      const double reps = quad_area(gx, gy, a, b, e) - A * rhov[u];
+      double av, bv;
+      quartic_ab(gx, gy, hx, hy, a, b, e, av, bv);
       const double j = jdv[u];
-      acc[0] += j * (reps - rv[u]);
+      acc[0] += j * (av * (t * t) + bv * t);  // Jd (r_eps - r_k)
       acc[1] += j * j;
     }
   }
