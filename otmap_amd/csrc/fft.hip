@@ -142,6 +142,40 @@ __device__ inline void dft8(const cd (&a)[RA], cd (&c)[RA]) {
 // has no twiddles; later stages multiply every input by its twiddle, also
 // w^0 = (1, 0), which is exact, so the butterfly has no branches and all its
 // LDS reads are issued together.
+template <int R, int SIGN>
+__device__ inline void fft_butterfly(const cd* __restrict__ s, cd* __restrict__ dst, const cd* __restrict__ tw, int NR,
+                                     int L, int b, int j, int m) {
+  cd a[R], w[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[r] = s[b + r * NR];
+  if (L > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[r] = tw[(L - 1) + (r - 1) * L + j];
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], SIGN < 0 ? w[r] : conjc(w[r]));
+  }
+  cd c[R];
+  if constexpr (R == 2) {
+    c[0] = cadd(a[0], a[1]);
+    c[1] = csub(a[0], a[1]);
+  } else if constexpr (R == 4) {
+    dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
+  } else if constexpr (R == 8) {
+    dft8<SIGN>(a, c);
+  } else {  // R == 3: w3 = e^{SIGN 2 pi i / 3} = (-1/2, SIGN sqrt(3)/2)
+    const double h3 = 0.86602540378443864676;  // sqrt(3) / 2
+    const cd s12 = cadd(a[1], a[2]), d12 = csub(a[1], a[2]);
+    const cd t = {a[0].x - 0.5 * s12.x, a[0].y - 0.5 * s12.y};
+    const cd u = mul_i(cd{h3 * d12.x, h3 * d12.y}, SIGN);  // SIGN i sqrt(3)/2 (a1 - a2)
+    c[0] = cadd(a[0], s12);
+    c[1] = cadd(t, u);
+    c[2] = csub(t, u);
+  }
+  cd* d = dst + m * L * R + j;
+#pragma unroll
+  for (int r = 0; r < R; ++r) d[r * L] = c[r];
+}
+
 template <int FS, int R, int SIGN>
 __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
                                  int L) {
@@ -153,37 +187,46 @@ __device__ inline void fft_stage(const cd* __restrict__ src, cd* __restrict__ ds
     const int q = FS == 1 ? 0 : b2 >= NR ? 1 : 0;
     const int b = b2 - q * NR;
     const int j = pow2 ? (b & (L - 1)) : b % L, m = pow2 ? (b >> lshift) : b / L;
-    const cd* s = src + q * N;
-    cd a[R], w[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) a[r] = s[b + r * NR];
-    if (L > 1) {
-#pragma unroll
-      for (int r = 1; r < R; ++r) w[r] = tw[(L - 1) + (r - 1) * L + j];
-#pragma unroll
-      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], SIGN < 0 ? w[r] : conjc(w[r]));
-    }
-    cd c[R];
-    if constexpr (R == 2) {
-      c[0] = cadd(a[0], a[1]);
-      c[1] = csub(a[0], a[1]);
-    } else if constexpr (R == 4) {
-      dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
-    } else if constexpr (R == 8) {
-      dft8<SIGN>(a, c);
-    } else {  // R == 3: w3 = e^{SIGN 2 pi i / 3} = (-1/2, SIGN sqrt(3)/2)
-      const double h3 = 0.86602540378443864676;  // sqrt(3) / 2
-      const cd s12 = cadd(a[1], a[2]), d12 = csub(a[1], a[2]);
-      const cd t = {a[0].x - 0.5 * s12.x, a[0].y - 0.5 * s12.y};
-      const cd u = mul_i(cd{h3 * d12.x, h3 * d12.y}, SIGN);  // SIGN i sqrt(3)/2 (a1 - a2)
-      c[0] = cadd(a[0], s12);
-      c[1] = cadd(t, u);
-      c[2] = csub(t, u);
-    }
-    cd* d = dst + q * N + m * L * R + j;
-#pragma unroll
-    for (int r = 0; r < R; ++r) d[r * L] = c[r];
+    fft_butterfly<R, SIGN>(src + q * N, dst + q * N, tw, NR, L, b, j, m);
   }
+}
+
+// The same stage in the packed form: the thread's own sequence (src, dst
+// already offset to it), butterflies tt, tt + tps, ...
+template <int R, int SIGN>
+__device__ inline void fft_stage_p(const cd* __restrict__ src, cd* __restrict__ dst, const cd* __restrict__ tw, int N,
+                                   int L, int tt, int tps) {
+  const int NR = N / R;
+  const bool pow2 = (L & (L - 1)) == 0;
+  const int lshift = __ffs(L) - 1;
+  for (int b = tt; b < NR; b += tps) {
+    const int j = pow2 ? (b & (L - 1)) : b % L, m = pow2 ? (b >> lshift) : b / L;
+    fft_butterfly<R, SIGN>(src, dst, tw, NR, L, b, j, m);
+  }
+}
+
+// FFT of every sequence of a packed workgroup: sequence q at a[q N ..] (b
+// scratch); threads of no sequence (act false) only keep the barriers.
+// Returns the buffer that holds the result (radices as fft_seq).
+template <int SIGN>
+__device__ inline cd* fft_seq_p(cd* a, cd* b, const cd* tw, int N, int qoff, int tt, int tps, bool act) {
+  int rest = N, L = 1;
+  while (rest > 1) {
+    const int R = rest % 8 == 0 ? 8 : rest % 4 == 0 ? 4 : rest % 2 == 0 ? 2 : 3;
+    if (act) {
+      if (R == 8) fft_stage_p<8, SIGN>(a + qoff, b + qoff, tw, N, L, tt, tps);
+      else if (R == 4) fft_stage_p<4, SIGN>(a + qoff, b + qoff, tw, N, L, tt, tps);
+      else if (R == 2) fft_stage_p<2, SIGN>(a + qoff, b + qoff, tw, N, L, tt, tps);
+      else fft_stage_p<3, SIGN>(a + qoff, b + qoff, tw, N, L, tt, tps);
+    }
+    L *= R;
+    rest /= R;
+    __syncthreads();
+    cd* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
 }
 
 // FFT of the FS sequences of `a` (length N each), `b` scratch; returns the
@@ -867,6 +910,281 @@ void launch_reg_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_
   OTM_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Packed workgroup form (n <= 1024 without a register form, launch kind
+// KIND_FFT): 256 threads; a sequence of n points (two rows) takes tps = n/4
+// threads, each with four frequencies / columns k = tt + u tps, so one
+// workgroup holds nseq = fft_pack_seqs(n) = 1024 / n sequences (2 nseq rows:
+// 8 at n = 128, 2 at n = 384, 1 from n = 513 up) instead of leaving the
+// threads beyond n/4 idle.  LDS: buffers A and B of nseq sequences, then the
+// twiddles: (2 nseq + 1) n complex.  Each thread runs the butterflies of its
+// own sequence (tt, tt + tps, ...); the stages of all sequences share the
+// workgroup barriers.
+// ---------------------------------------------------------------------------
+struct PackLoc {
+  int N, tps, nseq, q, tt, r0;  // r0: first row of the thread's sequence
+  bool act;                     // the thread belongs to a sequence inside the grid
+};
+__device__ inline PackLoc pack_locate(int N, int lt) {
+  PackLoc P;
+  P.N = N;
+  P.tps = N / 4;
+  P.nseq = fft_pack_seqs(N);
+  P.q = int(threadIdx.x) / P.tps;
+  P.tt = int(threadIdx.x) - P.q * P.tps;
+  P.r0 = 2 * (P.nseq * lt + P.q);
+  P.act = P.q < P.nseq && P.r0 < N;
+  return P;
+}
+// the n stage twiddles into LDS (issued with the other loads, stored later)
+constexpr int PACK_TW = FFT_SPLIT_N / 256;  // twiddles per thread at most
+__device__ inline void pack_load_tw(cd (&tv)[PACK_TW], const double* table, int N) {
+  const auto tw = gp(reinterpret_cast<const cd*>(table));
+#pragma unroll
+  for (int u = 0; u < PACK_TW; ++u) {
+    const int k = threadIdx.x + u * 256;
+    tv[u] = k < N ? tw[k] : cd{0., 0.};
+  }
+}
+__device__ inline void pack_store_tw(const cd (&tv)[PACK_TW], cd* twl, int N) {
+#pragma unroll
+  for (int u = 0; u < PACK_TW; ++u) {
+    const int k = threadIdx.x + u * 256;
+    if (k < N) twl[k] = tv[u];
+  }
+}
+// rows r0, r0 + 1 at columns tt + u tps (0 outside the grid)
+__device__ inline void pack_load_rows(double (&v)[2][KPT], const double* in_, const PackLoc& P) {
+  const auto in = gp(in_);
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) v[rr][u] = P.act ? in[long(P.r0 + rr) * P.N + P.tt + u * P.tps] : 0.;
+}
+__device__ inline void pack_store_permuted(const double (&v)[2][KPT], cd* A, const PackLoc& P) {
+  if (!P.act) return;
+  cd* z = A + P.q * P.N;
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) reinterpret_cast<double*>(z + perm_pos(P.tt + u * P.tps, P.N))[rr] = v[rr][u];
+}
+
+// Pass 1 (packed form): rows of R' -> DCT-II along j, written transposed.
+__global__ __launch_bounds__(256) void k_fftp_rows_fwd(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                       int count, bool gate, bool rhs) {
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const PackLoc P = pack_locate(J.f.n, L.lt);
+  const int N = P.N;
+  cd* A = fsm;
+  cd* B = fsm + P.nseq * N;
+  cd* twl = fsm + 2 * P.nseq * N;
+  cd tv[PACK_TW];
+  pack_load_tw(tv, J.fft, N);
+  const auto tg = gp(reinterpret_cast<const cd*>(J.fft));
+  cd ph[KPT];
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) ph[u] = P.act ? tg[N + P.tt + u * P.tps] : cd{0., 0.};
+  double v[2][KPT];
+  pack_load_rows(v, J.f.r, P);
+  if (rhs && P.act && P.r0 == 0 && P.tt == 0) v[0][0] -= st->sum_r;  // R' = R - (sum r) e0 e0^T
+  if (gate && early && st->done) return;
+  pack_store_tw(tv, twl, N);
+  pack_store_permuted(v, A, P);
+  __syncthreads();
+  const cd* z = fft_seq_p<-1>(A, B, twl, N, P.q * N, P.tt, P.tps, P.act);
+  if (!P.act) return;
+  const Scales sc(N);
+  const auto out = gp(J.f.tmp0);
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int k = P.tt + u * P.tps;
+    double x0, x1;
+    dct2_post(z + P.q * N, ph[u], sc, N, k, x0, x1);
+    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + P.r0) = cd{x0, x1};
+  }
+}
+
+// Pass 2 (packed form): DCT-II along i, / lambda, DCT-III along i, written
+// transposed back.
+__global__ __launch_bounds__(256) void k_fftp_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                   int count, bool gate) {
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const PackLoc P = pack_locate(J.f.n, L.lt);
+  const int N = P.N, l0 = P.r0;
+  cd* A = fsm;
+  cd* B = fsm + P.nseq * N;
+  cd* twl = fsm + 2 * P.nseq * N;
+  cd tv[PACK_TW];
+  pack_load_tw(tv, J.fft, N);
+  const auto tg = gp(reinterpret_cast<const cd*>(J.fft));
+  cd ph[KPT];
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) ph[u] = P.act ? tg[N + P.tt + u * P.tps] : cd{0., 0.};
+  double v[2][KPT];
+  pack_load_rows(v, J.f.tmp0, P);
+  const auto sq = gp(J.sq);
+  double lam[2][KPT];
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = P.tt + u * P.tps, l = l0 + rr;
+      const int dm = k > l ? k - l : l - k;
+      lam[rr][u] = P.act ? sq[dm] + sq[k + l] : 1.;
+    }
+  if (gate && early && st->done) return;
+  pack_store_tw(tv, twl, N);
+  pack_store_permuted(v, A, P);
+  __syncthreads();
+  cd* z = fft_seq_p<-1>(A, B, twl, N, P.q * N, P.tt, P.tps, P.act);
+  cd* other = z == A ? B : A;
+  const Scales sc(N);
+  // spectrum / lambda of the sequence's two rows, in its region of the free buffer
+  double* X = reinterpret_cast<double*>(other + P.q * N);
+  if (P.act) {
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = P.tt + u * P.tps;
+      double x[2];
+      dct2_post(z + P.q * N, ph[u], sc, N, k, x[0], x[1]);
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr) X[rr * N + k] = (k == 0 && l0 + rr == 0) ? 0. : x[rr] / (2. * lam[rr][u]);
+    }
+  }
+  __syncthreads();
+  if (P.act) {
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = P.tt + u * P.tps;
+      const int kn = k == 0 ? 0 : N - k;
+      z[P.q * N + k] = dct3_pre(X[k], X[kn], X[N + k], X[N + kn], ph[u], sc, k);
+    }
+  }
+  __syncthreads();
+  const cd* y = fft_seq_p<1>(z, other, twl, N, P.q * N, P.tt, P.tps, P.act);
+  if (!P.act) return;
+  const auto out = gp(J.f.tmp1);
+  const double inv = 1.0 / N;
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) {
+    const int i = P.tt + u * P.tps;
+    const cd a = y[P.q * N + perm_pos(i, N)];
+    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(i) * N + l0) = cd{a.x * inv, a.y * inv};
+  }
+}
+
+// Pass 3 (packed form): DCT-III along j = d_hat, and one |d_hat|^2 partial
+// per workgroup.
+__global__ __launch_bounds__(256) void k_fftp_rows_inv(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                       int count, bool gate) {
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const PackLoc P = pack_locate(J.f.n, L.lt);
+  const int N = P.N;
+  cd* A = fsm;
+  cd* B = fsm + P.nseq * N;
+  cd* twl = fsm + 2 * P.nseq * N;
+  cd tv[PACK_TW];
+  pack_load_tw(tv, J.fft, N);
+  const auto tg = gp(reinterpret_cast<const cd*>(J.fft));
+  cd ph[KPT];
+#pragma unroll
+  for (int u = 0; u < KPT; ++u) ph[u] = P.act ? tg[N + P.tt + u * P.tps] : cd{0., 0.};
+  // X[row][k] and X[row][N-k] of the sequence's two rows
+  const auto in = gp((const double*)J.f.tmp1) + long(P.act ? P.r0 : 0) * N;
+  double xk[2][KPT], xn[2][KPT];
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = P.tt + u * P.tps;
+      const int kn = k == 0 ? 0 : N - k;
+      xk[rr][u] = P.act ? in[rr * N + k] : 0.;
+      xn[rr][u] = P.act ? in[rr * N + kn] : 0.;
+    }
+  if (gate && early && st->done) return;
+  pack_store_tw(tv, twl, N);
+  const Scales sc(N);
+  if (P.act) {
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = P.tt + u * P.tps;
+      A[P.q * N + k] = dct3_pre(xk[0][u], xn[0][u], xk[1][u], xn[1][u], ph[u], sc, k);
+    }
+  }
+  __syncthreads();
+  const cd* y = fft_seq_p<1>(A, B, twl, N, P.q * N, P.tt, P.tps, P.act);
+  const double inv = 1.0 / N;
+  double nrm[1] = {0.};
+  if (P.act) {
+    const auto out = gp(J.f.dhat) + long(P.r0) * N;
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+      for (int u = 0; u < KPT; ++u) {
+        const int c = P.tt + u * P.tps;
+        const cd a = y[P.q * N + perm_pos(c, N)];
+        const double x = (rr ? a.y : a.x) * inv;
+        out[long(rr) * N + c] = x;
+        nrm[0] += x * x;
+      }
+  }
+  // workgroup partial: DPP wave sums, then the wave totals in a fixed order
+  // through LDS, in the buffer that does not hold y (the last FFT stage read
+  // it before its closing barrier)
+  wave_sum<1>(nrm);
+  double* red = reinterpret_cast<double*>(y == A ? B : A);
+  if ((threadIdx.x & 63) == 63) red[threadIdx.x >> 6] = nrm[0];
+  __syncthreads();
+  if (threadIdx.x == 0) gp(J.f.site(SITE_DNORM))[L.lt] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_pack_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  if (js.total[kind] == 0) return;
+  size_t lds = 0;
+  for (int p = 0; p < js.count; ++p) {
+    const int n = js.host[p].f.n;
+    if (js.jroute[p] == 2 && fft_pack(n)) lds = std::max(lds, size_t(2 * fft_pack_seqs(n) + 1) * n * sizeof(cd));
+  }
+  // the kernels use no static LDS; the dynamic size is allowed once per
+  // device (bit d of the mask) up to the largest it can take, n = 1024
+  static std::atomic<unsigned long long> attr{0};
+  int dev = 0;
+  OTM_HIP(hipGetDevice(&dev));
+  const unsigned long long bit = 1ull << (dev & 63);
+  if (!(attr.load() & bit)) {
+    const int cap = int(size_t(2 * fft_pack_seqs(FFT_SPLIT_N) + 1) * FFT_SPLIT_N * sizeof(cd));
+    for (const void* k : {reinterpret_cast<const void*>(&k_fftp_rows_fwd), reinterpret_cast<const void*>(&k_fftp_cols),
+                          reinterpret_cast<const void*>(&k_fftp_rows_inv)})
+      OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+    attr.fetch_or(bit);
+  }
+  const dim3 grid(unsigned(js.total[kind])), block(256);
+  const int* st = js.starts_of(kind);
+  const Job* jobs = js.jobs;
+  switch (pass) {
+    case 0: hipLaunchKernelGGL(k_fftp_rows_fwd, grid, block, lds, s, jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL(k_fftp_cols, grid, block, lds, s, jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL(k_fftp_rows_inv, grid, block, lds, s, jobs, st, js.count, gate);
+  }
+  OTM_HIP(hipGetLastError());
+}
+
 template <int FS>
 void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
   if (js.total[kind] == 0) return;
@@ -908,7 +1226,7 @@ void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
   launch_reg_kind<8>(js, KIND_FFTR8, pass, gate, s);
   launch_reg_kind<16>(js, KIND_FFTR16, pass, gate, s);
-  launch_pass_kind<1>(js, KIND_FFT, pass, gate, s);
+  launch_pack_kind(js, KIND_FFT, pass, gate, s);
   launch_pass_kind<2>(js, KIND_FFT2, pass, gate, s);
 }
 

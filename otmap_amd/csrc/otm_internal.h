@@ -189,7 +189,7 @@ enum LaunchKind {
   KIND_DNORM = 5,      // each job's workgroups in the last stage of its transform (writers of the |d_hat|^2 partials)
   KIND_STENCIL64 = 6,  // stencil tiles of the jobs on the FP64 transform (their plain update pass)
   KIND_FLATMX = 7,     // grid-stride blocks of the jobs on the mixed transform (first right-hand side)
-  KIND_FFT = 8,        // fast-DCT passes of the FFT-route jobs with n <= FFT_SPLIT_N: one workgroup per 2 rows
+  KIND_FFT = 8,        // fast-DCT passes of the FFT-route jobs with n <= FFT_SPLIT_N, packed: 1024/n sequences per workgroup
   KIND_FFT2 = 9,       // fast-DCT passes of the FFT-route jobs with n > FFT_SPLIT_N: one workgroup per 4 rows
   KIND_FFTR8 = 10,     // fast-DCT passes in register form, n = 512: one 64-thread workgroup per 2 rows
   KIND_FFTR16 = 11,    // the same for n = 1024
@@ -245,6 +245,14 @@ constexpr int fft_rows(int n) { return n <= FFT_SPLIT_N ? 2 : 4; }
 // Points per lane of the register form of the fast-DCT passes (one wave per
 // pair of rows, fft.hip), 0 for sizes that use the workgroup form.
 constexpr int fft_reg_e(int n) { return n == 512 ? 8 : n == 1024 ? 16 : 0; }
+// The packed workgroup form of the other sizes up to FFT_SPLIT_N (fft.hip):
+// 256 threads, n/4 per complex sequence (two rows), so 1024 / n sequences
+// per workgroup; its workgroups per grid.
+constexpr bool fft_pack(int n) { return n <= FFT_SPLIT_N && fft_reg_e(n) == 0; }
+constexpr int fft_pack_seqs(int n) { return FFT_SPLIT_N / n; }
+constexpr int fft_pack_wgs(int n) { return (n / 2 + fft_pack_seqs(n) - 1) / fft_pack_seqs(n); }
+// workgroups per grid of the fast-DCT passes (one |d_hat|^2 partial each)
+constexpr int fft_wgs(int n) { return fft_reg_e(n) ? n / 2 : fft_pack(n) ? fft_pack_wgs(n) : n / 4; }
 bool fft_supported(int n);
 // FP64 route of a grid: the fast DCT when asked for and supported, else the GEMMs
 inline int fp64_route(int n, int transform) { return transform == 1 && fft_supported(n) ? 2 : 0; }
@@ -297,7 +305,7 @@ struct DctPlan {
   double* fft = nullptr;  // fast-DCT tables (fft.hip), when fft_supported(n)
 };
 void fft_plan_create(double*& table, int n);  // the 4n-double table of Job::fft, or nullptr when unsupported
-// d_hat of every job on the fast-DCT route: three passes, n / fft_rows(n) workgroups per job
+// d_hat of every job on the fast-DCT route: three passes, fft_wgs(n) workgroups per job
 void launch_fft_solve(const JobSet& js, bool gate, hipStream_t s);
 // one of the three passes (0 rows forward, 1 columns, 2 rows inverse)
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s);

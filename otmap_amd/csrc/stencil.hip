@@ -1140,7 +1140,7 @@ std::vector<int> job_starts(JobSet& js) {
       if (kind == KIND_STENCIL) v = num_tiles(n);
       else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
       else if (kind == KIND_STENCIL64) v = (r64 || rfft) ? num_tiles(n) : 0;  // the plain update pass
-      else if (kind == KIND_FFT) v = rfft && fft_rows(n) == 2 && !fft_reg_e(n) ? n / 2 : 0;
+      else if (kind == KIND_FFT) v = rfft && fft_pack(n) ? fft_pack_wgs(n) : 0;
       else if (kind == KIND_FFTR8) v = rfft && fft_reg_e(n) == 8 ? n / 2 : 0;
       else if (kind == KIND_FFTR16) v = rfft && fft_reg_e(n) == 16 ? n / 2 : 0;
       else if (kind == KIND_FFT2) v = rfft && fft_rows(n) == 4 ? n / 4 : 0;
@@ -1150,7 +1150,7 @@ std::vector<int> job_starts(JobSet& js) {
       else if (kind == KIND_UPAIR) v = rmx && mixed_supported(n) ? upair_tiles(n) : 0;
       else if (kind == KIND_DNORM) {
         if (js.tuning) v = 0;
-        else if (js.jroute[p] == 2) v = n / fft_rows(n);
+        else if (js.jroute[p] == 2) v = fft_wgs(n);
         else if (js.jroute[p] == 1) v = mixed_cfg_valid(3, n, mc[3]) ? pad8(mixed_tiles(3, n, mc[3])) : 0;
         else if (js.fold) v = fold_cfg_valid(n, fc[3]) ? pad8(fold_tiles(n, 3, fc[3])) : 0;
         else v = ((n + 63) / 64) * ((n + 63) / 64);
