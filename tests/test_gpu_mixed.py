@@ -135,10 +135,10 @@ def test_mixed_sensitive_trajectory(oracle, solver_factory):
     """Density 29 of the 64-density batched workload (n = 192, 3 Gaussians)
     has a trajectory whose step lengths amplify a perturbation about 3x per
     iteration from iteration ~17 on.  The default mixed mode runs the FP64
-    transform at this size and holds the 1e-6 bar; forced onto the BF16x3
-    transform (mixed_min_n = 0), a Laplacian-solve error of 1e-12 (one
-    refinement) ends ~1.3e-6 away from the oracle's positions and 4e-14 (two
-    refinements) below 1e-7.  Same iteration count in every mode."""
+    transform at this size and holds the 1e-6 bar (3.4e-9 from the oracle's
+    positions); forced onto the BF16x3 transform (mixed_min_n = 0), a
+    Laplacian-solve error of 1e-12 (one refinement) ends ~3e-6 away and 4e-14
+    (two refinements) ~9e-10.  Same iteration count in every mode."""
     specs = D.batch_specs(64)
     n = specs[29][0]
     rho = D.batch_densities(64)[29]
