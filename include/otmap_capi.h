@@ -116,9 +116,9 @@ int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, doubl
  * solve (which it overwrites), and returns the mean ms per launch. */
 enum {
   OTM_KERNEL_DCT_GEMM = 0,  /* one n x n x n FP64-MFMA GEMM stage of the DCT solve */
-  OTM_KERNEL_BETA = 1,      /* conjugate-Jacobian residual + reductions */
+  OTM_KERNEL_BETA = 1,      /* conjugate-Jacobian pass + beta step */
   OTM_KERNEL_DIR_COEF = 2,  /* direction, quartic coefficients + line search */
-  OTM_KERNEL_UPDATE = 3,    /* psi / G0 / r update */
+  OTM_KERNEL_UPDATE = 3,    /* psi / r / Jd update */
   OTM_KERNEL_DCT_SOLVE = 4, /* the whole FP64 Laplacian solve (4 GEMM stages) */
   OTM_KERNEL_MIXED_GEMM = 5, /* one 3-term BF16-split GEMM stage (6 MFMA products) */
   OTM_KERNEL_MIXED_SOLVE = 6, /* the whole mixed Laplacian solve (split, 8 stages, refinement) */

@@ -61,25 +61,35 @@ void fft_plan_create(double*& table, int n) {
   // Then the DCT phases exp(-i pi k / 2n), k < n.
   std::vector<double> h(4 * size_t(n), 0.);
   const long double pi = 3.141592653589793238462643383279502884L;
-  // the register form (n = 1024: radices 16, 16, 4) or stage_radix's order
-  const int reg16[3] = {16, 16, 4};
+  auto put = [&](size_t e, long double a) {
+    h[2 * e] = double(cosl(a));
+    h[2 * e + 1] = double(-sinl(a));
+  };
+  // the register form (n = 512: radices 8, 8, 8; n = 1024: 16, 16, 4) or
+  // stage_radix's order
+  const int reg[2][3] = {{8, 8, 8}, {16, 16, 4}};
+  const int re = fft_reg_e(n);
   int si = 0;
   for (int rest = n, L = 1; rest > 1; ++si) {
-    const int R = fft_reg_e(n) == 16 ? reg16[si] : stage_radix(rest);
-    for (int r = 1; r < R; ++r)
-      for (int j = 0; j < L; ++j) {
-        const long double a = 2.0L * pi * (long double)(j * r) / (long double)(L * R);
-        const size_t e = size_t(L - 1 + (r - 1) * L + j);
-        h[2 * e] = double(cosl(a));
-        h[2 * e + 1] = double(-sinl(a));
-      }
+    const int R = re ? reg[re / 16][si] : stage_radix(rest);
+    // the register form's last stage reads the quarter table below instead
+    if (!(re && L * R == n))
+      for (int r = 1; r < R; ++r)
+        for (int j = 0; j < L; ++j) put(size_t(L - 1 + (r - 1) * L + j), 2.0L * pi * (j * r) / (long double)(L * R));
     L *= R;
     rest /= R;
   }
-  for (int t = 0; t < n; ++t) {
-    const long double b = pi * t / (2.0L * n);
-    h[2 * n + 2 * t] = double(cosl(b));
-    h[2 * n + 2 * t + 1] = double(-sinl(b));
+  if (re) {
+    // register form: after the earlier stages' n / R_last - 1 entries, the
+    // quarter table exp(-2 pi i m / n), m < n/4 (the last stage's twiddle
+    // exp(-2 pi i r j / n) is (-i)^(rj div n/4) times entry rj mod n/4, the
+    // rotation exact), then the DCT phases of k <= n/2 (phase n - k is
+    // -i conj(phase k): the real and imaginary parts swapped and negated)
+    const size_t q0 = size_t(n / reg[re / 16][2] - 1), p0 = q0 + size_t(n / 4);
+    for (int m = 0; m < n / 4; ++m) put(q0 + size_t(m), 2.0L * pi * m / (long double)n);
+    for (int t = 0; t <= n / 2; ++t) put(p0 + size_t(t), pi * t / (2.0L * n));
+  } else {
+    for (int t = 0; t < n; ++t) put(size_t(n) + size_t(t), pi * t / (2.0L * n));
   }
   OTM_HIP(hipMalloc(&table, h.size() * sizeof(double)));
   OTM_HIP(hipMemcpy(table, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -591,9 +601,12 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict
 // 8, 8 (n = 512) and 16, 16, 4 (n = 1024): two exchanges per FFT instead of
 // a workgroup barrier after each of five radix-8/4/2 stages.  The DCT
 // post/pre-processing pairs frequency k with n - k through one more
-// exchange.  The stage twiddles and DCT phases are staged in LDS at the
-// start, as are the exchange buffer's padded positions p + p / 16 (lanes of a
-// 16-lane group write distinct banks).
+// exchange.  The exchange buffer's positions are padded, p + p / 16, so the
+// lanes of a 16-lane group write distinct banks.  A workgroup holds W waves,
+// one sequence each (W = 1 for a single grid, whose n / 2 sequences fill the
+// chip one wave per SIMD pair; W = 2 in batches), with W exchange buffers
+// and one copy of the twiddle and phase table (RegTab), staged in LDS at the
+// start.
 // ---------------------------------------------------------------------------
 template <int E>
 struct RegRadix;
@@ -605,6 +618,39 @@ template <>
 struct RegRadix<16> {
   static constexpr int R0 = 16, R1 = 16, R2 = 4;
 };
+
+// The table of the register form (fft_plan_create): the stage-major twiddles
+// of the stages before the last (entries 0 .. Q0 - 1), the quarter table
+// e^{-2 pi i m / N}, m < N/4, for the last stage, the DCT phases
+// e^{-i pi k / 2N}, k <= N/2.  16 KB at N = 1024 (instead of the 32 KB of the
+// full stage and phase tables), so four 1-wave workgroups fit one CU's LDS.
+template <int E>
+struct RegTab {
+  static constexpr int N = 64 * E;
+  static constexpr int Q0 = N / RegRadix<E>::R2 - 1;
+  static constexpr int PH0 = Q0 + N / 4;
+  static constexpr int SIZE = PH0 + N / 2 + 1;
+};
+// the last stage's twiddle e^{-2 pi i m / N}, m < N: (-i)^(m div N/4) times
+// the quarter-table entry m mod N/4 (the rotation is exact)
+template <int E>
+__device__ inline cd reg_twiddle(const cd* tab, int m) {
+  constexpr int Q = 16 * E;  // N / 4
+  const cd q = tab[RegTab<E>::Q0 + (m & (Q - 1))];
+  const int s = m / Q;
+  cd w = (s & 1) ? cd{q.y, -q.x} : q;
+  if (s & 2) w = cd{-w.x, -w.y};
+  return w;
+}
+// the DCT phase e^{-i pi k / 2N} of frequency k = t + 64 v (v a constant after
+// unrolling): entries k <= N/2 directly, above as phase N - k swapped and negated
+template <int E>
+__device__ inline cd reg_phase(const cd* tab, int v, int k) {
+  constexpr int N = 64 * E;
+  if (v < E / 2) return tab[RegTab<E>::PH0 + k];
+  const cd p = tab[RegTab<E>::PH0 + N - k];
+  return cd{-p.y, -p.x};
+}
 
 __device__ inline int xpad(int p) { return p + (p >> 4); }
 
@@ -653,7 +699,7 @@ __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
     if constexpr (L > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const cd w = tw[(L - 1) + (r - 1) * L + j];
+        const cd w = LAST ? reg_twiddle<E>(tw, r * j) : tw[(L - 1) + (r - 1) * L + j];
         a[r] = cmul(a[r], SIGN < 0 ? w : conjc(w));
       }
     }
@@ -685,23 +731,33 @@ __device__ inline void reg_fft(cd (&x)[E], cd* xb, const cd* tw, int t) {
 }
 
 // Common start of a register-form pass: the job, the gate, the table loads.
-// LDS: the exchange buffer (n + n/16 complex), then the stage twiddles (n
-// complex, n - 1 used) and the DCT phases (n complex).
-template <int E>
+// LDS: W exchange buffers (N + N/16 complex each), then the table (RegTab),
+// loaded by all 64 W threads.
+// table entries per thread of a W-wave workgroup
+constexpr int reg_tpt(int size, int w) { return (size + 64 * w - 1) / (64 * w); }
+template <int E, int W>
 struct RegPass {
   static constexpr int N = 64 * E;
-  static constexpr size_t lds_bytes() { return size_t(3 * N + N / 16) * sizeof(cd); }
+  static constexpr int XB = N + N / 16;
+  static constexpr int TPT = reg_tpt(RegTab<E>::SIZE, W);
+  static constexpr size_t lds_bytes() { return size_t(W * XB + RegTab<E>::SIZE) * sizeof(cd); }
 };
-template <int E>
-__device__ inline void reg_load_table(cd (&tv)[2 * E], const double* table, int t) {
+template <int E, int W>
+__device__ inline void reg_load_table(cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], const double* table) {
   const auto tg = gp(reinterpret_cast<const cd*>(table));
 #pragma unroll
-  for (int q = 0; q < 2 * E; ++q) tv[q] = tg[t + 64 * q];
+  for (int q = 0; q < RegPass<E, W>::TPT; ++q) {
+    const int e = int(threadIdx.x) + 64 * W * q;
+    tv[q] = e < RegTab<E>::SIZE ? tg[e] : cd{0., 0.};
+  }
 }
-template <int E>
-__device__ inline void reg_store_table(const cd (&tv)[2 * E], cd* tws, int t) {
+template <int E, int W>
+__device__ inline void reg_store_table(const cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], cd* tab) {
 #pragma unroll
-  for (int q = 0; q < 2 * E; ++q) tws[t + 64 * q] = tv[q];
+  for (int q = 0; q < RegPass<E, W>::TPT; ++q) {
+    const int e = int(threadIdx.x) + 64 * W * q;
+    if (e < RegTab<E>::SIZE) tab[e] = tv[q];
+  }
 }
 // two real rows r0, r0 + 1 as the permuted complex sequence z = v_a + i v_b,
 // v[m] = x[2m], v[n-1-m] = x[2m+1]: lane t slot v holds z[t + 64 v]
@@ -716,91 +772,113 @@ __device__ inline void reg_load_rows(cd (&x)[E], const double* in_, int r0, int 
     x[v] = cd{in[c], in[N + c]};
   }
 }
+// The wave's lane t, its exchange buffer and first row r0 (sequence W lt + wave)
+template <int E, int W>
+struct RegWave {
+  int t, r0;
+  cd* xb;
+  cd* tab;
+  __device__ RegWave(cd* fsm, int lt) {
+    const int wv = int(threadIdx.x) >> 6;
+    t = int(threadIdx.x) & 63;
+    r0 = 2 * (W * lt + wv);
+    xb = fsm + wv * RegPass<E, W>::XB;
+    tab = fsm + W * RegPass<E, W>::XB;
+  }
+};
 
 // Pass 1 (register form): rows of R' -> DCT-II along j, written transposed.
-template <int E>
-__global__ __launch_bounds__(64) void k_fftr_rows_fwd(const Job* __restrict__ jobs, const int* __restrict__ starts,
-                                                      int count, bool gate, bool rhs) {
-  constexpr int N = 64 * E;
+template <int E, int W>
+__global__ __launch_bounds__(64 * W) void k_fftr_rows_fwd(const Job* __restrict__ jobs,
+                                                          const int* __restrict__ starts, int count, bool gate,
+                                                          bool rhs) {
+  using P = RegPass<E, W>;
+  constexpr int N = P::N;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   const bool early = count == 1;
   if (gate && !early && st->done) return;
-  const int t = threadIdx.x, r0 = 2 * L.lt;
-  cd* xb = fsm;
-  cd* tws = fsm + N + N / 16;
-  cd tv[2 * E];
-  reg_load_table<E>(tv, J.fft, t);
+  const RegWave<E, W> G(fsm, L.lt);
+  const int t = G.t, r0 = G.r0;
+  cd* xb = G.xb;
+  cd tv[P::TPT];
+  reg_load_table<E, W>(tv, J.fft);
   cd x[E];
   reg_load_rows<E>(x, J.f.r, r0, t);
   if (rhs && r0 == 0 && t == 0) x[0].x -= st->sum_r;  // R' = R - (sum r) e0 e0^T
   if (gate && early && st->done) return;
-  reg_store_table<E>(tv, tws, t);
+  reg_store_table<E, W>(tv, G.tab);
   __syncthreads();
-  reg_fft<E, -1>(x, xb, tws, t);
+  reg_fft<E, -1>(x, xb, G.tab, t);
 #pragma unroll
   for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = x[v];
   __syncthreads();
   const Scales sc(N);
   const auto out = gp(J.f.tmp0);
-  const cd* ph = tws + N;
 #pragma unroll
   for (int v = 0; v < E; ++v) {
     const int k = t + 64 * v;
     double xa, xo;
-    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], ph[k], sc.s(k), xa, xo);
+    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], reg_phase<E>(G.tab, v, k), sc.s(k), xa, xo);
     *reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + r0) = cd{xa, xo};
   }
 }
 
 // Pass 2 (register form): rows of B^T, DCT-II along i, / lambda, DCT-III
 // along i, written transposed back.
-template <int E>
-__global__ __launch_bounds__(64) void k_fftr_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
-                                                  int count, bool gate) {
-  constexpr int N = 64 * E;
+template <int E, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 1 : 2))) void k_fftr_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                      int count, bool gate) {
+  using P = RegPass<E, W>;
+  constexpr int N = P::N;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   const bool early = count == 1;
   if (gate && !early && st->done) return;
-  const int t = threadIdx.x, l0 = 2 * L.lt;
-  cd* xb = fsm;
-  cd* tws = fsm + N + N / 16;
-  cd tv[2 * E];
-  reg_load_table<E>(tv, J.fft, t);
+  const RegWave<E, W> G(fsm, L.lt);
+  const int t = G.t, l0 = G.r0;
+  cd* xb = G.xb;
+  cd tv[P::TPT];
+  reg_load_table<E, W>(tv, J.fft);
   cd x[E];
   reg_load_rows<E>(x, J.f.tmp0, l0, t);
-  // eigenvalue halves sin^2(pi |k-l| / 2N) + sin^2(pi (k+l) / 2N) of this lane's (k, l)
+  // eigenvalue halves sin^2(pi |k-l| / 2N) + sin^2(pi (k+l) / 2N) of this
+  // lane's (k, l): loaded with the rows for a single grid; in batches, where
+  // other waves hide the latency, after the forward FFT, so that the 2E
+  // values are not held through it (two waves per SIMD)
   const auto sq = gp(J.sq);
   double lam[2][E];
+  auto load_lam = [&]() {
 #pragma unroll
-  for (int v = 0; v < E; ++v)
+    for (int v = 0; v < E; ++v)
 #pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-      const int k = t + 64 * v, l = l0 + rr;
-      const int dm = k > l ? k - l : l - k;
-      lam[rr][v] = sq[dm] + sq[k + l];
-    }
+      for (int rr = 0; rr < 2; ++rr) {
+        const int k = t + 64 * v, l = l0 + rr;
+        const int dm = k > l ? k - l : l - k;
+        lam[rr][v] = sq[dm] + sq[k + l];
+      }
+  };
+  if constexpr (W == 1) load_lam();
   if (gate && early && st->done) return;
-  reg_store_table<E>(tv, tws, t);
+  reg_store_table<E, W>(tv, G.tab);
   __syncthreads();
-  reg_fft<E, -1>(x, xb, tws, t);
+  reg_fft<E, -1>(x, xb, G.tab, t);
+  if constexpr (W > 1) load_lam();
 #pragma unroll
   for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = x[v];
   __syncthreads();
   const Scales sc(N);
-  const cd* ph = tws + N;
   // spectrum / lambda, scaled by 1 / s_k for the inverse: (U_a[k], U_b[k])
   cd w[E];
 #pragma unroll
   for (int v = 0; v < E; ++v) {
     const int k = t + 64 * v;
     double xa, xo;
-    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], ph[k], sc.s(k), xa, xo);
+    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], reg_phase<E>(G.tab, v, k), sc.s(k), xa, xo);
     const double Xa = (k == 0 && l0 == 0) ? 0. : xa / (2. * lam[0][v]);
     const double Xb = xo / (2. * lam[1][v]);
     w[v] = cd{Xa * sc.r(k), Xb * sc.r(k)};
@@ -813,10 +891,10 @@ __global__ __launch_bounds__(64) void k_fftr_cols(const Job* __restrict__ jobs, 
   for (int v = 0; v < E; ++v) {
     const int k = t + 64 * v;
     const cd wn = k == 0 ? cd{0., 0.} : xb[xpad(N - k)];
-    x[v] = dct3_pre_u(w[v].x, wn.x, w[v].y, wn.y, ph[k]);
+    x[v] = dct3_pre_u(w[v].x, wn.x, w[v].y, wn.y, reg_phase<E>(G.tab, v, k));
   }
   __syncthreads();
-  reg_fft<E, 1>(x, xb, tws, t);
+  reg_fft<E, 1>(x, xb, G.tab, t);
   const auto out = gp(J.f.tmp1);
   const double inv = 1.0 / N;
 #pragma unroll
@@ -827,22 +905,23 @@ __global__ __launch_bounds__(64) void k_fftr_cols(const Job* __restrict__ jobs, 
   }
 }
 
-// Pass 3 (register form): DCT-III along j = d_hat, and the |d_hat|^2 partial.
-template <int E>
-__global__ __launch_bounds__(64) void k_fftr_rows_inv(const Job* __restrict__ jobs, const int* __restrict__ starts,
-                                                      int count, bool gate) {
-  constexpr int N = 64 * E;
+// Pass 3 (register form): DCT-III along j = d_hat, and the |d_hat|^2 partial
+// of each sequence (row pair).
+template <int E, int W>
+__global__ __launch_bounds__(64 * W) void k_fftr_rows_inv(const Job* __restrict__ jobs,
+                                                          const int* __restrict__ starts, int count, bool gate) {
+  using P = RegPass<E, W>;
+  constexpr int N = P::N;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   const bool early = count == 1;
   if (gate && !early && st->done) return;
-  const int t = threadIdx.x, r0 = 2 * L.lt;
-  cd* xb = fsm;
-  cd* tws = fsm + N + N / 16;
-  cd tv[2 * E];
-  reg_load_table<E>(tv, J.fft, t);
+  const RegWave<E, W> G(fsm, L.lt);
+  const int t = G.t, r0 = G.r0;
+  cd tv[P::TPT];
+  reg_load_table<E, W>(tv, J.fft);
   // X[row][k] and X[row][N-k] of the two rows
   const auto in = gp((const double*)J.f.tmp1) + long(r0) * N;
   double xk[2][E], xn[2][E];
@@ -857,16 +936,16 @@ __global__ __launch_bounds__(64) void k_fftr_rows_inv(const Job* __restrict__ jo
     }
   }
   if (gate && early && st->done) return;
-  reg_store_table<E>(tv, tws, t);
+  reg_store_table<E, W>(tv, G.tab);
   const Scales sc(N);
+  __syncthreads();
   cd x[E];
 #pragma unroll
   for (int v = 0; v < E; ++v) {
     const int k = t + 64 * v;
-    x[v] = dct3_pre(xk[0][v], xn[0][v], xk[1][v], xn[1][v], tws[N + k], sc, k);
+    x[v] = dct3_pre(xk[0][v], xn[0][v], xk[1][v], xn[1][v], reg_phase<E>(G.tab, v, k), sc, k);
   }
-  __syncthreads();
-  reg_fft<E, 1>(x, xb, tws, t);
+  reg_fft<E, 1>(x, G.xb, G.tab, t);
   const auto out = gp(J.f.dhat) + long(r0) * N;
   const double inv = 1.0 / N;
   double nrm[1] = {0.};
@@ -881,33 +960,40 @@ __global__ __launch_bounds__(64) void k_fftr_rows_inv(const Job* __restrict__ jo
     nrm[0] += b * b;
   }
   wave_sum<1>(nrm);
-  if (t == 63) gp(J.f.site(SITE_DNORM))[L.lt] = nrm[0];
+  if (t == 63) gp(J.f.site(SITE_DNORM))[r0 / 2] = nrm[0];
 }
 
-template <int E>
-void launch_reg_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
-  if (js.total[kind] == 0) return;
-  const size_t lds = RegPass<E>::lds_bytes();
+template <int E, int W>
+void launch_reg(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  const size_t lds = RegPass<E, W>::lds_bytes();
   static std::atomic<unsigned long long> attr{0};
   int dev = 0;
   OTM_HIP(hipGetDevice(&dev));
   const unsigned long long bit = 1ull << (dev & 63);
   if (!(attr.load() & bit)) {
-    for (const void* k : {reinterpret_cast<const void*>(&k_fftr_rows_fwd<E>),
-                          reinterpret_cast<const void*>(&k_fftr_cols<E>),
-                          reinterpret_cast<const void*>(&k_fftr_rows_inv<E>)})
+    for (const void* k : {reinterpret_cast<const void*>(&k_fftr_rows_fwd<E, W>),
+                          reinterpret_cast<const void*>(&k_fftr_cols<E, W>),
+                          reinterpret_cast<const void*>(&k_fftr_rows_inv<E, W>)})
       OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     attr.fetch_or(bit);
   }
-  const dim3 grid(unsigned(js.total[kind])), block(64);
+  const dim3 grid(unsigned(js.total[kind])), block(64 * W);
   const int* st = js.starts_of(kind);
   const Job* jobs = js.jobs;
   switch (pass) {
-    case 0: hipLaunchKernelGGL(k_fftr_rows_fwd<E>, grid, block, lds, s, jobs, st, js.count, gate, true); break;
-    case 1: hipLaunchKernelGGL(k_fftr_cols<E>, grid, block, lds, s, jobs, st, js.count, gate); break;
-    default: hipLaunchKernelGGL(k_fftr_rows_inv<E>, grid, block, lds, s, jobs, st, js.count, gate);
+    case 0: hipLaunchKernelGGL((k_fftr_rows_fwd<E, W>), grid, block, lds, s, jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL((k_fftr_cols<E, W>), grid, block, lds, s, jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL((k_fftr_rows_inv<E, W>), grid, block, lds, s, jobs, st, js.count, gate);
   }
   OTM_HIP(hipGetLastError());
+}
+
+// workgroups of fft_reg_waves(js.count) waves (job_starts counts them)
+template <int E>
+void launch_reg_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  if (js.total[kind] == 0) return;
+  if (fft_reg_waves(js.count) == 1) launch_reg<E, 1>(js, kind, pass, gate, s);
+  else launch_reg<E, FFT_REG_W>(js, kind, pass, gate, s);
 }
 
 // ---------------------------------------------------------------------------

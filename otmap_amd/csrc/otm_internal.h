@@ -245,6 +245,11 @@ constexpr int fft_rows(int n) { return n <= FFT_SPLIT_N ? 2 : 4; }
 // Points per lane of the register form of the fast-DCT passes (one wave per
 // pair of rows, fft.hip), 0 for sizes that use the workgroup form.
 constexpr int fft_reg_e(int n) { return n == 512 ? 8 : n == 1024 ? 16 : 0; }
+// Waves (complex sequences) per workgroup of the register form: one for a
+// single grid (n / 2 workgroups spread over every CU), FFT_REG_W in batches,
+// where the waves of a workgroup share one LDS copy of the twiddle table.
+constexpr int FFT_REG_W = 2;
+constexpr int fft_reg_waves(int jobs) { return jobs == 1 ? 1 : FFT_REG_W; }
 // The packed workgroup form of the other sizes up to FFT_SPLIT_N (fft.hip):
 // 256 threads, n/4 per complex sequence (two rows), so 1024 / n sequences
 // per workgroup; its workgroups per grid.

@@ -2,12 +2,13 @@
 // host-side orchestration of GridBasedTransportSolver::init()/solve()
 // (otlib/otsolver_2dgrid.cpp:75-245).
 //
-// One Newton iteration is nine kernels on the handle's stream, each launched
+// One Newton iteration is eight kernels on the handle's stream with the
+// default fast-DCT transform (nine with the GEMM transform), each launched
 // over the handle's one-job table:
-//   4 DCT GEMM stages (d_hat = M^+ (r - S e0))                otsolver_2dgrid.cpp:174-183
-//   k_beta + k_scalar_beta (conjugate-Jacobian beta)           :191-200, :248-261
-//   k_dir_coef + k_scalar_step (d, quartic coefficients, alpha) :467-598
-//   k_update (psi, G0, r)                                      :599-605
+//   3 fast-DCT passes, or 4 DCT GEMM stages (d_hat = M^+ (r - S e0))  otsolver_2dgrid.cpp:174-183
+//   k_beta + k_scalar_beta (conjugate-Jacobian beta)                  :191-200, :248-261
+//   k_dir_coef + k_scalar_step (d, quartic coefficients, alpha)       :467-598
+//   k_update (psi, r, Jd)                                              :599-605
 // All scalar decisions are taken on the device; the host only checks the
 // stop flag once per chunk of iterations, and kernels enqueued past the stop
 // return immediately.

@@ -559,7 +559,7 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restr
 // coefficients a, b per quad (r(psi + t d) = r + b t + a t^2), and the
 // partials of b.r, a.r, b.b, a.b, a.a, sum a, sum b   (otsolver_2dgrid.cpp:467-561).
 // a and b are not stored: the update pass evaluates them again from the same
-// inputs (d and the not yet updated G0), which costs less than their HBM round trip.
+// inputs (d and psi before its update), which costs less than their HBM round trip.
 __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
   const Loc L = locate(starts, count);
   const Fields& f = jobs[L.p].f;
@@ -1141,8 +1141,8 @@ std::vector<int> job_starts(JobSet& js) {
       else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
       else if (kind == KIND_STENCIL64) v = (r64 || rfft) ? num_tiles(n) : 0;  // the plain update pass
       else if (kind == KIND_FFT) v = rfft && fft_pack(n) ? fft_pack_wgs(n) : 0;
-      else if (kind == KIND_FFTR8) v = rfft && fft_reg_e(n) == 8 ? n / 2 : 0;
-      else if (kind == KIND_FFTR16) v = rfft && fft_reg_e(n) == 16 ? n / 2 : 0;
+      else if (kind == KIND_FFTR8) v = rfft && fft_reg_e(n) == 8 ? n / (2 * fft_reg_waves(js.count)) : 0;
+      else if (kind == KIND_FFTR16) v = rfft && fft_reg_e(n) == 16 ? n / (2 * fft_reg_waves(js.count)) : 0;
       else if (kind == KIND_FFT2) v = rfft && fft_rows(n) == 4 ? n / 4 : 0;
       else if (kind == KIND_FLATMX) v = rmx && mixed_supported(n) ? flat_blocks(J.f.F) : 0;
       else if (kind == KIND_PLAIN64) v = r64 ? ((n + 63) / 64) * ((n + 63) / 64) : 0;
