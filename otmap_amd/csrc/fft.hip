@@ -17,9 +17,9 @@
 // V[k] = e^{i pi k/2n} (U[k] - i U[n-k]); v = IFFT(V); x[2m] = v[m],
 // x[2m+1] = v[n-1-m]; again two rows per complex sequence.
 //
-// Passes (one workgroup per 2 rows, one complex sequence, for n <= 1024;
-// per 4 rows, two sequences, above; for n = 512 and 1024 a 64-thread
-// workgroup per 2 rows in the register form below):
+// Passes (workgroups of 2 rows per complex sequence, packed, for n <= 1024;
+// of 4 rows, two sequences, above; for n = 512 one wave per 2 rows in the
+// register form, for n = 1024 one wave per row in the row form below):
 //   1  rows of R' -> DCT-II along j, written transposed (B^T, 16- or 32-byte runs)
 //   2  rows of B^T (columns of B) -> DCT-II along i, / lambda, DCT-III along
 //      i, written transposed back (spatial i, frequency l along j)
@@ -65,30 +65,33 @@ void fft_plan_create(double*& table, int n) {
     h[2 * e] = double(cosl(a));
     h[2 * e + 1] = double(-sinl(a));
   };
-  // the register form (n = 512: radices 8, 8, 8; n = 1024: 16, 16, 4) or
-  // stage_radix's order
-  const int reg[2][3] = {{8, 8, 8}, {16, 16, 4}};
-  const int re = fft_reg_e(n);
-  int si = 0;
-  for (int rest = n, L = 1; rest > 1; ++si) {
-    const int R = re ? reg[re / 16][si] : stage_radix(rest);
-    // the register form's last stage reads the quarter table below instead
-    if (!(re && L * R == n))
+  if (fft_reg_e(n) || fft_row_form(n)) {
+    // Register forms.  Their m-point complex FFT (m = n for the 2-row form,
+    // n/2 for the row form; radices 8, 8, 8): the twiddles of the first two
+    // stages by stage as above (entries 0 .. 62), then the quarter table
+    // exp(-2 pi i q / m), q < m/4, from which the last stage takes its
+    // twiddle exp(-2 pi i r j / m) as (-i)^(rj div m/4) times entry rj mod
+    // m/4 (the rotation is exact).  The row form then holds the quarter table
+    // exp(-2 pi i q / n), q < n/4, of its real-FFT split.  Last, the DCT
+    // phases of k <= n/2 (phase n - k is -i conj(phase k): the real and
+    // imaginary parts swapped and negated).
+    const int m = fft_row_form(n) ? n / 2 : n;
+    for (int L = 1; L < m / 8; L *= 8)
+      for (int r = 1; r < 8; ++r)
+        for (int j = 0; j < L; ++j) put(size_t(L - 1 + (r - 1) * L + j), 2.0L * pi * (j * r) / (long double)(L * 8));
+    size_t e = size_t(m / 8 - 1);
+    for (int q = 0; q < m / 4; ++q) put(e++, 2.0L * pi * q / (long double)m);
+    if (fft_row_form(n))
+      for (int q = 0; q < n / 4; ++q) put(e++, 2.0L * pi * q / (long double)n);
+    for (int t = 0; t <= n / 2; ++t) put(e++, pi * t / (2.0L * n));
+  } else {
+    for (int rest = n, L = 1; rest > 1;) {
+      const int R = stage_radix(rest);
       for (int r = 1; r < R; ++r)
         for (int j = 0; j < L; ++j) put(size_t(L - 1 + (r - 1) * L + j), 2.0L * pi * (j * r) / (long double)(L * R));
-    L *= R;
-    rest /= R;
-  }
-  if (re) {
-    // register form: after the earlier stages' n / R_last - 1 entries, the
-    // quarter table exp(-2 pi i m / n), m < n/4 (the last stage's twiddle
-    // exp(-2 pi i r j / n) is (-i)^(rj div n/4) times entry rj mod n/4, the
-    // rotation exact), then the DCT phases of k <= n/2 (phase n - k is
-    // -i conj(phase k): the real and imaginary parts swapped and negated)
-    const size_t q0 = size_t(n / reg[re / 16][2] - 1), p0 = q0 + size_t(n / 4);
-    for (int m = 0; m < n / 4; ++m) put(q0 + size_t(m), 2.0L * pi * m / (long double)n);
-    for (int t = 0; t <= n / 2; ++t) put(p0 + size_t(t), pi * t / (2.0L * n));
-  } else {
+      L *= R;
+      rest /= R;
+    }
     for (int t = 0; t < n; ++t) put(size_t(n) + size_t(t), pi * t / (2.0L * n));
   }
   OTM_HIP(hipMalloc(&table, h.size() * sizeof(double)));
@@ -591,22 +594,21 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Register form (n = 512 and 1024, fft_reg_e(n) = E = n / 64): one wave per
-// complex sequence (two grid rows), 64-thread workgroups, lane t holding the
-// E points t + 64 v (v < E) of its sequence in registers.  A stage of radix R
-// after span L gives lane t the butterflies b = t + 64 u (u < E / R), whose
-// inputs b + r n / R are its own slots u + r E / R; its outputs go through
-// the wave's LDS exchange buffer back into the same layout, except in the
-// last stage (L R = n), where they land in the lane's own slots.  Radices 8,
-// 8, 8 (n = 512) and 16, 16, 4 (n = 1024): two exchanges per FFT instead of
-// a workgroup barrier after each of five radix-8/4/2 stages.  The DCT
-// post/pre-processing pairs frequency k with n - k through one more
+// Register form (n = 512, fft_reg_e(n) = E = n / 64 = 8): one wave per
+// complex sequence (two grid rows), lane t holding the E points t + 64 v
+// (v < E) of its sequence in registers.  A stage of radix R after span L
+// gives lane t the butterflies b = t + 64 u (u < E / R), whose inputs
+// b + r n / R are its own slots u + r E / R; its outputs go through the
+// wave's LDS exchange buffer back into the same layout, except in the last
+// stage (L R = n), where they land in the lane's own slots.  Radices 8, 8,
+// 8: two exchanges per FFT instead of a workgroup barrier after each stage.
+// The DCT post/pre-processing pairs frequency k with n - k through one more
 // exchange.  The exchange buffer's positions are padded, p + p / 16, so the
 // lanes of a 16-lane group write distinct banks.  A workgroup holds W waves,
-// one sequence each (W = 1 for a single grid, whose n / 2 sequences fill the
-// chip one wave per SIMD pair; W = 2 in batches), with W exchange buffers
-// and one copy of the twiddle and phase table (RegTab), staged in LDS at the
-// start.
+// one sequence each (W = 1 for a single grid, W = 2 in batches), with W
+// exchange buffers and one copy of the twiddle and phase table (RegTab),
+// staged in LDS at the start.  The same register FFT serves the row form
+// below (n = 1024) as its n/2-point FFT.
 // ---------------------------------------------------------------------------
 template <int E>
 struct RegRadix;
@@ -614,16 +616,12 @@ template <>
 struct RegRadix<8> {
   static constexpr int R0 = 8, R1 = 8, R2 = 8;
 };
-template <>
-struct RegRadix<16> {
-  static constexpr int R0 = 16, R1 = 16, R2 = 4;
-};
 
 // The table of the register form (fft_plan_create): the stage-major twiddles
 // of the stages before the last (entries 0 .. Q0 - 1), the quarter table
 // e^{-2 pi i m / N}, m < N/4, for the last stage, the DCT phases
-// e^{-i pi k / 2N}, k <= N/2.  16 KB at N = 1024 (instead of the 32 KB of the
-// full stage and phase tables), so four 1-wave workgroups fit one CU's LDS.
+// e^{-i pi k / 2N}, k <= N/2: 7 KB at N = 512 instead of the 16 KB of the
+// full stage and phase tables.
 template <int E>
 struct RegTab {
   static constexpr int N = 64 * E;
@@ -654,36 +652,6 @@ __device__ inline cd reg_phase(const cd* tab, int v, int k) {
 
 __device__ inline int xpad(int p) { return p + (p >> 4); }
 
-// DFT of 16 points as 4 x 4: DFT4 over n1 of a[n2 + 4 n1], times w16^(n2 k1),
-// DFT4 over n2; X[k1 + 4 k2] lands in c[k1 + 4 k2]
-template <int SIGN>
-__device__ inline cd w16mul(cd y, int m) {
-  const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;  // cos, sin(pi/8)
-  const double h = 0.70710678118654752440;
-  switch (m) {
-    case 1: return cmul(y, cd{c1, SIGN * s1});
-    case 2: return cmul(y, cd{h, SIGN * h});
-    case 3: return cmul(y, cd{s1, SIGN * c1});
-    case 4: return mul_i(y, SIGN);
-    case 6: return cmul(y, cd{-h, SIGN * h});
-    case 9: return cmul(y, cd{-c1, -SIGN * s1});
-    default: return y;
-  }
-}
-template <int SIGN>
-__device__ inline void dft16(const cd (&a)[16], cd (&c)[16]) {
-  cd y[4][4];
-#pragma unroll
-  for (int n2 = 0; n2 < 4; ++n2)
-    dft4<SIGN>(a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12], y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
-#pragma unroll
-  for (int n2 = 1; n2 < 4; ++n2)
-#pragma unroll
-    for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = w16mul<SIGN>(y[n2][k1], n2 * k1);
-#pragma unroll
-  for (int k1 = 0; k1 < 4; ++k1) dft4<SIGN>(y[0][k1], y[1][k1], y[2][k1], y[3][k1], c[k1], c[k1 + 4], c[k1 + 8], c[k1 + 12]);
-}
-
 template <int E, int R, int L, int SIGN>
 __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
   constexpr int N = 64 * E, U = E / R;
@@ -703,9 +671,9 @@ __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
         a[r] = cmul(a[r], SIGN < 0 ? w : conjc(w));
       }
     }
+    static_assert(R == 4 || R == 8, "radix 4 or 8");
     if constexpr (R == 4) dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
-    else if constexpr (R == 8) dft8<SIGN>(a, c);
-    else dft16<SIGN>(a, c);
+    else dft8<SIGN>(a, c);
     if constexpr (LAST) {
 #pragma unroll
       for (int q = 0; q < R; ++q) x[u + q * U] = c[q];  // position b + q L = t + 64 (u + q U)
@@ -742,22 +710,31 @@ struct RegPass {
   static constexpr int TPT = reg_tpt(RegTab<E>::SIZE, W);
   static constexpr size_t lds_bytes() { return size_t(W * XB + RegTab<E>::SIZE) * sizeof(cd); }
 };
-template <int E, int W>
-__device__ inline void reg_load_table(cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], const double* table) {
+// the SIZE-entry table into registers (all 64 W threads), then into LDS
+template <int SIZE, int W>
+__device__ inline void tab_load(cd (&tv)[reg_tpt(SIZE, W)], const double* table) {
   const auto tg = gp(reinterpret_cast<const cd*>(table));
 #pragma unroll
-  for (int q = 0; q < RegPass<E, W>::TPT; ++q) {
+  for (int q = 0; q < reg_tpt(SIZE, W); ++q) {
     const int e = int(threadIdx.x) + 64 * W * q;
-    tv[q] = e < RegTab<E>::SIZE ? tg[e] : cd{0., 0.};
+    tv[q] = e < SIZE ? tg[e] : cd{0., 0.};
+  }
+}
+template <int SIZE, int W>
+__device__ inline void tab_store(const cd (&tv)[reg_tpt(SIZE, W)], cd* tab) {
+#pragma unroll
+  for (int q = 0; q < reg_tpt(SIZE, W); ++q) {
+    const int e = int(threadIdx.x) + 64 * W * q;
+    if (e < SIZE) tab[e] = tv[q];
   }
 }
 template <int E, int W>
+__device__ inline void reg_load_table(cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], const double* table) {
+  tab_load<RegTab<E>::SIZE, W>(tv, table);
+}
+template <int E, int W>
 __device__ inline void reg_store_table(const cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], cd* tab) {
-#pragma unroll
-  for (int q = 0; q < RegPass<E, W>::TPT; ++q) {
-    const int e = int(threadIdx.x) + 64 * W * q;
-    if (e < RegTab<E>::SIZE) tab[e] = tv[q];
-  }
+  tab_store<RegTab<E>::SIZE, W>(tv, tab);
 }
 // two real rows r0, r0 + 1 as the permuted complex sequence z = v_a + i v_b,
 // v[m] = x[2m], v[n-1-m] = x[2m+1]: lane t slot v holds z[t + 64 v]
@@ -994,6 +971,322 @@ void launch_reg_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_
   if (js.total[kind] == 0) return;
   if (fft_reg_waves(js.count) == 1) launch_reg<E, 1>(js, kind, pass, gate, s);
   else launch_reg<E, FFT_REG_W>(js, kind, pass, gate, s);
+}
+
+
+// ---------------------------------------------------------------------------
+// Row form (n = 1024, launch kind KIND_FFTW): one wave per grid row, four
+// rows per workgroup, so a single 1024^2 grid runs 1024 waves, one per SIMD,
+// each with half the FFT work of a two-row wave.  The passes with a
+// transposed output stage their four rows in LDS and write 32-byte runs
+// (two lanes each), half the store line touches of the two-row form.  The DCT
+// of a real row x needs V = FFT_n(v) of the real sequence v (Makhoul's order,
+// v[j] = x[2j], v[n-1-j] = x[2j+1]); it comes from the n/2-point complex FFT
+// Z of z[m] = v[2m] + i v[2m+1] (register form, 8 points per lane, radices
+// 8, 8, 8) and the split
+//   E_k = (Z[k] + conj Z[n/2-k]) / 2,  O_k = (Z[k] - conj Z[n/2-k]) / 2i,
+//   V[k] = E_k + w^k O_k,  V[k + n/2] = E_k - w^k O_k,  w = e^{-2 pi i/n}.
+// The inverse (V Hermitian, v real) joins E_k = (V[k] + V[k+n/2]) / 2 and
+// O_k = (V[k] - V[k+n/2]) conj(w^k) / 2 into Z[k] = E_k + i O_k, runs the
+// inverse n/2-point FFT and reads z[m] / (n/2) = v[2m] + i v[2m+1].  Lane t
+// holds m = t + 64 u (u < 8) and the frequencies k = t + 64 u and k + n/2;
+// frequency k pairs with n/2 - k (one exchange), which also holds n - k.
+// z[m] holds x[4m], x[4m+2] for m < n/4 and x[2n-1-4m], x[2n-3-4m] above.
+// ---------------------------------------------------------------------------
+struct RowTab {
+  static constexpr int N = 1024, M = N / 2;
+  static constexpr int WQ0 = RegTab<8>::PH0;  // after the n/2-point FFT's table: w^q, q < n/4
+  static constexpr int PH0 = WQ0 + N / 4;     // DCT phases e^{-i pi k / 2n}, k <= n/2
+  static constexpr int SIZE = PH0 + N / 2 + 1;
+  static constexpr int XB = M + M / 16;  // exchange buffer per wave
+  static constexpr int SN = N + 4;       // staging row stride (doubles), over the exchange buffers
+  static constexpr size_t lds_bytes(int w) { return size_t(w * XB + SIZE) * sizeof(cd); }
+};
+static_assert(RowTab::SN <= 2 * RowTab::XB, "a staging row fits its wave's exchange buffer");
+static_assert(RegTab<8>::N == RowTab::M, "the row form's FFT is the 8-point-per-lane register FFT");
+// w^k = e^{-2 pi i k / n}, k < n/2
+__device__ inline cd row_w(const cd* tab, int k) {
+  const cd q = tab[RowTab::WQ0 + (k & (RowTab::N / 4 - 1))];
+  return k >= RowTab::N / 4 ? cd{q.y, -q.x} : q;
+}
+// phases of k and k + n/2 (the latter as phase n/2 - k swapped and negated)
+__device__ inline cd row_phase_lo(const cd* tab, int k) { return tab[RowTab::PH0 + k]; }
+__device__ inline cd row_phase_hi(const cd* tab, int k) {
+  const cd p = tab[RowTab::PH0 + RowTab::M - k];
+  return cd{-p.y, -p.x};
+}
+// the row's z in lane t (position of Re z[m] and of Im z[m] in the row)
+__device__ inline int row_c0(int u, int m) { return u < 4 ? 4 * m : 2 * RowTab::N - 1 - 4 * m; }
+__device__ inline int row_c1(int u, int m) { return u < 4 ? 4 * m + 2 : 2 * RowTab::N - 3 - 4 * m; }
+__device__ inline void row_load(cd (&x)[8], const double* in_, int row, int t) {
+  const auto in = gp(in_) + long(row) * RowTab::N;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int m = t + 64 * u;
+    x[u] = cd{in[row_c0(u, m)], in[row_c1(u, m)]};
+  }
+}
+// DCT-II coefficients X[k], X[k + n/2] from Z[k] and Zp = Z[(n/2 - k) mod n/2]
+__device__ inline void row_post(cd Z, cd Zp, cd w, cd plo, cd phi, double slo, double shi, double& xlo,
+                                double& xhi) {
+  const cd Zc = conjc(Zp);
+  const cd e = {0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y)};
+  const cd d = csub(Z, Zc);
+  const cd o = {0.5 * d.y, -0.5 * d.x};  // (Z - Zc) / 2i
+  const cd wo = cmul(w, o);
+  const cd vlo = cadd(e, wo), vhi = csub(e, wo);
+  xlo = slo * (plo.x * vlo.x - plo.y * vlo.y);
+  xhi = shi * (phi.x * vhi.x - phi.y * vhi.y);
+}
+// Z[k] = E_k + i O_k of the inverse from the scaled coefficients U = X / s_k:
+// ulo = U[k], uhi = U[k + n/2], umk = U[n/2 - k], unk = U[n - k] (U[n] = 0);
+// V[k] = e^{i pi k/2n} (U[k] - i U[n-k])
+__device__ inline cd row_pre(double ulo, double uhi, double umk, double unk, cd w, cd plo, cd phi) {
+  const cd vlo = cmul(conjc(plo), cd{ulo, -unk});
+  const cd vhi = cmul(conjc(phi), cd{uhi, -umk});
+  const cd e = {0.5 * (vlo.x + vhi.x), 0.5 * (vlo.y + vhi.y)};
+  const cd o = cmul(cd{0.5 * (vlo.x - vhi.x), 0.5 * (vlo.y - vhi.y)}, conjc(w));
+  return {e.x - o.y, e.y + o.x};
+}
+// the wave's lane, row, exchange buffer and the workgroup's table
+template <int W>
+struct RowWave {
+  int t, row;
+  cd* xb;
+  cd* tab;
+  __device__ RowWave(cd* fsm, int lt) {
+    const int wv = int(threadIdx.x) >> 6;
+    t = int(threadIdx.x) & 63;
+    row = W * lt + wv;
+    xb = fsm + wv * RowTab::XB;
+    tab = fsm + W * RowTab::XB;
+  }
+  // the wave's staging row over the exchange buffers (row stride RowTab::SN)
+  __device__ double* stage(cd* fsm) const {
+    return reinterpret_cast<double*>(fsm) + (int(threadIdx.x) >> 6) * RowTab::SN;
+  }
+};
+// The transposed write of a pass: the W rows of the workgroup (first row
+// W lt) sit in the staging rows; output row k takes them as one W-value run
+// at column W lt.  Lane pairs write the two 16-byte halves of a run, so a
+// store instruction touches 32 lines instead of 64.
+template <int W>
+__device__ inline void row_store_transposed(const cd* fsm, double* out_, int lt) {
+  static_assert(W == 4, "two lanes per four-row run");
+  constexpr int N = RowTab::N, SN = RowTab::SN;
+  const double* sg = reinterpret_cast<const double*>(fsm);
+  const auto out = gp(out_) + W * lt;
+  const int q = int(threadIdx.x), h = q & 1;
+#pragma unroll
+  for (int v = 0; v < N / 128; ++v) {
+    const int k = (q >> 1) + 128 * v;
+    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + 2 * h) = cd{sg[2 * h * SN + k], sg[(2 * h + 1) * SN + k]};
+  }
+}
+
+// Pass 1 (row form): a row of R' -> DCT-II along j, written transposed.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_fftw_rows_fwd(const Job* __restrict__ jobs,
+                                                          const int* __restrict__ starts, int count, bool gate,
+                                                          bool rhs) {
+  constexpr int N = RowTab::N, M = RowTab::M;
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const RowWave<W> G(fsm, L.lt);
+  const int t = G.t, row = G.row;
+  cd tv[reg_tpt(RowTab::SIZE, W)];
+  tab_load<RowTab::SIZE, W>(tv, J.fft);
+  cd x[8];
+  row_load(x, J.f.r, row, t);
+  if (rhs && row == 0 && t == 0) x[0].x -= st->sum_r;  // R' = R - (sum r) e0 e0^T
+  if (gate && early && st->done) return;
+  tab_store<RowTab::SIZE, W>(tv, G.tab);
+  __syncthreads();
+  reg_fft<8, -1>(x, G.xb, G.tab, t);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = x[u];
+  __syncthreads();
+  const double s0 = sqrt(1.0 / N), s1 = sqrt(2.0 / N);
+  double xlo[8], xhi[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = t + 64 * u;
+    row_post(x[u], G.xb[xpad(k == 0 ? 0 : M - k)], row_w(G.tab, k), row_phase_lo(G.tab, k),
+             row_phase_hi(G.tab, k), k == 0 ? s0 : s1, s1, xlo[u], xhi[u]);
+  }
+  __syncthreads();  // every partner read is done: the exchange buffers become the staging rows
+  double* const sg = G.stage(fsm);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = t + 64 * u;
+    sg[k] = xlo[u];
+    sg[k + M] = xhi[u];
+  }
+  __syncthreads();
+  row_store_transposed<W>(fsm, J.f.tmp0, L.lt);
+}
+
+// Pass 2 (row form): a row of B^T, DCT-II along i, / lambda, DCT-III along
+// i, written transposed back.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_fftw_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
+                                                      int count, bool gate) {
+  constexpr int N = RowTab::N, M = RowTab::M;
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const RowWave<W> G(fsm, L.lt);
+  const int t = G.t, l = G.row;
+  cd tv[reg_tpt(RowTab::SIZE, W)];
+  tab_load<RowTab::SIZE, W>(tv, J.fft);
+  cd x[8];
+  row_load(x, J.f.tmp0, l, t);
+  // eigenvalue halves sin^2(pi |k-l| / 2n) + sin^2(pi (k+l) / 2n) of k and k + n/2
+  const auto sq = gp(J.sq);
+  double lam[2][8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = t + 64 * u + h * M;
+      const int dm = k > l ? k - l : l - k;
+      lam[h][u] = sq[dm] + sq[k + l];
+    }
+  if (gate && early && st->done) return;
+  tab_store<RowTab::SIZE, W>(tv, G.tab);
+  __syncthreads();
+  reg_fft<8, -1>(x, G.xb, G.tab, t);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = x[u];
+  __syncthreads();
+  const double s0 = sqrt(1.0 / N), s1 = sqrt(2.0 / N);
+  const double r0 = sqrt(double(N)), r1 = sqrt(0.5 * N);
+  // spectrum / lambda, scaled by 1 / s_k for the inverse: (U[k], U[k + n/2])
+  cd w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = t + 64 * u;
+    double xlo, xhi;
+    row_post(x[u], G.xb[xpad(k == 0 ? 0 : M - k)], row_w(G.tab, k), row_phase_lo(G.tab, k),
+             row_phase_hi(G.tab, k), k == 0 ? s0 : s1, s1, xlo, xhi);
+    const double ulo = (k == 0 && l == 0) ? 0. : xlo / (2. * lam[0][u]);
+    const double uhi = xhi / (2. * lam[1][u]);
+    w[u] = cd{ulo * (k == 0 ? r0 : r1), uhi * r1};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = w[u];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = t + 64 * u;
+    const cd q = G.xb[xpad(k == 0 ? 0 : M - k)];  // U[n/2 - k], U[n - k]
+    const double umk = k == 0 ? w[u].y : q.x, unk = k == 0 ? 0. : q.y;
+    x[u] = row_pre(w[u].x, w[u].y, umk, unk, row_w(G.tab, k), row_phase_lo(G.tab, k), row_phase_hi(G.tab, k));
+  }
+  __syncthreads();
+  reg_fft<8, 1>(x, G.xb, G.tab, t);  // ends reading its exchange buffer before a barrier
+  const double inv = 1.0 / M;
+  double* const sg = G.stage(fsm);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int m = t + 64 * u;
+    sg[row_c0(u, m)] = x[u].x * inv;
+    sg[row_c1(u, m)] = x[u].y * inv;
+  }
+  __syncthreads();
+  row_store_transposed<W>(fsm, J.f.tmp1, L.lt);
+}
+
+// Pass 3 (row form): DCT-III along j = d_hat, and the row's |d_hat|^2 partial.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_fftw_rows_inv(const Job* __restrict__ jobs,
+                                                          const int* __restrict__ starts, int count, bool gate) {
+  constexpr int N = RowTab::N, M = RowTab::M;
+  extern __shared__ cd fsm[];
+  const FftLoc L = fft_locate(starts, count);
+  const Job& J = jobs[L.p];
+  const auto st = gp((const SolveState*)J.f.state);
+  const bool early = count == 1;
+  if (gate && !early && st->done) return;
+  const RowWave<W> G(fsm, L.lt);
+  const int t = G.t, row = G.row;
+  cd tv[reg_tpt(RowTab::SIZE, W)];
+  tab_load<RowTab::SIZE, W>(tv, J.fft);
+  // X[k], X[k + n/2], X[n/2 - k], X[n - k] of the row (k = 0: X[n/2] and 0)
+  const auto in = gp((const double*)J.f.tmp1) + long(row) * N;
+  double xl[8], xh[8], xm[8], xn[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = t + 64 * u;
+    xl[u] = in[k];
+    xh[u] = in[k + M];
+    xm[u] = in[M - k];
+    xn[u] = k == 0 ? 0. : in[N - k];
+  }
+  if (gate && early && st->done) return;
+  tab_store<RowTab::SIZE, W>(tv, G.tab);
+  __syncthreads();
+  const double r0 = sqrt(double(N)), r1 = sqrt(0.5 * N);
+  cd x[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = t + 64 * u;
+    x[u] = row_pre(xl[u] * (k == 0 ? r0 : r1), xh[u] * r1, xm[u] * r1, xn[u] * r1, row_w(G.tab, k),
+                   row_phase_lo(G.tab, k), row_phase_hi(G.tab, k));
+  }
+  reg_fft<8, 1>(x, G.xb, G.tab, t);
+  const auto out = gp(J.f.dhat) + long(row) * N;
+  const double inv = 1.0 / M;
+  double nrm[1] = {0.};
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int m = t + 64 * u;
+    const double a = x[u].x * inv, b = x[u].y * inv;
+    out[row_c0(u, m)] = a;
+    out[row_c1(u, m)] = b;
+    nrm[0] += a * a;
+    nrm[0] += b * b;
+  }
+  wave_sum<1>(nrm);
+  if (t == 63) gp(J.f.site(SITE_DNORM))[row] = nrm[0];
+}
+
+template <int W>
+void launch_row(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  const size_t lds = RowTab::lds_bytes(W);
+  static std::atomic<unsigned long long> attr{0};
+  int dev = 0;
+  OTM_HIP(hipGetDevice(&dev));
+  const unsigned long long bit = 1ull << (dev & 63);
+  if (!(attr.load() & bit)) {
+    for (const void* k : {reinterpret_cast<const void*>(&k_fftw_rows_fwd<W>),
+                          reinterpret_cast<const void*>(&k_fftw_cols<W>),
+                          reinterpret_cast<const void*>(&k_fftw_rows_inv<W>)})
+      OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    attr.fetch_or(bit);
+  }
+  const dim3 grid(unsigned(js.total[kind])), block(64 * W);
+  const int* st = js.starts_of(kind);
+  const Job* jobs = js.jobs;
+  switch (pass) {
+    case 0: hipLaunchKernelGGL(k_fftw_rows_fwd<W>, grid, block, lds, s, jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL(k_fftw_cols<W>, grid, block, lds, s, jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL(k_fftw_rows_inv<W>, grid, block, lds, s, jobs, st, js.count, gate);
+  }
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_row_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
+  if (js.total[kind] == 0) return;
+  launch_row<FFT_ROW_W>(js, kind, pass, gate, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1311,7 +1604,7 @@ void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream
 // sized for the largest grid of each kind.
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
   launch_reg_kind<8>(js, KIND_FFTR8, pass, gate, s);
-  launch_reg_kind<16>(js, KIND_FFTR16, pass, gate, s);
+  launch_row_kind(js, KIND_FFTW, pass, gate, s);
   launch_pack_kind(js, KIND_FFT, pass, gate, s);
   launch_pass_kind<2>(js, KIND_FFT2, pass, gate, s);
 }
