@@ -1220,40 +1220,52 @@ __global__ __launch_bounds__(64 * W) void k_fftw_rows_inv(const Job* __restrict_
   const int t = G.t, row = G.row;
   cd tv[reg_tpt(RowTab::SIZE, W)];
   tab_load<RowTab::SIZE, W>(tv, J.fft);
-  // X[k], X[k + n/2], X[n/2 - k], X[n - k] of the row (k = 0: X[n/2] and 0)
+  // X[k], X[k + n/2] of the row; X[n/2 - k], X[n - k] come from the lane of
+  // n/2 - k through the exchange buffer (k = 0: X[n/2] and 0)
   const auto in = gp((const double*)J.f.tmp1) + long(row) * N;
-  double xl[8], xh[8], xm[8], xn[8];
+  double xl[8], xh[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int k = t + 64 * u;
     xl[u] = in[k];
     xh[u] = in[k + M];
-    xm[u] = in[M - k];
-    xn[u] = k == 0 ? 0. : in[N - k];
   }
   if (gate && early && st->done) return;
   tab_store<RowTab::SIZE, W>(tv, G.tab);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = cd{xl[u], xh[u]};
   __syncthreads();
   const double r0 = sqrt(double(N)), r1 = sqrt(0.5 * N);
   cd x[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int k = t + 64 * u;
-    x[u] = row_pre(xl[u] * (k == 0 ? r0 : r1), xh[u] * r1, xm[u] * r1, xn[u] * r1, row_w(G.tab, k),
+    const cd q = G.xb[xpad(k == 0 ? 0 : M - k)];
+    const double xm = k == 0 ? xh[u] : q.x, xn = k == 0 ? 0. : q.y;
+    x[u] = row_pre(xl[u] * (k == 0 ? r0 : r1), xh[u] * r1, xm * r1, xn * r1, row_w(G.tab, k),
                    row_phase_lo(G.tab, k), row_phase_hi(G.tab, k));
   }
-  reg_fft<8, 1>(x, G.xb, G.tab, t);
-  const auto out = gp(J.f.dhat) + long(row) * N;
+  __syncthreads();  // partner reads done before the FFT's first exchange
+  reg_fft<8, 1>(x, G.xb, G.tab, t);  // ends reading its exchange buffer before a barrier
+  // the row through the wave's exchange buffer, written in 16-byte runs
+  double* const sg = reinterpret_cast<double*>(G.xb);
   const double inv = 1.0 / M;
   double nrm[1] = {0.};
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int m = t + 64 * u;
     const double a = x[u].x * inv, b = x[u].y * inv;
-    out[row_c0(u, m)] = a;
-    out[row_c1(u, m)] = b;
+    sg[row_c0(u, m)] = a;
+    sg[row_c1(u, m)] = b;
     nrm[0] += a * a;
     nrm[0] += b * b;
+  }
+  __syncthreads();
+  const auto out = gp(J.f.dhat) + long(row) * N;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int c = 2 * (t + 64 * v);
+    *reinterpret_cast<OTM_GLOBAL cd*>(out + c) = cd{sg[c], sg[c + 1]};
   }
   wave_sum<1>(nrm);
   if (t == 63) gp(J.f.site(SITE_DNORM))[row] = nrm[0];
