@@ -537,7 +537,7 @@ def main():
         achieved = 16.0 * n * n / (t_ms * 1e-3) / 1e9
         # the column pass runs in register form for n = 512 and 1024 (one wave
         # per pair of columns), in the workgroup form at other sizes
-        cname = {512: "k_fftr_cols<8, 1>", 1024: "k_fftw_cols<4>"}.get(n, "k_fft_cols<1>" if n <= 1024 else "k_fft_cols<2>")
+        cname = {512: "k_fftw_cols<4, 4>", 1024: "k_fftw_cols<8, 4>"}.get(n, "k_fft_cols<1>" if n <= 1024 else "k_fft_cols<2>")
         roof_fft = {"kernel": f"{cname}: fast-DCT column pass (DCT-II, / lambda, DCT-III of n columns), "
                               "one read + one write of the n x n FP64 field",
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -65,24 +65,23 @@ void fft_plan_create(double*& table, int n) {
     h[2 * e] = double(cosl(a));
     h[2 * e + 1] = double(-sinl(a));
   };
-  if (fft_reg_e(n) || fft_row_form(n)) {
-    // Register forms.  Their m-point complex FFT (m = n for the 2-row form,
-    // n/2 for the row form; radices 8, 8, 8): the twiddles of the first two
-    // stages by stage as above (entries 0 .. 62), then the quarter table
+  if (fft_row_form(n)) {
+    // Row form.  Its m = n/2-point complex FFT (radix 8 for m = 512, 4 for
+    // m = 256): the twiddles of the stages before the last by stage as above
+    // (entries 0 .. m/R - 2), then the quarter table
     // exp(-2 pi i q / m), q < m/4, from which the last stage takes its
     // twiddle exp(-2 pi i r j / m) as (-i)^(rj div m/4) times entry rj mod
-    // m/4 (the rotation is exact).  The row form then holds the quarter table
-    // exp(-2 pi i q / n), q < n/4, of its real-FFT split.  Last, the DCT
+    // m/4 (the rotation is exact).  Then the quarter table exp(-2 pi i q / n),
+    // q < n/4, of the real-FFT split.  Last, the DCT
     // phases of k <= n/2 (phase n - k is -i conj(phase k): the real and
     // imaginary parts swapped and negated).
-    const int m = fft_row_form(n) ? n / 2 : n;
-    for (int L = 1; L < m / 8; L *= 8)
-      for (int r = 1; r < 8; ++r)
-        for (int j = 0; j < L; ++j) put(size_t(L - 1 + (r - 1) * L + j), 2.0L * pi * (j * r) / (long double)(L * 8));
-    size_t e = size_t(m / 8 - 1);
+    const int m = n / 2, R = m == 256 ? 4 : 8;
+    for (int L = 1; L < m / R; L *= R)
+      for (int r = 1; r < R; ++r)
+        for (int j = 0; j < L; ++j) put(size_t(L - 1 + (r - 1) * L + j), 2.0L * pi * (j * r) / (long double)(L * R));
+    size_t e = size_t(m / R - 1);
     for (int q = 0; q < m / 4; ++q) put(e++, 2.0L * pi * q / (long double)m);
-    if (fft_row_form(n))
-      for (int q = 0; q < n / 4; ++q) put(e++, 2.0L * pi * q / (long double)n);
+    for (int q = 0; q < n / 4; ++q) put(e++, 2.0L * pi * q / (long double)n);
     for (int t = 0; t <= n / 2; ++t) put(e++, pi * t / (2.0L * n));
   } else {
     for (int rest = n, L = 1; rest > 1;) {
@@ -594,40 +593,36 @@ __global__ __launch_bounds__(256 * FS) void k_fft_rows_inv(const Job* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Register form (n = 512, fft_reg_e(n) = E = n / 64 = 8): one wave per
-// complex sequence (two grid rows), lane t holding the E points t + 64 v
-// (v < E) of its sequence in registers.  A stage of radix R after span L
-// gives lane t the butterflies b = t + 64 u (u < E / R), whose inputs
-// b + r n / R are its own slots u + r E / R; its outputs go through the
-// wave's LDS exchange buffer back into the same layout, except in the last
-// stage (L R = n), where they land in the lane's own slots.  Radices 8, 8,
-// 8: two exchanges per FFT instead of a workgroup barrier after each stage.
-// The DCT post/pre-processing pairs frequency k with n - k through one more
-// exchange.  The exchange buffer's positions are padded, p + p / 16, so the
-// lanes of a 16-lane group write distinct banks.  A workgroup holds W waves,
-// one sequence each (W = 1 for a single grid, W = 2 in batches), with W
-// exchange buffers and one copy of the twiddle and phase table (RegTab),
-// staged in LDS at the start.  The same register FFT serves the row form
-// below (n = 1024) as its n/2-point FFT.
+// Register FFT (the row form below runs it per grid row): one wave per
+// complex sequence of N = 64 E points, lane t holding the E points t + 64 v
+// (v < E) in registers.  A stage of radix R after span L gives lane t the
+// butterflies b = t + 64 u (u < E / R), whose inputs b + r N / R are its own
+// slots u + r E / R; its outputs go through the wave's LDS exchange buffer
+// back into the same layout, except in the last stage (L R = N), where they
+// land in the lane's own slots.  Radices 8, 8, 8 (N = 512) or 4, 4, 4, 4
+// (N = 256): two or three exchanges per FFT instead of a workgroup barrier
+// after each stage.  The exchange buffer's positions are padded, p + p / 16,
+// so the lanes of a 16-lane group write distinct banks.
 // ---------------------------------------------------------------------------
 template <int E>
 struct RegRadix;
 template <>
-struct RegRadix<8> {
-  static constexpr int R0 = 8, R1 = 8, R2 = 8;
+struct RegRadix<8> {  // 512 points: 8 x 8 x 8
+  static constexpr int R = 8, LAST = 8;
+};
+template <>
+struct RegRadix<4> {  // 256 points: 4 x 4 x 4 x 4
+  static constexpr int R = 4, LAST = 4;
 };
 
-// The table of the register form (fft_plan_create): the stage-major twiddles
-// of the stages before the last (entries 0 .. Q0 - 1), the quarter table
-// e^{-2 pi i m / N}, m < N/4, for the last stage, the DCT phases
-// e^{-i pi k / 2N}, k <= N/2: 7 KB at N = 512 instead of the 16 KB of the
-// full stage and phase tables.
+// The register FFT's twiddles (fft_plan_create): the stage-major twiddles
+// of the stages before the last (entries 0 .. Q0 - 1) and the quarter table
+// e^{-2 pi i m / N}, m < N/4, for the last stage (PH0: the next entry).
 template <int E>
 struct RegTab {
   static constexpr int N = 64 * E;
-  static constexpr int Q0 = N / RegRadix<E>::R2 - 1;
+  static constexpr int Q0 = N / RegRadix<E>::LAST - 1;
   static constexpr int PH0 = Q0 + N / 4;
-  static constexpr int SIZE = PH0 + N / 2 + 1;
 };
 // the last stage's twiddle e^{-2 pi i m / N}, m < N: (-i)^(m div N/4) times
 // the quarter-table entry m mod N/4 (the rotation is exact)
@@ -640,16 +635,6 @@ __device__ inline cd reg_twiddle(const cd* tab, int m) {
   if (s & 2) w = cd{-w.x, -w.y};
   return w;
 }
-// the DCT phase e^{-i pi k / 2N} of frequency k = t + 64 v (v a constant after
-// unrolling): entries k <= N/2 directly, above as phase N - k swapped and negated
-template <int E>
-__device__ inline cd reg_phase(const cd* tab, int v, int k) {
-  constexpr int N = 64 * E;
-  if (v < E / 2) return tab[RegTab<E>::PH0 + k];
-  const cd p = tab[RegTab<E>::PH0 + N - k];
-  return cd{-p.y, -p.x};
-}
-
 __device__ inline int xpad(int p) { return p + (p >> 4); }
 
 template <int E, int R, int L, int SIGN>
@@ -690,26 +675,16 @@ __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
   }
 }
 
-template <int E, int SIGN>
+// the stages of radix R after spans L = 1, R, R^2, ... up to 64 E
+template <int E, int SIGN, int L = 1>
 __device__ inline void reg_fft(cd (&x)[E], cd* xb, const cd* tw, int t) {
-  using RR = RegRadix<E>;
-  rstage<E, RR::R0, 1, SIGN>(x, xb, tw, t);
-  rstage<E, RR::R1, RR::R0, SIGN>(x, xb, tw, t);
-  rstage<E, RR::R2, RR::R0 * RR::R1, SIGN>(x, xb, tw, t);
+  constexpr int R = RegRadix<E>::R;
+  rstage<E, R, L, SIGN>(x, xb, tw, t);
+  if constexpr (L * R < 64 * E) reg_fft<E, SIGN, L * R>(x, xb, tw, t);
 }
 
-// Common start of a register-form pass: the job, the gate, the table loads.
-// LDS: W exchange buffers (N + N/16 complex each), then the table (RegTab),
-// loaded by all 64 W threads.
 // table entries per thread of a W-wave workgroup
 constexpr int reg_tpt(int size, int w) { return (size + 64 * w - 1) / (64 * w); }
-template <int E, int W>
-struct RegPass {
-  static constexpr int N = 64 * E;
-  static constexpr int XB = N + N / 16;
-  static constexpr int TPT = reg_tpt(RegTab<E>::SIZE, W);
-  static constexpr size_t lds_bytes() { return size_t(W * XB + RegTab<E>::SIZE) * sizeof(cd); }
-};
 // the SIZE-entry table into registers (all 64 W threads), then into LDS
 template <int SIZE, int W>
 __device__ inline void tab_load(cd (&tv)[reg_tpt(SIZE, W)], const double* table) {
@@ -728,302 +703,65 @@ __device__ inline void tab_store(const cd (&tv)[reg_tpt(SIZE, W)], cd* tab) {
     if (e < SIZE) tab[e] = tv[q];
   }
 }
-template <int E, int W>
-__device__ inline void reg_load_table(cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], const double* table) {
-  tab_load<RegTab<E>::SIZE, W>(tv, table);
-}
-template <int E, int W>
-__device__ inline void reg_store_table(const cd (&tv)[reg_tpt(RegTab<E>::SIZE, W)], cd* tab) {
-  tab_store<RegTab<E>::SIZE, W>(tv, tab);
-}
-// two real rows r0, r0 + 1 as the permuted complex sequence z = v_a + i v_b,
-// v[m] = x[2m], v[n-1-m] = x[2m+1]: lane t slot v holds z[t + 64 v]
-template <int E>
-__device__ inline void reg_load_rows(cd (&x)[E], const double* in_, int r0, int t) {
-  constexpr int N = 64 * E;
-  const auto in = gp(in_) + long(r0) * N;
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int m = t + 64 * v;
-    const int c = m < N / 2 ? 2 * m : 2 * N - 1 - 2 * m;
-    x[v] = cd{in[c], in[N + c]};
-  }
-}
-// The wave's lane t, its exchange buffer and first row r0 (sequence W lt + wave)
-template <int E, int W>
-struct RegWave {
-  int t, r0;
-  cd* xb;
-  cd* tab;
-  __device__ RegWave(cd* fsm, int lt) {
-    const int wv = int(threadIdx.x) >> 6;
-    t = int(threadIdx.x) & 63;
-    r0 = 2 * (W * lt + wv);
-    xb = fsm + wv * RegPass<E, W>::XB;
-    tab = fsm + W * RegPass<E, W>::XB;
-  }
-};
-
-// Pass 1 (register form): rows of R' -> DCT-II along j, written transposed.
-template <int E, int W>
-__global__ __launch_bounds__(64 * W) void k_fftr_rows_fwd(const Job* __restrict__ jobs,
-                                                          const int* __restrict__ starts, int count, bool gate,
-                                                          bool rhs) {
-  using P = RegPass<E, W>;
-  constexpr int N = P::N;
-  extern __shared__ cd fsm[];
-  const FftLoc L = fft_locate(starts, count);
-  const Job& J = jobs[L.p];
-  const auto st = gp((const SolveState*)J.f.state);
-  const bool early = count == 1;
-  if (gate && !early && st->done) return;
-  const RegWave<E, W> G(fsm, L.lt);
-  const int t = G.t, r0 = G.r0;
-  cd* xb = G.xb;
-  cd tv[P::TPT];
-  reg_load_table<E, W>(tv, J.fft);
-  cd x[E];
-  reg_load_rows<E>(x, J.f.r, r0, t);
-  if (rhs && r0 == 0 && t == 0) x[0].x -= st->sum_r;  // R' = R - (sum r) e0 e0^T
-  if (gate && early && st->done) return;
-  reg_store_table<E, W>(tv, G.tab);
-  __syncthreads();
-  reg_fft<E, -1>(x, xb, G.tab, t);
-#pragma unroll
-  for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = x[v];
-  __syncthreads();
-  const Scales sc(N);
-  const auto out = gp(J.f.tmp0);
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int k = t + 64 * v;
-    double xa, xo;
-    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], reg_phase<E>(G.tab, v, k), sc.s(k), xa, xo);
-    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(k) * N + r0) = cd{xa, xo};
-  }
-}
-
-// Pass 2 (register form): rows of B^T, DCT-II along i, / lambda, DCT-III
-// along i, written transposed back.
-template <int E, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 1 : 2))) void k_fftr_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
-                                                      int count, bool gate) {
-  using P = RegPass<E, W>;
-  constexpr int N = P::N;
-  extern __shared__ cd fsm[];
-  const FftLoc L = fft_locate(starts, count);
-  const Job& J = jobs[L.p];
-  const auto st = gp((const SolveState*)J.f.state);
-  const bool early = count == 1;
-  if (gate && !early && st->done) return;
-  const RegWave<E, W> G(fsm, L.lt);
-  const int t = G.t, l0 = G.r0;
-  cd* xb = G.xb;
-  cd tv[P::TPT];
-  reg_load_table<E, W>(tv, J.fft);
-  cd x[E];
-  reg_load_rows<E>(x, J.f.tmp0, l0, t);
-  // eigenvalue halves sin^2(pi |k-l| / 2N) + sin^2(pi (k+l) / 2N) of this
-  // lane's (k, l): loaded with the rows for a single grid; in batches, where
-  // other waves hide the latency, after the forward FFT, so that the 2E
-  // values are not held through it (two waves per SIMD)
-  const auto sq = gp(J.sq);
-  double lam[2][E];
-  auto load_lam = [&]() {
-#pragma unroll
-    for (int v = 0; v < E; ++v)
-#pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const int k = t + 64 * v, l = l0 + rr;
-        const int dm = k > l ? k - l : l - k;
-        lam[rr][v] = sq[dm] + sq[k + l];
-      }
-  };
-  if constexpr (W == 1) load_lam();
-  if (gate && early && st->done) return;
-  reg_store_table<E, W>(tv, G.tab);
-  __syncthreads();
-  reg_fft<E, -1>(x, xb, G.tab, t);
-  if constexpr (W > 1) load_lam();
-#pragma unroll
-  for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = x[v];
-  __syncthreads();
-  const Scales sc(N);
-  // spectrum / lambda, scaled by 1 / s_k for the inverse: (U_a[k], U_b[k])
-  cd w[E];
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int k = t + 64 * v;
-    double xa, xo;
-    dct2_post_pair(x[v], xb[xpad(k == 0 ? 0 : N - k)], reg_phase<E>(G.tab, v, k), sc.s(k), xa, xo);
-    const double Xa = (k == 0 && l0 == 0) ? 0. : xa / (2. * lam[0][v]);
-    const double Xb = xo / (2. * lam[1][v]);
-    w[v] = cd{Xa * sc.r(k), Xb * sc.r(k)};
-  }
-  __syncthreads();
-#pragma unroll
-  for (int v = 0; v < E; ++v) xb[xpad(t + 64 * v)] = w[v];
-  __syncthreads();
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int k = t + 64 * v;
-    const cd wn = k == 0 ? cd{0., 0.} : xb[xpad(N - k)];
-    x[v] = dct3_pre_u(w[v].x, wn.x, w[v].y, wn.y, reg_phase<E>(G.tab, v, k));
-  }
-  __syncthreads();
-  reg_fft<E, 1>(x, xb, G.tab, t);
-  const auto out = gp(J.f.tmp1);
-  const double inv = 1.0 / N;
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int m = t + 64 * v;
-    const int i = m < N / 2 ? 2 * m : 2 * (N - 1 - m) + 1;  // perm_pos(i) = m
-    *reinterpret_cast<OTM_GLOBAL cd*>(out + long(i) * N + l0) = cd{x[v].x * inv, x[v].y * inv};
-  }
-}
-
-// Pass 3 (register form): DCT-III along j = d_hat, and the |d_hat|^2 partial
-// of each sequence (row pair).
-template <int E, int W>
-__global__ __launch_bounds__(64 * W) void k_fftr_rows_inv(const Job* __restrict__ jobs,
-                                                          const int* __restrict__ starts, int count, bool gate) {
-  using P = RegPass<E, W>;
-  constexpr int N = P::N;
-  extern __shared__ cd fsm[];
-  const FftLoc L = fft_locate(starts, count);
-  const Job& J = jobs[L.p];
-  const auto st = gp((const SolveState*)J.f.state);
-  const bool early = count == 1;
-  if (gate && !early && st->done) return;
-  const RegWave<E, W> G(fsm, L.lt);
-  const int t = G.t, r0 = G.r0;
-  cd tv[P::TPT];
-  reg_load_table<E, W>(tv, J.fft);
-  // X[row][k] and X[row][N-k] of the two rows
-  const auto in = gp((const double*)J.f.tmp1) + long(r0) * N;
-  double xk[2][E], xn[2][E];
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int k = t + 64 * v;
-    const int kn = k == 0 ? 0 : N - k;
-#pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-      xk[rr][v] = in[rr * N + k];
-      xn[rr][v] = in[rr * N + kn];
-    }
-  }
-  if (gate && early && st->done) return;
-  reg_store_table<E, W>(tv, G.tab);
-  const Scales sc(N);
-  __syncthreads();
-  cd x[E];
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int k = t + 64 * v;
-    x[v] = dct3_pre(xk[0][v], xn[0][v], xk[1][v], xn[1][v], reg_phase<E>(G.tab, v, k), sc, k);
-  }
-  reg_fft<E, 1>(x, G.xb, G.tab, t);
-  const auto out = gp(J.f.dhat) + long(r0) * N;
-  const double inv = 1.0 / N;
-  double nrm[1] = {0.};
-#pragma unroll
-  for (int v = 0; v < E; ++v) {
-    const int m = t + 64 * v;
-    const int c = m < N / 2 ? 2 * m : 2 * (N - 1 - m) + 1;
-    const double a = x[v].x * inv, b = x[v].y * inv;
-    out[c] = a;
-    out[N + c] = b;
-    nrm[0] += a * a;
-    nrm[0] += b * b;
-  }
-  wave_sum<1>(nrm);
-  if (t == 63) gp(J.f.site(SITE_DNORM))[r0 / 2] = nrm[0];
-}
-
-template <int E, int W>
-void launch_reg(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
-  const size_t lds = RegPass<E, W>::lds_bytes();
-  static std::atomic<unsigned long long> attr{0};
-  int dev = 0;
-  OTM_HIP(hipGetDevice(&dev));
-  const unsigned long long bit = 1ull << (dev & 63);
-  if (!(attr.load() & bit)) {
-    for (const void* k : {reinterpret_cast<const void*>(&k_fftr_rows_fwd<E, W>),
-                          reinterpret_cast<const void*>(&k_fftr_cols<E, W>),
-                          reinterpret_cast<const void*>(&k_fftr_rows_inv<E, W>)})
-      OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    attr.fetch_or(bit);
-  }
-  const dim3 grid(unsigned(js.total[kind])), block(64 * W);
-  const int* st = js.starts_of(kind);
-  const Job* jobs = js.jobs;
-  switch (pass) {
-    case 0: hipLaunchKernelGGL((k_fftr_rows_fwd<E, W>), grid, block, lds, s, jobs, st, js.count, gate, true); break;
-    case 1: hipLaunchKernelGGL((k_fftr_cols<E, W>), grid, block, lds, s, jobs, st, js.count, gate); break;
-    default: hipLaunchKernelGGL((k_fftr_rows_inv<E, W>), grid, block, lds, s, jobs, st, js.count, gate);
-  }
-  OTM_HIP(hipGetLastError());
-}
-
-// workgroups of fft_reg_waves(js.count) waves (job_starts counts them)
-template <int E>
-void launch_reg_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
-  if (js.total[kind] == 0) return;
-  if (fft_reg_waves(js.count) == 1) launch_reg<E, 1>(js, kind, pass, gate, s);
-  else launch_reg<E, FFT_REG_W>(js, kind, pass, gate, s);
-}
-
 
 // ---------------------------------------------------------------------------
-// Row form (n = 1024, launch kind KIND_FFTW): one wave per grid row, four
-// rows per workgroup, so a single 1024^2 grid runs 1024 waves, one per SIMD,
-// each with half the FFT work of a two-row wave.  The passes with a
+// Row form (n = 1024 and 512, launch kinds KIND_FFTW8 and KIND_FFTW4): one
+// wave per grid row, four rows per workgroup, so a single 1024^2 grid runs
+// 1024 waves, one per SIMD, each with half the FFT work of a two-row wave.  The passes with a
 // transposed output stage their four rows in LDS and write 32-byte runs
 // (two lanes each), half the store line touches of the two-row form.  The DCT
 // of a real row x needs V = FFT_n(v) of the real sequence v (Makhoul's order,
 // v[j] = x[2j], v[n-1-j] = x[2j+1]); it comes from the n/2-point complex FFT
-// Z of z[m] = v[2m] + i v[2m+1] (register form, 8 points per lane, radices
-// 8, 8, 8) and the split
+// Z of z[m] = v[2m] + i v[2m+1] (the register FFT, E = n/128 points per
+// lane) and the split
 //   E_k = (Z[k] + conj Z[n/2-k]) / 2,  O_k = (Z[k] - conj Z[n/2-k]) / 2i,
 //   V[k] = E_k + w^k O_k,  V[k + n/2] = E_k - w^k O_k,  w = e^{-2 pi i/n}.
 // The inverse (V Hermitian, v real) joins E_k = (V[k] + V[k+n/2]) / 2 and
 // O_k = (V[k] - V[k+n/2]) conj(w^k) / 2 into Z[k] = E_k + i O_k, runs the
 // inverse n/2-point FFT and reads z[m] / (n/2) = v[2m] + i v[2m+1].  Lane t
-// holds m = t + 64 u (u < 8) and the frequencies k = t + 64 u and k + n/2;
+// holds m = t + 64 u (u < E) and the frequencies k = t + 64 u and k + n/2;
 // frequency k pairs with n/2 - k (one exchange), which also holds n - k.
 // z[m] holds x[4m], x[4m+2] for m < n/4 and x[2n-1-4m], x[2n-3-4m] above.
 // ---------------------------------------------------------------------------
+template <int E>
 struct RowTab {
-  static constexpr int N = 1024, M = N / 2;
-  static constexpr int WQ0 = RegTab<8>::PH0;  // after the n/2-point FFT's table: w^q, q < n/4
+  static constexpr int N = 128 * E, M = N / 2;  // row length, FFT points (E per lane)
+  static constexpr int WQ0 = RegTab<E>::PH0;  // after the n/2-point FFT's table: w^q, q < n/4
   static constexpr int PH0 = WQ0 + N / 4;     // DCT phases e^{-i pi k / 2n}, k <= n/2
   static constexpr int SIZE = PH0 + N / 2 + 1;
   static constexpr int XB = M + M / 16;  // exchange buffer per wave
   static constexpr int SN = N + 4;       // staging row stride (doubles), over the exchange buffers
   static constexpr size_t lds_bytes(int w) { return size_t(w * XB + SIZE) * sizeof(cd); }
+  static_assert(SN <= 2 * XB, "a staging row fits its wave's exchange buffer");
+  static_assert(RegTab<E>::N == M, "the row form's FFT is the register FFT of E points per lane");
 };
-static_assert(RowTab::SN <= 2 * RowTab::XB, "a staging row fits its wave's exchange buffer");
-static_assert(RegTab<8>::N == RowTab::M, "the row form's FFT is the 8-point-per-lane register FFT");
 // w^k = e^{-2 pi i k / n}, k < n/2
+template <int E>
 __device__ inline cd row_w(const cd* tab, int k) {
-  const cd q = tab[RowTab::WQ0 + (k & (RowTab::N / 4 - 1))];
-  return k >= RowTab::N / 4 ? cd{q.y, -q.x} : q;
+  constexpr int Q = RowTab<E>::N / 4;
+  const cd q = tab[RowTab<E>::WQ0 + (k & (Q - 1))];
+  return k >= Q ? cd{q.y, -q.x} : q;
 }
 // phases of k and k + n/2 (the latter as phase n/2 - k swapped and negated)
-__device__ inline cd row_phase_lo(const cd* tab, int k) { return tab[RowTab::PH0 + k]; }
+template <int E>
+__device__ inline cd row_phase_lo(const cd* tab, int k) { return tab[RowTab<E>::PH0 + k]; }
+template <int E>
 __device__ inline cd row_phase_hi(const cd* tab, int k) {
-  const cd p = tab[RowTab::PH0 + RowTab::M - k];
+  const cd p = tab[RowTab<E>::PH0 + RowTab<E>::M - k];
   return cd{-p.y, -p.x};
 }
-// the row's z in lane t (position of Re z[m] and of Im z[m] in the row)
-__device__ inline int row_c0(int u, int m) { return u < 4 ? 4 * m : 2 * RowTab::N - 1 - 4 * m; }
-__device__ inline int row_c1(int u, int m) { return u < 4 ? 4 * m + 2 : 2 * RowTab::N - 3 - 4 * m; }
-__device__ inline void row_load(cd (&x)[8], const double* in_, int row, int t) {
-  const auto in = gp(in_) + long(row) * RowTab::N;
+// the row's z in lane t, slot u (u < E/2: m < n/4): position of Re z[m] and of Im z[m] in the row
+template <int E>
+__device__ inline int row_c0(int u, int m) { return u < E / 2 ? 4 * m : 2 * RowTab<E>::N - 1 - 4 * m; }
+template <int E>
+__device__ inline int row_c1(int u, int m) { return u < E / 2 ? 4 * m + 2 : 2 * RowTab<E>::N - 3 - 4 * m; }
+template <int E>
+__device__ inline void row_load(cd (&x)[E], const double* in_, int row, int t) {
+  const auto in = gp(in_) + long(row) * RowTab<E>::N;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int m = t + 64 * u;
-    x[u] = cd{in[row_c0(u, m)], in[row_c1(u, m)]};
+    x[u] = cd{in[row_c0<E>(u, m)], in[row_c1<E>(u, m)]};
   }
 }
 // DCT-II coefficients X[k], X[k + n/2] from Z[k] and Zp = Z[(n/2 - k) mod n/2]
@@ -1049,7 +787,7 @@ __device__ inline cd row_pre(double ulo, double uhi, double umk, double unk, cd 
   return {e.x - o.y, e.y + o.x};
 }
 // the wave's lane, row, exchange buffer and the workgroup's table
-template <int W>
+template <int E, int W>
 struct RowWave {
   int t, row;
   cd* xb;
@@ -1058,22 +796,22 @@ struct RowWave {
     const int wv = int(threadIdx.x) >> 6;
     t = int(threadIdx.x) & 63;
     row = W * lt + wv;
-    xb = fsm + wv * RowTab::XB;
-    tab = fsm + W * RowTab::XB;
+    xb = fsm + wv * RowTab<E>::XB;
+    tab = fsm + W * RowTab<E>::XB;
   }
   // the wave's staging row over the exchange buffers (row stride RowTab::SN)
   __device__ double* stage(cd* fsm) const {
-    return reinterpret_cast<double*>(fsm) + (int(threadIdx.x) >> 6) * RowTab::SN;
+    return reinterpret_cast<double*>(fsm) + (int(threadIdx.x) >> 6) * RowTab<E>::SN;
   }
 };
 // The transposed write of a pass: the W rows of the workgroup (first row
 // W lt) sit in the staging rows; output row k takes them as one W-value run
 // at column W lt.  Lane pairs write the two 16-byte halves of a run, so a
 // store instruction touches 32 lines instead of 64.
-template <int W>
+template <int E, int W>
 __device__ inline void row_store_transposed(const cd* fsm, double* out_, int lt) {
   static_assert(W == 4, "two lanes per four-row run");
-  constexpr int N = RowTab::N, SN = RowTab::SN;
+  constexpr int N = RowTab<E>::N, SN = RowTab<E>::SN;
   const double* sg = reinterpret_cast<const double*>(fsm);
   const auto out = gp(out_) + W * lt;
   const int q = int(threadIdx.x), h = q & 1;
@@ -1085,74 +823,74 @@ __device__ inline void row_store_transposed(const cd* fsm, double* out_, int lt)
 }
 
 // Pass 1 (row form): a row of R' -> DCT-II along j, written transposed.
-template <int W>
+template <int E, int W>
 __global__ __launch_bounds__(64 * W) void k_fftw_rows_fwd(const Job* __restrict__ jobs,
                                                           const int* __restrict__ starts, int count, bool gate,
                                                           bool rhs) {
-  constexpr int N = RowTab::N, M = RowTab::M;
+  constexpr int N = RowTab<E>::N, M = RowTab<E>::M;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   const bool early = count == 1;
   if (gate && !early && st->done) return;
-  const RowWave<W> G(fsm, L.lt);
+  const RowWave<E, W> G(fsm, L.lt);
   const int t = G.t, row = G.row;
-  cd tv[reg_tpt(RowTab::SIZE, W)];
-  tab_load<RowTab::SIZE, W>(tv, J.fft);
-  cd x[8];
-  row_load(x, J.f.r, row, t);
+  cd tv[reg_tpt(RowTab<E>::SIZE, W)];
+  tab_load<RowTab<E>::SIZE, W>(tv, J.fft);
+  cd x[E];
+  row_load<E>(x, J.f.r, row, t);
   if (rhs && row == 0 && t == 0) x[0].x -= st->sum_r;  // R' = R - (sum r) e0 e0^T
   if (gate && early && st->done) return;
-  tab_store<RowTab::SIZE, W>(tv, G.tab);
+  tab_store<RowTab<E>::SIZE, W>(tv, G.tab);
   __syncthreads();
-  reg_fft<8, -1>(x, G.xb, G.tab, t);
+  reg_fft<E, -1>(x, G.xb, G.tab, t);
 #pragma unroll
-  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = x[u];
+  for (int u = 0; u < E; ++u) G.xb[xpad(t + 64 * u)] = x[u];
   __syncthreads();
   const double s0 = sqrt(1.0 / N), s1 = sqrt(2.0 / N);
-  double xlo[8], xhi[8];
+  double xlo[E], xhi[E];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int k = t + 64 * u;
-    row_post(x[u], G.xb[xpad(k == 0 ? 0 : M - k)], row_w(G.tab, k), row_phase_lo(G.tab, k),
-             row_phase_hi(G.tab, k), k == 0 ? s0 : s1, s1, xlo[u], xhi[u]);
+    row_post(x[u], G.xb[xpad(k == 0 ? 0 : M - k)], row_w<E>(G.tab, k), row_phase_lo<E>(G.tab, k),
+             row_phase_hi<E>(G.tab, k), k == 0 ? s0 : s1, s1, xlo[u], xhi[u]);
   }
   __syncthreads();  // every partner read is done: the exchange buffers become the staging rows
   double* const sg = G.stage(fsm);
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int k = t + 64 * u;
     sg[k] = xlo[u];
     sg[k + M] = xhi[u];
   }
   __syncthreads();
-  row_store_transposed<W>(fsm, J.f.tmp0, L.lt);
+  row_store_transposed<E, W>(fsm, J.f.tmp0, L.lt);
 }
 
 // Pass 2 (row form): a row of B^T, DCT-II along i, / lambda, DCT-III along
 // i, written transposed back.
-template <int W>
+template <int E, int W>
 __global__ __launch_bounds__(64 * W) void k_fftw_cols(const Job* __restrict__ jobs, const int* __restrict__ starts,
                                                       int count, bool gate) {
-  constexpr int N = RowTab::N, M = RowTab::M;
+  constexpr int N = RowTab<E>::N, M = RowTab<E>::M;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   const bool early = count == 1;
   if (gate && !early && st->done) return;
-  const RowWave<W> G(fsm, L.lt);
+  const RowWave<E, W> G(fsm, L.lt);
   const int t = G.t, l = G.row;
-  cd tv[reg_tpt(RowTab::SIZE, W)];
-  tab_load<RowTab::SIZE, W>(tv, J.fft);
-  cd x[8];
-  row_load(x, J.f.tmp0, l, t);
+  cd tv[reg_tpt(RowTab<E>::SIZE, W)];
+  tab_load<RowTab<E>::SIZE, W>(tv, J.fft);
+  cd x[E];
+  row_load<E>(x, J.f.tmp0, l, t);
   // eigenvalue halves sin^2(pi |k-l| / 2n) + sin^2(pi (k+l) / 2n) of k and k + n/2
   const auto sq = gp(J.sq);
-  double lam[2][8];
+  double lam[2][E];
 #pragma unroll
-  for (int u = 0; u < 8; ++u)
+  for (int u = 0; u < E; ++u)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int k = t + 64 * u + h * M;
@@ -1160,110 +898,110 @@ __global__ __launch_bounds__(64 * W) void k_fftw_cols(const Job* __restrict__ jo
       lam[h][u] = sq[dm] + sq[k + l];
     }
   if (gate && early && st->done) return;
-  tab_store<RowTab::SIZE, W>(tv, G.tab);
+  tab_store<RowTab<E>::SIZE, W>(tv, G.tab);
   __syncthreads();
-  reg_fft<8, -1>(x, G.xb, G.tab, t);
+  reg_fft<E, -1>(x, G.xb, G.tab, t);
 #pragma unroll
-  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = x[u];
+  for (int u = 0; u < E; ++u) G.xb[xpad(t + 64 * u)] = x[u];
   __syncthreads();
   const double s0 = sqrt(1.0 / N), s1 = sqrt(2.0 / N);
   const double r0 = sqrt(double(N)), r1 = sqrt(0.5 * N);
   // spectrum / lambda, scaled by 1 / s_k for the inverse: (U[k], U[k + n/2])
-  cd w[8];
+  cd w[E];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int k = t + 64 * u;
     double xlo, xhi;
-    row_post(x[u], G.xb[xpad(k == 0 ? 0 : M - k)], row_w(G.tab, k), row_phase_lo(G.tab, k),
-             row_phase_hi(G.tab, k), k == 0 ? s0 : s1, s1, xlo, xhi);
+    row_post(x[u], G.xb[xpad(k == 0 ? 0 : M - k)], row_w<E>(G.tab, k), row_phase_lo<E>(G.tab, k),
+             row_phase_hi<E>(G.tab, k), k == 0 ? s0 : s1, s1, xlo, xhi);
     const double ulo = (k == 0 && l == 0) ? 0. : xlo / (2. * lam[0][u]);
     const double uhi = xhi / (2. * lam[1][u]);
     w[u] = cd{ulo * (k == 0 ? r0 : r1), uhi * r1};
   }
   __syncthreads();
 #pragma unroll
-  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = w[u];
+  for (int u = 0; u < E; ++u) G.xb[xpad(t + 64 * u)] = w[u];
   __syncthreads();
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int k = t + 64 * u;
     const cd q = G.xb[xpad(k == 0 ? 0 : M - k)];  // U[n/2 - k], U[n - k]
     const double umk = k == 0 ? w[u].y : q.x, unk = k == 0 ? 0. : q.y;
-    x[u] = row_pre(w[u].x, w[u].y, umk, unk, row_w(G.tab, k), row_phase_lo(G.tab, k), row_phase_hi(G.tab, k));
+    x[u] = row_pre(w[u].x, w[u].y, umk, unk, row_w<E>(G.tab, k), row_phase_lo<E>(G.tab, k), row_phase_hi<E>(G.tab, k));
   }
   __syncthreads();
-  reg_fft<8, 1>(x, G.xb, G.tab, t);  // ends reading its exchange buffer before a barrier
+  reg_fft<E, 1>(x, G.xb, G.tab, t);  // ends reading its exchange buffer before a barrier
   const double inv = 1.0 / M;
   double* const sg = G.stage(fsm);
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int m = t + 64 * u;
-    sg[row_c0(u, m)] = x[u].x * inv;
-    sg[row_c1(u, m)] = x[u].y * inv;
+    sg[row_c0<E>(u, m)] = x[u].x * inv;
+    sg[row_c1<E>(u, m)] = x[u].y * inv;
   }
   __syncthreads();
-  row_store_transposed<W>(fsm, J.f.tmp1, L.lt);
+  row_store_transposed<E, W>(fsm, J.f.tmp1, L.lt);
 }
 
 // Pass 3 (row form): DCT-III along j = d_hat, and the row's |d_hat|^2 partial.
-template <int W>
+template <int E, int W>
 __global__ __launch_bounds__(64 * W) void k_fftw_rows_inv(const Job* __restrict__ jobs,
                                                           const int* __restrict__ starts, int count, bool gate) {
-  constexpr int N = RowTab::N, M = RowTab::M;
+  constexpr int N = RowTab<E>::N, M = RowTab<E>::M;
   extern __shared__ cd fsm[];
   const FftLoc L = fft_locate(starts, count);
   const Job& J = jobs[L.p];
   const auto st = gp((const SolveState*)J.f.state);
   const bool early = count == 1;
   if (gate && !early && st->done) return;
-  const RowWave<W> G(fsm, L.lt);
+  const RowWave<E, W> G(fsm, L.lt);
   const int t = G.t, row = G.row;
-  cd tv[reg_tpt(RowTab::SIZE, W)];
-  tab_load<RowTab::SIZE, W>(tv, J.fft);
+  cd tv[reg_tpt(RowTab<E>::SIZE, W)];
+  tab_load<RowTab<E>::SIZE, W>(tv, J.fft);
   // X[k], X[k + n/2] of the row; X[n/2 - k], X[n - k] come from the lane of
   // n/2 - k through the exchange buffer (k = 0: X[n/2] and 0)
   const auto in = gp((const double*)J.f.tmp1) + long(row) * N;
-  double xl[8], xh[8];
+  double xl[E], xh[E];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int k = t + 64 * u;
     xl[u] = in[k];
     xh[u] = in[k + M];
   }
   if (gate && early && st->done) return;
-  tab_store<RowTab::SIZE, W>(tv, G.tab);
+  tab_store<RowTab<E>::SIZE, W>(tv, G.tab);
 #pragma unroll
-  for (int u = 0; u < 8; ++u) G.xb[xpad(t + 64 * u)] = cd{xl[u], xh[u]};
+  for (int u = 0; u < E; ++u) G.xb[xpad(t + 64 * u)] = cd{xl[u], xh[u]};
   __syncthreads();
   const double r0 = sqrt(double(N)), r1 = sqrt(0.5 * N);
-  cd x[8];
+  cd x[E];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int k = t + 64 * u;
     const cd q = G.xb[xpad(k == 0 ? 0 : M - k)];
     const double xm = k == 0 ? xh[u] : q.x, xn = k == 0 ? 0. : q.y;
-    x[u] = row_pre(xl[u] * (k == 0 ? r0 : r1), xh[u] * r1, xm * r1, xn * r1, row_w(G.tab, k),
-                   row_phase_lo(G.tab, k), row_phase_hi(G.tab, k));
+    x[u] = row_pre(xl[u] * (k == 0 ? r0 : r1), xh[u] * r1, xm * r1, xn * r1, row_w<E>(G.tab, k),
+                   row_phase_lo<E>(G.tab, k), row_phase_hi<E>(G.tab, k));
   }
   __syncthreads();  // partner reads done before the FFT's first exchange
-  reg_fft<8, 1>(x, G.xb, G.tab, t);  // ends reading its exchange buffer before a barrier
+  reg_fft<E, 1>(x, G.xb, G.tab, t);  // ends reading its exchange buffer before a barrier
   // the row through the wave's exchange buffer, written in 16-byte runs
   double* const sg = reinterpret_cast<double*>(G.xb);
   const double inv = 1.0 / M;
   double nrm[1] = {0.};
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < E; ++u) {
     const int m = t + 64 * u;
     const double a = x[u].x * inv, b = x[u].y * inv;
-    sg[row_c0(u, m)] = a;
-    sg[row_c1(u, m)] = b;
+    sg[row_c0<E>(u, m)] = a;
+    sg[row_c1<E>(u, m)] = b;
     nrm[0] += a * a;
     nrm[0] += b * b;
   }
   __syncthreads();
   const auto out = gp(J.f.dhat) + long(row) * N;
 #pragma unroll
-  for (int v = 0; v < 8; ++v) {
+  for (int v = 0; v < E; ++v) {
     const int c = 2 * (t + 64 * v);
     *reinterpret_cast<OTM_GLOBAL cd*>(out + c) = cd{sg[c], sg[c + 1]};
   }
@@ -1271,17 +1009,17 @@ __global__ __launch_bounds__(64 * W) void k_fftw_rows_inv(const Job* __restrict_
   if (t == 63) gp(J.f.site(SITE_DNORM))[row] = nrm[0];
 }
 
-template <int W>
+template <int E, int W>
 void launch_row(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
-  const size_t lds = RowTab::lds_bytes(W);
+  const size_t lds = RowTab<E>::lds_bytes(W);
   static std::atomic<unsigned long long> attr{0};
   int dev = 0;
   OTM_HIP(hipGetDevice(&dev));
   const unsigned long long bit = 1ull << (dev & 63);
   if (!(attr.load() & bit)) {
-    for (const void* k : {reinterpret_cast<const void*>(&k_fftw_rows_fwd<W>),
-                          reinterpret_cast<const void*>(&k_fftw_cols<W>),
-                          reinterpret_cast<const void*>(&k_fftw_rows_inv<W>)})
+    for (const void* k : {reinterpret_cast<const void*>(&k_fftw_rows_fwd<E, W>),
+                          reinterpret_cast<const void*>(&k_fftw_cols<E, W>),
+                          reinterpret_cast<const void*>(&k_fftw_rows_inv<E, W>)})
       OTM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     attr.fetch_or(bit);
   }
@@ -1289,16 +1027,17 @@ void launch_row(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) 
   const int* st = js.starts_of(kind);
   const Job* jobs = js.jobs;
   switch (pass) {
-    case 0: hipLaunchKernelGGL(k_fftw_rows_fwd<W>, grid, block, lds, s, jobs, st, js.count, gate, true); break;
-    case 1: hipLaunchKernelGGL(k_fftw_cols<W>, grid, block, lds, s, jobs, st, js.count, gate); break;
-    default: hipLaunchKernelGGL(k_fftw_rows_inv<W>, grid, block, lds, s, jobs, st, js.count, gate);
+    case 0: hipLaunchKernelGGL((k_fftw_rows_fwd<E, W>), grid, block, lds, s, jobs, st, js.count, gate, true); break;
+    case 1: hipLaunchKernelGGL((k_fftw_cols<E, W>), grid, block, lds, s, jobs, st, js.count, gate); break;
+    default: hipLaunchKernelGGL((k_fftw_rows_inv<E, W>), grid, block, lds, s, jobs, st, js.count, gate);
   }
   OTM_HIP(hipGetLastError());
 }
 
+template <int E>
 void launch_row_kind(const JobSet& js, int kind, int pass, bool gate, hipStream_t s) {
   if (js.total[kind] == 0) return;
-  launch_row<FFT_ROW_W>(js, kind, pass, gate, s);
+  launch_row<E, FFT_ROW_W>(js, kind, pass, gate, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1581,7 +1320,7 @@ void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream
   if (js.total[kind] == 0) return;
   int nmax = 0;
   for (int p = 0; p < js.count; ++p)
-    if (js.jroute[p] == 2 && fft_rows(js.host[p].f.n) == 2 * FS && !fft_reg_e(js.host[p].f.n))
+    if (js.jroute[p] == 2 && fft_rows(js.host[p].f.n) == 2 * FS && !fft_row_form(js.host[p].f.n))
       nmax = std::max(nmax, js.host[p].f.n);
   const size_t lds = size_t(fft_lds(FS)) * nmax;
   // the kernels use no static LDS: up to fft_lds(FS) times the largest n of
@@ -1615,8 +1354,8 @@ void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream
 // one workgroup per 2 rows) and KIND_FFT2 (one per 4 rows); dynamic LDS
 // sized for the largest grid of each kind.
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
-  launch_reg_kind<8>(js, KIND_FFTR8, pass, gate, s);
-  launch_row_kind(js, KIND_FFTW, pass, gate, s);
+  launch_row_kind<4>(js, KIND_FFTW4, pass, gate, s);
+  launch_row_kind<8>(js, KIND_FFTW8, pass, gate, s);
   launch_pack_kind(js, KIND_FFT, pass, gate, s);
   launch_pass_kind<2>(js, KIND_FFT2, pass, gate, s);
 }

@@ -191,8 +191,8 @@ enum LaunchKind {
   KIND_FLATMX = 7,     // grid-stride blocks of the jobs on the mixed transform (first right-hand side)
   KIND_FFT = 8,        // fast-DCT passes of the FFT-route jobs with n <= FFT_SPLIT_N, packed: 1024/n sequences per workgroup
   KIND_FFT2 = 9,       // fast-DCT passes of the FFT-route jobs with n > FFT_SPLIT_N: one workgroup per 4 rows
-  KIND_FFTR8 = 10,     // fast-DCT passes in register form, n = 512: one 64-thread workgroup per 2 rows
-  KIND_FFTW = 11,      // fast-DCT passes in row form, n = 1024: one wave per row
+  KIND_FFTW8 = 10,     // fast-DCT passes in row form, n = 1024: one wave per row, four per workgroup
+  KIND_FFTW4 = 11,     // the same for n = 512
   KIND_M0 = 12,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
@@ -242,29 +242,21 @@ constexpr int FFT_MAX_N = 2048;
 // workgroups would no longer fit one CU's LDS.
 constexpr int FFT_SPLIT_N = 1024;
 constexpr int fft_rows(int n) { return n <= FFT_SPLIT_N ? 2 : 4; }
-// Points per lane of the register form of the fast-DCT passes (one wave per
-// pair of rows, fft.hip), 0 for sizes that use another form.
-constexpr int fft_reg_e(int n) { return n == 512 ? 8 : 0; }
-// The row form (fft.hip): one wave per grid row, whose DCT runs through an
-// n/2-point complex FFT held in registers (8 points per lane).
-constexpr bool fft_row_form(int n) { return n == 1024; }
-// Waves per workgroup of the register and row forms: one for a single grid
-// (its workgroups spread over every CU), FFT_REG_W in batches, where the
-// waves of a workgroup share one LDS copy of the twiddle table.
-constexpr int FFT_REG_W = 2;
-constexpr int fft_reg_waves(int jobs) { return jobs == 1 ? 1 : FFT_REG_W; }
-// Waves (rows) per workgroup of the row form, for any number of jobs: the
-// four rows leave each pass as 32-byte runs of its transposed output.
+// The row form of the fast-DCT passes (fft.hip): one wave per grid row,
+// whose DCT runs through an n/2-point complex FFT held in registers (n/128
+// points per lane), four rows per workgroup (their transposed outputs leave
+// as 32-byte runs).
+constexpr bool fft_row_form(int n) { return n == 512 || n == 1024; }
 constexpr int FFT_ROW_W = 4;
 // The packed workgroup form of the other sizes up to FFT_SPLIT_N (fft.hip):
 // 256 threads, n/4 per complex sequence (two rows), so 1024 / n sequences
 // per workgroup; its workgroups per grid.
-constexpr bool fft_pack(int n) { return n <= FFT_SPLIT_N && fft_reg_e(n) == 0 && !fft_row_form(n); }
+constexpr bool fft_pack(int n) { return n <= FFT_SPLIT_N && !fft_row_form(n); }
 constexpr int fft_pack_seqs(int n) { return FFT_SPLIT_N / n; }
 constexpr int fft_pack_wgs(int n) { return (n / 2 + fft_pack_seqs(n) - 1) / fft_pack_seqs(n); }
 // workgroups per grid of the fast-DCT passes (one |d_hat|^2 partial each)
 constexpr int fft_wgs(int n) {
-  return fft_row_form(n) ? n : fft_reg_e(n) ? n / 2 : fft_pack(n) ? fft_pack_wgs(n) : n / 4;
+  return fft_row_form(n) ? n : fft_pack(n) ? fft_pack_wgs(n) : n / 4;
 }
 bool fft_supported(int n);
 // FP64 route of a grid: the fast DCT when asked for and supported, else the GEMMs

@@ -1141,8 +1141,8 @@ std::vector<int> job_starts(JobSet& js) {
       else if (kind == KIND_FLAT) v = flat_blocks(J.f.F);
       else if (kind == KIND_STENCIL64) v = (r64 || rfft) ? num_tiles(n) : 0;  // the plain update pass
       else if (kind == KIND_FFT) v = rfft && fft_pack(n) ? fft_pack_wgs(n) : 0;
-      else if (kind == KIND_FFTR8) v = rfft && fft_reg_e(n) == 8 ? n / (2 * fft_reg_waves(js.count)) : 0;
-      else if (kind == KIND_FFTW) v = rfft && fft_row_form(n) ? n / FFT_ROW_W : 0;
+      else if (kind == KIND_FFTW8) v = rfft && n == 1024 ? n / FFT_ROW_W : 0;
+      else if (kind == KIND_FFTW4) v = rfft && n == 512 ? n / FFT_ROW_W : 0;
       else if (kind == KIND_FFT2) v = rfft && fft_rows(n) == 4 ? n / 4 : 0;
       else if (kind == KIND_FLATMX) v = rmx && mixed_supported(n) ? flat_blocks(J.f.F) : 0;
       else if (kind == KIND_PLAIN64) v = r64 ? ((n + 63) / 64) * ((n + 63) / 64) : 0;
