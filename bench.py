@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 GEMM_TRAFFIC = "profiles/gemm_1024_hbm_traffic.md"
 FFT_TRAFFIC = "profiles/fft_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
-    "k_fft_cols": ((4478.0 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
+    "k_fft_cols": ((4333.5 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
     "k_mgemm<M2,32>": ((9288.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
     "k_mgemm<M2,64>": ((9294.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
     **{f"k_gemm_fold<F1,cfg{c}>": ((f * 2.0 + 8192.0) * 1024, GEMM_TRAFFIC)
