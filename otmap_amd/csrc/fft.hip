@@ -37,6 +37,11 @@
 #include "otm_internal.h"
 #include "reduce.h"
 
+// The transform's arithmetic follows no reference operation order (the
+// reference solves by a sparse factorisation), so its multiply-adds contract
+// to fused FMAs.
+#pragma clang fp contract(fast)
+
 namespace otm {
 
 bool fft_supported(int n) {
