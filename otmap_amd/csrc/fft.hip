@@ -71,20 +71,22 @@ void fft_plan_create(double*& table, int n) {
     h[2 * e + 1] = double(-sinl(a));
   };
   if (fft_row_form(n)) {
-    // Row form.  Its m = n/2-point complex FFT (radix 8 for m = 512, 4 for
-    // m = 256): the twiddles of the stages before the last by stage as above
-    // (entries 0 .. m/R - 2), then the quarter table
+    // Row form.  Its m = n/2-point complex FFT (radices 4, 4, 4, 4 for m =
+    // 256; 8, 8, 8 for 512; 16, 16, 4 for 1024): the twiddles of the stages
+    // before the last by stage as above (entries 0 .. L - 2, L the span
+    // before the last stage), then the quarter table
     // exp(-2 pi i q / m), q < m/4, from which the last stage takes its
     // twiddle exp(-2 pi i r j / m) as (-i)^(rj div m/4) times entry rj mod
     // m/4 (the rotation is exact).  Then the quarter table exp(-2 pi i q / n),
     // q < n/4, of the real-FFT split.  Last, the DCT
     // phases of k <= n/2 (phase n - k is -i conj(phase k): the real and
     // imaginary parts swapped and negated).
-    const int m = n / 2, R = m == 256 ? 4 : 8;
-    for (int L = 1; L < m / R; L *= R)
+    const int m = n / 2, R = m == 256 ? 4 : m == 512 ? 8 : 16;
+    int L = 1;
+    for (; L * R < m; L *= R)
       for (int r = 1; r < R; ++r)
         for (int j = 0; j < L; ++j) put(size_t(L - 1 + (r - 1) * L + j), 2.0L * pi * (j * r) / (long double)(L * R));
-    size_t e = size_t(m / R - 1);
+    size_t e = size_t(L - 1);  // L: the span before the last stage (m / its radix)
     for (int q = 0; q < m / 4; ++q) put(e++, 2.0L * pi * q / (long double)m);
     for (int q = 0; q < n / 4; ++q) put(e++, 2.0L * pi * q / (long double)n);
     for (int t = 0; t <= n / 2; ++t) put(e++, pi * t / (2.0L * n));
@@ -619,6 +621,10 @@ template <>
 struct RegRadix<4> {  // 256 points: 4 x 4 x 4 x 4
   static constexpr int R = 4, LAST = 4;
 };
+template <>
+struct RegRadix<16> {  // 1024 points: 16 x 16 x 4
+  static constexpr int R = 16, LAST = 4;
+};
 
 // The register FFT's twiddles (fft_plan_create): the stage-major twiddles
 // of the stages before the last (entries 0 .. Q0 - 1) and the quarter table
@@ -642,6 +648,37 @@ __device__ inline cd reg_twiddle(const cd* tab, int m) {
 }
 __device__ inline int xpad(int p) { return p + (p >> 4); }
 
+// DFT of 16 points as 4 x 4: DFT4 over n1 of a[n2 + 4 n1], times w16^(n2 k1),
+// DFT4 over n2; X[k1 + 4 k2] lands in c[k1 + 4 k2]
+template <int SIGN>
+__device__ inline cd w16mul(cd y, int m) {
+  const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;  // cos, sin(pi/8)
+  const double h = 0.70710678118654752440;
+  switch (m) {
+    case 1: return cmul(y, cd{c1, SIGN * s1});
+    case 2: return cmul(y, cd{h, SIGN * h});
+    case 3: return cmul(y, cd{s1, SIGN * c1});
+    case 4: return mul_i(y, SIGN);
+    case 6: return cmul(y, cd{-h, SIGN * h});
+    case 9: return cmul(y, cd{-c1, -SIGN * s1});
+    default: return y;
+  }
+}
+template <int SIGN>
+__device__ inline void dft16(const cd (&a)[16], cd (&c)[16]) {
+  cd y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2)
+    dft4<SIGN>(a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12], y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+#pragma unroll
+  for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = w16mul<SIGN>(y[n2][k1], n2 * k1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) dft4<SIGN>(y[0][k1], y[1][k1], y[2][k1], y[3][k1], c[k1], c[k1 + 4], c[k1 + 8], c[k1 + 12]);
+}
+
+
 template <int E, int R, int L, int SIGN>
 __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
   constexpr int N = 64 * E, U = E / R;
@@ -661,9 +698,10 @@ __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
         a[r] = cmul(a[r], SIGN < 0 ? w : conjc(w));
       }
     }
-    static_assert(R == 4 || R == 8, "radix 4 or 8");
+    static_assert(R == 4 || R == 8 || R == 16, "radix 4, 8 or 16");
     if constexpr (R == 4) dft4<SIGN>(a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]);
-    else dft8<SIGN>(a, c);
+    else if constexpr (R == 8) dft8<SIGN>(a, c);
+    else dft16<SIGN>(a, c);
     if constexpr (LAST) {
 #pragma unroll
       for (int q = 0; q < R; ++q) x[u + q * U] = c[q];  // position b + q L = t + 64 (u + q U)
@@ -680,10 +718,11 @@ __device__ inline void rstage(cd (&x)[E], cd* xb, const cd* tw, int t) {
   }
 }
 
-// the stages of radix R after spans L = 1, R, R^2, ... up to 64 E
+// the stages of radix R after spans L = 1, R, R^2, ..., the last one of the
+// radix that completes 64 E points
 template <int E, int SIGN, int L = 1>
 __device__ inline void reg_fft(cd (&x)[E], cd* xb, const cd* tw, int t) {
-  constexpr int R = RegRadix<E>::R;
+  constexpr int R = L * RegRadix<E>::R <= 64 * E ? RegRadix<E>::R : 64 * E / L;
   rstage<E, R, L, SIGN>(x, xb, tw, t);
   if constexpr (L * R < 64 * E) reg_fft<E, SIGN, L * R>(x, xb, tw, t);
 }
@@ -1361,6 +1400,7 @@ void launch_pass_kind(const JobSet& js, int kind, int pass, bool gate, hipStream
 void launch_fft_pass(const JobSet& js, int pass, bool gate, hipStream_t s) {
   launch_row_kind<4>(js, KIND_FFTW4, pass, gate, s);
   launch_row_kind<8>(js, KIND_FFTW8, pass, gate, s);
+  launch_row_kind<16>(js, KIND_FFTW16, pass, gate, s);
   launch_pack_kind(js, KIND_FFT, pass, gate, s);
   launch_pass_kind<2>(js, KIND_FFT2, pass, gate, s);
 }

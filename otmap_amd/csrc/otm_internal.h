@@ -193,7 +193,8 @@ enum LaunchKind {
   KIND_FFT2 = 9,       // fast-DCT passes of the FFT-route jobs with n > FFT_SPLIT_N: one workgroup per 4 rows
   KIND_FFTW8 = 10,     // fast-DCT passes in row form, n = 1024: one wave per row, four per workgroup
   KIND_FFTW4 = 11,     // the same for n = 512
-  KIND_M0 = 12,
+  KIND_FFTW16 = 12,    // the same for n = 2048
+  KIND_M0 = 13,
   KIND_GEMM0 = KIND_M0 + 4 * NMIXED,
   KIND_COUNT = KIND_GEMM0 + 4 * NFOLD
 };
@@ -246,7 +247,7 @@ constexpr int fft_rows(int n) { return n <= FFT_SPLIT_N ? 2 : 4; }
 // whose DCT runs through an n/2-point complex FFT held in registers (n/128
 // points per lane), four rows per workgroup (their transposed outputs leave
 // as 32-byte runs).
-constexpr bool fft_row_form(int n) { return n == 512 || n == 1024; }
+constexpr bool fft_row_form(int n) { return n == 512 || n == 1024 || n == 2048; }
 constexpr int FFT_ROW_W = 4;
 // The packed workgroup form of the other sizes up to FFT_SPLIT_N (fft.hip):
 // 256 threads, n/4 per complex sequence (two rows), so 1024 / n sequences

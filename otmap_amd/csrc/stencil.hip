@@ -1143,7 +1143,8 @@ std::vector<int> job_starts(JobSet& js) {
       else if (kind == KIND_FFT) v = rfft && fft_pack(n) ? fft_pack_wgs(n) : 0;
       else if (kind == KIND_FFTW8) v = rfft && n == 1024 ? n / FFT_ROW_W : 0;
       else if (kind == KIND_FFTW4) v = rfft && n == 512 ? n / FFT_ROW_W : 0;
-      else if (kind == KIND_FFT2) v = rfft && fft_rows(n) == 4 ? n / 4 : 0;
+      else if (kind == KIND_FFTW16) v = rfft && n == 2048 ? n / FFT_ROW_W : 0;
+      else if (kind == KIND_FFT2) v = rfft && fft_rows(n) == 4 && !fft_row_form(n) ? n / 4 : 0;
       else if (kind == KIND_FLATMX) v = rmx && mixed_supported(n) ? flat_blocks(J.f.F) : 0;
       else if (kind == KIND_PLAIN64) v = r64 ? ((n + 63) / 64) * ((n + 63) / 64) : 0;
       else if (kind == KIND_MRES) v = rmx && mixed_supported(n) ? mres_tiles(n) : 0;
