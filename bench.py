@@ -43,7 +43,9 @@ PEAK_HBM_GBS = 8000.0
 # profile named beside each figure.
 GEMM_TRAFFIC = "profiles/gemm_1024_hbm_traffic.md"
 FFT_TRAFFIC = "profiles/fft_1024_hbm_traffic.md"
+STENCIL_TRAFFIC = "profiles/stencil_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
+    "k_dir_coef": ((22215.0 * 2.0 + 8640.0) * 1024, STENCIL_TRAFFIC),
     "k_fft_cols": ((4333.5 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
     "k_mgemm<M2,32>": ((9288.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
     "k_mgemm<M2,64>": ((9294.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
@@ -67,7 +69,8 @@ def parse():
                     help="1024^2 densities of the batched mixed-vs-FP64 workload (0: skip)")
     ap.add_argument("--kernel", default=None,
                     choices=["dct_gemm_f64", "mixed_gemm", "beta", "dir_coef", "update", "dct_solve_f64",
-                             "mixed_solve", "fft_solve", "fft_rows_fwd", "fft_cols", "fft_rows_inv", "copy"],
+                             "mixed_solve", "fft_solve", "fft_rows_fwd", "fft_cols", "fft_rows_inv", "dir_coef_pass",
+                             "copy"],
                     help="only time this kernel (after one solve that fills the buffers); for profilers")
     ap.add_argument("--tile", type=int, default=None,
                     help="with --kernel mixed_gemm / dct_gemm_f64: force this tile config on the timed stage")
@@ -75,7 +78,7 @@ def parse():
 
 
 KERNEL_IDS = {"dct_gemm_f64": 0, "beta": 1, "dir_coef": 2, "update": 3, "dct_solve_f64": 4, "mixed_gemm": 5,
-              "mixed_solve": 6, "fft_solve": 7, "fft_rows_fwd": 8, "fft_cols": 9, "fft_rows_inv": 10}
+              "mixed_solve": 6, "fft_solve": 7, "fft_rows_fwd": 8, "fft_cols": 9, "fft_rows_inv": 10, "dir_coef_pass": 11}
 
 
 def kernel_only(args):
@@ -509,7 +512,8 @@ def main():
     ms = _lib.C.c_double()
     kernel_ms = {}
     kernels = [("beta", _lib.OTM_KERNEL_BETA), ("dir_coef", _lib.OTM_KERNEL_DIR_COEF),
-               ("update", _lib.OTM_KERNEL_UPDATE), ("dct_gemm_f64", _lib.OTM_KERNEL_DCT_GEMM),
+               ("update", _lib.OTM_KERNEL_UPDATE), ("dir_coef_pass", _lib.OTM_KERNEL_DIR_COEF_PASS),
+               ("dct_gemm_f64", _lib.OTM_KERNEL_DCT_GEMM),
                ("dct_solve_f64", _lib.OTM_KERNEL_DCT_SOLVE)]
     if fft:
         kernels += [("fft_solve", _lib.OTM_KERNEL_FFT_SOLVE), ("fft_rows_fwd", _lib.OTM_KERNEL_FFT_ROWS_FWD),
@@ -526,23 +530,35 @@ def main():
         t, src = TRAFFIC_BYTES_1024.get(kname, (None, None)) if n == 1024 else (None, None)
         return {"traffic": round(t) if t else None, "traffic_source": src}
 
-    # Roofline of the dominant kernel of the default path: the column pass of
-    # the fast-DCT solve (the longest kernel of an iteration), HBM-bound.  Its
-    # algorithmic bytes per launch are one read and one write of the n x n
-    # FP64 field (16 n^2 bytes; the twiddle, phase and eigenvalue tables add
-    # 64 n bytes and are not counted).
+    # Roofline of the dominant kernel of the default iteration, HBM-bound: the
+    # longer (live) of the line-search pass k_dir_coef and the fast-DCT column
+    # pass, the two longest kernels of the iteration; the other is reported
+    # beside it.  Algorithmic bytes per launch: k_dir_coef reads d_hat, d_prev,
+    # psi and r and writes d (5 n^2 FP64 values = 40 n^2 bytes; tile halos and
+    # the workgroup partials are not counted); the column pass reads and
+    # writes the n x n FP64 field once (16 n^2 bytes; the twiddle, phase and
+    # eigenvalue tables add 64 n bytes and are not counted).
+    def hbm_roofline(kname, t_ms, nbytes, label):
+        achieved = nbytes / (t_ms * 1e-3) / 1e9
+        return {"kernel": label, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), **traffic_of(kname),
+                "ms_per_launch": round(t_ms, 5)}
+
+    roof_stencil = hbm_roofline("k_dir_coef", kernel_ms["dir_coef_pass"], 40.0 * n * n,
+                                "k_dir_coef: line-search pass (direction d = d_hat + beta d_prev, quartic "
+                                "coefficients, 7 partial dots), reads d_hat, d_prev, psi, r and writes d")
     roof_fft = None
     if fft:
-        t_ms = kernel_ms["fft_cols"]
-        achieved = 16.0 * n * n / (t_ms * 1e-3) / 1e9
-        # the column pass runs in register form for n = 512 and 1024 (one wave
-        # per pair of columns), in the workgroup form at other sizes
-        cname = {512: "k_fftw_cols<4, 4>", 1024: "k_fftw_cols<8, 4>"}.get(n, "k_fft_cols<1>" if n <= 1024 else "k_fft_cols<2>")
-        roof_fft = {"kernel": f"{cname}: fast-DCT column pass (DCT-II, / lambda, DCT-III of n columns), "
-                              "one read + one write of the n x n FP64 field",
-                    "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(achieved / PEAK_HBM_GBS, 4), **traffic_of("k_fft_cols"),
-                    "ms_per_launch": round(t_ms, 5)}
+        # the column pass runs in row form for n = 512, 1024 and 2048 (one wave
+        # per grid row), in the workgroup form at other sizes
+        cname = {512: "k_fftw_cols<4, 4>", 1024: "k_fftw_cols<8, 4>", 2048: "k_fftw_cols<16, 4>"}.get(
+            n, "k_fft_cols<1>" if n <= 1024 else "k_fft_cols<2>")
+        roof_fft = hbm_roofline("k_fft_cols", kernel_ms["fft_cols"], 16.0 * n * n,
+                                f"{cname}: fast-DCT column pass (DCT-II, / lambda, DCT-III of n columns), "
+                                "one read + one write of the n x n FP64 field")
+    dominant = roof_stencil
+    if roof_fft and kernel_ms["fft_cols"] > kernel_ms["dir_coef_pass"]:
+        dominant = roof_fft
 
     # Rooflines of the GEMM routes: one stage of the even/odd-folded DCT
     # (k_mgemm stage M2 for mixed, k_gemm_fold stage F1 for FP64).
@@ -622,7 +638,9 @@ def main():
         "barycentre_4x512": bary,
         "stippling_2048": stipple,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
-        "roofline": roof_fft if roof_fft else roof_f64,
+        "roofline": dominant,
+        "roofline_dir_coef": roof_stencil,
+        "roofline_fft_cols": roof_fft,
         "roofline_fp64_gemm": roof_f64,
         "roofline_mixed_gemm": roof_x3,
         "iteration_roofline": iter_roof,

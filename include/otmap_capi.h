@@ -125,7 +125,8 @@ enum {
   OTM_KERNEL_FFT_SOLVE = 7,   /* the whole fast-DCT Laplacian solve (3 passes) */
   OTM_KERNEL_FFT_ROWS_FWD = 8, /* fast-DCT pass 1: DCT-II of the rows, transposed write */
   OTM_KERNEL_FFT_COLS = 9,     /* fast-DCT pass 2: DCT-II, / lambda, DCT-III of the columns */
-  OTM_KERNEL_FFT_ROWS_INV = 10 /* fast-DCT pass 3: DCT-III of the rows + |d_hat|^2 partials */
+  OTM_KERNEL_FFT_ROWS_INV = 10, /* fast-DCT pass 3: DCT-III of the rows + |d_hat|^2 partials */
+  OTM_KERNEL_DIR_COEF_PASS = 11 /* the line-search pass alone (direction, quartic coefficients, partials) */
 };
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch);
 

@@ -23,7 +23,7 @@ OTM_TRANSFORM_GEMM, OTM_TRANSFORM_FFT = 0, 1
 OTM_EXIT_NONE, OTM_EXIT_CONVERGED_OR_MAX_ITER, OTM_EXIT_ZERO_DIRECTION = 0, 1, 2
 (OTM_KERNEL_DCT_GEMM, OTM_KERNEL_BETA, OTM_KERNEL_DIR_COEF, OTM_KERNEL_UPDATE, OTM_KERNEL_DCT_SOLVE,
  OTM_KERNEL_MIXED_GEMM, OTM_KERNEL_MIXED_SOLVE, OTM_KERNEL_FFT_SOLVE, OTM_KERNEL_FFT_ROWS_FWD, OTM_KERNEL_FFT_COLS,
- OTM_KERNEL_FFT_ROWS_INV) = range(11)
+ OTM_KERNEL_FFT_ROWS_INV, OTM_KERNEL_DIR_COEF_PASS) = range(12)
 
 
 class OtmOptions(C.Structure):

@@ -67,6 +67,13 @@ __device__ inline void lds_dma16(const void* sbase, unsigned voff, unsigned lds)
 // Wait until at most N of this wave's vector memory operations are
 // outstanding and its LDS reads have completed, then the workgroup barrier
 // (also a compiler memory barrier: no memory access moves across it).
+// Callers pass the number of transfers this wave issued for the K step that
+// may stay in flight.  The count cannot be too permissive: vector memory
+// operations of a wave complete in issue order, so the outstanding ones are
+// always the most recently issued; the transfers are volatile asm and keep
+// their order, and any load or store the compiler places after them only
+// makes the wait stricter for the older step.  The build rejects a mixed
+// stage kernel that spills to scratch.
 template <int N>
 __device__ inline void wait_vm_barrier() {
   static_assert(N == 0 || N == 4 || N == 5 || N == 6 || N == 12, "waits used by the mixed stages");

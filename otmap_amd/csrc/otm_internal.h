@@ -281,6 +281,7 @@ void launch_start(const JobSet& js, double max_ratio, int max_iter, double thres
 // by the workgroups of the last stage of its transform (KIND_DNORM)
 void launch_beta(const JobSet& js, hipStream_t s, int* flags = nullptr);
 void launch_dir_coef(const JobSet& js, hipStream_t s, int* flags = nullptr);   // k_dir_coef + scalar kernel (line search)
+void launch_dir_coef_pass(const JobSet& js, hipStream_t s);                     // k_dir_coef alone (kernel timing)
 // jobs on the mixed transform run the folding update pass, which also writes
 // the folded BF16 planes of their next right-hand side
 void launch_update(const JobSet& js, hipStream_t s);

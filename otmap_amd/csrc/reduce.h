@@ -137,4 +137,50 @@ __device__ static inline void fold_partials(double (&v)[K], const double* partia
   block_sum<K>(v);
 }
 
+// Wide pass 2: a workgroup of G groups of NT threads.  Group g folds
+// batches g, g + G, ... (a batch is NT * FOLD_BATCH partials) in
+// fold_share's order, so more than one batch costs fewer memory round trips;
+// group_sum adds the group totals in group order.  With nb <= NT *
+// FOLD_BATCH only group 0 reads partials and the others add exact zeros: the
+// result equals the NT-thread fold's bit for bit.
+template <int K, int G>
+__device__ static inline void fold_share_grouped(double (&s)[K], const double* partials, int nb) {
+  const int g = threadIdx.x / NT, lt = threadIdx.x % NT;
+  for (int base = g * NT * FOLD_BATCH; base < nb; base += G * NT * FOLD_BATCH) {
+    double tmp[K][FOLD_BATCH];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int u = 0; u < FOLD_BATCH; ++u) {
+        const int j = base + u * NT + lt;
+        tmp[k][u] = (j < nb) ? gp(partials)[long(k) * nb + j] : 0.;
+      }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int u = 0; u < FOLD_BATCH; ++u) s[k] += tmp[k][u];
+  }
+}
+
+// block_sum over G * NT threads: each group's four wave totals as block_sum
+// adds them, then the groups in order; valid in thread 0.
+template <int K, int G>
+__device__ static inline void group_sum(double (&v)[K]) {
+  static_assert(NT == 256, "four waves per group");
+  constexpr int NW = G * NT / 64;
+  __shared__ double red[K][NW];
+  wave_sum<K>(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 63)
+    for (int k = 0; k < K; ++k) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; ++k) {
+      double x = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+      for (int g = 1; g < G; ++g) x += (red[k][4 * g] + red[k][4 * g + 1]) + (red[k][4 * g + 2] + red[k][4 * g + 3]);
+      v[k] = x;
+    }
+}
+
 }  // namespace otm

@@ -332,7 +332,7 @@ int otm_solver_trace(const otm_solver* s, double* residual, double* alpha, doubl
 
 int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_per_launch) {
   if (int rc = check_ready(s)) return rc;
-  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_FFT_ROWS_INV) {
+  if (reps < 1 || !ms_per_launch || kernel < OTM_KERNEL_DCT_GEMM || kernel > OTM_KERNEL_DIR_COEF_PASS) {
     set_error("invalid profile arguments");
     return OTM_ERR_ARG;
   }
@@ -341,7 +341,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
     set_error("mixed precision needs a grid size that is a multiple of 64");
     return OTM_ERR_ARG;
   }
-  const bool fft = kernel >= OTM_KERNEL_FFT_SOLVE;
+  const bool fft = kernel >= OTM_KERNEL_FFT_SOLVE && kernel <= OTM_KERNEL_FFT_ROWS_INV;
   if (fft && !fft_supported(s->n)) {
     set_error("the fast-DCT transform does not support this grid size");
     return OTM_ERR_ARG;
@@ -376,7 +376,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
   // advance the iteration count), so every kernel runs back-to-back between
   // one event pair: the figure is the kernel duration with launch latency
   // overlapped, as rocprof reports it.  BETA and DIR_COEF include their
-  // single-workgroup scalar kernel.
+  // single-workgroup scalar kernel; DIR_COEF_PASS is the line-search pass alone.
   if (kernel == OTM_KERNEL_DCT_SOLVE || kernel == OTM_KERNEL_MIXED_SOLVE || fft)
     tpl.update = 0;
   OTM_HIP(hipMemcpyAsync(scratch, &tpl, sizeof(tpl), hipMemcpyHostToDevice, s->drv.stream));
@@ -392,6 +392,7 @@ int otm_solver_profile_kernel(otm_solver* s, int kernel, int reps, double* ms_pe
       case OTM_KERNEL_FFT_ROWS_FWD: launch_fft_pass(pj, 0, true, s->drv.stream); break;
       case OTM_KERNEL_FFT_COLS: launch_fft_pass(pj, 1, true, s->drv.stream); break;
       case OTM_KERNEL_FFT_ROWS_INV: launch_fft_pass(pj, 2, true, s->drv.stream); break;
+      case OTM_KERNEL_DIR_COEF_PASS: launch_dir_coef_pass(pj, s->drv.stream); break;
       default: launch_mixed_solve(pj, 1, true, s->drv.stream); break;
     }
   };

@@ -515,7 +515,12 @@ __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
 // then beta = max(-1, Jd.(r_eps - r_k) / |Jd|^2 / eps * alpha) and the choice
 // d = d_hat + beta d_prev (it >= 1, conjugate-Jacobian) or d = d_hat.
 // dstarts: workgroup prefix of the launch that wrote the |d_hat|^2 partials.
-__global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restrict__ dstarts, int* flags) {
+// Wide form (jobs of more than NT * FOLD_BATCH stencil tiles): the
+// partials are folded by BETA_GROUPS groups of NT threads, with the same
+// result as NT threads where there are at most that many.
+constexpr int BETA_GROUPS = 4;
+template <bool WIDE>
+__global__ __launch_bounds__(WIDE ? BETA_GROUPS * NT : NT) void k_scalar_beta(JOB_ARGS, const int* __restrict__ dstarts, int* flags) {
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   const auto st = gp(f.state);
@@ -526,13 +531,21 @@ __global__ __launch_bounds__(NT) void k_scalar_beta(JOB_ARGS, const int* __restr
   double v[3];  // |d_hat|^2, Jd.(r_eps - r_k), Jd.Jd: all loads issued before the sum
   {
     double dn[1] = {0.}, bd[2] = {0., 0.};
-    fold_share<1>(dn, f.site(SITE_DNORM), dstarts[p + 1] - dstarts[p]);
-    fold_share<2>(bd, f.site(SITE_BETA), starts[p + 1] - starts[p]);
+    if (WIDE) {
+      fold_share_grouped<1, BETA_GROUPS>(dn, f.site(SITE_DNORM), dstarts[p + 1] - dstarts[p]);
+      fold_share_grouped<2, BETA_GROUPS>(bd, f.site(SITE_BETA), starts[p + 1] - starts[p]);
+    } else {
+      fold_share<1>(dn, f.site(SITE_DNORM), dstarts[p + 1] - dstarts[p]);
+      fold_share<2>(bd, f.site(SITE_BETA), starts[p + 1] - starts[p]);
+    }
     v[0] = dn[0];
     v[1] = bd[0];
     v[2] = bd[1];
   }
-  block_sum<3>(v);
+  if (WIDE)
+    group_sum<3, BETA_GROUPS>(v);
+  else
+    block_sum<3>(v);
   if (threadIdx.x != 0) return;
   st->update = 0;  // the previous iteration's update has run
   if (done) return;
@@ -657,7 +670,11 @@ __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
 // pass 2 (one workgroup per job): z = [e A, 2 b.r, b.b + 2 a.r, 2 a.b, a.a], the
 // step alpha, e(alpha)/A, traces, the iteration count and the stop test
 // (otsolver_2dgrid.cpp:556-605, :160); sum r_{k+1} = alpha^2 sum a + alpha sum b + sum r_k.
-__global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS, int* flags) {
+// Wide form: two groups of NT threads fold the partials (the step logic of
+// thread 0 needs more registers than four groups leave a thread).
+constexpr int STEP_GROUPS = 2;
+template <bool WIDE>
+__global__ __launch_bounds__(WIDE ? STEP_GROUPS * NT : NT) void k_scalar_step(JOB_ARGS, int* flags) {
   const int p = blockIdx.x;
   const Fields& f = jobs[p].f;
   const auto st = gp(f.state);
@@ -666,7 +683,13 @@ __global__ __launch_bounds__(NT) void k_scalar_step(JOB_ARGS, int* flags) {
   const int done = st->done, k = st->it, max_iter = st->max_iter;
   const double e_prev = st->e, sum_r = st->sum_r, beta = st->beta, threshold = st->threshold;
   double acc[7];
-  fold_partials<7>(acc, f.site(SITE_COEF), starts[p + 1] - starts[p]);
+  if (WIDE) {
+    for (int q = 0; q < 7; ++q) acc[q] = 0.;
+    fold_share_grouped<7, STEP_GROUPS>(acc, f.site(SITE_COEF), starts[p + 1] - starts[p]);
+    group_sum<7, STEP_GROUPS>(acc);
+  } else {
+    fold_partials<7>(acc, f.site(SITE_COEF), starts[p + 1] - starts[p]);
+  }
   if (threadIdx.x == 0 && !done) {
     const double A = 1.0 / (double(f.n) * double(f.n));
     double z[5];
@@ -1068,21 +1091,43 @@ void launch_start(const JobSet& js, double max_ratio, int max_iter, double thres
   OTM_HIP(hipGetLastError());
 }
 
+// Scalar kernels of the iteration in the wide form when a job has more
+// stencil tiles than one fold batch (n > 1024); a job's result does not
+// depend on the form as long as it has at most that many (and, of the
+// transform, at most that many |d_hat|^2 partials, which holds up to 1024^2).
+static bool wide_fold(const JobSet& js) {
+  for (const Job& J : js.host)
+    if (num_tiles(J.f.n) > NT * FOLD_BATCH) return true;
+  return false;
+}
+
 // the |d_hat|^2 partials of each job were written by the workgroups of the
 // last stage of its transform (KIND_DNORM)
 void launch_beta(const JobSet& js, hipStream_t s, int* flags) {
   hipLaunchKernelGGL(k_beta, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_beta, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
-                     (const int*)js.starts_of(KIND_DNORM), js.count == 1 ? flags : nullptr);
+  if (wide_fold(js))
+    hipLaunchKernelGGL(k_scalar_beta<true>, dim3(js.count), dim3(BETA_GROUPS * NT), 0, s, JOB_PASS(KIND_STENCIL),
+                       (const int*)js.starts_of(KIND_DNORM), js.count == 1 ? flags : nullptr);
+  else
+    hipLaunchKernelGGL(k_scalar_beta<false>, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
+                       (const int*)js.starts_of(KIND_DNORM), js.count == 1 ? flags : nullptr);
+  OTM_HIP(hipGetLastError());
+}
+
+void launch_dir_coef_pass(const JobSet& js, hipStream_t s) {
+  hipLaunchKernelGGL(k_dir_coef, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
   OTM_HIP(hipGetLastError());
 }
 
 void launch_dir_coef(const JobSet& js, hipStream_t s, int* flags) {
-  hipLaunchKernelGGL(k_dir_coef, grid_of(js, KIND_STENCIL), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL));
-  OTM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scalar_step, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
-                     js.count == 1 ? flags : nullptr);
+  launch_dir_coef_pass(js, s);
+  if (wide_fold(js))
+    hipLaunchKernelGGL(k_scalar_step<true>, dim3(js.count), dim3(STEP_GROUPS * NT), 0, s, JOB_PASS(KIND_STENCIL),
+                       js.count == 1 ? flags : nullptr);
+  else
+    hipLaunchKernelGGL(k_scalar_step<false>, dim3(js.count), dim3(NT), 0, s, JOB_PASS(KIND_STENCIL),
+                       js.count == 1 ? flags : nullptr);
   OTM_HIP(hipGetLastError());
 }
 

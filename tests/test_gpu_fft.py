@@ -175,10 +175,10 @@ def test_fft_pass_timing_entry_points(solver_factory):
     s = solver_factory(256)
     s.laplacian_solve_fft(_rng(256).standard_normal(256 * 256))
     for k in (_lib.OTM_KERNEL_FFT_SOLVE, _lib.OTM_KERNEL_FFT_ROWS_FWD, _lib.OTM_KERNEL_FFT_COLS,
-              _lib.OTM_KERNEL_FFT_ROWS_INV):
+              _lib.OTM_KERNEL_FFT_ROWS_INV, _lib.OTM_KERNEL_DIR_COEF_PASS):
         assert _lib.lib().otm_solver_profile_kernel(s._h, k, 3, _lib.C.byref(ms)) == 0
         assert ms.value > 0
-    assert _lib.lib().otm_solver_profile_kernel(s._h, _lib.OTM_KERNEL_FFT_ROWS_INV + 1, 3, _lib.C.byref(ms)) != 0
+    assert _lib.lib().otm_solver_profile_kernel(s._h, _lib.OTM_KERNEL_DIR_COEF_PASS + 1, 3, _lib.C.byref(ms)) != 0
     u = solver_factory(100)
     for k in (_lib.OTM_KERNEL_FFT_SOLVE, _lib.OTM_KERNEL_FFT_COLS):
         assert _lib.lib().otm_solver_profile_kernel(u._h, k, 3, _lib.C.byref(ms)) != 0
