@@ -188,7 +188,10 @@ def test_fft_batch_mixing_both_workgroup_forms():
     """One job table with grids on both fast-DCT workgroup forms (2 rows per
     workgroup up to n = 1024, 4 rows above; 1152 = 2^7 3^2 needs a radix-3
     stage) and one on the GEMM route (320 = 64 * 5): bit-identical to single
-    solves."""
+    solves.  The batch has grids above 1024^2, so its scalar kernels run in
+    the wide forms (thread groups folding more than 2048 partials), while the
+    single solves of 192, 320 and 1024 run the 256-thread forms: the two
+    forms give those grids the same bits."""
     sizes = [1152, 192, 2048, 320, 1024]
     dens = [D.gaussian_mixture(n, 70 + k, K=3) for k, n in enumerate(sizes)]
     print("fft batch of both forms", flush=True)
