@@ -45,7 +45,7 @@ GEMM_TRAFFIC = "profiles/gemm_1024_hbm_traffic.md"
 FFT_TRAFFIC = "profiles/fft_1024_hbm_traffic.md"
 STENCIL_TRAFFIC = "profiles/stencil_1024_hbm_traffic.md"
 TRAFFIC_BYTES_1024 = {
-    "k_dir_coef": ((22215.0 * 2.0 + 8640.0) * 1024, STENCIL_TRAFFIC),
+    "k_dir_coef": ((16623.0 * 2.0 + 8304.0) * 1024, STENCIL_TRAFFIC),
     "k_fft_cols": ((4333.5 * 2.0 + 8192.0) * 1024, FFT_TRAFFIC),
     "k_mgemm<M2,32>": ((9288.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),
     "k_mgemm<M2,64>": ((9294.0 * 2.0 + 6144.0) * 1024, GEMM_TRAFFIC),

@@ -262,6 +262,22 @@ __device__ inline Loc locate(const int* __restrict__ starts, int count) {
   return {p, b - starts[p], starts[p + 1] - starts[p]};
 }
 
+// Stencil launches run one workgroup per tile; workgroup w of a job takes
+// tile band_tile(w, T) of its T tiles.  Workgroups are dealt to the 8 XCDs
+// round-robin, so XCD x gets tiles [x T/8, (x + 1) T/8): a band of
+// consecutive tile rows, in which the left and right neighbours of a tile
+// (whose edge faces it reads as its halo) share its L2, as do the upper and
+// lower ones except at the band's edges.  (With the row-major order an
+// XCD got every eighth tile column, and no horizontal neighbour.)  A
+// permutation of [0, T), the identity when T % 8 != 0; partials are written
+// at the tile index, so the fold order does not depend on it.
+__device__ inline int band_tile(int w, int T) { return (T & 7) ? w : (w & 7) * (T >> 3) + (w >> 3); }
+__device__ inline Loc locate_tile(const int* __restrict__ starts, int count) {
+  Loc L = locate(starts, count);
+  L.t = band_tile(L.t, L.nb);
+  return L;
+}
+
 // tile origin of local stencil tile t (row-major over the tiles_j(n) x tiles_i(n) grid)
 __device__ inline void tile_origin(int t, int n, int& i0, int& j0) {
   const int tj = (n + TJ - 1) / TJ;
@@ -366,7 +382,7 @@ __global__ __launch_bounds__(NT) void k_adjust_apply(JOB_ARGS) {
 
 // initial residual at psi = 0: r_0 and the partials of |r|^2, sum r
 __global__ __launch_bounds__(NT) void k_initial_residual(JOB_ARGS) {
-  const Loc L = locate(starts, count);
+  const Loc L = locate_tile(starts, count);
   const Fields& f = jobs[L.p].f;
   const int n = f.n;
   __shared__ double gx[NV], gy[NV];
@@ -445,7 +461,7 @@ __global__ __launch_bounds__(NT) void k_scalar_init(JOB_ARGS, int max_iter, doub
 // (The reference evaluates r_eps afresh and subtracts the running r_k; the
 // two differ by the running residual's rounding drift.)
 __global__ __launch_bounds__(NT) void k_beta(JOB_ARGS) {
-  const Loc L = locate(starts, count);
+  const Loc L = locate_tile(starts, count);
   const Fields& f = jobs[L.p].f;
   const auto st = gp((const SolveState*)f.state);
   // one job: the tile's reads are issued before the gate is read (the two
@@ -574,7 +590,7 @@ __global__ __launch_bounds__(WIDE ? BETA_GROUPS * NT : NT) void k_scalar_beta(JO
 // a and b are not stored: the update pass evaluates them again from the same
 // inputs (d and psi before its update), which costs less than their HBM round trip.
 __global__ __launch_bounds__(NT) void k_dir_coef(JOB_ARGS) {
-  const Loc L = locate(starts, count);
+  const Loc L = locate_tile(starts, count);
   const Fields& f = jobs[L.p].f;
   const auto st = gp((const SolveState*)f.state);
   if (st->done) return;
@@ -809,7 +825,7 @@ __device__ inline void update_apply(const Fields& f, int k, double alpha, int i0
 
 // eight waves per SIMD: every tile of a 1024^2 grid resident at once
 __global__ __launch_bounds__(NT, 8) void k_update(JOB_ARGS) {
-  const Loc L = locate(starts, count);
+  const Loc L = locate_tile(starts, count);
   const Fields& f = jobs[L.p].f;
   const auto st = gp((const SolveState*)f.state);
   if (!st->update) return;
@@ -891,7 +907,7 @@ __global__ __launch_bounds__(NT) void k_update_fold(JOB_ARGS) {
 // (otsolver_2dgrid.cpp:223-227); the final potential, in the buffer of the
 // last completed iteration, goes back to its home buffer f.psi
 __global__ __launch_bounds__(NT) void k_forward(JOB_ARGS) {
-  const Loc L = locate(starts, count);
+  const Loc L = locate_tile(starts, count);
   const Job& J = jobs[L.p];
   const int n = J.f.n;
   const auto st = gp((const SolveState*)J.f.state);
